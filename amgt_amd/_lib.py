@@ -1,0 +1,140 @@
+"""ctypes binding of libamgt_mbsr.so (the C-ABI declared in include/amgt_mbsr.h).
+
+The library must be built in-tree (``python -m amgt_amd.build``).  There is no
+CPU fallback: every entry point raises when the HIP library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libamgt_mbsr.so")
+
+AMGT_OK = 0
+AMGT_ERR_ARG = 1
+AMGT_ERR_DIM = 2
+AMGT_ERR_ALLOC = 3
+AMGT_ERR_HIP = 4
+AMGT_ERR_INPUT = 5
+AMGT_ERR_UNSUPPORTED = 6
+
+AMGT_DOUBLE = 0
+AMGT_FLOAT = 1
+AMGT_HALF = 2
+
+# every symbol include/amgt_mbsr.h declares
+EXPORTS = (
+    "amgt_status_string", "amgt_last_error", "amgt_set_allocator", "amgt_precision_for_level",
+    "amgt_set_mixed_precision", "amgt_mbsr_create_from_csr", "amgt_mbsr_destroy",
+    "amgt_mbsr_update_values", "amgt_mbsr_get_info", "amgt_mbsr_get_arrays",
+    "amgt_mbsr_set_level", "amgt_mbsr_set_precision", "amgt_mbsr_set_spmv_lanes",
+    "amgt_mbsr_autotune", "amgt_mbsr_spmv", "amgt_mbsr_spmm", "amgt_mbsr_spgemm",
+    "amgt_mbsr_to_csr", "amgt_free", "amgt_csr_spgemm", "amgt_csr_matvec",
+)
+
+
+class AmgtError(RuntimeError):
+    """A libamgt_mbsr entry point returned a non-zero status."""
+
+    def __init__(self, status: int, where: str, message: str):
+        super().__init__(f"{where}: status {status}: {message}")
+        self.status = status
+
+
+class AmgtLibraryMissing(ImportError):
+    pass
+
+
+class MbsrInfo(ctypes.Structure):
+    _fields_ = [
+        ("nrows", ctypes.c_int32), ("ncols", ctypes.c_int32), ("brow", ctypes.c_int32),
+        ("bcol", ctypes.c_int32), ("nblk", ctypes.c_int64), ("nnz", ctypes.c_int64),
+        ("stand", ctypes.c_double), ("avg_nnz", ctypes.c_double), ("fill", ctypes.c_double),
+        ("max_row_blocks", ctypes.c_int32), ("level", ctypes.c_int32),
+        ("precision", ctypes.c_int32), ("spmv_lanes", ctypes.c_int32),
+    ]
+
+
+class MbsrArrays(ctypes.Structure):
+    _fields_ = [
+        ("blk_ptr", ctypes.c_void_p), ("blk_col", ctypes.c_void_p), ("val_ptr", ctypes.c_void_p),
+        ("blk_map", ctypes.c_void_p), ("val", ctypes.c_void_p),
+    ]
+
+
+_lib = None
+_hip = None
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_dbl = ctypes.c_double
+_pvp = ctypes.POINTER(ctypes.c_void_p)
+_pi64 = ctypes.POINTER(ctypes.c_int64)
+
+
+def load():
+    """Loads the in-tree HIP library; raises AmgtLibraryMissing when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise AmgtLibraryMissing(
+            f"{LIB_PATH} is not built; run `python -m amgt_amd.build` (no CPU fallback exists)")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "amgt_status_string": (ctypes.c_char_p, [_int]),
+        "amgt_last_error": (ctypes.c_char_p, []),
+        "amgt_set_allocator": (_int, [_vp, _vp, _vp]),
+        "amgt_precision_for_level": (_int, [_int]),
+        "amgt_set_mixed_precision": (_int, [_int]),
+        "amgt_mbsr_create_from_csr": (_int, [_i32, _i32, _i64, _vp, _vp, _vp, _vp, _pvp]),
+        "amgt_mbsr_destroy": (_int, [_vp]),
+        "amgt_mbsr_update_values": (_int, [_vp, _vp, _vp, _vp, _vp]),
+        "amgt_mbsr_get_info": (_int, [_vp, ctypes.POINTER(MbsrInfo)]),
+        "amgt_mbsr_get_arrays": (_int, [_vp, ctypes.POINTER(MbsrArrays)]),
+        "amgt_mbsr_set_level": (_int, [_vp, _int]),
+        "amgt_mbsr_set_precision": (_int, [_vp, _int]),
+        "amgt_mbsr_set_spmv_lanes": (_int, [_vp, _int]),
+        "amgt_mbsr_autotune": (_int, [_vp, _vp]),
+        "amgt_mbsr_spmv": (_int, [_vp, _int, _dbl, _vp, _dbl, _vp, _vp, _vp]),
+        "amgt_mbsr_spmm": (_int, [_vp, _int, _int, _dbl, _vp, _i64, _dbl, _vp, _i64, _vp, _i64,
+                                  _vp]),
+        "amgt_mbsr_spgemm": (_int, [_vp, _vp, _vp, _pvp]),
+        "amgt_mbsr_to_csr": (_int, [_vp, _vp, _pi64, _pvp, _pvp, _pvp]),
+        "amgt_free": (_int, [_vp]),
+        "amgt_csr_spgemm": (_int, [_i32, _i32, _i32, _i64, _vp, _vp, _vp, _pvp, _i64, _vp, _vp,
+                                   _vp, _pvp, _pi64, _pvp, _pvp, _pvp, _pvp, _vp]),
+        "amgt_csr_matvec": (_int, [_int, _dbl, _i32, _i32, _i64, _vp, _vp, _vp, _pvp, _vp, _dbl,
+                                   _vp, _vp, _vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(status: int, where: str) -> None:
+    if status != AMGT_OK:
+        msg = load().amgt_last_error()
+        raise AmgtError(status, where, msg.decode() if msg else "")
+
+
+def hip():
+    """The HIP runtime (the same instance torch uses), for raw device copies."""
+    global _hip
+    if _hip is None:
+        h = ctypes.CDLL("libamdhip64.so")
+        h.hipMemcpy.restype = ctypes.c_int
+        h.hipMemcpy.argtypes = [_vp, _vp, ctypes.c_size_t, ctypes.c_int]
+        h.hipMemcpyAsync.restype = ctypes.c_int
+        h.hipMemcpyAsync.argtypes = [_vp, _vp, ctypes.c_size_t, ctypes.c_int, _vp]
+        _hip = h
+    return _hip
+
+
+HIP_MEMCPY_D2D = 3

@@ -1,0 +1,141 @@
+// Internal declarations shared by the HIP translation units of libamgt_mbsr.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/amgt_mbsr.h"
+
+namespace amgt {
+
+constexpr int kWave = 64;
+
+// ---- error plumbing -------------------------------------------------------
+void set_error(const std::string &msg);
+
+struct Status {
+    int code = AMGT_OK;
+};
+
+#define AMGT_HIP(call)                                                                     \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            ::amgt::set_error(std::string(#call) + ": " + hipGetErrorString(e_));          \
+            return AMGT_ERR_HIP;                                                           \
+        }                                                                                  \
+    } while (0)
+
+#define AMGT_TRY(expr)                 \
+    do {                               \
+        int s_ = (expr);               \
+        if (s_ != AMGT_OK) return s_;  \
+    } while (0)
+
+#define AMGT_CHECK_ARG(cond, msg)                 \
+    do {                                          \
+        if (!(cond)) {                            \
+            ::amgt::set_error(msg);               \
+            return AMGT_ERR_ARG;                  \
+        }                                         \
+    } while (0)
+
+// ---- allocation -------------------------------------------------------------
+// Long-lived arrays (handle arrays, CSR exports) go through the user hooks.
+void *dev_alloc(size_t bytes);
+void dev_free(void *p);
+// Stream-ordered temporaries.
+int tmp_alloc(void **p, size_t bytes, hipStream_t s);
+void tmp_free(void *p, hipStream_t s);
+
+template <typename T>
+int tmp_alloc_t(T **p, size_t n, hipStream_t s) {
+    return tmp_alloc(reinterpret_cast<void **>(p), sizeof(T) * (n ? n : 1), s);
+}
+
+// ---- handle -----------------------------------------------------------------
+struct Mbsr {
+    int32_t nrows = 0, ncols = 0, brow = 0, bcol = 0;
+    int64_t nblk = 0, nnz = 0;
+    int32_t *blk_ptr = nullptr;
+    int32_t *blk_col = nullptr;
+    uint16_t *blk_map = nullptr;
+    int32_t *val_ptr = nullptr;
+    double *val = nullptr;
+    float *val32 = nullptr;   // lazily made FP32 copy
+    __half *val16 = nullptr;  // lazily made FP16 copy
+    bool pattern_only = false;
+    double stand = 0.0, avg_nnz = 0.0;
+    int32_t max_row_blocks = 0;
+    int32_t level = 0;
+    int32_t precision = -1;   // -1: follow the level policy
+    int32_t spmv_lanes = 0;   // 0: heuristic
+    Mbsr *trans = nullptr;    // cached A^T
+};
+
+}  // namespace amgt
+
+struct amgt_mbsr {
+    amgt::Mbsr m;
+};
+
+namespace amgt {
+
+bool mixed_precision_enabled();
+
+// ---- component entry points (each in its own translation unit) -------------
+int csr_to_mbsr(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *rowptr,
+                const int32_t *colidx, const double *val, hipStream_t s, Mbsr *out);
+int mbsr_refresh_values(Mbsr *A, const int32_t *rowptr, const int32_t *colidx,
+                        const double *val, hipStream_t s);
+int mbsr_transpose(const Mbsr &A, hipStream_t s, Mbsr *out);
+int mbsr_ensure_lowp(Mbsr *A, int precision, hipStream_t s);
+int spmv(const Mbsr &A, int precision, int lanes, double alpha, const double *x, double beta,
+         const double *b, double *y, hipStream_t s);
+int spmm(const Mbsr &A, int precision, int lanes, int k, double alpha, const double *x,
+         int64_t ldx, double beta, const double *b, int64_t ldb, double *y, int64_t ldy,
+         hipStream_t s);
+int spmv_default_lanes(const Mbsr &A);
+// y = beta*b over n entries (beta == 0 writes zeros without reading b)
+int scale_vector(int32_t n, double beta, const double *b, double *y, hipStream_t s);
+int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *C);
+int mbsr_to_csr(const Mbsr &A, hipStream_t s, int64_t *nnz, int32_t **rowptr, int32_t **colidx,
+                double **val);
+void mbsr_release(Mbsr *A);
+
+// exclusive scan of n int32 counts into out[0..n] (out[n] = total); int64 total
+// is returned through *total if non-null (synchronises the stream)
+int exclusive_scan_i32(const int32_t *in, int32_t *out, int64_t n, hipStream_t s);
+int exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t s);
+
+// ---- device helpers ----------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t bitmap_mul(uint32_t a, uint32_t b) {
+    // bitMatrixMul, csr_spgemm_device.c:189-205: OR_k (col_k(a) x row_k(b))
+    uint32_t res = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        uint32_t col = (a >> k) & 0x1111u;
+        uint32_t row = (b >> (4 * k)) & 0xFu;
+        res |= (col * 0xFu) & (row * 0x1111u);
+    }
+    return res;
+}
+
+// transpose of a 4x4 occupancy bitmap: bit r*4+c -> bit c*4+r
+__device__ __forceinline__ uint32_t bitmap_transpose(uint32_t m) {
+    uint32_t t = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            t |= ((m >> (r * 4 + c)) & 1u) << (c * 4 + r);
+    return t;
+}
+
+}  // namespace amgt

@@ -1,0 +1,374 @@
+// mBSR SpMV / SpMM (counterparts of spmv_amgT_fp64/fp32/fp16,
+// csr_matvec_device.c:1302-1411, 2098-2309).
+//
+// Kernel shape: G lanes per 4-row block row (G = 4, 8, 16, 32, 64 chosen from
+// the matrix statistics or by the tuner), 64/G block rows per wave, 4 waves per
+// workgroup.  Within a block row, lane (slot s, row r) walks blocks s, s+G/4,
+// ... and, for each, the set bits of row r's nibble of the 16-bit bitmap.  Its
+// values sit at  val_ptr[br] + (popcounts of the earlier blocks) +
+// popc(map & ((1 << 4r) - 1)),  so the popcount walk needs no per-nonzero
+// index.  Row sums are reduced across slots with shuffles; beta*b is fused in
+// the store (no separate beta pass, no atomics: results are deterministic).
+// Values may be FP64, FP32 or FP16; below FP64 the products accumulate in FP32.
+// x, b, y are FP64.
+#include "internal.hpp"
+
+#include <string>
+
+namespace amgt {
+
+namespace {
+
+template <typename VT>
+__device__ __forceinline__ float to_f(VT v);
+template <>
+__device__ __forceinline__ float to_f<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ float to_f<__half>(__half v) { return __half2float(v); }
+
+template <typename VT, typename AT>
+__device__ __forceinline__ AT mul_acc(AT acc, VT a, double x);
+
+template <>
+__device__ __forceinline__ double mul_acc<double, double>(double acc, double a, double x) {
+    return fma(a, x, acc);
+}
+template <>
+__device__ __forceinline__ float mul_acc<float, float>(float acc, float a, double x) {
+    return fmaf(a, float(x), acc);
+}
+template <>
+__device__ __forceinline__ float mul_acc<__half, float>(float acc, __half a, double x) {
+    return fmaf(__half2float(a), float(x), acc);
+}
+
+// XCD-aware workgroup remap: hardware hands out workgroups round-robin over
+// the 8 XCDs; give each XCD a contiguous range of block rows so that x reuse
+// between neighbouring block rows stays within one L2.
+__device__ __forceinline__ int remap_wg(int wg, int nwg) {
+    const int xcd = wg & 7;
+    const int idx = wg >> 3;
+    const int per = nwg >> 3;
+    const int rem = nwg & 7;
+    return xcd * per + min(xcd, rem) + idx;
+}
+
+template <int G, typename VT, typename AT>
+__global__ __launch_bounds__(256) void k_spmv(
+    int32_t brow, int32_t nrows, const int32_t *__restrict__ blk_ptr,
+    const int32_t *__restrict__ blk_col, const uint16_t *__restrict__ blk_map,
+    const int32_t *__restrict__ val_ptr, const VT *__restrict__ val, const double *__restrict__ x,
+    double alpha, double beta, const double *b, double *y) {
+    constexpr int S = G / 4;
+    constexpr int RPW = 64 / G;
+    const int lane = threadIdx.x & 63;
+    const int wg = remap_wg(blockIdx.x, gridDim.x);
+    const int wave = wg * 4 + (threadIdx.x >> 6);
+    const int br = wave * RPW + lane / G;
+    const int sub = lane % G;
+    const int r = sub & 3;
+    const int s = sub >> 2;
+    if (br >= brow) return;  // whole G-lane group leaves together
+    const int p0 = blk_ptr[br];
+    const int p1 = blk_ptr[br + 1];
+    int voff = val_ptr[br];
+    const uint32_t low = (1u << (4 * r)) - 1u;
+    AT acc = AT(0);
+    for (int jb = p0; jb < p1; jb += S) {
+        const int j = jb + s;
+        uint32_t m = 0;
+        int col = 0;
+        if (j < p1) {
+            m = blk_map[j];
+            col = blk_col[j];
+        }
+        const int cnt = __popc(m);
+        int incl = cnt;
+        int tot = cnt;
+        if constexpr (S > 1) {
+#pragma unroll
+            for (int d = 1; d < S; d <<= 1) {
+                int t = __shfl_up(incl, 4 * d, G);
+                if (s >= d) incl += t;
+            }
+            tot = __shfl(incl, 4 * (S - 1) + r, G);
+        }
+        int o = voff + (incl - cnt) + __popc(m & low);
+        uint32_t nib = (m >> (4 * r)) & 0xFu;
+        const double *xs = x + 4 * int64_t(col);
+        while (nib) {
+            const int c = __builtin_ctz(nib);
+            nib &= nib - 1u;
+            acc = mul_acc<VT, AT>(acc, val[o], xs[c]);
+            ++o;
+        }
+        voff += tot;
+    }
+    if constexpr (S > 1) {
+#pragma unroll
+        for (int d = S / 2; d >= 1; d >>= 1) acc += __shfl_xor(acc, 4 * d, G);
+    }
+    if (s == 0) {
+        const int row = 4 * br + r;
+        if (row < nrows) {
+            double v = alpha * double(acc);
+            if (beta != 0.0) v = fma(beta, b[row], v);
+            y[row] = v;
+        }
+    }
+}
+
+// k right-hand sides (column-major, leading dimensions), KT at a time.
+template <int G, int KT, typename VT, typename AT>
+__global__ __launch_bounds__(256) void k_spmm(
+    int32_t brow, int32_t nrows, const int32_t *__restrict__ blk_ptr,
+    const int32_t *__restrict__ blk_col, const uint16_t *__restrict__ blk_map,
+    const int32_t *__restrict__ val_ptr, const VT *__restrict__ val, int k0, int kn,
+    const double *__restrict__ x, int64_t ldx, double alpha, double beta, const double *b,
+    int64_t ldb, double *y, int64_t ldy) {
+    constexpr int S = G / 4;
+    constexpr int RPW = 64 / G;
+    const int lane = threadIdx.x & 63;
+    const int wg = remap_wg(blockIdx.x, gridDim.x);
+    const int wave = wg * 4 + (threadIdx.x >> 6);
+    const int br = wave * RPW + lane / G;
+    const int sub = lane % G;
+    const int r = sub & 3;
+    const int s = sub >> 2;
+    if (br >= brow) return;
+    const int p0 = blk_ptr[br];
+    const int p1 = blk_ptr[br + 1];
+    int voff = val_ptr[br];
+    const uint32_t low = (1u << (4 * r)) - 1u;
+    AT acc[KT];
+#pragma unroll
+    for (int v = 0; v < KT; ++v) acc[v] = AT(0);
+    const double *xk = x + int64_t(k0) * ldx;
+    for (int jb = p0; jb < p1; jb += S) {
+        const int j = jb + s;
+        uint32_t m = 0;
+        int col = 0;
+        if (j < p1) {
+            m = blk_map[j];
+            col = blk_col[j];
+        }
+        const int cnt = __popc(m);
+        int incl = cnt;
+        int tot = cnt;
+        if constexpr (S > 1) {
+#pragma unroll
+            for (int d = 1; d < S; d <<= 1) {
+                int t = __shfl_up(incl, 4 * d, G);
+                if (s >= d) incl += t;
+            }
+            tot = __shfl(incl, 4 * (S - 1) + r, G);
+        }
+        int o = voff + (incl - cnt) + __popc(m & low);
+        uint32_t nib = (m >> (4 * r)) & 0xFu;
+        while (nib) {
+            const int c = __builtin_ctz(nib);
+            nib &= nib - 1u;
+            const VT a = val[o];
+            const double *xs = xk + 4 * int64_t(col) + c;
+#pragma unroll
+            for (int v = 0; v < KT; ++v)
+                if (v < kn) acc[v] = mul_acc<VT, AT>(acc[v], a, xs[v * ldx]);
+            ++o;
+        }
+        voff += tot;
+    }
+#pragma unroll
+    for (int v = 0; v < KT; ++v) {
+        if constexpr (S > 1) {
+#pragma unroll
+            for (int d = S / 2; d >= 1; d >>= 1) acc[v] += __shfl_xor(acc[v], 4 * d, G);
+        }
+    }
+    if (s == 0) {
+        const int row = 4 * br + r;
+        if (row < nrows) {
+#pragma unroll
+            for (int v = 0; v < KT; ++v) {
+                if (v < kn) {
+                    double val_out = alpha * double(acc[v]);
+                    if (beta != 0.0) val_out = fma(beta, b[int64_t(k0 + v) * ldb + row], val_out);
+                    y[int64_t(k0 + v) * ldy + row] = val_out;
+                }
+            }
+        }
+    }
+}
+
+// y = beta*b (beta == 0 -> y = 0): used when the matrix has no blocks
+__global__ void k_scale(int32_t n, double beta, const double *b, double *y) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = (beta == 0.0) ? 0.0 : beta * b[i];
+}
+
+__global__ void k_cast_f32(int64_t n, const double *__restrict__ in, float *__restrict__ out) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = float(in[i]);
+}
+
+__global__ void k_cast_f16(int64_t n, const double *__restrict__ in, __half *__restrict__ out) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = __float2half(float(in[i]));
+}
+
+template <int G, typename VT, typename AT>
+int launch_spmv_g(const Mbsr &A, const VT *val, double alpha, const double *x, double beta,
+                  const double *b, double *y, hipStream_t s) {
+    constexpr int RPW = 64 / G;
+    const int64_t waves = (int64_t(A.brow) + RPW - 1) / RPW;
+    const int64_t grid = (waves + 3) / 4;
+    if (grid > 0x7fffffff) {
+        set_error("SpMV grid too large");
+        return AMGT_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL((k_spmv<G, VT, AT>), dim3(unsigned(grid)), dim3(256), 0, s, A.brow,
+                       A.nrows, A.blk_ptr, A.blk_col, A.blk_map, A.val_ptr, val, x, alpha, beta,
+                       b, y);
+    AMGT_HIP(hipGetLastError());
+    return AMGT_OK;
+}
+
+template <typename VT, typename AT>
+int launch_spmv_t(const Mbsr &A, const VT *val, int lanes, double alpha, const double *x,
+                  double beta, const double *b, double *y, hipStream_t s) {
+    switch (lanes) {
+        case 4: return launch_spmv_g<4, VT, AT>(A, val, alpha, x, beta, b, y, s);
+        case 8: return launch_spmv_g<8, VT, AT>(A, val, alpha, x, beta, b, y, s);
+        case 16: return launch_spmv_g<16, VT, AT>(A, val, alpha, x, beta, b, y, s);
+        case 32: return launch_spmv_g<32, VT, AT>(A, val, alpha, x, beta, b, y, s);
+        case 64: return launch_spmv_g<64, VT, AT>(A, val, alpha, x, beta, b, y, s);
+        default:
+            set_error("lanes per block row must be 4, 8, 16, 32 or 64");
+            return AMGT_ERR_ARG;
+    }
+}
+
+template <int G, int KT, typename VT, typename AT>
+int launch_spmm_g(const Mbsr &A, const VT *val, int k, double alpha, const double *x,
+                  int64_t ldx, double beta, const double *b, int64_t ldb, double *y, int64_t ldy,
+                  hipStream_t s) {
+    constexpr int RPW = 64 / G;
+    const int64_t waves = (int64_t(A.brow) + RPW - 1) / RPW;
+    const int64_t grid = (waves + 3) / 4;
+    for (int k0 = 0; k0 < k; k0 += KT) {
+        int kn = k - k0 < KT ? k - k0 : KT;
+        hipLaunchKernelGGL((k_spmm<G, KT, VT, AT>), dim3(unsigned(grid)), dim3(256), 0, s, A.brow,
+                           A.nrows, A.blk_ptr, A.blk_col, A.blk_map, A.val_ptr, val, k0, kn, x,
+                           ldx, alpha, beta, b, ldb, y, ldy);
+        AMGT_HIP(hipGetLastError());
+    }
+    return AMGT_OK;
+}
+
+template <typename VT, typename AT>
+int launch_spmm_t(const Mbsr &A, const VT *val, int lanes, int k, double alpha, const double *x,
+                  int64_t ldx, double beta, const double *b, int64_t ldb, double *y, int64_t ldy,
+                  hipStream_t s) {
+    switch (lanes) {
+        case 4: return launch_spmm_g<4, 8, VT, AT>(A, val, k, alpha, x, ldx, beta, b, ldb, y, ldy, s);
+        case 8: return launch_spmm_g<8, 8, VT, AT>(A, val, k, alpha, x, ldx, beta, b, ldb, y, ldy, s);
+        case 16: return launch_spmm_g<16, 8, VT, AT>(A, val, k, alpha, x, ldx, beta, b, ldb, y, ldy, s);
+        case 32: return launch_spmm_g<32, 8, VT, AT>(A, val, k, alpha, x, ldx, beta, b, ldb, y, ldy, s);
+        default: return launch_spmm_g<64, 8, VT, AT>(A, val, k, alpha, x, ldx, beta, b, ldb, y, ldy, s);
+    }
+}
+
+}  // namespace
+
+int spmv_default_lanes(const Mbsr &A) {
+    // enough slots that a typical block row takes about two block steps
+    const double avg = A.brow ? double(A.nblk) / double(A.brow) : 0.0;
+    if (avg <= 8.0) return 4;
+    if (avg <= 16.0) return 8;
+    if (avg <= 32.0) return 16;
+    if (avg <= 64.0) return 32;
+    return 64;
+}
+
+int mbsr_ensure_lowp(Mbsr *A, int precision, hipStream_t s) {
+    if (precision == AMGT_DOUBLE || A->nnz == 0 || A->pattern_only) return AMGT_OK;
+    const int64_t n = A->nnz;
+    const unsigned grid = unsigned((n + 255) / 256);
+    if (precision == AMGT_FLOAT && !A->val32) {
+        A->val32 = static_cast<float *>(dev_alloc(sizeof(float) * size_t(n)));
+        if (!A->val32) {
+            set_error("FP32 value copy allocation failed");
+            return AMGT_ERR_ALLOC;
+        }
+        hipLaunchKernelGGL(k_cast_f32, dim3(grid), dim3(256), 0, s, n, A->val, A->val32);
+        AMGT_HIP(hipGetLastError());
+    }
+    if (precision == AMGT_HALF && !A->val16) {
+        A->val16 = static_cast<__half *>(dev_alloc(sizeof(__half) * size_t(n)));
+        if (!A->val16) {
+            set_error("FP16 value copy allocation failed");
+            return AMGT_ERR_ALLOC;
+        }
+        hipLaunchKernelGGL(k_cast_f16, dim3(grid), dim3(256), 0, s, n, A->val, A->val16);
+        AMGT_HIP(hipGetLastError());
+    }
+    return AMGT_OK;
+}
+
+int spmv(const Mbsr &A, int precision, int lanes, double alpha, const double *x, double beta,
+         const double *b, double *y, hipStream_t s) {
+    if (A.nrows == 0) return AMGT_OK;
+    if (A.nblk == 0 || alpha == 0.0) {
+        hipLaunchKernelGGL(k_scale, dim3((A.nrows + 255) / 256), dim3(256), 0, s, A.nrows, beta,
+                           b, y);
+        AMGT_HIP(hipGetLastError());
+        return AMGT_OK;
+    }
+    switch (precision) {
+        case AMGT_DOUBLE:
+            return launch_spmv_t<double, double>(A, A.val, lanes, alpha, x, beta, b, y, s);
+        case AMGT_FLOAT:
+            return launch_spmv_t<float, float>(A, A.val32, lanes, alpha, x, beta, b, y, s);
+        case AMGT_HALF:
+            return launch_spmv_t<__half, float>(A, A.val16, lanes, alpha, x, beta, b, y, s);
+        default:
+            set_error("unknown precision");
+            return AMGT_ERR_ARG;
+    }
+}
+
+int spmm(const Mbsr &A, int precision, int lanes, int k, double alpha, const double *x,
+         int64_t ldx, double beta, const double *b, int64_t ldb, double *y, int64_t ldy,
+         hipStream_t s) {
+    if (A.nrows == 0) return AMGT_OK;
+    if (A.nblk == 0 || alpha == 0.0) {
+        for (int v = 0; v < k; ++v) {
+            hipLaunchKernelGGL(k_scale, dim3((A.nrows + 255) / 256), dim3(256), 0, s, A.nrows,
+                               beta, b ? b + int64_t(v) * ldb : nullptr, y + int64_t(v) * ldy);
+        }
+        AMGT_HIP(hipGetLastError());
+        return AMGT_OK;
+    }
+    switch (precision) {
+        case AMGT_DOUBLE:
+            return launch_spmm_t<double, double>(A, A.val, lanes, k, alpha, x, ldx, beta, b, ldb,
+                                                 y, ldy, s);
+        case AMGT_FLOAT:
+            return launch_spmm_t<float, float>(A, A.val32, lanes, k, alpha, x, ldx, beta, b, ldb,
+                                               y, ldy, s);
+        case AMGT_HALF:
+            return launch_spmm_t<__half, float>(A, A.val16, lanes, k, alpha, x, ldx, beta, b, ldb,
+                                                y, ldy, s);
+        default:
+            set_error("unknown precision");
+            return AMGT_ERR_ARG;
+    }
+}
+
+int scale_vector(int32_t n, double beta, const double *b, double *y, hipStream_t s) {
+    if (n == 0) return AMGT_OK;
+    hipLaunchKernelGGL(k_scale, dim3((n + 255) / 256), dim3(256), 0, s, n, beta, b, y);
+    AMGT_HIP(hipGetLastError());
+    return AMGT_OK;
+}
+
+}  // namespace amgt

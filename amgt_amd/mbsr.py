@@ -1,0 +1,210 @@
+"""Python host side of the mBSR path over the C-ABI (torch supplies device memory
+and the stream; all compute runs in libamgt_mbsr.so).
+
+``MBSRMatrix`` owns one library handle (the bsrMAT of seq_mv.h:105-129).
+Operator names mirroring AmgT's functions (CSR2BSR_GPU, spmv_amgT_fp64, ...)
+live in ``amgt_amd.hypre`` next to the HYPRE-shaped boundary.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib as L
+
+
+def _stream_ptr(stream=None) -> ctypes.c_void_p:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _need_cuda(t, name, dtype=None):
+    if t is None:
+        return
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{name} must be a device tensor")
+    if dtype is not None and t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
+def copy_from_device_ptr(ptr: int, n: int, dtype, device, stream=None, free=True):
+    """Copies n elements at a library-owned device pointer into a new tensor and
+    (optionally) frees the pointer through the library allocator."""
+    out = torch.empty(n, dtype=dtype, device=device)
+    sp = _stream_ptr(stream)
+    if n and ptr:
+        nbytes = n * out.element_size()
+        rc = L.hip().hipMemcpyAsync(ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ptr), nbytes,
+                                    L.HIP_MEMCPY_D2D, sp)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpyAsync failed ({rc})")
+    if free and ptr:
+        torch.cuda.current_stream().synchronize() if stream is None else stream.synchronize()
+        L.check(L.load().amgt_free(ctypes.c_void_p(ptr)), "amgt_free")
+    return out
+
+
+class MBSRMatrix:
+    """A device mBSR matrix (4x4 blocks, 16-bit occupancy bitmaps, compressed values)."""
+
+    def __init__(self, handle: ctypes.c_void_p, device: torch.device):
+        self._h = handle
+        self.device = device
+
+    # -- construction -----------------------------------------------------------------
+    @classmethod
+    def from_csr(cls, rowptr: torch.Tensor, colidx: torch.Tensor, values, nrows: int, ncols: int,
+                 stream=None) -> "MBSRMatrix":
+        """CSR2BSR_GPU (csr_matvec_device.c:1201-1267).  values may be None (pattern only)."""
+        _need_cuda(rowptr, "rowptr", torch.int32)
+        _need_cuda(colidx, "colidx", torch.int32)
+        _need_cuda(values, "values", torch.float64)
+        h = ctypes.c_void_p()
+        st = L.load().amgt_mbsr_create_from_csr(nrows, ncols, colidx.numel(), _ptr(rowptr),
+                                                _ptr(colidx), _ptr(values), _stream_ptr(stream),
+                                                ctypes.byref(h))
+        L.check(st, "amgt_mbsr_create_from_csr")
+        return cls(h, rowptr.device)
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        if self._h is None:
+            raise ValueError("matrix already destroyed")
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.load().amgt_mbsr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- properties -----------------------------------------------------------------------
+    def info(self) -> dict:
+        inf = L.MbsrInfo()
+        L.check(L.load().amgt_mbsr_get_info(self.handle, ctypes.byref(inf)), "amgt_mbsr_get_info")
+        return {f: getattr(inf, f) for f, _ in L.MbsrInfo._fields_}
+
+    @property
+    def shape(self):
+        i = self.info()
+        return (i["nrows"], i["ncols"])
+
+    def arrays(self, stream=None) -> dict:
+        """Device copies of blk_ptr, blk_col, blk_map, val_ptr, val."""
+        inf = self.info()
+        arr = L.MbsrArrays()
+        L.check(L.load().amgt_mbsr_get_arrays(self.handle, ctypes.byref(arr)), "amgt_mbsr_get_arrays")
+        dev = self.device
+        out = {
+            "blk_ptr": copy_from_device_ptr(arr.blk_ptr, inf["brow"] + 1, torch.int32, dev, stream, False),
+            "blk_col": copy_from_device_ptr(arr.blk_col, inf["nblk"], torch.int32, dev, stream, False),
+            "blk_map": copy_from_device_ptr(arr.blk_map, inf["nblk"], torch.int16, dev, stream, False),
+            "val_ptr": copy_from_device_ptr(arr.val_ptr, inf["brow"] + 1, torch.int32, dev, stream, False),
+            "val": copy_from_device_ptr(arr.val, inf["nnz"] if arr.val else 0, torch.float64, dev,
+                                        stream, False),
+        }
+        return out
+
+    def set_level(self, level: int):
+        L.check(L.load().amgt_mbsr_set_level(self.handle, level), "amgt_mbsr_set_level")
+
+    def set_precision(self, precision: int):
+        L.check(L.load().amgt_mbsr_set_precision(self.handle, precision), "amgt_mbsr_set_precision")
+
+    def set_spmv_lanes(self, lanes: int):
+        L.check(L.load().amgt_mbsr_set_spmv_lanes(self.handle, lanes), "amgt_mbsr_set_spmv_lanes")
+
+    def autotune(self, stream=None):
+        L.check(L.load().amgt_mbsr_autotune(self.handle, _stream_ptr(stream)), "amgt_mbsr_autotune")
+
+    def update_values(self, rowptr, colidx, values, stream=None):
+        L.check(L.load().amgt_mbsr_update_values(self.handle, _ptr(rowptr), _ptr(colidx),
+                                                 _ptr(values), _stream_ptr(stream)),
+                "amgt_mbsr_update_values")
+
+    # -- products ----------------------------------------------------------------------------
+    def spmv(self, x: torch.Tensor, y: torch.Tensor | None = None, alpha: float = 1.0,
+             beta: float = 0.0, b: torch.Tensor | None = None, trans: bool = False,
+             stream=None) -> torch.Tensor:
+        """y = alpha*op(A)*x + beta*b (b defaults to y)."""
+        nrows, ncols = self.shape
+        ny = ncols if trans else nrows
+        _need_cuda(x, "x", torch.float64)
+        if y is None:
+            y = torch.empty(ny, dtype=torch.float64, device=self.device)
+        _need_cuda(y, "y", torch.float64)
+        if b is None and beta != 0.0:
+            b = y
+        _need_cuda(b, "b", torch.float64)
+        st = L.load().amgt_mbsr_spmv(self.handle, int(bool(trans)), float(alpha), _ptr(x),
+                                     float(beta), _ptr(b), _ptr(y), _stream_ptr(stream))
+        L.check(st, "amgt_mbsr_spmv")
+        return y
+
+    def spmm(self, X: torch.Tensor, Y: torch.Tensor | None = None, alpha: float = 1.0,
+             beta: float = 0.0, B: torch.Tensor | None = None, trans: bool = False,
+             stream=None) -> torch.Tensor:
+        """Y = alpha*op(A)*X + beta*B for k columns.  X is (k, n) row-major, i.e. each
+        right-hand side contiguous (HYPRE multivector storage method 0)."""
+        nrows, ncols = self.shape
+        ny = ncols if trans else nrows
+        _need_cuda(X, "X", torch.float64)
+        k = X.shape[0]
+        if Y is None:
+            Y = torch.empty((k, ny), dtype=torch.float64, device=self.device)
+        _need_cuda(Y, "Y", torch.float64)
+        if B is None and beta != 0.0:
+            B = Y
+        _need_cuda(B, "B", torch.float64)
+        st = L.load().amgt_mbsr_spmm(self.handle, int(bool(trans)), k, float(alpha), _ptr(X),
+                                     X.stride(0), float(beta), _ptr(B),
+                                     B.stride(0) if B is not None else 0, _ptr(Y), Y.stride(0),
+                                     _stream_ptr(stream))
+        L.check(st, "amgt_mbsr_spmm")
+        return Y
+
+    def spgemm(self, other: "MBSRMatrix", stream=None) -> "MBSRMatrix":
+        h = ctypes.c_void_p()
+        st = L.load().amgt_mbsr_spgemm(self.handle, other.handle, _stream_ptr(stream), ctypes.byref(h))
+        L.check(st, "amgt_mbsr_spgemm")
+        return MBSRMatrix(h, self.device)
+
+    def __matmul__(self, other):
+        if isinstance(other, MBSRMatrix):
+            return self.spgemm(other)
+        return self.spmv(other)
+
+    def to_csr(self, stream=None):
+        """(rowptr, colidx, values) with ascending columns per row; values None if
+        the matrix is pattern-only."""
+        nnz = ctypes.c_int64()
+        pi, pj, pv = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        st = L.load().amgt_mbsr_to_csr(self.handle, _stream_ptr(stream), ctypes.byref(nnz),
+                                       ctypes.byref(pi), ctypes.byref(pj), ctypes.byref(pv))
+        L.check(st, "amgt_mbsr_to_csr")
+        nrows = self.info()["nrows"]
+        I = copy_from_device_ptr(pi.value, nrows + 1, torch.int32, self.device, stream)
+        J = copy_from_device_ptr(pj.value, nnz.value, torch.int32, self.device, stream)
+        V = (copy_from_device_ptr(pv.value, nnz.value, torch.float64, self.device, stream)
+             if pv.value else None)
+        return I, J, V
+
+
+def set_mixed_precision(enable: bool) -> None:
+    L.check(L.load().amgt_set_mixed_precision(int(bool(enable))), "amgt_set_mixed_precision")
+
+
+def precision_for_level(level: int) -> int:
+    return int(L.load().amgt_precision_for_level(level))

@@ -1,0 +1,118 @@
+"""Synthetic inputs for the configurations the path is measured on.
+
+Vectorised numpy generators (independent of the CPU oracle, which checks them):
+
+* HYPRE stencil matrices, rows diagonal-first then ascending columns, as
+  GenerateLaplacian / 9pt / 27pt build them (src/parcsr_ls/par_laplace*.c);
+  7pt couplings c = (cx, cy, cz) give diagonal sum(2c) over dimensions > 1
+  (src/test/ij.c:9565-9580).
+* HYPRE's Park-Miller stream (src/utilities/random.c:43-101) for right-hand
+  sides: uniform 2U-1 (src/seq_mv/vector.c:395-418) and N(0,1) by Box-Muller
+  on consecutive pairs.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+PM_A, PM_M = 16807, 2147483647
+
+
+def _offsets(kind: int):
+    if kind == 7:
+        return [(0, 0, -1), (0, -1, 0), (-1, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)]
+    if kind == 9:
+        return [(i, j, 0) for j in (-1, 0, 1) for i in (-1, 0, 1) if i or j]
+    if kind == 27:
+        return [(i, j, l) for l in (-1, 0, 1) for j in (-1, 0, 1) for i in (-1, 0, 1)
+                if i or j or l]
+    raise ValueError(f"unknown stencil kind {kind}")
+
+
+def stencil(kind: int, nx: int, ny: int, nz: int = 1, c=(1.0, 1.0, 1.0)):
+    """CSR (I int32, J int32, A float64) of a HYPRE stencil matrix."""
+    offs = _offsets(kind)
+    n = nx * ny * nz
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    x, y, z = x.ravel(), y.ravel(), z.ravel()
+    row = np.arange(n, dtype=np.int64)
+    if kind == 7:
+        diag = (2.0 * c[0] if nx > 1 else 0.0) + (2.0 * c[1] if ny > 1 else 0.0) + \
+               (2.0 * c[2] if nz > 1 else 0.0)
+        offv = [-(c[0] if dx else (c[1] if dy else c[2])) for dx, dy, dz in offs]
+    else:
+        diag = 8.0 if kind == 9 else 26.0
+        offv = [-1.0] * len(offs)
+    # per-row slot masks: slot 0 diagonal, then neighbours in ascending column order
+    cols = [row]
+    vals = [np.full(n, diag)]
+    masks = [np.ones(n, dtype=bool)]
+    for (dx, dy, dz), v in zip(offs, offv):
+        xx, yy, zz = x + dx, y + dy, z + dz
+        m = (xx >= 0) & (xx < nx) & (yy >= 0) & (yy < ny) & (zz >= 0) & (zz < nz)
+        cols.append(row + dz * nx * ny + dy * nx + dx)
+        vals.append(np.full(n, v))
+        masks.append(m)
+    C = np.stack(cols, axis=1)
+    V = np.stack(vals, axis=1)
+    Mk = np.stack(masks, axis=1)
+    counts = Mk.sum(axis=1)
+    I = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(counts, out=I[1:])
+    J = C[Mk].astype(np.int32)
+    A = V[Mk].astype(np.float64)
+    return I.astype(np.int32), J, A
+
+
+def pm_stream(seed: int, n: int) -> np.ndarray:
+    """n consecutive hypre_RandI values after hypre_SeedRand(seed), as int64.
+
+    Computed by jumping: x_k = a^k * x_0 mod m, in blocks with exact int64
+    arithmetic (a^k mod m < 2^31, so products fit in 62 bits)."""
+    s = 1 if seed < 1 else (PM_M - 1 if seed >= PM_M else int(seed))
+    out = np.empty(n, dtype=np.int64)
+    blk = 4096
+    # powers a^1..a^blk mod m
+    pw = np.empty(blk, dtype=np.int64)
+    p = 1
+    for k in range(blk):
+        p = (p * PM_A) % PM_M
+        pw[k] = p
+    a_blk = int(pw[-1])
+    cur = s
+    for start in range(0, n, blk):
+        m = min(blk, n - start)
+        out[start:start + m] = (pw[:m] * cur) % PM_M
+        cur = (cur * a_blk) % PM_M
+    return out
+
+
+def rand_uniform(seed: int, n: int) -> np.ndarray:
+    """hypre_SeqVectorSetRandomValues: 2*Rand - 1 with Rand = RandI/m."""
+    return 2.0 * (pm_stream(seed, n).astype(np.float64) / float(PM_M)) - 1.0
+
+
+def rand_normal(seed: int, n: int) -> np.ndarray:
+    """N(0,1): Box-Muller on consecutive (u1, u2) pairs of the Park-Miller stream."""
+    m = n + (n & 1)
+    u = pm_stream(seed, m).astype(np.float64) / float(PM_M)
+    u1, u2 = u[0::2], u[1::2]
+    rad = np.sqrt(-2.0 * np.log(u1))
+    out = np.empty(m, dtype=np.float64)
+    out[0::2] = rad * np.cos(2.0 * np.pi * u2)
+    out[1::2] = rad * np.sin(2.0 * np.pi * u2)
+    return out[:n]
+
+
+# Configurations (sizes the path is measured on)
+CONFIGS = {
+    "c1_5pt_128": dict(kind=7, n=(128, 128, 1), c=(1.0, 1.0, 1.0), rhs=("uniform", 0)),
+    "c2_7pt_128": dict(kind=7, n=(128, 128, 128), c=(1.0, 1.0, 1.0), rhs=("normal", 1)),
+    "c3_7pt_aniso_160": dict(kind=7, n=(160, 160, 160), c=(1.0, 1.0, 0.01), rhs=("normal", 2)),
+    "c4_27pt_64": dict(kind=27, n=(64, 64, 64), c=(1.0, 1.0, 1.0), rhs=("normal", 3)),
+}
+
+
+def config_matrix(name: str):
+    cfg = CONFIGS[name]
+    nx, ny, nz = cfg["n"]
+    return stencil(cfg["kind"], nx, ny, nz, cfg["c"])

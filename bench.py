@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Benchmark of the mBSR path on one MI355X.
+
+Workload: 3D 7-point Poisson on 128^3 (2,097,152 rows, 14,581,760 nnz, FP64;
+x ~ N(0,1) from HYPRE's Park-Miller stream, seed 1).  One step = one FP64 mBSR
+SpMV y = A*x over the whole finest-level matrix (the solve-phase hot path behind
+hypre_CSRMatrixMatvecDevice), inputs resident in HBM.
+
+value = algorithmic bytes per SpMV / wall time per step (GB/s), where
+  B_alg = 8*nnz + 6*nblk + 4*(brow+1) + 8*ncols + 8*nrows     (beta = 0)
+counts each structural nonzero, block index/bitmap, block-row pointer and
+vector entry once.  roofline.achieved uses the same bytes over the kernel's
+average duration from HIP events on the launch stream.
+
+Also reported (not the headline value): conversion time, per-variant SpMV
+times, and the residual form y = b - A x.
+
+cpu_baseline: the oracle's restatement of HYPRE's host CSR SpMV
+(hypre_CSRMatrixMatvecOutOfPlaceHost), one thread, on the same matrix.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def spmv_bytes(info):
+    return (8 * info["nnz"] + 6 * info["nblk"] + 4 * (info["brow"] + 1) + 8 * info["ncols"]
+            + 8 * info["nrows"])
+
+
+def cpu_baseline(I, J, A, x, bytes_per_call, budget_s=10.0):
+    from oracle import oracle as O
+
+    n = len(I) - 1
+    O.csr_matvec(n, I, J, A, 1.0, x)  # warm
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        O.csr_matvec(n, I, J, A, 1.0, x)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    per = el / reps
+    return {"value": round(bytes_per_call / per / 1e9, 3), "unit": "GB/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{reps} HYPRE host CSR SpMVs (oracle restatement of "
+                      f"hypre_CSRMatrixMatvecOutOfPlaceHost, 1 thread) on the same 128^3 7-pt "
+                      f"matrix, {el:.1f} s; {per * 1e3:.2f} ms per SpMV"}
+
+
+def load_traffic(kernel_key):
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        d = json.load(open(p))
+        v = d.get(kernel_key)
+        return None if v is None else float(v["bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--lanes", type=int, default=0, help="force lanes per block row (0 = tuner)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the mBSR path is "
+              "single-GPU in this build", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from amgt_amd import workloads as W
+    from amgt_amd.mbsr import MBSRMatrix
+
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    I, J, A = W.stencil(7, 128, 128, 128)
+    n = len(I) - 1
+    x_h = W.rand_normal(1, n)
+    gen_s = time.perf_counter() - t0
+    dI, dJ, dA = (torch.from_numpy(v).to(dev) for v in (I, J, A))
+    x = torch.from_numpy(x_h).to(dev)
+    y = torch.empty(n, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    tc = time.perf_counter()
+    M = MBSRMatrix.from_csr(dI, dJ, dA, n, n)
+    torch.cuda.synchronize()
+    conv_ms = (time.perf_counter() - tc) * 1e3
+    info = M.info()
+
+    # per-variant timings (lanes per block row), then the tuner's pick
+    variants = {}
+    for lanes in (4, 8, 16, 32, 64):
+        M.set_spmv_lanes(lanes)
+        for _ in range(10):
+            M.spmv(x, y)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(50):
+            M.spmv(x, y)
+        e1.record()
+        torch.cuda.synchronize()
+        variants[lanes] = e0.elapsed_time(e1) / 50
+    if args.lanes:
+        M.set_spmv_lanes(args.lanes)
+    else:
+        M.set_spmv_lanes(0)
+        M.autotune()
+    lanes = M.info()["spmv_lanes"]
+
+    for _ in range(args.warmup):
+        M.spmv(x, y)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    w0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        M.spmv(x, y)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    kern_ms = e0.elapsed_time(e1) / args.steps
+    ms_per_step = wall * 1e3 / args.steps
+
+    # residual form y = b - A x (the V-cycle residual, beta = 1)
+    b = torch.ones(n, dtype=torch.float64, device=dev)
+    r = torch.empty_like(b)
+    for _ in range(10):
+        M.spmv(x, r, alpha=-1.0, beta=1.0, b=b)
+    e0.record()
+    for _ in range(50):
+        M.spmv(x, r, alpha=-1.0, beta=1.0, b=b)
+    e1.record()
+    torch.cuda.synchronize()
+    resid_ms = e0.elapsed_time(e1) / 50
+
+    B = spmv_bytes(info)
+    value = B / (ms_per_step * 1e-3) / 1e9
+    achieved = B / (kern_ms * 1e-3) / 1e9
+    out = {
+        "metric": "AMG setup+solve s to 1e-8 resid, mixed vs FP64; SpMV HBM GB/s % of peak",
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (HYPRE 7-pt Laplacian generator, x ~ N(0,1) Park-Miller seed 1)",
+        "config": {"workload": "fp64 mBSR SpMV y=A*x, 3D 7-pt Poisson 128^3 (finest level)",
+                   "rows": n, "nnz": int(info["nnz"]), "blocks": int(info["nblk"]),
+                   "block_fill": round(info["fill"], 4), "lanes_per_block_row": lanes,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": load_traffic("spmv_c2")},
+        "spmv_alg_bytes": int(B),
+        "kernel_ms": round(kern_ms, 5),
+        "residual_form_ms": round(resid_ms, 5),
+        "lanes_variant_ms": {str(k): round(v, 5) for k, v in variants.items()},
+        "conversion_ms": round(conv_ms, 2),
+        "input_gen_s": round(gen_s, 2),
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(I, J, A, x_h, B)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,150 @@
+/*
+ * amgt_mbsr.h -- C-ABI of the MI355X (gfx950) mBSR library.
+ *
+ * The library replaces AmgT's mBSR engine inside HYPRE's sequential CSR layer
+ * (reference: AmgT_HYPRE/src/seq_mv/csr_matvec_device.c and
+ * csr_spgemm_device.c).  Plain C types only: device pointers, sizes, a HIP
+ * stream passed as `void *` (hipStream_t), int status codes.  Every entry point
+ * is stream-ordered on the stream it is given.  No entry point keeps global
+ * mutable state except the allocator hooks and the last-error string.
+ *
+ * mBSR layout (device memory, owned by the handle):
+ *   blk_ptr[brow+1]  int32   first block of each 4-row block row
+ *   blk_col[nblk]    int32   block column, strictly increasing within a row
+ *   blk_map[nblk]    uint16  occupancy bitmap, bit r*4+c = entry (r, c)
+ *   val_ptr[brow+1]  int32   first value of each block row (= CSR I[4*br])
+ *   val[nnz]         double  nonzeros only, in (block, bit) order
+ * Semantics of blk_ptr/blk_col/blk_map follow bsrMAT, seq_mv.h:105-129; the
+ * reference stores 16 dense values per block (csr_matvec_device.c:936-945),
+ * this library stores only the nonzeros.
+ */
+#ifndef AMGT_MBSR_H
+#define AMGT_MBSR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+enum {
+    AMGT_OK = 0,
+    AMGT_ERR_ARG = 1,          /* invalid argument (null pointer, negative size, x == y ...) */
+    AMGT_ERR_DIM = 2,          /* incompatible dimensions (csr_matop_device.c:257-263) */
+    AMGT_ERR_ALLOC = 3,        /* device allocation failed */
+    AMGT_ERR_HIP = 4,          /* a HIP runtime call failed */
+    AMGT_ERR_INPUT = 5,        /* CSR input invalid: duplicate entry or column out of range */
+    AMGT_ERR_UNSUPPORTED = 6   /* valid input outside the sizes this build handles */
+};
+
+/* AMGT_PRECISION, seq_mv.h:131-136 */
+enum { AMGT_DOUBLE = 0, AMGT_FLOAT = 1, AMGT_HALF = 2 };
+
+typedef struct amgt_mbsr *amgt_mbsr_t;
+
+typedef struct {
+    int32_t nrows, ncols, brow, bcol;
+    int64_t nblk, nnz;
+    double stand;       /* std-dev of blocks per block row (getStand, csr_matvec_device.c:638-671) */
+    double avg_nnz;     /* nnz / nblk (csr_matvec_device.c:1225) */
+    double fill;        /* nnz / (16 * nblk) */
+    int32_t max_row_blocks;
+    int32_t level;      /* coarsening level tag (hypre_Level, seq_mv.h:199) */
+    int32_t precision;  /* value precision used by SpMV (AMGT_DOUBLE/FLOAT/HALF) */
+    int32_t spmv_lanes; /* lanes per block row chosen by the tuner (4..64) */
+} amgt_mbsr_info_t;
+
+typedef struct {
+    const int32_t *blk_ptr, *blk_col, *val_ptr;
+    const uint16_t *blk_map;
+    const double *val;
+} amgt_mbsr_arrays_t;
+
+/* ---- library ---- */
+const char *amgt_status_string(int status);
+/* message of the last failing call on the calling thread */
+const char *amgt_last_error(void);
+/* Device allocator hooks.  HYPRE passes wrappers of hypre_TAlloc/hypre_TFree
+ * (HYPRE_MEMORY_DEVICE) so that CSR arrays returned by amgt_csr_spgemm are
+ * owned and freed by hypre_CSRMatrixDestroy.  NULL restores hipMalloc/hipFree. */
+typedef void *(*amgt_alloc_fn)(size_t bytes, void *ctx);
+typedef void (*amgt_free_fn)(void *ptr, void *ctx);
+int amgt_set_allocator(amgt_alloc_fn alloc_fn, amgt_free_fn free_fn, void *ctx);
+/* level -> precision policy, get_calculationPrecison csr_spgemm_device.c:1847-1862 */
+int amgt_precision_for_level(int level);
+/* 1: SpMV/SpGEMM follow the level policy (reference MIXED_PRESION build,
+ * seq_mv.h:75); 0 (default): FP64 everywhere (reference AmgT_FP64 build). */
+int amgt_set_mixed_precision(int enable);
+
+/* ---- mBSR handle ---- */
+/* CSR2BSR_GPU, csr_matvec_device.c:1201-1267.  Rows may be in any column
+ * order (HYPRE keeps the diagonal first).  d_val may be NULL (pattern only). */
+int amgt_mbsr_create_from_csr(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *d_rowptr,
+                              const int32_t *d_colidx, const double *d_val, void *stream,
+                              amgt_mbsr_t *out);
+/* frees the handle and every array it owns (csr_matrix.c:94-103) */
+int amgt_mbsr_destroy(amgt_mbsr_t A);
+/* Re-reads values from a CSR with the same pattern (the reference never
+ * invalidates its cache; this is the refresh hook). */
+int amgt_mbsr_update_values(amgt_mbsr_t A, const int32_t *d_rowptr, const int32_t *d_colidx,
+                            const double *d_val, void *stream);
+int amgt_mbsr_get_info(amgt_mbsr_t A, amgt_mbsr_info_t *info);
+int amgt_mbsr_get_arrays(amgt_mbsr_t A, amgt_mbsr_arrays_t *arrays);
+int amgt_mbsr_set_level(amgt_mbsr_t A, int level);
+/* explicit SpMV value precision; overrides the level policy for this handle */
+int amgt_mbsr_set_precision(amgt_mbsr_t A, int precision);
+/* lanes per block row for SpMV: 4, 8, 16, 32 or 64; 0 = choose from the
+ * matrix statistics */
+int amgt_mbsr_set_spmv_lanes(amgt_mbsr_t A, int lanes);
+/* times every lanes-per-block-row variant on this matrix and keeps the fastest */
+int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream);
+
+/* ---- SpMV: y = alpha*op(A)*x + beta*b ----
+ * spmv_amgT_fp64 csr_matvec_device.c:1302-1411 (+ fp32 2205-2309, fp16
+ * 2098-2203).  op = A (trans 0) or A^T (trans 1, through a cached transposed
+ * mBSR).  b may equal y; x must not alias y.  beta == 0 never reads b.
+ * Values in the handle's precision; FP32 accumulation below FP64; x, b, y are
+ * FP64. */
+int amgt_mbsr_spmv(amgt_mbsr_t A, int trans, double alpha, const double *d_x, double beta,
+                   const double *d_b, double *d_y, void *stream);
+/* k right-hand sides, column-major with leading dimensions ldx/ldb/ldy
+ * (HYPRE multivector storage method 0, vector.c) */
+int amgt_mbsr_spmm(amgt_mbsr_t A, int trans, int k, double alpha, const double *d_x, int64_t ldx,
+                   double beta, const double *d_b, int64_t ldb, double *d_y, int64_t ldy,
+                   void *stream);
+
+/* ---- SpGEMM: C = A*B on mBSR (spgemm_amgT_fp64 csr_spgemm_device.c:1527-1846) ---- */
+int amgt_mbsr_spgemm(amgt_mbsr_t A, amgt_mbsr_t B, void *stream, amgt_mbsr_t *C);
+/* mBSR -> CSR (BSR2CSR_step1-3, csr_spgemm_device.c:1501-1525), columns
+ * ascending within a row; arrays come from the allocator hooks */
+int amgt_mbsr_to_csr(amgt_mbsr_t A, void *stream, int64_t *nnz, int32_t **d_rowptr,
+                     int32_t **d_colidx, double **d_val);
+/* frees a pointer obtained from this library's allocator */
+int amgt_free(void *ptr);
+
+/* ---- HYPRE-shaped entry points (plain pointers; the mBSR cache lives in a
+ *      caller-owned slot, the `bsr_mat` field of hypre_CSRMatrix,
+ *      seq_mv.h:156-163) ---- */
+/* hypreDevice_CSRSpGemm, csr_spgemm_device.c:2579-2608 (seq_mv.h:727).
+ * A_data or B_data NULL: pattern only, C_data returned NULL.  nnzA or
+ * nnzB == 0: C has a zeroed row pointer (csr_spgemm_device.c:2633-2638). */
+int amgt_csr_spgemm(int32_t m, int32_t k, int32_t n,
+                    int64_t nnzA, const int32_t *A_i, const int32_t *A_j, const double *A_data,
+                    amgt_mbsr_t *A_cache,
+                    int64_t nnzB, const int32_t *B_i, const int32_t *B_j, const double *B_data,
+                    amgt_mbsr_t *B_cache,
+                    int64_t *nnzC, int32_t **C_i, int32_t **C_j, double **C_data,
+                    amgt_mbsr_t *C_cache, void *stream);
+/* hypre_CSRMatrixMatvecDevice, csr_matvec_device.c:128-191 (seq_mv.h:562):
+ * y = alpha*op(A)*x + beta*b, num_vectors == 1, offset == 0. */
+int amgt_csr_matvec(int trans, double alpha, int32_t nrows, int32_t ncols, int64_t nnz,
+                    const int32_t *A_i, const int32_t *A_j, const double *A_data,
+                    amgt_mbsr_t *A_cache, const double *d_x, double beta, const double *d_b,
+                    double *d_y, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,622 @@
+/*
+ * CPU oracle for the mBSR data path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Loaded only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg, as the checker.  The product library (amgt_amd/libamgt_mbsr.so) never
+ * links or calls it.  Each function restates the reference algorithm whose
+ * file:line it cites (relative to the reference tree; `src/` = AmgT_HYPRE/src/).
+ * How each function is pinned is listed in tests/test_oracle.py.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* Park-Miller minimal standard generator (Schrage decomposition).
+ * src/utilities/random.c:43-101. */
+#define PM_A 16807
+#define PM_M 2147483647
+#define PM_Q 127773
+#define PM_R 2836
+
+int32_t orc_pm_seed(int64_t seed)
+{
+    /* random.c:56-63: seed < 1 -> 1, seed >= m -> m-1 */
+    if (seed < 1) return 1;
+    if (seed >= PM_M) return PM_M - 1;
+    return (int32_t)seed;
+}
+
+int32_t orc_pm_next(int32_t *state)
+{
+    int32_t s = *state;
+    int32_t high = s / PM_Q;
+    int32_t low = s % PM_Q;
+    int32_t test = PM_A * low - PM_R * high;
+    *state = (test > 0) ? test : test + PM_M;
+    return *state;
+}
+
+static double pm_rand(int32_t *state) /* hypre_Rand, random.c:99-101: (0, 1] */
+{
+    return (double)orc_pm_next(state) / (double)PM_M;
+}
+
+void orc_rand_uniform(int64_t seed, int64_t n, double *out)
+{
+    /* vector.c:406-413 */
+    int32_t st = orc_pm_seed(seed);
+    for (int64_t i = 0; i < n; i++) out[i] = 2.0 * pm_rand(&st) - 1.0;
+}
+
+void orc_rand_normal(int64_t seed, int64_t n, double *out)
+{
+    /* Box-Muller on consecutive pairs of the same stream. u1 in (0,1] so the
+     * log is finite. */
+    int32_t st = orc_pm_seed(seed);
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int64_t i = 0; i < n; i += 2) {
+        double u1 = pm_rand(&st);
+        double u2 = pm_rand(&st);
+        double rad = sqrt(-2.0 * log(u1));
+        out[i] = rad * cos(two_pi * u2);
+        if (i + 1 < n) out[i + 1] = rad * sin(two_pi * u2);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Stencils.  Single-process restatement of GenerateLaplacian
+ * (src/parcsr_ls/par_laplace.c:202-300: diagonal first, then -z,-y,-x,+x,+y,+z),
+ * GenerateLaplacian9pt (par_laplace_9pt.c:243-360) and GenerateLaplacian27pt
+ * (par_laplace_27pt.c:778-1300): diagonal first, neighbours in ascending
+ * (dz, dy, dx) order, which is ascending column order. */
+static int stencil_offsets(int kind, int *dx, int *dy, int *dz)
+{
+    int k = 0;
+    if (kind == 7) {
+        const int o[6][3] = {{0, 0, -1}, {0, -1, 0}, {-1, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+        for (int t = 0; t < 6; t++) { dx[k] = o[t][0]; dy[k] = o[t][1]; dz[k] = o[t][2]; k++; }
+    } else if (kind == 9) {
+        for (int j = -1; j <= 1; j++)
+            for (int i = -1; i <= 1; i++)
+                if (i || j) { dx[k] = i; dy[k] = j; dz[k] = 0; k++; }
+    } else if (kind == 27) {
+        for (int l = -1; l <= 1; l++)
+            for (int j = -1; j <= 1; j++)
+                for (int i = -1; i <= 1; i++)
+                    if (i || j || l) { dx[k] = i; dy[k] = j; dz[k] = l; k++; }
+    } else {
+        return -1;
+    }
+    return k;
+}
+
+int64_t orc_stencil_nnz(int kind, int nx, int ny, int nz)
+{
+    int dx[26], dy[26], dz[26];
+    int k = stencil_offsets(kind, dx, dy, dz);
+    if (k < 0) return -1;
+    int64_t nnz = 0;
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
+                nnz++;
+                for (int t = 0; t < k; t++) {
+                    int xx = x + dx[t], yy = y + dy[t], zz = z + dz[t];
+                    if (xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz) nnz++;
+                }
+            }
+    return nnz;
+}
+
+int orc_stencil(int kind, int nx, int ny, int nz, double cx, double cy, double cz,
+                int *I, int *J, double *A)
+{
+    int dx[26], dy[26], dz[26];
+    int k = stencil_offsets(kind, dx, dy, dz);
+    if (k < 0) return -1;
+    double diag, off[26];
+    if (kind == 7) {
+        /* ij.c:9565-9580: diagonal = sum of 2c over dimensions longer than 1 */
+        diag = 0.0;
+        if (nx > 1) diag += 2.0 * cx;
+        if (ny > 1) diag += 2.0 * cy;
+        if (nz > 1) diag += 2.0 * cz;
+        for (int t = 0; t < k; t++) off[t] = dx[t] ? -cx : (dy[t] ? -cy : -cz);
+    } else {
+        diag = (kind == 9) ? 8.0 : 26.0;
+        for (int t = 0; t < k; t++) off[t] = -1.0;
+    }
+    int64_t cnt = 0;
+    int row = 0;
+    const int64_t nxy = (int64_t)nx * ny;
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
+                I[row] = (int)cnt;
+                J[cnt] = row;
+                A[cnt++] = diag;
+                for (int t = 0; t < k; t++) {
+                    int xx = x + dx[t], yy = y + dy[t], zz = z + dz[t];
+                    if (xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
+                        J[cnt] = (int)(zz * nxy + (int64_t)yy * nx + xx);
+                        A[cnt++] = off[t];
+                    }
+                }
+                row++;
+            }
+    I[row] = (int)cnt;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* bitMatrixMul, csr_spgemm_device.c:189-205: OR over k of
+ * Bitmap_col_x_row(col_k(A), row_k(B)) = outer product of column k of A
+ * (bits 4r+k) with row k of B (bits 4k+c). */
+uint16_t orc_bitmap_mul(uint16_t a, uint16_t b)
+{
+    uint32_t res = 0;
+    uint32_t ca = a, rb = b;
+    for (int k = 0; k < 4; k++) {
+        uint32_t col = ca & 0x1111u;      /* A[r][k] at bit 4r */
+        uint32_t row = rb & 0xFu;         /* B[k][c] at bit c  */
+        res |= (col * 0xFu) & (row * 0x1111u);
+        ca >>= 1;
+        rb >>= 4;
+    }
+    return (uint16_t)res;
+}
+
+/* ------------------------------------------------------------------------ */
+/* CSR -> mBSR.  Structure per csr_matvec_device.c:683-1092: a block (br,bc)
+ * exists iff an entry (r,c) of rows 4br..4br+3 has c/4 == bc; block columns
+ * ascending; bit (r%4)*4 + c%4.  Values are stored compressed: the nonzeros of
+ * a block row in (block column, bit) order, starting at I[4br]. */
+typedef struct { uint32_t bc; uint32_t bit; double v; } ent_t;
+
+static int ent_cmp(const void *p, const void *q)
+{
+    const ent_t *a = (const ent_t *)p, *b = (const ent_t *)q;
+    if (a->bc != b->bc) return a->bc < b->bc ? -1 : 1;
+    if (a->bit != b->bit) return a->bit < b->bit ? -1 : 1;
+    return 0;
+}
+
+/* gathers and sorts the entries of block row br; returns the entry count or
+ * -1 (duplicate) / -2 (column out of range) */
+static int64_t gather_block_row(int nrows, int ncols, const int *I, const int *J,
+                                const double *A, int br, ent_t **buf, int64_t *cap)
+{
+    int r0 = 4 * br;
+    int r1 = r0 + 4 < nrows ? r0 + 4 : nrows;
+    int64_t n = (int64_t)I[r1] - I[r0];
+    if (n > *cap) {
+        *cap = n * 2 + 16;
+        *buf = (ent_t *)realloc(*buf, sizeof(ent_t) * (size_t)*cap);
+    }
+    int64_t t = 0;
+    for (int r = r0; r < r1; r++)
+        for (int p = I[r]; p < I[r + 1]; p++) {
+            int c = J[p];
+            if (c < 0 || c >= ncols) return -2;
+            (*buf)[t].bc = (uint32_t)(c >> 2);
+            (*buf)[t].bit = (uint32_t)(((r & 3) << 2) | (c & 3));
+            (*buf)[t].v = A ? A[p] : 0.0;
+            t++;
+        }
+    qsort(*buf, (size_t)n, sizeof(ent_t), ent_cmp);
+    for (int64_t q = 1; q < n; q++)
+        if ((*buf)[q].bc == (*buf)[q - 1].bc && (*buf)[q].bit == (*buf)[q - 1].bit) return -1;
+    return n;
+}
+
+int64_t orc_csr2mbsr_count(int nrows, int ncols, const int *I, const int *J, int *blk_ptr)
+{
+    int brow = (nrows + 3) / 4;
+    ent_t *buf = NULL;
+    int64_t cap = 0, total = 0;
+    blk_ptr[0] = 0;
+    for (int br = 0; br < brow; br++) {
+        int64_t n = gather_block_row(nrows, ncols, I, J, NULL, br, &buf, &cap);
+        if (n < 0) { free(buf); return n; }
+        int64_t cnt = 0;
+        for (int64_t q = 0; q < n; q++)
+            if (q == 0 || buf[q].bc != buf[q - 1].bc) cnt++;
+        total += cnt;
+        blk_ptr[br + 1] = (int)total;
+    }
+    free(buf);
+    return total;
+}
+
+int orc_csr2mbsr_fill(int nrows, int ncols, const int *I, const int *J, const double *A,
+                      const int *blk_ptr, int *blk_col, uint16_t *blk_map, int *val_ptr,
+                      double *val)
+{
+    int brow = (nrows + 3) / 4;
+    ent_t *buf = NULL;
+    int64_t cap = 0;
+    for (int br = 0; br < brow; br++) {
+        int64_t n = gather_block_row(nrows, ncols, I, J, A, br, &buf, &cap);
+        if (n < 0) { free(buf); return (int)n; }
+        int64_t base = I[4 * br];
+        val_ptr[br] = (int)base;
+        int b = blk_ptr[br] - 1;
+        for (int64_t q = 0; q < n; q++) {
+            if (q == 0 || buf[q].bc != buf[q - 1].bc) {
+                b++;
+                blk_col[b] = (int)buf[q].bc;
+                blk_map[b] = 0;
+            }
+            blk_map[b] |= (uint16_t)(1u << buf[q].bit);
+            val[base + q] = buf[q].v;
+        }
+    }
+    val_ptr[brow] = I[nrows];
+    free(buf);
+    return 0;
+}
+
+void orc_mbsr_dense_values(int brow, const int *blk_ptr, const uint16_t *blk_map,
+                           const int *val_ptr, const double *val, double *dense16)
+{
+    for (int br = 0; br < brow; br++) {
+        int64_t o = val_ptr[br];
+        for (int b = blk_ptr[br]; b < blk_ptr[br + 1]; b++) {
+            for (int s = 0; s < 16; s++)
+                dense16[(int64_t)b * 16 + s] = ((blk_map[b] >> s) & 1) ? val[o++] : 0.0;
+        }
+    }
+}
+
+void orc_mbsr_stats(int brow, const int *blk_ptr, int64_t nnz, double *stand, double *avg_nnz)
+{
+    /* csr_matvec_device.c:1239-1249: avg_len = nblk/brow, stand = sqrt of the
+     * mean squared deviation of blocks per block row; avg_nnz = nnz/nblk. */
+    int64_t nblk = blk_ptr[brow];
+    double avg = brow ? (double)nblk / (double)brow : 0.0;
+    double s = 0.0;
+    for (int br = 0; br < brow; br++) {
+        double d = (double)(blk_ptr[br + 1] - blk_ptr[br]) - avg;
+        s += d * d;
+    }
+    *stand = brow ? sqrt(s / brow) : 0.0;
+    *avg_nnz = nblk ? (double)nnz / (double)nblk : 0.0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* mBSR -> CSR, columns ascending within a row (bsr2csr_get_ptr and
+ * bsr2csr_get_idx_val, csr_spgemm_device.c:1344-1440). */
+int64_t orc_mbsr2csr_count(int nrows, int brow, const int *blk_ptr, const uint16_t *blk_map, int *I)
+{
+    int64_t total = 0;
+    I[0] = 0;
+    for (int r = 0; r < nrows; r++) {
+        int br = r >> 2, sh = (r & 3) * 4;
+        for (int b = blk_ptr[br]; b < blk_ptr[br + 1]; b++)
+            total += __builtin_popcount((blk_map[b] >> sh) & 0xF);
+        I[r + 1] = (int)total;
+    }
+    (void)brow;
+    return total;
+}
+
+void orc_mbsr2csr_fill(int nrows, int brow, const int *blk_ptr, const int *blk_col,
+                       const uint16_t *blk_map, const int *val_ptr, const double *val,
+                       const int *I, int *J, double *A)
+{
+    for (int r = 0; r < nrows; r++) {
+        int br = r >> 2, rr = r & 3;
+        int64_t o = val_ptr[br];
+        int64_t w = I[r];
+        for (int b = blk_ptr[br]; b < blk_ptr[br + 1]; b++) {
+            uint32_t m = blk_map[b];
+            int64_t ob = o + __builtin_popcount(m & ((1u << (4 * rr)) - 1u));
+            uint32_t nib = (m >> (4 * rr)) & 0xF;
+            for (int c = 0; c < 4; c++)
+                if ((nib >> c) & 1) {
+                    J[w] = blk_col[b] * 4 + c;
+                    A[w] = val[ob++];
+                    w++;
+                }
+            o += __builtin_popcount(m);
+        }
+    }
+    (void)brow;
+}
+
+/* ------------------------------------------------------------------------ */
+/* mBSR SpMV, bsr_spmv_cc_fp64 semantics (csr_matvec_device.c:509-555) with
+ * beta applied as y = alpha*acc + beta*b (beta_vecY 673-679 fused). */
+void orc_mbsr_spmv(int nrows, int brow, const int *blk_ptr, const int *blk_col,
+                   const uint16_t *blk_map, const int *val_ptr, const double *val,
+                   double alpha, const double *x, double beta, const double *b, double *y)
+{
+    for (int br = 0; br < brow; br++) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        int64_t o = val_ptr[br];
+        for (int k = blk_ptr[br]; k < blk_ptr[br + 1]; k++) {
+            uint32_t m = blk_map[k];
+            const double *xs = x + (int64_t)blk_col[k] * 4;
+            for (int s = 0; s < 16; s++)
+                if ((m >> s) & 1) acc[s >> 2] += val[o++] * xs[s & 3];
+        }
+        for (int r = 0; r < 4; r++) {
+            int row = 4 * br + r;
+            if (row >= nrows) break;
+            y[row] = (beta == 0.0) ? alpha * acc[r] : alpha * acc[r] + beta * b[row];
+        }
+    }
+}
+
+void orc_mbsr_spmv_t(int nrows, int ncols, int brow, const int *blk_ptr, const int *blk_col,
+                     const uint16_t *blk_map, const int *val_ptr, const double *val,
+                     double alpha, const double *x, double beta, const double *b, double *y)
+{
+    double *acc = (double *)calloc((size_t)ncols + 4, sizeof(double));
+    for (int br = 0; br < brow; br++) {
+        int64_t o = val_ptr[br];
+        for (int k = blk_ptr[br]; k < blk_ptr[br + 1]; k++) {
+            uint32_t m = blk_map[k];
+            int64_t c0 = (int64_t)blk_col[k] * 4;
+            for (int s = 0; s < 16; s++)
+                if ((m >> s) & 1) acc[c0 + (s & 3)] += val[o++] * x[4 * br + (s >> 2)];
+        }
+    }
+    for (int c = 0; c < ncols; c++) y[c] = (beta == 0.0) ? alpha * acc[c] : alpha * acc[c] + beta * b[c];
+    free(acc);
+    (void)nrows;
+}
+
+/* ------------------------------------------------------------------------ */
+/* hypre_CSRMatrixMatvecOutOfPlaceHost, csr_matvec.c:22-700, single vector
+ * without rownnz: temp = beta/alpha; alpha == 0 -> y = beta*b; temp == 0 ->
+ * y = alpha*(A x); otherwise y = beta*b + alpha*(A x) with the row sum in CSR
+ * order starting from 0. */
+void orc_csr_matvec(int nrows, const int *I, const int *J, const double *A, double alpha,
+                    const double *x, double beta, const double *b, double *y)
+{
+    if (alpha == 0.0) {
+        for (int i = 0; i < nrows; i++) y[i] = beta * b[i];
+        return;
+    }
+    double temp = beta / alpha;
+    for (int i = 0; i < nrows; i++) {
+        double t = 0.0;
+        for (int p = I[i]; p < I[i + 1]; p++) t += A[p] * x[J[p]];
+        if (temp == 0.0) {
+            y[i] = (alpha == 1.0) ? t : ((alpha == -1.0) ? -t : alpha * t);
+        } else if (alpha == 1.0) {
+            y[i] = beta * b[i] + t;
+        } else if (alpha == -1.0) {
+            y[i] = -temp * b[i] - t;
+        } else {
+            y[i] = beta * b[i] + alpha * t;
+        }
+    }
+}
+
+/* hypre_CSRMatrixMatvecTHost, csr_matvec.c:914: y = alpha*A^T*x + beta*b,
+ * scattered in row order. */
+void orc_csr_matvec_t(int nrows, int ncols, const int *I, const int *J, const double *A,
+                      double alpha, const double *x, double beta, const double *b, double *y)
+{
+    double *acc = (double *)calloc((size_t)ncols + 1, sizeof(double));
+    for (int i = 0; i < nrows; i++)
+        for (int p = I[i]; p < I[i + 1]; p++) acc[J[p]] += A[p] * x[i];
+    for (int c = 0; c < ncols; c++) y[c] = (beta == 0.0) ? alpha * acc[c] : alpha * acc[c] + beta * b[c];
+    free(acc);
+}
+
+/* ------------------------------------------------------------------------ */
+/* hypre_CSRMatrixMultiplyHost, csr_matop.c:665-900 with one thread and no
+ * rownnz: first pass counts with a marker array (the diagonal is counted first
+ * when C is square), second pass fills in first-touch order and accumulates
+ * a_ik*b_kj in A then B order. */
+int64_t orc_csr_spgemm_count(int m, int n, const int *AI, const int *AJ, const int *BI,
+                             const int *BJ, int diag_first, int *CI)
+{
+    int *marker = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    for (int j = 0; j < n; j++) marker[j] = -1;
+    int64_t nnz = 0;
+    for (int i = 0; i < m; i++) {
+        CI[i] = (int)nnz;
+        if (diag_first && i < n) { marker[i] = i; nnz++; }
+        for (int p = AI[i]; p < AI[i + 1]; p++) {
+            int k = AJ[p];
+            for (int q = BI[k]; q < BI[k + 1]; q++) {
+                int j = BJ[q];
+                if (marker[j] != i) { marker[j] = i; nnz++; }
+            }
+        }
+    }
+    CI[m] = (int)nnz;
+    free(marker);
+    return nnz;
+}
+
+int64_t orc_csr_spgemm_nnz(int m, int n, const int *AI, const int *AJ, const int *BI,
+                           const int *BJ, int diag_first)
+{
+    int *marker = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    for (int j = 0; j < n; j++) marker[j] = -1;
+    int64_t nnz = 0;
+    for (int i = 0; i < m; i++) {
+        if (diag_first && i < n) { marker[i] = i; nnz++; }
+        for (int p = AI[i]; p < AI[i + 1]; p++) {
+            int k = AJ[p];
+            for (int q = BI[k]; q < BI[k + 1]; q++) {
+                int j = BJ[q];
+                if (marker[j] != i) { marker[j] = i; nnz++; }
+            }
+        }
+    }
+    free(marker);
+    return nnz;
+}
+
+void orc_csr_spgemm_fill(int m, int n, const int *AI, const int *AJ, const double *AA,
+                         const int *BI, const int *BJ, const double *BA, int diag_first,
+                         const int *CI, int *CJ, double *CA)
+{
+    int *marker = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    for (int j = 0; j < n; j++) marker[j] = -1;
+    int64_t counter = 0;
+    for (int i = 0; i < m; i++) {
+        int64_t row_start = counter;
+        if (diag_first && i < n) {
+            marker[i] = (int)counter;
+            CJ[counter] = i;
+            CA[counter] = 0.0;
+            counter++;
+        }
+        for (int p = AI[i]; p < AI[i + 1]; p++) {
+            int k = AJ[p];
+            double a = AA[p];
+            for (int q = BI[k]; q < BI[k + 1]; q++) {
+                int j = BJ[q];
+                double bv = BA[q];
+                if (marker[j] < row_start) {
+                    marker[j] = (int)counter;
+                    CJ[counter] = j;
+                    CA[counter] = a * bv;
+                    counter++;
+                } else {
+                    CA[marker[j]] += a * bv;
+                }
+            }
+        }
+        (void)CI;
+    }
+    free(marker);
+}
+
+/* ------------------------------------------------------------------------ */
+/* mBSR SpGEMM.  Symbolic (csr_spgemm_device.c:206-752): C block (i,j) exists
+ * iff some A block (i,k) and B block (k,j) have a non-zero bitmap product;
+ * block columns ascending; mapC = OR of the products.  Numeric (1142-1342):
+ * per A block in order, per B block in order with a non-zero product,
+ * C[r][c] += sum over t ascending of A[r][t]*B[t][c] (the 4-term partial sum is
+ * formed first, then added, as the scalar path's temp_valC does). */
+static int cmp_int(const void *p, const void *q)
+{
+    int a = *(const int *)p, b = *(const int *)q;
+    return (a > b) - (a < b);
+}
+
+int64_t orc_mbsr_spgemm_count(int brow_a, const int *ap, const int *ac, const uint16_t *am,
+                              int bcol_b, const int *bp, const int *bc, const uint16_t *bm,
+                              int *cp)
+{
+    int *marker = (int *)malloc(sizeof(int) * (size_t)(bcol_b > 0 ? bcol_b : 1));
+    for (int j = 0; j < bcol_b; j++) marker[j] = -1;
+    int64_t total = 0;
+    cp[0] = 0;
+    for (int i = 0; i < brow_a; i++) {
+        for (int p = ap[i]; p < ap[i + 1]; p++) {
+            int k = ac[p];
+            for (int q = bp[k]; q < bp[k + 1]; q++) {
+                if (!orc_bitmap_mul(am[p], bm[q])) continue;
+                int j = bc[q];
+                if (marker[j] != i) { marker[j] = i; total++; }
+            }
+        }
+        cp[i + 1] = (int)total;
+    }
+    free(marker);
+    return total;
+}
+
+void orc_mbsr_spgemm_fill(int brow_a, const int *ap, const int *ac, const uint16_t *am,
+                          const int *avp, const double *av,
+                          int bcol_b, const int *bp, const int *bc, const uint16_t *bm,
+                          const int *bvp, const double *bv,
+                          const int *cp, int *cc, uint16_t *cm, int *cvp, double *cv)
+{
+    int *slot = (int *)malloc(sizeof(int) * (size_t)(bcol_b > 0 ? bcol_b : 1));
+    for (int j = 0; j < bcol_b; j++) slot[j] = -1;
+    /* per-block value offsets of A and B (prefix of popcount within rows) */
+    int64_t nblk_a = ap[brow_a];
+    int64_t *aoff = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nblk_a + 1));
+    for (int i = 0; i < brow_a; i++) {
+        int64_t o = avp[i];
+        for (int p = ap[i]; p < ap[i + 1]; p++) { aoff[p] = o; o += __builtin_popcount(am[p]); }
+    }
+    /* B's block-row count is A's block-column count; derive it from the largest
+     * block column A references */
+    int brow_b = 0;
+    for (int64_t p = 0; p < nblk_a; p++) if (ac[p] + 1 > brow_b) brow_b = ac[p] + 1;
+    int64_t nblk_b = brow_b ? bp[brow_b] : 0;
+    int64_t *boff = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nblk_b + 1));
+    for (int k = 0; k < brow_b; k++) {
+        int64_t o = bvp[k];
+        for (int q = bp[k]; q < bp[k + 1]; q++) { boff[q] = o; o += __builtin_popcount(bm[q]); }
+    }
+    int64_t vo = 0;
+    double *dense = NULL;
+    int64_t dense_cap = 0;
+    for (int i = 0; i < brow_a; i++) {
+        int64_t c0 = cp[i], c1 = cp[i + 1];
+        int64_t cnt = 0;
+        /* symbolic: collect block columns, then sort */
+        for (int p = ap[i]; p < ap[i + 1]; p++) {
+            int k = ac[p];
+            for (int q = bp[k]; q < bp[k + 1]; q++) {
+                if (!orc_bitmap_mul(am[p], bm[q])) continue;
+                int j = bc[q];
+                if (slot[j] < 0) { slot[j] = 1; cc[c0 + cnt++] = j; }
+            }
+        }
+        qsort(cc + c0, (size_t)cnt, sizeof(int), cmp_int);
+        for (int64_t t = 0; t < cnt; t++) { slot[cc[c0 + t]] = (int)t; cm[c0 + t] = 0; }
+        if (cnt > dense_cap) {
+            dense_cap = cnt * 2;
+            dense = (double *)realloc(dense, sizeof(double) * 16 * (size_t)dense_cap);
+        }
+        memset(dense, 0, sizeof(double) * 16 * (size_t)cnt);
+        /* numeric */
+        for (int p = ap[i]; p < ap[i + 1]; p++) {
+            int k = ac[p];
+            uint32_t ma = am[p];
+            double a[16];
+            int64_t o = aoff[p];
+            for (int s = 0; s < 16; s++) a[s] = ((ma >> s) & 1) ? av[o++] : 0.0;
+            for (int q = bp[k]; q < bp[k + 1]; q++) {
+                uint16_t mc = orc_bitmap_mul((uint16_t)ma, bm[q]);
+                if (!mc) continue;
+                uint32_t mb = bm[q];
+                double b[16];
+                int64_t ob = boff[q];
+                for (int s = 0; s < 16; s++) b[s] = ((mb >> s) & 1) ? bv[ob++] : 0.0;
+                int t = slot[bc[q]];
+                cm[c0 + t] |= mc;
+                double *d = dense + 16 * (int64_t)t;
+                for (int r = 0; r < 4; r++)
+                    for (int c = 0; c < 4; c++) {
+                        if (!((mc >> (4 * r + c)) & 1)) continue;
+                        double tmp = 0.0;
+                        for (int kk = 0; kk < 4; kk++)
+                            if (((ma >> (4 * r + kk)) & 1) && ((mb >> (4 * kk + c)) & 1))
+                                tmp += a[4 * r + kk] * b[4 * kk + c];
+                        d[4 * r + c] += tmp;
+                    }
+            }
+        }
+        /* compress */
+        cvp[i] = (int)vo;
+        for (int64_t t = 0; t < cnt; t++) {
+            uint32_t m = cm[c0 + t];
+            for (int s = 0; s < 16; s++)
+                if ((m >> s) & 1) cv[vo++] = dense[16 * t + s];
+        }
+        for (int64_t t = 0; t < cnt; t++) slot[cc[c0 + t]] = -1;
+        (void)c1;
+    }
+    cvp[brow_a] = (int)vo;
+    free(dense);
+    free(slot);
+    free(aoff);
+    free(boff);
+}

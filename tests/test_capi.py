@@ -1,0 +1,61 @@
+"""C-ABI library: builds, loads, exports every symbol include/amgt_mbsr.h declares,
+and its host-only helpers behave (no device calls here)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "amgt_mbsr.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(amgt_[a-z0-9_]+)\s*\(", text))
+    # typedef'd callback types are not functions
+    return sorted(n for n in names if not n.endswith("_fn"))
+
+
+def test_library_is_built_and_exports_header_symbols():
+    from amgt_amd import _lib
+
+    assert os.path.exists(_lib.LIB_PATH), "run `python -m amgt_amd.build` first"
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    syms = declared_symbols()
+    assert len(syms) >= 20
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+    assert sorted(_lib.EXPORTS) == syms
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True,
+                         text=True).stdout
+    for s in syms:
+        assert re.search(rf"\bT {s}$", out, flags=re.M), s
+
+
+def test_host_only_helpers():
+    from amgt_amd import _lib
+
+    L = _lib.load()
+    # get_calculationPrecison, csr_spgemm_device.c:1847-1862
+    assert [L.amgt_precision_for_level(v) for v in (-1, 0, 1, 2, 7)] == [0, 0, 1, 2, 2]
+    assert L.amgt_status_string(0) == b"ok"
+    assert L.amgt_status_string(5) == b"invalid CSR input"
+    # argument errors are reported without touching the device
+    st = L.amgt_mbsr_create_from_csr(4, 4, 0, None, None, None, None, None)
+    assert st == _lib.AMGT_ERR_ARG
+    assert b"NULL" in L.amgt_last_error()
+    with pytest.raises(_lib.AmgtError):
+        _lib.check(st, "create")
+    assert L.amgt_set_allocator(ctypes.c_void_p(1), None, None) == _lib.AMGT_ERR_ARG
+
+
+def test_package_fails_loudly_without_library(tmp_path, monkeypatch):
+    from amgt_amd import _lib
+
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "missing.so"))
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(_lib.AmgtLibraryMissing):
+        _lib.load()

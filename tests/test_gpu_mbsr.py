@@ -1,0 +1,263 @@
+"""GPU parity of CSR->mBSR conversion and mBSR SpMV/SpMM against the CPU oracle,
+through the C-ABI (amgt_amd.mbsr / amgt_amd.hypre are thin ctypes layers).
+
+Bars: integer/structure arrays and values moved by the conversion bit-exact;
+FP64 products within 1e-12 relative to sum(|a_ij x_j|) + |beta b_i| per row;
+FP32 / FP16 value storage within 2e-6 / 2e-3 of the same scale (FP32
+accumulation; FP16 values round at 2^-11 relative).
+"""
+import numpy as np
+import pytest
+import torch
+
+from amgt_amd import _lib as L
+from amgt_amd import hypre as H
+from amgt_amd.mbsr import MBSRMatrix
+from oracle import oracle as O
+from tests.util_matrices import dense_rows_csr, rand_csr
+
+pytestmark = pytest.mark.gpu
+
+
+def T(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def make(dev, m, n, I, J, A):
+    return MBSRMatrix.from_csr(T(I, dev), T(J, dev), None if A is None else T(A, dev), m, n)
+
+
+CASES = {
+    "c1_5pt_128": lambda: ((16384, 16384), *O.stencil(7, 128, 128, 1)),
+    "7pt_16": lambda: ((4096, 4096), *O.stencil(7, 16, 16, 16)),
+    "7pt_aniso": lambda: ((6 * 7 * 9, 6 * 7 * 9), *O.stencil(7, 6, 7, 9, c=(1, 1, 0.01))),
+    "27pt_10": lambda: ((1000, 1000), *O.stencil(27, 10, 10, 10)),
+    "rand_rect": lambda: ((37, 53), *rand_csr(37, 53, 0.15, 1)),
+    "rand_tall_empty": lambda: ((101, 9), *rand_csr(101, 9, 0.3, 2, empty_rows=(0, 5, 99, 100))),
+    "rand_wide": lambda: ((13, 5000), *rand_csr(13, 5000, 0.05, 3)),
+    "bin1_rows": lambda: ((45, 700), *dense_rows_csr(45, 700, 150, 4)),
+    "bin2_rows": lambda: ((9, 9000), *dense_rows_csr(9, 9000, 1900, 5)),
+    "one_row": lambda: ((1, 3), np.array([0, 2], np.int32), np.array([2, 0], np.int32),
+                        np.array([1.5, -2.0])),
+}
+
+
+def check_structure(M, ref):
+    a = M.arrays()
+    assert torch.equal(a["blk_ptr"].cpu(), torch.from_numpy(ref["blk_ptr"]))
+    assert torch.equal(a["blk_col"].cpu(), torch.from_numpy(ref["blk_col"]))
+    assert np.array_equal(a["blk_map"].cpu().numpy().view(np.uint16), ref["blk_map"])
+    assert torch.equal(a["val_ptr"].cpu(), torch.from_numpy(ref["val_ptr"]))
+    if ref["nnz"]:
+        assert np.array_equal(a["val"].cpu().numpy(), ref["val"])
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_conversion_bit_exact(dev, name):
+    print(f"[conversion] {name}", flush=True)
+    (m, n), I, J, A = CASES[name]()
+    ref = O.csr2mbsr(m, n, I, J, A)
+    M = make(dev, m, n, I, J, A)
+    inf = M.info()
+    assert (inf["nrows"], inf["ncols"], inf["nblk"], inf["nnz"]) == (m, n, ref["nblk"], len(J))
+    stand, avg = O.mbsr_stats(ref)
+    assert inf["stand"] == pytest.approx(stand, rel=1e-12, abs=1e-12)
+    assert inf["avg_nnz"] == pytest.approx(avg, rel=1e-15)
+    assert inf["max_row_blocks"] == int(np.diff(ref["blk_ptr"]).max())
+    check_structure(M, ref)
+
+
+def test_conversion_pattern_only_and_empty(dev):
+    (m, n), I, J, A = CASES["rand_rect"]()
+    M = make(dev, m, n, I, J, None)
+    ref = O.csr2mbsr(m, n, I, J, A)
+    a = M.arrays()
+    assert torch.equal(a["blk_col"].cpu(), torch.from_numpy(ref["blk_col"]))
+    assert a["val"].numel() == 0
+    # no nonzeros at all
+    I0 = np.zeros(8, np.int32)
+    M0 = make(dev, 7, 5, I0, np.zeros(0, np.int32), np.zeros(0))
+    assert M0.info()["nblk"] == 0
+    y = M0.spmv(T(np.ones(5), dev), alpha=2.0, beta=3.0, b=T(np.arange(7.0), dev))
+    assert np.array_equal(y.cpu().numpy(), 3.0 * np.arange(7.0))
+
+
+def test_conversion_errors(dev):
+    I = np.array([0, 3], np.int32)
+    with pytest.raises(L.AmgtError) as e:
+        make(dev, 1, 8, I, np.array([1, 5, 1], np.int32), np.ones(3))
+    assert e.value.status == L.AMGT_ERR_INPUT and "duplicate" in str(e.value)
+    with pytest.raises(L.AmgtError) as e:
+        make(dev, 1, 8, I, np.array([1, 5, 8], np.int32), np.ones(3))
+    assert e.value.status == L.AMGT_ERR_INPUT and "out of range" in str(e.value)
+    with pytest.raises(L.AmgtError) as e:
+        I2, J2, A2 = dense_rows_csr(4, 9000, 2100, 6)  # 8400 entries in one block row
+        make(dev, 4, 9000, I2, J2, A2)
+    assert e.value.status == L.AMGT_ERR_UNSUPPORTED
+    with pytest.raises(L.AmgtError) as e:
+        make(dev, 1, 8, np.array([0, 5], np.int32), np.array([1, 2, 3], np.int32), np.ones(3))
+    assert e.value.status == L.AMGT_ERR_INPUT
+
+
+def row_scale(m, n, I, J, A, x, beta, b):
+    import scipy.sparse as sp
+    S = sp.csr_matrix((np.abs(A), J, I), shape=(m, n))
+    return S @ np.abs(x) + abs(beta) * (np.abs(b) if b is not None else 0.0) + 1e-300
+
+
+@pytest.mark.parametrize("name", ["c1_5pt_128", "7pt_aniso", "27pt_10", "rand_rect",
+                                  "rand_tall_empty", "rand_wide", "bin1_rows", "bin2_rows"])
+@pytest.mark.parametrize("lanes", [4, 8, 16, 32, 64])
+def test_spmv_fp64(dev, name, lanes):
+    print(f"[spmv] {name} lanes={lanes}", flush=True)
+    (m, n), I, J, A = CASES[name]()
+    ref = O.csr2mbsr(m, n, I, J, A)
+    M = make(dev, m, n, I, J, A)
+    M.set_spmv_lanes(lanes)
+    rng = np.random.default_rng(lanes)
+    x, b = rng.standard_normal(n), rng.standard_normal(m)
+    for alpha, beta in [(1.0, 0.0), (-1.0, 1.0), (0.5, -2.0)]:
+        exp = O.mbsr_spmv(ref, alpha, x, beta, b)
+        y = M.spmv(T(x, dev), alpha=alpha, beta=beta, b=T(b, dev)).cpu().numpy()
+        scale = abs(alpha) * row_scale(m, n, I, J, A, x, 0.0, None) + abs(beta) * np.abs(b) + 1e-300
+        assert np.all(np.abs(y - exp) <= 1e-12 * scale), (alpha, beta)
+    # b aliasing y (residual form, hypre_CSRMatrixMatvecDevice b == y)
+    yb = T(b, dev)
+    M.spmv(T(x, dev), yb, alpha=-1.0, beta=1.0, b=yb)
+    exp = O.mbsr_spmv(ref, -1.0, x, 1.0, b)
+    assert np.allclose(yb.cpu().numpy(), exp, rtol=1e-12, atol=1e-12)
+    # beta == 0 never reads b
+    nan_b = T(np.full(m, np.nan), dev)
+    y0 = M.spmv(T(x, dev), alpha=1.0, beta=0.0, b=nan_b).cpu().numpy()
+    assert np.all(np.isfinite(y0))
+
+
+def test_spmv_deterministic(dev):
+    (m, n), I, J, A = CASES["27pt_10"]()
+    M = make(dev, m, n, I, J, A)
+    x = T(np.random.default_rng(0).standard_normal(n), dev)
+    y1 = M.spmv(x).clone()
+    y2 = M.spmv(x).clone()
+    assert torch.equal(y1, y2)
+
+
+@pytest.mark.parametrize("name", ["rand_rect", "rand_tall_empty", "7pt_aniso", "bin1_rows"])
+def test_spmv_transposed(dev, name):
+    (m, n), I, J, A = CASES[name]()
+    ref = O.csr2mbsr(m, n, I, J, A)
+    M = make(dev, m, n, I, J, A)
+    rng = np.random.default_rng(3)
+    x, b = rng.standard_normal(m), rng.standard_normal(n)
+    exp = O.mbsr_spmv_t(ref, 1.5, x, -0.5, b)
+    y = M.spmv(T(x, dev), alpha=1.5, beta=-0.5, b=T(b, dev), trans=True).cpu().numpy()
+    At = np.abs(A)
+    import scipy.sparse as sp
+    scale = 1.5 * (sp.csr_matrix((At, J, I), shape=(m, n)).T @ np.abs(x)) + 0.5 * np.abs(b)
+    assert np.all(np.abs(y - exp) <= 1e-12 * scale + 1e-300)
+
+
+@pytest.mark.parametrize("prec,tol", [(L.AMGT_FLOAT, 2e-6), (L.AMGT_HALF, 2e-3)])
+def test_spmv_low_precision(dev, prec, tol):
+    (m, n), I, J, A = CASES["7pt_aniso"]()
+    ref = O.csr2mbsr(m, n, I, J, A)
+    M = make(dev, m, n, I, J, A)
+    M.set_precision(prec)
+    rng = np.random.default_rng(11)
+    x, b = rng.standard_normal(n), rng.standard_normal(m)
+    exp = O.mbsr_spmv(ref, -1.0, x, 1.0, b)
+    y = M.spmv(T(x, dev), alpha=-1.0, beta=1.0, b=T(b, dev)).cpu().numpy()
+    scale = row_scale(m, n, I, J, A, x, 1.0, b)
+    err = np.abs(y - exp) / scale
+    assert err.max() <= tol
+    assert err.max() > 0  # really ran in reduced precision
+
+
+@pytest.mark.parametrize("k", [1, 3, 8, 11])
+def test_spmm(dev, k):
+    (m, n), I, J, A = CASES["rand_rect"]()
+    ref = O.csr2mbsr(m, n, I, J, A)
+    M = make(dev, m, n, I, J, A)
+    rng = np.random.default_rng(k)
+    X, B = rng.standard_normal((k, n)), rng.standard_normal((k, m))
+    Y = M.spmm(T(X, dev), alpha=0.5, beta=2.0, B=T(B, dev)).cpu().numpy()
+    for v in range(k):
+        exp = O.mbsr_spmv(ref, 0.5, X[v], 2.0, B[v])
+        assert np.allclose(Y[v], exp, rtol=1e-12, atol=1e-12)
+
+
+def test_full_size_c2_spmv(dev):
+    """3D 7-point 128^3 (2,097,152 rows): bit-exact structure and FP64 SpMV vs the oracle."""
+    from amgt_amd import workloads as W
+    I, J, A = W.stencil(7, 128, 128, 128)
+    n = len(I) - 1
+    print(f"[full C2] n={n} nnz={len(J)}", flush=True)
+    ref = O.csr2mbsr(n, n, I, J, A)
+    M = make(dev, n, n, I, J, A)
+    check_structure(M, ref)
+    x = W.rand_normal(1, n)
+    exp = O.mbsr_spmv(ref, 1.0, x)
+    y = M.spmv(T(x, dev)).cpu().numpy()
+    scale = row_scale(n, n, I, J, A, x, 0.0, None)
+    assert np.all(np.abs(y - exp) <= 1e-12 * scale)
+    # closed form: A*1 = number of missing neighbours per row
+    y1 = M.spmv(T(np.ones(n), dev)).cpu().numpy()
+    exp1 = O.csr_matvec(n, I, J, A, 1.0, np.ones(n))
+    assert np.array_equal(y1, exp1)
+
+
+def test_hypre_matvec_boundary(dev):
+    (m, n), I, J, A = CASES["7pt_aniso"]()
+    C = H.CSRMatrix(T(I, dev), T(J, dev), T(A, dev), m, n)
+    rng = np.random.default_rng(2)
+    x, b = rng.standard_normal(n), rng.standard_normal(m)
+    y = torch.empty(m, dtype=torch.float64, device=dev)
+    assert H.hypre_CSRMatrixMatvecDevice(0, -1.0, C, T(x, dev), 1.0, T(b, dev), y) == 0
+    exp = O.csr_matvec(m, I, J, A, -1.0, x, 1.0, b)
+    assert np.allclose(y.cpu().numpy(), exp, rtol=1e-12, atol=1e-12)
+    assert C.bsr_mat  # cached after the first call
+    # transposed (MatvecT): y = A^T x
+    yt = torch.empty(n, dtype=torch.float64, device=dev)
+    H.hypre_CSRMatrixMatvecDevice(1, 1.0, C, T(b, dev), 0.0, yt, yt)
+    expt = O.csr_matvec_t(m, n, I, J, A, 1.0, b, 0.0, np.zeros(n))
+    assert np.allclose(yt.cpu().numpy(), expt, rtol=1e-12, atol=1e-12)
+    # alpha == 0: y = beta*b without touching A
+    H.hypre_CSRMatrixMatvecDevice(0, 0.0, C, T(x, dev), 2.0, T(b, dev), y)
+    assert np.array_equal(y.cpu().numpy(), 2.0 * b)
+    # x == y is rejected (csr_matvec_device.c:65-69)
+    xx = T(x, dev)
+    with pytest.raises(L.AmgtError):
+        H.hypre_CSRMatrixMatvecDevice(0, 1.0, C, xx, 0.0, xx, xx)
+    # precision follows the level tag only in mixed mode
+    C.coasen_level = 2
+    from amgt_amd.mbsr import set_mixed_precision
+    set_mixed_precision(True)
+    try:
+        H.hypre_CSRMatrixMatvecDevice(0, 1.0, C, T(x, dev), 0.0, y, y)
+        assert C.bsr().info()["precision"] == L.AMGT_HALF
+        e16 = np.abs(y.cpu().numpy() - O.csr_matvec(m, I, J, A, 1.0, x, 0.0, b))
+        e16 = (e16 / row_scale(m, n, I, J, A, x, 0.0, None)).max()
+        assert 0 < e16 < 2e-3
+    finally:
+        set_mixed_precision(False)
+    assert C.bsr().info()["precision"] == L.AMGT_DOUBLE
+    C.invalidate()
+
+
+def test_autotune_keeps_results(dev):
+    (m, n), I, J, A = CASES["27pt_10"]()
+    M = make(dev, m, n, I, J, A)
+    x = T(np.random.default_rng(1).standard_normal(n), dev)
+    y0 = M.spmv(x).cpu().numpy()
+    M.autotune()
+    assert M.info()["spmv_lanes"] in (4, 8, 16, 32, 64)
+    y1 = M.spmv(x).cpu().numpy()
+    assert np.allclose(y0, y1, rtol=1e-13, atol=1e-13)
+
+
+def test_update_values(dev):
+    (m, n), I, J, A = CASES["rand_rect"]()
+    M = make(dev, m, n, I, J, A)
+    A2 = A * 3.0 - 1.0
+    M.update_values(T(I, dev), T(J, dev), T(A2, dev))
+    ref = O.csr2mbsr(m, n, I, J, A2)
+    assert np.array_equal(M.arrays()["val"].cpu().numpy(), ref["val"])

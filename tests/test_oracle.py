@@ -1,0 +1,321 @@
+"""Pins the CPU oracle before anything is checked against it (CPU only).
+
+Each oracle function is checked against (a) the reference's own fixtures
+(tests/golden/hypre_fixtures.json: HYPRE's benchmark_spgemm nnz counts), (b)
+an independent library (scipy.sparse, in this container only), (c) closed
+forms / invariants / brute force, chosen so that a dropped term, a wrong index
+or a transposed operand fails one of them.
+
+Pin map (oracle function -> tests):
+  orc_pm_next / orc_rand_uniform / orc_rand_normal -> test_park_miller_*, test_rand_*
+  orc_stencil -> test_stencil_nnz_fixtures, test_stencil_row_sums, test_stencil_layout
+  orc_bitmap_mul -> test_bitmap_mul_brute_force
+  orc_csr2mbsr_* / orc_mbsr_dense_values -> test_csr2mbsr_vs_scipy_bsr, test_csr2mbsr_errors
+  orc_mbsr_stats -> test_mbsr_stats
+  orc_mbsr2csr_* -> test_mbsr_roundtrip
+  orc_mbsr_spmv / _t -> test_mbsr_spmv_vs_scipy, test_spmv_closed_form
+  orc_csr_matvec / _t -> test_csr_matvec_vs_scipy
+  orc_csr_spgemm_* -> test_spgemm_nnz_fixtures, test_csr_spgemm_vs_scipy
+  orc_mbsr_spgemm_* -> test_mbsr_spgemm_vs_scipy, test_mbsr_spgemm_brute_force
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import oracle as O
+from tests.util_matrices import rand_csr, to_scipy
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "hypre_fixtures.json")))
+
+
+# ---------------------------------------------------------------- RNG
+def test_park_miller_check_value():
+    g = GOLD["park_miller"]
+    seq = O.pm_sequence(g["seed"], g["index"])
+    assert int(seq[-1]) == g["value"]
+
+
+def test_park_miller_recurrence_and_seed_clamp():
+    seq = O.pm_sequence(12345, 200)
+    # x_{k+1} = 16807 x_k mod (2^31 - 1), exact integer arithmetic
+    prev = 12345
+    for v in seq:
+        assert int(v) == (16807 * prev) % 2147483647
+        prev = int(v)
+    # random.c:56-59: seeds below 1 become 1
+    assert np.array_equal(O.pm_sequence(0, 50), O.pm_sequence(1, 50))
+    assert np.array_equal(O.pm_sequence(-7, 50), O.pm_sequence(1, 50))
+
+
+def test_rand_uniform_and_normal():
+    u = O.rand_uniform(3, 5)
+    seq = O.pm_sequence(3, 5)
+    assert np.array_equal(u, 2.0 * (seq / 2147483647.0) - 1.0)
+    z = O.rand_normal(1, 200000)
+    assert abs(z.mean()) < 0.01 and abs(z.std() - 1.0) < 0.01
+    s = O.pm_sequence(1, 2) / 2147483647.0
+    assert z[0] == pytest.approx(np.sqrt(-2 * np.log(s[0])) * np.cos(2 * np.pi * s[1]), rel=1e-15)
+
+
+# ---------------------------------------------------------------- stencils
+@pytest.mark.parametrize("key,kind,dims", [
+    ("7pt_128_128_128", 7, (128, 128, 128)),
+    ("27pt_128_128_128", 27, (128, 128, 128)),
+    ("5pt_1024_1024", 7, (1024, 1024, 1)),
+    ("9pt_1024_1024", 9, (1024, 1024, 1)),
+])
+def test_stencil_nnz_fixtures(key, kind, dims):
+    # Diag(A)*A has exactly A's pattern: HYPRE's fixture is A's nnz
+    assert O.stencil_nnz(kind, *dims) == GOLD["diag_times_A_nnz"][key]["nnz"]
+
+
+@pytest.mark.parametrize("kind,dims,c", [(7, (6, 5, 4), (1.0, 1.0, 1.0)), (7, (6, 5, 1), (1.0, 1.0, 1.0)),
+                                         (7, (5, 4, 6), (1.0, 1.0, 0.01)), (9, (5, 6, 1), (1, 1, 1)),
+                                         (27, (4, 5, 3), (1, 1, 1))])
+def test_stencil_row_sums(kind, dims, c):
+    I, J, A = O.stencil(kind, *dims, c=c)
+    n = len(I) - 1
+    S = to_scipy(n, n, I, J, A)
+    rs = S @ np.ones(n)
+    nx, ny, nz = dims
+    idx = np.arange(n)
+    x, y, z = idx % nx, (idx // nx) % ny, idx // (nx * ny)
+    if kind == 7:
+        # row sum = sum over missing neighbours of the coupling of that direction
+        exp = np.zeros(n)
+        for (cc, q, N) in ((c[0], x, nx), (c[1], y, ny), (c[2], z, nz)):
+            if N > 1:
+                exp += cc * ((q == 0).astype(float) + (q == N - 1).astype(float))
+    else:
+        exp = np.zeros(n)
+        full = 8 if kind == 9 else 26
+        for i in range(n):
+            exp[i] = full - (I[i + 1] - I[i] - 1)
+    assert np.allclose(rs, exp, atol=1e-13)
+    assert abs(S - S.T).max() == 0  # symmetric
+
+
+def test_stencil_layout_diag_first_then_ascending():
+    I, J, A = O.stencil(27, 4, 4, 4)
+    for r in range(len(I) - 1):
+        row = J[I[r]:I[r + 1]]
+        assert row[0] == r
+        assert np.all(np.diff(row[1:]) > 0)
+        assert A[I[r]] == 26.0 and np.all(A[I[r] + 1:I[r + 1]] == -1.0)
+
+
+# ---------------------------------------------------------------- bitmaps
+def test_bitmap_mul_brute_force():
+    rng = np.random.default_rng(0)
+    pairs = [(0, 0), (0xFFFF, 0xFFFF), (0x8421, 0x8421), (1, 1), (0x1000, 0x0008)]
+    pairs += [tuple(int(v) for v in rng.integers(0, 1 << 16, 2)) for _ in range(3000)]
+    for a, b in pairs:
+        Ab = np.array([[(a >> (4 * r + c)) & 1 for c in range(4)] for r in range(4)])
+        Bb = np.array([[(b >> (4 * r + c)) & 1 for c in range(4)] for r in range(4)])
+        Cb = (Ab @ Bb) > 0
+        exp = sum(1 << (4 * r + c) for r in range(4) for c in range(4) if Cb[r, c])
+        assert O.bitmap_mul(a, b) == exp
+    # non-commutative: a transposed operand fails
+    assert O.bitmap_mul(0x0002, 0x0001) != O.bitmap_mul(0x0001, 0x0002)
+
+
+# ---------------------------------------------------------------- CSR -> mBSR
+CASES = [
+    ("stencil7", lambda: O.stencil(7, 10, 7, 5), None),
+    ("stencil27", lambda: O.stencil(27, 6, 5, 3), None),
+    ("rand_rect", lambda: rand_csr(37, 53, 0.15, 1), (37, 53)),
+    ("rand_tall", lambda: rand_csr(101, 9, 0.3, 2, empty_rows=(0, 5, 99, 100)), (101, 9)),
+    ("rand_dense", lambda: rand_csr(21, 22, 0.9, 3), (21, 22)),
+]
+
+
+def _case(name):
+    for nm, f, shape in CASES:
+        if nm == name:
+            I, J, A = f()
+            if shape is None:
+                n = len(I) - 1
+                shape = (n, n)
+            return shape, I, J, A
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("name", [c[0] for c in CASES])
+def test_csr2mbsr_vs_scipy_bsr(name):
+    (m, n), I, J, A = _case(name)
+    M = O.csr2mbsr(m, n, I, J, A)
+    S = to_scipy(m, n, I, J, A)
+    mp, npad = -(-m // 4) * 4, -(-n // 4) * 4
+    Sp = sp.csr_matrix((S.data, S.indices, np.concatenate([S.indptr, np.full(mp - m, S.nnz)])),
+                       shape=(mp, npad))
+    B = sp.bsr_matrix(Sp, blocksize=(4, 4))
+    B.sort_indices()
+    # scipy keeps all-zero blocks only if present structurally; values are nonzero here
+    assert np.array_equal(M["blk_ptr"], B.indptr)
+    assert np.array_equal(M["blk_col"], B.indices)
+    dense = O.mbsr_dense_values(M).reshape(-1, 4, 4)
+    assert np.array_equal(dense, B.data)
+    # bitmap = pattern of the dense block
+    for b in range(M["nblk"]):
+        exp = sum(1 << (4 * r + c) for r in range(4) for c in range(4) if B.data[b, r, c] != 0)
+        assert int(M["blk_map"][b]) == exp
+    # invariants: sum popc = nnz; val_ptr = I[4br]
+    assert sum(bin(int(v)).count("1") for v in M["blk_map"]) == len(J)
+    assert np.array_equal(M["val_ptr"][:-1], I[np.minimum(np.arange(M["brow"]) * 4, m)])
+
+
+def test_csr2mbsr_errors():
+    I = np.array([0, 2], dtype=np.int32)
+    with pytest.raises(ValueError, match="duplicate"):
+        O.csr2mbsr(1, 4, I, np.array([1, 1], dtype=np.int32), np.ones(2))
+    with pytest.raises(ValueError, match="out of range"):
+        O.csr2mbsr(1, 4, I, np.array([1, 4], dtype=np.int32), np.ones(2))
+
+
+def test_mbsr_stats():
+    (m, n), I, J, A = _case("rand_rect")
+    M = O.csr2mbsr(m, n, I, J, A)
+    stand, avg = O.mbsr_stats(M)
+    lens = np.diff(M["blk_ptr"])
+    assert stand == pytest.approx(np.std(lens), rel=1e-14)
+    assert avg == pytest.approx(len(J) / M["nblk"], rel=1e-15)
+
+
+@pytest.mark.parametrize("name", [c[0] for c in CASES])
+def test_mbsr_roundtrip(name):
+    (m, n), I, J, A = _case(name)
+    M = O.csr2mbsr(m, n, I, J, A)
+    I2, J2, A2 = O.mbsr2csr(M)
+    S = to_scipy(m, n, I, J, A)
+    S.sort_indices()
+    assert np.array_equal(I2, S.indptr) and np.array_equal(J2, S.indices)
+    assert np.array_equal(A2, S.data)
+
+
+# ---------------------------------------------------------------- SpMV
+@pytest.mark.parametrize("name", [c[0] for c in CASES])
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (-1.0, 1.0), (0.7, -2.5)])
+def test_mbsr_spmv_vs_scipy(name, alpha, beta):
+    (m, n), I, J, A = _case(name)
+    M = O.csr2mbsr(m, n, I, J, A)
+    S = to_scipy(m, n, I, J, A)
+    rng = np.random.default_rng(5)
+    x, b = rng.standard_normal(n), rng.standard_normal(m)
+    ref = alpha * (S @ x) + beta * b
+    y = O.mbsr_spmv(M, alpha, x, beta, b)
+    scale = np.abs(alpha) * (np.abs(S) @ np.abs(x)) + abs(beta) * np.abs(b) + 1e-300
+    assert np.all(np.abs(y - ref) <= 1e-14 * scale)
+    # transposed product
+    xt, bt = rng.standard_normal(m), rng.standard_normal(n)
+    yt = O.mbsr_spmv_t(M, alpha, xt, beta, bt)
+    reft = alpha * (S.T @ xt) + beta * bt
+    assert np.allclose(yt, reft, rtol=1e-13, atol=1e-13)
+    # beta == 0 never reads b
+    y0 = O.mbsr_spmv(M, alpha, x, 0.0, np.full(m, np.nan))
+    assert np.all(np.isfinite(y0))
+
+
+def test_spmv_closed_form():
+    I, J, A = O.stencil(7, 128, 128, 1)
+    n = len(I) - 1
+    M = O.csr2mbsr(n, n, I, J, A)
+    y = O.mbsr_spmv(M, 1.0, np.ones(n))
+    idx = np.arange(n)
+    x, yy = idx % 128, idx // 128
+    exp = (x == 0).astype(int) + (x == 127) + (yy == 0) + (yy == 127)
+    assert np.array_equal(y, exp.astype(float))  # exact: small integers
+
+
+def test_csr_matvec_vs_scipy():
+    (m, n), I, J, A = _case("rand_rect")
+    S = to_scipy(m, n, I, J, A)
+    rng = np.random.default_rng(9)
+    x, b = rng.standard_normal(n), rng.standard_normal(m)
+    for alpha, beta in [(1.0, 0.0), (-1.0, 1.0), (1.0, 1.0), (2.0, 0.5), (-1.0, 0.3), (0.0, 2.0)]:
+        y = O.csr_matvec(m, I, J, A, alpha, x, beta, b)
+        assert np.allclose(y, alpha * (S @ x) + beta * b, rtol=1e-13, atol=1e-13)
+    xt, bt = rng.standard_normal(m), rng.standard_normal(n)
+    yt = O.csr_matvec_t(m, n, I, J, A, 1.5, xt, -0.5, bt)
+    assert np.allclose(yt, 1.5 * (S.T @ xt) - 0.5 * bt, rtol=1e-13, atol=1e-13)
+
+
+# ---------------------------------------------------------------- SpGEMM
+@pytest.mark.parametrize("key,kind,dims", [
+    ("7pt_128_128_128", 7, (128, 128, 128)),
+    ("5pt_1024_1024", 7, (1024, 1024, 1)),
+    ("9pt_1024_1024", 9, (1024, 1024, 1)),
+])
+def test_spgemm_nnz_fixtures(key, kind, dims):
+    I, J, A = O.stencil(kind, *dims)
+    n = len(I) - 1
+    assert O.csr_spgemm_nnz(n, n, I, J, I, J) == GOLD["A_squared_nnz"][key]["nnz"]
+
+
+@pytest.mark.slow
+def test_spgemm_nnz_fixture_27pt():
+    I, J, A = O.stencil(27, 128, 128, 128)
+    n = len(I) - 1
+    assert O.csr_spgemm_nnz(n, n, I, J, I, J) == GOLD["A_squared_nnz"]["27pt_128_128_128"]["nnz"]
+
+
+def test_csr_spgemm_vs_scipy():
+    (m, k), AI, AJ, AA = (37, 53), *rand_csr(37, 53, 0.15, 11)
+    BI, BJ, BA = rand_csr(53, 29, 0.2, 12)
+    CI, CJ, CA = O.csr_spgemm(m, 29, AI, AJ, AA, BI, BJ, BA)
+    C = to_scipy(m, 29, CI, CJ, CA)
+    ref = to_scipy(m, k, AI, AJ, AA) @ to_scipy(k, 29, BI, BJ, BA)
+    assert abs(C - ref).max() < 1e-13
+    # square product: diagonal first in every row (csr_matop.c:760-770)
+    I, J, A = O.stencil(7, 6, 5, 4)
+    n = len(I) - 1
+    CI, CJ, CA = O.csr_spgemm(n, n, I, J, A, I, J, A)
+    assert all(CJ[CI[r]] == r for r in range(n))
+    S = to_scipy(n, n, I, J, A)
+    assert abs(to_scipy(n, n, CI, CJ, CA) - S @ S).max() < 1e-12
+
+
+def _mbsr_to_scipy(M):
+    I, J, A = O.mbsr2csr(M)
+    return to_scipy(M["nrows"], M["ncols"], I, J, A)
+
+
+@pytest.mark.parametrize("shapes,seed", [((37, 53, 29), 21), ((16, 16, 16), 22), ((9, 70, 13), 23)])
+def test_mbsr_spgemm_vs_scipy(shapes, seed):
+    m, k, n = shapes
+    AI, AJ, AA = rand_csr(m, k, 0.2, seed)
+    BI, BJ, BA = rand_csr(k, n, 0.25, seed + 100)
+    MA = O.csr2mbsr(m, k, AI, AJ, AA)
+    MB = O.csr2mbsr(k, n, BI, BJ, BA)
+    MC = O.mbsr_spgemm(MA, MB)
+    C = _mbsr_to_scipy(MC)
+    SA, SB = to_scipy(m, k, AI, AJ, AA), to_scipy(k, n, BI, BJ, BA)
+    ref = SA @ SB
+    assert abs(C - ref).max() < 1e-13
+    # structural pattern = boolean product (cancellation keeps the entry)
+    pat = ((SA != 0).astype(int) @ (SB != 0).astype(int)).tocsr()
+    pat.sort_indices()
+    C.sort_indices()
+    assert np.array_equal(C.indptr, pat.indptr) and np.array_equal(C.indices, pat.indices)
+    # same pattern as HYPRE's host product up to its forced diagonal
+    CI, CJ, CA = O.csr_spgemm(m, n, AI, AJ, AA, BI, BJ, BA, diag_first=0)
+    H = to_scipy(m, n, CI, CJ, CA)
+    H.sort_indices()
+    assert np.array_equal(H.indptr, C.indptr) and np.array_equal(H.indices, C.indices)
+
+
+def test_mbsr_spgemm_brute_force():
+    rng = np.random.default_rng(7)
+    Ad = np.where(rng.random((8, 12)) < 0.3, rng.integers(-3, 4, (8, 12)), 0).astype(float)
+    Bd = np.where(rng.random((12, 7)) < 0.3, rng.integers(-3, 4, (12, 7)), 0).astype(float)
+    SA, SB = sp.csr_matrix(Ad), sp.csr_matrix(Bd)
+    MA = O.csr2mbsr(8, 12, SA.indptr.astype(np.int32), SA.indices.astype(np.int32), SA.data)
+    MB = O.csr2mbsr(12, 7, SB.indptr.astype(np.int32), SB.indices.astype(np.int32), SB.data)
+    C = _mbsr_to_scipy(O.mbsr_spgemm(MA, MB)).toarray()
+    exact = np.zeros((8, 7))
+    for i in range(8):
+        for j in range(7):
+            exact[i, j] = sum(Ad[i, t] * Bd[t, j] for t in range(12))
+    assert np.array_equal(C, exact)  # small integers: exact
