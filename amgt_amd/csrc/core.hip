@@ -234,6 +234,13 @@ int amgt_mbsr_get_info(amgt_mbsr_t A, amgt_mbsr_info_t *info) {
     info->level = m.level;
     info->precision = effective_precision(m);
     info->spmv_lanes = m.spmv_lanes ? m.spmv_lanes : spmv_default_lanes(m);
+    info->spmv_staged = m.spmv_staged;
+    return AMGT_OK;
+}
+
+int amgt_mbsr_set_spmv_staged(amgt_mbsr_t A, int staged) {
+    AMGT_CHECK_ARG(A != nullptr, "handle is NULL");
+    A->m.spmv_staged = staged ? 1 : 0;
     return AMGT_OK;
 }
 
@@ -285,20 +292,24 @@ int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream) {
     AMGT_HIP(hipEventCreate(&e1));
     const int cand[5] = {4, 8, 16, 32, 64};
     float best = 1e30f;
-    int best_lanes = spmv_default_lanes(m);
-    for (int c = 0; c < 5; ++c) {
-        int st = spmv(m, prec, cand[c], 1.0, x, 0.0, nullptr, y, s);  // warm-up
-        if (st != AMGT_OK) break;
-        AMGT_HIP(hipEventRecord(e0, s));
-        const int reps = 10;
-        for (int r = 0; r < reps; ++r) spmv(m, prec, cand[c], 1.0, x, 0.0, nullptr, y, s);
-        AMGT_HIP(hipEventRecord(e1, s));
-        AMGT_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        AMGT_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) {
-            best = ms;
-            best_lanes = cand[c];
+    int best_lanes = spmv_default_lanes(m), best_staged = 1;
+    for (int staged = 1; staged >= 0; --staged) {
+        m.spmv_staged = staged;
+        for (int c = 0; c < 5; ++c) {
+            int st = spmv(m, prec, cand[c], 1.0, x, 0.0, nullptr, y, s);  // warm-up
+            if (st != AMGT_OK) break;
+            AMGT_HIP(hipEventRecord(e0, s));
+            const int reps = 10;
+            for (int r = 0; r < reps; ++r) spmv(m, prec, cand[c], 1.0, x, 0.0, nullptr, y, s);
+            AMGT_HIP(hipEventRecord(e1, s));
+            AMGT_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            AMGT_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                best_lanes = cand[c];
+                best_staged = staged;
+            }
         }
     }
     (void)hipEventDestroy(e0);
@@ -306,6 +317,7 @@ int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream) {
     tmp_free(x, s);
     tmp_free(y, s);
     m.spmv_lanes = best_lanes;
+    m.spmv_staged = best_staged;
     return AMGT_OK;
 }
 

@@ -58,17 +58,35 @@ __device__ __forceinline__ int bin_of(int n) {
 }
 
 // Bins block rows by entry count; list[bin*brow + t] = block row.
-__global__ void k_bin_rows(int32_t nrows, int32_t brow, const int32_t *__restrict__ I,
-                           int32_t *__restrict__ list, ConvCounters *cnt) {
-    int br = blockIdx.x * blockDim.x + threadIdx.x;
-    if (br >= brow) return;
-    int r0 = 4 * br;
-    int r1 = min(r0 + 4, nrows);
-    int n = I[r1] - I[r0];
-    int b = bin_of(n);
-    int pos = atomicAdd(&cnt->bin_count[b], 1);
-    list[int64_t(b) * brow + pos] = br;
-    atomicMax(&cnt->max_entries, n);
+// One atomic per wave and bin (wave-aggregated): a single hot counter would
+// otherwise serialise every block row.
+__global__ __launch_bounds__(256) void k_bin_rows(int32_t nrows, int32_t brow,
+                                                  const int32_t *__restrict__ I,
+                                                  int32_t *__restrict__ list, ConvCounters *cnt) {
+    const int br = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool valid = br < brow;
+    int n = 0, b = -1;
+    if (valid) {
+        const int r0 = 4 * br;
+        const int r1 = min(r0 + 4, nrows);
+        n = I[r1] - I[r0];
+        b = bin_of(n);
+    }
+    const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint64_t mask = __ballot(b == q);
+        if (mask == 0ull) continue;
+        const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&cnt->bin_count[q], __popcll(mask));
+        base = __shfl(base, leader);
+        if (b == q) list[int64_t(q) * brow + base + __popcll(mask & lt)] = br;
+    }
+    int mx = n;
+    for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off));
+    if (lane == 0 && mx > 0) atomicMax(&cnt->max_entries, mx);
 }
 
 // Key of CSR entry p in block row starting at row r0.

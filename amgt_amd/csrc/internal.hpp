@@ -74,6 +74,7 @@ struct Mbsr {
     int32_t level = 0;
     int32_t precision = -1;   // -1: follow the level policy
     int32_t spmv_lanes = 0;   // 0: heuristic
+    int32_t spmv_staged = 1;  // 1: LDS-staged tile kernel, 0: direct kernel
     Mbsr *trans = nullptr;    // cached A^T
 };
 

@@ -1,15 +1,647 @@
-// mBSR SpGEMM (placeholder until the symbolic/numeric kernels land).
+// mBSR SpGEMM C = A*B (counterpart of spgemm_amgT_fp64, csr_spgemm_device.c:1527-1846).
+//
+// 1. upper bound  ub_i = sum over A blocks (i,k) of |B row k|  (compute_Cub_bin,
+//    csr_spgemm_device.c:82-134), scanned into per-row slots of a temporary
+//    buffer; rows are binned by ub.
+// 2. symbolic: every product (A block, B block) whose 4x4 bitmap product
+//    (bitMatrixMul, 189-205) is non-zero inserts its block column into a hash
+//    table (LDS for ub <= 2048, a global slab otherwise) and ORs the product
+//    bitmap into the entry; the distinct block columns are then sorted with
+//    their bitmaps and written to the row's slot.  One pass replaces the
+//    reference's count + fill passes (206-752): the slot layout is sized by ub.
+//    Products are flattened over the lanes of a group (A blocks x B blocks),
+//    not "one warp per A block".
+// 3. scans of block counts and value counts give C.blk_ptr / C.val_ptr; one
+//    device->host copy sizes C.
+// 4. numeric (1142-1342): one wave per C block row keeps the row's 4x4
+//    accumulators in LDS (global scratch for rows with more than 64 blocks);
+//    for each A block in order, four B blocks at a time are multiplied with one
+//    v_mfma_f64_4x4x4f64 (four independent 4x4x4 products: the A block
+//    replicated, B block g in block slot g) and added to the C slots of their
+//    block columns.  A B row never repeats a block column, so the four slots
+//    are distinct: no atomics, deterministic results.
+// 5. accumulators are compressed to the mBSR value layout.
 #include "internal.hpp"
+
+#include <algorithm>
+#include <string>
 
 namespace amgt {
 
-int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *C) {
-    (void)A;
-    (void)B;
-    (void)s;
-    (void)C;
-    set_error("SpGEMM not built yet");
-    return AMGT_ERR_UNSUPPORTED;
+namespace {
+
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr int kBin0Ub = 128;    // wave per row, LDS table of 256
+constexpr int kBin1Ub = 2048;   // workgroup per row, LDS table of 4096
+constexpr int kNumCap = 64;     // C blocks per row kept in LDS by the numeric kernel
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int NT>
+__device__ __forceinline__ void grp_sync() {
+    if constexpr (NT == 64) {
+        wave_sync();
+    } else {
+        __syncthreads();
+    }
+}
+
+// inclusive scan over the NT threads of a group; `tot` gets the group total.
+// sc: NT/64 ints of LDS scratch for NT > 64.
+template <int NT>
+__device__ __forceinline__ int grp_scan(int v, int *sc, int *tot) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int t = __shfl_up(v, d);
+        if (lane >= d) v += t;
+    }
+    if constexpr (NT == 64) {
+        *tot = __shfl(v, 63);
+        return v;
+    } else {
+        const int w = (threadIdx.x % NT) >> 6;
+        __syncthreads();
+        if (lane == 63) sc[w] = v;
+        __syncthreads();
+        int before = 0, all = 0;
+#pragma unroll
+        for (int q = 0; q < NT / 64; ++q) {
+            int s = sc[q];
+            if (q < w) before += s;
+            all += s;
+        }
+        *tot = all;
+        return v + before;
+    }
+}
+
+__device__ __forceinline__ uint32_t hash_slot(uint32_t key, int hs) {
+    return (key * 2654435761u) & uint32_t(hs - 1);
+}
+
+// Symbolic work for one C block row by a group of NT threads (tid in [0, NT)).
+// hkey/hmap: hash table of hs slots (LDS or global), initialised here.
+// s_excl/s_q0/s_ma: NT-entry LDS staging for the product enumeration.
+template <int NT>
+__device__ void symbolic_row(int i, int tid, uint32_t *hkey, uint32_t *hmap, int hs,
+                             int *s_excl, int *s_q0, int *s_ma, int *s_sc,
+                             const int32_t *__restrict__ ap, const int32_t *__restrict__ ac,
+                             const uint16_t *__restrict__ am, const int32_t *__restrict__ bp,
+                             const int32_t *__restrict__ bc, const uint16_t *__restrict__ bm,
+                             int64_t slot, int32_t *__restrict__ tcol, uint32_t *__restrict__ tmap,
+                             int32_t *__restrict__ cnt_out, int32_t *__restrict__ nnz_out) {
+    for (int s = tid; s < hs; s += NT) {
+        hkey[s] = kEmpty;
+        hmap[s] = 0u;
+    }
+    grp_sync<NT>();
+    const int a_beg = ap[i], a_end = ap[i + 1];
+    for (int a0 = a_beg; a0 < a_end; a0 += NT) {
+        const int a = a0 + tid;
+        int len = 0, q0 = 0, ma = 0;
+        if (a < a_end) {
+            const int k = ac[a];
+            ma = am[a];
+            q0 = bp[k];
+            len = bp[k + 1] - q0;
+        }
+        int total;
+        const int incl = grp_scan<NT>(len, s_sc, &total);
+        s_excl[tid] = incl - len;
+        s_q0[tid] = q0;
+        s_ma[tid] = ma;
+        grp_sync<NT>();
+        const int na = min(NT, a_end - a0);
+        for (int t = tid; t < total; t += NT) {
+            // last staged A block with excl <= t
+            int lo = 0, hi = na - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_excl[mid] <= t) lo = mid; else hi = mid - 1;
+            }
+            const int q = s_q0[lo] + (t - s_excl[lo]);
+            const uint32_t mc = bitmap_mul(uint32_t(s_ma[lo]), bm[q]);
+            if (mc) {
+                const uint32_t key = uint32_t(bc[q]);
+                uint32_t h = hash_slot(key, hs);
+                while (true) {
+                    const uint32_t old = atomicCAS(&hkey[h], kEmpty, key);
+                    if (old == kEmpty || old == key) {
+                        atomicOr(&hmap[h], mc);
+                        break;
+                    }
+                    h = (h + 1) & uint32_t(hs - 1);
+                }
+            }
+        }
+        grp_sync<NT>();
+    }
+    // compact occupied slots to the front (in place: destinations never pass sources)
+    int base = 0;
+    for (int s0 = 0; s0 < hs; s0 += NT) {
+        const int s = s0 + tid;
+        uint32_t k = kEmpty, m = 0;
+        if (s < hs) {
+            k = hkey[s];
+            m = hmap[s];
+        }
+        const int occ = (k != kEmpty);
+        int tot;
+        const int incl = grp_scan<NT>(occ, s_sc, &tot);
+        grp_sync<NT>();
+        if (occ) {
+            hkey[base + incl - 1] = k;
+            hmap[base + incl - 1] = m;
+        }
+        base += tot;
+        grp_sync<NT>();
+    }
+    const int cnt = base;
+    int np = 1;
+    while (np < cnt) np <<= 1;
+    for (int s = cnt + tid; s < np; s += NT) hkey[s] = kEmpty;
+    grp_sync<NT>();
+    // bitonic sort of (key, map) by key
+    for (int k = 2; k <= np; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int x = tid; x < np; x += NT) {
+                const int p = x ^ j;
+                if (p > x) {
+                    const bool up = (x & k) == 0;
+                    const uint32_t kx = hkey[x], kp = hkey[p];
+                    if ((kx > kp) == up) {
+                        hkey[x] = kp;
+                        hkey[p] = kx;
+                        const uint32_t mx = hmap[x];
+                        hmap[x] = hmap[p];
+                        hmap[p] = mx;
+                    }
+                }
+            }
+            grp_sync<NT>();
+        }
+    }
+    int nz = 0;
+    for (int t = tid; t < cnt; t += NT) {
+        const uint32_t m = hmap[t];
+        tcol[slot + t] = int32_t(hkey[t]);
+        tmap[slot + t] = m;
+        nz += __popc(m);
+    }
+    int nz_tot;
+    grp_scan<NT>(nz, s_sc, &nz_tot);
+    if (tid == 0) {
+        cnt_out[i] = cnt;
+        nnz_out[i] = nz_tot;
+    }
+    grp_sync<NT>();
+}
+
+struct SpgemmArgs {
+    const int32_t *ap, *ac, *avp;
+    const uint16_t *am;
+    const double *av;
+    const int32_t *bp, *bc, *bvp;
+    const uint16_t *bm;
+    const double *bv;
+};
+
+// ---- bin 0: one wave per row, 4 rows per workgroup, LDS table of 256
+__global__ __launch_bounds__(256) void k_symbolic_wave(int count, const int32_t *__restrict__ rows,
+                                                       SpgemmArgs g,
+                                                       const int64_t *__restrict__ ub_off,
+                                                       int32_t *tcol, uint32_t *tmap,
+                                                       int32_t *cnt_out, int32_t *nnz_out) {
+    constexpr int HS = 2 * kBin0Ub;
+    __shared__ uint32_t s_key[4][HS];
+    __shared__ uint32_t s_map[4][HS];
+    __shared__ int s_excl[4][64], s_q0[4][64], s_ma[4][64];
+    const int w = threadIdx.x >> 6;
+    const int r = blockIdx.x * 4 + w;
+    if (r >= count) return;  // the whole wave leaves; waves never sync across
+    const int i = rows[r];
+    symbolic_row<64>(i, threadIdx.x & 63, s_key[w], s_map[w], HS, s_excl[w], s_q0[w], s_ma[w],
+                     nullptr, g.ap, g.ac, g.am, g.bp, g.bc, g.bm, ub_off[i], tcol, tmap, cnt_out,
+                     nnz_out);
+}
+
+// ---- bin 1: one workgroup per row, LDS table of 4096
+__global__ __launch_bounds__(256) void k_symbolic_block(int count, const int32_t *__restrict__ rows,
+                                                        SpgemmArgs g,
+                                                        const int64_t *__restrict__ ub_off,
+                                                        int32_t *tcol, uint32_t *tmap,
+                                                        int32_t *cnt_out, int32_t *nnz_out) {
+    constexpr int HS = 2 * kBin1Ub;
+    __shared__ uint32_t s_key[HS];
+    __shared__ uint32_t s_map[HS];
+    __shared__ int s_excl[256], s_q0[256], s_ma[256], s_sc[4];
+    if (int(blockIdx.x) >= count) return;
+    const int i = rows[blockIdx.x];
+    symbolic_row<256>(i, threadIdx.x, s_key, s_map, HS, s_excl, s_q0, s_ma, s_sc, g.ap, g.ac, g.am,
+                      g.bp, g.bc, g.bm, ub_off[i], tcol, tmap, cnt_out, nnz_out);
+}
+
+// ---- bin 2: workgroups loop over rows, each with a global slab of hs slots
+__global__ __launch_bounds__(256) void k_symbolic_global(int count, const int32_t *__restrict__ rows,
+                                                         SpgemmArgs g,
+                                                         const int64_t *__restrict__ ub_off,
+                                                         int32_t *tcol, uint32_t *tmap,
+                                                         int32_t *cnt_out, int32_t *nnz_out,
+                                                         uint32_t *slab, int hs) {
+    __shared__ int s_excl[256], s_q0[256], s_ma[256], s_sc[4];
+    uint32_t *hkey = slab + int64_t(blockIdx.x) * 2 * hs;
+    uint32_t *hmap = hkey + hs;
+    for (int r = blockIdx.x; r < count; r += gridDim.x) {
+        const int i = rows[r];
+        symbolic_row<256>(i, threadIdx.x, hkey, hmap, hs, s_excl, s_q0, s_ma, s_sc, g.ap, g.ac,
+                          g.am, g.bp, g.bc, g.bm, ub_off[i], tcol, tmap, cnt_out, nnz_out);
+    }
+}
+
+struct Counters {
+    int32_t bin[3];
+    int32_t max_ub;
+    int32_t big_rows;   // numeric rows with more than kNumCap blocks
+    int32_t max_blocks;
+    int64_t total_ub;
+    int64_t big_blocks;
+};
+
+__global__ void k_zero_counters(Counters *c) {
+    if (threadIdx.x == 0) {
+        c->bin[0] = c->bin[1] = c->bin[2] = 0;
+        c->max_ub = 0;
+        c->big_rows = 0;
+        c->max_blocks = 0;
+        c->total_ub = 0;
+        c->big_blocks = 0;
+    }
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_upper_bound(int32_t brow, const int32_t *__restrict__ ap,
+                                                     const int32_t *__restrict__ ac,
+                                                     const int32_t *__restrict__ bp,
+                                                     int64_t *__restrict__ ub, Counters *c) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int v = 0;
+    if (i < brow) {
+        int64_t s = 0;
+        for (int p = ap[i]; p < ap[i + 1]; ++p) {
+            const int k = ac[p];
+            s += bp[k + 1] - bp[k];
+        }
+        ub[i] = s;
+        v = s > 0x7fffffff ? 0x7fffffff : int(s);
+    }
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(&c->max_ub, v);
+}
+
+// wave-aggregated: one atomic per wave and bin
+__global__ __launch_bounds__(256) void k_bin(int32_t brow, const int64_t *__restrict__ ub,
+                                             int32_t *__restrict__ list, Counters *c) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    int b = -1;
+    if (i < brow) {
+        const int64_t u = ub[i];
+        b = u <= kBin0Ub ? 0 : (u <= kBin1Ub ? 1 : 2);
+    }
+    const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const uint64_t mask = __ballot(b == q);
+        if (mask == 0ull) continue;
+        const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&c->bin[q], __popcll(mask));
+        base = __shfl(base, leader);
+        if (b == q) list[int64_t(q) * brow + base + __popcll(mask & lt)] = i;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_c_stats(int32_t brow, const int32_t *__restrict__ cnt,
+                                                 int32_t *__restrict__ big, Counters *c) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = (i < brow) ? cnt[i] : 0;
+    const int mx = wave_max(n);
+    if ((threadIdx.x & 63) == 0 && mx > 0) atomicMax(&c->max_blocks, mx);
+    if (n > kNumCap) {
+        const int pos = atomicAdd(&c->big_rows, 1);
+        big[pos] = i;
+        atomicAdd(reinterpret_cast<unsigned long long *>(&c->big_blocks),
+                  static_cast<unsigned long long>(n));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_compact_c(int32_t brow, const int64_t *__restrict__ ub_off,
+                                                   const int32_t *__restrict__ cptr,
+                                                   const int32_t *__restrict__ tcol,
+                                                   const uint32_t *__restrict__ tmap,
+                                                   int32_t *__restrict__ ccol,
+                                                   uint16_t *__restrict__ cmap) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= brow) return;
+    const int64_t s = ub_off[i];
+    const int p0 = cptr[i], n = cptr[i + 1] - p0;
+    for (int t = lane; t < n; t += 64) {
+        ccol[p0 + t] = tcol[s + t];
+        cmap[p0 + t] = uint16_t(tmap[s + t]);
+    }
+}
+
+// ---- numeric -------------------------------------------------------------------------
+// Operand layout of v_mfma_f64_4x4x4f64 on gfx950 (measured with an in-bounds
+// probe: unit B entries against distinct A entries), block slot b of the four:
+//   A_b[i][k] in lane 16k + 4b + i,  B_b[k][n] in lane 16k + 4b + n,
+//   D_b[i][n] in lane 16i + 4b + n.
+// So in lane L: slot b = (L >> 2) & 3; the A operand is bit 4*(L&3) + (L>>4)
+// of the A bitmap; the B operand and the result are bit 4*(L>>4) + (L&3).
+__device__ __forceinline__ double block_product(double a, double b) {
+    return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, 0.0, 0, 0, 0);
+}
+
+__device__ __forceinline__ int lower_bound_key(const int32_t *keys, int n, int key) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (keys[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// one wave per C block row; acc points to LDS (cnt <= kNumCap) or a global slab
+__device__ void numeric_row(int i, const SpgemmArgs &g, const int32_t *__restrict__ cptr,
+                            const int32_t *__restrict__ ccol, const uint16_t *__restrict__ cmap,
+                            const int32_t *__restrict__ cvp, double *__restrict__ cval,
+                            int32_t *skey, double *acc) {
+    const int lane = threadIdx.x & 63;
+    // MFMA operand roles of this lane (see block_product)
+    const int bs = (lane >> 2) & 3;                       // block slot
+    const int eA = 4 * (lane & 3) + (lane >> 4);          // A operand bit
+    const int eB = 4 * (lane >> 4) + (lane & 3);          // B operand bit = result bit
+    const uint32_t belowA = (1u << eA) - 1u, belowB = (1u << eB) - 1u;
+    // compression roles: slot gs = lane >> 4, bit e = lane & 15
+    const int gs = lane >> 4, e = lane & 15;
+    const uint32_t below = (1u << e) - 1u;
+    const int c0 = cptr[i], cnt = cptr[i + 1] - c0;
+    const int32_t *keys = ccol + c0;
+    if (skey) {
+        for (int t = lane; t < cnt; t += 64) skey[t] = keys[t];
+        keys = skey;
+    }
+    for (int x = lane; x < cnt * 16; x += 64) acc[x] = 0.0;
+    wave_sync();
+    int aoff = g.avp[i];
+    for (int p = g.ap[i]; p < g.ap[i + 1]; ++p) {
+        const int k = g.ac[p];
+        const uint32_t ma = g.am[p];
+        const double a = ((ma >> eA) & 1u) ? g.av[aoff + __popc(ma & belowA)] : 0.0;
+        aoff += __popc(ma);
+        const int q1 = g.bp[k + 1];
+        int boff = g.bvp[k];
+        for (int q0 = g.bp[k]; q0 < q1; q0 += 4) {
+            const int q = q0 + bs;
+            const uint32_t mb = (q < q1) ? uint32_t(g.bm[q]) : 0u;
+            const int pc = __popc(mb);
+            // value offset of B block q: popcounts of the earlier slots (slot b in lane 4b)
+            const int p0c = __shfl(pc, 0), p1c = __shfl(pc, 4), p2c = __shfl(pc, 8),
+                      p3c = __shfl(pc, 12);
+            const int excl = (bs > 0 ? p0c : 0) + (bs > 1 ? p1c : 0) + (bs > 2 ? p2c : 0);
+            const double b = ((mb >> eB) & 1u) ? g.bv[boff + excl + __popc(mb & belowB)] : 0.0;
+            boff += p0c + p1c + p2c + p3c;
+            const uint32_t mc = bitmap_mul(ma, mb);
+            if (__ballot(mc != 0u) == 0ull) continue;  // wave-uniform: MFMA runs with full EXEC
+            const double d = block_product(a, b);
+            if (mc) {
+                const int slot = lower_bound_key(keys, cnt, g.bc[q]);
+                acc[slot * 16 + eB] += d;
+            }
+            wave_sync();
+        }
+    }
+    wave_sync();
+    // compress: four C blocks per step, slot g takes block t0 + g
+    int voff = cvp[i];
+    for (int t0 = 0; t0 < cnt; t0 += 4) {
+        const int t = t0 + gs;
+        const uint32_t m = (t < cnt) ? uint32_t(cmap[c0 + t]) : 0u;
+        const int pc = __popc(m);
+        const int p0c = __shfl(pc, 0), p1c = __shfl(pc, 16), p2c = __shfl(pc, 32),
+                  p3c = __shfl(pc, 48);
+        const int excl = (gs > 0 ? p0c : 0) + (gs > 1 ? p1c : 0) + (gs > 2 ? p2c : 0);
+        if ((m >> e) & 1u) cval[voff + excl + __popc(m & below)] = acc[t * 16 + e];
+        voff += p0c + p1c + p2c + p3c;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_numeric_lds(int32_t brow, SpgemmArgs g,
+                                                     const int32_t *__restrict__ cptr,
+                                                     const int32_t *__restrict__ ccol,
+                                                     const uint16_t *__restrict__ cmap,
+                                                     const int32_t *__restrict__ cvp,
+                                                     double *__restrict__ cval) {
+    __shared__ double s_acc[4][kNumCap * 16];
+    __shared__ int32_t s_key[4][kNumCap];
+    const int w = threadIdx.x >> 6;
+    const int i = blockIdx.x * 4 + w;
+    if (i >= brow) return;
+    if (cptr[i + 1] - cptr[i] > kNumCap) return;  // handled by k_numeric_global
+    numeric_row(i, g, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w]);
+}
+
+__global__ __launch_bounds__(256) void k_numeric_global(int count, const int32_t *__restrict__ rows,
+                                                        SpgemmArgs g,
+                                                        const int32_t *__restrict__ cptr,
+                                                        const int32_t *__restrict__ ccol,
+                                                        const uint16_t *__restrict__ cmap,
+                                                        const int32_t *__restrict__ cvp,
+                                                        double *__restrict__ cval, double *slab,
+                                                        int64_t slab_elems) {
+    const int w = threadIdx.x >> 6;
+    const int wave = blockIdx.x * 4 + w;
+    const int nwaves = gridDim.x * 4;
+    double *acc = slab + int64_t(wave) * slab_elems;
+    for (int r = wave; r < count; r += nwaves) {
+        numeric_row(rows[r], g, cptr, ccol, cmap, cvp, cval, nullptr, acc);
+    }
+}
+
+__global__ void k_max_row(int32_t n, const int32_t *__restrict__ ptr, int32_t *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicMax(out, ptr[i + 1] - ptr[i]);
+}
+
+}  // namespace
+
+int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *Cp) {
+    Mbsr &C = *Cp;
+    C = Mbsr();
+    C.nrows = A.nrows;
+    C.ncols = B.ncols;
+    C.brow = A.brow;
+    C.bcol = B.bcol;
+    C.pattern_only = A.pattern_only || B.pattern_only;
+    if (A.bcol != B.brow) {
+        set_error("SpGEMM: block dimensions of A and B differ");
+        return AMGT_ERR_DIM;
+    }
+    const int32_t brow = A.brow;
+    C.blk_ptr = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(brow) + 1)));
+    C.val_ptr = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(brow) + 1)));
+    if (!C.blk_ptr || !C.val_ptr) {
+        set_error("SpGEMM: allocation failed");
+        return AMGT_ERR_ALLOC;
+    }
+    SpgemmArgs g{A.blk_ptr, A.blk_col, A.val_ptr, A.blk_map, A.val,
+                 B.blk_ptr, B.blk_col, B.val_ptr, B.blk_map, B.val};
+
+    Counters *cnt = nullptr;
+    int64_t *ub = nullptr, *ub_off = nullptr;
+    int32_t *list = nullptr, *ccnt = nullptr, *cnnz = nullptr, *big = nullptr;
+    int32_t *tcol = nullptr;
+    uint32_t *tmap = nullptr, *slab = nullptr;
+    double *nslab = nullptr;
+    int st = AMGT_OK;
+    Counters hc{};
+    auto cleanup = [&]() {
+        tmp_free(cnt, s);
+        tmp_free(ub, s);
+        tmp_free(ub_off, s);
+        tmp_free(list, s);
+        tmp_free(ccnt, s);
+        tmp_free(cnnz, s);
+        tmp_free(big, s);
+        tmp_free(tcol, s);
+        tmp_free(tmap, s);
+        tmp_free(slab, s);
+        tmp_free(nslab, s);
+    };
+#define SG_TRY(expr)               \
+    do {                           \
+        st = (expr);               \
+        if (st != AMGT_OK) {       \
+            cleanup();             \
+            return st;             \
+        }                          \
+    } while (0)
+#define SG_HIP(expr)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (expr);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            set_error(std::string("SpGEMM: ") + #expr + ": " + hipGetErrorString(e_)); \
+            cleanup();                                                            \
+            return AMGT_ERR_HIP;                                                  \
+        }                                                                         \
+    } while (0)
+
+    SG_TRY(tmp_alloc_t(&cnt, 1, s));
+    SG_TRY(tmp_alloc_t(&ub, size_t(brow) + 1, s));
+    SG_TRY(tmp_alloc_t(&ub_off, size_t(brow) + 1, s));
+    SG_TRY(tmp_alloc_t(&list, size_t(brow) * 3 + 1, s));
+    SG_TRY(tmp_alloc_t(&ccnt, size_t(brow) + 1, s));
+    SG_TRY(tmp_alloc_t(&cnnz, size_t(brow) + 1, s));
+    SG_TRY(tmp_alloc_t(&big, size_t(brow) + 1, s));
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, cnt);
+    const unsigned gr = unsigned((brow + 255) / 256);
+    if (brow > 0) {
+        hipLaunchKernelGGL(k_upper_bound, dim3(gr), dim3(256), 0, s, brow, A.blk_ptr, A.blk_col,
+                           B.blk_ptr, ub, cnt);
+        hipLaunchKernelGGL(k_bin, dim3(gr), dim3(256), 0, s, brow, ub, list, cnt);
+    }
+    SG_HIP(hipGetLastError());
+    SG_TRY(exclusive_scan_i64(ub, ub_off, brow, s));
+    int64_t total_ub = 0;
+    SG_HIP(hipMemcpyAsync(&total_ub, ub_off + brow, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    SG_HIP(hipMemcpyAsync(&hc, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    SG_HIP(hipStreamSynchronize(s));  // sync 1: sizes of the temporaries and bins
+    SG_TRY(tmp_alloc_t(&tcol, size_t(total_ub), s));
+    SG_TRY(tmp_alloc_t(&tmap, size_t(total_ub), s));
+    if (hc.bin[0] > 0)
+        hipLaunchKernelGGL(k_symbolic_wave, dim3((hc.bin[0] + 3) / 4), dim3(256), 0, s, hc.bin[0],
+                           list, g, ub_off, tcol, tmap, ccnt, cnnz);
+    if (hc.bin[1] > 0)
+        hipLaunchKernelGGL(k_symbolic_block, dim3(hc.bin[1]), dim3(256), 0, s, hc.bin[1],
+                           list + brow, g, ub_off, tcol, tmap, ccnt, cnnz);
+    if (hc.bin[2] > 0) {
+        int hs = 1;
+        const int64_t want = std::min<int64_t>(2 * int64_t(hc.max_ub), 2 * int64_t(B.bcol) + 2);
+        while (hs < want) hs <<= 1;
+        const int64_t per_wg = 2 * int64_t(hs);
+        int nwg = std::min(hc.bin[2], 512);
+        const int64_t budget = int64_t(1) << 29;  // 2 GiB of uint32 slots
+        while (nwg > 1 && per_wg * nwg > budget) nwg >>= 1;
+        if (per_wg > budget) {
+            set_error("SpGEMM: a block row's product is too large for the global hash table");
+            cleanup();
+            return AMGT_ERR_UNSUPPORTED;
+        }
+        SG_TRY(tmp_alloc_t(&slab, size_t(per_wg * nwg), s));
+        hipLaunchKernelGGL(k_symbolic_global, dim3(nwg), dim3(256), 0, s, hc.bin[2],
+                           list + 2 * int64_t(brow), g, ub_off, tcol, tmap, ccnt, cnnz, slab, hs);
+    }
+    SG_HIP(hipGetLastError());
+    SG_TRY(exclusive_scan_i32(ccnt, C.blk_ptr, brow, s));
+    SG_TRY(exclusive_scan_i32(cnnz, C.val_ptr, brow, s));
+    if (brow > 0)
+        hipLaunchKernelGGL(k_c_stats, dim3(gr), dim3(256), 0, s, brow, ccnt, big, cnt);
+    SG_HIP(hipGetLastError());
+    int32_t nblk = 0, nnz = 0;
+    SG_HIP(hipMemcpyAsync(&nblk, C.blk_ptr + brow, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    SG_HIP(hipMemcpyAsync(&nnz, C.val_ptr + brow, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    SG_HIP(hipMemcpyAsync(&hc, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    SG_HIP(hipStreamSynchronize(s));  // sync 2: sizes of C
+    C.nblk = nblk;
+    C.nnz = nnz;
+    C.max_row_blocks = hc.max_blocks;
+    C.avg_nnz = nblk ? double(nnz) / double(nblk) : 0.0;
+    C.blk_col = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * size_t(nblk ? nblk : 1)));
+    C.blk_map = static_cast<uint16_t *>(dev_alloc(sizeof(uint16_t) * size_t(nblk ? nblk : 1)));
+    if (!C.pattern_only)
+        C.val = static_cast<double *>(dev_alloc(sizeof(double) * size_t(nnz ? nnz : 1)));
+    if (!C.blk_col || !C.blk_map || (!C.pattern_only && !C.val)) {
+        set_error("SpGEMM: allocation of C failed");
+        cleanup();
+        return AMGT_ERR_ALLOC;
+    }
+    if (brow > 0)
+        hipLaunchKernelGGL(k_compact_c, dim3((brow + 3) / 4), dim3(256), 0, s, brow, ub_off,
+                           C.blk_ptr, tcol, tmap, C.blk_col, C.blk_map);
+    SG_HIP(hipGetLastError());
+    if (!C.pattern_only && brow > 0) {
+        hipLaunchKernelGGL(k_numeric_lds, dim3((brow + 3) / 4), dim3(256), 0, s, brow, g,
+                           C.blk_ptr, C.blk_col, C.blk_map, C.val_ptr, C.val);
+        if (hc.big_rows > 0) {
+            const int64_t elems = int64_t(hc.max_blocks) * 16;
+            int nwaves = std::min(hc.big_rows, 1024);
+            while (nwaves > 4 && elems * nwaves * 8 > (int64_t(1) << 31)) nwaves >>= 1;
+            nwaves = (nwaves + 3) / 4 * 4;  // every wave of the grid owns a slab
+            SG_TRY(tmp_alloc_t(&nslab, size_t(elems * nwaves), s));
+            hipLaunchKernelGGL(k_numeric_global, dim3((nwaves + 3) / 4), dim3(256), 0, s,
+                               hc.big_rows, big, g, C.blk_ptr, C.blk_col, C.blk_map, C.val_ptr,
+                               C.val, nslab, elems);
+        }
+        SG_HIP(hipGetLastError());
+    }
+    // block-row length statistics of C
+    if (brow > 0 && nblk > 0) {
+        double avg = double(nblk) / double(brow);
+        (void)avg;
+    }
+    cleanup();
+#undef SG_TRY
+#undef SG_HIP
+    return AMGT_OK;
 }
 
 }  // namespace amgt

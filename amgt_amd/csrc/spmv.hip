@@ -118,6 +118,119 @@ __global__ __launch_bounds__(256) void k_spmv(
     }
 }
 
+// Staged variant: a workgroup owns 256/G consecutive block rows.  Their block
+// columns, bitmaps and values (contiguous ranges) are first copied to LDS with
+// coalesced loads that depend only on the two pointer pairs, so every matrix
+// byte is requested at once; the popcount walk then reads LDS and only the x
+// gathers wait on loaded indices.  Tiles larger than the LDS capacity read
+// global memory directly (same arithmetic).
+template <typename T>
+__device__ __forceinline__ T stream_load(const T *p) {
+    return __builtin_nontemporal_load(p);
+}
+template <>
+__device__ __forceinline__ __half stream_load<__half>(const __half *p) {
+    return *p;
+}
+
+template <int G, typename VT, typename AT>
+__device__ __forceinline__ AT row_walk(const int32_t *__restrict__ col, const uint16_t *__restrict__ map,
+                                       const VT *__restrict__ val, int p0, int p1, int voff,
+                                       int s, int r, const double *__restrict__ x) {
+    constexpr int S = G / 4;
+    const uint32_t low = (1u << (4 * r)) - 1u;
+    AT acc = AT(0);
+    for (int jb = p0; jb < p1; jb += S) {
+        const int j = jb + s;
+        uint32_t m = 0;
+        int c4 = 0;
+        if (j < p1) {
+            m = map[j];
+            c4 = col[j];
+        }
+        const int cnt = __popc(m);
+        int incl = cnt;
+        int tot = cnt;
+        if constexpr (S > 1) {
+#pragma unroll
+            for (int d = 1; d < S; d <<= 1) {
+                int t = __shfl_up(incl, 4 * d, G);
+                if (s >= d) incl += t;
+            }
+            tot = __shfl(incl, 4 * (S - 1) + r, G);
+        }
+        int o = voff + (incl - cnt) + __popc(m & low);
+        uint32_t nib = (m >> (4 * r)) & 0xFu;
+        const double *xs = x + 4 * int64_t(c4);
+        while (nib) {
+            const int c = __builtin_ctz(nib);
+            nib &= nib - 1u;
+            acc = mul_acc<VT, AT>(acc, val[o], xs[c]);
+            ++o;
+        }
+        voff += tot;
+    }
+    if constexpr (S > 1) {
+#pragma unroll
+        for (int d = S / 2; d >= 1; d >>= 1) acc += __shfl_xor(acc, 4 * d, G);
+    }
+    return acc;
+}
+
+template <int G, typename VT, typename AT, int CAPV, int CAPB>
+__global__ __launch_bounds__(256) void k_spmv_staged(
+    int32_t brow, int32_t nrows, const int32_t *__restrict__ blk_ptr,
+    const int32_t *__restrict__ blk_col, const uint16_t *__restrict__ blk_map,
+    const int32_t *__restrict__ val_ptr, const VT *__restrict__ val, const double *__restrict__ x,
+    double alpha, double beta, const double *b, double *y) {
+    constexpr int RPW = 64 / G;
+    constexpr int RPB = 256 / G;
+    __shared__ VT s_val[CAPV];
+    __shared__ int32_t s_col[CAPB];
+    __shared__ uint16_t s_map[CAPB];
+    __shared__ int32_t s_bp[RPB + 1], s_vp[RPB + 1];
+    const int wg = remap_wg(blockIdx.x, gridDim.x);
+    const int br0 = wg * RPB;
+    const int nr = min(RPB, brow - br0);
+    for (int t = threadIdx.x; t <= nr; t += 256) {
+        s_bp[t] = blk_ptr[br0 + t];
+        s_vp[t] = val_ptr[br0 + t];
+    }
+    __syncthreads();
+    const int pb = s_bp[0], pe = s_bp[nr];
+    const int vb = s_vp[0], ve = s_vp[nr];
+    const bool staged = (pe - pb <= CAPB) && (ve - vb <= CAPV);
+    if (staged) {
+        for (int t = threadIdx.x; t < ve - vb; t += 256) s_val[t] = stream_load(val + vb + t);
+        for (int t = threadIdx.x; t < pe - pb; t += 256) {
+            s_col[t] = stream_load(blk_col + pb + t);
+            s_map[t] = stream_load(blk_map + pb + t);
+        }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int lr = (threadIdx.x >> 6) * RPW + lane / G;  // block row within the tile
+    if (lr >= nr) return;
+    const int sub = lane % G;
+    const int r = sub & 3;
+    const int s = sub >> 2;
+    AT acc;
+    if (staged) {
+        acc = row_walk<G, VT, AT>(s_col, s_map, s_val, s_bp[lr] - pb, s_bp[lr + 1] - pb,
+                                  s_vp[lr] - vb, s, r, x);
+    } else {
+        acc = row_walk<G, VT, AT>(blk_col, blk_map, val, s_bp[lr], s_bp[lr + 1], s_vp[lr], s, r, x);
+    }
+    if (s == 0) {
+        const int row = 4 * (br0 + lr) + r;
+        if (row < nrows) {
+            double v = alpha * double(acc);
+            if (beta != 0.0) v = fma(beta, b[row], v);
+            y[row] = v;
+        }
+    }
+}
+
 // k right-hand sides (column-major, leading dimensions), KT at a time.
 template <int G, int KT, typename VT, typename AT>
 __global__ __launch_bounds__(256) void k_spmm(
@@ -232,9 +345,34 @@ int launch_spmv_g(const Mbsr &A, const VT *val, double alpha, const double *x, d
     return AMGT_OK;
 }
 
+constexpr int kStageVals = 2048;
+constexpr int kStageBlocks = 1024;
+
+template <int G, typename VT, typename AT>
+int launch_spmv_staged_g(const Mbsr &A, const VT *val, double alpha, const double *x, double beta,
+                         const double *b, double *y, hipStream_t s) {
+    constexpr int RPB = 256 / G;
+    const int64_t grid = (int64_t(A.brow) + RPB - 1) / RPB;
+    hipLaunchKernelGGL((k_spmv_staged<G, VT, AT, kStageVals, kStageBlocks>), dim3(unsigned(grid)),
+                       dim3(256), 0, s, A.brow, A.nrows, A.blk_ptr, A.blk_col, A.blk_map,
+                       A.val_ptr, val, x, alpha, beta, b, y);
+    AMGT_HIP(hipGetLastError());
+    return AMGT_OK;
+}
+
 template <typename VT, typename AT>
 int launch_spmv_t(const Mbsr &A, const VT *val, int lanes, double alpha, const double *x,
                   double beta, const double *b, double *y, hipStream_t s) {
+    if (A.spmv_staged) {
+        switch (lanes) {
+            case 4: return launch_spmv_staged_g<4, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            case 8: return launch_spmv_staged_g<8, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            case 16: return launch_spmv_staged_g<16, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            case 32: return launch_spmv_staged_g<32, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            case 64: return launch_spmv_staged_g<64, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            default: break;
+        }
+    }
     switch (lanes) {
         case 4: return launch_spmv_g<4, VT, AT>(A, val, alpha, x, beta, b, y, s);
         case 8: return launch_spmv_g<8, VT, AT>(A, val, alpha, x, beta, b, y, s);

@@ -126,6 +126,10 @@ class MBSRMatrix:
     def set_spmv_lanes(self, lanes: int):
         L.check(L.load().amgt_mbsr_set_spmv_lanes(self.handle, lanes), "amgt_mbsr_set_spmv_lanes")
 
+    def set_spmv_staged(self, staged: bool):
+        L.check(L.load().amgt_mbsr_set_spmv_staged(self.handle, int(bool(staged))),
+                "amgt_mbsr_set_spmv_staged")
+
     def autotune(self, stream=None):
         L.check(L.load().amgt_mbsr_autotune(self.handle, _stream_ptr(stream)), "amgt_mbsr_autotune")
 

@@ -116,3 +116,69 @@ def config_matrix(name: str):
     cfg = CONFIGS[name]
     nx, ny, nz = cfg["n"]
     return stencil(cfg["kind"], nx, ny, nz, cfg["c"])
+
+
+def _interp_1d(n: int):
+    """Fine index -> list of (coarse index, weight) for coarsening by 2 (coarse
+    points at even fine indices, linear interpolation, zero Dirichlet outside)."""
+    nc = (n + 1) // 2
+    f = np.arange(n)
+    even = (f % 2) == 0
+    c_lo = f // 2
+    c_hi = c_lo + 1
+    w_lo = np.where(even, 1.0, 0.5)
+    w_hi = np.where(even, 0.0, 0.5)
+    has_hi = (~even) & (c_hi < nc)
+    return nc, c_lo, w_lo, c_hi, np.where(has_hi, w_hi, 0.0), has_hi
+
+
+def trilinear_interpolation(nx: int, ny: int, nz: int = 1):
+    """CSR (I, J, A) of the geometric prolongation from the grid of even-index
+    points to the full nx x ny x nz grid (tensor product of 1D linear weights,
+    up to 8 entries per row, ascending columns).  Returns (I, J, A, ncoarse)."""
+    parts = []
+    for n in (nx, ny, nz):
+        nc, c_lo, w_lo, c_hi, w_hi, has_hi = _interp_1d(n)
+        cols = np.stack([c_lo, c_hi], axis=1)
+        wts = np.stack([w_lo, w_hi], axis=1)
+        ok = np.stack([np.ones(n, bool), has_hi], axis=1)
+        parts.append((nc, cols, wts, ok))
+    (ncx, cx, wx, okx), (ncy, cy, wy, oky), (ncz, cz, wz, okz) = parts
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    x, y, z = x.ravel(), y.ravel(), z.ravel()
+    cols, vals, mask = [], [], []
+    for kz in range(2):
+        for ky in range(2):
+            for kx in range(2):
+                m = okx[x, kx] & oky[y, ky] & okz[z, kz]
+                c = (cz[z, kz] * ncy + cy[y, ky]) * ncx + cx[x, kx]
+                v = wx[x, kx] * wy[y, ky] * wz[z, kz]
+                cols.append(c)
+                vals.append(v)
+                mask.append(m & (v != 0.0))
+    C = np.stack(cols, axis=1)
+    V = np.stack(vals, axis=1)
+    M = np.stack(mask, axis=1)
+    # ascending columns within a row: order slots by column
+    order = np.argsort(np.where(M, C, np.iinfo(np.int64).max), axis=1, kind="stable")
+    C = np.take_along_axis(C, order, axis=1)
+    V = np.take_along_axis(V, order, axis=1)
+    M = np.take_along_axis(M, order, axis=1)
+    counts = M.sum(axis=1)
+    I = np.zeros(len(x) + 1, dtype=np.int64)
+    np.cumsum(counts, out=I[1:])
+    return I.astype(np.int32), C[M].astype(np.int32), V[M].astype(np.float64), ncx * ncy * ncz
+
+
+def csr_transpose(nrows: int, ncols: int, I, J, A):
+    """CSR of the transpose (rows of the result in ascending column order)."""
+    nnz = len(J)
+    rows = np.repeat(np.arange(nrows, dtype=np.int64), np.diff(I.astype(np.int64)))
+    order = np.lexsort((rows, J))
+    TJ = rows[order].astype(np.int32)
+    TA = np.asarray(A)[order]
+    counts = np.bincount(J, minlength=ncols)
+    TI = np.zeros(ncols + 1, dtype=np.int64)
+    np.cumsum(counts, out=TI[1:])
+    assert TI[-1] == nnz
+    return TI.astype(np.int32), TJ, TA

@@ -56,6 +56,62 @@ def cpu_baseline(I, J, A, x, bytes_per_call, budget_s=10.0):
                       f"matrix, {el:.1f} s; {per * 1e3:.2f} ms per SpMV"}
 
 
+def csr_flops(AI, AJ, BI):
+    """F_alg = 2 * sum over nonzeros a_ik of nnz(B row k) (CSR product flops)."""
+    import numpy as np
+    blen = np.diff(BI.astype(np.int64))
+    return int(2 * blen[AJ].sum())
+
+
+def spgemm_timings(W, MBSRMatrix, M, I, J, A, dev, reps=3):
+    """Setup-phase products on the same 128^3 7-pt matrix: A^2 (HYPRE's
+    ij_mm benchmark job) and the Galerkin pair A*P, P^T*(AP) with the
+    trilinear prolongation to the 64^3 grid (a geometric P, not HYPRE's HMIS +
+    ext+i P).  Wall time per call including the size syncs."""
+    import numpy as np
+    import torch
+
+    n = len(I) - 1
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            C = fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+            del C
+        return sorted(ts)[len(ts) // 2] * 1e3
+
+    out = {}
+    a2_ms = timed(lambda: M.spgemm(M))
+    C = M.spgemm(M)
+    f = csr_flops(I, J, I)
+    out["a2"] = {"ms": round(a2_ms, 3), "nnz": int(C.info()["nnz"]), "flops": f,
+                 "gflops": round(f / (a2_ms * 1e-3) / 1e9, 2)}
+    del C
+    PI, PJ, PA, nc = W.trilinear_interpolation(128, 128, 128)
+    RI, RJ, RA = W.csr_transpose(n, nc, PI, PJ, PA)
+    t = lambda v: torch.from_numpy(v).to(dev)  # noqa: E731
+    P = MBSRMatrix.from_csr(t(PI), t(PJ), t(PA), n, nc)
+    R = MBSRMatrix.from_csr(t(RI), t(RJ), t(RA), nc, n)
+    ap_ms = timed(lambda: M.spgemm(P))
+    AP = M.spgemm(P)
+    APi, APj, _ = AP.to_csr()
+    rap_ms = timed(lambda: R.spgemm(AP))
+    RAP = R.spgemm(AP)
+    f_ap = csr_flops(I, J, PI)
+    f_rap = csr_flops(RI, RJ, APi.cpu().numpy())
+    out["rap_level0_trilinear"] = {
+        "ap_ms": round(ap_ms, 3), "rap_ms": round(rap_ms, 3),
+        "total_ms": round(ap_ms + rap_ms, 3), "coarse_rows": nc,
+        "coarse_nnz": int(RAP.info()["nnz"]), "flops": f_ap + f_rap,
+        "gflops": round((f_ap + f_rap) / ((ap_ms + rap_ms) * 1e-3) / 1e9, 2)}
+    return out
+
+
 def load_traffic(kernel_key):
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
@@ -75,6 +131,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--lanes", type=int, default=0, help="force lanes per block row (0 = tuner)")
+    ap.add_argument("--no-spgemm", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the mBSR path is "
@@ -104,25 +161,29 @@ def main():
     conv_ms = (time.perf_counter() - tc) * 1e3
     info = M.info()
 
-    # per-variant timings (lanes per block row), then the tuner's pick
+    # per-variant timings (staged/direct x lanes per block row), then the tuner's pick
     variants = {}
-    for lanes in (4, 8, 16, 32, 64):
-        M.set_spmv_lanes(lanes)
-        for _ in range(10):
-            M.spmv(x, y)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(50):
-            M.spmv(x, y)
-        e1.record()
-        torch.cuda.synchronize()
-        variants[lanes] = e0.elapsed_time(e1) / 50
+    for staged in (1, 0):
+        M.set_spmv_staged(staged)
+        for lanes in (4, 8, 16, 32, 64):
+            M.set_spmv_lanes(lanes)
+            for _ in range(10):
+                M.spmv(x, y)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(50):
+                M.spmv(x, y)
+            e1.record()
+            torch.cuda.synchronize()
+            variants[f"{'staged' if staged else 'direct'}_{lanes}"] = e0.elapsed_time(e1) / 50
     if args.lanes:
+        M.set_spmv_staged(1)
         M.set_spmv_lanes(args.lanes)
     else:
         M.set_spmv_lanes(0)
         M.autotune()
     lanes = M.info()["spmv_lanes"]
+    staged = M.info()["spmv_staged"]
 
     for _ in range(args.warmup):
         M.spmv(x, y)
@@ -150,6 +211,8 @@ def main():
     torch.cuda.synchronize()
     resid_ms = e0.elapsed_time(e1) / 50
 
+    spgemm = None if args.no_spgemm else spgemm_timings(W, MBSRMatrix, M, I, J, A, dev)
+
     B = spmv_bytes(info)
     value = B / (ms_per_step * 1e-3) / 1e9
     achieved = B / (kern_ms * 1e-3) / 1e9
@@ -176,10 +239,13 @@ def main():
         "spmv_alg_bytes": int(B),
         "kernel_ms": round(kern_ms, 5),
         "residual_form_ms": round(resid_ms, 5),
-        "lanes_variant_ms": {str(k): round(v, 5) for k, v in variants.items()},
+        "spmv_kernel": "staged" if staged else "direct",
+        "variant_ms": {str(k): round(v, 5) for k, v in variants.items()},
         "conversion_ms": round(conv_ms, 2),
         "input_gen_s": round(gen_s, 2),
     }
+    if spgemm is not None:
+        out["spgemm"] = spgemm
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(I, J, A, x_h, B)
     print(json.dumps(out), flush=True)

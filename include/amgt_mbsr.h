@@ -53,6 +53,7 @@ typedef struct {
     int32_t level;      /* coarsening level tag (hypre_Level, seq_mv.h:199) */
     int32_t precision;  /* value precision used by SpMV (AMGT_DOUBLE/FLOAT/HALF) */
     int32_t spmv_lanes; /* lanes per block row chosen by the tuner (4..64) */
+    int32_t spmv_staged; /* 1: LDS-staged tile kernel, 0: direct kernel */
 } amgt_mbsr_info_t;
 
 typedef struct {
@@ -97,6 +98,9 @@ int amgt_mbsr_set_precision(amgt_mbsr_t A, int precision);
 /* lanes per block row for SpMV: 4, 8, 16, 32 or 64; 0 = choose from the
  * matrix statistics */
 int amgt_mbsr_set_spmv_lanes(amgt_mbsr_t A, int lanes);
+/* 1 (default): a workgroup copies its tile's block columns, bitmaps and values
+ * to LDS before the walk; 0: the walk reads global memory directly */
+int amgt_mbsr_set_spmv_staged(amgt_mbsr_t A, int staged);
 /* times every lanes-per-block-row variant on this matrix and keeps the fastest */
 int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream);
 

@@ -108,12 +108,14 @@ def row_scale(m, n, I, J, A, x, beta, b):
 @pytest.mark.parametrize("name", ["c1_5pt_128", "7pt_aniso", "27pt_10", "rand_rect",
                                   "rand_tall_empty", "rand_wide", "bin1_rows", "bin2_rows"])
 @pytest.mark.parametrize("lanes", [4, 8, 16, 32, 64])
-def test_spmv_fp64(dev, name, lanes):
-    print(f"[spmv] {name} lanes={lanes}", flush=True)
+@pytest.mark.parametrize("staged", [1, 0])
+def test_spmv_fp64(dev, name, lanes, staged):
+    print(f"[spmv] {name} lanes={lanes} staged={staged}", flush=True)
     (m, n), I, J, A = CASES[name]()
     ref = O.csr2mbsr(m, n, I, J, A)
     M = make(dev, m, n, I, J, A)
     M.set_spmv_lanes(lanes)
+    M.set_spmv_staged(staged)
     rng = np.random.default_rng(lanes)
     x, b = rng.standard_normal(n), rng.standard_normal(m)
     for alpha, beta in [(1.0, 0.0), (-1.0, 1.0), (0.5, -2.0)]:

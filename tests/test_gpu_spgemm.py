@@ -1,0 +1,162 @@
+"""GPU parity of the mBSR SpGEMM and the CSR export against the CPU oracle,
+through the C-ABI.
+
+Bars: C's block structure (blk_ptr, blk_col, blk_map, val_ptr) and the CSR
+pattern bit-exact; FP64 values within 1e-12 relative to (|A| |B|) entrywise.
+Patterns are also checked against HYPRE's host product (orc_csr_spgemm, without
+its forced diagonal) and, at full size, against HYPRE's own A^2 nnz fixture.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+from amgt_amd import _lib as L
+from amgt_amd import hypre as H
+from amgt_amd import workloads as W
+from amgt_amd.mbsr import MBSRMatrix
+from oracle import oracle as O
+from tests.util_matrices import dense_rows_csr, rand_csr
+
+pytestmark = pytest.mark.gpu
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "hypre_fixtures.json")))
+
+
+def T(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def make(dev, m, n, I, J, A):
+    return MBSRMatrix.from_csr(T(I, dev), T(J, dev), None if A is None else T(A, dev), m, n)
+
+
+def abs_product_scale(m, k, n, AI, AJ, AA, BI, BJ, BA):
+    SA = sp.csr_matrix((np.abs(AA), AJ, AI), shape=(m, k))
+    SB = sp.csr_matrix((np.abs(BA), BJ, BI), shape=(k, n))
+    return (SA @ SB).tocsr()
+
+
+def check_product(dev, m, k, n, AI, AJ, AA, BI, BJ, BA, tag=""):
+    print(f"[spgemm] {tag} {m}x{k}x{n}", flush=True)
+    MA, MB = O.csr2mbsr(m, k, AI, AJ, AA), O.csr2mbsr(k, n, BI, BJ, BA)
+    ref = O.mbsr_spgemm(MA, MB)
+    GA, GB = make(dev, m, k, AI, AJ, AA), make(dev, k, n, BI, BJ, BA)
+    GC = GA.spgemm(GB)
+    inf = GC.info()
+    assert (inf["nblk"], inf["nnz"]) == (ref["nblk"], ref["nnz"])
+    a = GC.arrays()
+    assert np.array_equal(a["blk_ptr"].cpu().numpy(), ref["blk_ptr"])
+    assert np.array_equal(a["blk_col"].cpu().numpy(), ref["blk_col"])
+    assert np.array_equal(a["blk_map"].cpu().numpy().view(np.uint16), ref["blk_map"])
+    assert np.array_equal(a["val_ptr"].cpu().numpy(), ref["val_ptr"])
+    # values through the CSR export (sorted columns)
+    CI, CJ, CV = GC.to_csr()
+    RI, RJ, RV = O.mbsr2csr(ref)
+    assert np.array_equal(CI.cpu().numpy(), RI) and np.array_equal(CJ.cpu().numpy(), RJ)
+    # |A||B| has exactly the structural pattern of A*B (no cancellation), in the
+    # same sorted CSR order as the export
+    scale = abs_product_scale(m, k, n, AI, AJ, AA, BI, BJ, BA)
+    scale.sort_indices()
+    assert np.array_equal(scale.indptr, RI) and np.array_equal(scale.indices, RJ)
+    sc = scale.data
+    err = np.abs(CV.cpu().numpy() - RV)
+    assert np.all(err <= 1e-12 * sc + 1e-300)
+    return GC, ref
+
+
+@pytest.mark.parametrize("shape,dens,seed", [((37, 53, 29), (0.15, 0.2), 1),
+                                             ((16, 16, 16), (0.3, 0.3), 2),
+                                             ((9, 70, 13), (0.2, 0.25), 3),
+                                             ((101, 9, 77), (0.3, 0.1), 4)])
+def test_spgemm_random(dev, shape, dens, seed):
+    m, k, n = shape
+    AI, AJ, AA = rand_csr(m, k, dens[0], seed)
+    BI, BJ, BA = rand_csr(k, n, dens[1], seed + 50)
+    check_product(dev, m, k, n, AI, AJ, AA, BI, BJ, BA, "random")
+
+
+def test_spgemm_stencils(dev):
+    for kind, dims in [(7, (10, 9, 8)), (27, (7, 6, 5)), (7, (33, 31, 1)), (9, (20, 17, 1))]:
+        I, J, A = W.stencil(kind, *dims)
+        n = len(I) - 1
+        check_product(dev, n, n, n, I, J, A, I, J, A, f"stencil{kind}")
+
+
+def test_spgemm_bins_and_big_rows(dev):
+    # block rows with ub in (128, 2048] (workgroup LDS table) and > 2048 (global table),
+    # and C block rows with more than 64 blocks (global numeric accumulators)
+    AI, AJ, AA = dense_rows_csr(24, 600, 10, 7)      # ub ~ 350 per block row: LDS table bin
+    BI, BJ, BA = rand_csr(600, 900, 0.01, 8)
+    check_product(dev, 24, 600, 900, AI, AJ, AA, BI, BJ, BA, "bin1")
+    AI, AJ, AA = dense_rows_csr(24, 600, 40, 7)      # ub up to ~13500: global table bin
+    BI, BJ, BA = rand_csr(600, 900, 0.05, 8)
+    check_product(dev, 24, 600, 900, AI, AJ, AA, BI, BJ, BA, "bin2")
+    AI, AJ, AA = dense_rows_csr(8, 3000, 400, 9)
+    BI, BJ, BA = rand_csr(3000, 5000, 0.004, 10)
+    check_product(dev, 8, 3000, 5000, AI, AJ, AA, BI, BJ, BA, "bin2")
+
+
+def test_spgemm_rap(dev):
+    nx, ny, nz = 17, 14, 11
+    AI, AJ, AA = W.stencil(7, nx, ny, nz)
+    n = len(AI) - 1
+    PI, PJ, PA, nc = W.trilinear_interpolation(nx, ny, nz)
+    RI, RJ, RA = W.csr_transpose(n, nc, PI, PJ, PA)
+    GAP, refAP = check_product(dev, n, n, nc, AI, AJ, AA, PI, PJ, PA, "A*P")
+    APi, APj, APv = O.mbsr2csr(refAP)
+    check_product(dev, nc, n, nc, RI, RJ, RA, APi, APj, APv, "R*(AP)")
+
+
+def test_spgemm_deterministic(dev):
+    AI, AJ, AA = rand_csr(60, 70, 0.2, 11)
+    BI, BJ, BA = rand_csr(70, 80, 0.2, 12)
+    GA, GB = make(dev, 60, 70, AI, AJ, AA), make(dev, 70, 80, BI, BJ, BA)
+    v1 = GA.spgemm(GB).arrays()["val"].cpu()
+    v2 = GA.spgemm(GB).arrays()["val"].cpu()
+    assert torch.equal(v1, v2)
+
+
+def test_spgemm_full_size_c2_a_squared(dev):
+    """HYPRE benchmark job 1 (ij_mm -n 128 128 128 -7pt): A^2 has 51,742,208 nonzeros."""
+    I, J, A = W.stencil(7, 128, 128, 128)
+    n = len(I) - 1
+    G = make(dev, n, n, I, J, A)
+    C = G.spgemm(G)
+    nnz = C.info()["nnz"]
+    assert nnz == GOLD["A_squared_nnz"]["7pt_128_128_128"]["nnz"]
+    # values: row sums of A^2 equal A*(A*1) exactly (small integers)
+    ones = torch.ones(n, dtype=torch.float64, device=dev)
+    y = C.spmv(ones)
+    z = G.spmv(G.spmv(ones))
+    assert torch.equal(y, z)
+
+
+def test_hypre_spgemm_boundary(dev):
+    AI, AJ, AA = rand_csr(37, 53, 0.15, 21)
+    BI, BJ, BA = rand_csr(53, 29, 0.2, 22)
+    A = H.CSRMatrix(T(AI, dev), T(AJ, dev), T(AA, dev), 37, 53)
+    B = H.CSRMatrix(T(BI, dev), T(BJ, dev), T(BA, dev), 53, 29)
+    C = H.hypreDevice_CSRSpGemm(A, B)
+    CI, CJ, CA = O.csr_spgemm(37, 29, AI, AJ, AA, BI, BJ, BA, diag_first=0)
+    Sref = sp.csr_matrix((CA, CJ, CI), shape=(37, 29))
+    Sref.sort_indices()
+    Sg = sp.csr_matrix((C.data.cpu().numpy(), C.j.cpu().numpy(), C.i.cpu().numpy()), shape=(37, 29))
+    assert np.array_equal(Sg.indptr, Sref.indptr) and np.array_equal(Sg.indices, Sref.indices)
+    assert abs(Sg - Sref).max() < 1e-12
+    assert A.bsr_mat and B.bsr_mat and C.bsr_mat  # caches kept
+    # trivial case: an empty operand gives a zero row pointer
+    E = H.CSRMatrix(T(np.zeros(54, np.int32), dev), T(np.zeros(0, np.int32), dev),
+                    T(np.zeros(0), dev), 53, 29)
+    Z = H.hypreDevice_CSRSpGemm(A, E)
+    assert Z.num_nonzeros == 0 and torch.count_nonzero(Z.i).item() == 0
+    # dimension mismatch
+    with pytest.raises(L.AmgtError):
+        H.hypreDevice_CSRSpGemm(B, B)
+    # pattern-only operand: pattern computed, no values
+    Ap = H.CSRMatrix(T(AI, dev), T(AJ, dev), None, 37, 53)
+    Cp = H.hypreDevice_CSRSpGemm(Ap, B)
+    assert Cp.data is None
+    assert np.array_equal(Cp.j.cpu().numpy(), Sg.indices)
