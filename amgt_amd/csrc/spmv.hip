@@ -177,9 +177,79 @@ __device__ __forceinline__ AT row_walk(const int32_t *__restrict__ col, const ui
     return acc;
 }
 
+// Walk over LDS-resident blocks that first gathers the x segments (4 doubles,
+// two 16-byte loads) of up to MAXB blocks per lane, then consumes them with
+// straight-line predicated FMAs, so the x loads of a lane are all in flight
+// together.  x must be 16-byte aligned (checked by the launcher); the last
+// block column is read element-wise when ncols is not a multiple of 4.
+template <int G, int MAXB, typename VT, typename AT>
+__device__ __forceinline__ AT row_walk_prefetch(const int32_t *s_col, const uint16_t *s_map,
+                                                const VT *s_val, int p0, int p1, int voff, int s,
+                                                int r, const double *__restrict__ x, int ncols) {
+    constexpr int S = G / 4;
+    const uint32_t low = (1u << (4 * r)) - 1u;
+    AT acc = AT(0);
+    for (int j0 = p0; j0 < p1; j0 += S * MAXB) {
+        double xs[MAXB][4];
+        uint32_t m[MAXB];
+#pragma unroll
+        for (int u = 0; u < MAXB; ++u) {
+            const int j = j0 + s + S * u;
+            m[u] = 0u;
+            xs[u][0] = xs[u][1] = xs[u][2] = xs[u][3] = 0.0;
+            if (j < p1) {
+                m[u] = s_map[j];
+                const int c4 = 4 * s_col[j];
+                const double *xp = x + c4;
+                if (c4 + 4 <= ncols) {
+                    const double2 lo2 = *reinterpret_cast<const double2 *>(xp);
+                    const double2 hi2 = *reinterpret_cast<const double2 *>(xp + 2);
+                    xs[u][0] = lo2.x;
+                    xs[u][1] = lo2.y;
+                    xs[u][2] = hi2.x;
+                    xs[u][3] = hi2.y;
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        if (c4 + c < ncols) xs[u][c] = xp[c];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < MAXB; ++u) {
+            const int cnt = __popc(m[u]);
+            int incl = cnt;
+            int tot = cnt;
+            if constexpr (S > 1) {
+#pragma unroll
+                for (int d = 1; d < S; d <<= 1) {
+                    int t = __shfl_up(incl, 4 * d, G);
+                    if (s >= d) incl += t;
+                }
+                tot = __shfl(incl, 4 * (S - 1) + r, G);
+            }
+            int o = voff + (incl - cnt) + __popc(m[u] & low);
+            const uint32_t nib = (m[u] >> (4 * r)) & 0xFu;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                if ((nib >> c) & 1u) {
+                    acc = mul_acc<VT, AT>(acc, s_val[o], xs[u][c]);
+                    ++o;
+                }
+            }
+            voff += tot;
+        }
+    }
+    if constexpr (S > 1) {
+#pragma unroll
+        for (int d = S / 2; d >= 1; d >>= 1) acc += __shfl_xor(acc, 4 * d, G);
+    }
+    return acc;
+}
+
 template <int G, typename VT, typename AT, int CAPV, int CAPB>
 __global__ __launch_bounds__(256) void k_spmv_staged(
-    int32_t brow, int32_t nrows, const int32_t *__restrict__ blk_ptr,
+    int32_t brow, int32_t nrows, int32_t ncols, const int32_t *__restrict__ blk_ptr,
     const int32_t *__restrict__ blk_col, const uint16_t *__restrict__ blk_map,
     const int32_t *__restrict__ val_ptr, const VT *__restrict__ val, const double *__restrict__ x,
     double alpha, double beta, const double *b, double *y) {
@@ -201,10 +271,38 @@ __global__ __launch_bounds__(256) void k_spmv_staged(
     const int vb = s_vp[0], ve = s_vp[nr];
     const bool staged = (pe - pb <= CAPB) && (ve - vb <= CAPV);
     if (staged) {
-        for (int t = threadIdx.x; t < ve - vb; t += 256) s_val[t] = stream_load(val + vb + t);
-        for (int t = threadIdx.x; t < pe - pb; t += 256) {
-            s_col[t] = stream_load(blk_col + pb + t);
-            s_map[t] = stream_load(blk_map + pb + t);
+        // all loads of a thread are issued before any LDS store (memory-level parallelism)
+        constexpr int VPT = CAPV / 256;
+        constexpr int BPT = CAPB / 256;
+        const int nv = ve - vb, nb = pe - pb;
+        VT tv[VPT];
+        int32_t tc[BPT];
+        uint16_t tm[BPT];
+#pragma unroll
+        for (int u = 0; u < VPT; ++u) {
+            const int t = u * 256 + threadIdx.x;
+            if (t < nv) tv[u] = stream_load(val + vb + t);
+        }
+#pragma unroll
+        for (int u = 0; u < BPT; ++u) {
+            const int t = u * 256 + threadIdx.x;
+            if (t < nb) {
+                tc[u] = stream_load(blk_col + pb + t);
+                tm[u] = stream_load(blk_map + pb + t);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < VPT; ++u) {
+            const int t = u * 256 + threadIdx.x;
+            if (t < nv) s_val[t] = tv[u];
+        }
+#pragma unroll
+        for (int u = 0; u < BPT; ++u) {
+            const int t = u * 256 + threadIdx.x;
+            if (t < nb) {
+                s_col[t] = tc[u];
+                s_map[t] = tm[u];
+            }
         }
     }
     __syncthreads();
@@ -216,8 +314,8 @@ __global__ __launch_bounds__(256) void k_spmv_staged(
     const int s = sub >> 2;
     AT acc;
     if (staged) {
-        acc = row_walk<G, VT, AT>(s_col, s_map, s_val, s_bp[lr] - pb, s_bp[lr + 1] - pb,
-                                  s_vp[lr] - vb, s, r, x);
+        acc = row_walk_prefetch<G, 8, VT, AT>(s_col, s_map, s_val, s_bp[lr] - pb,
+                                              s_bp[lr + 1] - pb, s_vp[lr] - vb, s, r, x, ncols);
     } else {
         acc = row_walk<G, VT, AT>(blk_col, blk_map, val, s_bp[lr], s_bp[lr + 1], s_vp[lr], s, r, x);
     }
@@ -354,8 +452,8 @@ int launch_spmv_staged_g(const Mbsr &A, const VT *val, double alpha, const doubl
     constexpr int RPB = 256 / G;
     const int64_t grid = (int64_t(A.brow) + RPB - 1) / RPB;
     hipLaunchKernelGGL((k_spmv_staged<G, VT, AT, kStageVals, kStageBlocks>), dim3(unsigned(grid)),
-                       dim3(256), 0, s, A.brow, A.nrows, A.blk_ptr, A.blk_col, A.blk_map,
-                       A.val_ptr, val, x, alpha, beta, b, y);
+                       dim3(256), 0, s, A.brow, A.nrows, A.ncols, A.blk_ptr, A.blk_col,
+                       A.blk_map, A.val_ptr, val, x, alpha, beta, b, y);
     AMGT_HIP(hipGetLastError());
     return AMGT_OK;
 }
@@ -363,7 +461,9 @@ int launch_spmv_staged_g(const Mbsr &A, const VT *val, double alpha, const doubl
 template <typename VT, typename AT>
 int launch_spmv_t(const Mbsr &A, const VT *val, int lanes, double alpha, const double *x,
                   double beta, const double *b, double *y, hipStream_t s) {
-    if (A.spmv_staged) {
+    // the staged walk reads x in 16-byte pieces
+    const bool x_aligned = (reinterpret_cast<uintptr_t>(x) & 15u) == 0;
+    if (A.spmv_staged && x_aligned) {
         switch (lanes) {
             case 4: return launch_spmv_staged_g<4, VT, AT>(A, val, alpha, x, beta, b, y, s);
             case 8: return launch_spmv_staged_g<8, VT, AT>(A, val, alpha, x, beta, b, y, s);

@@ -132,6 +132,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--lanes", type=int, default=0, help="force lanes per block row (0 = tuner)")
     ap.add_argument("--no-spgemm", action="store_true")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="profiling mode: tuned SpMV launches plus a calibration copy, no report")
+    ap.add_argument("--copy-mb", type=int, default=256,
+                    help="size of the calibration copy in --kernel-only mode")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the mBSR path is "
@@ -160,6 +164,25 @@ def main():
     torch.cuda.synchronize()
     conv_ms = (time.perf_counter() - tc) * 1e3
     info = M.info()
+
+    if args.kernel_only:
+        # counter runs: the tuned SpMV kernel alone, then a plain device copy of
+        # known size whose FETCH_SIZE / WRITE_SIZE calibrate the counters
+        M.autotune()
+        for _ in range(args.warmup + args.steps):
+            M.spmv(x, y)
+        torch.cuda.synchronize()
+        nel = args.copy_mb * (1 << 20) // 8
+        src = torch.ones(nel, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        inf = M.info()
+        print(json.dumps({"kernel_only": True, "spmv_lanes": inf["spmv_lanes"],
+                          "spmv_staged": inf["spmv_staged"], "spmv_alg_bytes": spmv_bytes(inf),
+                          "copy_bytes_each_way": nel * 8}), flush=True)
+        return
 
     # per-variant timings (staged/direct x lanes per block row), then the tuner's pick
     variants = {}
