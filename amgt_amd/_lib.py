@@ -19,6 +19,12 @@ AMGT_ERR_HIP = 4
 AMGT_ERR_INPUT = 5
 AMGT_ERR_UNSUPPORTED = 6
 
+AMGT_SPMV_DIRECT = 0
+AMGT_SPMV_STAGED = 1
+AMGT_SPMV_ILP2 = 2
+AMGT_SPMV_ILP4 = 3
+SPMV_KERNEL_NAMES = {0: "direct", 1: "staged", 2: "ilp2", 3: "ilp4"}
+
 AMGT_DOUBLE = 0
 AMGT_FLOAT = 1
 AMGT_HALF = 2
@@ -29,7 +35,7 @@ EXPORTS = (
     "amgt_set_mixed_precision", "amgt_mbsr_create_from_csr", "amgt_mbsr_destroy",
     "amgt_mbsr_update_values", "amgt_mbsr_get_info", "amgt_mbsr_get_arrays",
     "amgt_mbsr_set_level", "amgt_mbsr_set_precision", "amgt_mbsr_set_spmv_lanes",
-    "amgt_mbsr_set_spmv_staged",
+    "amgt_mbsr_set_spmv_kernel",
     "amgt_mbsr_autotune", "amgt_mbsr_spmv", "amgt_mbsr_spmm", "amgt_mbsr_spgemm",
     "amgt_mbsr_to_csr", "amgt_free", "amgt_csr_spgemm", "amgt_csr_matvec",
 )
@@ -54,7 +60,7 @@ class MbsrInfo(ctypes.Structure):
         ("stand", ctypes.c_double), ("avg_nnz", ctypes.c_double), ("fill", ctypes.c_double),
         ("max_row_blocks", ctypes.c_int32), ("level", ctypes.c_int32),
         ("precision", ctypes.c_int32), ("spmv_lanes", ctypes.c_int32),
-        ("spmv_staged", ctypes.c_int32),
+        ("spmv_kernel", ctypes.c_int32),
     ]
 
 
@@ -100,7 +106,7 @@ def load():
         "amgt_mbsr_set_level": (_int, [_vp, _int]),
         "amgt_mbsr_set_precision": (_int, [_vp, _int]),
         "amgt_mbsr_set_spmv_lanes": (_int, [_vp, _int]),
-        "amgt_mbsr_set_spmv_staged": (_int, [_vp, _int]),
+        "amgt_mbsr_set_spmv_kernel": (_int, [_vp, _int]),
         "amgt_mbsr_autotune": (_int, [_vp, _vp]),
         "amgt_mbsr_spmv": (_int, [_vp, _int, _dbl, _vp, _dbl, _vp, _vp, _vp]),
         "amgt_mbsr_spmm": (_int, [_vp, _int, _int, _dbl, _vp, _i64, _dbl, _vp, _i64, _vp, _i64,

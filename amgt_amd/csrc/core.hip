@@ -234,13 +234,14 @@ int amgt_mbsr_get_info(amgt_mbsr_t A, amgt_mbsr_info_t *info) {
     info->level = m.level;
     info->precision = effective_precision(m);
     info->spmv_lanes = m.spmv_lanes ? m.spmv_lanes : spmv_default_lanes(m);
-    info->spmv_staged = m.spmv_staged;
+    info->spmv_kernel = m.spmv_kernel;
     return AMGT_OK;
 }
 
-int amgt_mbsr_set_spmv_staged(amgt_mbsr_t A, int staged) {
+int amgt_mbsr_set_spmv_kernel(amgt_mbsr_t A, int kind) {
     AMGT_CHECK_ARG(A != nullptr, "handle is NULL");
-    A->m.spmv_staged = staged ? 1 : 0;
+    AMGT_CHECK_ARG(kind >= AMGT_SPMV_DIRECT && kind <= AMGT_SPMV_ILP4, "unknown SpMV kernel kind");
+    A->m.spmv_kernel = kind;
     return AMGT_OK;
 }
 
@@ -292,9 +293,9 @@ int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream) {
     AMGT_HIP(hipEventCreate(&e1));
     const int cand[5] = {4, 8, 16, 32, 64};
     float best = 1e30f;
-    int best_lanes = spmv_default_lanes(m), best_staged = 1;
-    for (int staged = 1; staged >= 0; --staged) {
-        m.spmv_staged = staged;
+    int best_lanes = spmv_default_lanes(m), best_kind = m.spmv_kernel;
+    for (int kind = AMGT_SPMV_ILP4; kind >= AMGT_SPMV_DIRECT; --kind) {
+        m.spmv_kernel = kind;
         for (int c = 0; c < 5; ++c) {
             int st = spmv(m, prec, cand[c], 1.0, x, 0.0, nullptr, y, s);  // warm-up
             if (st != AMGT_OK) break;
@@ -308,7 +309,7 @@ int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream) {
             if (ms < best) {
                 best = ms;
                 best_lanes = cand[c];
-                best_staged = staged;
+                best_kind = kind;
             }
         }
     }
@@ -317,7 +318,7 @@ int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream) {
     tmp_free(x, s);
     tmp_free(y, s);
     m.spmv_lanes = best_lanes;
-    m.spmv_staged = best_staged;
+    m.spmv_kernel = best_kind;
     return AMGT_OK;
 }
 

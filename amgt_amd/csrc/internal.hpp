@@ -74,7 +74,7 @@ struct Mbsr {
     int32_t level = 0;
     int32_t precision = -1;   // -1: follow the level policy
     int32_t spmv_lanes = 0;   // 0: heuristic
-    int32_t spmv_staged = 1;  // 1: LDS-staged tile kernel, 0: direct kernel
+    int32_t spmv_kernel = 3;  // kSpmvDirect / kSpmvStaged / kSpmvIlp2 / kSpmvIlp4
     Mbsr *trans = nullptr;    // cached A^T
 };
 
@@ -87,6 +87,9 @@ struct amgt_mbsr {
 namespace amgt {
 
 bool mixed_precision_enabled();
+
+// SpMV kernel kinds (amgt_mbsr_set_spmv_kernel)
+enum : int { kSpmvDirect = 0, kSpmvStaged = 1, kSpmvIlp2 = 2, kSpmvIlp4 = 3 };
 
 // ---- component entry points (each in its own translation unit) -------------
 int csr_to_mbsr(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *rowptr,

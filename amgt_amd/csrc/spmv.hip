@@ -118,6 +118,113 @@ __global__ __launch_bounds__(256) void k_spmv(
     }
 }
 
+// Load-level-parallel variant: each lane takes U blocks per step and issues
+// all their bitmap/column loads, then all their x-segment loads (two 16-byte
+// loads per block) and value loads, before the first FMA.  The value of bit c
+// of row nibble `nib` sits at  o + popc(nib & ((1 << c) - 1)).  FMAs are
+// predicated on the bit (a zero value never meets an unused x entry), in the
+// same block/column order as the other kernels.  x must be 16-byte aligned.
+template <int G, int U, typename VT, typename AT>
+__global__ __launch_bounds__(256) void k_spmv_ilp(
+    int32_t brow, int32_t nrows, int32_t ncols, const int32_t *__restrict__ blk_ptr,
+    const int32_t *__restrict__ blk_col, const uint16_t *__restrict__ blk_map,
+    const int32_t *__restrict__ val_ptr, const VT *__restrict__ val, const double *__restrict__ x,
+    double alpha, double beta, const double *b, double *y) {
+    constexpr int S = G / 4;
+    constexpr int RPW = 64 / G;
+    const int lane = threadIdx.x & 63;
+    const int wg = remap_wg(blockIdx.x, gridDim.x);
+    const int wave = wg * 4 + (threadIdx.x >> 6);
+    const int br = wave * RPW + lane / G;
+    const int sub = lane % G;
+    const int r = sub & 3;
+    const int s = sub >> 2;
+    if (br >= brow) return;
+    const int p0 = blk_ptr[br];
+    const int p1 = blk_ptr[br + 1];
+    int voff = val_ptr[br];
+    const uint32_t low = (1u << (4 * r)) - 1u;
+    AT acc = AT(0);
+    for (int j0 = p0; j0 < p1; j0 += S * U) {
+        uint32_t m[U];
+        int c4[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = j0 + s + S * u;
+            m[u] = 0u;
+            c4[u] = 0;
+            if (j < p1) {
+                m[u] = blk_map[j];
+                c4[u] = 4 * blk_col[j];
+            }
+        }
+        double xs[U][4];
+        int o[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t nib = (m[u] >> (4 * r)) & 0xFu;
+            xs[u][0] = xs[u][1] = xs[u][2] = xs[u][3] = 0.0;
+            if (nib) {
+                const double *xp = x + c4[u];
+                if (c4[u] + 4 <= ncols) {
+                    const double2 lo2 = *reinterpret_cast<const double2 *>(xp);
+                    const double2 hi2 = *reinterpret_cast<const double2 *>(xp + 2);
+                    xs[u][0] = lo2.x;
+                    xs[u][1] = lo2.y;
+                    xs[u][2] = hi2.x;
+                    xs[u][3] = hi2.y;
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        if (c4[u] + c < ncols) xs[u][c] = xp[c];
+                }
+            }
+            const int cnt = __popc(m[u]);
+            int incl = cnt;
+            int tot = cnt;
+            if constexpr (S > 1) {
+#pragma unroll
+                for (int d = 1; d < S; d <<= 1) {
+                    int t = __shfl_up(incl, 4 * d, G);
+                    if (s >= d) incl += t;
+                }
+                tot = __shfl(incl, 4 * (S - 1) + r, G);
+            }
+            o[u] = voff + (incl - cnt) + __popc(m[u] & low);
+            voff += tot;
+        }
+        VT v[U][4];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t nib = (m[u] >> (4 * r)) & 0xFu;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                v[u][c] = VT(0);
+                if ((nib >> c) & 1u) v[u][c] = val[o[u] + __popc(nib & ((1u << c) - 1u))];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t nib = (m[u] >> (4 * r)) & 0xFu;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                if ((nib >> c) & 1u) acc = mul_acc<VT, AT>(acc, v[u][c], xs[u][c]);
+        }
+    }
+    if constexpr (S > 1) {
+#pragma unroll
+        for (int d = S / 2; d >= 1; d >>= 1) acc += __shfl_xor(acc, 4 * d, G);
+    }
+    if (s == 0) {
+        const int row = 4 * br + r;
+        if (row < nrows) {
+            double v = alpha * double(acc);
+            if (beta != 0.0) v = fma(beta, b[row], v);
+            y[row] = v;
+        }
+    }
+}
+
 // Staged variant: a workgroup owns 256/G consecutive block rows.  Their block
 // columns, bitmaps and values (contiguous ranges) are first copied to LDS with
 // coalesced loads that depend only on the two pointer pairs, so every matrix
@@ -458,12 +565,45 @@ int launch_spmv_staged_g(const Mbsr &A, const VT *val, double alpha, const doubl
     return AMGT_OK;
 }
 
+template <int G, int U, typename VT, typename AT>
+int launch_spmv_ilp_g(const Mbsr &A, const VT *val, double alpha, const double *x, double beta,
+                      const double *b, double *y, hipStream_t s) {
+    constexpr int RPW = 64 / G;
+    const int64_t waves = (int64_t(A.brow) + RPW - 1) / RPW;
+    const int64_t grid = (waves + 3) / 4;
+    hipLaunchKernelGGL((k_spmv_ilp<G, U, VT, AT>), dim3(unsigned(grid)), dim3(256), 0, s, A.brow,
+                       A.nrows, A.ncols, A.blk_ptr, A.blk_col, A.blk_map, A.val_ptr, val, x,
+                       alpha, beta, b, y);
+    AMGT_HIP(hipGetLastError());
+    return AMGT_OK;
+}
+
+template <int U, typename VT, typename AT>
+int launch_spmv_ilp(const Mbsr &A, const VT *val, int lanes, double alpha, const double *x,
+                    double beta, const double *b, double *y, hipStream_t s) {
+    switch (lanes) {
+        case 4: return launch_spmv_ilp_g<4, U, VT, AT>(A, val, alpha, x, beta, b, y, s);
+        case 8: return launch_spmv_ilp_g<8, U, VT, AT>(A, val, alpha, x, beta, b, y, s);
+        case 16: return launch_spmv_ilp_g<16, U, VT, AT>(A, val, alpha, x, beta, b, y, s);
+        case 32: return launch_spmv_ilp_g<32, U, VT, AT>(A, val, alpha, x, beta, b, y, s);
+        default: return launch_spmv_ilp_g<64, U, VT, AT>(A, val, alpha, x, beta, b, y, s);
+    }
+}
+
 template <typename VT, typename AT>
 int launch_spmv_t(const Mbsr &A, const VT *val, int lanes, double alpha, const double *x,
                   double beta, const double *b, double *y, hipStream_t s) {
-    // the staged walk reads x in 16-byte pieces
+    if (lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64) {
+        set_error("lanes per block row must be 4, 8, 16, 32 or 64");
+        return AMGT_ERR_ARG;
+    }
+    // the staged and ILP walks read x in 16-byte pieces; otherwise the direct kernel
     const bool x_aligned = (reinterpret_cast<uintptr_t>(x) & 15u) == 0;
-    if (A.spmv_staged && x_aligned) {
+    if (A.spmv_kernel == kSpmvIlp2 && x_aligned)
+        return launch_spmv_ilp<2, VT, AT>(A, val, lanes, alpha, x, beta, b, y, s);
+    if (A.spmv_kernel == kSpmvIlp4 && x_aligned)
+        return launch_spmv_ilp<4, VT, AT>(A, val, lanes, alpha, x, beta, b, y, s);
+    if (A.spmv_kernel == kSpmvStaged && x_aligned) {
         switch (lanes) {
             case 4: return launch_spmv_staged_g<4, VT, AT>(A, val, alpha, x, beta, b, y, s);
             case 8: return launch_spmv_staged_g<8, VT, AT>(A, val, alpha, x, beta, b, y, s);

@@ -126,9 +126,10 @@ class MBSRMatrix:
     def set_spmv_lanes(self, lanes: int):
         L.check(L.load().amgt_mbsr_set_spmv_lanes(self.handle, lanes), "amgt_mbsr_set_spmv_lanes")
 
-    def set_spmv_staged(self, staged: bool):
-        L.check(L.load().amgt_mbsr_set_spmv_staged(self.handle, int(bool(staged))),
-                "amgt_mbsr_set_spmv_staged")
+    def set_spmv_kernel(self, kind: int):
+        """AMGT_SPMV_DIRECT / _STAGED / _ILP2 / _ILP4."""
+        L.check(L.load().amgt_mbsr_set_spmv_kernel(self.handle, int(kind)),
+                "amgt_mbsr_set_spmv_kernel")
 
     def autotune(self, stream=None):
         L.check(L.load().amgt_mbsr_autotune(self.handle, _stream_ptr(stream)), "amgt_mbsr_autotune")

@@ -180,14 +180,15 @@ def main():
         torch.cuda.synchronize()
         inf = M.info()
         print(json.dumps({"kernel_only": True, "spmv_lanes": inf["spmv_lanes"],
-                          "spmv_staged": inf["spmv_staged"], "spmv_alg_bytes": spmv_bytes(inf),
+                          "spmv_kernel": inf["spmv_kernel"], "spmv_alg_bytes": spmv_bytes(inf),
                           "copy_bytes_each_way": nel * 8}), flush=True)
         return
 
     # per-variant timings (staged/direct x lanes per block row), then the tuner's pick
     variants = {}
-    for staged in (1, 0):
-        M.set_spmv_staged(staged)
+    from amgt_amd._lib import SPMV_KERNEL_NAMES
+    for kind in (3, 2, 1, 0):
+        M.set_spmv_kernel(kind)
         for lanes in (4, 8, 16, 32, 64):
             M.set_spmv_lanes(lanes)
             for _ in range(10):
@@ -198,15 +199,15 @@ def main():
                 M.spmv(x, y)
             e1.record()
             torch.cuda.synchronize()
-            variants[f"{'staged' if staged else 'direct'}_{lanes}"] = e0.elapsed_time(e1) / 50
+            variants[f"{SPMV_KERNEL_NAMES[kind]}_{lanes}"] = e0.elapsed_time(e1) / 50
     if args.lanes:
-        M.set_spmv_staged(1)
+        M.set_spmv_kernel(3)
         M.set_spmv_lanes(args.lanes)
     else:
         M.set_spmv_lanes(0)
         M.autotune()
     lanes = M.info()["spmv_lanes"]
-    staged = M.info()["spmv_staged"]
+    kind = M.info()["spmv_kernel"]
 
     for _ in range(args.warmup):
         M.spmv(x, y)
@@ -262,7 +263,7 @@ def main():
         "spmv_alg_bytes": int(B),
         "kernel_ms": round(kern_ms, 5),
         "residual_form_ms": round(resid_ms, 5),
-        "spmv_kernel": "staged" if staged else "direct",
+        "spmv_kernel": SPMV_KERNEL_NAMES[kind],
         "variant_ms": {str(k): round(v, 5) for k, v in variants.items()},
         "conversion_ms": round(conv_ms, 2),
         "input_gen_s": round(gen_s, 2),

@@ -53,7 +53,7 @@ typedef struct {
     int32_t level;      /* coarsening level tag (hypre_Level, seq_mv.h:199) */
     int32_t precision;  /* value precision used by SpMV (AMGT_DOUBLE/FLOAT/HALF) */
     int32_t spmv_lanes; /* lanes per block row chosen by the tuner (4..64) */
-    int32_t spmv_staged; /* 1: LDS-staged tile kernel, 0: direct kernel */
+    int32_t spmv_kernel; /* AMGT_SPMV_DIRECT / _STAGED / _ILP2 / _ILP4 */
 } amgt_mbsr_info_t;
 
 typedef struct {
@@ -98,9 +98,14 @@ int amgt_mbsr_set_precision(amgt_mbsr_t A, int precision);
 /* lanes per block row for SpMV: 4, 8, 16, 32 or 64; 0 = choose from the
  * matrix statistics */
 int amgt_mbsr_set_spmv_lanes(amgt_mbsr_t A, int lanes);
-/* 1 (default): a workgroup copies its tile's block columns, bitmaps and values
- * to LDS before the walk; 0: the walk reads global memory directly */
-int amgt_mbsr_set_spmv_staged(amgt_mbsr_t A, int staged);
+/* SpMV kernel kind:
+ *   AMGT_SPMV_DIRECT  the walk reads bitmaps, columns and values from global memory
+ *   AMGT_SPMV_STAGED  a workgroup first copies its tile of block rows to LDS
+ *   AMGT_SPMV_ILP2/4  each lane issues the loads of 2 / 4 blocks before using them
+ * The staged and ILP kernels need a 16-byte aligned x; otherwise the direct
+ * kernel runs. */
+enum { AMGT_SPMV_DIRECT = 0, AMGT_SPMV_STAGED = 1, AMGT_SPMV_ILP2 = 2, AMGT_SPMV_ILP4 = 3 };
+int amgt_mbsr_set_spmv_kernel(amgt_mbsr_t A, int kind);
 /* times every lanes-per-block-row variant on this matrix and keeps the fastest */
 int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream);
 

@@ -108,14 +108,14 @@ def row_scale(m, n, I, J, A, x, beta, b):
 @pytest.mark.parametrize("name", ["c1_5pt_128", "7pt_aniso", "27pt_10", "rand_rect",
                                   "rand_tall_empty", "rand_wide", "bin1_rows", "bin2_rows"])
 @pytest.mark.parametrize("lanes", [4, 8, 16, 32, 64])
-@pytest.mark.parametrize("staged", [1, 0])
-def test_spmv_fp64(dev, name, lanes, staged):
-    print(f"[spmv] {name} lanes={lanes} staged={staged}", flush=True)
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_spmv_fp64(dev, name, lanes, kind):
+    print(f"[spmv] {name} lanes={lanes} kernel={kind}", flush=True)
     (m, n), I, J, A = CASES[name]()
     ref = O.csr2mbsr(m, n, I, J, A)
     M = make(dev, m, n, I, J, A)
     M.set_spmv_lanes(lanes)
-    M.set_spmv_staged(staged)
+    M.set_spmv_kernel(kind)
     rng = np.random.default_rng(lanes)
     x, b = rng.standard_normal(n), rng.standard_normal(m)
     for alpha, beta in [(1.0, 0.0), (-1.0, 1.0), (0.5, -2.0)]:
