@@ -23,7 +23,8 @@ AMGT_SPMV_DIRECT = 0
 AMGT_SPMV_STAGED = 1
 AMGT_SPMV_ILP2 = 2
 AMGT_SPMV_ILP4 = 3
-SPMV_KERNEL_NAMES = {0: "direct", 1: "staged", 2: "ilp2", 3: "ilp4"}
+AMGT_SPMV_TILE = 4
+SPMV_KERNEL_NAMES = {0: "direct", 1: "staged", 2: "ilp2", 3: "ilp4", 4: "tile"}
 
 AMGT_DOUBLE = 0
 AMGT_FLOAT = 1

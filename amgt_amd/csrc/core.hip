@@ -240,7 +240,7 @@ int amgt_mbsr_get_info(amgt_mbsr_t A, amgt_mbsr_info_t *info) {
 
 int amgt_mbsr_set_spmv_kernel(amgt_mbsr_t A, int kind) {
     AMGT_CHECK_ARG(A != nullptr, "handle is NULL");
-    AMGT_CHECK_ARG(kind >= AMGT_SPMV_DIRECT && kind <= AMGT_SPMV_ILP4, "unknown SpMV kernel kind");
+    AMGT_CHECK_ARG(kind >= AMGT_SPMV_DIRECT && kind <= AMGT_SPMV_TILE, "unknown SpMV kernel kind");
     A->m.spmv_kernel = kind;
     return AMGT_OK;
 }
@@ -294,7 +294,7 @@ int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream) {
     const int cand[5] = {4, 8, 16, 32, 64};
     float best = 1e30f;
     int best_lanes = spmv_default_lanes(m), best_kind = m.spmv_kernel;
-    for (int kind = AMGT_SPMV_ILP4; kind >= AMGT_SPMV_DIRECT; --kind) {
+    for (int kind = AMGT_SPMV_TILE; kind >= AMGT_SPMV_DIRECT; --kind) {
         m.spmv_kernel = kind;
         for (int c = 0; c < 5; ++c) {
             int st = spmv(m, prec, cand[c], 1.0, x, 0.0, nullptr, y, s);  // warm-up

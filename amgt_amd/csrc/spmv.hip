@@ -354,6 +354,169 @@ __device__ __forceinline__ AT row_walk_prefetch(const int32_t *s_col, const uint
     return acc;
 }
 
+// Tile variant (the CSR-adaptive pattern applied to mBSR).  A workgroup owns
+// 256/G block rows whose values, bitmaps and block columns fit LDS:
+//   1. coalesced, unrolled copy of the tile's values, bitmaps and columns;
+//   2. decode: lane (slot, row) walks its nibbles and writes, at each value's
+//      position, that value's x index 4*col + c;
+//   3. multiply: lane per value, x[x index] * value in place (the loads of a
+//      thread are independent and issued together);
+//   4. reduce: the same walk sums the products of its nibbles.
+// Every matrix byte arrives through coalesced loads and each x entry is
+// gathered once per nonzero.  The summation order per row (blocks ascending,
+// columns ascending) matches the other kernels; products are rounded before
+// the sum instead of fused.  Tiles above the LDS capacity use the direct walk.
+template <int G, typename F>
+__device__ __forceinline__ void nibble_walk(const int32_t *s_col, const uint16_t *s_map, int p0,
+                                            int p1, int voff, int s, int r, F f) {
+    constexpr int S = G / 4;
+    const uint32_t low = (1u << (4 * r)) - 1u;
+    for (int jb = p0; jb < p1; jb += S) {
+        const int j = jb + s;
+        uint32_t m = 0;
+        int c4 = 0;
+        if (j < p1) {
+            m = s_map[j];
+            c4 = 4 * s_col[j];
+        }
+        const int cnt = __popc(m);
+        int incl = cnt;
+        int tot = cnt;
+        if constexpr (S > 1) {
+#pragma unroll
+            for (int d = 1; d < S; d <<= 1) {
+                int t = __shfl_up(incl, 4 * d, G);
+                if (s >= d) incl += t;
+            }
+            tot = __shfl(incl, 4 * (S - 1) + r, G);
+        }
+        int o = voff + (incl - cnt) + __popc(m & low);
+        const uint32_t nib = (m >> (4 * r)) & 0xFu;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            if ((nib >> c) & 1u) {
+                f(o, c4 + c);
+                ++o;
+            }
+        }
+        voff += tot;
+    }
+}
+
+template <int G, typename VT, typename AT, int CAPV, int CAPB>
+__global__ __launch_bounds__(256) void k_spmv_tile(
+    int32_t brow, int32_t nrows, const int32_t *__restrict__ blk_ptr,
+    const int32_t *__restrict__ blk_col, const uint16_t *__restrict__ blk_map,
+    const int32_t *__restrict__ val_ptr, const VT *__restrict__ val, const double *__restrict__ x,
+    double alpha, double beta, const double *b, double *y) {
+    constexpr int RPW = 64 / G;
+    constexpr int RPB = 256 / G;
+    constexpr int S = G / 4;
+    __shared__ AT s_prod[CAPV];
+    __shared__ int32_t s_xi[CAPV];
+    __shared__ int32_t s_col[CAPB];
+    __shared__ uint16_t s_map[CAPB];
+    __shared__ int32_t s_bp[RPB + 1], s_vp[RPB + 1];
+    const int wg = remap_wg(blockIdx.x, gridDim.x);
+    const int br0 = wg * RPB;
+    const int nr = min(RPB, brow - br0);
+    for (int t = threadIdx.x; t <= nr; t += 256) {
+        s_bp[t] = blk_ptr[br0 + t];
+        s_vp[t] = val_ptr[br0 + t];
+    }
+    __syncthreads();
+    const int pb = s_bp[0], pe = s_bp[nr];
+    const int vb = s_vp[0], ve = s_vp[nr];
+    const int nv = ve - vb, nb = pe - pb;
+    const bool staged = (nb <= CAPB) && (nv <= CAPV);
+    const int lane = threadIdx.x & 63;
+    const int lr = (threadIdx.x >> 6) * RPW + lane / G;
+    const int sub = lane % G;
+    const int r = sub & 3;
+    const int s = sub >> 2;
+    const bool active = lr < nr;
+    if (!staged) {
+        if (!active) return;
+        const AT acc = row_walk<G, VT, AT>(blk_col, blk_map, val, s_bp[lr], s_bp[lr + 1],
+                                           s_vp[lr], s, r, x);
+        if (s == 0) {
+            const int row = 4 * (br0 + lr) + r;
+            if (row < nrows) {
+                double v = alpha * double(acc);
+                if (beta != 0.0) v = fma(beta, b[row], v);
+                y[row] = v;
+            }
+        }
+        return;
+    }
+    constexpr int VPT = CAPV / 256;
+    constexpr int BPT = CAPB / 256;
+    {
+        VT tv[VPT];
+        int32_t tc[BPT];
+        uint16_t tm[BPT];
+#pragma unroll
+        for (int u = 0; u < VPT; ++u) {
+            const int t = u * 256 + threadIdx.x;
+            if (t < nv) tv[u] = stream_load(val + vb + t);
+        }
+#pragma unroll
+        for (int u = 0; u < BPT; ++u) {
+            const int t = u * 256 + threadIdx.x;
+            if (t < nb) {
+                tc[u] = stream_load(blk_col + pb + t);
+                tm[u] = stream_load(blk_map + pb + t);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < VPT; ++u) {
+            const int t = u * 256 + threadIdx.x;
+            if (t < nv) s_prod[t] = AT(tv[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < BPT; ++u) {
+            const int t = u * 256 + threadIdx.x;
+            if (t < nb) {
+                s_col[t] = tc[u];
+                s_map[t] = tm[u];
+            }
+        }
+    }
+    __syncthreads();
+    const int p0 = active ? s_bp[lr] - pb : 0, p1 = active ? s_bp[lr + 1] - pb : 0;
+    const int v0 = active ? s_vp[lr] - vb : 0;
+    nibble_walk<G>(s_col, s_map, p0, p1, v0, s, r, [&](int o, int xc) { s_xi[o] = xc; });
+    __syncthreads();
+    {
+        double xv[VPT];
+#pragma unroll
+        for (int u = 0; u < VPT; ++u) {
+            const int t = u * 256 + threadIdx.x;
+            if (t < nv) xv[u] = x[s_xi[t]];
+        }
+#pragma unroll
+        for (int u = 0; u < VPT; ++u) {
+            const int t = u * 256 + threadIdx.x;
+            if (t < nv) s_prod[t] = AT(s_prod[t] * AT(xv[u]));
+        }
+    }
+    __syncthreads();
+    AT acc = AT(0);
+    nibble_walk<G>(s_col, s_map, p0, p1, v0, s, r, [&](int o, int) { acc += s_prod[o]; });
+    if constexpr (S > 1) {
+#pragma unroll
+        for (int d = S / 2; d >= 1; d >>= 1) acc += __shfl_xor(acc, 4 * d, G);
+    }
+    if (active && s == 0) {
+        const int row = 4 * (br0 + lr) + r;
+        if (row < nrows) {
+            double v = alpha * double(acc);
+            if (beta != 0.0) v = fma(beta, b[row], v);
+            y[row] = v;
+        }
+    }
+}
+
 template <int G, typename VT, typename AT, int CAPV, int CAPB>
 __global__ __launch_bounds__(256) void k_spmv_staged(
     int32_t brow, int32_t nrows, int32_t ncols, const int32_t *__restrict__ blk_ptr,
@@ -565,6 +728,18 @@ int launch_spmv_staged_g(const Mbsr &A, const VT *val, double alpha, const doubl
     return AMGT_OK;
 }
 
+template <int G, typename VT, typename AT>
+int launch_spmv_tile_g(const Mbsr &A, const VT *val, double alpha, const double *x, double beta,
+                       const double *b, double *y, hipStream_t s) {
+    constexpr int RPB = 256 / G;
+    const int64_t grid = (int64_t(A.brow) + RPB - 1) / RPB;
+    hipLaunchKernelGGL((k_spmv_tile<G, VT, AT, kStageVals, kStageBlocks>), dim3(unsigned(grid)),
+                       dim3(256), 0, s, A.brow, A.nrows, A.blk_ptr, A.blk_col, A.blk_map,
+                       A.val_ptr, val, x, alpha, beta, b, y);
+    AMGT_HIP(hipGetLastError());
+    return AMGT_OK;
+}
+
 template <int G, int U, typename VT, typename AT>
 int launch_spmv_ilp_g(const Mbsr &A, const VT *val, double alpha, const double *x, double beta,
                       const double *b, double *y, hipStream_t s) {
@@ -603,6 +778,15 @@ int launch_spmv_t(const Mbsr &A, const VT *val, int lanes, double alpha, const d
         return launch_spmv_ilp<2, VT, AT>(A, val, lanes, alpha, x, beta, b, y, s);
     if (A.spmv_kernel == kSpmvIlp4 && x_aligned)
         return launch_spmv_ilp<4, VT, AT>(A, val, lanes, alpha, x, beta, b, y, s);
+    if (A.spmv_kernel == kSpmvTile) {
+        switch (lanes) {
+            case 4: return launch_spmv_tile_g<4, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            case 8: return launch_spmv_tile_g<8, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            case 16: return launch_spmv_tile_g<16, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            case 32: return launch_spmv_tile_g<32, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            default: return launch_spmv_tile_g<64, VT, AT>(A, val, alpha, x, beta, b, y, s);
+        }
+    }
     if (A.spmv_kernel == kSpmvStaged && x_aligned) {
         switch (lanes) {
             case 4: return launch_spmv_staged_g<4, VT, AT>(A, val, alpha, x, beta, b, y, s);

@@ -102,9 +102,12 @@ int amgt_mbsr_set_spmv_lanes(amgt_mbsr_t A, int lanes);
  *   AMGT_SPMV_DIRECT  the walk reads bitmaps, columns and values from global memory
  *   AMGT_SPMV_STAGED  a workgroup first copies its tile of block rows to LDS
  *   AMGT_SPMV_ILP2/4  each lane issues the loads of 2 / 4 blocks before using them
+ *   AMGT_SPMV_TILE    tile staged in LDS, bitmaps decoded to per-value x indices,
+ *                     one independent x gather per nonzero, LDS row reduction
  * The staged and ILP kernels need a 16-byte aligned x; otherwise the direct
  * kernel runs. */
-enum { AMGT_SPMV_DIRECT = 0, AMGT_SPMV_STAGED = 1, AMGT_SPMV_ILP2 = 2, AMGT_SPMV_ILP4 = 3 };
+enum { AMGT_SPMV_DIRECT = 0, AMGT_SPMV_STAGED = 1, AMGT_SPMV_ILP2 = 2, AMGT_SPMV_ILP4 = 3,
+       AMGT_SPMV_TILE = 4 };
 int amgt_mbsr_set_spmv_kernel(amgt_mbsr_t A, int kind);
 /* times every lanes-per-block-row variant on this matrix and keeps the fastest */
 int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream);
