@@ -225,6 +225,7 @@ __global__ __launch_bounds__(256) void k_spmv_ilp(
     }
 }
 
+
 // Staged variant: a workgroup owns 256/G consecutive block rows.  Their block
 // columns, bitmaps and values (contiguous ranges) are first copied to LDS with
 // coalesced loads that depend only on the two pointer pairs, so every matrix
@@ -233,7 +234,7 @@ __global__ __launch_bounds__(256) void k_spmv_ilp(
 // global memory directly (same arithmetic).
 template <typename T>
 __device__ __forceinline__ T stream_load(const T *p) {
-    return __builtin_nontemporal_load(p);
+    return *p;  // plain load (non-temporal loads measured slower here)
 }
 template <>
 __device__ __forceinline__ __half stream_load<__half>(const __half *p) {
@@ -289,7 +290,7 @@ __device__ __forceinline__ AT row_walk(const int32_t *__restrict__ col, const ui
 // straight-line predicated FMAs, so the x loads of a lane are all in flight
 // together.  x must be 16-byte aligned (checked by the launcher); the last
 // block column is read element-wise when ncols is not a multiple of 4.
-template <int G, int MAXB, typename VT, typename AT>
+template <int G, int MAXB, bool XSEG, typename VT, typename AT>
 __device__ __forceinline__ AT row_walk_prefetch(const int32_t *s_col, const uint16_t *s_map,
                                                 const VT *s_val, int p0, int p1, int voff, int s,
                                                 int r, const double *__restrict__ x, int ncols) {
@@ -308,7 +309,13 @@ __device__ __forceinline__ AT row_walk_prefetch(const int32_t *s_col, const uint
                 m[u] = s_map[j];
                 const int c4 = 4 * s_col[j];
                 const double *xp = x + c4;
-                if (c4 + 4 <= ncols) {
+                if constexpr (!XSEG) {
+                    // gather only the columns of this lane's row nibble
+                    const uint32_t nib = (m[u] >> (4 * r)) & 0xFu;
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        if ((nib >> c) & 1u) xs[u][c] = xp[c];
+                } else if (c4 + 4 <= ncols) {
                     const double2 lo2 = *reinterpret_cast<const double2 *>(xp);
                     const double2 hi2 = *reinterpret_cast<const double2 *>(xp + 2);
                     xs[u][0] = lo2.x;
@@ -517,7 +524,125 @@ __global__ __launch_bounds__(256) void k_spmv_tile(
     }
 }
 
-template <int G, typename VT, typename AT, int CAPV, int CAPB>
+// Pipelined tile variant: persistent workgroups, each owning a contiguous
+// range of tiles (256/G block rows each).  While the current tile is walked
+// from LDS, the next tile's block-row pointers, block columns, bitmaps and
+// values are already in flight into registers, so HBM streaming overlaps the
+// x gathers and FMAs.  Same walk and arithmetic as the staged kernel.
+template <int G, typename VT, typename AT, int CAPV, int CAPB, int MAXB>
+__global__ __launch_bounds__(256) void k_spmv_pipe(
+    int32_t brow, int32_t nrows, int32_t ncols, const int32_t *__restrict__ blk_ptr,
+    const int32_t *__restrict__ blk_col, const uint16_t *__restrict__ blk_map,
+    const int32_t *__restrict__ val_ptr, const VT *__restrict__ val, const double *__restrict__ x,
+    double alpha, double beta, const double *b, double *y, int ntiles, int tiles_per_wg) {
+    constexpr int RPW = 64 / G;
+    constexpr int RPB = 256 / G;
+    constexpr int VPT = CAPV / 256;
+    constexpr int BPT = CAPB / 256;
+    __shared__ VT s_val[CAPV];
+    __shared__ int32_t s_col[CAPB];
+    __shared__ uint16_t s_map[CAPB];
+    __shared__ int32_t s_bp[RPB + 1], s_vp[RPB + 1];
+    const int tid = threadIdx.x;
+    const int w = remap_wg(blockIdx.x, gridDim.x);
+    const int t_beg = w * tiles_per_wg;
+    const int t_end = min(t_beg + tiles_per_wg, ntiles);
+    if (t_beg >= t_end) return;  // uniform across the workgroup
+    const int lane = tid & 63;
+    const int lr = (tid >> 6) * RPW + lane / G;
+    const int sub = lane % G;
+    const int r = sub & 3;
+    const int s = sub >> 2;
+
+    // registers holding the next tile
+    VT tv[VPT];
+    int32_t tc[BPT];
+    uint16_t tm[BPT];
+    int32_t tbp = 0, tvp = 0;
+    int pb = 0, pe = 0, vb = 0, ve = 0;
+    bool staged = false;
+    auto issue = [&](int t) {
+        const int br0 = t * RPB;
+        const int nr = min(RPB, brow - br0);
+        pb = blk_ptr[br0];
+        pe = blk_ptr[br0 + nr];
+        vb = val_ptr[br0];
+        ve = val_ptr[br0 + nr];
+        staged = (pe - pb <= CAPB) && (ve - vb <= CAPV);
+        if (tid <= nr) {
+            tbp = blk_ptr[br0 + tid];
+            tvp = val_ptr[br0 + tid];
+        }
+        if (staged) {
+            const int nv = ve - vb, nb = pe - pb;
+#pragma unroll
+            for (int u = 0; u < VPT; ++u) {
+                const int q = u * 256 + tid;
+                if (q < nv) tv[u] = val[vb + q];
+            }
+#pragma unroll
+            for (int u = 0; u < BPT; ++u) {
+                const int q = u * 256 + tid;
+                if (q < nb) {
+                    tc[u] = blk_col[pb + q];
+                    tm[u] = blk_map[pb + q];
+                }
+            }
+        }
+    };
+    issue(t_beg);
+    for (int t = t_beg; t < t_end; ++t) {
+        const int br0 = t * RPB;
+        const int nr = min(RPB, brow - br0);
+        const bool cur_staged = staged;
+        const int cpb = pb, cvb = vb;
+        // registers -> LDS for tile t
+        if (tid <= nr) {
+            s_bp[tid] = tbp;
+            s_vp[tid] = tvp;
+        }
+        if (cur_staged) {
+            const int nv = ve - vb, nb = pe - pb;
+#pragma unroll
+            for (int u = 0; u < VPT; ++u) {
+                const int q = u * 256 + tid;
+                if (q < nv) s_val[q] = tv[u];
+            }
+#pragma unroll
+            for (int u = 0; u < BPT; ++u) {
+                const int q = u * 256 + tid;
+                if (q < nb) {
+                    s_col[q] = tc[u];
+                    s_map[q] = tm[u];
+                }
+            }
+        }
+        __syncthreads();
+        if (t + 1 < t_end) issue(t + 1);  // next tile's loads fly during this walk
+        if (lr < nr) {
+            AT acc;
+            if (cur_staged) {
+                acc = row_walk_prefetch<G, MAXB, false, VT, AT>(
+                    s_col, s_map, s_val, s_bp[lr] - cpb, s_bp[lr + 1] - cpb, s_vp[lr] - cvb, s, r,
+                    x, ncols);
+            } else {
+                acc = row_walk<G, VT, AT>(blk_col, blk_map, val, s_bp[lr], s_bp[lr + 1], s_vp[lr],
+                                          s, r, x);
+            }
+            if (s == 0) {
+                const int row = 4 * (br0 + lr) + r;
+                if (row < nrows) {
+                    double v = alpha * double(acc);
+                    if (beta != 0.0) v = fma(beta, b[row], v);
+                    y[row] = v;
+                }
+            }
+        }
+        __syncthreads();  // LDS is rewritten for the next tile
+    }
+}
+
+template <int G, typename VT, typename AT, int CAPV, int CAPB, int MAXB, bool XSEG>
 __global__ __launch_bounds__(256) void k_spmv_staged(
     int32_t brow, int32_t nrows, int32_t ncols, const int32_t *__restrict__ blk_ptr,
     const int32_t *__restrict__ blk_col, const uint16_t *__restrict__ blk_map,
@@ -584,7 +709,7 @@ __global__ __launch_bounds__(256) void k_spmv_staged(
     const int s = sub >> 2;
     AT acc;
     if (staged) {
-        acc = row_walk_prefetch<G, 8, VT, AT>(s_col, s_map, s_val, s_bp[lr] - pb,
+        acc = row_walk_prefetch<G, MAXB, XSEG, VT, AT>(s_col, s_map, s_val, s_bp[lr] - pb,
                                               s_bp[lr + 1] - pb, s_vp[lr] - vb, s, r, x, ncols);
     } else {
         acc = row_walk<G, VT, AT>(blk_col, blk_map, val, s_bp[lr], s_bp[lr + 1], s_vp[lr], s, r, x);
@@ -716,14 +841,42 @@ int launch_spmv_g(const Mbsr &A, const VT *val, double alpha, const double *x, d
 constexpr int kStageVals = 2048;
 constexpr int kStageBlocks = 1024;
 
-template <int G, typename VT, typename AT>
+template <int G, typename VT, typename AT, int MAXB = 8, bool XSEG = true>
 int launch_spmv_staged_g(const Mbsr &A, const VT *val, double alpha, const double *x, double beta,
                          const double *b, double *y, hipStream_t s) {
     constexpr int RPB = 256 / G;
     const int64_t grid = (int64_t(A.brow) + RPB - 1) / RPB;
-    hipLaunchKernelGGL((k_spmv_staged<G, VT, AT, kStageVals, kStageBlocks>), dim3(unsigned(grid)),
+    hipLaunchKernelGGL((k_spmv_staged<G, VT, AT, kStageVals, kStageBlocks, MAXB, XSEG>),
+                       dim3(unsigned(grid)),
                        dim3(256), 0, s, A.brow, A.nrows, A.ncols, A.blk_ptr, A.blk_col,
                        A.blk_map, A.val_ptr, val, x, alpha, beta, b, y);
+    AMGT_HIP(hipGetLastError());
+    return AMGT_OK;
+}
+
+int cu_count() {
+    static int n = 0;
+    if (n == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            n <= 0)
+            n = 256;
+    }
+    return n;
+}
+
+template <int G, typename VT, typename AT>
+int launch_spmv_pipe_g(const Mbsr &A, const VT *val, double alpha, const double *x, double beta,
+                       const double *b, double *y, hipStream_t s) {
+    constexpr int RPB = 256 / G;
+    const int ntiles = int((int64_t(A.brow) + RPB - 1) / RPB);
+    const int max_wg = cu_count() * 6;
+    const int tpw = (ntiles + max_wg - 1) / max_wg;
+    const int nwg = (ntiles + tpw - 1) / tpw;
+    hipLaunchKernelGGL((k_spmv_pipe<G, VT, AT, kStageVals, kStageBlocks, 4>), dim3(unsigned(nwg)),
+                       dim3(256), 0, s, A.brow, A.nrows, A.ncols, A.blk_ptr, A.blk_col, A.blk_map,
+                       A.val_ptr, val, x, alpha, beta, b, y, ntiles, tpw);
     AMGT_HIP(hipGetLastError());
     return AMGT_OK;
 }
@@ -778,6 +931,31 @@ int launch_spmv_t(const Mbsr &A, const VT *val, int lanes, double alpha, const d
         return launch_spmv_ilp<2, VT, AT>(A, val, lanes, alpha, x, beta, b, y, s);
     if (A.spmv_kernel == kSpmvIlp4 && x_aligned)
         return launch_spmv_ilp<4, VT, AT>(A, val, lanes, alpha, x, beta, b, y, s);
+    if (A.spmv_kernel >= kSpmvStagedX0 && A.spmv_kernel <= kSpmvStagedX3) {
+        // staged kernel with per-bit x gathers (no alignment need), MAXB = 8/4/2
+        // and a 4-block segment variant
+        const int v = A.spmv_kernel - kSpmvStagedX0;
+        if (lanes == 4) {
+            if (v == 0) return launch_spmv_staged_g<4, VT, AT, 8, false>(A, val, alpha, x, beta, b, y, s);
+            if (v == 1) return launch_spmv_staged_g<4, VT, AT, 4, false>(A, val, alpha, x, beta, b, y, s);
+            if (v == 2) return launch_spmv_staged_g<4, VT, AT, 2, false>(A, val, alpha, x, beta, b, y, s);
+            if (x_aligned) return launch_spmv_staged_g<4, VT, AT, 4, true>(A, val, alpha, x, beta, b, y, s);
+        } else if (lanes == 8) {
+            if (v == 0) return launch_spmv_staged_g<8, VT, AT, 8, false>(A, val, alpha, x, beta, b, y, s);
+            if (v == 1) return launch_spmv_staged_g<8, VT, AT, 4, false>(A, val, alpha, x, beta, b, y, s);
+            if (v == 2) return launch_spmv_staged_g<8, VT, AT, 2, false>(A, val, alpha, x, beta, b, y, s);
+            if (x_aligned) return launch_spmv_staged_g<8, VT, AT, 4, true>(A, val, alpha, x, beta, b, y, s);
+        }
+    }
+    if (A.spmv_kernel == kSpmvPipe) {
+        switch (lanes) {
+            case 4: return launch_spmv_pipe_g<4, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            case 8: return launch_spmv_pipe_g<8, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            case 16: return launch_spmv_pipe_g<16, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            case 32: return launch_spmv_pipe_g<32, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            default: return launch_spmv_pipe_g<64, VT, AT>(A, val, alpha, x, beta, b, y, s);
+        }
+    }
     if (A.spmv_kernel == kSpmvTile) {
         switch (lanes) {
             case 4: return launch_spmv_tile_g<4, VT, AT>(A, val, alpha, x, beta, b, y, s);
