@@ -240,7 +240,7 @@ int amgt_mbsr_get_info(amgt_mbsr_t A, amgt_mbsr_info_t *info) {
 
 int amgt_mbsr_set_spmv_kernel(amgt_mbsr_t A, int kind) {
     AMGT_CHECK_ARG(A != nullptr, "handle is NULL");
-    AMGT_CHECK_ARG(kind >= AMGT_SPMV_DIRECT && kind <= 9, "unknown SpMV kernel kind");
+    AMGT_CHECK_ARG(kind >= AMGT_SPMV_DIRECT && kind <= 10, "unknown SpMV kernel kind");
     A->m.spmv_kernel = kind;
     return AMGT_OK;
 }

@@ -90,7 +90,7 @@ bool mixed_precision_enabled();
 
 // SpMV kernel kinds (amgt_mbsr_set_spmv_kernel)
 enum : int { kSpmvDirect = 0, kSpmvStaged = 1, kSpmvIlp2 = 2, kSpmvIlp4 = 3, kSpmvTile = 4,
-             kSpmvStagedX0 = 5, kSpmvStagedX3 = 8, kSpmvPipe = 9 };
+             kSpmvStagedX0 = 5, kSpmvStagedX3 = 8, kSpmvPipe = 9, kSpmvBlock = 10 };
 
 // ---- component entry points (each in its own translation unit) -------------
 int csr_to_mbsr(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *rowptr,

@@ -642,6 +642,156 @@ __global__ __launch_bounds__(256) void k_spmv_pipe(
     }
 }
 
+// Block-lane variant: one lane per 4x4 block.  A workgroup owns RPB block rows
+// (at most 512 blocks, 2048 values).  Thread t takes tile blocks 2t and 2t+1;
+// the tile's values are staged in LDS with coalesced loads; each lane loads the
+// whole 32-byte x segment of its block once (two 16-byte loads), walks the 16
+// bitmap bits against its LDS values into 4 row partials, and after a barrier
+// the partials replace the values in LDS.  Lane (block row, r) then sums row
+// r's partials over its blocks in ascending block order, so results are
+// deterministic.  Block value offsets come from a workgroup scan of the
+// bitmap popcounts (blocks and values of a tile are both contiguous).  Tiles
+// over the capacity fall back to the walk on global memory.  x must be
+// 16-byte aligned.
+template <int RPB, typename VT, typename AT>
+__global__ __launch_bounds__(256) void k_spmv_blk(
+    int32_t brow, int32_t nrows, int32_t ncols, const int32_t *__restrict__ blk_ptr,
+    const int32_t *__restrict__ blk_col, const uint16_t *__restrict__ blk_map,
+    const int32_t *__restrict__ val_ptr, const VT *__restrict__ val, const double *__restrict__ x,
+    double alpha, double beta, const double *b, double *y) {
+    constexpr int CAPV = 2048, CAPB = 512, VPT = CAPV / 256;
+    __shared__ AT s_buf[CAPV];
+    __shared__ int32_t s_bp[RPB + 1], s_vp[RPB + 1];
+    __shared__ int32_t s_sc[4];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wg = remap_wg(blockIdx.x, gridDim.x);
+    const int br0 = wg * RPB;
+    const int nr = min(RPB, brow - br0);
+    for (int t = tid; t <= nr; t += 256) {
+        s_bp[t] = blk_ptr[br0 + t];
+        s_vp[t] = val_ptr[br0 + t];
+    }
+    __syncthreads();
+    const int pb = s_bp[0], pe = s_bp[nr];
+    const int vb = s_vp[0], ve = s_vp[nr];
+    const int nb = pe - pb, nv = ve - vb;
+    if (nb > CAPB || nv > CAPV) {
+        // oversized tile: 256/RPB lanes per block row walk global memory
+        constexpr int G = 256 / RPB >= 4 ? (256 / RPB > 64 ? 64 : 256 / RPB) : 4;
+        constexpr int RPW = 64 / G;
+        const int lr = (tid >> 6) * RPW + lane / G;
+        if (lr >= nr || lr >= 4 * RPW) return;
+        const int sub = lane % G, r = sub & 3, s = sub >> 2;
+        const AT acc = row_walk<G, VT, AT>(blk_col, blk_map, val, s_bp[lr], s_bp[lr + 1],
+                                           s_vp[lr], s, r, x);
+        if (s == 0) {
+            const int row = 4 * (br0 + lr) + r;
+            if (row < nrows) {
+                double v = alpha * double(acc);
+                if (beta != 0.0) v = fma(beta, b[row], v);
+                y[row] = v;
+            }
+        }
+        return;
+    }
+    // this thread's two blocks
+    uint32_t m[2];
+    int c4[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int q = 2 * tid + u;
+        m[u] = 0u;
+        c4[u] = 0;
+        if (q < nb) {
+            m[u] = blk_map[pb + q];
+            c4[u] = 4 * blk_col[pb + q];
+        }
+    }
+    {
+        VT tv[VPT];
+#pragma unroll
+        for (int u = 0; u < VPT; ++u) {
+            const int q = u * 256 + tid;
+            if (q < nv) tv[u] = val[vb + q];
+        }
+#pragma unroll
+        for (int u = 0; u < VPT; ++u) {
+            const int q = u * 256 + tid;
+            if (q < nv) s_buf[q] = AT(tv[u]);
+        }
+    }
+    // x segments of both blocks (independent of LDS)
+    double xs[2][4];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        xs[u][0] = xs[u][1] = xs[u][2] = xs[u][3] = 0.0;
+        if (m[u]) {
+            const double *xp = x + c4[u];
+            if (c4[u] + 4 <= ncols) {
+                const double2 lo2 = *reinterpret_cast<const double2 *>(xp);
+                const double2 hi2 = *reinterpret_cast<const double2 *>(xp + 2);
+                xs[u][0] = lo2.x;
+                xs[u][1] = lo2.y;
+                xs[u][2] = hi2.x;
+                xs[u][3] = hi2.y;
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    if (c4[u] + c < ncols) xs[u][c] = xp[c];
+            }
+        }
+    }
+    // exclusive scan of the popcounts over the workgroup -> value offsets
+    const int pc0 = __popc(m[0]), pc1 = __popc(m[1]);
+    int incl = pc0 + pc1;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(incl, d);
+        if (lane >= d) incl += t;
+    }
+    if (lane == 63) s_sc[tid >> 6] = incl;
+    __syncthreads();  // also publishes the staged values
+    int before = 0;
+    for (int q = 0; q < (tid >> 6); ++q) before += s_sc[q];
+    const int o0 = before + incl - pc0 - pc1;
+    int oo[2] = {o0, o0 + pc0};
+    AT part[2][4];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        part[u][0] = part[u][1] = part[u][2] = part[u][3] = AT(0);
+        int o = oo[u];
+#pragma unroll
+        for (int bit = 0; bit < 16; ++bit) {
+            if ((m[u] >> bit) & 1u) {
+                part[u][bit >> 2] = mul_acc<AT, AT>(part[u][bit >> 2], s_buf[o], xs[u][bit & 3]);
+                ++o;
+            }
+        }
+    }
+    __syncthreads();  // every value read: the buffer now takes 4 partials per block
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int q = 2 * tid + u;
+        if (q < nb) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) s_buf[4 * q + r] = part[u][r];
+        }
+    }
+    __syncthreads();
+    const int lr = tid >> 2, r = tid & 3;
+    if (lr < nr) {
+        AT acc = AT(0);
+        for (int j = s_bp[lr] - pb; j < s_bp[lr + 1] - pb; ++j) acc += s_buf[4 * j + r];
+        const int row = 4 * (br0 + lr) + r;
+        if (row < nrows) {
+            double v = alpha * double(acc);
+            if (beta != 0.0) v = fma(beta, b[row], v);
+            y[row] = v;
+        }
+    }
+}
+
 template <int G, typename VT, typename AT, int CAPV, int CAPB, int MAXB, bool XSEG>
 __global__ __launch_bounds__(256) void k_spmv_staged(
     int32_t brow, int32_t nrows, int32_t ncols, const int32_t *__restrict__ blk_ptr,
@@ -881,6 +1031,17 @@ int launch_spmv_pipe_g(const Mbsr &A, const VT *val, double alpha, const double 
     return AMGT_OK;
 }
 
+template <int RPB, typename VT, typename AT>
+int launch_spmv_blk_g(const Mbsr &A, const VT *val, double alpha, const double *x, double beta,
+                      const double *b, double *y, hipStream_t s) {
+    const int64_t grid = (int64_t(A.brow) + RPB - 1) / RPB;
+    hipLaunchKernelGGL((k_spmv_blk<RPB, VT, AT>), dim3(unsigned(grid)), dim3(256), 0, s, A.brow,
+                       A.nrows, A.ncols, A.blk_ptr, A.blk_col, A.blk_map, A.val_ptr, val, x,
+                       alpha, beta, b, y);
+    AMGT_HIP(hipGetLastError());
+    return AMGT_OK;
+}
+
 template <int G, typename VT, typename AT>
 int launch_spmv_tile_g(const Mbsr &A, const VT *val, double alpha, const double *x, double beta,
                        const double *b, double *y, hipStream_t s) {
@@ -945,6 +1106,16 @@ int launch_spmv_t(const Mbsr &A, const VT *val, int lanes, double alpha, const d
             if (v == 1) return launch_spmv_staged_g<8, VT, AT, 4, false>(A, val, alpha, x, beta, b, y, s);
             if (v == 2) return launch_spmv_staged_g<8, VT, AT, 2, false>(A, val, alpha, x, beta, b, y, s);
             if (x_aligned) return launch_spmv_staged_g<8, VT, AT, 4, true>(A, val, alpha, x, beta, b, y, s);
+        }
+    }
+    if (A.spmv_kernel == kSpmvBlock && x_aligned) {
+        // lanes selects the tile height: 4 -> 64 block rows ... 64 -> 4 block rows
+        switch (lanes) {
+            case 4: return launch_spmv_blk_g<64, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            case 8: return launch_spmv_blk_g<32, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            case 16: return launch_spmv_blk_g<16, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            case 32: return launch_spmv_blk_g<8, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            default: return launch_spmv_blk_g<4, VT, AT>(A, val, alpha, x, beta, b, y, s);
         }
     }
     if (A.spmv_kernel == kSpmvPipe) {
