@@ -292,9 +292,11 @@ int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream) {
     AMGT_HIP(hipEventCreate(&e0));
     AMGT_HIP(hipEventCreate(&e1));
     const int cand[5] = {4, 8, 16, 32, 64};
+    // kernels worth timing (the others stay selectable by hand)
+    const int kinds[4] = {kSpmvBlock, kSpmvStagedX3, kSpmvStaged, kSpmvDirect};
     float best = 1e30f;
     int best_lanes = spmv_default_lanes(m), best_kind = m.spmv_kernel;
-    for (int kind = AMGT_SPMV_TILE; kind >= AMGT_SPMV_DIRECT; --kind) {
+    for (int kind : kinds) {
         m.spmv_kernel = kind;
         for (int c = 0; c < 5; ++c) {
             int st = spmv(m, prec, cand[c], 1.0, x, 0.0, nullptr, y, s);  // warm-up

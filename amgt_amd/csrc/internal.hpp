@@ -74,7 +74,7 @@ struct Mbsr {
     int32_t level = 0;
     int32_t precision = -1;   // -1: follow the level policy
     int32_t spmv_lanes = 0;   // 0: heuristic
-    int32_t spmv_kernel = 3;  // kSpmvDirect / kSpmvStaged / kSpmvIlp2 / kSpmvIlp4
+    int32_t spmv_kernel = 10; // kSpmv* kind (amgt_mbsr_set_spmv_kernel); block-lane by default
     Mbsr *trans = nullptr;    // cached A^T
 };
 

@@ -653,13 +653,13 @@ __global__ __launch_bounds__(256) void k_spmv_pipe(
 // bitmap popcounts (blocks and values of a tile are both contiguous).  Tiles
 // over the capacity fall back to the walk on global memory.  x must be
 // 16-byte aligned.
-template <int RPB, typename VT, typename AT>
+template <int RPB, int BPT, typename VT, typename AT>
 __global__ __launch_bounds__(256) void k_spmv_blk(
     int32_t brow, int32_t nrows, int32_t ncols, const int32_t *__restrict__ blk_ptr,
     const int32_t *__restrict__ blk_col, const uint16_t *__restrict__ blk_map,
     const int32_t *__restrict__ val_ptr, const VT *__restrict__ val, const double *__restrict__ x,
     double alpha, double beta, const double *b, double *y) {
-    constexpr int CAPV = 2048, CAPB = 512, VPT = CAPV / 256;
+    constexpr int CAPB = 256 * BPT, CAPV = 4 * CAPB, VPT = CAPV / 256;
     __shared__ AT s_buf[CAPV];
     __shared__ int32_t s_bp[RPB + 1], s_vp[RPB + 1];
     __shared__ int32_t s_sc[4];
@@ -680,27 +680,29 @@ __global__ __launch_bounds__(256) void k_spmv_blk(
         // oversized tile: 256/RPB lanes per block row walk global memory
         constexpr int G = 256 / RPB >= 4 ? (256 / RPB > 64 ? 64 : 256 / RPB) : 4;
         constexpr int RPW = 64 / G;
-        const int lr = (tid >> 6) * RPW + lane / G;
-        if (lr >= nr || lr >= 4 * RPW) return;
-        const int sub = lane % G, r = sub & 3, s = sub >> 2;
-        const AT acc = row_walk<G, VT, AT>(blk_col, blk_map, val, s_bp[lr], s_bp[lr + 1],
-                                           s_vp[lr], s, r, x);
-        if (s == 0) {
-            const int row = 4 * (br0 + lr) + r;
-            if (row < nrows) {
-                double v = alpha * double(acc);
-                if (beta != 0.0) v = fma(beta, b[row], v);
-                y[row] = v;
+        for (int base = 0; base < nr; base += 4 * RPW) {  // uniform trip count
+            const int lr = base + (tid >> 6) * RPW + lane / G;
+            if (lr >= nr) continue;  // whole G-lane group
+            const int sub = lane % G, r = sub & 3, s = sub >> 2;
+            const AT acc = row_walk<G, VT, AT>(blk_col, blk_map, val, s_bp[lr], s_bp[lr + 1],
+                                               s_vp[lr], s, r, x);
+            if (s == 0) {
+                const int row = 4 * (br0 + lr) + r;
+                if (row < nrows) {
+                    double v = alpha * double(acc);
+                    if (beta != 0.0) v = fma(beta, b[row], v);
+                    y[row] = v;
+                }
             }
         }
         return;
     }
-    // this thread's two blocks
-    uint32_t m[2];
-    int c4[2];
+    // this thread's BPT consecutive blocks
+    uint32_t m[BPT];
+    int c4[BPT];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        const int q = 2 * tid + u;
+    for (int u = 0; u < BPT; ++u) {
+        const int q = BPT * tid + u;
         m[u] = 0u;
         c4[u] = 0;
         if (q < nb) {
@@ -721,10 +723,10 @@ __global__ __launch_bounds__(256) void k_spmv_blk(
             if (q < nv) s_buf[q] = AT(tv[u]);
         }
     }
-    // x segments of both blocks (independent of LDS)
-    double xs[2][4];
+    // x segments of the blocks (independent of LDS)
+    double xs[BPT][4];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < BPT; ++u) {
         xs[u][0] = xs[u][1] = xs[u][2] = xs[u][3] = 0.0;
         if (m[u]) {
             const double *xp = x + c4[u];
@@ -743,8 +745,14 @@ __global__ __launch_bounds__(256) void k_spmv_blk(
         }
     }
     // exclusive scan of the popcounts over the workgroup -> value offsets
-    const int pc0 = __popc(m[0]), pc1 = __popc(m[1]);
-    int incl = pc0 + pc1;
+    int pcs[BPT];
+    int mine = 0;
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) {
+        pcs[u] = __popc(m[u]);
+        mine += pcs[u];
+    }
+    int incl = mine;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
         const int t = __shfl_up(incl, d);
@@ -752,15 +760,12 @@ __global__ __launch_bounds__(256) void k_spmv_blk(
     }
     if (lane == 63) s_sc[tid >> 6] = incl;
     __syncthreads();  // also publishes the staged values
-    int before = 0;
-    for (int q = 0; q < (tid >> 6); ++q) before += s_sc[q];
-    const int o0 = before + incl - pc0 - pc1;
-    int oo[2] = {o0, o0 + pc0};
-    AT part[2][4];
+    int o = incl - mine;
+    for (int q = 0; q < (tid >> 6); ++q) o += s_sc[q];
+    AT part[BPT][4];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < BPT; ++u) {
         part[u][0] = part[u][1] = part[u][2] = part[u][3] = AT(0);
-        int o = oo[u];
 #pragma unroll
         for (int bit = 0; bit < 16; ++bit) {
             if ((m[u] >> bit) & 1u) {
@@ -771,16 +776,16 @@ __global__ __launch_bounds__(256) void k_spmv_blk(
     }
     __syncthreads();  // every value read: the buffer now takes 4 partials per block
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        const int q = 2 * tid + u;
+    for (int u = 0; u < BPT; ++u) {
+        const int q = BPT * tid + u;
         if (q < nb) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) s_buf[4 * q + r] = part[u][r];
         }
     }
     __syncthreads();
-    const int lr = tid >> 2, r = tid & 3;
-    if (lr < nr) {
+    for (int rr = tid; rr < 4 * nr; rr += 256) {
+        const int lr = rr >> 2, r = rr & 3;
         AT acc = AT(0);
         for (int j = s_bp[lr] - pb; j < s_bp[lr + 1] - pb; ++j) acc += s_buf[4 * j + r];
         const int row = 4 * (br0 + lr) + r;
@@ -1031,11 +1036,11 @@ int launch_spmv_pipe_g(const Mbsr &A, const VT *val, double alpha, const double 
     return AMGT_OK;
 }
 
-template <int RPB, typename VT, typename AT>
+template <int RPB, int BPT, typename VT, typename AT>
 int launch_spmv_blk_g(const Mbsr &A, const VT *val, double alpha, const double *x, double beta,
                       const double *b, double *y, hipStream_t s) {
     const int64_t grid = (int64_t(A.brow) + RPB - 1) / RPB;
-    hipLaunchKernelGGL((k_spmv_blk<RPB, VT, AT>), dim3(unsigned(grid)), dim3(256), 0, s, A.brow,
+    hipLaunchKernelGGL((k_spmv_blk<RPB, BPT, VT, AT>), dim3(unsigned(grid)), dim3(256), 0, s, A.brow,
                        A.nrows, A.ncols, A.blk_ptr, A.blk_col, A.blk_map, A.val_ptr, val, x,
                        alpha, beta, b, y);
     AMGT_HIP(hipGetLastError());
@@ -1109,13 +1114,14 @@ int launch_spmv_t(const Mbsr &A, const VT *val, int lanes, double alpha, const d
         }
     }
     if (A.spmv_kernel == kSpmvBlock && x_aligned) {
-        // lanes selects the tile height: 4 -> 64 block rows ... 64 -> 4 block rows
+        // lanes selects the tile: 4 -> 64 block rows (2 blocks per thread),
+        // 8 -> 128 (4 per thread), 16 -> 32, 32 -> 16, 64 -> 8 block rows
         switch (lanes) {
-            case 4: return launch_spmv_blk_g<64, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            case 8: return launch_spmv_blk_g<32, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            case 16: return launch_spmv_blk_g<16, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            case 32: return launch_spmv_blk_g<8, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            default: return launch_spmv_blk_g<4, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            case 4: return launch_spmv_blk_g<64, 2, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            case 8: return launch_spmv_blk_g<128, 4, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            case 16: return launch_spmv_blk_g<32, 2, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            case 32: return launch_spmv_blk_g<16, 2, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            default: return launch_spmv_blk_g<8, 2, VT, AT>(A, val, alpha, x, beta, b, y, s);
         }
     }
     if (A.spmv_kernel == kSpmvPipe) {
