@@ -21,10 +21,8 @@ AMGT_ERR_UNSUPPORTED = 6
 
 AMGT_SPMV_DIRECT = 0
 AMGT_SPMV_STAGED = 1
-AMGT_SPMV_ILP2 = 2
-AMGT_SPMV_ILP4 = 3
-AMGT_SPMV_TILE = 4
-SPMV_KERNEL_NAMES = {0: "direct", 1: "staged", 2: "ilp2", 3: "ilp4", 4: "tile", 5: "stg_g8", 6: "stg_g4", 7: "stg_g2", 8: "stg_s4", 9: "pipe", 10: "block"}
+AMGT_SPMV_BLOCK = 2
+SPMV_KERNEL_NAMES = {0: "direct", 1: "staged", 2: "block"}
 
 AMGT_DOUBLE = 0
 AMGT_FLOAT = 1

@@ -240,7 +240,7 @@ int amgt_mbsr_get_info(amgt_mbsr_t A, amgt_mbsr_info_t *info) {
 
 int amgt_mbsr_set_spmv_kernel(amgt_mbsr_t A, int kind) {
     AMGT_CHECK_ARG(A != nullptr, "handle is NULL");
-    AMGT_CHECK_ARG(kind >= AMGT_SPMV_DIRECT && kind <= 10, "unknown SpMV kernel kind");
+    AMGT_CHECK_ARG(kind >= AMGT_SPMV_DIRECT && kind <= AMGT_SPMV_BLOCK, "unknown SpMV kernel kind");
     A->m.spmv_kernel = kind;
     return AMGT_OK;
 }
@@ -293,15 +293,15 @@ int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream) {
     AMGT_HIP(hipEventCreate(&e1));
     const int cand[5] = {4, 8, 16, 32, 64};
     // kernels worth timing (the others stay selectable by hand)
-    const int kinds[4] = {kSpmvBlock, kSpmvStagedX3, kSpmvStaged, kSpmvDirect};
-    float t_min[4][5];
+    const int kinds[3] = {kSpmvBlock, kSpmvStaged, kSpmvDirect};
+    float t_min[3][5];
     for (auto &row : t_min)
         for (float &v : row) v = 1e30f;
     // bring the clocks up before the first measurement
     for (int r = 0; r < 30; ++r) AMGT_TRY(spmv(m, prec, spmv_default_lanes(m), 1.0, x, 0.0, nullptr, y, s));
     // two sweeps, keep each candidate's best time
     for (int sweep = 0; sweep < 2; ++sweep) {
-        for (int kk = 0; kk < 4; ++kk) {
+        for (int kk = 0; kk < 3; ++kk) {
             m.spmv_kernel = kinds[kk];
             for (int c = 0; c < 5; ++c) {
                 int st = AMGT_OK;
@@ -321,7 +321,7 @@ int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream) {
     }
     float best = 1e30f;
     int best_lanes = spmv_default_lanes(m), best_kind = kinds[0];
-    for (int kk = 0; kk < 4; ++kk)
+    for (int kk = 0; kk < 3; ++kk)
         for (int c = 0; c < 5; ++c)
             if (t_min[kk][c] < best) {
                 best = t_min[kk][c];

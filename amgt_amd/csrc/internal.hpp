@@ -74,7 +74,7 @@ struct Mbsr {
     int32_t level = 0;
     int32_t precision = -1;   // -1: follow the level policy
     int32_t spmv_lanes = 0;   // 0: heuristic
-    int32_t spmv_kernel = 10; // kSpmv* kind (amgt_mbsr_set_spmv_kernel); block-lane by default
+    int32_t spmv_kernel = 2;  // kSpmv* kind (amgt_mbsr_set_spmv_kernel); block-lane by default
     Mbsr *trans = nullptr;    // cached A^T
 };
 
@@ -89,8 +89,7 @@ namespace amgt {
 bool mixed_precision_enabled();
 
 // SpMV kernel kinds (amgt_mbsr_set_spmv_kernel)
-enum : int { kSpmvDirect = 0, kSpmvStaged = 1, kSpmvIlp2 = 2, kSpmvIlp4 = 3, kSpmvTile = 4,
-             kSpmvStagedX0 = 5, kSpmvStagedX3 = 8, kSpmvPipe = 9, kSpmvBlock = 10 };
+enum : int { kSpmvDirect = 0, kSpmvStaged = 1, kSpmvBlock = 2 };
 
 // ---- component entry points (each in its own translation unit) -------------
 int csr_to_mbsr(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *rowptr,

@@ -118,114 +118,6 @@ __global__ __launch_bounds__(256) void k_spmv(
     }
 }
 
-// Load-level-parallel variant: each lane takes U blocks per step and issues
-// all their bitmap/column loads, then all their x-segment loads (two 16-byte
-// loads per block) and value loads, before the first FMA.  The value of bit c
-// of row nibble `nib` sits at  o + popc(nib & ((1 << c) - 1)).  FMAs are
-// predicated on the bit (a zero value never meets an unused x entry), in the
-// same block/column order as the other kernels.  x must be 16-byte aligned.
-template <int G, int U, typename VT, typename AT>
-__global__ __launch_bounds__(256) void k_spmv_ilp(
-    int32_t brow, int32_t nrows, int32_t ncols, const int32_t *__restrict__ blk_ptr,
-    const int32_t *__restrict__ blk_col, const uint16_t *__restrict__ blk_map,
-    const int32_t *__restrict__ val_ptr, const VT *__restrict__ val, const double *__restrict__ x,
-    double alpha, double beta, const double *b, double *y) {
-    constexpr int S = G / 4;
-    constexpr int RPW = 64 / G;
-    const int lane = threadIdx.x & 63;
-    const int wg = remap_wg(blockIdx.x, gridDim.x);
-    const int wave = wg * 4 + (threadIdx.x >> 6);
-    const int br = wave * RPW + lane / G;
-    const int sub = lane % G;
-    const int r = sub & 3;
-    const int s = sub >> 2;
-    if (br >= brow) return;
-    const int p0 = blk_ptr[br];
-    const int p1 = blk_ptr[br + 1];
-    int voff = val_ptr[br];
-    const uint32_t low = (1u << (4 * r)) - 1u;
-    AT acc = AT(0);
-    for (int j0 = p0; j0 < p1; j0 += S * U) {
-        uint32_t m[U];
-        int c4[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int j = j0 + s + S * u;
-            m[u] = 0u;
-            c4[u] = 0;
-            if (j < p1) {
-                m[u] = blk_map[j];
-                c4[u] = 4 * blk_col[j];
-            }
-        }
-        double xs[U][4];
-        int o[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t nib = (m[u] >> (4 * r)) & 0xFu;
-            xs[u][0] = xs[u][1] = xs[u][2] = xs[u][3] = 0.0;
-            if (nib) {
-                const double *xp = x + c4[u];
-                if (c4[u] + 4 <= ncols) {
-                    const double2 lo2 = *reinterpret_cast<const double2 *>(xp);
-                    const double2 hi2 = *reinterpret_cast<const double2 *>(xp + 2);
-                    xs[u][0] = lo2.x;
-                    xs[u][1] = lo2.y;
-                    xs[u][2] = hi2.x;
-                    xs[u][3] = hi2.y;
-                } else {
-#pragma unroll
-                    for (int c = 0; c < 4; ++c)
-                        if (c4[u] + c < ncols) xs[u][c] = xp[c];
-                }
-            }
-            const int cnt = __popc(m[u]);
-            int incl = cnt;
-            int tot = cnt;
-            if constexpr (S > 1) {
-#pragma unroll
-                for (int d = 1; d < S; d <<= 1) {
-                    int t = __shfl_up(incl, 4 * d, G);
-                    if (s >= d) incl += t;
-                }
-                tot = __shfl(incl, 4 * (S - 1) + r, G);
-            }
-            o[u] = voff + (incl - cnt) + __popc(m[u] & low);
-            voff += tot;
-        }
-        VT v[U][4];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t nib = (m[u] >> (4 * r)) & 0xFu;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                v[u][c] = VT(0);
-                if ((nib >> c) & 1u) v[u][c] = val[o[u] + __popc(nib & ((1u << c) - 1u))];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t nib = (m[u] >> (4 * r)) & 0xFu;
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-                if ((nib >> c) & 1u) acc = mul_acc<VT, AT>(acc, v[u][c], xs[u][c]);
-        }
-    }
-    if constexpr (S > 1) {
-#pragma unroll
-        for (int d = S / 2; d >= 1; d >>= 1) acc += __shfl_xor(acc, 4 * d, G);
-    }
-    if (s == 0) {
-        const int row = 4 * br + r;
-        if (row < nrows) {
-            double v = alpha * double(acc);
-            if (beta != 0.0) v = fma(beta, b[row], v);
-            y[row] = v;
-        }
-    }
-}
-
-
 // Staged variant: a workgroup owns 256/G consecutive block rows.  Their block
 // columns, bitmaps and values (contiguous ranges) are first copied to LDS with
 // coalesced loads that depend only on the two pointer pairs, so every matrix
@@ -359,287 +251,6 @@ __device__ __forceinline__ AT row_walk_prefetch(const int32_t *s_col, const uint
         for (int d = S / 2; d >= 1; d >>= 1) acc += __shfl_xor(acc, 4 * d, G);
     }
     return acc;
-}
-
-// Tile variant (the CSR-adaptive pattern applied to mBSR).  A workgroup owns
-// 256/G block rows whose values, bitmaps and block columns fit LDS:
-//   1. coalesced, unrolled copy of the tile's values, bitmaps and columns;
-//   2. decode: lane (slot, row) walks its nibbles and writes, at each value's
-//      position, that value's x index 4*col + c;
-//   3. multiply: lane per value, x[x index] * value in place (the loads of a
-//      thread are independent and issued together);
-//   4. reduce: the same walk sums the products of its nibbles.
-// Every matrix byte arrives through coalesced loads and each x entry is
-// gathered once per nonzero.  The summation order per row (blocks ascending,
-// columns ascending) matches the other kernels; products are rounded before
-// the sum instead of fused.  Tiles above the LDS capacity use the direct walk.
-template <int G, typename F>
-__device__ __forceinline__ void nibble_walk(const int32_t *s_col, const uint16_t *s_map, int p0,
-                                            int p1, int voff, int s, int r, F f) {
-    constexpr int S = G / 4;
-    const uint32_t low = (1u << (4 * r)) - 1u;
-    for (int jb = p0; jb < p1; jb += S) {
-        const int j = jb + s;
-        uint32_t m = 0;
-        int c4 = 0;
-        if (j < p1) {
-            m = s_map[j];
-            c4 = 4 * s_col[j];
-        }
-        const int cnt = __popc(m);
-        int incl = cnt;
-        int tot = cnt;
-        if constexpr (S > 1) {
-#pragma unroll
-            for (int d = 1; d < S; d <<= 1) {
-                int t = __shfl_up(incl, 4 * d, G);
-                if (s >= d) incl += t;
-            }
-            tot = __shfl(incl, 4 * (S - 1) + r, G);
-        }
-        int o = voff + (incl - cnt) + __popc(m & low);
-        const uint32_t nib = (m >> (4 * r)) & 0xFu;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            if ((nib >> c) & 1u) {
-                f(o, c4 + c);
-                ++o;
-            }
-        }
-        voff += tot;
-    }
-}
-
-template <int G, typename VT, typename AT, int CAPV, int CAPB>
-__global__ __launch_bounds__(256) void k_spmv_tile(
-    int32_t brow, int32_t nrows, const int32_t *__restrict__ blk_ptr,
-    const int32_t *__restrict__ blk_col, const uint16_t *__restrict__ blk_map,
-    const int32_t *__restrict__ val_ptr, const VT *__restrict__ val, const double *__restrict__ x,
-    double alpha, double beta, const double *b, double *y) {
-    constexpr int RPW = 64 / G;
-    constexpr int RPB = 256 / G;
-    constexpr int S = G / 4;
-    __shared__ AT s_prod[CAPV];
-    __shared__ int32_t s_xi[CAPV];
-    __shared__ int32_t s_col[CAPB];
-    __shared__ uint16_t s_map[CAPB];
-    __shared__ int32_t s_bp[RPB + 1], s_vp[RPB + 1];
-    const int wg = remap_wg(blockIdx.x, gridDim.x);
-    const int br0 = wg * RPB;
-    const int nr = min(RPB, brow - br0);
-    for (int t = threadIdx.x; t <= nr; t += 256) {
-        s_bp[t] = blk_ptr[br0 + t];
-        s_vp[t] = val_ptr[br0 + t];
-    }
-    __syncthreads();
-    const int pb = s_bp[0], pe = s_bp[nr];
-    const int vb = s_vp[0], ve = s_vp[nr];
-    const int nv = ve - vb, nb = pe - pb;
-    const bool staged = (nb <= CAPB) && (nv <= CAPV);
-    const int lane = threadIdx.x & 63;
-    const int lr = (threadIdx.x >> 6) * RPW + lane / G;
-    const int sub = lane % G;
-    const int r = sub & 3;
-    const int s = sub >> 2;
-    const bool active = lr < nr;
-    if (!staged) {
-        if (!active) return;
-        const AT acc = row_walk<G, VT, AT>(blk_col, blk_map, val, s_bp[lr], s_bp[lr + 1],
-                                           s_vp[lr], s, r, x);
-        if (s == 0) {
-            const int row = 4 * (br0 + lr) + r;
-            if (row < nrows) {
-                double v = alpha * double(acc);
-                if (beta != 0.0) v = fma(beta, b[row], v);
-                y[row] = v;
-            }
-        }
-        return;
-    }
-    constexpr int VPT = CAPV / 256;
-    constexpr int BPT = CAPB / 256;
-    {
-        VT tv[VPT];
-        int32_t tc[BPT];
-        uint16_t tm[BPT];
-#pragma unroll
-        for (int u = 0; u < VPT; ++u) {
-            const int t = u * 256 + threadIdx.x;
-            if (t < nv) tv[u] = stream_load(val + vb + t);
-        }
-#pragma unroll
-        for (int u = 0; u < BPT; ++u) {
-            const int t = u * 256 + threadIdx.x;
-            if (t < nb) {
-                tc[u] = stream_load(blk_col + pb + t);
-                tm[u] = stream_load(blk_map + pb + t);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < VPT; ++u) {
-            const int t = u * 256 + threadIdx.x;
-            if (t < nv) s_prod[t] = AT(tv[u]);
-        }
-#pragma unroll
-        for (int u = 0; u < BPT; ++u) {
-            const int t = u * 256 + threadIdx.x;
-            if (t < nb) {
-                s_col[t] = tc[u];
-                s_map[t] = tm[u];
-            }
-        }
-    }
-    __syncthreads();
-    const int p0 = active ? s_bp[lr] - pb : 0, p1 = active ? s_bp[lr + 1] - pb : 0;
-    const int v0 = active ? s_vp[lr] - vb : 0;
-    nibble_walk<G>(s_col, s_map, p0, p1, v0, s, r, [&](int o, int xc) { s_xi[o] = xc; });
-    __syncthreads();
-    {
-        double xv[VPT];
-#pragma unroll
-        for (int u = 0; u < VPT; ++u) {
-            const int t = u * 256 + threadIdx.x;
-            if (t < nv) xv[u] = x[s_xi[t]];
-        }
-#pragma unroll
-        for (int u = 0; u < VPT; ++u) {
-            const int t = u * 256 + threadIdx.x;
-            if (t < nv) s_prod[t] = AT(s_prod[t] * AT(xv[u]));
-        }
-    }
-    __syncthreads();
-    AT acc = AT(0);
-    nibble_walk<G>(s_col, s_map, p0, p1, v0, s, r, [&](int o, int) { acc += s_prod[o]; });
-    if constexpr (S > 1) {
-#pragma unroll
-        for (int d = S / 2; d >= 1; d >>= 1) acc += __shfl_xor(acc, 4 * d, G);
-    }
-    if (active && s == 0) {
-        const int row = 4 * (br0 + lr) + r;
-        if (row < nrows) {
-            double v = alpha * double(acc);
-            if (beta != 0.0) v = fma(beta, b[row], v);
-            y[row] = v;
-        }
-    }
-}
-
-// Pipelined tile variant: persistent workgroups, each owning a contiguous
-// range of tiles (256/G block rows each).  While the current tile is walked
-// from LDS, the next tile's block-row pointers, block columns, bitmaps and
-// values are already in flight into registers, so HBM streaming overlaps the
-// x gathers and FMAs.  Same walk and arithmetic as the staged kernel.
-template <int G, typename VT, typename AT, int CAPV, int CAPB, int MAXB>
-__global__ __launch_bounds__(256) void k_spmv_pipe(
-    int32_t brow, int32_t nrows, int32_t ncols, const int32_t *__restrict__ blk_ptr,
-    const int32_t *__restrict__ blk_col, const uint16_t *__restrict__ blk_map,
-    const int32_t *__restrict__ val_ptr, const VT *__restrict__ val, const double *__restrict__ x,
-    double alpha, double beta, const double *b, double *y, int ntiles, int tiles_per_wg) {
-    constexpr int RPW = 64 / G;
-    constexpr int RPB = 256 / G;
-    constexpr int VPT = CAPV / 256;
-    constexpr int BPT = CAPB / 256;
-    __shared__ VT s_val[CAPV];
-    __shared__ int32_t s_col[CAPB];
-    __shared__ uint16_t s_map[CAPB];
-    __shared__ int32_t s_bp[RPB + 1], s_vp[RPB + 1];
-    const int tid = threadIdx.x;
-    const int w = remap_wg(blockIdx.x, gridDim.x);
-    const int t_beg = w * tiles_per_wg;
-    const int t_end = min(t_beg + tiles_per_wg, ntiles);
-    if (t_beg >= t_end) return;  // uniform across the workgroup
-    const int lane = tid & 63;
-    const int lr = (tid >> 6) * RPW + lane / G;
-    const int sub = lane % G;
-    const int r = sub & 3;
-    const int s = sub >> 2;
-
-    // registers holding the next tile
-    VT tv[VPT];
-    int32_t tc[BPT];
-    uint16_t tm[BPT];
-    int32_t tbp = 0, tvp = 0;
-    int pb = 0, pe = 0, vb = 0, ve = 0;
-    bool staged = false;
-    auto issue = [&](int t) {
-        const int br0 = t * RPB;
-        const int nr = min(RPB, brow - br0);
-        pb = blk_ptr[br0];
-        pe = blk_ptr[br0 + nr];
-        vb = val_ptr[br0];
-        ve = val_ptr[br0 + nr];
-        staged = (pe - pb <= CAPB) && (ve - vb <= CAPV);
-        if (tid <= nr) {
-            tbp = blk_ptr[br0 + tid];
-            tvp = val_ptr[br0 + tid];
-        }
-        if (staged) {
-            const int nv = ve - vb, nb = pe - pb;
-#pragma unroll
-            for (int u = 0; u < VPT; ++u) {
-                const int q = u * 256 + tid;
-                if (q < nv) tv[u] = val[vb + q];
-            }
-#pragma unroll
-            for (int u = 0; u < BPT; ++u) {
-                const int q = u * 256 + tid;
-                if (q < nb) {
-                    tc[u] = blk_col[pb + q];
-                    tm[u] = blk_map[pb + q];
-                }
-            }
-        }
-    };
-    issue(t_beg);
-    for (int t = t_beg; t < t_end; ++t) {
-        const int br0 = t * RPB;
-        const int nr = min(RPB, brow - br0);
-        const bool cur_staged = staged;
-        const int cpb = pb, cvb = vb;
-        // registers -> LDS for tile t
-        if (tid <= nr) {
-            s_bp[tid] = tbp;
-            s_vp[tid] = tvp;
-        }
-        if (cur_staged) {
-            const int nv = ve - vb, nb = pe - pb;
-#pragma unroll
-            for (int u = 0; u < VPT; ++u) {
-                const int q = u * 256 + tid;
-                if (q < nv) s_val[q] = tv[u];
-            }
-#pragma unroll
-            for (int u = 0; u < BPT; ++u) {
-                const int q = u * 256 + tid;
-                if (q < nb) {
-                    s_col[q] = tc[u];
-                    s_map[q] = tm[u];
-                }
-            }
-        }
-        __syncthreads();
-        if (t + 1 < t_end) issue(t + 1);  // next tile's loads fly during this walk
-        if (lr < nr) {
-            AT acc;
-            if (cur_staged) {
-                acc = row_walk_prefetch<G, MAXB, false, VT, AT>(
-                    s_col, s_map, s_val, s_bp[lr] - cpb, s_bp[lr + 1] - cpb, s_vp[lr] - cvb, s, r,
-                    x, ncols);
-            } else {
-                acc = row_walk<G, VT, AT>(blk_col, blk_map, val, s_bp[lr], s_bp[lr + 1], s_vp[lr],
-                                          s, r, x);
-            }
-            if (s == 0) {
-                const int row = 4 * (br0 + lr) + r;
-                if (row < nrows) {
-                    double v = alpha * double(acc);
-                    if (beta != 0.0) v = fma(beta, b[row], v);
-                    y[row] = v;
-                }
-            }
-        }
-        __syncthreads();  // LDS is rewritten for the next tile
-    }
 }
 
 // Block-lane variant: one lane per 4x4 block.  A workgroup owns RPB block rows
@@ -1009,33 +620,6 @@ int launch_spmv_staged_g(const Mbsr &A, const VT *val, double alpha, const doubl
     return AMGT_OK;
 }
 
-int cu_count() {
-    static int n = 0;
-    if (n == 0) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            n <= 0)
-            n = 256;
-    }
-    return n;
-}
-
-template <int G, typename VT, typename AT>
-int launch_spmv_pipe_g(const Mbsr &A, const VT *val, double alpha, const double *x, double beta,
-                       const double *b, double *y, hipStream_t s) {
-    constexpr int RPB = 256 / G;
-    const int ntiles = int((int64_t(A.brow) + RPB - 1) / RPB);
-    const int max_wg = cu_count() * 6;
-    const int tpw = (ntiles + max_wg - 1) / max_wg;
-    const int nwg = (ntiles + tpw - 1) / tpw;
-    hipLaunchKernelGGL((k_spmv_pipe<G, VT, AT, kStageVals, kStageBlocks, 4>), dim3(unsigned(nwg)),
-                       dim3(256), 0, s, A.brow, A.nrows, A.ncols, A.blk_ptr, A.blk_col, A.blk_map,
-                       A.val_ptr, val, x, alpha, beta, b, y, ntiles, tpw);
-    AMGT_HIP(hipGetLastError());
-    return AMGT_OK;
-}
-
 template <int RPB, int BPT, typename VT, typename AT>
 int launch_spmv_blk_g(const Mbsr &A, const VT *val, double alpha, const double *x, double beta,
                       const double *b, double *y, hipStream_t s) {
@@ -1047,43 +631,6 @@ int launch_spmv_blk_g(const Mbsr &A, const VT *val, double alpha, const double *
     return AMGT_OK;
 }
 
-template <int G, typename VT, typename AT>
-int launch_spmv_tile_g(const Mbsr &A, const VT *val, double alpha, const double *x, double beta,
-                       const double *b, double *y, hipStream_t s) {
-    constexpr int RPB = 256 / G;
-    const int64_t grid = (int64_t(A.brow) + RPB - 1) / RPB;
-    hipLaunchKernelGGL((k_spmv_tile<G, VT, AT, kStageVals, kStageBlocks>), dim3(unsigned(grid)),
-                       dim3(256), 0, s, A.brow, A.nrows, A.blk_ptr, A.blk_col, A.blk_map,
-                       A.val_ptr, val, x, alpha, beta, b, y);
-    AMGT_HIP(hipGetLastError());
-    return AMGT_OK;
-}
-
-template <int G, int U, typename VT, typename AT>
-int launch_spmv_ilp_g(const Mbsr &A, const VT *val, double alpha, const double *x, double beta,
-                      const double *b, double *y, hipStream_t s) {
-    constexpr int RPW = 64 / G;
-    const int64_t waves = (int64_t(A.brow) + RPW - 1) / RPW;
-    const int64_t grid = (waves + 3) / 4;
-    hipLaunchKernelGGL((k_spmv_ilp<G, U, VT, AT>), dim3(unsigned(grid)), dim3(256), 0, s, A.brow,
-                       A.nrows, A.ncols, A.blk_ptr, A.blk_col, A.blk_map, A.val_ptr, val, x,
-                       alpha, beta, b, y);
-    AMGT_HIP(hipGetLastError());
-    return AMGT_OK;
-}
-
-template <int U, typename VT, typename AT>
-int launch_spmv_ilp(const Mbsr &A, const VT *val, int lanes, double alpha, const double *x,
-                    double beta, const double *b, double *y, hipStream_t s) {
-    switch (lanes) {
-        case 4: return launch_spmv_ilp_g<4, U, VT, AT>(A, val, alpha, x, beta, b, y, s);
-        case 8: return launch_spmv_ilp_g<8, U, VT, AT>(A, val, alpha, x, beta, b, y, s);
-        case 16: return launch_spmv_ilp_g<16, U, VT, AT>(A, val, alpha, x, beta, b, y, s);
-        case 32: return launch_spmv_ilp_g<32, U, VT, AT>(A, val, alpha, x, beta, b, y, s);
-        default: return launch_spmv_ilp_g<64, U, VT, AT>(A, val, alpha, x, beta, b, y, s);
-    }
-}
-
 template <typename VT, typename AT>
 int launch_spmv_t(const Mbsr &A, const VT *val, int lanes, double alpha, const double *x,
                   double beta, const double *b, double *y, hipStream_t s) {
@@ -1091,28 +638,8 @@ int launch_spmv_t(const Mbsr &A, const VT *val, int lanes, double alpha, const d
         set_error("lanes per block row must be 4, 8, 16, 32 or 64");
         return AMGT_ERR_ARG;
     }
-    // the staged and ILP walks read x in 16-byte pieces; otherwise the direct kernel
+    // the staged and block kernels read x in 16-byte pieces; otherwise the direct kernel
     const bool x_aligned = (reinterpret_cast<uintptr_t>(x) & 15u) == 0;
-    if (A.spmv_kernel == kSpmvIlp2 && x_aligned)
-        return launch_spmv_ilp<2, VT, AT>(A, val, lanes, alpha, x, beta, b, y, s);
-    if (A.spmv_kernel == kSpmvIlp4 && x_aligned)
-        return launch_spmv_ilp<4, VT, AT>(A, val, lanes, alpha, x, beta, b, y, s);
-    if (A.spmv_kernel >= kSpmvStagedX0 && A.spmv_kernel <= kSpmvStagedX3) {
-        // staged kernel with per-bit x gathers (no alignment need), MAXB = 8/4/2
-        // and a 4-block segment variant
-        const int v = A.spmv_kernel - kSpmvStagedX0;
-        if (lanes == 4) {
-            if (v == 0) return launch_spmv_staged_g<4, VT, AT, 8, false>(A, val, alpha, x, beta, b, y, s);
-            if (v == 1) return launch_spmv_staged_g<4, VT, AT, 4, false>(A, val, alpha, x, beta, b, y, s);
-            if (v == 2) return launch_spmv_staged_g<4, VT, AT, 2, false>(A, val, alpha, x, beta, b, y, s);
-            if (x_aligned) return launch_spmv_staged_g<4, VT, AT, 4, true>(A, val, alpha, x, beta, b, y, s);
-        } else if (lanes == 8) {
-            if (v == 0) return launch_spmv_staged_g<8, VT, AT, 8, false>(A, val, alpha, x, beta, b, y, s);
-            if (v == 1) return launch_spmv_staged_g<8, VT, AT, 4, false>(A, val, alpha, x, beta, b, y, s);
-            if (v == 2) return launch_spmv_staged_g<8, VT, AT, 2, false>(A, val, alpha, x, beta, b, y, s);
-            if (x_aligned) return launch_spmv_staged_g<8, VT, AT, 4, true>(A, val, alpha, x, beta, b, y, s);
-        }
-    }
     if (A.spmv_kernel == kSpmvBlock && x_aligned) {
         // lanes selects the tile: 4 -> 64 block rows (2 blocks per thread),
         // 8 -> 128 (4 per thread), 16 -> 32, 32 -> 16, 64 -> 8 block rows
@@ -1124,32 +651,13 @@ int launch_spmv_t(const Mbsr &A, const VT *val, int lanes, double alpha, const d
             default: return launch_spmv_blk_g<8, 2, VT, AT>(A, val, alpha, x, beta, b, y, s);
         }
     }
-    if (A.spmv_kernel == kSpmvPipe) {
-        switch (lanes) {
-            case 4: return launch_spmv_pipe_g<4, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            case 8: return launch_spmv_pipe_g<8, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            case 16: return launch_spmv_pipe_g<16, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            case 32: return launch_spmv_pipe_g<32, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            default: return launch_spmv_pipe_g<64, VT, AT>(A, val, alpha, x, beta, b, y, s);
-        }
-    }
-    if (A.spmv_kernel == kSpmvTile) {
-        switch (lanes) {
-            case 4: return launch_spmv_tile_g<4, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            case 8: return launch_spmv_tile_g<8, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            case 16: return launch_spmv_tile_g<16, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            case 32: return launch_spmv_tile_g<32, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            default: return launch_spmv_tile_g<64, VT, AT>(A, val, alpha, x, beta, b, y, s);
-        }
-    }
     if (A.spmv_kernel == kSpmvStaged && x_aligned) {
         switch (lanes) {
-            case 4: return launch_spmv_staged_g<4, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            case 8: return launch_spmv_staged_g<8, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            case 16: return launch_spmv_staged_g<16, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            case 32: return launch_spmv_staged_g<32, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            case 64: return launch_spmv_staged_g<64, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            default: break;
+            case 4: return launch_spmv_staged_g<4, VT, AT, 4, true>(A, val, alpha, x, beta, b, y, s);
+            case 8: return launch_spmv_staged_g<8, VT, AT, 4, true>(A, val, alpha, x, beta, b, y, s);
+            case 16: return launch_spmv_staged_g<16, VT, AT, 4, true>(A, val, alpha, x, beta, b, y, s);
+            case 32: return launch_spmv_staged_g<32, VT, AT, 4, true>(A, val, alpha, x, beta, b, y, s);
+            default: return launch_spmv_staged_g<64, VT, AT, 4, true>(A, val, alpha, x, beta, b, y, s);
         }
     }
     switch (lanes) {

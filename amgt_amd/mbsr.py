@@ -127,7 +127,7 @@ class MBSRMatrix:
         L.check(L.load().amgt_mbsr_set_spmv_lanes(self.handle, lanes), "amgt_mbsr_set_spmv_lanes")
 
     def set_spmv_kernel(self, kind: int):
-        """AMGT_SPMV_DIRECT / _STAGED / _ILP2 / _ILP4."""
+        """AMGT_SPMV_DIRECT / _STAGED / _BLOCK."""
         L.check(L.load().amgt_mbsr_set_spmv_kernel(self.handle, int(kind)),
                 "amgt_mbsr_set_spmv_kernel")
 

@@ -53,7 +53,7 @@ typedef struct {
     int32_t level;      /* coarsening level tag (hypre_Level, seq_mv.h:199) */
     int32_t precision;  /* value precision used by SpMV (AMGT_DOUBLE/FLOAT/HALF) */
     int32_t spmv_lanes; /* lanes per block row chosen by the tuner (4..64) */
-    int32_t spmv_kernel; /* AMGT_SPMV_DIRECT / _STAGED / _ILP2 / _ILP4 */
+    int32_t spmv_kernel; /* AMGT_SPMV_DIRECT / _STAGED / _BLOCK */
 } amgt_mbsr_info_t;
 
 typedef struct {
@@ -99,15 +99,14 @@ int amgt_mbsr_set_precision(amgt_mbsr_t A, int precision);
  * matrix statistics */
 int amgt_mbsr_set_spmv_lanes(amgt_mbsr_t A, int lanes);
 /* SpMV kernel kind:
- *   AMGT_SPMV_DIRECT  the walk reads bitmaps, columns and values from global memory
- *   AMGT_SPMV_STAGED  a workgroup first copies its tile of block rows to LDS
- *   AMGT_SPMV_ILP2/4  each lane issues the loads of 2 / 4 blocks before using them
- *   AMGT_SPMV_TILE    tile staged in LDS, bitmaps decoded to per-value x indices,
- *                     one independent x gather per nonzero, LDS row reduction
- * The staged and ILP kernels need a 16-byte aligned x; otherwise the direct
+ *   AMGT_SPMV_DIRECT  lane per (block slot, row) walks bitmaps, columns and values in
+ *                     global memory
+ *   AMGT_SPMV_STAGED  a workgroup stages its tile of block rows in LDS, lanes walk it
+ *   AMGT_SPMV_BLOCK   (default) lane per 4x4 block over an LDS-staged tile, whole x
+ *                     segments, per-row partial sums reduced in LDS
+ * The staged and block kernels need a 16-byte aligned x; otherwise the direct
  * kernel runs. */
-enum { AMGT_SPMV_DIRECT = 0, AMGT_SPMV_STAGED = 1, AMGT_SPMV_ILP2 = 2, AMGT_SPMV_ILP4 = 3,
-       AMGT_SPMV_TILE = 4 };
+enum { AMGT_SPMV_DIRECT = 0, AMGT_SPMV_STAGED = 1, AMGT_SPMV_BLOCK = 2 };
 int amgt_mbsr_set_spmv_kernel(amgt_mbsr_t A, int kind);
 /* times every lanes-per-block-row variant on this matrix and keeps the fastest */
 int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream);

@@ -108,7 +108,7 @@ def row_scale(m, n, I, J, A, x, beta, b):
 @pytest.mark.parametrize("name", ["c1_5pt_128", "7pt_aniso", "27pt_10", "rand_rect",
                                   "rand_tall_empty", "rand_wide", "bin1_rows", "bin2_rows"])
 @pytest.mark.parametrize("lanes", [4, 8, 16, 32, 64])
-@pytest.mark.parametrize("kind", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
+@pytest.mark.parametrize("kind", [0, 1, 2])
 def test_spmv_fp64(dev, name, lanes, kind):
     print(f"[spmv] {name} lanes={lanes} kernel={kind}", flush=True)
     (m, n), I, J, A = CASES[name]()
