@@ -22,7 +22,8 @@ AMGT_ERR_UNSUPPORTED = 6
 AMGT_SPMV_DIRECT = 0
 AMGT_SPMV_STAGED = 1
 AMGT_SPMV_BLOCK = 2
-SPMV_KERNEL_NAMES = {0: "direct", 1: "staged", 2: "block"}
+AMGT_SPMV_BALANCED = 3
+SPMV_KERNEL_NAMES = {0: "direct", 1: "staged", 2: "block", 3: "balanced"}
 
 AMGT_DOUBLE = 0
 AMGT_FLOAT = 1

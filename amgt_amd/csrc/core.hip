@@ -25,7 +25,44 @@ void set_error(const std::string &msg) { g_last_error = msg; }
 
 bool mixed_precision_enabled() { return g_mixed.load() != 0; }
 
-void *dev_alloc(size_t bytes) {
+// Device memory pool of the library, one per device, created on first use.
+// Freed blocks stay in the pool (release threshold: unlimited), so the many
+// temporaries of a conversion or product, and the arrays of matrices made and
+// dropped level after level, are served without new device mappings.
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_pool_mu;
+hipMemPool_t g_pool[kMaxDevices] = {};
+}  // namespace
+
+static hipMemPool_t lib_pool() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    if (!g_pool[dev]) {
+        hipMemPoolProps props{};
+        props.allocType = hipMemAllocationTypePinned;
+        props.handleTypes = hipMemHandleTypeNone;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        hipMemPool_t pool = nullptr;
+        if (hipMemPoolCreate(&pool, &props) != hipSuccess) return nullptr;
+        uint64_t keep = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        g_pool[dev] = pool;
+    }
+    return g_pool[dev];
+}
+
+static void *pool_alloc(size_t bytes, hipStream_t s) {
+    hipMemPool_t pool = lib_pool();
+    if (!pool) return nullptr;
+    void *p = nullptr;
+    if (hipMallocFromPoolAsync(&p, bytes, pool, s) != hipSuccess) return nullptr;
+    return p;
+}
+
+void *dev_alloc(size_t bytes, hipStream_t s) {
     if (bytes == 0) bytes = 1;
     amgt_alloc_fn fn;
     void *ctx;
@@ -35,12 +72,10 @@ void *dev_alloc(size_t bytes) {
         ctx = g_alloc_ctx;
     }
     if (fn) return fn(bytes, ctx);
-    void *p = nullptr;
-    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
-    return p;
+    return pool_alloc(bytes, s);
 }
 
-void dev_free(void *p) {
+void dev_free(void *p, hipStream_t s) {
     if (!p) return;
     amgt_free_fn fn;
     void *ctx;
@@ -53,16 +88,15 @@ void dev_free(void *p) {
         fn(p, ctx);
         return;
     }
-    (void)hipFree(p);
+    (void)hipFreeAsync(p, s);
 }
 
 int tmp_alloc(void **p, size_t bytes, hipStream_t s) {
     *p = nullptr;
     if (bytes == 0) bytes = 1;
-    hipError_t e = hipMallocAsync(p, bytes, s);
-    if (e != hipSuccess) {
-        set_error(std::string("hipMallocAsync: ") + hipGetErrorString(e));
-        *p = nullptr;
+    *p = pool_alloc(bytes, s);
+    if (!*p) {
+        set_error("device pool allocation of " + std::to_string(bytes) + " bytes failed");
         return AMGT_ERR_ALLOC;
     }
     return AMGT_OK;
@@ -112,20 +146,31 @@ int exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t s
     return inclusive_into_exclusive<int64_t>(in, out, n, s);
 }
 
-void mbsr_release(Mbsr *A) {
+void mbsr_release_balanced(Mbsr *A, hipStream_t s) {
+    dev_free(A->bal_lo, s);
+    dev_free(A->bal_head, s);
+    dev_free(A->bal_vb, s);
+    dev_free(A->bal_carry, s);
+    A->bal_lo = A->bal_head = A->bal_vb = nullptr;
+    A->bal_carry = nullptr;
+    A->bal_capb = A->bal_ntiles = 0;
+}
+
+void mbsr_release(Mbsr *A, hipStream_t s) {
     if (!A) return;
+    mbsr_release_balanced(A, s);
     if (A->trans) {
-        mbsr_release(A->trans);
+        mbsr_release(A->trans, s);
         delete A->trans;
         A->trans = nullptr;
     }
-    dev_free(A->blk_ptr);
-    dev_free(A->blk_col);
-    dev_free(A->blk_map);
-    dev_free(A->val_ptr);
-    dev_free(A->val);
-    dev_free(A->val32);
-    dev_free(A->val16);
+    dev_free(A->blk_ptr, s);
+    dev_free(A->blk_col, s);
+    dev_free(A->blk_map, s);
+    dev_free(A->val_ptr, s);
+    dev_free(A->val, s);
+    dev_free(A->val32, s);
+    dev_free(A->val16, s);
     *A = Mbsr();
 }
 
@@ -190,10 +235,11 @@ int amgt_mbsr_create_from_csr(int32_t nrows, int32_t ncols, int64_t nnz, const i
     AMGT_CHECK_ARG(nrows >= 0 && ncols >= 0 && nnz >= 0, "negative size");
     AMGT_CHECK_ARG(d_rowptr != nullptr, "rowptr is NULL");
     AMGT_CHECK_ARG(nnz == 0 || d_colidx != nullptr, "colidx is NULL");
+    hipStream_t s = as_stream(stream);
     auto *h = new amgt_mbsr();
-    int st = csr_to_mbsr(nrows, ncols, nnz, d_rowptr, d_colidx, d_val, as_stream(stream), &h->m);
+    int st = csr_to_mbsr(nrows, ncols, nnz, d_rowptr, d_colidx, d_val, s, &h->m);
     if (st != AMGT_OK) {
-        mbsr_release(&h->m);
+        mbsr_release(&h->m, s);
         delete h;
         return st;
     }
@@ -203,8 +249,9 @@ int amgt_mbsr_create_from_csr(int32_t nrows, int32_t ncols, int64_t nnz, const i
 
 int amgt_mbsr_destroy(amgt_mbsr_t A) {
     if (!A) return AMGT_OK;
+    // no stream is given: wait for all work that may still read the arrays
     (void)hipDeviceSynchronize();
-    mbsr_release(&A->m);
+    mbsr_release(&A->m, nullptr);
     delete A;
     return AMGT_OK;
 }
@@ -240,7 +287,7 @@ int amgt_mbsr_get_info(amgt_mbsr_t A, amgt_mbsr_info_t *info) {
 
 int amgt_mbsr_set_spmv_kernel(amgt_mbsr_t A, int kind) {
     AMGT_CHECK_ARG(A != nullptr, "handle is NULL");
-    AMGT_CHECK_ARG(kind >= AMGT_SPMV_DIRECT && kind <= AMGT_SPMV_BLOCK, "unknown SpMV kernel kind");
+    AMGT_CHECK_ARG(kind >= AMGT_SPMV_DIRECT && kind <= AMGT_SPMV_BALANCED, "unknown SpMV kernel kind");
     A->m.spmv_kernel = kind;
     return AMGT_OK;
 }
@@ -293,18 +340,21 @@ int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream) {
     AMGT_HIP(hipEventCreate(&e1));
     const int cand[5] = {4, 8, 16, 32, 64};
     // kernels worth timing (the others stay selectable by hand)
-    const int kinds[3] = {kSpmvBlock, kSpmvStaged, kSpmvDirect};
-    float t_min[3][5];
+    const int kinds[4] = {kSpmvBlock, kSpmvBalanced, kSpmvStaged, kSpmvDirect};
+    float t_min[4][5];
     for (auto &row : t_min)
         for (float &v : row) v = 1e30f;
     // bring the clocks up before the first measurement
+    AMGT_TRY(spmv_prepare(&m, spmv_default_lanes(m), s));
     for (int r = 0; r < 30; ++r) AMGT_TRY(spmv(m, prec, spmv_default_lanes(m), 1.0, x, 0.0, nullptr, y, s));
     // two sweeps, keep each candidate's best time
     for (int sweep = 0; sweep < 2; ++sweep) {
-        for (int kk = 0; kk < 3; ++kk) {
+        for (int kk = 0; kk < 4; ++kk) {
             m.spmv_kernel = kinds[kk];
             for (int c = 0; c < 5; ++c) {
-                int st = AMGT_OK;
+                // the balanced kernel has three tile sizes (lanes 4, 8, 16)
+                if (kinds[kk] == kSpmvBalanced && cand[c] > 16) continue;
+                int st = spmv_prepare(&m, cand[c], s);
                 for (int r = 0; r < 3 && st == AMGT_OK; ++r)  // warm-up
                     st = spmv(m, prec, cand[c], 1.0, x, 0.0, nullptr, y, s);
                 if (st != AMGT_OK) continue;
@@ -321,7 +371,7 @@ int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream) {
     }
     float best = 1e30f;
     int best_lanes = spmv_default_lanes(m), best_kind = kinds[0];
-    for (int kk = 0; kk < 3; ++kk)
+    for (int kk = 0; kk < 4; ++kk)
         for (int c = 0; c < 5; ++c)
             if (t_min[kk][c] < best) {
                 best = t_min[kk][c];
@@ -334,7 +384,7 @@ int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream) {
     tmp_free(y, s);
     m.spmv_lanes = best_lanes;
     m.spmv_kernel = best_kind;
-    return AMGT_OK;
+    return spmv_prepare(&m, best_lanes, s);
 }
 
 static int lanes_of(const Mbsr &m) { return m.spmv_lanes ? m.spmv_lanes : spmv_default_lanes(m); }
@@ -344,7 +394,7 @@ static int get_transpose(Mbsr &m, hipStream_t s, Mbsr **t) {
         auto *tt = new Mbsr();
         int st = mbsr_transpose(m, s, tt);
         if (st != AMGT_OK) {
-            mbsr_release(tt);
+            mbsr_release(tt, s);
             delete tt;
             return st;
         }
@@ -373,6 +423,7 @@ int amgt_mbsr_spmv(amgt_mbsr_t A, int trans, double alpha, const double *d_x, do
     if (trans) AMGT_TRY(get_transpose(m, s, &op));
     int prec = effective_precision(m);
     AMGT_TRY(mbsr_ensure_lowp(op, prec, s));
+    AMGT_TRY(spmv_prepare(op, lanes_of(*op), s));
     return spmv(*op, prec, lanes_of(*op), alpha, d_x, beta, d_b, d_y, s);
 }
 
@@ -403,10 +454,11 @@ int amgt_mbsr_spgemm(amgt_mbsr_t A, amgt_mbsr_t B, void *stream, amgt_mbsr_t *C)
         set_error("amgt_mbsr_spgemm: A.ncols != B.nrows");
         return AMGT_ERR_DIM;
     }
+    hipStream_t s = as_stream(stream);
     auto *h = new amgt_mbsr();
-    int st = spgemm(A->m, B->m, as_stream(stream), &h->m);
+    int st = spgemm(A->m, B->m, s, &h->m);
     if (st != AMGT_OK) {
-        mbsr_release(&h->m);
+        mbsr_release(&h->m, s);
         delete h;
         return st;
     }
@@ -422,7 +474,10 @@ int amgt_mbsr_to_csr(amgt_mbsr_t A, void *stream, int64_t *nnz, int32_t **d_rowp
 }
 
 int amgt_free(void *ptr) {
-    dev_free(ptr);
+    if (!ptr) return AMGT_OK;
+    // no stream is given: wait for all work that may still use the array
+    (void)hipDeviceSynchronize();
+    dev_free(ptr, nullptr);
     return AMGT_OK;
 }
 
@@ -441,7 +496,7 @@ int amgt_csr_spgemm(int32_t m, int32_t k, int32_t n, int64_t nnzA, const int32_t
     if (C_cache) *C_cache = nullptr;
     if (nnzA == 0 || nnzB == 0) {
         // trivial case, csr_spgemm_device.c:2633-2638
-        int32_t *ci = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(m) + 1)));
+        int32_t *ci = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(m) + 1), s));
         if (!ci) {
             set_error("allocation of C row pointer failed");
             return AMGT_ERR_ALLOC;

@@ -13,6 +13,7 @@
 // lanes), <= 1024 (256-thread workgroup, LDS bitonic), <= 8192 (1024-thread
 // workgroup, LDS bitonic).  Larger block rows return AMGT_ERR_UNSUPPORTED.
 #include "internal.hpp"
+#include "rowpass.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -29,12 +30,9 @@ enum : int { kErrDup = 1, kErrCol = 2 };
 
 // Per-conversion device scratch counters.
 struct ConvCounters {
-    int32_t bin_count[4];
-    int32_t max_entries;
+    int32_t bin_count[4];  // host side only: rows per bin, [3] = 1 if a row is too large
     int32_t err;
-    int32_t max_blocks;
     int32_t pad;
-    double sq_dev;
 };
 
 __global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n) {
@@ -45,11 +43,8 @@ __global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n) {
 __global__ void k_zero_counters(ConvCounters *c) {
     if (threadIdx.x == 0) {
         for (int i = 0; i < 4; ++i) c->bin_count[i] = 0;
-        c->max_entries = 0;
         c->err = 0;
-        c->max_blocks = 0;
         c->pad = 0;
-        c->sq_dev = 0.0;
     }
 }
 
@@ -57,37 +52,12 @@ __device__ __forceinline__ int bin_of(int n) {
     return n <= 64 ? 0 : (n <= 1024 ? 1 : (n <= 8192 ? 2 : 3));
 }
 
-// Bins block rows by entry count; list[bin*brow + t] = block row.
-// One atomic per wave and bin (wave-aggregated): a single hot counter would
-// otherwise serialise every block row.
-__global__ __launch_bounds__(256) void k_bin_rows(int32_t nrows, int32_t brow,
-                                                  const int32_t *__restrict__ I,
-                                                  int32_t *__restrict__ list, ConvCounters *cnt) {
-    const int br = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    const bool valid = br < brow;
-    int n = 0, b = -1;
-    if (valid) {
-        const int r0 = 4 * br;
-        const int r1 = min(r0 + 4, nrows);
-        n = I[r1] - I[r0];
-        b = bin_of(n);
-    }
-    const uint64_t lt = (1ull << lane) - 1ull;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const uint64_t mask = __ballot(b == q);
-        if (mask == 0ull) continue;
-        const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&cnt->bin_count[q], __popcll(mask));
-        base = __shfl(base, leader);
-        if (b == q) list[int64_t(q) * brow + base + __popcll(mask & lt)] = br;
-    }
-    int mx = n;
-    for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off));
-    if (lane == 0 && mx > 0) atomicMax(&cnt->max_entries, mx);
-}
+// Entries of block row i (CSR rows 4i .. 4i+3), for the row passes.
+struct RowEntries {
+    const int32_t *I;
+    int32_t nrows;
+    __device__ int64_t operator()(int i) const { return I[min(4 * i + 4, nrows)] - I[4 * i]; }
+};
 
 // Key of CSR entry p in block row starting at row r0.
 __device__ __forceinline__ uint32_t entry_key(int p, int r0, int nrows, int ncols,
@@ -107,17 +77,17 @@ __device__ __forceinline__ uint32_t entry_key(int p, int r0, int nrows, int ncol
 // tmp_col/tmp_map receive the row's blocks at offset I[4br] (upper-bound slot).
 __global__ __launch_bounds__(256) void k_sort_wave(
     int32_t nrows, int32_t ncols, const int32_t *__restrict__ I, const int32_t *__restrict__ J,
-    const double *__restrict__ A, const int32_t *__restrict__ list, int32_t count,
-    double *__restrict__ val, int32_t *__restrict__ tmp_col, uint32_t *__restrict__ tmp_map,
+    const double *__restrict__ A, int32_t brow, double *__restrict__ val,
+    int32_t *__restrict__ tmp_col, uint32_t *__restrict__ tmp_map,
     int32_t *__restrict__ blk_cnt, ConvCounters *cnt, int values_only) {
     const int lane = threadIdx.x & 63;
-    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (w >= count) return;  // whole wave exits together
-    const int br = list[w];
+    const int br = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (br >= brow) return;  // whole wave exits together
     const int r0 = 4 * br;
     const int s = I[r0];
     const int e = I[min(r0 + 4, nrows)];
     const int n = e - s;
+    if (n > 64) return;  // listed for the workgroup sorts
     int err = 0;
     uint32_t key = kPadKey;
     int src = 0;
@@ -165,9 +135,7 @@ __global__ __launch_bounds__(256) void k_sort_wave(
             tmp_map[s + rank] = map;
         }
         if (lane == 0) {
-            int nb = __popcll(heads);
-            blk_cnt[br] = nb;
-            atomicMax(&cnt->max_blocks, nb);
+            blk_cnt[br] = __popcll(heads);
         }
     }
     if (err) atomicOr(&cnt->err, err);
@@ -271,10 +239,7 @@ __global__ __launch_bounds__(THREADS) void k_sort_block(
                 tmp_map[s + rank] = map;
             }
         }
-        if (t == 0) {
-            blk_cnt[br] = carry;
-            atomicMax(&cnt->max_blocks, carry);
-        }
+        if (t == 0) blk_cnt[br] = carry;
     }
     if (t == 0 && s_err) atomicOr(&cnt->err, s_err);
 }
@@ -305,22 +270,28 @@ __global__ __launch_bounds__(256) void k_compact(int32_t nrows, int32_t brow,
     if (lane == 0) val_ptr[br] = s;
 }
 
-// Sum of squared deviations of blocks per block row (getStand,
-// csr_matvec_device.c:638-671).
-__global__ __launch_bounds__(256) void k_stand(int32_t brow, const int32_t *__restrict__ blk_cnt,
-                                               double avg, ConvCounters *cnt) {
-    __shared__ double s_part[4];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double d = 0.0;
-    if (i < brow) {
-        double v = double(blk_cnt[i]) - avg;
-        d = v * v;
-    }
-    for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off);
-    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = d;
-    __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(&cnt->sq_dev, s_part[0] + s_part[1] + s_part[2] + s_part[3]);
+}  // namespace
+
+int row_stats_launch(const int32_t *blk_ptr, int32_t brow, RowStats *d_stats, hipStream_t s) {
+    rowpass::Part *parts = nullptr;
+    AMGT_TRY(tmp_alloc_t(&parts, size_t(rowpass::groups(brow)) + 1, s));
+    const int st = rowpass::count(brow, rowpass::BlockDiff{blk_ptr}, INT64_MAX, INT64_MAX, parts,
+                                  d_stats, s);
+    tmp_free(parts, s);
+    return st;
 }
+
+double stand_from(const RowStats &h, int64_t nblk, int32_t brow) {
+    // sqrt(sum (c - avg)^2 / brow) with sum (c - avg)^2 = (brow*sum c^2 - nblk^2) / brow,
+    // the numerator exact in 128-bit integers (the reference sums doubles with
+    // atomics and takes sqrtf, csr_matvec_device.c:1245-1249)
+    if (brow <= 0) return 0.0;
+    const __int128 num = static_cast<__int128>(h.sumsq) * brow - static_cast<__int128>(nblk) * nblk;
+    const double var = double(num) / (double(brow) * double(brow));
+    return var > 0.0 ? std::sqrt(var) : 0.0;
+}
+
+namespace {
 
 int launch_sorts(int32_t nrows, int32_t ncols, const int32_t *I, const int32_t *J, const double *A,
                  const int32_t *list, int32_t brow, const int32_t bin_count[4], double *val,
@@ -331,10 +302,10 @@ int launch_sorts(int32_t nrows, int32_t ncols, const int32_t *I, const int32_t *
                   "nonzeros on average); this build does not convert such rows");
         return AMGT_ERR_UNSUPPORTED;
     }
-    if (bin_count[0] > 0) {
-        int grid = (bin_count[0] + 3) / 4;
-        hipLaunchKernelGGL(k_sort_wave, dim3(grid), dim3(256), 0, s, nrows, ncols, I, J, A,
-                           list, bin_count[0], val, tmp_col, tmp_map, blk_cnt, cnt, values_only);
+    if (brow > 0) {
+        int grid = (brow + 3) / 4;
+        hipLaunchKernelGGL(k_sort_wave, dim3(grid), dim3(256), 0, s, nrows, ncols, I, J, A, brow,
+                           val, tmp_col, tmp_map, blk_cnt, cnt, values_only);
         AMGT_HIP(hipGetLastError());
     }
     if (bin_count[1] > 0) {
@@ -358,17 +329,39 @@ struct HostHeader {
 
 }  // namespace
 
+// Bins block rows by entries: <= 64 (wave sort, not listed: it walks every
+// block row), <= 1024 and larger (listed at list + brow and list + 2*brow for
+// the workgroup sorts).  bin_count[3] flags a block row over 8192 entries.
 static int bin_block_rows(int32_t nrows, int32_t brow, const int32_t *I, int32_t **list,
                           ConvCounters *cnt, ConvCounters *h_cnt, hipStream_t s) {
-    AMGT_TRY(tmp_alloc_t(list, size_t(brow) * 4, s));
+    AMGT_TRY(tmp_alloc_t(list, size_t(brow) * 3 + 1, s));
+    rowpass::Part *parts = nullptr;
+    RowStats *tot = nullptr;
+    AMGT_TRY(tmp_alloc_t(&parts, size_t(rowpass::groups(brow)) + 1, s));
+    AMGT_TRY(tmp_alloc_t(&tot, 1, s));
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, cnt);
-    if (brow > 0)
-        hipLaunchKernelGGL(k_bin_rows, dim3((brow + 255) / 256), dim3(256), 0, s, nrows, brow, I,
-                           *list, cnt);
     AMGT_HIP(hipGetLastError());
-    AMGT_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof(ConvCounters), hipMemcpyDeviceToHost, s));
-    AMGT_HIP(hipStreamSynchronize(s));
-    return AMGT_OK;
+    const RowEntries f{I, nrows};
+    int st = rowpass::count(brow, f, 64, 1024, parts, tot, s);
+    RowStats ht{};
+    if (st == AMGT_OK) {
+        hipError_t e = hipMemcpyAsync(&ht, tot, sizeof(RowStats), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            set_error(std::string("CSR->mBSR: ") + hipGetErrorString(e));
+            st = AMGT_ERR_HIP;
+        }
+    }
+    *h_cnt = ConvCounters{};
+    h_cnt->bin_count[0] = ht.cnt[0];
+    h_cnt->bin_count[1] = ht.cnt[1];
+    h_cnt->bin_count[2] = ht.cnt[2];
+    h_cnt->bin_count[3] = ht.max_blocks > 8192 ? 1 : 0;
+    if (st == AMGT_OK && h_cnt->bin_count[3] == 0 && ht.cnt[1] + ht.cnt[2] > 0)
+        st = rowpass::scatter(brow, f, 64, 1024, parts, *list, int64_t(brow), 6u, s);
+    tmp_free(parts, s);
+    tmp_free(tot, s);
+    return st;
 }
 
 int csr_to_mbsr(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *rowptr,
@@ -396,14 +389,14 @@ int csr_to_mbsr(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *rowptr
     AMGT_HIP(hipMemcpyAsync(&hh.i0, rowptr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     AMGT_HIP(hipMemcpyAsync(&hh.in, rowptr + nrows, sizeof(int32_t), hipMemcpyDeviceToHost, s));
 
-    m.blk_ptr = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(m.brow) + 1)));
-    m.val_ptr = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(m.brow) + 1)));
+    m.blk_ptr = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(m.brow) + 1), s));
+    m.val_ptr = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(m.brow) + 1), s));
     if (!m.blk_ptr || !m.val_ptr) {
         set_error("CSR->mBSR: allocation failed");
         return AMGT_ERR_ALLOC;
     }
     if (!m.pattern_only) {
-        m.val = static_cast<double *>(dev_alloc(sizeof(double) * size_t(nnz ? nnz : 1)));
+        m.val = static_cast<double *>(dev_alloc(sizeof(double) * size_t(nnz ? nnz : 1), s));
         if (!m.val) {
             set_error("CSR->mBSR: value allocation failed");
             return AMGT_ERR_ALLOC;
@@ -426,7 +419,10 @@ int csr_to_mbsr(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *rowptr
     }
     int32_t *tmp_col = nullptr, *blk_cnt = nullptr;
     uint32_t *tmp_map = nullptr;
-    st = tmp_alloc_t(&tmp_col, size_t(nnz), s);
+    RowStats *rstats = nullptr;
+    RowStats hrs{};
+    st = tmp_alloc_t(&rstats, 1, s);
+    if (st == AMGT_OK) st = tmp_alloc_t(&tmp_col, size_t(nnz), s);
     if (st == AMGT_OK) st = tmp_alloc_t(&tmp_map, size_t(nnz), s);
     if (st == AMGT_OK) st = tmp_alloc_t(&blk_cnt, size_t(m.brow) + 1, s);
     if (st == AMGT_OK && m.brow > 0) {
@@ -448,8 +444,8 @@ int csr_to_mbsr(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *rowptr
     }
     if (st == AMGT_OK) {
         m.nblk = nblk;
-        m.blk_col = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * size_t(nblk ? nblk : 1)));
-        m.blk_map = static_cast<uint16_t *>(dev_alloc(sizeof(uint16_t) * size_t(nblk ? nblk : 1)));
+        m.blk_col = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * size_t(nblk ? nblk : 1), s));
+        m.blk_map = static_cast<uint16_t *>(dev_alloc(sizeof(uint16_t) * size_t(nblk ? nblk : 1), s));
         if (!m.blk_col || !m.blk_map) {
             set_error("CSR->mBSR: block allocation failed");
             st = AMGT_ERR_ALLOC;
@@ -458,23 +454,21 @@ int csr_to_mbsr(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *rowptr
     if (st == AMGT_OK) {
         hipLaunchKernelGGL(k_compact, dim3((m.brow + 1 + 3) / 4), dim3(256), 0, s, nrows, m.brow,
                            rowptr, m.blk_ptr, tmp_col, tmp_map, m.blk_col, m.blk_map, m.val_ptr);
-        double avg = m.brow ? double(nblk) / double(m.brow) : 0.0;
-        if (m.brow > 0)
-            hipLaunchKernelGGL(k_stand, dim3((m.brow + 255) / 256), dim3(256), 0, s, m.brow,
-                               blk_cnt, avg, cnt);
+        st = row_stats_launch(m.blk_ptr, m.brow, rstats, s);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess)
             e = hipMemcpyAsync(&hc, cnt, sizeof(ConvCounters), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(&hrs, rstats, sizeof(RowStats), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess) {
+        if (st == AMGT_OK && e != hipSuccess) {
             set_error(std::string("CSR->mBSR: ") + hipGetErrorString(e));
             st = AMGT_ERR_HIP;
-        } else {
-            // reference computes sqrtf on the double sum (csr_matvec_device.c:1249);
-            // here the square root is taken in double
-            m.stand = m.brow ? std::sqrt(hc.sq_dev / double(m.brow)) : 0.0;
+        } else if (st == AMGT_OK) {
+            m.stand = stand_from(hrs, nblk, m.brow);
             m.avg_nnz = nblk ? double(nnz) / double(nblk) : 0.0;
-            m.max_row_blocks = hc.max_blocks;
+            m.max_row_blocks = hrs.max_blocks;
+            m.spmv_kernel = default_spmv_kernel(m.stand);
             if (hc.err & kErrCol) {
                 set_error("CSR->mBSR: column index out of range");
                 st = AMGT_ERR_INPUT;
@@ -487,6 +481,7 @@ int csr_to_mbsr(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *rowptr
     tmp_free(tmp_col, s);
     tmp_free(tmp_map, s);
     tmp_free(blk_cnt, s);
+    tmp_free(rstats, s);
     tmp_free(list, s);
     tmp_free(cnt, s);
     return st;
@@ -496,7 +491,7 @@ int mbsr_refresh_values(Mbsr *Ap, const int32_t *rowptr, const int32_t *colidx,
                         const double *A, hipStream_t s) {
     Mbsr &m = *Ap;
     if (m.pattern_only || !m.val) {
-        m.val = static_cast<double *>(dev_alloc(sizeof(double) * size_t(m.nnz ? m.nnz : 1)));
+        m.val = static_cast<double *>(dev_alloc(sizeof(double) * size_t(m.nnz ? m.nnz : 1), s));
         if (!m.val) {
             set_error("value allocation failed");
             return AMGT_ERR_ALLOC;
@@ -514,12 +509,12 @@ int mbsr_refresh_values(Mbsr *Ap, const int32_t *rowptr, const int32_t *colidx,
     tmp_free(list, s);
     tmp_free(cnt, s);
     // low-precision copies and the transpose are stale now
-    dev_free(m.val32);
+    dev_free(m.val32, s);
     m.val32 = nullptr;
-    dev_free(m.val16);
+    dev_free(m.val16, s);
     m.val16 = nullptr;
     if (m.trans) {
-        mbsr_release(m.trans);
+        mbsr_release(m.trans, s);
         delete m.trans;
         m.trans = nullptr;
     }

@@ -45,10 +45,11 @@ struct Status {
     } while (0)
 
 // ---- allocation -------------------------------------------------------------
-// Long-lived arrays (handle arrays, CSR exports) go through the user hooks.
-void *dev_alloc(size_t bytes);
-void dev_free(void *p);
-// Stream-ordered temporaries.
+// Long-lived arrays (handle arrays, CSR exports) go through the user hooks
+// when set, else the library's device pool, stream-ordered on s.
+void *dev_alloc(size_t bytes, hipStream_t s);
+void dev_free(void *p, hipStream_t s);
+// Stream-ordered temporaries from the library's device pool.
 int tmp_alloc(void **p, size_t bytes, hipStream_t s);
 void tmp_free(void *p, hipStream_t s);
 
@@ -76,6 +77,10 @@ struct Mbsr {
     int32_t spmv_lanes = 0;   // 0: heuristic
     int32_t spmv_kernel = 2;  // kSpmv* kind (amgt_mbsr_set_spmv_kernel); block-lane by default
     Mbsr *trans = nullptr;    // cached A^T
+    // tile tables of the balanced SpMV (built on first use, for one tile size)
+    int32_t bal_capb = 0, bal_ntiles = 0;
+    int32_t *bal_lo = nullptr, *bal_head = nullptr, *bal_vb = nullptr;
+    double *bal_carry = nullptr;  // per-tile partial sums of rows cut by tile edges
 };
 
 }  // namespace amgt
@@ -89,7 +94,14 @@ namespace amgt {
 bool mixed_precision_enabled();
 
 // SpMV kernel kinds (amgt_mbsr_set_spmv_kernel)
-enum : int { kSpmvDirect = 0, kSpmvStaged = 1, kSpmvBlock = 2 };
+enum : int { kSpmvDirect = 0, kSpmvStaged = 1, kSpmvBlock = 2, kSpmvBalanced = 3 };
+
+// Spread of blocks per block row above which the balanced SpMV is the default
+// (the reference's switch, csr_matvec_device.c:1336)
+constexpr double kBalancedStand = 12.0;
+inline int default_spmv_kernel(double stand) {
+    return stand >= kBalancedStand ? kSpmvBalanced : kSpmvBlock;
+}
 
 // ---- component entry points (each in its own translation unit) -------------
 int csr_to_mbsr(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *rowptr,
@@ -98,6 +110,11 @@ int mbsr_refresh_values(Mbsr *A, const int32_t *rowptr, const int32_t *colidx,
                         const double *val, hipStream_t s);
 int mbsr_transpose(const Mbsr &A, hipStream_t s, Mbsr *out);
 int mbsr_ensure_lowp(Mbsr *A, int precision, hipStream_t s);
+// builds what the handle's SpMV kernel kind needs for this lanes setting (the
+// balanced kernel's tile tables); no-op for the other kinds
+int spmv_prepare(Mbsr *A, int lanes, hipStream_t s);
+int balanced_bpt(int lanes);
+void mbsr_release_balanced(Mbsr *A, hipStream_t s);
 int spmv(const Mbsr &A, int precision, int lanes, double alpha, const double *x, double beta,
          const double *b, double *y, hipStream_t s);
 int spmm(const Mbsr &A, int precision, int lanes, int k, double alpha, const double *x,
@@ -109,7 +126,19 @@ int scale_vector(int32_t n, double beta, const double *b, double *y, hipStream_t
 int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *C);
 int mbsr_to_csr(const Mbsr &A, hipStream_t s, int64_t *nnz, int32_t **rowptr, int32_t **colidx,
                 double **val);
-void mbsr_release(Mbsr *A);
+void mbsr_release(Mbsr *A, hipStream_t s);
+
+// Totals of a row pass (rowpass.hpp): rows per bin, the largest value and the
+// sum of squares.  For blocks per block row (row_stats_launch) max_blocks and
+// sumsq give max_row_blocks and, through stand_from, the standard deviation
+// of blocks per block row.
+struct RowStats {
+    int32_t cnt[3];
+    int32_t max_blocks;
+    unsigned long long sumsq;
+};
+int row_stats_launch(const int32_t *blk_ptr, int32_t brow, RowStats *d_stats, hipStream_t s);
+double stand_from(const RowStats &h, int64_t nblk, int32_t brow);
 
 // exclusive scan of n int32 counts into out[0..n] (out[n] = total); int64 total
 // is returned through *total if non-null (synchronises the stream)

@@ -86,7 +86,7 @@ int mbsr_to_csr(const Mbsr &M, hipStream_t s, int64_t *nnz, int32_t **rowptr, in
     *rowptr = nullptr;
     *colidx = nullptr;
     *val = nullptr;
-    int32_t *I = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(M.nrows) + 1)));
+    int32_t *I = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(M.nrows) + 1), s));
     if (!I) {
         set_error("CSR export: row pointer allocation failed");
         return AMGT_ERR_ALLOC;
@@ -94,7 +94,7 @@ int mbsr_to_csr(const Mbsr &M, hipStream_t s, int64_t *nnz, int32_t **rowptr, in
     int32_t *cnt = nullptr;
     int st = tmp_alloc_t(&cnt, size_t(M.nrows) + 1, s);
     if (st != AMGT_OK) {
-        dev_free(I);
+        dev_free(I, s);
         return st;
     }
     if (M.nrows > 0)
@@ -103,18 +103,18 @@ int mbsr_to_csr(const Mbsr &M, hipStream_t s, int64_t *nnz, int32_t **rowptr, in
     st = exclusive_scan_i32(cnt, I, M.nrows, s);
     tmp_free(cnt, s);
     if (st != AMGT_OK) {
-        dev_free(I);
+        dev_free(I, s);
         return st;
     }
     // nnz of the export equals the handle's nnz (sum of popcounts)
     const int64_t n = M.nnz;
-    int32_t *J = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * size_t(n ? n : 1)));
+    int32_t *J = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * size_t(n ? n : 1), s));
     double *A = nullptr;
-    if (!M.pattern_only) A = static_cast<double *>(dev_alloc(sizeof(double) * size_t(n ? n : 1)));
+    if (!M.pattern_only) A = static_cast<double *>(dev_alloc(sizeof(double) * size_t(n ? n : 1), s));
     if (!J || (!M.pattern_only && !A)) {
-        dev_free(I);
-        dev_free(J);
-        dev_free(A);
+        dev_free(I, s);
+        dev_free(J, s);
+        dev_free(A, s);
         set_error("CSR export: allocation failed");
         return AMGT_ERR_ALLOC;
     }
@@ -123,9 +123,9 @@ int mbsr_to_csr(const Mbsr &M, hipStream_t s, int64_t *nnz, int32_t **rowptr, in
                            M.blk_ptr, M.blk_col, M.blk_map, M.val_ptr, M.val, I, J, A);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
-        dev_free(I);
-        dev_free(J);
-        dev_free(A);
+        dev_free(I, s);
+        dev_free(J, s);
+        dev_free(A, s);
         set_error(std::string("CSR export: ") + hipGetErrorString(e));
         return AMGT_ERR_HIP;
     }

@@ -22,6 +22,7 @@
 //    are distinct: no atomics, deterministic results.
 // 5. accumulators are compressed to the mBSR value layout.
 #include "internal.hpp"
+#include "rowpass.hpp"
 
 #include <algorithm>
 #include <string>
@@ -262,85 +263,23 @@ __global__ __launch_bounds__(256) void k_symbolic_global(int count, const int32_
     }
 }
 
-struct Counters {
-    int32_t bin[3];
-    int32_t max_ub;
-    int32_t big_rows;   // numeric rows with more than kNumCap blocks
-    int32_t max_blocks;
-    int64_t total_ub;
-    int64_t big_blocks;
-};
-
-__global__ void k_zero_counters(Counters *c) {
-    if (threadIdx.x == 0) {
-        c->bin[0] = c->bin[1] = c->bin[2] = 0;
-        c->max_ub = 0;
-        c->big_rows = 0;
-        c->max_blocks = 0;
-        c->total_ub = 0;
-        c->big_blocks = 0;
-    }
-}
-
-__device__ __forceinline__ int wave_max(int v) {
-    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
-    return v;
-}
-
+// ub_i = sum over A blocks (i,k) of |B row k|; the workgroup's partial bins
+// rows by ub (<= kBin0Ub, <= kBin1Ub, larger) and keeps the largest ub.
 __global__ __launch_bounds__(256) void k_upper_bound(int32_t brow, const int32_t *__restrict__ ap,
                                                      const int32_t *__restrict__ ac,
                                                      const int32_t *__restrict__ bp,
-                                                     int64_t *__restrict__ ub, Counters *c) {
+                                                     int64_t *__restrict__ ub,
+                                                     rowpass::Part *parts) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int v = 0;
+    int64_t v = 0;
     if (i < brow) {
-        int64_t s = 0;
         for (int p = ap[i]; p < ap[i + 1]; ++p) {
             const int k = ac[p];
-            s += bp[k + 1] - bp[k];
+            v += bp[k + 1] - bp[k];
         }
-        ub[i] = s;
-        v = s > 0x7fffffff ? 0x7fffffff : int(s);
+        ub[i] = v;
     }
-    v = wave_max(v);
-    if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(&c->max_ub, v);
-}
-
-// wave-aggregated: one atomic per wave and bin
-__global__ __launch_bounds__(256) void k_bin(int32_t brow, const int64_t *__restrict__ ub,
-                                             int32_t *__restrict__ list, Counters *c) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    int b = -1;
-    if (i < brow) {
-        const int64_t u = ub[i];
-        b = u <= kBin0Ub ? 0 : (u <= kBin1Ub ? 1 : 2);
-    }
-    const uint64_t lt = (1ull << lane) - 1ull;
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        const uint64_t mask = __ballot(b == q);
-        if (mask == 0ull) continue;
-        const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&c->bin[q], __popcll(mask));
-        base = __shfl(base, leader);
-        if (b == q) list[int64_t(q) * brow + base + __popcll(mask & lt)] = i;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_c_stats(int32_t brow, const int32_t *__restrict__ cnt,
-                                                 int32_t *__restrict__ big, Counters *c) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = (i < brow) ? cnt[i] : 0;
-    const int mx = wave_max(n);
-    if ((threadIdx.x & 63) == 0 && mx > 0) atomicMax(&c->max_blocks, mx);
-    if (n > kNumCap) {
-        const int pos = atomicAdd(&c->big_rows, 1);
-        big[pos] = i;
-        atomicAdd(reinterpret_cast<unsigned long long *>(&c->big_blocks),
-                  static_cast<unsigned long long>(n));
-    }
+    rowpass::write_part(i < brow, v, kBin0Ub, kBin1Ub, parts);
 }
 
 __global__ __launch_bounds__(256) void k_compact_c(int32_t brow, const int64_t *__restrict__ ub_off,
@@ -477,11 +416,6 @@ __global__ __launch_bounds__(256) void k_numeric_global(int count, const int32_t
     }
 }
 
-__global__ void k_max_row(int32_t n, const int32_t *__restrict__ ptr, int32_t *__restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicMax(out, ptr[i + 1] - ptr[i]);
-}
-
 }  // namespace
 
 int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *Cp) {
@@ -497,8 +431,8 @@ int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *Cp) {
         return AMGT_ERR_DIM;
     }
     const int32_t brow = A.brow;
-    C.blk_ptr = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(brow) + 1)));
-    C.val_ptr = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(brow) + 1)));
+    C.blk_ptr = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(brow) + 1), s));
+    C.val_ptr = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(brow) + 1), s));
     if (!C.blk_ptr || !C.val_ptr) {
         set_error("SpGEMM: allocation failed");
         return AMGT_ERR_ALLOC;
@@ -506,16 +440,18 @@ int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *Cp) {
     SpgemmArgs g{A.blk_ptr, A.blk_col, A.val_ptr, A.blk_map, A.val,
                  B.blk_ptr, B.blk_col, B.val_ptr, B.blk_map, B.val};
 
-    Counters *cnt = nullptr;
+    rowpass::Part *parts = nullptr;
+    RowStats *tots = nullptr;  // [0]: upper-bound pass, [1]: C block counts
+    RowStats ht[2] = {};
     int64_t *ub = nullptr, *ub_off = nullptr;
     int32_t *list = nullptr, *ccnt = nullptr, *cnnz = nullptr, *big = nullptr;
     int32_t *tcol = nullptr;
     uint32_t *tmap = nullptr, *slab = nullptr;
     double *nslab = nullptr;
     int st = AMGT_OK;
-    Counters hc{};
     auto cleanup = [&]() {
-        tmp_free(cnt, s);
+        tmp_free(parts, s);
+        tmp_free(tots, s);
         tmp_free(ub, s);
         tmp_free(ub_off, s);
         tmp_free(list, s);
@@ -545,40 +481,43 @@ int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *Cp) {
         }                                                                         \
     } while (0)
 
-    SG_TRY(tmp_alloc_t(&cnt, 1, s));
+    SG_TRY(tmp_alloc_t(&parts, size_t(rowpass::groups(brow)) + 1, s));
+    SG_TRY(tmp_alloc_t(&tots, 2, s));
     SG_TRY(tmp_alloc_t(&ub, size_t(brow) + 1, s));
     SG_TRY(tmp_alloc_t(&ub_off, size_t(brow) + 1, s));
     SG_TRY(tmp_alloc_t(&list, size_t(brow) * 3 + 1, s));
     SG_TRY(tmp_alloc_t(&ccnt, size_t(brow) + 1, s));
     SG_TRY(tmp_alloc_t(&cnnz, size_t(brow) + 1, s));
     SG_TRY(tmp_alloc_t(&big, size_t(brow) + 1, s));
-    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, cnt);
-    const unsigned gr = unsigned((brow + 255) / 256);
-    if (brow > 0) {
+    const unsigned gr = unsigned(rowpass::groups(brow));
+    if (brow > 0)
         hipLaunchKernelGGL(k_upper_bound, dim3(gr), dim3(256), 0, s, brow, A.blk_ptr, A.blk_col,
-                           B.blk_ptr, ub, cnt);
-        hipLaunchKernelGGL(k_bin, dim3(gr), dim3(256), 0, s, brow, ub, list, cnt);
-    }
+                           B.blk_ptr, ub, parts);
+    hipLaunchKernelGGL(rowpass::k_pass_finish, dim3(1), dim3(256), 0, s, int32_t(gr), parts,
+                       &tots[0]);
     SG_HIP(hipGetLastError());
+    SG_TRY(rowpass::scatter(brow, rowpass::ArrayI64{ub}, kBin0Ub, kBin1Ub, parts, list,
+                            int64_t(brow), 7u, s));
     SG_TRY(exclusive_scan_i64(ub, ub_off, brow, s));
     int64_t total_ub = 0;
     SG_HIP(hipMemcpyAsync(&total_ub, ub_off + brow, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    SG_HIP(hipMemcpyAsync(&hc, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    SG_HIP(hipMemcpyAsync(&ht[0], &tots[0], sizeof(RowStats), hipMemcpyDeviceToHost, s));
     SG_HIP(hipStreamSynchronize(s));  // sync 1: sizes of the temporaries and bins
+    const int32_t *bin = ht[0].cnt;
     SG_TRY(tmp_alloc_t(&tcol, size_t(total_ub), s));
     SG_TRY(tmp_alloc_t(&tmap, size_t(total_ub), s));
-    if (hc.bin[0] > 0)
-        hipLaunchKernelGGL(k_symbolic_wave, dim3((hc.bin[0] + 3) / 4), dim3(256), 0, s, hc.bin[0],
+    if (bin[0] > 0)
+        hipLaunchKernelGGL(k_symbolic_wave, dim3((bin[0] + 3) / 4), dim3(256), 0, s, bin[0],
                            list, g, ub_off, tcol, tmap, ccnt, cnnz);
-    if (hc.bin[1] > 0)
-        hipLaunchKernelGGL(k_symbolic_block, dim3(hc.bin[1]), dim3(256), 0, s, hc.bin[1],
+    if (bin[1] > 0)
+        hipLaunchKernelGGL(k_symbolic_block, dim3(bin[1]), dim3(256), 0, s, bin[1],
                            list + brow, g, ub_off, tcol, tmap, ccnt, cnnz);
-    if (hc.bin[2] > 0) {
+    if (bin[2] > 0) {
         int hs = 1;
-        const int64_t want = std::min<int64_t>(2 * int64_t(hc.max_ub), 2 * int64_t(B.bcol) + 2);
+        const int64_t want = std::min<int64_t>(2 * int64_t(ht[0].max_blocks), 2 * int64_t(B.bcol) + 2);
         while (hs < want) hs <<= 1;
         const int64_t per_wg = 2 * int64_t(hs);
-        int nwg = std::min(hc.bin[2], 512);
+        int nwg = std::min(bin[2], 512);
         const int64_t budget = int64_t(1) << 29;  // 2 GiB of uint32 slots
         while (nwg > 1 && per_wg * nwg > budget) nwg >>= 1;
         if (per_wg > budget) {
@@ -587,28 +526,33 @@ int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *Cp) {
             return AMGT_ERR_UNSUPPORTED;
         }
         SG_TRY(tmp_alloc_t(&slab, size_t(per_wg * nwg), s));
-        hipLaunchKernelGGL(k_symbolic_global, dim3(nwg), dim3(256), 0, s, hc.bin[2],
+        hipLaunchKernelGGL(k_symbolic_global, dim3(nwg), dim3(256), 0, s, bin[2],
                            list + 2 * int64_t(brow), g, ub_off, tcol, tmap, ccnt, cnnz, slab, hs);
     }
     SG_HIP(hipGetLastError());
     SG_TRY(exclusive_scan_i32(ccnt, C.blk_ptr, brow, s));
     SG_TRY(exclusive_scan_i32(cnnz, C.val_ptr, brow, s));
-    if (brow > 0)
-        hipLaunchKernelGGL(k_c_stats, dim3(gr), dim3(256), 0, s, brow, ccnt, big, cnt);
-    SG_HIP(hipGetLastError());
+    // C block counts: rows over kNumCap go to the global-memory numeric kernel;
+    // largest count and sum of squares give max_row_blocks and stand
+    SG_TRY(rowpass::count(brow, rowpass::ArrayI32{ccnt}, kNumCap, INT64_MAX, parts, &tots[1], s));
+    SG_TRY(rowpass::scatter(brow, rowpass::ArrayI32{ccnt}, kNumCap, INT64_MAX, parts, big, 0,
+                            2u, s));
     int32_t nblk = 0, nnz = 0;
     SG_HIP(hipMemcpyAsync(&nblk, C.blk_ptr + brow, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     SG_HIP(hipMemcpyAsync(&nnz, C.val_ptr + brow, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    SG_HIP(hipMemcpyAsync(&hc, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    SG_HIP(hipMemcpyAsync(&ht[1], &tots[1], sizeof(RowStats), hipMemcpyDeviceToHost, s));
     SG_HIP(hipStreamSynchronize(s));  // sync 2: sizes of C
+    const int32_t big_rows = ht[1].cnt[1];
     C.nblk = nblk;
     C.nnz = nnz;
-    C.max_row_blocks = hc.max_blocks;
+    C.max_row_blocks = ht[1].max_blocks;
     C.avg_nnz = nblk ? double(nnz) / double(nblk) : 0.0;
-    C.blk_col = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * size_t(nblk ? nblk : 1)));
-    C.blk_map = static_cast<uint16_t *>(dev_alloc(sizeof(uint16_t) * size_t(nblk ? nblk : 1)));
+    C.stand = stand_from(ht[1], nblk, brow);
+    C.spmv_kernel = default_spmv_kernel(C.stand);
+    C.blk_col = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * size_t(nblk ? nblk : 1), s));
+    C.blk_map = static_cast<uint16_t *>(dev_alloc(sizeof(uint16_t) * size_t(nblk ? nblk : 1), s));
     if (!C.pattern_only)
-        C.val = static_cast<double *>(dev_alloc(sizeof(double) * size_t(nnz ? nnz : 1)));
+        C.val = static_cast<double *>(dev_alloc(sizeof(double) * size_t(nnz ? nnz : 1), s));
     if (!C.blk_col || !C.blk_map || (!C.pattern_only && !C.val)) {
         set_error("SpGEMM: allocation of C failed");
         cleanup();
@@ -621,22 +565,17 @@ int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *Cp) {
     if (!C.pattern_only && brow > 0) {
         hipLaunchKernelGGL(k_numeric_lds, dim3((brow + 3) / 4), dim3(256), 0, s, brow, g,
                            C.blk_ptr, C.blk_col, C.blk_map, C.val_ptr, C.val);
-        if (hc.big_rows > 0) {
-            const int64_t elems = int64_t(hc.max_blocks) * 16;
-            int nwaves = std::min(hc.big_rows, 1024);
+        if (big_rows > 0) {
+            const int64_t elems = int64_t(C.max_row_blocks) * 16;
+            int nwaves = std::min(big_rows, 1024);
             while (nwaves > 4 && elems * nwaves * 8 > (int64_t(1) << 31)) nwaves >>= 1;
             nwaves = (nwaves + 3) / 4 * 4;  // every wave of the grid owns a slab
             SG_TRY(tmp_alloc_t(&nslab, size_t(elems * nwaves), s));
             hipLaunchKernelGGL(k_numeric_global, dim3((nwaves + 3) / 4), dim3(256), 0, s,
-                               hc.big_rows, big, g, C.blk_ptr, C.blk_col, C.blk_map, C.val_ptr,
+                               big_rows, big, g, C.blk_ptr, C.blk_col, C.blk_map, C.val_ptr,
                                C.val, nslab, elems);
         }
         SG_HIP(hipGetLastError());
-    }
-    // block-row length statistics of C
-    if (brow > 0 && nblk > 0) {
-        double avg = double(nblk) / double(brow);
-        (void)avg;
     }
     cleanup();
 #undef SG_TRY

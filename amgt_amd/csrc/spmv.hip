@@ -408,6 +408,218 @@ __global__ __launch_bounds__(256) void k_spmv_blk(
     }
 }
 
+// Balanced variant (counterpart of bsr_spmv_balanced_tc_fp64,
+// csr_matvec_device.c:556-613, with the split of BSR_BALANCED_PREPROCESS_GPU
+// :1269-1300): workgroup t owns blocks [t*CAPB, (t+1)*CAPB) whatever rows they
+// belong to, so a block row of 10^5 blocks is spread over many workgroups and
+// no workgroup waits on a long row.  The products are the block-lane ones
+// (lane per block, whole x segment, LDS row reduction).  A row that lies
+// entirely in the tile is stored to y; a row cut by a tile edge leaves its
+// partial sums in carry[t] (slots 0-3: the row the tile starts inside, slots
+// 4-7: the row the tile ends inside) and k_spmv_bal_fix adds them in tile
+// order.  No atomics: results are deterministic.
+//   tile_lo[t]   first block row whose first block lies in tile t
+//                (tile_lo[ntiles] = brow, so trailing empty rows belong to the
+//                last tile)
+//   tile_head[t] block row that starts before the tile and continues into it,
+//                or -1
+//   tile_vb[t]   value offset of block t*CAPB (tile_vb[ntiles] = nnz)
+// Values are staged in LDS when the tile holds at most 8 per block on
+// average; denser tiles read them from global memory.  x must be 16-byte
+// aligned.
+template <int BPT, typename VT, typename AT>
+__global__ __launch_bounds__(256) void k_spmv_bal(
+    int32_t ntiles, int32_t nblk, int32_t nrows, int32_t ncols,
+    const int32_t *__restrict__ blk_ptr, const int32_t *__restrict__ blk_col,
+    const uint16_t *__restrict__ blk_map, const VT *__restrict__ val,
+    const int32_t *__restrict__ tile_lo, const int32_t *__restrict__ tile_head,
+    const int32_t *__restrict__ tile_vb, double *__restrict__ carry, const double *__restrict__ x,
+    double alpha, double beta, const double *b, double *y) {
+    constexpr int CAPB = 256 * BPT, CAPV = 8 * CAPB, VPT = CAPV / 256;
+    __shared__ AT s_buf[CAPV];
+    __shared__ int32_t s_sc[4];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int t = remap_wg(blockIdx.x, gridDim.x);
+    const int q0 = t * CAPB;
+    const int q1 = int(min(int64_t(q0) + CAPB, int64_t(nblk)));
+    const int nb = q1 - q0;
+    const int vb = tile_vb[t];
+    const int nv = tile_vb[t + 1] - vb;
+    const bool staged = nv <= CAPV;
+    uint32_t m[BPT];
+    int c4[BPT];
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) {
+        const int q = BPT * tid + u;
+        m[u] = 0u;
+        c4[u] = 0;
+        if (q < nb) {
+            m[u] = blk_map[q0 + q];
+            c4[u] = 4 * blk_col[q0 + q];
+        }
+    }
+    if (staged) {
+        VT tv[VPT];
+#pragma unroll
+        for (int u = 0; u < VPT; ++u) {
+            const int q = u * 256 + tid;
+            if (q < nv) tv[u] = val[vb + q];
+        }
+#pragma unroll
+        for (int u = 0; u < VPT; ++u) {
+            const int q = u * 256 + tid;
+            if (q < nv) s_buf[q] = AT(tv[u]);
+        }
+    }
+    double xs[BPT][4];
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) {
+        xs[u][0] = xs[u][1] = xs[u][2] = xs[u][3] = 0.0;
+        if (m[u]) {
+            const double *xp = x + c4[u];
+            if (c4[u] + 4 <= ncols) {
+                const double2 lo2 = *reinterpret_cast<const double2 *>(xp);
+                const double2 hi2 = *reinterpret_cast<const double2 *>(xp + 2);
+                xs[u][0] = lo2.x;
+                xs[u][1] = lo2.y;
+                xs[u][2] = hi2.x;
+                xs[u][3] = hi2.y;
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    if (c4[u] + c < ncols) xs[u][c] = xp[c];
+            }
+        }
+    }
+    int mine = 0;
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) mine += __popc(m[u]);
+    int incl = mine;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(incl, d);
+        if (lane >= d) incl += v;
+    }
+    if (lane == 63) s_sc[tid >> 6] = incl;
+    __syncthreads();  // also publishes the staged values
+    int o = incl - mine;
+    for (int q = 0; q < (tid >> 6); ++q) o += s_sc[q];
+    AT part[BPT][4];
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) {
+        part[u][0] = part[u][1] = part[u][2] = part[u][3] = AT(0);
+#pragma unroll
+        for (int bit = 0; bit < 16; ++bit) {
+            if ((m[u] >> bit) & 1u) {
+                const AT v = staged ? s_buf[o] : AT(val[vb + o]);
+                part[u][bit >> 2] = mul_acc<AT, AT>(part[u][bit >> 2], v, xs[u][bit & 3]);
+                ++o;
+            }
+        }
+    }
+    __syncthreads();  // every staged value read: the buffer takes 4 partials per block
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) {
+        const int q = BPT * tid + u;
+        if (q < nb) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) s_buf[4 * q + r] = part[u][r];
+        }
+    }
+    __syncthreads();
+    const int lo = tile_lo[t], hi = tile_lo[t + 1];
+    const int head = tile_head[t];
+    const int nh = head >= 0 ? 1 : 0;
+    const int nrow_t = nh + (hi - lo);
+    for (int rr = tid; rr < 4 * nrow_t; rr += 256) {
+        const int idx = rr >> 2, r = rr & 3;
+        const int R = (idx < nh) ? head : lo + idx - nh;
+        const int rb = blk_ptr[R], re = blk_ptr[R + 1];
+        const int j0 = max(rb, q0) - q0, j1 = min(re, q1) - q0;
+        AT acc = AT(0);
+        for (int j = j0; j < j1; ++j) acc += s_buf[4 * j + r];
+        if (rb < q0) {
+            carry[8 * int64_t(t) + r] = double(acc);  // row started in an earlier tile
+        } else if (re > q1) {
+            carry[8 * int64_t(t) + 4 + r] = double(acc);  // row continues in later tiles
+        } else {
+            const int row = 4 * R + r;
+            if (row < nrows) {
+                double v = alpha * double(acc);
+                if (beta != 0.0) v = fma(beta, b[row], v);
+                y[row] = v;
+            }
+        }
+    }
+}
+
+// Finishes the block rows cut by tile edges: thread (t, r) owns the row that
+// starts in tile t and ends in a later one, and adds its partials in tile
+// order (tile t's tail, then the heads of t+1, t+2, ...).
+__global__ __launch_bounds__(256) void k_spmv_bal_fix(
+    int32_t ntiles, int32_t capb, int32_t nblk, int32_t nrows,
+    const int32_t *__restrict__ blk_ptr, const int32_t *__restrict__ tile_lo,
+    const int32_t *__restrict__ tile_head, const double *__restrict__ carry, double alpha,
+    double beta, const double *b, double *y) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = gid >> 2, r = gid & 3;
+    if (t >= ntiles) return;
+    const int lo = tile_lo[t], hi = tile_lo[t + 1];
+    if (hi == lo) return;  // no row starts in this tile
+    const int R = hi - 1;
+    const int q1 = int(min(int64_t(t + 1) * capb, int64_t(nblk)));
+    if (blk_ptr[R + 1] <= q1) return;  // the tile's last row ends inside it
+    double acc = carry[8 * int64_t(t) + 4 + r];
+    for (int t2 = t + 1; t2 < ntiles && tile_head[t2] == R; ++t2) acc += carry[8 * int64_t(t2) + r];
+    const int row = 4 * R + r;
+    if (row < nrows) {
+        double v = alpha * acc;
+        if (beta != 0.0) v = fma(beta, b[row], v);
+        y[row] = v;
+    }
+}
+
+// Tile tables of the balanced kernel, one thread per tile (tile_vb comes from
+// a scan of k_tile_popc's per-tile value counts).
+__global__ __launch_bounds__(256) void k_tile_rows(int32_t ntiles, int32_t capb, int32_t brow,
+                                                   const int32_t *__restrict__ blk_ptr,
+                                                   int32_t *__restrict__ tile_lo,
+                                                   int32_t *__restrict__ tile_head) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > ntiles) return;
+    if (t == ntiles) {
+        tile_lo[t] = brow;
+        return;
+    }
+    const int q0 = t * capb;
+    // lo = first block row R with blk_ptr[R] >= q0 (blk_ptr[brow] = nblk > q0)
+    int a = 0, c = brow;
+    while (a < c) {
+        const int mid = (a + c) >> 1;
+        if (blk_ptr[mid] >= q0) c = mid;
+        else a = mid + 1;
+    }
+    const int lo = (t == 0) ? 0 : a;
+    tile_lo[t] = lo;
+    // the block row holding block q0 starts before it: it is this tile's head
+    tile_head[t] = (t > 0 && blk_ptr[lo] > q0) ? lo - 1 : -1;
+}
+
+__global__ __launch_bounds__(256) void k_tile_popc(int32_t nblk, int32_t capb,
+                                                   const uint16_t *__restrict__ blk_map,
+                                                   int32_t *__restrict__ tile_cnt) {
+    __shared__ int s_part[4];
+    const int t = blockIdx.x;
+    const int q0 = t * capb, q1 = int(min(int64_t(q0) + capb, int64_t(nblk)));
+    int n = 0;
+    for (int q = q0 + int(threadIdx.x); q < q1; q += 256) n += __popc(uint32_t(blk_map[q]));
+    for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) tile_cnt[t] = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+}
+
 template <int G, typename VT, typename AT, int CAPV, int CAPB, int MAXB, bool XSEG>
 __global__ __launch_bounds__(256) void k_spmv_staged(
     int32_t brow, int32_t nrows, int32_t ncols, const int32_t *__restrict__ blk_ptr,
@@ -631,6 +843,21 @@ int launch_spmv_blk_g(const Mbsr &A, const VT *val, double alpha, const double *
     return AMGT_OK;
 }
 
+template <int BPT, typename VT, typename AT>
+int launch_spmv_bal_g(const Mbsr &A, const VT *val, double alpha, const double *x, double beta,
+                      const double *b, double *y, hipStream_t s) {
+    const int nt = A.bal_ntiles;
+    hipLaunchKernelGGL((k_spmv_bal<BPT, VT, AT>), dim3(unsigned(nt)), dim3(256), 0, s, nt,
+                       int32_t(A.nblk), A.nrows, A.ncols, A.blk_ptr, A.blk_col, A.blk_map, val,
+                       A.bal_lo, A.bal_head, A.bal_vb, A.bal_carry, x, alpha, beta, b, y);
+    AMGT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_spmv_bal_fix, dim3(unsigned((4 * int64_t(nt) + 255) / 256)), dim3(256),
+                       0, s, nt, A.bal_capb, int32_t(A.nblk), A.nrows, A.blk_ptr, A.bal_lo,
+                       A.bal_head, A.bal_carry, alpha, beta, b, y);
+    AMGT_HIP(hipGetLastError());
+    return AMGT_OK;
+}
+
 template <typename VT, typename AT>
 int launch_spmv_t(const Mbsr &A, const VT *val, int lanes, double alpha, const double *x,
                   double beta, const double *b, double *y, hipStream_t s) {
@@ -640,7 +867,15 @@ int launch_spmv_t(const Mbsr &A, const VT *val, int lanes, double alpha, const d
     }
     // the staged and block kernels read x in 16-byte pieces; otherwise the direct kernel
     const bool x_aligned = (reinterpret_cast<uintptr_t>(x) & 15u) == 0;
-    if (A.spmv_kernel == kSpmvBlock && x_aligned) {
+    if (A.spmv_kernel == kSpmvBalanced && x_aligned && A.bal_ntiles > 0 &&
+        A.bal_capb == 256 * balanced_bpt(lanes)) {
+        switch (balanced_bpt(lanes)) {
+            case 1: return launch_spmv_bal_g<1, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            case 4: return launch_spmv_bal_g<4, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            default: return launch_spmv_bal_g<2, VT, AT>(A, val, alpha, x, beta, b, y, s);
+        }
+    }
+    if ((A.spmv_kernel == kSpmvBlock || A.spmv_kernel == kSpmvBalanced) && x_aligned) {
         // lanes selects the tile: 4 -> 64 block rows (2 blocks per thread),
         // 8 -> 128 (4 per thread), 16 -> 32, 32 -> 16, 64 -> 8 block rows
         switch (lanes) {
@@ -714,12 +949,52 @@ int spmv_default_lanes(const Mbsr &A) {
     return 64;
 }
 
+int balanced_bpt(int lanes) { return lanes == 16 ? 1 : (lanes == 8 ? 4 : 2); }
+
+int spmv_prepare(Mbsr *A, int lanes, hipStream_t s) {
+    if (A->spmv_kernel != kSpmvBalanced || A->nblk == 0) return AMGT_OK;
+    const int capb = 256 * balanced_bpt(lanes);
+    if (A->bal_capb == capb && A->bal_ntiles > 0) return AMGT_OK;
+    mbsr_release_balanced(A, s);
+    const int64_t nt = (A->nblk + capb - 1) / capb;
+    A->bal_lo = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * size_t(nt + 1), s));
+    A->bal_head = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * size_t(nt), s));
+    A->bal_vb = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * size_t(nt + 1), s));
+    A->bal_carry = static_cast<double *>(dev_alloc(sizeof(double) * size_t(8 * nt), s));
+    if (!A->bal_lo || !A->bal_head || !A->bal_vb || !A->bal_carry) {
+        mbsr_release_balanced(A, s);
+        set_error("balanced SpMV: tile table allocation failed");
+        return AMGT_ERR_ALLOC;
+    }
+    int32_t *cnt = nullptr;
+    AMGT_TRY(tmp_alloc_t(&cnt, size_t(nt), s));
+    hipLaunchKernelGGL(k_tile_rows, dim3(unsigned((nt + 1 + 255) / 256)), dim3(256), 0, s,
+                       int32_t(nt), capb, A->brow, A->blk_ptr, A->bal_lo, A->bal_head);
+    hipLaunchKernelGGL(k_tile_popc, dim3(unsigned(nt)), dim3(256), 0, s, int32_t(A->nblk), capb,
+                       A->blk_map, cnt);
+    hipError_t e = hipGetLastError();
+    int st = AMGT_OK;
+    if (e != hipSuccess) {
+        set_error(std::string("balanced SpMV setup: ") + hipGetErrorString(e));
+        st = AMGT_ERR_HIP;
+    }
+    if (st == AMGT_OK) st = exclusive_scan_i32(cnt, A->bal_vb, nt, s);
+    tmp_free(cnt, s);
+    if (st != AMGT_OK) {
+        mbsr_release_balanced(A, s);
+        return st;
+    }
+    A->bal_capb = capb;
+    A->bal_ntiles = int32_t(nt);
+    return AMGT_OK;
+}
+
 int mbsr_ensure_lowp(Mbsr *A, int precision, hipStream_t s) {
     if (precision == AMGT_DOUBLE || A->nnz == 0 || A->pattern_only) return AMGT_OK;
     const int64_t n = A->nnz;
     const unsigned grid = unsigned((n + 255) / 256);
     if (precision == AMGT_FLOAT && !A->val32) {
-        A->val32 = static_cast<float *>(dev_alloc(sizeof(float) * size_t(n)));
+        A->val32 = static_cast<float *>(dev_alloc(sizeof(float) * size_t(n), s));
         if (!A->val32) {
             set_error("FP32 value copy allocation failed");
             return AMGT_ERR_ALLOC;
@@ -728,7 +1003,7 @@ int mbsr_ensure_lowp(Mbsr *A, int precision, hipStream_t s) {
         AMGT_HIP(hipGetLastError());
     }
     if (precision == AMGT_HALF && !A->val16) {
-        A->val16 = static_cast<__half *>(dev_alloc(sizeof(__half) * size_t(n)));
+        A->val16 = static_cast<__half *>(dev_alloc(sizeof(__half) * size_t(n), s));
         if (!A->val16) {
             set_error("FP16 value copy allocation failed");
             return AMGT_ERR_ALLOC;

@@ -83,11 +83,6 @@ __global__ void k_tval_ptr(int32_t tbrow, const int32_t *__restrict__ tptr,
     tval_ptr[c] = (p < nblk) ? dst_voff[p] : int32_t(nnz);
 }
 
-__global__ void k_row_max(int32_t n, const int32_t *__restrict__ ptr, int32_t *__restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicMax(out, ptr[i + 1] - ptr[i]);
-}
-
 }  // namespace
 
 int mbsr_transpose(const Mbsr &A, hipStream_t s, Mbsr *out) {
@@ -101,18 +96,19 @@ int mbsr_transpose(const Mbsr &A, hipStream_t s, Mbsr *out) {
     T.nnz = A.nnz;
     T.pattern_only = A.pattern_only;
     const int64_t nb = A.nblk;
-    T.blk_ptr = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(T.brow) + 1)));
-    T.val_ptr = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(T.brow) + 1)));
-    T.blk_col = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * size_t(nb ? nb : 1)));
-    T.blk_map = static_cast<uint16_t *>(dev_alloc(sizeof(uint16_t) * size_t(nb ? nb : 1)));
+    T.blk_ptr = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(T.brow) + 1), s));
+    T.val_ptr = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(T.brow) + 1), s));
+    T.blk_col = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * size_t(nb ? nb : 1), s));
+    T.blk_map = static_cast<uint16_t *>(dev_alloc(sizeof(uint16_t) * size_t(nb ? nb : 1), s));
     if (!A.pattern_only)
-        T.val = static_cast<double *>(dev_alloc(sizeof(double) * size_t(A.nnz ? A.nnz : 1)));
+        T.val = static_cast<double *>(dev_alloc(sizeof(double) * size_t(A.nnz ? A.nnz : 1), s));
     if (!T.blk_ptr || !T.val_ptr || !T.blk_col || !T.blk_map || (!A.pattern_only && !T.val)) {
         set_error("transpose: allocation failed");
         return AMGT_ERR_ALLOC;
     }
     int32_t *row_of = nullptr, *idx = nullptr, *perm = nullptr, *keys = nullptr, *cnt = nullptr;
-    int32_t *pc = nullptr, *src_voff = nullptr, *dst_voff = nullptr, *hmax = nullptr;
+    int32_t *pc = nullptr, *src_voff = nullptr, *dst_voff = nullptr;
+    RowStats *rstats = nullptr;
     AMGT_TRY(tmp_alloc_t(&row_of, size_t(nb), s));
     AMGT_TRY(tmp_alloc_t(&idx, size_t(nb), s));
     AMGT_TRY(tmp_alloc_t(&perm, size_t(nb), s));
@@ -121,7 +117,7 @@ int mbsr_transpose(const Mbsr &A, hipStream_t s, Mbsr *out) {
     AMGT_TRY(tmp_alloc_t(&pc, size_t(nb) + 1, s));
     AMGT_TRY(tmp_alloc_t(&src_voff, size_t(nb) + 1, s));
     AMGT_TRY(tmp_alloc_t(&dst_voff, size_t(nb) + 1, s));
-    AMGT_TRY(tmp_alloc_t(&hmax, 1, s));
+    AMGT_TRY(tmp_alloc_t(&rstats, 1, s));
     const unsigned gb = unsigned((nb + 255) / 256);
     if (A.brow > 0)
         hipLaunchKernelGGL(k_block_rows, dim3((A.brow + 255) / 256), dim3(256), 0, s, A.brow,
@@ -157,13 +153,10 @@ int mbsr_transpose(const Mbsr &A, hipStream_t s, Mbsr *out) {
                            T.blk_col, T.blk_map, T.val);
     hipLaunchKernelGGL(k_tval_ptr, dim3((T.brow + 256) / 256), dim3(256), 0, s, T.brow, T.blk_ptr,
                        dst_voff, nb, A.nnz, T.val_ptr);
-    hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, int64_t(1), hmax);
-    if (T.brow > 0)
-        hipLaunchKernelGGL(k_row_max, dim3((T.brow + 255) / 256), dim3(256), 0, s, T.brow,
-                           T.blk_ptr, hmax);
     AMGT_HIP(hipGetLastError());
-    int32_t mx = 0;
-    AMGT_HIP(hipMemcpyAsync(&mx, hmax, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    AMGT_TRY(row_stats_launch(T.blk_ptr, T.brow, rstats, s));
+    RowStats hrs{};
+    AMGT_HIP(hipMemcpyAsync(&hrs, rstats, sizeof(RowStats), hipMemcpyDeviceToHost, s));
     tmp_free(row_of, s);
     tmp_free(idx, s);
     tmp_free(perm, s);
@@ -172,11 +165,12 @@ int mbsr_transpose(const Mbsr &A, hipStream_t s, Mbsr *out) {
     tmp_free(pc, s);
     tmp_free(src_voff, s);
     tmp_free(dst_voff, s);
-    tmp_free(hmax, s);
+    tmp_free(rstats, s);
     AMGT_HIP(hipStreamSynchronize(s));
-    T.max_row_blocks = mx;
+    T.max_row_blocks = hrs.max_blocks;
     T.avg_nnz = nb ? double(A.nnz) / double(nb) : 0.0;
-    T.stand = 0.0;  // not needed for the transposed operator's kernel choice
+    T.stand = stand_from(hrs, nb, T.brow);
+    T.spmv_kernel = default_spmv_kernel(T.stand);
     return AMGT_OK;
 }
 

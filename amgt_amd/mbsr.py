@@ -127,7 +127,7 @@ class MBSRMatrix:
         L.check(L.load().amgt_mbsr_set_spmv_lanes(self.handle, lanes), "amgt_mbsr_set_spmv_lanes")
 
     def set_spmv_kernel(self, kind: int):
-        """AMGT_SPMV_DIRECT / _STAGED / _BLOCK."""
+        """AMGT_SPMV_DIRECT / _STAGED / _BLOCK / _BALANCED."""
         L.check(L.load().amgt_mbsr_set_spmv_kernel(self.handle, int(kind)),
                 "amgt_mbsr_set_spmv_kernel")
 

@@ -187,7 +187,7 @@ def main():
     # per-variant timings (staged/direct x lanes per block row), then the tuner's pick
     variants = {}
     from amgt_amd._lib import SPMV_KERNEL_NAMES
-    for kind in (2, 1, 0):
+    for kind in (2, 3, 1, 0):
         M.set_spmv_kernel(kind)
         for lanes in (4, 8, 16, 32, 64):
             M.set_spmv_lanes(lanes)

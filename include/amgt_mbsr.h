@@ -53,7 +53,7 @@ typedef struct {
     int32_t level;      /* coarsening level tag (hypre_Level, seq_mv.h:199) */
     int32_t precision;  /* value precision used by SpMV (AMGT_DOUBLE/FLOAT/HALF) */
     int32_t spmv_lanes; /* lanes per block row chosen by the tuner (4..64) */
-    int32_t spmv_kernel; /* AMGT_SPMV_DIRECT / _STAGED / _BLOCK */
+    int32_t spmv_kernel; /* AMGT_SPMV_DIRECT / _STAGED / _BLOCK / _BALANCED */
 } amgt_mbsr_info_t;
 
 typedef struct {
@@ -99,14 +99,19 @@ int amgt_mbsr_set_precision(amgt_mbsr_t A, int precision);
  * matrix statistics */
 int amgt_mbsr_set_spmv_lanes(amgt_mbsr_t A, int lanes);
 /* SpMV kernel kind:
- *   AMGT_SPMV_DIRECT  lane per (block slot, row) walks bitmaps, columns and values in
- *                     global memory
- *   AMGT_SPMV_STAGED  a workgroup stages its tile of block rows in LDS, lanes walk it
- *   AMGT_SPMV_BLOCK   (default) lane per 4x4 block over an LDS-staged tile, whole x
- *                     segments, per-row partial sums reduced in LDS
- * The staged and block kernels need a 16-byte aligned x; otherwise the direct
- * kernel runs. */
-enum { AMGT_SPMV_DIRECT = 0, AMGT_SPMV_STAGED = 1, AMGT_SPMV_BLOCK = 2 };
+ *   AMGT_SPMV_DIRECT    lane per (block slot, row) walks bitmaps, columns and values in
+ *                       global memory
+ *   AMGT_SPMV_STAGED    a workgroup stages its tile of block rows in LDS, lanes walk it
+ *   AMGT_SPMV_BLOCK     lane per 4x4 block over an LDS-staged tile of block rows, whole
+ *                       x segments, per-row partial sums reduced in LDS
+ *   AMGT_SPMV_BALANCED  the block kernel over equal tiles of blocks whatever rows they
+ *                       belong to; rows cut by tile edges are finished by a second pass
+ *                       (bsr_spmv_balanced_tc_fp64 + BSR_BALANCED_PREPROCESS_GPU,
+ *                       csr_matvec_device.c:556-613, 1269-1300)
+ * Default: BALANCED when the spread of blocks per block row (stand) is >= 12, as in
+ * the reference (csr_matvec_device.c:1336), else BLOCK.  The staged, block and
+ * balanced kernels need a 16-byte aligned x; otherwise the direct kernel runs. */
+enum { AMGT_SPMV_DIRECT = 0, AMGT_SPMV_STAGED = 1, AMGT_SPMV_BLOCK = 2, AMGT_SPMV_BALANCED = 3 };
 int amgt_mbsr_set_spmv_kernel(amgt_mbsr_t A, int kind);
 /* times every lanes-per-block-row variant on this matrix and keeps the fastest */
 int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream);

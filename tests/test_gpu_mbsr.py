@@ -14,7 +14,7 @@ from amgt_amd import _lib as L
 from amgt_amd import hypre as H
 from amgt_amd.mbsr import MBSRMatrix
 from oracle import oracle as O
-from tests.util_matrices import dense_rows_csr, rand_csr
+from tests.util_matrices import block_count_csr, dense_rows_csr, rand_csr, skewed_csr
 
 pytestmark = pytest.mark.gpu
 
@@ -39,7 +39,18 @@ CASES = {
     "bin2_rows": lambda: ((9, 9000), *dense_rows_csr(9, 9000, 1900, 5)),
     "one_row": lambda: ((1, 3), np.array([0, 2], np.int32), np.array([2, 0], np.int32),
                         np.array([1.5, -2.0])),
+    # block rows 2, 500 and 998 hold ~8000 blocks each, the rest ~12: stand >> 12
+    "skewed": lambda: ((3995, 40000), *skewed_csr(3995, 40000, (2, 500, 998), 2000, 3, 6,
+                                                  empty_rows=(0, 7, 2001, 3994))),
+    # block totals landing on multiples of 256 / 512 / 1024 (balanced tile edges)
+    "tile_edges": lambda: _tile_edges(),
 }
+
+
+def _tile_edges():
+    counts = [256, 0, 512, 0, 0, 100, 156, 1024, 1, 255, 0, 256, 2000, 48, 0, 0]
+    m, I, J, A = block_count_csr(counts, 8192, 7)
+    return (m, 8192), I, J, A
 
 
 def check_structure(M, ref):
@@ -106,9 +117,10 @@ def row_scale(m, n, I, J, A, x, beta, b):
 
 
 @pytest.mark.parametrize("name", ["c1_5pt_128", "7pt_aniso", "27pt_10", "rand_rect",
-                                  "rand_tall_empty", "rand_wide", "bin1_rows", "bin2_rows"])
+                                  "rand_tall_empty", "rand_wide", "bin1_rows", "bin2_rows",
+                                  "skewed", "tile_edges"])
 @pytest.mark.parametrize("lanes", [4, 8, 16, 32, 64])
-@pytest.mark.parametrize("kind", [0, 1, 2])
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
 def test_spmv_fp64(dev, name, lanes, kind):
     print(f"[spmv] {name} lanes={lanes} kernel={kind}", flush=True)
     (m, n), I, J, A = CASES[name]()
@@ -134,13 +146,42 @@ def test_spmv_fp64(dev, name, lanes, kind):
     assert np.all(np.isfinite(y0))
 
 
-def test_spmv_deterministic(dev):
-    (m, n), I, J, A = CASES["27pt_10"]()
+@pytest.mark.parametrize("name,kind", [("27pt_10", 2), ("skewed", 3), ("tile_edges", 3)])
+def test_spmv_deterministic(dev, name, kind):
+    (m, n), I, J, A = CASES[name]()
     M = make(dev, m, n, I, J, A)
+    M.set_spmv_kernel(kind)
     x = T(np.random.default_rng(0).standard_normal(n), dev)
     y1 = M.spmv(x).clone()
     y2 = M.spmv(x).clone()
     assert torch.equal(y1, y2)
+
+
+def test_default_kernel_follows_stand(dev):
+    """stand >= 12 selects the balanced kernel (csr_matvec_device.c:1336)."""
+    for name, want in [("skewed", L.AMGT_SPMV_BALANCED), ("27pt_10", L.AMGT_SPMV_BLOCK),
+                       ("c1_5pt_128", L.AMGT_SPMV_BLOCK)]:
+        (m, n), I, J, A = CASES[name]()
+        ref = O.csr2mbsr(m, n, I, J, A)
+        stand, _ = O.mbsr_stats(ref)
+        inf = make(dev, m, n, I, J, A).info()
+        assert (stand >= 12.0) == (want == L.AMGT_SPMV_BALANCED)
+        assert inf["spmv_kernel"] == want, name
+
+
+@pytest.mark.parametrize("prec,tol", [(L.AMGT_FLOAT, 2e-6), (L.AMGT_HALF, 2e-3)])
+def test_spmv_balanced_low_precision(dev, prec, tol):
+    (m, n), I, J, A = CASES["skewed"]()
+    ref = O.csr2mbsr(m, n, I, J, A)
+    M = make(dev, m, n, I, J, A)
+    M.set_precision(prec)
+    assert M.info()["spmv_kernel"] == L.AMGT_SPMV_BALANCED
+    rng = np.random.default_rng(3)
+    x, b = rng.standard_normal(n), rng.standard_normal(m)
+    exp = O.mbsr_spmv(ref, -1.0, x, 1.0, b)
+    y = M.spmv(T(x, dev), alpha=-1.0, beta=1.0, b=T(b, dev)).cpu().numpy()
+    scale = row_scale(m, n, I, J, A, x, 1.0, b)
+    assert np.all(np.abs(y - exp) <= tol * scale)
 
 
 @pytest.mark.parametrize("name", ["rand_rect", "rand_tall_empty", "7pt_aniso", "bin1_rows"])
