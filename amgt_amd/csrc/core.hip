@@ -5,8 +5,10 @@
 
 #include <atomic>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 
 #include "internal.hpp"
 
@@ -25,41 +27,107 @@ void set_error(const std::string &msg) { g_last_error = msg; }
 
 bool mixed_precision_enabled() { return g_mixed.load() != 0; }
 
-// Device memory pool of the library, one per device, created on first use.
-// Freed blocks stay in the pool (release threshold: unlimited), so the many
-// temporaries of a conversion or product, and the arrays of matrices made and
-// dropped level after level, are served without new device mappings.
+// Caching device allocator of the library.  hipMalloc/hipFree cost 0.1-1 ms
+// for large blocks and the stream-ordered pool of this ROCm returns memory to
+// the driver inside hipFreeAsync (measured: 0.6-3 ms per call even with an
+// unlimited release threshold), so blocks are kept here instead: sizes round
+// up to a class (four steps per power of two), a freed block is reused at once
+// by the stream that freed it (later work on that stream runs after the work
+// that used it), and by another stream only after an event recorded at the
+// free has completed.  Blocks stay cached until the process ends.
 namespace {
 constexpr int kMaxDevices = 64;
-std::mutex g_pool_mu;
-hipMemPool_t g_pool[kMaxDevices] = {};
+
+struct CachedBlock {
+    void *p;
+    hipStream_t stream;
+    hipEvent_t ev;
+};
+
+struct DeviceCache {
+    std::multimap<size_t, CachedBlock> free_blocks;  // by class size
+    std::unordered_map<void *, std::pair<size_t, hipEvent_t>> live;  // ptr -> (class size, event)
+};
+
+std::mutex g_cache_mu;
+DeviceCache g_cache[kMaxDevices];
+
+size_t size_class(size_t n) {
+    if (n <= 4096) return 4096;
+    size_t p = 4096;
+    while (p * 2 < n) p *= 2;  // p < n <= 2p
+    const size_t step = p / 4;
+    return (n + step - 1) / step * step;
+}
 }  // namespace
 
-static hipMemPool_t lib_pool() {
+static void *pool_alloc(size_t bytes, hipStream_t s) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
-    std::lock_guard<std::mutex> lk(g_pool_mu);
-    if (!g_pool[dev]) {
-        hipMemPoolProps props{};
-        props.allocType = hipMemAllocationTypePinned;
-        props.handleTypes = hipMemHandleTypeNone;
-        props.location.type = hipMemLocationTypeDevice;
-        props.location.id = dev;
-        hipMemPool_t pool = nullptr;
-        if (hipMemPoolCreate(&pool, &props) != hipSuccess) return nullptr;
-        uint64_t keep = UINT64_MAX;
-        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
-        g_pool[dev] = pool;
+    const size_t sz = size_class(bytes);
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    DeviceCache &dc = g_cache[dev];
+    // a block of this class, or up to twice as large: same stream first, then
+    // any block whose last use has completed
+    auto lo = dc.free_blocks.lower_bound(sz), hi = dc.free_blocks.upper_bound(2 * sz);
+    auto pick = dc.free_blocks.end();
+    for (auto it = lo; it != hi; ++it)
+        if (it->second.stream == s) {
+            pick = it;
+            break;
+        }
+    if (pick == dc.free_blocks.end())
+        for (auto it = lo; it != hi; ++it)
+            if (hipEventQuery(it->second.ev) == hipSuccess) {
+                pick = it;
+                break;
+            }
+    if (pick != dc.free_blocks.end()) {
+        void *p = pick->second.p;
+        dc.live[p] = {pick->first, pick->second.ev};
+        dc.free_blocks.erase(pick);
+        return p;
     }
-    return g_pool[dev];
+    void *p = nullptr;
+    if (hipMalloc(&p, sz) != hipSuccess) {
+        (void)hipGetLastError();
+        // give back the cached blocks that are idle, then try once more
+        for (auto it = dc.free_blocks.begin(); it != dc.free_blocks.end();) {
+            if (hipEventQuery(it->second.ev) == hipSuccess) {
+                (void)hipFree(it->second.p);
+                (void)hipEventDestroy(it->second.ev);
+                it = dc.free_blocks.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        if (hipMalloc(&p, sz) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    hipEvent_t ev = nullptr;
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        (void)hipFree(p);
+        return nullptr;
+    }
+    dc.live[p] = {sz, ev};
+    return p;
 }
 
-static void *pool_alloc(size_t bytes, hipStream_t s) {
-    hipMemPool_t pool = lib_pool();
-    if (!pool) return nullptr;
-    void *p = nullptr;
-    if (hipMallocFromPoolAsync(&p, bytes, pool, s) != hipSuccess) return nullptr;
-    return p;
+static void pool_free(void *p, hipStream_t s) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    for (auto &dc : g_cache) {
+        auto it = dc.live.find(p);
+        if (it == dc.live.end()) continue;
+        const size_t sz = it->second.first;
+        hipEvent_t ev = it->second.second;
+        dc.live.erase(it);
+        (void)hipEventRecord(ev, s);
+        dc.free_blocks.emplace(sz, CachedBlock{p, s, ev});
+        return;
+    }
 }
 
 void *dev_alloc(size_t bytes, hipStream_t s) {
@@ -88,7 +156,7 @@ void dev_free(void *p, hipStream_t s) {
         fn(p, ctx);
         return;
     }
-    (void)hipFreeAsync(p, s);
+    pool_free(p, s);
 }
 
 int tmp_alloc(void **p, size_t bytes, hipStream_t s) {
@@ -96,15 +164,13 @@ int tmp_alloc(void **p, size_t bytes, hipStream_t s) {
     if (bytes == 0) bytes = 1;
     *p = pool_alloc(bytes, s);
     if (!*p) {
-        set_error("device pool allocation of " + std::to_string(bytes) + " bytes failed");
+        set_error("device allocation of " + std::to_string(bytes) + " bytes failed");
         return AMGT_ERR_ALLOC;
     }
     return AMGT_OK;
 }
 
-void tmp_free(void *p, hipStream_t s) {
-    if (p) (void)hipFreeAsync(p, s);
-}
+void tmp_free(void *p, hipStream_t s) { pool_free(p, s); }
 
 template <typename T>
 __global__ void k_set_zero(T *p) {

@@ -319,20 +319,34 @@ __device__ __forceinline__ int lower_bound_key(const int32_t *keys, int n, int k
     return lo;
 }
 
-// one wave per C block row; acc points to LDS (cnt <= kNumCap) or a global slab
-__device__ void numeric_row(int i, const SpgemmArgs &g, const int32_t *__restrict__ cptr,
-                            const int32_t *__restrict__ ccol, const uint16_t *__restrict__ cmap,
-                            const int32_t *__restrict__ cvp, double *__restrict__ cval,
-                            int32_t *skey, double *acc) {
+// LDS staging of one wave in the numeric pass: the current chunk of A blocks
+// and the compacted list of products whose bitmap product is non-zero.
+struct NumStage {
+    int32_t a_bq0[64], a_pst[64], a_ma[64], a_aoff[64];
+    int32_t p_ma[64], p_aoff[64], p_mb[64], p_boff[64], p_slot[64];
+};
+
+// One wave per C block row; acc points to LDS (cnt <= kNumCap) or a global
+// slab, keys to the row's block columns (LDS copy or global).
+// The row's products (A block, B block) are flattened over the 64 lanes, so
+// the gathers of a chunk of 64 products are all in flight together; products
+// with an empty bitmap product are dropped; the rest run four at a time on
+// one v_mfma_f64_4x4x4f64 (block slot b = product 4g + b).  Four products
+// may hit the same C block, so their results are added one block slot after
+// the other (fixed order: deterministic, no atomics).  boff[q] is the value
+// offset of B block q (values are stored in block order).
+__device__ void numeric_row(int i, const SpgemmArgs &g, const int32_t *__restrict__ boff,
+                            const int32_t *__restrict__ cptr, const int32_t *__restrict__ ccol,
+                            const uint16_t *__restrict__ cmap, const int32_t *__restrict__ cvp,
+                            double *__restrict__ cval, int32_t *skey, double *acc,
+                            NumStage &st) {
     const int lane = threadIdx.x & 63;
     // MFMA operand roles of this lane (see block_product)
     const int bs = (lane >> 2) & 3;                       // block slot
     const int eA = 4 * (lane & 3) + (lane >> 4);          // A operand bit
     const int eB = 4 * (lane >> 4) + (lane & 3);          // B operand bit = result bit
     const uint32_t belowA = (1u << eA) - 1u, belowB = (1u << eB) - 1u;
-    // compression roles: slot gs = lane >> 4, bit e = lane & 15
-    const int gs = lane >> 4, e = lane & 15;
-    const uint32_t below = (1u << e) - 1u;
+    const uint64_t lt = (1ull << lane) - 1ull;
     const int c0 = cptr[i], cnt = cptr[i + 1] - c0;
     const int32_t *keys = ccol + c0;
     if (skey) {
@@ -341,36 +355,105 @@ __device__ void numeric_row(int i, const SpgemmArgs &g, const int32_t *__restric
     }
     for (int x = lane; x < cnt * 16; x += 64) acc[x] = 0.0;
     wave_sync();
-    int aoff = g.avp[i];
-    for (int p = g.ap[i]; p < g.ap[i + 1]; ++p) {
-        const int k = g.ac[p];
-        const uint32_t ma = g.am[p];
-        const double a = ((ma >> eA) & 1u) ? g.av[aoff + __popc(ma & belowA)] : 0.0;
-        aoff += __popc(ma);
-        const int q1 = g.bp[k + 1];
-        int boff = g.bvp[k];
-        for (int q0 = g.bp[k]; q0 < q1; q0 += 4) {
-            const int q = q0 + bs;
-            const uint32_t mb = (q < q1) ? uint32_t(g.bm[q]) : 0u;
-            const int pc = __popc(mb);
-            // value offset of B block q: popcounts of the earlier slots (slot b in lane 4b)
-            const int p0c = __shfl(pc, 0), p1c = __shfl(pc, 4), p2c = __shfl(pc, 8),
-                      p3c = __shfl(pc, 12);
-            const int excl = (bs > 0 ? p0c : 0) + (bs > 1 ? p1c : 0) + (bs > 2 ? p2c : 0);
-            const double b = ((mb >> eB) & 1u) ? g.bv[boff + excl + __popc(mb & belowB)] : 0.0;
-            boff += p0c + p1c + p2c + p3c;
-            const uint32_t mc = bitmap_mul(ma, mb);
-            if (__ballot(mc != 0u) == 0ull) continue;  // wave-uniform: MFMA runs with full EXEC
-            const double d = block_product(a, b);
-            if (mc) {
-                const int slot = lower_bound_key(keys, cnt, g.bc[q]);
-                acc[slot * 16 + eB] += d;
+    int aoff_run = g.avp[i];
+    const int a_beg = g.ap[i], a_end = g.ap[i + 1];
+    for (int a0 = a_beg; a0 < a_end; a0 += 64) {
+        const int p = a0 + lane;
+        int ma = 0, bq0 = 0, blen = 0;
+        if (p < a_end) {
+            const int k = g.ac[p];
+            ma = g.am[p];
+            bq0 = g.bp[k];
+            blen = g.bp[k + 1] - bq0;
+        }
+        const int pc = __popc(uint32_t(ma));
+        int pinc = pc, linc = blen;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int t1 = __shfl_up(pinc, d), t2 = __shfl_up(linc, d);
+            if (lane >= d) {
+                pinc += t1;
+                linc += t2;
+            }
+        }
+        const int ptot = __shfl(linc, 63), atot = __shfl(pinc, 63);
+        wave_sync();  // the previous chunk's staging is consumed
+        st.a_bq0[lane] = bq0;
+        st.a_pst[lane] = linc - blen;
+        st.a_ma[lane] = ma;
+        st.a_aoff[lane] = aoff_run + pinc - pc;
+        aoff_run += atot;
+        const int na = min(64, a_end - a0);
+        wave_sync();
+        for (int t0 = 0; t0 < ptot; t0 += 64) {
+            const int t = t0 + lane;
+            bool valid = false;
+            int pma = 0, pmb = 0, paoff = 0, pboff = 0, slot = 0;
+            if (t < ptot) {
+                // owner: the last A block of the chunk whose first product is <= t
+                int lo = 0, hi = na - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (st.a_pst[mid] <= t) lo = mid; else hi = mid - 1;
+                }
+                const int q = st.a_bq0[lo] + (t - st.a_pst[lo]);
+                pma = st.a_ma[lo];
+                pmb = g.bm[q];
+                if (bitmap_mul(uint32_t(pma), uint32_t(pmb))) {
+                    valid = true;
+                    paoff = st.a_aoff[lo];
+                    pboff = boff[q];
+                    slot = lower_bound_key(keys, cnt, g.bc[q]);
+                }
+            }
+            const uint64_t vm = __ballot(valid);
+            const int np = __popcll(vm);
+            wave_sync();  // the previous product chunk is consumed
+            if (valid) {
+                const int r = __popcll(vm & lt);
+                st.p_ma[r] = pma;
+                st.p_mb[r] = pmb;
+                st.p_aoff[r] = paoff;
+                st.p_boff[r] = pboff;
+                st.p_slot[r] = slot;
             }
             wave_sync();
+            // batches of four MFMA groups (16 products): all gathers first
+            for (int g0 = 0; g0 < np; g0 += 16) {
+                double av[4], bv[4];
+                int sl[4];
+                bool on[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int idx = g0 + 4 * u + bs;
+                    on[u] = idx < np;
+                    av[u] = 0.0;
+                    bv[u] = 0.0;
+                    sl[u] = 0;
+                    if (on[u]) {
+                        const uint32_t xa = uint32_t(st.p_ma[idx]), xb = uint32_t(st.p_mb[idx]);
+                        sl[u] = st.p_slot[idx];
+                        if ((xa >> eA) & 1u) av[u] = g.av[st.p_aoff[idx] + __popc(xa & belowA)];
+                        if ((xb >> eB) & 1u) bv[u] = g.bv[st.p_boff[idx] + __popc(xb & belowB)];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (g0 + 4 * u >= np) break;  // wave-uniform: MFMA runs with full EXEC
+                    const double d = block_product(av[u], bv[u]);
+#pragma unroll
+                    for (int b2 = 0; b2 < 4; ++b2) {
+                        if (bs == b2 && on[u]) acc[sl[u] * 16 + eB] += d;
+                        wave_sync();
+                    }
+                }
+            }
         }
     }
     wave_sync();
-    // compress: four C blocks per step, slot g takes block t0 + g
+    // compress: four C blocks per step, slot gs takes block t0 + gs
+    const int gs = lane >> 4, e = lane & 15;
+    const uint32_t below = (1u << e) - 1u;
     int voff = cvp[i];
     for (int t0 = 0; t0 < cnt; t0 += 4) {
         const int t = t0 + gs;
@@ -385,6 +468,7 @@ __device__ void numeric_row(int i, const SpgemmArgs &g, const int32_t *__restric
 }
 
 __global__ __launch_bounds__(256) void k_numeric_lds(int32_t brow, SpgemmArgs g,
+                                                     const int32_t *__restrict__ boff,
                                                      const int32_t *__restrict__ cptr,
                                                      const int32_t *__restrict__ ccol,
                                                      const uint16_t *__restrict__ cmap,
@@ -392,28 +476,39 @@ __global__ __launch_bounds__(256) void k_numeric_lds(int32_t brow, SpgemmArgs g,
                                                      double *__restrict__ cval) {
     __shared__ double s_acc[4][kNumCap * 16];
     __shared__ int32_t s_key[4][kNumCap];
+    __shared__ NumStage s_st[4];
     const int w = threadIdx.x >> 6;
     const int i = blockIdx.x * 4 + w;
     if (i >= brow) return;
     if (cptr[i + 1] - cptr[i] > kNumCap) return;  // handled by k_numeric_global
-    numeric_row(i, g, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w]);
+    numeric_row(i, g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w], s_st[w]);
 }
 
 __global__ __launch_bounds__(256) void k_numeric_global(int count, const int32_t *__restrict__ rows,
                                                         SpgemmArgs g,
+                                                        const int32_t *__restrict__ boff,
                                                         const int32_t *__restrict__ cptr,
                                                         const int32_t *__restrict__ ccol,
                                                         const uint16_t *__restrict__ cmap,
                                                         const int32_t *__restrict__ cvp,
                                                         double *__restrict__ cval, double *slab,
                                                         int64_t slab_elems) {
+    __shared__ NumStage s_st[4];
     const int w = threadIdx.x >> 6;
     const int wave = blockIdx.x * 4 + w;
     const int nwaves = gridDim.x * 4;
     double *acc = slab + int64_t(wave) * slab_elems;
     for (int r = wave; r < count; r += nwaves) {
-        numeric_row(rows[r], g, cptr, ccol, cmap, cvp, cval, nullptr, acc);
+        numeric_row(rows[r], g, boff, cptr, ccol, cmap, cvp, cval, nullptr, acc, s_st[w]);
     }
+}
+
+// popcount of every block bitmap (values are stored in block order, so the
+// exclusive scan of these is each block's value offset)
+__global__ __launch_bounds__(256) void k_block_popc(int64_t nblk, const uint16_t *__restrict__ map,
+                                                    int32_t *__restrict__ out) {
+    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q < nblk) out[q] = __popc(uint32_t(map[q]));
 }
 
 }  // namespace
@@ -448,6 +543,7 @@ int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *Cp) {
     int32_t *tcol = nullptr;
     uint32_t *tmap = nullptr, *slab = nullptr;
     double *nslab = nullptr;
+    int32_t *bpop = nullptr, *boff = nullptr;
     int st = AMGT_OK;
     auto cleanup = [&]() {
         tmp_free(parts, s);
@@ -462,6 +558,8 @@ int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *Cp) {
         tmp_free(tmap, s);
         tmp_free(slab, s);
         tmp_free(nslab, s);
+        tmp_free(bpop, s);
+        tmp_free(boff, s);
     };
 #define SG_TRY(expr)               \
     do {                           \
@@ -563,7 +661,15 @@ int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *Cp) {
                            C.blk_ptr, tcol, tmap, C.blk_col, C.blk_map);
     SG_HIP(hipGetLastError());
     if (!C.pattern_only && brow > 0) {
-        hipLaunchKernelGGL(k_numeric_lds, dim3((brow + 3) / 4), dim3(256), 0, s, brow, g,
+        // value offset of every B block
+        SG_TRY(tmp_alloc_t(&bpop, size_t(B.nblk) + 1, s));
+        SG_TRY(tmp_alloc_t(&boff, size_t(B.nblk) + 1, s));
+        if (B.nblk > 0)
+            hipLaunchKernelGGL(k_block_popc, dim3(unsigned((B.nblk + 255) / 256)), dim3(256), 0, s,
+                               B.nblk, B.blk_map, bpop);
+        SG_HIP(hipGetLastError());
+        SG_TRY(exclusive_scan_i32(bpop, boff, B.nblk, s));
+        hipLaunchKernelGGL(k_numeric_lds, dim3((brow + 3) / 4), dim3(256), 0, s, brow, g, boff,
                            C.blk_ptr, C.blk_col, C.blk_map, C.val_ptr, C.val);
         if (big_rows > 0) {
             const int64_t elems = int64_t(C.max_row_blocks) * 16;
@@ -572,7 +678,7 @@ int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *Cp) {
             nwaves = (nwaves + 3) / 4 * 4;  // every wave of the grid owns a slab
             SG_TRY(tmp_alloc_t(&nslab, size_t(elems * nwaves), s));
             hipLaunchKernelGGL(k_numeric_global, dim3((nwaves + 3) / 4), dim3(256), 0, s,
-                               big_rows, big, g, C.blk_ptr, C.blk_col, C.blk_map, C.val_ptr,
+                               big_rows, big, g, boff, C.blk_ptr, C.blk_col, C.blk_map, C.val_ptr,
                                C.val, nslab, elems);
         }
         SG_HIP(hipGetLastError());

@@ -5,8 +5,10 @@
  * (reference: AmgT_HYPRE/src/seq_mv/csr_matvec_device.c and
  * csr_spgemm_device.c).  Plain C types only: device pointers, sizes, a HIP
  * stream passed as `void *` (hipStream_t), int status codes.  Every entry point
- * is stream-ordered on the stream it is given.  No entry point keeps global
- * mutable state except the allocator hooks and the last-error string.
+ * is stream-ordered on the stream it is given.  Global mutable state: the
+ * allocator hooks, the last-error string (per thread) and, without hooks, the
+ * library's cache of device blocks (freed blocks are kept for reuse until the
+ * process ends).
  *
  * mBSR layout (device memory, owned by the handle):
  *   blk_ptr[brow+1]  int32   first block of each 4-row block row
