@@ -34,14 +34,15 @@ __device__ __forceinline__ int bin3(int64_t v, int64_t t0, int64_t t1) {
     return v <= t0 ? 0 : (v <= t1 ? 1 : 2);
 }
 
-// Every thread of the (256-thread) workgroup calls this once with its row.
-__device__ __forceinline__ void write_part(bool valid, int64_t v, int64_t t0, int64_t t1,
-                                           Part *parts) {
+// Every thread of the (256-thread) workgroup calls this once with its row:
+// vb picks the bin, v feeds the largest value and the sum of squares.
+__device__ __forceinline__ void write_part(bool valid, int64_t vb, int64_t v, int64_t t0,
+                                           int64_t t1, Part *parts) {
     __shared__ int s_cnt[4][3];
     __shared__ int s_mx[4];
     __shared__ unsigned long long s_sq[4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int b = valid ? bin3(v, t0, t1) : -1;
+    const int b = valid ? bin3(vb, t0, t1) : -1;
     int c[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) c[q] = __popcll(__ballot(b == q));
@@ -70,12 +71,13 @@ __device__ __forceinline__ void write_part(bool valid, int64_t v, int64_t t0, in
     }
 }
 
-template <typename F>
-__global__ __launch_bounds__(256) void k_pass_count(int32_t n, F f, int64_t t0, int64_t t1,
-                                                    Part *parts) {
+// fb(i) picks the bin, fs(i) feeds the largest value and the sum of squares
+template <typename FB, typename FS>
+__global__ __launch_bounds__(256) void k_pass_count(int32_t n, FB fb, FS fs, int64_t t0,
+                                                    int64_t t1, Part *parts) {
     const int i = blockIdx.x * kThreads + threadIdx.x;
     const bool valid = i < n;
-    write_part(valid, valid ? f(i) : 0, t0, t1, parts);
+    write_part(valid, valid ? fb(i) : 0, valid ? fs(i) : 0, t0, t1, parts);
 }
 
 // exclusive scan over the 256 threads of the workgroup; *tot = sum
@@ -186,15 +188,21 @@ struct ArrayI64 {
 inline int32_t groups(int32_t n) { return (n + kThreads - 1) / kThreads; }
 
 // count + finish; parts needs groups(n) entries, *tot is written on the device
-template <typename F>
-int count(int32_t n, F f, int64_t t0, int64_t t1, Part *parts, Totals *tot, hipStream_t s) {
+template <typename FB, typename FS>
+int count(int32_t n, FB fb, FS fs, int64_t t0, int64_t t1, Part *parts, Totals *tot,
+          hipStream_t s) {
     const int32_t ng = groups(n);
     if (ng > 0)
-        hipLaunchKernelGGL((k_pass_count<F>), dim3(unsigned(ng)), dim3(kThreads), 0, s, n, f, t0,
-                           t1, parts);
+        hipLaunchKernelGGL((k_pass_count<FB, FS>), dim3(unsigned(ng)), dim3(kThreads), 0, s, n, fb,
+                           fs, t0, t1, parts);
     hipLaunchKernelGGL(k_pass_finish, dim3(1), dim3(kThreads), 0, s, ng, parts, tot);
     AMGT_HIP(hipGetLastError());
     return AMGT_OK;
+}
+
+template <typename F>
+int count(int32_t n, F f, int64_t t0, int64_t t1, Part *parts, Totals *tot, hipStream_t s) {
+    return count(n, f, f, t0, t1, parts, tot, s);
 }
 
 template <typename F>

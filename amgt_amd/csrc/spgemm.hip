@@ -32,7 +32,6 @@ namespace amgt {
 namespace {
 
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
-constexpr int kBin0Ub = 128;    // wave per row, LDS table of 256
 constexpr int kBin1Ub = 2048;   // workgroup per row, LDS table of 4096
 constexpr int kNumCap = 64;     // C blocks per row kept in LDS by the numeric kernel
 
@@ -211,24 +210,114 @@ struct SpgemmArgs {
     const double *bv;
 };
 
-// ---- bin 0: one wave per row, 4 rows per workgroup, LDS table of 256
-__global__ __launch_bounds__(256) void k_symbolic_wave(int count, const int32_t *__restrict__ rows,
-                                                       SpgemmArgs g,
-                                                       const int64_t *__restrict__ ub_off,
-                                                       int32_t *tcol, uint32_t *tmap,
-                                                       int32_t *cnt_out, int32_t *nnz_out) {
-    constexpr int HS = 2 * kBin0Ub;
-    __shared__ uint32_t s_key[4][HS];
-    __shared__ uint32_t s_map[4][HS];
-    __shared__ int s_excl[4][64], s_q0[4][64], s_ma[4][64];
-    const int w = threadIdx.x >> 6;
+// ---- bin 0: rows with at most 64 products and 64 A blocks, one wave per row.
+// Lane j holds A block j, lane t product t (owner found with a branchless
+// shuffle search).  The block columns of the products with a non-zero bitmap
+// product are rank-sorted (each lane counts the keys below its own over an LDS
+// copy of the 64 keys, ties by lane), placed in sorted order, and equal
+// columns are merged by a segmented OR of their bitmap products; the last lane
+// of every run writes the C block.  No hashing, no atomics.
+__global__ __launch_bounds__(256) void k_symbolic_reg(int count, const int32_t *__restrict__ rows,
+                                                      SpgemmArgs g,
+                                                      const int64_t *__restrict__ ub_off,
+                                                      int32_t *__restrict__ tcol,
+                                                      uint32_t *__restrict__ tmap,
+                                                      int32_t *__restrict__ cnt_out,
+                                                      int32_t *__restrict__ nnz_out) {
+    __shared__ __align__(16) int32_t s_key[4][64];
+    __shared__ int32_t s_sk[4][64];
+    __shared__ uint32_t s_map[4][64], s_sm[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r = blockIdx.x * 4 + w;
-    if (r >= count) return;  // the whole wave leaves; waves never sync across
+    if (r >= count) return;  // whole wave
     const int i = rows[r];
-    symbolic_row<64>(i, threadIdx.x & 63, s_key[w], s_map[w], HS, s_excl[w], s_q0[w], s_ma[w],
-                     nullptr, g.ap, g.ac, g.am, g.bp, g.bc, g.bm, ub_off[i], tcol, tmap, cnt_out,
-                     nnz_out);
+    const int a_beg = g.ap[i], na = g.ap[i + 1] - a_beg;
+    int ma = 0, bq0 = 0, blen = 0;
+    if (lane < na) {
+        const int k = g.ac[a_beg + lane];
+        ma = g.am[a_beg + lane];
+        bq0 = g.bp[k];
+        blen = g.bp[k + 1] - bq0;
+    }
+    int linc = blen;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t2 = __shfl_up(linc, d);
+        if (lane >= d) linc += t2;
+    }
+    const int pst = linc - blen;
+    const int ptot = __shfl(linc, 63);
+    int own = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1) {
+        const int cand = own + step;
+        const int v = __shfl(pst, cand < 64 ? cand : 63);
+        if (cand < na && v <= lane) own = cand;
+    }
+    const int q = __shfl(bq0, own) + (lane - __shfl(pst, own));
+    const uint32_t pma = uint32_t(__shfl(ma, own));
+    int key = INT_MAX;
+    uint32_t mc = 0u;
+    if (lane < ptot) {
+        mc = bitmap_mul(pma, uint32_t(g.bm[q]));
+        if (mc) key = g.bc[q];
+    }
+    const int nvalid = __popcll(__ballot(key != INT_MAX));
+    s_key[w][lane] = key;
+    s_map[w][lane] = mc;
+    wave_sync();
+    int rank = 0;
+    for (int j = 0; j < ptot; j += 4) {
+        const int4 kk = *reinterpret_cast<const int4 *>(&s_key[w][j]);
+        rank += (kk.x < key || (kk.x == key && j < lane)) ? 1 : 0;
+        rank += (kk.y < key || (kk.y == key && j + 1 < lane)) ? 1 : 0;
+        rank += (kk.z < key || (kk.z == key && j + 2 < lane)) ? 1 : 0;
+        rank += (kk.w < key || (kk.w == key && j + 3 < lane)) ? 1 : 0;
+    }
+    // lanes at or past ptot hold INT_MAX and rank past every valid key
+    if (key != INT_MAX) {
+        s_sk[w][rank] = key;
+        s_sm[w][rank] = mc;
+    }
+    wave_sync();
+    int sk = lane < nvalid ? s_sk[w][lane] : INT_MAX;
+    uint32_t sm = lane < nvalid ? s_sm[w][lane] : 0u;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int k2 = __shfl_up(sk, d);
+        const uint32_t m2 = uint32_t(__shfl_up(int(sm), d));
+        if (lane >= d && k2 == sk) sm |= m2;
+    }
+    const int nxt = __shfl_down(sk, 1);
+    const bool tail = lane < nvalid && (lane == 63 || nxt != sk);
+    const uint64_t tails = __ballot(tail);
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const int64_t slot = ub_off[i];
+    if (tail) {
+        const int rk = __popcll(tails & lt);
+        tcol[slot + rk] = sk;
+        tmap[slot + rk] = sm;
+    }
+    int nz = tail ? __popc(sm) : 0;
+    for (int off = 32; off > 0; off >>= 1) nz += __shfl_xor(nz, off);
+    if (lane == 0) {
+        cnt_out[i] = __popcll(tails);
+        nnz_out[i] = nz;
+    }
 }
+
+// Symbolic bin of a C block row: 0 = k_symbolic_reg (<= 64 products and A
+// blocks), else by the upper bound: <= kBin1Ub workgroup hash in LDS, larger
+// global hash.
+struct SymbolicBin {
+    const int64_t *ub;
+    const int32_t *ap;
+    __device__ int64_t operator()(int i) const {
+        const int64_t u = ub[i];
+        if (u <= 64 && ap[i + 1] - ap[i] <= 64) return 0;
+        return u > 0 ? u : 1;
+    }
+};
 
 // ---- bin 1: one workgroup per row, LDS table of 4096
 __global__ __launch_bounds__(256) void k_symbolic_block(int count, const int32_t *__restrict__ rows,
@@ -264,22 +353,24 @@ __global__ __launch_bounds__(256) void k_symbolic_global(int count, const int32_
 }
 
 // ub_i = sum over A blocks (i,k) of |B row k|; the workgroup's partial bins
-// rows by ub (<= kBin0Ub, <= kBin1Ub, larger) and keeps the largest ub.
+// rows (SymbolicBin: register kernel, <= kBin1Ub, larger) and keeps the
+// largest ub.
 __global__ __launch_bounds__(256) void k_upper_bound(int32_t brow, const int32_t *__restrict__ ap,
                                                      const int32_t *__restrict__ ac,
                                                      const int32_t *__restrict__ bp,
                                                      int64_t *__restrict__ ub,
                                                      rowpass::Part *parts) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t v = 0;
+    int64_t v = 0, vb = 0;
     if (i < brow) {
         for (int p = ap[i]; p < ap[i + 1]; ++p) {
             const int k = ac[p];
             v += bp[k + 1] - bp[k];
         }
         ub[i] = v;
+        vb = (v <= 64 && ap[i + 1] - ap[i] <= 64) ? 0 : (v > 0 ? v : 1);  // as SymbolicBin
     }
-    rowpass::write_part(i < brow, v, kBin0Ub, kBin1Ub, parts);
+    rowpass::write_part(i < brow, vb, v, 0, kBin1Ub, parts);
 }
 
 __global__ __launch_bounds__(256) void k_compact_c(int32_t brow, const int64_t *__restrict__ ub_off,
@@ -467,7 +558,8 @@ __device__ void numeric_row(int i, const SpgemmArgs &g, const int32_t *__restric
     }
 }
 
-__global__ __launch_bounds__(256) void k_numeric_lds(int32_t brow, SpgemmArgs g,
+__global__ __launch_bounds__(256) void k_numeric_lds(int count, const int32_t *__restrict__ rows,
+                                                     SpgemmArgs g,
                                                      const int32_t *__restrict__ boff,
                                                      const int32_t *__restrict__ cptr,
                                                      const int32_t *__restrict__ ccol,
@@ -478,10 +570,9 @@ __global__ __launch_bounds__(256) void k_numeric_lds(int32_t brow, SpgemmArgs g,
     __shared__ int32_t s_key[4][kNumCap];
     __shared__ NumStage s_st[4];
     const int w = threadIdx.x >> 6;
-    const int i = blockIdx.x * 4 + w;
-    if (i >= brow) return;
-    if (cptr[i + 1] - cptr[i] > kNumCap) return;  // handled by k_numeric_global
-    numeric_row(i, g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w], s_st[w]);
+    const int r = blockIdx.x * 4 + w;
+    if (r >= count) return;
+    numeric_row(rows[r], g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w], s_st[w]);
 }
 
 __global__ __launch_bounds__(256) void k_numeric_global(int count, const int32_t *__restrict__ rows,
@@ -502,6 +593,126 @@ __global__ __launch_bounds__(256) void k_numeric_global(int count, const int32_t
         numeric_row(rows[r], g, boff, cptr, ccol, cmap, cvp, cval, nullptr, acc, s_st[w]);
     }
 }
+
+// Rows with at most 64 products and 64 A blocks: one wave per row, nothing in
+// LDS.  Lane j holds A block j, lane t product t, lane c the block column of C
+// block c (owner and slot found with branchless shuffle searches).  Products
+// are grouped by C block: block slot b of the MFMA accumulates C block
+// g0 + b in registers, chaining its products (in product order) through the
+// MFMA's accumulator input, and the finished blocks go straight to the
+// compressed value array.  Deterministic, no atomics.
+__global__ __launch_bounds__(256) void k_numeric_reg(int count, const int32_t *__restrict__ rows,
+                                                     SpgemmArgs g,
+                                                     const int32_t *__restrict__ boff,
+                                                     const int32_t *__restrict__ cptr,
+                                                     const int32_t *__restrict__ ccol,
+                                                     const uint16_t *__restrict__ cmap,
+                                                     const int32_t *__restrict__ cvp,
+                                                     double *__restrict__ cval) {
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= count) return;  // whole wave
+    const int i = rows[r];
+    const int bs = (lane >> 2) & 3;
+    const int eA = 4 * (lane & 3) + (lane >> 4);
+    const int eB = 4 * (lane >> 4) + (lane & 3);
+    const uint32_t belowA = (1u << eA) - 1u, belowB = (1u << eB) - 1u;
+    const int c0 = cptr[i], cnt = cptr[i + 1] - c0;
+    const int key = lane < cnt ? ccol[c0 + lane] : INT_MAX;
+    const int a_beg = g.ap[i], na = g.ap[i + 1] - a_beg;
+    int ma = 0, bq0 = 0, blen = 0;
+    if (lane < na) {
+        const int k = g.ac[a_beg + lane];
+        ma = g.am[a_beg + lane];
+        bq0 = g.bp[k];
+        blen = g.bp[k + 1] - bq0;
+    }
+    const int pc = __popc(uint32_t(ma));
+    int pinc = pc, linc = blen;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t1 = __shfl_up(pinc, d), t2 = __shfl_up(linc, d);
+        if (lane >= d) {
+            pinc += t1;
+            linc += t2;
+        }
+    }
+    const int aoff = g.avp[i] + pinc - pc;
+    const int pst = linc - blen;
+    const int ptot = __shfl(linc, 63);
+    // product t = lane: owner = last A block j < na with pst[j] <= t
+    const int t = lane;
+    int own = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1) {
+        const int cand = own + step;
+        const int v = __shfl(pst, cand < 64 ? cand : 63);
+        if (cand < na && v <= t) own = cand;
+    }
+    const int q = __shfl(bq0, own) + (t - __shfl(pst, own));
+    const int pma = __shfl(ma, own);
+    const int paoff = __shfl(aoff, own);
+    int pmb = 0, pboff = 0, bcol = 0;
+    if (t < ptot) {
+        pmb = g.bm[q];
+        bcol = g.bc[q];
+        pboff = boff[q];
+    }
+    const bool valid = (t < ptot) && bitmap_mul(uint32_t(pma), uint32_t(pmb)) != 0u;
+    // C slot: number of C block columns below bcol
+    int slot = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1) {
+        const int cand = slot + step;
+        const int kv = __shfl(key, cand - 1);
+        if (cand <= cnt && kv < bcol) slot = cand;
+    }
+    int voff = cvp[i];
+    for (int g0 = 0; g0 < cnt; g0 += 4) {
+        const uint64_t m0 = __ballot(valid && slot == g0), m1 = __ballot(valid && slot == g0 + 1),
+                       m2 = __ballot(valid && slot == g0 + 2), m3 = __ballot(valid && slot == g0 + 3);
+        uint64_t mine = bs == 0 ? m0 : (bs == 1 ? m1 : (bs == 2 ? m2 : m3));
+        const int iters = max(max(__popcll(m0), __popcll(m1)), max(__popcll(m2), __popcll(m3)));
+        double acc = 0.0;
+        for (int it = 0; it < iters; it += 2) {
+            // two products per step: both gathers before the MFMAs
+            double a[2], b[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const bool on = mine != 0ull;
+                const int src = on ? __builtin_ctzll(mine) : 0;
+                if (on) mine &= mine - 1ull;
+                const uint32_t xa = uint32_t(__shfl(pma, src)), xb = uint32_t(__shfl(pmb, src));
+                const int oa = __shfl(paoff, src), ob = __shfl(pboff, src);
+                a[u] = (on && ((xa >> eA) & 1u)) ? g.av[oa + __popc(xa & belowA)] : 0.0;
+                b[u] = (on && ((xb >> eB) & 1u)) ? g.bv[ob + __popc(xb & belowB)] : 0.0;
+            }
+            acc = __builtin_amdgcn_mfma_f64_4x4x4f64(a[0], b[0], acc, 0, 0, 0);
+            if (it + 1 < iters) acc = __builtin_amdgcn_mfma_f64_4x4x4f64(a[1], b[1], acc, 0, 0, 0);
+        }
+        // lane (slot bs, bit eB) holds entry eB of C block g0 + bs
+        const int tb = g0 + bs;
+        const uint32_t cm = tb < cnt ? uint32_t(cmap[c0 + tb]) : 0u;
+        const int cpc = __popc(cm);
+        const int p0c = __shfl(cpc, 0), p1c = __shfl(cpc, 4), p2c = __shfl(cpc, 8),
+                  p3c = __shfl(cpc, 12);
+        const int excl = (bs > 0 ? p0c : 0) + (bs > 1 ? p1c : 0) + (bs > 2 ? p2c : 0);
+        if ((cm >> eB) & 1u) cval[voff + excl + __popc(cm & belowB)] = acc;
+        voff += p0c + p1c + p2c + p3c;
+    }
+}
+
+// Numeric bin of a C block row: 0 = register kernel (<= 64 products and A
+// blocks), 1 = LDS accumulators (<= kNumCap C blocks), 2 = global slab.
+struct NumericBin {
+    const int64_t *ub;
+    const int32_t *ap;
+    const int32_t *ccnt;
+    __device__ int64_t operator()(int i) const {
+        if (ub[i] <= 64 && ap[i + 1] - ap[i] <= 64) return 0;
+        return ccnt[i] <= kNumCap ? 1 : 2;
+    }
+};
 
 // popcount of every block bitmap (values are stored in block order, so the
 // exclusive scan of these is each block's value offset)
@@ -539,7 +750,7 @@ int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *Cp) {
     RowStats *tots = nullptr;  // [0]: upper-bound pass, [1]: C block counts
     RowStats ht[2] = {};
     int64_t *ub = nullptr, *ub_off = nullptr;
-    int32_t *list = nullptr, *ccnt = nullptr, *cnnz = nullptr, *big = nullptr;
+    int32_t *list = nullptr, *ccnt = nullptr, *cnnz = nullptr;
     int32_t *tcol = nullptr;
     uint32_t *tmap = nullptr, *slab = nullptr;
     double *nslab = nullptr;
@@ -553,7 +764,6 @@ int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *Cp) {
         tmp_free(list, s);
         tmp_free(ccnt, s);
         tmp_free(cnnz, s);
-        tmp_free(big, s);
         tmp_free(tcol, s);
         tmp_free(tmap, s);
         tmp_free(slab, s);
@@ -586,7 +796,6 @@ int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *Cp) {
     SG_TRY(tmp_alloc_t(&list, size_t(brow) * 3 + 1, s));
     SG_TRY(tmp_alloc_t(&ccnt, size_t(brow) + 1, s));
     SG_TRY(tmp_alloc_t(&cnnz, size_t(brow) + 1, s));
-    SG_TRY(tmp_alloc_t(&big, size_t(brow) + 1, s));
     const unsigned gr = unsigned(rowpass::groups(brow));
     if (brow > 0)
         hipLaunchKernelGGL(k_upper_bound, dim3(gr), dim3(256), 0, s, brow, A.blk_ptr, A.blk_col,
@@ -594,7 +803,7 @@ int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *Cp) {
     hipLaunchKernelGGL(rowpass::k_pass_finish, dim3(1), dim3(256), 0, s, int32_t(gr), parts,
                        &tots[0]);
     SG_HIP(hipGetLastError());
-    SG_TRY(rowpass::scatter(brow, rowpass::ArrayI64{ub}, kBin0Ub, kBin1Ub, parts, list,
+    SG_TRY(rowpass::scatter(brow, SymbolicBin{ub, A.blk_ptr}, 0, kBin1Ub, parts, list,
                             int64_t(brow), 7u, s));
     SG_TRY(exclusive_scan_i64(ub, ub_off, brow, s));
     int64_t total_ub = 0;
@@ -605,7 +814,7 @@ int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *Cp) {
     SG_TRY(tmp_alloc_t(&tcol, size_t(total_ub), s));
     SG_TRY(tmp_alloc_t(&tmap, size_t(total_ub), s));
     if (bin[0] > 0)
-        hipLaunchKernelGGL(k_symbolic_wave, dim3((bin[0] + 3) / 4), dim3(256), 0, s, bin[0],
+        hipLaunchKernelGGL(k_symbolic_reg, dim3((bin[0] + 3) / 4), dim3(256), 0, s, bin[0],
                            list, g, ub_off, tcol, tmap, ccnt, cnnz);
     if (bin[1] > 0)
         hipLaunchKernelGGL(k_symbolic_block, dim3(bin[1]), dim3(256), 0, s, bin[1],
@@ -632,15 +841,15 @@ int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *Cp) {
     SG_TRY(exclusive_scan_i32(cnnz, C.val_ptr, brow, s));
     // C block counts: rows over kNumCap go to the global-memory numeric kernel;
     // largest count and sum of squares give max_row_blocks and stand
-    SG_TRY(rowpass::count(brow, rowpass::ArrayI32{ccnt}, kNumCap, INT64_MAX, parts, &tots[1], s));
-    SG_TRY(rowpass::scatter(brow, rowpass::ArrayI32{ccnt}, kNumCap, INT64_MAX, parts, big, 0,
-                            2u, s));
+    const NumericBin nbin{ub, A.blk_ptr, ccnt};
+    SG_TRY(rowpass::count(brow, nbin, rowpass::ArrayI32{ccnt}, 0, 1, parts, &tots[1], s));
+    SG_TRY(rowpass::scatter(brow, nbin, 0, 1, parts, list, int64_t(brow), 7u, s));
     int32_t nblk = 0, nnz = 0;
     SG_HIP(hipMemcpyAsync(&nblk, C.blk_ptr + brow, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     SG_HIP(hipMemcpyAsync(&nnz, C.val_ptr + brow, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     SG_HIP(hipMemcpyAsync(&ht[1], &tots[1], sizeof(RowStats), hipMemcpyDeviceToHost, s));
     SG_HIP(hipStreamSynchronize(s));  // sync 2: sizes of C
-    const int32_t big_rows = ht[1].cnt[1];
+    const int32_t reg_rows = ht[1].cnt[0], lds_rows = ht[1].cnt[1], big_rows = ht[1].cnt[2];
     C.nblk = nblk;
     C.nnz = nnz;
     C.max_row_blocks = ht[1].max_blocks;
@@ -669,8 +878,13 @@ int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *Cp) {
                                B.nblk, B.blk_map, bpop);
         SG_HIP(hipGetLastError());
         SG_TRY(exclusive_scan_i32(bpop, boff, B.nblk, s));
-        hipLaunchKernelGGL(k_numeric_lds, dim3((brow + 3) / 4), dim3(256), 0, s, brow, g, boff,
-                           C.blk_ptr, C.blk_col, C.blk_map, C.val_ptr, C.val);
+        if (reg_rows > 0)
+            hipLaunchKernelGGL(k_numeric_reg, dim3((reg_rows + 3) / 4), dim3(256), 0, s, reg_rows,
+                               list, g, boff, C.blk_ptr, C.blk_col, C.blk_map, C.val_ptr, C.val);
+        if (lds_rows > 0)
+            hipLaunchKernelGGL(k_numeric_lds, dim3((lds_rows + 3) / 4), dim3(256), 0, s, lds_rows,
+                               list + brow, g, boff, C.blk_ptr, C.blk_col, C.blk_map, C.val_ptr,
+                               C.val);
         if (big_rows > 0) {
             const int64_t elems = int64_t(C.max_row_blocks) * 16;
             int nwaves = std::min(big_rows, 1024);
@@ -678,7 +892,8 @@ int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *Cp) {
             nwaves = (nwaves + 3) / 4 * 4;  // every wave of the grid owns a slab
             SG_TRY(tmp_alloc_t(&nslab, size_t(elems * nwaves), s));
             hipLaunchKernelGGL(k_numeric_global, dim3((nwaves + 3) / 4), dim3(256), 0, s,
-                               big_rows, big, g, boff, C.blk_ptr, C.blk_col, C.blk_map, C.val_ptr,
+                               big_rows, list + 2 * int64_t(brow), g, boff, C.blk_ptr, C.blk_col,
+                               C.blk_map, C.val_ptr,
                                C.val, nslab, elems);
         }
         SG_HIP(hipGetLastError());
