@@ -19,6 +19,8 @@ AMGT_ERR_HIP = 4
 AMGT_ERR_INPUT = 5
 AMGT_ERR_UNSUPPORTED = 6
 
+AMGT_DIAG_VALUE = 0
+AMGT_DIAG_INV_ABS = 1
 AMGT_SPMV_DIRECT = 0
 AMGT_SPMV_STAGED = 1
 AMGT_SPMV_BLOCK = 2
@@ -38,6 +40,7 @@ EXPORTS = (
     "amgt_mbsr_set_spmv_kernel",
     "amgt_mbsr_autotune", "amgt_mbsr_spmv", "amgt_mbsr_spmm", "amgt_mbsr_spgemm",
     "amgt_mbsr_to_csr", "amgt_free", "amgt_csr_spgemm", "amgt_csr_matvec",
+    "amgt_mbsr_diagonal", "amgt_mbsr_relax_jacobi", "amgt_mbsr_transpose",
 )
 
 
@@ -118,6 +121,9 @@ def load():
                                    _vp, _pvp, _pi64, _pvp, _pvp, _pvp, _pvp, _vp]),
         "amgt_csr_matvec": (_int, [_int, _dbl, _i32, _i32, _i64, _vp, _vp, _vp, _pvp, _vp, _dbl,
                                    _vp, _vp, _vp]),
+        "amgt_mbsr_diagonal": (_int, [_vp, _int, _vp, _vp]),
+        "amgt_mbsr_transpose": (_int, [_vp, _vp, _pvp]),
+        "amgt_mbsr_relax_jacobi": (_int, [_vp, _dbl, _vp, _vp, _vp, _vp, _vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -1,0 +1,326 @@
+"""V-cycle and PCG on the mBSR path: the callers of the SpMV and SpGEMM.
+
+This is the solve side that drives the library the way BoomerAMG does
+(hypre_BoomerAMGCycle, par_cycle.c:443,704,751; relax type 18,
+par_relax.c:338-355; PCG, krylov/pcg.c), so that setup+solve can be timed in
+FP64 and in mixed precision on one MI355X.  The hierarchy is geometric, not
+BoomerAMG's: P is trilinear interpolation to the grid of even-index points
+(coarsening by 2 per direction), built on the device, R = P^T as the library's
+mBSR transpose, and every coarse operator is the
+Galerkin product R (A P) computed by the library's mBSR SpGEMM, as
+hypre_BoomerAMGBuildCoarseOperatorKT / RAPKT does on the GPU
+(par_csr_triplemat_device.c:844).  HMIS coarsening and ext+i interpolation
+(BoomerAMG's own setup) are not rebuilt here.
+
+Per level: A, P, R as mBSR handles tagged with the level, 1/|a_ii| for the
+smoother (inverse l1 norms of relax 18 on one process), and fixed work
+vectors.  V(1,1):
+    u  = w D^-1 f                      (first sweep from u = 0: no matvec, as HYPRE)
+    r  = f - A u                       (SpMV, residual form)
+    f' = R r,  coarse V-cycle -> u'    (SpMV)
+    u += P u'                          (SpMV with b aliasing y)
+    u2 = u + w D^-1 (f - A u)          (one fused SpMV launch, amgt_mbsr_relax_jacobi)
+The coarsest level (<= coarse_size rows) applies its dense inverse, held as
+an mBSR matrix with full blocks, with one SpMV.  Every step is a library
+kernel on the current stream, so one V-cycle can be captured in a hipGraph
+(torch.cuda.CUDAGraph) and replayed.
+
+Precision (mixed mode): per-level policies for SpMV/smoothing and for the
+Galerkin SpGEMM; the defaults follow the mixed configuration of the workload
+definition: FP64 on the finest level, FP32 on levels 1-2, FP16 below (FP32
+accumulation), FP64 SpGEMM on the two finest levels and the coarsest solve in
+FP64.  PCG itself (level 0 vectors, dot products, updates) is FP64.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import workloads as W
+from .mbsr import MBSRMatrix
+
+
+def _interp_1d_torch(n: int, device):
+    """Coarsening by 2 in one direction (coarse points at even fine indices,
+    linear weights, zero Dirichlet values outside): per fine index the low and
+    high coarse neighbours, their weights, and whether the high one exists."""
+    f = torch.arange(n, device=device)
+    nc = (n + 1) // 2
+    even = (f % 2) == 0
+    lo = f // 2
+    hi = lo + 1
+    w_lo = torch.where(even, 1.0, 0.5).to(torch.float64)
+    has_hi = (~even) & (hi < nc)
+    w_hi = torch.where(has_hi, 0.5, 0.0).to(torch.float64)
+    return nc, lo, hi, w_lo, w_hi, has_hi
+
+
+def trilinear_interpolation_device(grid, device):
+    """CSR (I, J, A) of the trilinear prolongation from the even-index grid to
+    the full nx x ny x nz grid, built on the device (same arrays as
+    workloads.trilinear_interpolation: up to 8 entries per row, ascending
+    columns, no stored zeros).  Returns (I, J, A, ncoarse)."""
+    nx, ny, nz = grid
+    (ncx, lx, hx, wlx, whx, okx), (ncy, ly, hy, wly, why, oky), (ncz, lz, hz, wlz, whz, okz) = (
+        _interp_1d_torch(n, device) for n in (nx, ny, nz))
+    n = nx * ny * nz
+    idx = torch.arange(n, device=device)
+    x, y, z = idx % nx, (idx // nx) % ny, idx // (nx * ny)
+    cols, vals, mask = [], [], []
+    # slots in (kz, ky, kx) order give ascending columns (high = low + 1)
+    for cz, wz, mz in ((lz, wlz, None), (hz, whz, okz)):
+        for cy, wy, my in ((ly, wly, None), (hy, why, oky)):
+            for cx, wx, mx in ((lx, wlx, None), (hx, whx, okx)):
+                v = wx[x] * wy[y] * wz[z]
+                m = v != 0.0
+                for mm, q in ((mx, x), (my, y), (mz, z)):
+                    if mm is not None:
+                        m = m & mm[q]
+                cols.append((cz[z] * ncy + cy[y]) * ncx + cx[x])
+                vals.append(v)
+                mask.append(m)
+    C = torch.stack(cols, 1)
+    V = torch.stack(vals, 1)
+    M = torch.stack(mask, 1)
+    I = torch.zeros(n + 1, dtype=torch.int64, device=device)
+    I[1:] = torch.cumsum(M.sum(1), 0)
+    return I.to(torch.int32), C[M].to(torch.int32), V[M], ncx * ncy * ncz
+
+
+def fp64_policy(level: int) -> int:
+    return L.AMGT_DOUBLE
+
+
+def mixed_spmv_policy(level: int) -> int:
+    """FP64 finest, FP32 levels 1-2, FP16 below."""
+    if level == 0:
+        return L.AMGT_DOUBLE
+    return L.AMGT_FLOAT if level <= 2 else L.AMGT_HALF
+
+
+def mixed_spgemm_policy(level: int) -> int:
+    """Galerkin products in FP64 on the two finest levels, FP32 below."""
+    return L.AMGT_DOUBLE if level <= 1 else L.AMGT_FLOAT
+
+
+@dataclass
+class Level:
+    n: int
+    grid: tuple
+    A: MBSRMatrix
+    dinv: torch.Tensor
+    P: MBSRMatrix | None = None
+    R: MBSRMatrix | None = None
+    f: torch.Tensor | None = None
+    u: torch.Tensor | None = None
+    u2: torch.Tensor | None = None
+    r: torch.Tensor | None = None
+
+
+@dataclass
+class SetupTimes:
+    host_s: float = 0.0          # coarsest dense inverse (numpy)
+    device_s: float = 0.0        # P, R = P^T, conversions, Galerkin SpGEMMs, diagonals
+    total_s: float = 0.0
+    spgemm_s: list = field(default_factory=list)
+
+
+class GeometricAMG:
+    """Galerkin hierarchy on a structured nx x ny x nz grid and its V(1,1) cycle."""
+
+    def __init__(self, I, J, V, grid, device, omega: float = 1.0, coarse_size: int = 512,
+                 spmv_policy=fp64_policy, spgemm_policy=fp64_policy, max_levels: int = 25):
+        self.device = device
+        self.omega = float(omega)
+        self.coarse_size = int(coarse_size)
+        self.spmv_policy = spmv_policy
+        self.spgemm_policy = spgemm_policy
+        self.max_levels = max_levels
+        self.levels: list[Level] = []
+        self.coarse_inv: MBSRMatrix | None = None
+        self.times = SetupTimes()
+        self._graph = None
+        self._setup(I, J, V, tuple(grid))
+
+    # ---- setup -------------------------------------------------------------------------
+    def _t(self, a):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
+
+    def _setup(self, I, J, V, grid):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        n = int(np.prod(grid))
+        A = MBSRMatrix.from_csr(I, J, V, n, n)
+        lvl = 0
+        while True:
+            A.set_level(lvl)
+            A.set_precision(self.spmv_policy(lvl))
+            dinv = A.diagonal(inverse_abs=True)
+            L_ = Level(n=n, grid=grid, A=A, dinv=dinv)
+            self.levels.append(L_)
+            if n <= self.coarse_size or lvl + 1 >= self.max_levels or min(grid) <= 1:
+                break
+            PI, PJ, PA, nc = trilinear_interpolation_device(grid, self.device)
+            P = MBSRMatrix.from_csr(PI, PJ, PA, n, nc)
+            R = P.transpose()
+            del PI, PJ, PA
+            for M in (P, R):
+                M.set_level(lvl)
+                M.set_precision(self.spmv_policy(lvl))
+            # Galerkin product in the level's SpGEMM precision (follows the left operand)
+            torch.cuda.synchronize()
+            tg = time.perf_counter()
+            A.set_precision(self.spgemm_policy(lvl))
+            AP = A.spgemm(P)
+            R.set_precision(self.spgemm_policy(lvl))
+            Ac = R.spgemm(AP)
+            torch.cuda.synchronize()
+            self.times.spgemm_s.append(time.perf_counter() - tg)
+            A.set_precision(self.spmv_policy(lvl))
+            R.set_precision(self.spmv_policy(lvl))
+            del AP
+            L_.P, L_.R = P, R
+            A, n = Ac, nc
+            grid = tuple((g + 1) // 2 for g in grid)
+            lvl += 1
+        # coarsest: dense inverse as an mBSR matrix with full blocks, applied in FP64
+        torch.cuda.synchronize()
+        th = time.perf_counter()
+        Lc = self.levels[-1]
+        CI, CJ, CV = Lc.A.to_csr()
+        dense = np.zeros((Lc.n, Lc.n))
+        ci, cj, cv = CI.cpu().numpy(), CJ.cpu().numpy(), CV.cpu().numpy()
+        rows = np.repeat(np.arange(Lc.n), np.diff(ci))
+        dense[rows, cj] = cv
+        inv = np.linalg.inv(dense)
+        II = (np.arange(Lc.n + 1) * Lc.n).astype(np.int32)
+        JJ = np.tile(np.arange(Lc.n, dtype=np.int32), Lc.n)
+        self.coarse_inv = MBSRMatrix.from_csr(self._t(II), self._t(JJ), self._t(inv.ravel()),
+                                              Lc.n, Lc.n)
+        self.coarse_inv.set_precision(L.AMGT_DOUBLE)
+        torch.cuda.synchronize()
+        host = time.perf_counter() - th
+        # work vectors (fixed addresses, so a captured V-cycle stays valid)
+        for L_ in self.levels:
+            z = lambda: torch.zeros(L_.n, dtype=torch.float64, device=self.device)  # noqa: E731
+            L_.f, L_.u, L_.u2, L_.r = z(), z(), z(), z()
+        torch.cuda.synchronize()
+        self.times.total_s = time.perf_counter() - t0
+        self.times.host_s = host
+        self.times.device_s = self.times.total_s - host
+
+    # ---- cycle ------------------------------------------------------------------------
+    def _vcycle(self, lvl: int) -> torch.Tensor:
+        """Solves approximately A_lvl u = f_lvl; returns the tensor holding u."""
+        L_ = self.levels[lvl]
+        if lvl == len(self.levels) - 1:
+            return self.coarse_inv.spmv(L_.f, L_.u)
+        # pre-smoothing from u = 0: u = w D^-1 f
+        torch.mul(L_.dinv, L_.f, out=L_.u)
+        if self.omega != 1.0:
+            L_.u.mul_(self.omega)
+        # residual and restriction
+        L_.A.spmv(L_.u, L_.r, alpha=-1.0, beta=1.0, b=L_.f)
+        Lc = self.levels[lvl + 1]
+        L_.R.spmv(L_.r, Lc.f)
+        uc = self._vcycle(lvl + 1)
+        # coarse-grid correction u += P uc, then post-smoothing into u2
+        L_.P.spmv(uc, L_.u, alpha=1.0, beta=1.0, b=L_.u)
+        L_.A.relax_jacobi(L_.f, L_.u, L_.dinv, self.omega, out=L_.u2)
+        return L_.u2
+
+    def precondition(self, r: torch.Tensor, z: torch.Tensor) -> torch.Tensor:
+        """z = M^-1 r with one V(1,1) cycle (captured graph if built)."""
+        L0 = self.levels[0]
+        L0.f.copy_(r)
+        if self._graph is not None:
+            self._graph.replay()
+            out = self._graph_out
+        else:
+            out = self._vcycle(0)
+        z.copy_(out)
+        return z
+
+    def capture(self, warmup: int = 2):
+        """Captures one V-cycle (on the fixed work vectors) in a hipGraph."""
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            for _ in range(warmup):
+                self._vcycle(0)
+        torch.cuda.current_stream().wait_stream(s)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            out = self._vcycle(0)
+        self._graph, self._graph_out = g, out
+        return g
+
+    # ---- Krylov ------------------------------------------------------------------------
+    def pcg(self, b: torch.Tensor, tol: float = 1e-8, max_iter: int = 200):
+        """Preconditioned CG (two-norm relative residual, as ij -tol with PCG)."""
+        A = self.levels[0].A
+        x = torch.zeros_like(b)
+        r = b.clone()
+        z = torch.empty_like(b)
+        Ap = torch.empty_like(b)
+        bnorm = torch.linalg.vector_norm(b).item()
+        if bnorm == 0.0:
+            return x, 0, [0.0]
+        self.precondition(r, z)
+        p = z.clone()
+        rz = torch.dot(r, z)
+        hist = [1.0]
+        it = 0
+        while it < max_iter:
+            A.spmv(p, Ap)
+            alpha = rz / torch.dot(p, Ap)
+            x.add_(alpha * p)
+            r.sub_(alpha * Ap)
+            it += 1
+            rel = torch.linalg.vector_norm(r).item() / bnorm
+            hist.append(rel)
+            if rel < tol:
+                break
+            self.precondition(r, z)
+            rz_new = torch.dot(r, z)
+            p.mul_(rz_new / rz).add_(z)
+            rz = rz_new
+        return x, it, hist
+
+    def operator_complexity(self) -> float:
+        nnz = [L_.A.info()["nnz"] for L_ in self.levels]
+        return sum(nnz) / nnz[0]
+
+    def describe(self) -> list:
+        return [{"level": i, "rows": L_.n, "nnz": L_.A.info()["nnz"],
+                 "spmv_precision": self.spmv_policy(i),
+                 "spgemm_precision": self.spgemm_policy(i) if L_.P is not None else None}
+                for i, L_ in enumerate(self.levels)]
+
+
+def solve_poisson(nx: int, ny: int, nz: int, b: np.ndarray, device, mixed: bool = False,
+                  tol: float = 1e-8, capture: bool = True, omega: float = 1.0,
+                  coarse_size: int = 512):
+    """7-pt Poisson on nx x ny x nz: setup + PCG/V(1,1) solve; returns a dict of
+    timings and the solution (the timed solve includes the residual checks)."""
+    I, J, V = W.stencil(7, nx, ny, nz)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
+    dI, dJ, dV = t(I), t(J), t(V)
+    db = t(b)
+    torch.cuda.synchronize()
+    amg = GeometricAMG(dI, dJ, dV, (nx, ny, nz), device, omega=omega, coarse_size=coarse_size,
+                       spmv_policy=mixed_spmv_policy if mixed else fp64_policy,
+                       spgemm_policy=mixed_spgemm_policy if mixed else fp64_policy)
+    if capture:
+        amg.capture()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, iters, hist = amg.pcg(db, tol=tol)
+    torch.cuda.synchronize()
+    solve_s = time.perf_counter() - t0
+    return {"amg": amg, "x": x, "iters": iters, "hist": hist, "setup": amg.times,
+            "solve_s": solve_s}

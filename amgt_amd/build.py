@@ -16,7 +16,8 @@ OUT = os.path.join(HERE, "libamgt_mbsr.so")
 OBJDIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["core.hip", "csr2mbsr.hip", "spmv.hip", "transpose.hip", "mbsr2csr.hip", "spgemm.hip"]
+SOURCES = ["core.hip", "csr2mbsr.hip", "spmv.hip", "transpose.hip", "mbsr2csr.hip", "spgemm.hip",
+           "relax.hip"]
 
 CFLAGS = [
     "--offload-arch=gfx950",

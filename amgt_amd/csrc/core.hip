@@ -513,6 +513,46 @@ int amgt_mbsr_spmm(amgt_mbsr_t A, int trans, int k, double alpha, const double *
     return spmm(*op, prec, lanes_of(*op), k, alpha, d_x, ldx, beta, d_b, ldb, d_y, ldy, s);
 }
 
+int amgt_mbsr_transpose(amgt_mbsr_t A, void *stream, amgt_mbsr_t *AT) {
+    AMGT_CHECK_ARG(A != nullptr && AT != nullptr, "NULL argument");
+    *AT = nullptr;
+    hipStream_t s = as_stream(stream);
+    auto *h = new amgt_mbsr();
+    int st = mbsr_transpose(A->m, s, &h->m);
+    if (st != AMGT_OK) {
+        mbsr_release(&h->m, s);
+        delete h;
+        return st;
+    }
+    h->m.level = A->m.level;
+    *AT = h;
+    return AMGT_OK;
+}
+
+int amgt_mbsr_diagonal(amgt_mbsr_t A, int kind, double *d_diag, void *stream) {
+    AMGT_CHECK_ARG(A != nullptr, "handle is NULL");
+    AMGT_CHECK_ARG(kind == AMGT_DIAG_VALUE || kind == AMGT_DIAG_INV_ABS, "unknown diagonal kind");
+    AMGT_CHECK_ARG(!A->m.pattern_only, "pattern-only matrix has no values");
+    const int32_t n = A->m.nrows < A->m.ncols ? A->m.nrows : A->m.ncols;
+    AMGT_CHECK_ARG(d_diag != nullptr || n == 0, "output is NULL");
+    return mbsr_diagonal(A->m, kind, d_diag, as_stream(stream));
+}
+
+int amgt_mbsr_relax_jacobi(amgt_mbsr_t A, double omega, const double *d_dinv, const double *d_f,
+                           const double *d_u, double *d_u_new, void *stream) {
+    AMGT_CHECK_ARG(A != nullptr, "handle is NULL");
+    Mbsr &m = A->m;
+    AMGT_CHECK_ARG(m.nrows == m.ncols, "Jacobi sweep needs a square matrix");
+    AMGT_CHECK_ARG(!m.pattern_only, "pattern-only matrix has no values");
+    AMGT_CHECK_ARG(m.nrows == 0 || (d_dinv && d_f && d_u && d_u_new), "NULL vector");
+    AMGT_CHECK_ARG(d_u != d_u_new || m.nrows == 0, "u_new must not alias u");
+    hipStream_t s = as_stream(stream);
+    const int prec = effective_precision(m);
+    AMGT_TRY(mbsr_ensure_lowp(&m, prec, s));
+    AMGT_TRY(spmv_prepare(&m, lanes_of(m), s));
+    return relax_jacobi(m, prec, lanes_of(m), omega, d_dinv, d_f, d_u, d_u_new, s);
+}
+
 int amgt_mbsr_spgemm(amgt_mbsr_t A, amgt_mbsr_t B, void *stream, amgt_mbsr_t *C) {
     AMGT_CHECK_ARG(A != nullptr && B != nullptr && C != nullptr, "NULL argument");
     *C = nullptr;
@@ -521,8 +561,15 @@ int amgt_mbsr_spgemm(amgt_mbsr_t A, amgt_mbsr_t B, void *stream, amgt_mbsr_t *C)
         return AMGT_ERR_DIM;
     }
     hipStream_t s = as_stream(stream);
+    // operand precision of A's level (get_calculationPrecison(hypre_Level(A)),
+    // csr_spgemm_device.c:2592) or A's explicit precision
+    const int prec = effective_precision(A->m);
+    if (!A->m.pattern_only && !B->m.pattern_only) {
+        AMGT_TRY(mbsr_ensure_lowp(&A->m, prec, s));
+        AMGT_TRY(mbsr_ensure_lowp(&B->m, prec, s));
+    }
     auto *h = new amgt_mbsr();
-    int st = spgemm(A->m, B->m, s, &h->m);
+    int st = spgemm(A->m, B->m, prec, s, &h->m);
     if (st != AMGT_OK) {
         mbsr_release(&h->m, s);
         delete h;

@@ -9,9 +9,10 @@
 // with the diagonal), unlike the reference's windowed bitmask that assumes
 // sorted columns once ncols > 32768 (csr_matvec_device.c:701-722).
 //
-// Bins by entries per block row: <= 64 (one wave, register bitonic sort over
-// lanes), <= 1024 (256-thread workgroup, LDS bitonic), <= 8192 (1024-thread
-// workgroup, LDS bitonic).  Larger block rows return AMGT_ERR_UNSUPPORTED.
+// Bins by entries per block row: <= 64 (one wave, rank sort over an LDS copy
+// of the keys), <= 1024 (256-thread workgroup, LDS bitonic), <= 8192
+// (1024-thread workgroup, LDS bitonic).  Larger block rows return
+// AMGT_ERR_UNSUPPORTED.
 #include "internal.hpp"
 #include "rowpass.hpp"
 
@@ -59,6 +60,11 @@ struct RowEntries {
     __device__ int64_t operator()(int i) const { return I[min(4 * i + 4, nrows)] - I[4 * i]; }
 };
 
+__device__ __forceinline__ void wave_sync_conv() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // Key of CSR entry p in block row starting at row r0.
 __device__ __forceinline__ uint32_t entry_key(int p, int r0, int nrows, int ncols,
                                               const int32_t *__restrict__ I,
@@ -73,13 +79,15 @@ __device__ __forceinline__ uint32_t entry_key(int p, int r0, int nrows, int ncol
     return (uint32_t(c >> 2) << 4) | uint32_t(r << 2) | uint32_t(c & 3);
 }
 
-// ---- bin 0: one wave per block row, <= 64 entries, bitonic sort across lanes.
+// ---- bin 0: one wave per block row, <= 64 entries, rank sort in LDS.
 // tmp_col/tmp_map receive the row's blocks at offset I[4br] (upper-bound slot).
 __global__ __launch_bounds__(256) void k_sort_wave(
     int32_t nrows, int32_t ncols, const int32_t *__restrict__ I, const int32_t *__restrict__ J,
     const double *__restrict__ A, int32_t brow, double *__restrict__ val,
     int32_t *__restrict__ tmp_col, uint32_t *__restrict__ tmp_map,
     int32_t *__restrict__ blk_cnt, ConvCounters *cnt, int values_only) {
+    __shared__ __align__(16) uint32_t s_key[4][64];
+    __shared__ uint32_t s_sorted[4][64];
     const int lane = threadIdx.x & 63;
     const int br = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (br >= brow) return;  // whole wave exits together
@@ -95,48 +103,52 @@ __global__ __launch_bounds__(256) void k_sort_wave(
         src = s + lane;
         key = entry_key(src, r0, nrows, ncols, I, J, &err);
     }
-    // bitonic sort, ascending by key, across the 64 lanes
-#pragma unroll
-    for (int k = 2; k <= 64; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            uint32_t pk = __shfl_xor(key, j);
-            int ps = __shfl_xor(src, j);
-            bool up = (lane & k) == 0;
-            bool lower = (lane & j) == 0;
-            bool take = (lower == up) ? (pk < key) : (pk > key);
-            if (take) {
-                key = pk;
-                src = ps;
-            }
-        }
-    }
-    uint32_t prev = __shfl_up(key, 1);
+    // Rank sort: every lane counts the keys below its own over an LDS copy of
+    // the row's keys (four per read; ties, i.e. duplicate entries, by lane), so
+    // the work is about three VALU operations per key instead of a 21-stage
+    // shuffle network (this kernel is bound by VALU issue).
+    const int w = threadIdx.x >> 6;
+    s_key[w][lane] = key;
+    wave_sync_conv();
     const bool valid = lane < n;
+    int rank = 0;
+    for (int j = 0; j < n; j += 4) {
+        const uint4 kk = *reinterpret_cast<const uint4 *>(&s_key[w][j]);
+        rank += (kk.x < key || (kk.x == key && j < lane)) ? 1 : 0;
+        rank += (kk.y < key || (kk.y == key && j + 1 < lane)) ? 1 : 0;
+        rank += (kk.z < key || (kk.z == key && j + 2 < lane)) ? 1 : 0;
+        rank += (kk.w < key || (kk.w == key && j + 3 < lane)) ? 1 : 0;
+    }
+    if (valid) {
+        s_sorted[w][rank] = key;
+        if (A) val[s + rank] = A[src];
+    }
+    wave_sync_conv();
+    // lane p now holds the p-th smallest key
+    key = valid ? s_sorted[w][lane] : kPadKey;
+    const uint32_t prev = __shfl_up(key, 1);
     if (valid && lane > 0 && prev == key) err |= kErrDup;
-    if (A && valid) val[s + lane] = A[src];
     if (!values_only) {
         const bool head = valid && (lane == 0 || (prev >> 4) != (key >> 4));
         const uint64_t heads = __ballot(head);
-        // block rank of this lane's entry
-        const uint64_t le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-        const int rank = __popcll(heads & le) - 1;
-        // OR of the bits of the block: walk forward from each head (<= 16 entries)
-        uint32_t map = 0;
+        // OR of the bits of each block: segmented inclusive scan over equal
+        // block columns; the block's last entry holds the whole bitmap
+        uint32_t map = valid ? (1u << (key & 15u)) : 0u;
 #pragma unroll
-        for (int d = 0; d < 16; ++d) {
-            uint32_t kk = __shfl(key, min(lane + d, 63));
-            bool same = (lane + d < n) && ((kk >> 4) == (key >> 4));
-            // stop at the first entry of another block (keys are sorted)
-            if (same) map |= 1u << (kk & 15u);
+        for (int d = 1; d < 16; d <<= 1) {
+            const uint32_t k2 = __shfl_up(key, d);
+            const uint32_t m2 = __shfl_up(map, d);
+            if (lane >= d && (k2 >> 4) == (key >> 4)) map |= m2;
         }
-        if (head) {
-            tmp_col[s + rank] = int32_t(key >> 4);
-            tmp_map[s + rank] = map;
+        const uint32_t nxt = __shfl_down(key, 1);
+        const bool tail = valid && (lane == n - 1 || (nxt >> 4) != (key >> 4));
+        if (tail) {
+            const uint64_t le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+            const int brank = __popcll(heads & le) - 1;
+            tmp_col[s + brank] = int32_t(key >> 4);
+            tmp_map[s + brank] = map;
         }
-        if (lane == 0) {
-            blk_cnt[br] = __popcll(heads);
-        }
+        if (lane == 0) blk_cnt[br] = __popcll(heads);
     }
     if (err) atomicOr(&cnt->err, err);
 }

@@ -121,9 +121,16 @@ int spmm(const Mbsr &A, int precision, int lanes, int k, double alpha, const dou
          int64_t ldx, double beta, const double *b, int64_t ldb, double *y, int64_t ldy,
          hipStream_t s);
 int spmv_default_lanes(const Mbsr &A);
+// u_new = u + omega * dinv .* (f - A u) in one SpMV launch (square A)
+int relax_jacobi(const Mbsr &A, int precision, int lanes, double omega, const double *dinv,
+                 const double *f, const double *u, double *u_new, hipStream_t s);
+// diagonal of A (kind AMGT_DIAG_VALUE) or 1/|a_ii| (AMGT_DIAG_INV_ABS, 0 where a_ii = 0)
+int mbsr_diagonal(const Mbsr &A, int kind, double *out, hipStream_t s);
 // y = beta*b over n entries (beta == 0 writes zeros without reading b)
 int scale_vector(int32_t n, double beta, const double *b, double *y, hipStream_t s);
-int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *C);
+// C = A*B with operand values in `precision` (AMGT_DOUBLE / FLOAT / HALF; the
+// FP32/FP16 copies of A and B must exist, mbsr_ensure_lowp)
+int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *C);
 int mbsr_to_csr(const Mbsr &A, hipStream_t s, int64_t *nnz, int32_t **rowptr, int32_t **colidx,
                 double **val);
 void mbsr_release(Mbsr *A, hipStream_t s);

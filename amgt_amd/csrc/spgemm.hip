@@ -204,11 +204,35 @@ __device__ void symbolic_row(int i, int tid, uint32_t *hkey, uint32_t *hmap, int
 struct SpgemmArgs {
     const int32_t *ap, *ac, *avp;
     const uint16_t *am;
-    const double *av;
+    const void *av;   // A values: double, float or __half (the numeric kernels' VT)
     const int32_t *bp, *bc, *bvp;
     const uint16_t *bm;
-    const double *bv;
+    const void *bv;   // B values, same type as av
 };
+
+// value of type VT at index idx, widened to double
+template <typename VT>
+__device__ __forceinline__ double ldv(const void *p, int64_t idx) {
+    return double(static_cast<const VT *>(p)[idx]);
+}
+template <>
+__device__ __forceinline__ double ldv<__half>(const void *p, int64_t idx) {
+    return double(__half2float(static_cast<const __half *>(p)[idx]));
+}
+
+// C entry as stored: the FP64 product for FP64 operands; below FP64 the
+// entry is rounded to FP32 (the reference keeps C in the operands' precision,
+// csr_spgemm_device.c:938-1140 TF32 and 753-937 FP16 with FP16 accumulation;
+// here products of the rounded operands accumulate in FP64 and the entry is
+// rounded once to FP32)
+template <typename VT>
+__device__ __forceinline__ double round_out(double x) {
+    return double(float(x));
+}
+template <>
+__device__ __forceinline__ double round_out<double>(double x) {
+    return x;
+}
 
 // ---- bin 0: rows with at most 64 products and 64 A blocks, one wave per row.
 // Lane j holds A block j, lane t product t (owner found with a branchless
@@ -426,6 +450,7 @@ struct NumStage {
 // may hit the same C block, so their results are added one block slot after
 // the other (fixed order: deterministic, no atomics).  boff[q] is the value
 // offset of B block q (values are stored in block order).
+template <typename VT>
 __device__ void numeric_row(int i, const SpgemmArgs &g, const int32_t *__restrict__ boff,
                             const int32_t *__restrict__ cptr, const int32_t *__restrict__ ccol,
                             const uint16_t *__restrict__ cmap, const int32_t *__restrict__ cvp,
@@ -524,8 +549,8 @@ __device__ void numeric_row(int i, const SpgemmArgs &g, const int32_t *__restric
                     if (on[u]) {
                         const uint32_t xa = uint32_t(st.p_ma[idx]), xb = uint32_t(st.p_mb[idx]);
                         sl[u] = st.p_slot[idx];
-                        if ((xa >> eA) & 1u) av[u] = g.av[st.p_aoff[idx] + __popc(xa & belowA)];
-                        if ((xb >> eB) & 1u) bv[u] = g.bv[st.p_boff[idx] + __popc(xb & belowB)];
+                        if ((xa >> eA) & 1u) av[u] = ldv<VT>(g.av, st.p_aoff[idx] + __popc(xa & belowA));
+                        if ((xb >> eB) & 1u) bv[u] = ldv<VT>(g.bv, st.p_boff[idx] + __popc(xb & belowB));
                     }
                 }
 #pragma unroll
@@ -553,11 +578,12 @@ __device__ void numeric_row(int i, const SpgemmArgs &g, const int32_t *__restric
         const int p0c = __shfl(pc, 0), p1c = __shfl(pc, 16), p2c = __shfl(pc, 32),
                   p3c = __shfl(pc, 48);
         const int excl = (gs > 0 ? p0c : 0) + (gs > 1 ? p1c : 0) + (gs > 2 ? p2c : 0);
-        if ((m >> e) & 1u) cval[voff + excl + __popc(m & below)] = acc[t * 16 + e];
+        if ((m >> e) & 1u) cval[voff + excl + __popc(m & below)] = round_out<VT>(acc[t * 16 + e]);
         voff += p0c + p1c + p2c + p3c;
     }
 }
 
+template <typename VT>
 __global__ __launch_bounds__(256) void k_numeric_lds(int count, const int32_t *__restrict__ rows,
                                                      SpgemmArgs g,
                                                      const int32_t *__restrict__ boff,
@@ -572,9 +598,10 @@ __global__ __launch_bounds__(256) void k_numeric_lds(int count, const int32_t *_
     const int w = threadIdx.x >> 6;
     const int r = blockIdx.x * 4 + w;
     if (r >= count) return;
-    numeric_row(rows[r], g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w], s_st[w]);
+    numeric_row<VT>(rows[r], g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w], s_st[w]);
 }
 
+template <typename VT>
 __global__ __launch_bounds__(256) void k_numeric_global(int count, const int32_t *__restrict__ rows,
                                                         SpgemmArgs g,
                                                         const int32_t *__restrict__ boff,
@@ -590,7 +617,7 @@ __global__ __launch_bounds__(256) void k_numeric_global(int count, const int32_t
     const int nwaves = gridDim.x * 4;
     double *acc = slab + int64_t(wave) * slab_elems;
     for (int r = wave; r < count; r += nwaves) {
-        numeric_row(rows[r], g, boff, cptr, ccol, cmap, cvp, cval, nullptr, acc, s_st[w]);
+        numeric_row<VT>(rows[r], g, boff, cptr, ccol, cmap, cvp, cval, nullptr, acc, s_st[w]);
     }
 }
 
@@ -601,6 +628,7 @@ __global__ __launch_bounds__(256) void k_numeric_global(int count, const int32_t
 // g0 + b in registers, chaining its products (in product order) through the
 // MFMA's accumulator input, and the finished blocks go straight to the
 // compressed value array.  Deterministic, no atomics.
+template <typename VT>
 __global__ __launch_bounds__(256) void k_numeric_reg(int count, const int32_t *__restrict__ rows,
                                                      SpgemmArgs g,
                                                      const int32_t *__restrict__ boff,
@@ -684,8 +712,8 @@ __global__ __launch_bounds__(256) void k_numeric_reg(int count, const int32_t *_
                 if (on) mine &= mine - 1ull;
                 const uint32_t xa = uint32_t(__shfl(pma, src)), xb = uint32_t(__shfl(pmb, src));
                 const int oa = __shfl(paoff, src), ob = __shfl(pboff, src);
-                a[u] = (on && ((xa >> eA) & 1u)) ? g.av[oa + __popc(xa & belowA)] : 0.0;
-                b[u] = (on && ((xb >> eB) & 1u)) ? g.bv[ob + __popc(xb & belowB)] : 0.0;
+                a[u] = (on && ((xa >> eA) & 1u)) ? ldv<VT>(g.av, oa + __popc(xa & belowA)) : 0.0;
+                b[u] = (on && ((xb >> eB) & 1u)) ? ldv<VT>(g.bv, ob + __popc(xb & belowB)) : 0.0;
             }
             acc = __builtin_amdgcn_mfma_f64_4x4x4f64(a[0], b[0], acc, 0, 0, 0);
             if (it + 1 < iters) acc = __builtin_amdgcn_mfma_f64_4x4x4f64(a[1], b[1], acc, 0, 0, 0);
@@ -697,7 +725,7 @@ __global__ __launch_bounds__(256) void k_numeric_reg(int count, const int32_t *_
         const int p0c = __shfl(cpc, 0), p1c = __shfl(cpc, 4), p2c = __shfl(cpc, 8),
                   p3c = __shfl(cpc, 12);
         const int excl = (bs > 0 ? p0c : 0) + (bs > 1 ? p1c : 0) + (bs > 2 ? p2c : 0);
-        if ((cm >> eB) & 1u) cval[voff + excl + __popc(cm & belowB)] = acc;
+        if ((cm >> eB) & 1u) cval[voff + excl + __popc(cm & belowB)] = round_out<VT>(acc);
         voff += p0c + p1c + p2c + p3c;
     }
 }
@@ -722,9 +750,37 @@ __global__ __launch_bounds__(256) void k_block_popc(int64_t nblk, const uint16_t
     if (q < nblk) out[q] = __popc(uint32_t(map[q]));
 }
 
+// numeric kernels of one value precision; *nslab receives the global slab of
+// the rows over kNumCap blocks (freed by the caller)
+template <typename VT>
+int launch_numeric(const SpgemmArgs &g, const int32_t *boff, const Mbsr &C, int32_t brow,
+                   const int32_t *list, int32_t reg_rows, int32_t lds_rows, int32_t big_rows,
+                   double **nslab, hipStream_t s) {
+    if (reg_rows > 0)
+        hipLaunchKernelGGL((k_numeric_reg<VT>), dim3((reg_rows + 3) / 4), dim3(256), 0, s,
+                           reg_rows, list, g, boff, C.blk_ptr, C.blk_col, C.blk_map, C.val_ptr,
+                           C.val);
+    if (lds_rows > 0)
+        hipLaunchKernelGGL((k_numeric_lds<VT>), dim3((lds_rows + 3) / 4), dim3(256), 0, s,
+                           lds_rows, list + brow, g, boff, C.blk_ptr, C.blk_col, C.blk_map,
+                           C.val_ptr, C.val);
+    if (big_rows > 0) {
+        const int64_t elems = int64_t(C.max_row_blocks) * 16;
+        int nwaves = std::min(big_rows, 1024);
+        while (nwaves > 4 && elems * nwaves * 8 > (int64_t(1) << 31)) nwaves >>= 1;
+        nwaves = (nwaves + 3) / 4 * 4;  // every wave of the grid owns a slab
+        AMGT_TRY(tmp_alloc_t(nslab, size_t(elems * nwaves), s));
+        hipLaunchKernelGGL((k_numeric_global<VT>), dim3((nwaves + 3) / 4), dim3(256), 0, s,
+                           big_rows, list + 2 * int64_t(brow), g, boff, C.blk_ptr, C.blk_col,
+                           C.blk_map, C.val_ptr, C.val, *nslab, elems);
+    }
+    AMGT_HIP(hipGetLastError());
+    return AMGT_OK;
+}
+
 }  // namespace
 
-int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *Cp) {
+int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp) {
     Mbsr &C = *Cp;
     C = Mbsr();
     C.nrows = A.nrows;
@@ -743,8 +799,21 @@ int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *Cp) {
         set_error("SpGEMM: allocation failed");
         return AMGT_ERR_ALLOC;
     }
-    SpgemmArgs g{A.blk_ptr, A.blk_col, A.val_ptr, A.blk_map, A.val,
-                 B.blk_ptr, B.blk_col, B.val_ptr, B.blk_map, B.val};
+    // operand values in the product's precision (FP32/FP16 copies made by the caller)
+    const void *avals = A.val, *bvals = B.val;
+    if (precision == AMGT_FLOAT) {
+        avals = A.val32;
+        bvals = B.val32;
+    } else if (precision == AMGT_HALF) {
+        avals = A.val16;
+        bvals = B.val16;
+    }
+    if (!C.pattern_only && (!avals || !bvals)) {
+        set_error("SpGEMM: operand values missing for the requested precision");
+        return AMGT_ERR_ARG;
+    }
+    SpgemmArgs g{A.blk_ptr, A.blk_col, A.val_ptr, A.blk_map, avals,
+                 B.blk_ptr, B.blk_col, B.val_ptr, B.blk_map, bvals};
 
     rowpass::Part *parts = nullptr;
     RowStats *tots = nullptr;  // [0]: upper-bound pass, [1]: C block counts
@@ -878,23 +947,19 @@ int spgemm(const Mbsr &A, const Mbsr &B, hipStream_t s, Mbsr *Cp) {
                                B.nblk, B.blk_map, bpop);
         SG_HIP(hipGetLastError());
         SG_TRY(exclusive_scan_i32(bpop, boff, B.nblk, s));
-        if (reg_rows > 0)
-            hipLaunchKernelGGL(k_numeric_reg, dim3((reg_rows + 3) / 4), dim3(256), 0, s, reg_rows,
-                               list, g, boff, C.blk_ptr, C.blk_col, C.blk_map, C.val_ptr, C.val);
-        if (lds_rows > 0)
-            hipLaunchKernelGGL(k_numeric_lds, dim3((lds_rows + 3) / 4), dim3(256), 0, s, lds_rows,
-                               list + brow, g, boff, C.blk_ptr, C.blk_col, C.blk_map, C.val_ptr,
-                               C.val);
-        if (big_rows > 0) {
-            const int64_t elems = int64_t(C.max_row_blocks) * 16;
-            int nwaves = std::min(big_rows, 1024);
-            while (nwaves > 4 && elems * nwaves * 8 > (int64_t(1) << 31)) nwaves >>= 1;
-            nwaves = (nwaves + 3) / 4 * 4;  // every wave of the grid owns a slab
-            SG_TRY(tmp_alloc_t(&nslab, size_t(elems * nwaves), s));
-            hipLaunchKernelGGL(k_numeric_global, dim3((nwaves + 3) / 4), dim3(256), 0, s,
-                               big_rows, list + 2 * int64_t(brow), g, boff, C.blk_ptr, C.blk_col,
-                               C.blk_map, C.val_ptr,
-                               C.val, nslab, elems);
+        switch (precision) {
+            case AMGT_FLOAT:
+                st = launch_numeric<float>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, &nslab, s);
+                break;
+            case AMGT_HALF:
+                st = launch_numeric<__half>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, &nslab, s);
+                break;
+            default:
+                st = launch_numeric<double>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, &nslab, s);
+        }
+        if (st != AMGT_OK) {
+            cleanup();
+            return st;
         }
         SG_HIP(hipGetLastError());
     }

@@ -53,12 +53,30 @@ __device__ __forceinline__ int remap_wg(int wg, int nwg) {
     return xcd * per + min(xcd, rem) + idx;
 }
 
-template <int G, typename VT, typename AT>
+// Row epilogues of the SpMV kernels: y[row] = ep(row, (A x)[row]).
+struct EpAxpby {  // alpha (A x) + beta b; beta == 0 never reads b
+    double alpha, beta;
+    const double *b;
+    __device__ __forceinline__ double operator()(int row, double ax) const {
+        double v = alpha * ax;
+        if (beta != 0.0) v = fma(beta, b[row], v);
+        return v;
+    }
+};
+struct EpJacobi {  // one weighted Jacobi sweep: u + omega dinv (f - A u), with x = u
+    double omega;
+    const double *dinv, *f, *u;
+    __device__ __forceinline__ double operator()(int row, double ax) const {
+        return fma(omega * dinv[row], f[row] - ax, u[row]);
+    }
+};
+
+template <int G, typename VT, typename AT, typename EP>
 __global__ __launch_bounds__(256) void k_spmv(
     int32_t brow, int32_t nrows, const int32_t *__restrict__ blk_ptr,
     const int32_t *__restrict__ blk_col, const uint16_t *__restrict__ blk_map,
     const int32_t *__restrict__ val_ptr, const VT *__restrict__ val, const double *__restrict__ x,
-    double alpha, double beta, const double *b, double *y) {
+    EP ep, double *y) {
     constexpr int S = G / 4;
     constexpr int RPW = 64 / G;
     const int lane = threadIdx.x & 63;
@@ -111,9 +129,7 @@ __global__ __launch_bounds__(256) void k_spmv(
     if (s == 0) {
         const int row = 4 * br + r;
         if (row < nrows) {
-            double v = alpha * double(acc);
-            if (beta != 0.0) v = fma(beta, b[row], v);
-            y[row] = v;
+            y[row] = ep(row, double(acc));
         }
     }
 }
@@ -264,12 +280,12 @@ __device__ __forceinline__ AT row_walk_prefetch(const int32_t *s_col, const uint
 // bitmap popcounts (blocks and values of a tile are both contiguous).  Tiles
 // over the capacity fall back to the walk on global memory.  x must be
 // 16-byte aligned.
-template <int RPB, int BPT, typename VT, typename AT>
+template <int RPB, int BPT, typename VT, typename AT, typename EP>
 __global__ __launch_bounds__(256) void k_spmv_blk(
     int32_t brow, int32_t nrows, int32_t ncols, const int32_t *__restrict__ blk_ptr,
     const int32_t *__restrict__ blk_col, const uint16_t *__restrict__ blk_map,
     const int32_t *__restrict__ val_ptr, const VT *__restrict__ val, const double *__restrict__ x,
-    double alpha, double beta, const double *b, double *y) {
+    EP ep, double *y) {
     constexpr int CAPB = 256 * BPT, CAPV = 4 * CAPB, VPT = CAPV / 256;
     __shared__ AT s_buf[CAPV];
     __shared__ int32_t s_bp[RPB + 1], s_vp[RPB + 1];
@@ -300,9 +316,7 @@ __global__ __launch_bounds__(256) void k_spmv_blk(
             if (s == 0) {
                 const int row = 4 * (br0 + lr) + r;
                 if (row < nrows) {
-                    double v = alpha * double(acc);
-                    if (beta != 0.0) v = fma(beta, b[row], v);
-                    y[row] = v;
+                    y[row] = ep(row, double(acc));
                 }
             }
         }
@@ -401,9 +415,7 @@ __global__ __launch_bounds__(256) void k_spmv_blk(
         for (int j = s_bp[lr] - pb; j < s_bp[lr + 1] - pb; ++j) acc += s_buf[4 * j + r];
         const int row = 4 * (br0 + lr) + r;
         if (row < nrows) {
-            double v = alpha * double(acc);
-            if (beta != 0.0) v = fma(beta, b[row], v);
-            y[row] = v;
+            y[row] = ep(row, double(acc));
         }
     }
 }
@@ -427,14 +439,14 @@ __global__ __launch_bounds__(256) void k_spmv_blk(
 // Values are staged in LDS when the tile holds at most 8 per block on
 // average; denser tiles read them from global memory.  x must be 16-byte
 // aligned.
-template <int BPT, typename VT, typename AT>
+template <int BPT, typename VT, typename AT, typename EP>
 __global__ __launch_bounds__(256) void k_spmv_bal(
     int32_t ntiles, int32_t nblk, int32_t nrows, int32_t ncols,
     const int32_t *__restrict__ blk_ptr, const int32_t *__restrict__ blk_col,
     const uint16_t *__restrict__ blk_map, const VT *__restrict__ val,
     const int32_t *__restrict__ tile_lo, const int32_t *__restrict__ tile_head,
     const int32_t *__restrict__ tile_vb, double *__restrict__ carry, const double *__restrict__ x,
-    double alpha, double beta, const double *b, double *y) {
+    EP ep, double *y) {
     constexpr int CAPB = 256 * BPT, CAPV = 8 * CAPB, VPT = CAPV / 256;
     __shared__ AT s_buf[CAPV];
     __shared__ int32_t s_sc[4];
@@ -546,9 +558,7 @@ __global__ __launch_bounds__(256) void k_spmv_bal(
         } else {
             const int row = 4 * R + r;
             if (row < nrows) {
-                double v = alpha * double(acc);
-                if (beta != 0.0) v = fma(beta, b[row], v);
-                y[row] = v;
+                y[row] = ep(row, double(acc));
             }
         }
     }
@@ -557,11 +567,11 @@ __global__ __launch_bounds__(256) void k_spmv_bal(
 // Finishes the block rows cut by tile edges: thread (t, r) owns the row that
 // starts in tile t and ends in a later one, and adds its partials in tile
 // order (tile t's tail, then the heads of t+1, t+2, ...).
+template <typename EP>
 __global__ __launch_bounds__(256) void k_spmv_bal_fix(
     int32_t ntiles, int32_t capb, int32_t nblk, int32_t nrows,
     const int32_t *__restrict__ blk_ptr, const int32_t *__restrict__ tile_lo,
-    const int32_t *__restrict__ tile_head, const double *__restrict__ carry, double alpha,
-    double beta, const double *b, double *y) {
+    const int32_t *__restrict__ tile_head, const double *__restrict__ carry, EP ep, double *y) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int t = gid >> 2, r = gid & 3;
     if (t >= ntiles) return;
@@ -574,9 +584,7 @@ __global__ __launch_bounds__(256) void k_spmv_bal_fix(
     for (int t2 = t + 1; t2 < ntiles && tile_head[t2] == R; ++t2) acc += carry[8 * int64_t(t2) + r];
     const int row = 4 * R + r;
     if (row < nrows) {
-        double v = alpha * acc;
-        if (beta != 0.0) v = fma(beta, b[row], v);
-        y[row] = v;
+        y[row] = ep(row, acc);
     }
 }
 
@@ -620,12 +628,12 @@ __global__ __launch_bounds__(256) void k_tile_popc(int32_t nblk, int32_t capb,
     if (threadIdx.x == 0) tile_cnt[t] = s_part[0] + s_part[1] + s_part[2] + s_part[3];
 }
 
-template <int G, typename VT, typename AT, int CAPV, int CAPB, int MAXB, bool XSEG>
+template <int G, typename VT, typename AT, int CAPV, int CAPB, int MAXB, bool XSEG, typename EP>
 __global__ __launch_bounds__(256) void k_spmv_staged(
     int32_t brow, int32_t nrows, int32_t ncols, const int32_t *__restrict__ blk_ptr,
     const int32_t *__restrict__ blk_col, const uint16_t *__restrict__ blk_map,
     const int32_t *__restrict__ val_ptr, const VT *__restrict__ val, const double *__restrict__ x,
-    double alpha, double beta, const double *b, double *y) {
+    EP ep, double *y) {
     constexpr int RPW = 64 / G;
     constexpr int RPB = 256 / G;
     __shared__ VT s_val[CAPV];
@@ -695,9 +703,7 @@ __global__ __launch_bounds__(256) void k_spmv_staged(
     if (s == 0) {
         const int row = 4 * (br0 + lr) + r;
         if (row < nrows) {
-            double v = alpha * double(acc);
-            if (beta != 0.0) v = fma(beta, b[row], v);
-            y[row] = v;
+            y[row] = ep(row, double(acc));
         }
     }
 }
@@ -799,9 +805,8 @@ __global__ void k_cast_f16(int64_t n, const double *__restrict__ in, __half *__r
     if (i < n) out[i] = __float2half(float(in[i]));
 }
 
-template <int G, typename VT, typename AT>
-int launch_spmv_g(const Mbsr &A, const VT *val, double alpha, const double *x, double beta,
-                  const double *b, double *y, hipStream_t s) {
+template <int G, typename VT, typename AT, typename EP>
+int launch_spmv_g(const Mbsr &A, const VT *val, const double *x, EP ep, double *y, hipStream_t s) {
     constexpr int RPW = 64 / G;
     const int64_t waves = (int64_t(A.brow) + RPW - 1) / RPW;
     const int64_t grid = (waves + 3) / 4;
@@ -809,9 +814,8 @@ int launch_spmv_g(const Mbsr &A, const VT *val, double alpha, const double *x, d
         set_error("SpMV grid too large");
         return AMGT_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL((k_spmv<G, VT, AT>), dim3(unsigned(grid)), dim3(256), 0, s, A.brow,
-                       A.nrows, A.blk_ptr, A.blk_col, A.blk_map, A.val_ptr, val, x, alpha, beta,
-                       b, y);
+    hipLaunchKernelGGL((k_spmv<G, VT, AT, EP>), dim3(unsigned(grid)), dim3(256), 0, s, A.brow,
+                       A.nrows, A.blk_ptr, A.blk_col, A.blk_map, A.val_ptr, val, x, ep, y);
     AMGT_HIP(hipGetLastError());
     return AMGT_OK;
 }
@@ -819,90 +823,102 @@ int launch_spmv_g(const Mbsr &A, const VT *val, double alpha, const double *x, d
 constexpr int kStageVals = 2048;
 constexpr int kStageBlocks = 1024;
 
-template <int G, typename VT, typename AT, int MAXB = 8, bool XSEG = true>
-int launch_spmv_staged_g(const Mbsr &A, const VT *val, double alpha, const double *x, double beta,
-                         const double *b, double *y, hipStream_t s) {
+template <int G, typename VT, typename AT, typename EP>
+int launch_spmv_staged_g(const Mbsr &A, const VT *val, const double *x, EP ep, double *y,
+                         hipStream_t s) {
     constexpr int RPB = 256 / G;
     const int64_t grid = (int64_t(A.brow) + RPB - 1) / RPB;
-    hipLaunchKernelGGL((k_spmv_staged<G, VT, AT, kStageVals, kStageBlocks, MAXB, XSEG>),
-                       dim3(unsigned(grid)),
-                       dim3(256), 0, s, A.brow, A.nrows, A.ncols, A.blk_ptr, A.blk_col,
-                       A.blk_map, A.val_ptr, val, x, alpha, beta, b, y);
+    hipLaunchKernelGGL((k_spmv_staged<G, VT, AT, kStageVals, kStageBlocks, 4, true, EP>),
+                       dim3(unsigned(grid)), dim3(256), 0, s, A.brow, A.nrows, A.ncols,
+                       A.blk_ptr, A.blk_col, A.blk_map, A.val_ptr, val, x, ep, y);
     AMGT_HIP(hipGetLastError());
     return AMGT_OK;
 }
 
-template <int RPB, int BPT, typename VT, typename AT>
-int launch_spmv_blk_g(const Mbsr &A, const VT *val, double alpha, const double *x, double beta,
-                      const double *b, double *y, hipStream_t s) {
+template <int RPB, int BPT, typename VT, typename AT, typename EP>
+int launch_spmv_blk_g(const Mbsr &A, const VT *val, const double *x, EP ep, double *y,
+                      hipStream_t s) {
     const int64_t grid = (int64_t(A.brow) + RPB - 1) / RPB;
-    hipLaunchKernelGGL((k_spmv_blk<RPB, BPT, VT, AT>), dim3(unsigned(grid)), dim3(256), 0, s, A.brow,
-                       A.nrows, A.ncols, A.blk_ptr, A.blk_col, A.blk_map, A.val_ptr, val, x,
-                       alpha, beta, b, y);
+    hipLaunchKernelGGL((k_spmv_blk<RPB, BPT, VT, AT, EP>), dim3(unsigned(grid)), dim3(256), 0, s,
+                       A.brow, A.nrows, A.ncols, A.blk_ptr, A.blk_col, A.blk_map, A.val_ptr, val,
+                       x, ep, y);
     AMGT_HIP(hipGetLastError());
     return AMGT_OK;
 }
 
-template <int BPT, typename VT, typename AT>
-int launch_spmv_bal_g(const Mbsr &A, const VT *val, double alpha, const double *x, double beta,
-                      const double *b, double *y, hipStream_t s) {
+template <int BPT, typename VT, typename AT, typename EP>
+int launch_spmv_bal_g(const Mbsr &A, const VT *val, const double *x, EP ep, double *y,
+                      hipStream_t s) {
     const int nt = A.bal_ntiles;
-    hipLaunchKernelGGL((k_spmv_bal<BPT, VT, AT>), dim3(unsigned(nt)), dim3(256), 0, s, nt,
+    hipLaunchKernelGGL((k_spmv_bal<BPT, VT, AT, EP>), dim3(unsigned(nt)), dim3(256), 0, s, nt,
                        int32_t(A.nblk), A.nrows, A.ncols, A.blk_ptr, A.blk_col, A.blk_map, val,
-                       A.bal_lo, A.bal_head, A.bal_vb, A.bal_carry, x, alpha, beta, b, y);
+                       A.bal_lo, A.bal_head, A.bal_vb, A.bal_carry, x, ep, y);
     AMGT_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_spmv_bal_fix, dim3(unsigned((4 * int64_t(nt) + 255) / 256)), dim3(256),
-                       0, s, nt, A.bal_capb, int32_t(A.nblk), A.nrows, A.blk_ptr, A.bal_lo,
-                       A.bal_head, A.bal_carry, alpha, beta, b, y);
+    hipLaunchKernelGGL((k_spmv_bal_fix<EP>), dim3(unsigned((4 * int64_t(nt) + 255) / 256)),
+                       dim3(256), 0, s, nt, A.bal_capb, int32_t(A.nblk), A.nrows, A.blk_ptr,
+                       A.bal_lo, A.bal_head, A.bal_carry, ep, y);
     AMGT_HIP(hipGetLastError());
     return AMGT_OK;
 }
 
-template <typename VT, typename AT>
-int launch_spmv_t(const Mbsr &A, const VT *val, int lanes, double alpha, const double *x,
-                  double beta, const double *b, double *y, hipStream_t s) {
+// y[row] = ep(row, (A x)[row]) with the handle's kernel kind and lanes setting
+template <typename VT, typename AT, typename EP>
+int launch_spmv_t(const Mbsr &A, const VT *val, int lanes, const double *x, EP ep, double *y,
+                  hipStream_t s) {
     if (lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64) {
         set_error("lanes per block row must be 4, 8, 16, 32 or 64");
         return AMGT_ERR_ARG;
     }
-    // the staged and block kernels read x in 16-byte pieces; otherwise the direct kernel
+    // the staged, block and balanced kernels read x in 16-byte pieces;
+    // otherwise the direct kernel
     const bool x_aligned = (reinterpret_cast<uintptr_t>(x) & 15u) == 0;
     if (A.spmv_kernel == kSpmvBalanced && x_aligned && A.bal_ntiles > 0 &&
         A.bal_capb == 256 * balanced_bpt(lanes)) {
         switch (balanced_bpt(lanes)) {
-            case 1: return launch_spmv_bal_g<1, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            case 4: return launch_spmv_bal_g<4, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            default: return launch_spmv_bal_g<2, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            case 1: return launch_spmv_bal_g<1, VT, AT>(A, val, x, ep, y, s);
+            case 4: return launch_spmv_bal_g<4, VT, AT>(A, val, x, ep, y, s);
+            default: return launch_spmv_bal_g<2, VT, AT>(A, val, x, ep, y, s);
         }
     }
     if ((A.spmv_kernel == kSpmvBlock || A.spmv_kernel == kSpmvBalanced) && x_aligned) {
         // lanes selects the tile: 4 -> 64 block rows (2 blocks per thread),
         // 8 -> 128 (4 per thread), 16 -> 32, 32 -> 16, 64 -> 8 block rows
         switch (lanes) {
-            case 4: return launch_spmv_blk_g<64, 2, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            case 8: return launch_spmv_blk_g<128, 4, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            case 16: return launch_spmv_blk_g<32, 2, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            case 32: return launch_spmv_blk_g<16, 2, VT, AT>(A, val, alpha, x, beta, b, y, s);
-            default: return launch_spmv_blk_g<8, 2, VT, AT>(A, val, alpha, x, beta, b, y, s);
+            case 4: return launch_spmv_blk_g<64, 2, VT, AT>(A, val, x, ep, y, s);
+            case 8: return launch_spmv_blk_g<128, 4, VT, AT>(A, val, x, ep, y, s);
+            case 16: return launch_spmv_blk_g<32, 2, VT, AT>(A, val, x, ep, y, s);
+            case 32: return launch_spmv_blk_g<16, 2, VT, AT>(A, val, x, ep, y, s);
+            default: return launch_spmv_blk_g<8, 2, VT, AT>(A, val, x, ep, y, s);
         }
     }
     if (A.spmv_kernel == kSpmvStaged && x_aligned) {
         switch (lanes) {
-            case 4: return launch_spmv_staged_g<4, VT, AT, 4, true>(A, val, alpha, x, beta, b, y, s);
-            case 8: return launch_spmv_staged_g<8, VT, AT, 4, true>(A, val, alpha, x, beta, b, y, s);
-            case 16: return launch_spmv_staged_g<16, VT, AT, 4, true>(A, val, alpha, x, beta, b, y, s);
-            case 32: return launch_spmv_staged_g<32, VT, AT, 4, true>(A, val, alpha, x, beta, b, y, s);
-            default: return launch_spmv_staged_g<64, VT, AT, 4, true>(A, val, alpha, x, beta, b, y, s);
+            case 4: return launch_spmv_staged_g<4, VT, AT>(A, val, x, ep, y, s);
+            case 8: return launch_spmv_staged_g<8, VT, AT>(A, val, x, ep, y, s);
+            case 16: return launch_spmv_staged_g<16, VT, AT>(A, val, x, ep, y, s);
+            case 32: return launch_spmv_staged_g<32, VT, AT>(A, val, x, ep, y, s);
+            default: return launch_spmv_staged_g<64, VT, AT>(A, val, x, ep, y, s);
         }
     }
     switch (lanes) {
-        case 4: return launch_spmv_g<4, VT, AT>(A, val, alpha, x, beta, b, y, s);
-        case 8: return launch_spmv_g<8, VT, AT>(A, val, alpha, x, beta, b, y, s);
-        case 16: return launch_spmv_g<16, VT, AT>(A, val, alpha, x, beta, b, y, s);
-        case 32: return launch_spmv_g<32, VT, AT>(A, val, alpha, x, beta, b, y, s);
-        case 64: return launch_spmv_g<64, VT, AT>(A, val, alpha, x, beta, b, y, s);
+        case 4: return launch_spmv_g<4, VT, AT>(A, val, x, ep, y, s);
+        case 8: return launch_spmv_g<8, VT, AT>(A, val, x, ep, y, s);
+        case 16: return launch_spmv_g<16, VT, AT>(A, val, x, ep, y, s);
+        case 32: return launch_spmv_g<32, VT, AT>(A, val, x, ep, y, s);
+        default: return launch_spmv_g<64, VT, AT>(A, val, x, ep, y, s);
+    }
+}
+
+// the handle's values in `precision`, with the kernel's accumulation type
+template <typename EP>
+int launch_spmv_prec(const Mbsr &A, int precision, int lanes, const double *x, EP ep, double *y,
+                     hipStream_t s) {
+    switch (precision) {
+        case AMGT_DOUBLE: return launch_spmv_t<double, double>(A, A.val, lanes, x, ep, y, s);
+        case AMGT_FLOAT: return launch_spmv_t<float, float>(A, A.val32, lanes, x, ep, y, s);
+        case AMGT_HALF: return launch_spmv_t<__half, float>(A, A.val16, lanes, x, ep, y, s);
         default:
-            set_error("lanes per block row must be 4, 8, 16, 32 or 64");
+            set_error("unknown precision");
             return AMGT_ERR_ARG;
     }
 }
@@ -1023,17 +1039,13 @@ int spmv(const Mbsr &A, int precision, int lanes, double alpha, const double *x,
         AMGT_HIP(hipGetLastError());
         return AMGT_OK;
     }
-    switch (precision) {
-        case AMGT_DOUBLE:
-            return launch_spmv_t<double, double>(A, A.val, lanes, alpha, x, beta, b, y, s);
-        case AMGT_FLOAT:
-            return launch_spmv_t<float, float>(A, A.val32, lanes, alpha, x, beta, b, y, s);
-        case AMGT_HALF:
-            return launch_spmv_t<__half, float>(A, A.val16, lanes, alpha, x, beta, b, y, s);
-        default:
-            set_error("unknown precision");
-            return AMGT_ERR_ARG;
-    }
+    return launch_spmv_prec(A, precision, lanes, x, EpAxpby{alpha, beta, b}, y, s);
+}
+
+int relax_jacobi(const Mbsr &A, int precision, int lanes, double omega, const double *dinv,
+                 const double *f, const double *u, double *u_new, hipStream_t s) {
+    if (A.nrows == 0) return AMGT_OK;
+    return launch_spmv_prec(A, precision, lanes, u, EpJacobi{omega, dinv, f, u}, u_new, s);
 }
 
 int spmm(const Mbsr &A, int precision, int lanes, int k, double alpha, const double *x,

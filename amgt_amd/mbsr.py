@@ -158,6 +158,33 @@ class MBSRMatrix:
         L.check(st, "amgt_mbsr_spmv")
         return y
 
+    def diagonal(self, inverse_abs: bool = False, out: torch.Tensor | None = None,
+                 stream=None) -> torch.Tensor:
+        """a_ii, or 1/|a_ii| (the inverse l1 norms of relax type 18 on one process)."""
+        n = min(self.shape)
+        if out is None:
+            out = torch.empty(n, dtype=torch.float64, device=self.device)
+        _need_cuda(out, "out", torch.float64)
+        kind = L.AMGT_DIAG_INV_ABS if inverse_abs else L.AMGT_DIAG_VALUE
+        L.check(L.load().amgt_mbsr_diagonal(self.handle, kind, _ptr(out), _stream_ptr(stream)),
+                "amgt_mbsr_diagonal")
+        return out
+
+    def relax_jacobi(self, f: torch.Tensor, u: torch.Tensor, dinv: torch.Tensor,
+                     omega: float = 1.0, out: torch.Tensor | None = None,
+                     stream=None) -> torch.Tensor:
+        """One weighted Jacobi sweep u + omega * dinv * (f - A u) (relax type 18), one launch."""
+        n = self.shape[0]
+        for t, name in ((f, "f"), (u, "u"), (dinv, "dinv")):
+            _need_cuda(t, name, torch.float64)
+        if out is None:
+            out = torch.empty(n, dtype=torch.float64, device=self.device)
+        _need_cuda(out, "out", torch.float64)
+        L.check(L.load().amgt_mbsr_relax_jacobi(self.handle, float(omega), _ptr(dinv), _ptr(f),
+                                                 _ptr(u), _ptr(out), _stream_ptr(stream)),
+                "amgt_mbsr_relax_jacobi")
+        return out
+
     def spmm(self, X: torch.Tensor, Y: torch.Tensor | None = None, alpha: float = 1.0,
              beta: float = 0.0, B: torch.Tensor | None = None, trans: bool = False,
              stream=None) -> torch.Tensor:
@@ -184,6 +211,13 @@ class MBSRMatrix:
         h = ctypes.c_void_p()
         st = L.load().amgt_mbsr_spgemm(self.handle, other.handle, _stream_ptr(stream), ctypes.byref(h))
         L.check(st, "amgt_mbsr_spgemm")
+        return MBSRMatrix(h, self.device)
+
+    def transpose(self, stream=None) -> "MBSRMatrix":
+        """A new handle holding A^T (amgt_mbsr_transpose)."""
+        h = ctypes.c_void_p()
+        st = L.load().amgt_mbsr_transpose(self.handle, _stream_ptr(stream), ctypes.byref(h))
+        L.check(st, "amgt_mbsr_transpose")
         return MBSRMatrix(h, self.device)
 
     def __matmul__(self, other):

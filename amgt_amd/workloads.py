@@ -156,14 +156,11 @@ def trilinear_interpolation(nx: int, ny: int, nz: int = 1):
                 cols.append(c)
                 vals.append(v)
                 mask.append(m & (v != 0.0))
+    # slots in (kz, ky, kx) order already give ascending columns: the high
+    # neighbour of each direction is the low one plus 1
     C = np.stack(cols, axis=1)
     V = np.stack(vals, axis=1)
     M = np.stack(mask, axis=1)
-    # ascending columns within a row: order slots by column
-    order = np.argsort(np.where(M, C, np.iinfo(np.int64).max), axis=1, kind="stable")
-    C = np.take_along_axis(C, order, axis=1)
-    V = np.take_along_axis(V, order, axis=1)
-    M = np.take_along_axis(M, order, axis=1)
     counts = M.sum(axis=1)
     I = np.zeros(len(x) + 1, dtype=np.int64)
     np.cumsum(counts, out=I[1:])

@@ -95,7 +95,8 @@ int amgt_mbsr_update_values(amgt_mbsr_t A, const int32_t *d_rowptr, const int32_
 int amgt_mbsr_get_info(amgt_mbsr_t A, amgt_mbsr_info_t *info);
 int amgt_mbsr_get_arrays(amgt_mbsr_t A, amgt_mbsr_arrays_t *arrays);
 int amgt_mbsr_set_level(amgt_mbsr_t A, int level);
-/* explicit SpMV value precision; overrides the level policy for this handle */
+/* explicit value precision of SpMV with A and of SpGEMM with A as left
+ * operand; overrides the level policy for this handle (-1 restores it) */
 int amgt_mbsr_set_precision(amgt_mbsr_t A, int precision);
 /* lanes per block row for SpMV: 4, 8, 16, 32 or 64; 0 = choose from the
  * matrix statistics */
@@ -132,7 +133,31 @@ int amgt_mbsr_spmm(amgt_mbsr_t A, int trans, int k, double alpha, const double *
                    double beta, const double *d_b, int64_t ldb, double *d_y, int64_t ldy,
                    void *stream);
 
-/* ---- SpGEMM: C = A*B on mBSR (spgemm_amgT_fp64 csr_spgemm_device.c:1527-1846) ---- */
+/* New handle holding A^T (hypre_CSRMatrixTransposeDevice, csr_matop_device.c:2471, as
+ * used for R = P^T): blocks sorted by block column with a stable radix sort,
+ * bitmaps and values transposed; block columns ascending within a row. */
+int amgt_mbsr_transpose(amgt_mbsr_t A, void *stream, amgt_mbsr_t *AT);
+
+/* ---- smoother support (V-cycle) ---- */
+enum { AMGT_DIAG_VALUE = 0, AMGT_DIAG_INV_ABS = 1 };
+/* d[i] = a_ii (AMGT_DIAG_VALUE) or 1/|a_ii| (AMGT_DIAG_INV_ABS, 0 where a_ii = 0) for
+ * i < min(nrows, ncols).  1/|a_ii| is the inverse l1 norm of relax type 18 on one
+ * process (hypre_ParCSRComputeL1Norms option 4, ams.c:659-690). */
+int amgt_mbsr_diagonal(amgt_mbsr_t A, int kind, double *d_diag, void *stream);
+/* One weighted l1-Jacobi sweep, u_new = u + omega * dinv .* (f - A u), fused into a
+ * single SpMV launch (relax type 18: hypre_BoomerAMGRelax18WeightedL1Jacobi ->
+ * hypre_BoomerAMGRelax7Jacobi, par_relax.c:338-355, 1178-1250, which takes a copy, a
+ * matvec and an element-wise update).  A square; u_new must not alias u; values in
+ * the handle's precision as for SpMV. */
+int amgt_mbsr_relax_jacobi(amgt_mbsr_t A, double omega, const double *d_dinv, const double *d_f,
+                           const double *d_u, double *d_u_new, void *stream);
+
+/* ---- SpGEMM: C = A*B on mBSR (spgemm_amgT_fp64 csr_spgemm_device.c:1527-1846) ----
+ * Operand precision follows A (level policy in mixed mode, or A's explicit
+ * precision): FP64, or FP32 / FP16 copies of both operands (spgemm_amgT_fp32
+ * 2224-2578, spgemm_amgT_fp16 1863-2223); products of the rounded operands
+ * accumulate in FP64 and, below FP64, every C entry is rounded to FP32.  C's
+ * values are returned as FP64. */
 int amgt_mbsr_spgemm(amgt_mbsr_t A, amgt_mbsr_t B, void *stream, amgt_mbsr_t *C);
 /* mBSR -> CSR (BSR2CSR_step1-3, csr_spgemm_device.c:1501-1525), columns
  * ascending within a row; arrays come from the allocator hooks */

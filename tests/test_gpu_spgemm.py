@@ -160,3 +160,49 @@ def test_hypre_spgemm_boundary(dev):
     Cp = H.hypreDevice_CSRSpGemm(Ap, B)
     assert Cp.data is None
     assert np.array_equal(Cp.j.cpu().numpy(), Sg.indices)
+
+
+@pytest.mark.parametrize("prec,dt", [(L.AMGT_FLOAT, np.float32), (L.AMGT_HALF, np.float16)])
+@pytest.mark.parametrize("case", ["random", "stencil_27", "big_rows"])
+def test_spgemm_reduced_precision(dev, prec, dt, case):
+    """spgemm_amgT_fp32 / _fp16 (csr_spgemm_device.c:2224-2578, 1863-2223):
+    operands rounded to FP32 / FP16, products of the rounded operands
+    accumulated at least in FP32 and C entries rounded to FP32.  Oracle: the
+    FP64 product of the rounded operands, rounded to FP32; bar: 2^-22 of the
+    (|A| |B|) scale of the rounded operands (one FP32 rounding of the entry plus
+    the FP64 accumulation order)."""
+    if case == "random":
+        m, k, n = 61, 47, 53
+        AI, AJ, AA = rand_csr(m, k, 0.2, 11)
+        BI, BJ, BA = rand_csr(k, n, 0.25, 12)
+    elif case == "stencil_27":
+        AI, AJ, AA = O.stencil(27, 6, 5, 7)
+        m = k = n = 6 * 5 * 7
+        BI, BJ, BA = AI, AJ, AA
+    else:
+        m, k, n = 24, 3000, 40
+        AI, AJ, AA = dense_rows_csr(m, k, 400, 13)
+        BI, BJ, BA = rand_csr(k, n, 0.05, 14)
+    print(f"[spgemm-lowp] {case} prec={prec}", flush=True)
+    ra = AA.astype(dt).astype(np.float64)
+    rb = BA.astype(dt).astype(np.float64)
+    ref = O.mbsr_spgemm(O.csr2mbsr(m, k, AI, AJ, ra), O.csr2mbsr(k, n, BI, BJ, rb))
+    RI, RJ, RV = O.mbsr2csr(ref)
+    RV32 = RV.astype(np.float32).astype(np.float64)
+    GA, GB = make(dev, m, k, AI, AJ, AA), make(dev, k, n, BI, BJ, BA)
+    GA.set_precision(prec)
+    GC = GA.spgemm(GB)
+    CI, CJ, CV = GC.to_csr()
+    assert np.array_equal(CI.cpu().numpy(), RI) and np.array_equal(CJ.cpu().numpy(), RJ)
+    cv = CV.cpu().numpy()
+    # every entry is an FP32 number
+    assert np.array_equal(cv, cv.astype(np.float32).astype(np.float64))
+    scale = abs_product_scale(m, k, n, AI, AJ, np.abs(ra), BI, BJ, np.abs(rb))
+    scale.sort_indices()
+    assert np.all(np.abs(cv - RV32) <= 2.0 ** -22 * scale.data + 1e-300)
+    # and the FP64 product stays within FP32 / FP16 input rounding of the exact one
+    GA.set_precision(-1)
+    exact = GA.spgemm(GB).to_csr()[2].cpu().numpy()
+    tol = 2.0 ** -22 if dt == np.float32 else 2.0 ** -9
+    assert np.all(np.abs(cv - exact) <= tol * abs_product_scale(
+        m, k, n, AI, AJ, AA, BI, BJ, BA).sorted_indices().data + 1e-300)
