@@ -41,6 +41,7 @@ EXPORTS = (
     "amgt_mbsr_autotune", "amgt_mbsr_spmv", "amgt_mbsr_spmm", "amgt_mbsr_spgemm",
     "amgt_mbsr_to_csr", "amgt_free", "amgt_csr_spgemm", "amgt_csr_matvec",
     "amgt_mbsr_diagonal", "amgt_mbsr_relax_jacobi", "amgt_mbsr_transpose",
+    "amgt_read_ij", "amgt_read_mm", "amgt_host_free",
 )
 
 
@@ -123,6 +124,11 @@ def load():
                                    _vp, _vp, _vp]),
         "amgt_mbsr_diagonal": (_int, [_vp, _int, _vp, _vp]),
         "amgt_mbsr_transpose": (_int, [_vp, _vp, _pvp]),
+        "amgt_read_ij": (_int, [ctypes.c_char_p, _int, ctypes.POINTER(_i32),
+                                ctypes.POINTER(_i32), _pi64, _pvp, _pvp, _pvp]),
+        "amgt_read_mm": (_int, [ctypes.c_char_p, ctypes.POINTER(_i32), ctypes.POINTER(_i32),
+                                _pi64, _pvp, _pvp, _pvp]),
+        "amgt_host_free": (None, [_vp]),
         "amgt_mbsr_relax_jacobi": (_int, [_vp, _dbl, _vp, _vp, _vp, _vp, _vp]),
     }
     for name, (res, args) in sig.items():

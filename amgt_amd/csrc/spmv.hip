@@ -789,6 +789,66 @@ __global__ __launch_bounds__(256) void k_spmm(
     }
 }
 
+// MFMA SpMM for k >= 8 right-hand sides: one wave per block row.  For each
+// block j of the row, v_mfma_f64_4x4x4f64 multiplies the 4x4 block (the same
+// A operand in all four block slots) by four 4x4 slices of the X segment
+// (block slot b takes right-hand sides 16g + 4b .. 16g + 4b + 3), so one
+// instruction advances 16 right-hand sides and the accumulators stay in
+// registers across the row's blocks.  NG groups of 16 share each block's A
+// operand.  Operand layout of the instruction (measured on gfx950, block slot
+// b): A_b[i][k] in lane 16k+4b+i, B_b[k][n] in lane 16k+4b+n, D_b[i][n] in lane
+// 16i+4b+n.  X, B and Y hold each right-hand side contiguously (leading
+// dimensions ldx, ldb, ldy).  Values of type VT are widened to FP64.
+template <int NG, typename VT>
+__global__ __launch_bounds__(256) void k_spmm_mfma(
+    int32_t brow, int32_t nrows, int32_t ncols, const int32_t *__restrict__ blk_ptr,
+    const int32_t *__restrict__ blk_col, const uint16_t *__restrict__ blk_map,
+    const int32_t *__restrict__ val_ptr, const VT *__restrict__ val, int k0, int kn,
+    const double *__restrict__ x, int64_t ldx, double alpha, double beta, const double *b,
+    int64_t ldb, double *y, int64_t ldy) {
+    const int lane = threadIdx.x & 63;
+    const int br = remap_wg(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    if (br >= brow) return;  // whole wave
+    const int kk = lane >> 4, bs = (lane >> 2) & 3, nn = lane & 3;
+    const int eA = 4 * nn + kk;  // A operand: entry (i = lane & 3, k = lane >> 4)
+    const uint32_t belowA = (1u << eA) - 1u;
+    const int p0 = blk_ptr[br], p1 = blk_ptr[br + 1];
+    int voff = val_ptr[br];
+    int64_t xoff[NG];
+    bool xon[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        const int rhs = 16 * g + 4 * bs + nn;  // relative to k0
+        xon[g] = rhs < kn;
+        xoff[g] = int64_t(k0 + (xon[g] ? rhs : 0)) * ldx;
+    }
+    double acc[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) acc[g] = 0.0;
+    for (int j = p0; j < p1; ++j) {
+        const uint32_t m = blk_map[j];
+        const int xrow = 4 * blk_col[j] + kk;
+        const double a = ((m >> eA) & 1u) ? double(val[voff + __popc(m & belowA)]) : 0.0;
+        double bx[NG];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) bx[g] = (xon[g] && xrow < ncols) ? x[xoff[g] + xrow] : 0.0;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) acc[g] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, bx[g], acc[g], 0, 0, 0);
+        voff += __popc(m);
+    }
+    const int row = 4 * br + kk;  // D: row i = lane >> 4, right-hand side 4b + n
+    if (row < nrows) {
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            if (!xon[g]) continue;
+            const int v = k0 + 16 * g + 4 * bs + nn;
+            double out = alpha * acc[g];
+            if (beta != 0.0) out = fma(beta, b[int64_t(v) * ldb + row], out);
+            y[int64_t(v) * ldy + row] = out;
+        }
+    }
+}
+
 // y = beta*b (beta == 0 -> y = 0): used when the matrix has no blocks
 __global__ void k_scale(int32_t n, double beta, const double *b, double *y) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -940,10 +1000,32 @@ int launch_spmm_g(const Mbsr &A, const VT *val, int k, double alpha, const doubl
     return AMGT_OK;
 }
 
+// k >= 8: MFMA kernel, 32 right-hand sides per launch (16 for the last < 17)
+template <typename VT>
+int launch_spmm_mfma(const Mbsr &A, const VT *val, int k, double alpha, const double *x,
+                     int64_t ldx, double beta, const double *b, int64_t ldb, double *y,
+                     int64_t ldy, hipStream_t s) {
+    const int64_t grid = (int64_t(A.brow) + 3) / 4;
+    for (int k0 = 0; k0 < k; k0 += 32) {
+        const int kn = k - k0 < 32 ? k - k0 : 32;
+        if (kn > 16)
+            hipLaunchKernelGGL((k_spmm_mfma<2, VT>), dim3(unsigned(grid)), dim3(256), 0, s, A.brow,
+                               A.nrows, A.ncols, A.blk_ptr, A.blk_col, A.blk_map, A.val_ptr, val,
+                               k0, kn, x, ldx, alpha, beta, b, ldb, y, ldy);
+        else
+            hipLaunchKernelGGL((k_spmm_mfma<1, VT>), dim3(unsigned(grid)), dim3(256), 0, s, A.brow,
+                               A.nrows, A.ncols, A.blk_ptr, A.blk_col, A.blk_map, A.val_ptr, val,
+                               k0, kn, x, ldx, alpha, beta, b, ldb, y, ldy);
+        AMGT_HIP(hipGetLastError());
+    }
+    return AMGT_OK;
+}
+
 template <typename VT, typename AT>
 int launch_spmm_t(const Mbsr &A, const VT *val, int lanes, int k, double alpha, const double *x,
                   int64_t ldx, double beta, const double *b, int64_t ldb, double *y, int64_t ldy,
                   hipStream_t s) {
+    if (k >= 8) return launch_spmm_mfma<VT>(A, val, k, alpha, x, ldx, beta, b, ldb, y, ldy, s);
     switch (lanes) {
         case 4: return launch_spmm_g<4, 8, VT, AT>(A, val, k, alpha, x, ldx, beta, b, ldb, y, ldy, s);
         case 8: return launch_spmm_g<8, 8, VT, AT>(A, val, k, alpha, x, ldx, beta, b, ldb, y, ldy, s);

@@ -23,6 +23,16 @@ def _ptr(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 
 
+def _need_rows(t, name):
+    """(k, n) float64 device tensor whose rows are contiguous (any row stride)."""
+    if t is None:
+        return
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float64:
+        raise TypeError(f"{name} must be a float64 device tensor")
+    if t.dim() != 2 or (t.shape[1] > 1 and t.stride(1) != 1) or t.stride(0) < t.shape[1]:
+        raise ValueError(f"{name} must be (k, n) with contiguous rows")
+
+
 def _need_cuda(t, name, dtype=None):
     if t is None:
         return
@@ -188,18 +198,19 @@ class MBSRMatrix:
     def spmm(self, X: torch.Tensor, Y: torch.Tensor | None = None, alpha: float = 1.0,
              beta: float = 0.0, B: torch.Tensor | None = None, trans: bool = False,
              stream=None) -> torch.Tensor:
-        """Y = alpha*op(A)*X + beta*B for k columns.  X is (k, n) row-major, i.e. each
-        right-hand side contiguous (HYPRE multivector storage method 0)."""
+        """Y = alpha*op(A)*X + beta*B for k columns.  X is (k, n) with each
+        right-hand side contiguous and any row stride (HYPRE multivector
+        storage method 0, vecstride = the row stride)."""
         nrows, ncols = self.shape
         ny = ncols if trans else nrows
-        _need_cuda(X, "X", torch.float64)
+        _need_rows(X, "X")
         k = X.shape[0]
         if Y is None:
             Y = torch.empty((k, ny), dtype=torch.float64, device=self.device)
-        _need_cuda(Y, "Y", torch.float64)
+        _need_rows(Y, "Y")
         if B is None and beta != 0.0:
             B = Y
-        _need_cuda(B, "B", torch.float64)
+        _need_rows(B, "B")
         st = L.load().amgt_mbsr_spmm(self.handle, int(bool(trans)), k, float(alpha), _ptr(X),
                                      X.stride(0), float(beta), _ptr(B),
                                      B.stride(0) if B is not None else 0, _ptr(Y), Y.stride(0),

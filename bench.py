@@ -167,6 +167,43 @@ def amg_timings(dev, n1=128):
     return out
 
 
+def spmm_timings(dev):
+    """Multi-RHS SpMM Y = A X on 3D 27-pt Poisson 64^3 (diag 26, off-diagonal
+    -1) for k in {1, 8, 32} right-hand sides (X ~ N(0,1), Park-Miller seed 3),
+    HYPRE multivector storage (each vector contiguous).  Algorithmic bytes per
+    call: 8 nnz + 6 nblk + 4 (brow+1) + 8 k (ncols + nrows)."""
+    import torch
+
+    from amgt_amd import workloads as W
+    from amgt_amd.mbsr import MBSRMatrix
+
+    I, J, V = W.stencil(27, 64, 64, 64)
+    n = len(I) - 1
+    t = lambda a: torch.from_numpy(a).to(dev)  # noqa: E731
+    M = MBSRMatrix.from_csr(t(I), t(J), t(V), n, n)
+    M.autotune()
+    inf = M.info()
+    mat_bytes = 8 * inf["nnz"] + 6 * inf["nblk"] + 4 * (inf["brow"] + 1)
+    out = {"rows": n, "nnz": int(inf["nnz"]), "blocks": int(inf["nblk"])}
+    for k in (1, 8, 32):
+        X = t(W.rand_normal(3, k * n).reshape(k, n))
+        Y = torch.empty_like(X)
+        for _ in range(5):
+            M.spmm(X, Y)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(20):
+            M.spmm(X, Y)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 20
+        byt = mat_bytes + 8 * k * 2 * n
+        out[f"k{k}"] = {"ms": round(ms, 4), "ms_per_rhs": round(ms / k, 5),
+                        "alg_bytes": int(byt), "gbs": round(byt / (ms * 1e-3) / 1e9, 1),
+                        "gflops": round(2 * inf["nnz"] * k / (ms * 1e-3) / 1e9, 1)}
+    return out
+
+
 def load_traffic(kernel_key):
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
@@ -299,6 +336,7 @@ def main():
     conv_warm_ms = (time.perf_counter() - tc) * 1e3
     del M2
     amg = None if args.no_amg else amg_timings(dev)
+    spmm = spmm_timings(dev)
 
     B = spmv_bytes(info)
     value = B / (ms_per_step * 1e-3) / 1e9
@@ -336,6 +374,7 @@ def main():
         out["spgemm"] = spgemm
     if amg is not None:
         out["amg_7pt128"] = amg
+    out["spmm_27pt64"] = spmm
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(I, J, A, x_h, B)
     print(json.dumps(out), flush=True)

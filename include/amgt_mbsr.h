@@ -166,6 +166,18 @@ int amgt_mbsr_to_csr(amgt_mbsr_t A, void *stream, int64_t *nnz, int32_t **d_rowp
 /* frees a pointer obtained from this library's allocator */
 int amgt_free(void *ptr);
 
+/* ---- host matrix readers (host CSR out, ascending columns; free with amgt_host_free) ----
+ * IJ files <prefix>.00000 .. <prefix>.<nfiles-1> (hypre_IJMatrixRead, IJ_mv/IJMatrix.c:112-242;
+ * hypre_CSRMatrixRead / Print* in seq_mv.h:524-530): header "ilower iupper jlower jupper",
+ * lines "i j value" (0-based); entries in the file's own rows are set (the last one wins),
+ * others are added to the owner's value.  All files are merged into one matrix. */
+int amgt_read_ij(const char *prefix, int nfiles, int32_t *nrows, int32_t *ncols, int64_t *nnz,
+                 int32_t **I, int32_t **J, double **V);
+/* Matrix Market coordinate real/integer, general or symmetric (HYPRE_IJMatrixReadMM) */
+int amgt_read_mm(const char *path, int32_t *nrows, int32_t *ncols, int64_t *nnz, int32_t **I,
+                 int32_t **J, double **V);
+void amgt_host_free(void *p);
+
 /* ---- HYPRE-shaped entry points (plain pointers; the mBSR cache lives in a
  *      caller-owned slot, the `bsr_mat` field of hypre_CSRMatrix,
  *      seq_mv.h:156-163) ---- */

@@ -215,17 +215,45 @@ def test_spmv_low_precision(dev, prec, tol):
     assert err.max() > 0  # really ran in reduced precision
 
 
-@pytest.mark.parametrize("k", [1, 3, 8, 11])
-def test_spmm(dev, k):
-    (m, n), I, J, A = CASES["rand_rect"]()
+@pytest.mark.parametrize("name", ["rand_rect", "27pt_10", "skewed", "rand_tall_empty"])
+@pytest.mark.parametrize("k", [1, 3, 8, 9, 16, 17, 32, 40])
+def test_spmm(dev, name, k):
+    """k right-hand sides (each contiguous, leading dimension padded past n):
+    the scalar kernel below 8, the MFMA kernel (16 right-hand sides per
+    instruction, 32 per launch) from 8; every column against the oracle."""
+    print(f"[spmm] {name} k={k}", flush=True)
+    (m, n), I, J, A = CASES[name]()
     ref = O.csr2mbsr(m, n, I, J, A)
     M = make(dev, m, n, I, J, A)
     rng = np.random.default_rng(k)
-    X, B = rng.standard_normal((k, n)), rng.standard_normal((k, m))
-    Y = M.spmm(T(X, dev), alpha=0.5, beta=2.0, B=T(B, dev)).cpu().numpy()
+    X, B = rng.standard_normal((k, n + 3)), rng.standard_normal((k, m + 5))
+    Y = T(np.full((k, m + 7), np.nan), dev)
+    M.spmm(T(X, dev)[:, :n], Y[:, :m], alpha=0.5, beta=2.0, B=T(B, dev)[:, :m])
+    Yh = Y.cpu().numpy()
+    assert np.all(np.isnan(Yh[:, m:]))  # padding untouched
     for v in range(k):
-        exp = O.mbsr_spmv(ref, 0.5, X[v], 2.0, B[v])
-        assert np.allclose(Y[v], exp, rtol=1e-12, atol=1e-12)
+        exp = O.mbsr_spmv(ref, 0.5, X[v, :n], 2.0, B[v, :m])
+        scale = 0.5 * row_scale(m, n, I, J, A, X[v, :n], 0.0, None) + 2.0 * np.abs(B[v, :m])
+        assert np.all(np.abs(Yh[v, :m] - exp) <= 1e-12 * scale), v
+
+
+@pytest.mark.parametrize("prec,tol", [(L.AMGT_FLOAT, 2e-6), (L.AMGT_HALF, 2e-3)])
+def test_spmm_low_precision_and_transposed(dev, prec, tol):
+    (m, n), I, J, A = CASES["27pt_10"]()
+    ref = O.csr2mbsr(m, n, I, J, A)
+    M = make(dev, m, n, I, J, A)
+    k = 24
+    X = np.random.default_rng(9).standard_normal((k, n))
+    Yt = M.spmm(T(X, dev), trans=True).cpu().numpy()
+    for v in range(k):
+        exp = O.mbsr_spmv_t(ref, 1.0, X[v], 0.0, None)
+        assert np.allclose(Yt[v], exp, rtol=1e-12, atol=1e-12 * np.abs(exp).max())
+    M.set_precision(prec)
+    Y = M.spmm(T(X, dev)).cpu().numpy()
+    for v in range(k):
+        exp = O.mbsr_spmv(ref, 1.0, X[v], 0.0, None)
+        scale = row_scale(m, n, I, J, A, X[v], 0.0, None)
+        assert np.all(np.abs(Y[v] - exp) <= tol * scale)
 
 
 def test_full_size_c2_spmv(dev):
