@@ -41,7 +41,7 @@ EXPORTS = (
     "amgt_mbsr_autotune", "amgt_mbsr_spmv", "amgt_mbsr_spmm", "amgt_mbsr_spgemm",
     "amgt_mbsr_to_csr", "amgt_free", "amgt_csr_spgemm", "amgt_csr_matvec",
     "amgt_mbsr_diagonal", "amgt_mbsr_relax_jacobi", "amgt_mbsr_transpose",
-    "amgt_read_ij", "amgt_read_mm", "amgt_host_free",
+    "amgt_read_ij", "amgt_read_mm", "amgt_host_free", "amgt_mbsr_spmm_interleaved",
 )
 
 
@@ -124,6 +124,7 @@ def load():
                                    _vp, _vp, _vp]),
         "amgt_mbsr_diagonal": (_int, [_vp, _int, _vp, _vp]),
         "amgt_mbsr_transpose": (_int, [_vp, _vp, _pvp]),
+        "amgt_mbsr_spmm_interleaved": (_int, [_vp, _int, _int, _dbl, _vp, _dbl, _vp, _vp, _vp]),
         "amgt_read_ij": (_int, [ctypes.c_char_p, _int, ctypes.POINTER(_i32),
                                 ctypes.POINTER(_i32), _pi64, _pvp, _pvp, _pvp]),
         "amgt_read_mm": (_int, [ctypes.c_char_p, ctypes.POINTER(_i32), ctypes.POINTER(_i32),

@@ -510,7 +510,30 @@ int amgt_mbsr_spmm(amgt_mbsr_t A, int trans, int k, double alpha, const double *
     if (trans) AMGT_TRY(get_transpose(m, s, &op));
     int prec = effective_precision(m);
     AMGT_TRY(mbsr_ensure_lowp(op, prec, s));
+    AMGT_TRY(spmv_prepare(op, lanes_of(*op), s));  // k == 1 runs the SpMV kernels
     return spmm(*op, prec, lanes_of(*op), k, alpha, d_x, ldx, beta, d_b, ldb, d_y, ldy, s);
+}
+
+int amgt_mbsr_spmm_interleaved(amgt_mbsr_t A, int trans, int k, double alpha, const double *d_x,
+                               double beta, const double *d_b, double *d_y, void *stream) {
+    AMGT_CHECK_ARG(A != nullptr, "handle is NULL");
+    AMGT_CHECK_ARG(k >= 1, "k must be >= 1");
+    Mbsr &m = A->m;
+    hipStream_t s = as_stream(stream);
+    const int ny = trans ? m.ncols : m.nrows;
+    AMGT_CHECK_ARG(d_y != nullptr || ny == 0, "y is NULL");
+    AMGT_CHECK_ARG(beta == 0.0 || d_b != nullptr, "b is NULL with beta != 0");
+    AMGT_CHECK_ARG(d_x != d_y || ny == 0, "x and y are the same pointer");
+    AMGT_CHECK_ARG(!m.pattern_only, "pattern-only matrix has no values");
+    Mbsr *op = &m;
+    if (trans) AMGT_TRY(get_transpose(m, s, &op));
+    const int prec = effective_precision(m);
+    AMGT_TRY(mbsr_ensure_lowp(op, prec, s));
+    if (op->nblk == 0 || alpha == 0.0) {
+        // y = beta b over all k*ny entries
+        return scale_vector(int32_t(int64_t(ny) * k), beta, d_b, d_y, s);
+    }
+    return spmm_interleaved(*op, prec, k, alpha, d_x, beta, d_b, d_y, s);
 }
 
 int amgt_mbsr_transpose(amgt_mbsr_t A, void *stream, amgt_mbsr_t *AT) {

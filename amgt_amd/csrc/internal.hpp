@@ -121,6 +121,9 @@ int spmm(const Mbsr &A, int precision, int lanes, int k, double alpha, const dou
          int64_t ldx, double beta, const double *b, int64_t ldb, double *y, int64_t ldy,
          hipStream_t s);
 int spmv_default_lanes(const Mbsr &A);
+// k right-hand sides interleaved (entry (row i, vector v) at x[i * k + v])
+int spmm_interleaved(const Mbsr &A, int precision, int k, double alpha, const double *x,
+                     double beta, const double *b, double *y, hipStream_t s);
 // u_new = u + omega * dinv .* (f - A u) in one SpMV launch (square A)
 int relax_jacobi(const Mbsr &A, int precision, int lanes, double omega, const double *dinv,
                  const double *f, const double *u, double *u_new, hipStream_t s);

@@ -794,57 +794,130 @@ __global__ __launch_bounds__(256) void k_spmm(
 // A operand in all four block slots) by four 4x4 slices of the X segment
 // (block slot b takes right-hand sides 16g + 4b .. 16g + 4b + 3), so one
 // instruction advances 16 right-hand sides and the accumulators stay in
-// registers across the row's blocks.  NG groups of 16 share each block's A
-// operand.  Operand layout of the instruction (measured on gfx950, block slot
-// b): A_b[i][k] in lane 16k+4b+i, B_b[k][n] in lane 16k+4b+n, D_b[i][n] in lane
-// 16i+4b+n.  X, B and Y hold each right-hand side contiguously (leading
-// dimensions ldx, ldb, ldy).  Values of type VT are widened to FP64.
+// registers across the row's blocks; NG groups of 16 share each block's A
+// operand.  The row's block bitmaps, columns and value offsets are loaded
+// across the lanes first, and the operands of U = 4 blocks are gathered before
+// their MFMAs.  Operand layout (measured on gfx950, block slot b): A_b[i][k] in
+// lane 16k+4b+i, B_b[k][n] in lane 16k+4b+n, D_b[i][n] in lane 16i+4b+n.
+// Multivectors are interleaved (HYPRE multivector storage
+// method 1): entry (row i, vector v) at x[i * kx + v], so the B operand of one
+// instruction is four rows x 16 consecutive right-hand sides (four 128-byte
+// lines) instead of 16 separate 32-byte pieces of column-major vectors.
+// Writes y[row * ky + v] = alpha (A x)[row][v] + beta b[row * kb + v].
 template <int NG, typename VT>
-__global__ __launch_bounds__(256) void k_spmm_mfma(
+__global__ __launch_bounds__(256) void k_spmm_mfma_il(
     int32_t brow, int32_t nrows, int32_t ncols, const int32_t *__restrict__ blk_ptr,
     const int32_t *__restrict__ blk_col, const uint16_t *__restrict__ blk_map,
     const int32_t *__restrict__ val_ptr, const VT *__restrict__ val, int k0, int kn,
-    const double *__restrict__ x, int64_t ldx, double alpha, double beta, const double *b,
-    int64_t ldb, double *y, int64_t ldy) {
+    const double *__restrict__ x, int64_t kx, double alpha, double beta, const double *b,
+    int64_t kb, double *y, int64_t ky) {
+    constexpr int U = 4;
     const int lane = threadIdx.x & 63;
     const int br = remap_wg(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (br >= brow) return;  // whole wave
     const int kk = lane >> 4, bs = (lane >> 2) & 3, nn = lane & 3;
-    const int eA = 4 * nn + kk;  // A operand: entry (i = lane & 3, k = lane >> 4)
+    const int eA = 4 * nn + kk;
     const uint32_t belowA = (1u << eA) - 1u;
     const int p0 = blk_ptr[br], p1 = blk_ptr[br + 1];
-    int voff = val_ptr[br];
-    int64_t xoff[NG];
+    int vrel[NG];
     bool xon[NG];
 #pragma unroll
     for (int g = 0; g < NG; ++g) {
-        const int rhs = 16 * g + 4 * bs + nn;  // relative to k0
+        const int rhs = 16 * g + 4 * bs + nn;
         xon[g] = rhs < kn;
-        xoff[g] = int64_t(k0 + (xon[g] ? rhs : 0)) * ldx;
+        vrel[g] = k0 + (xon[g] ? rhs : 0);
     }
     double acc[NG];
 #pragma unroll
     for (int g = 0; g < NG; ++g) acc[g] = 0.0;
-    for (int j = p0; j < p1; ++j) {
-        const uint32_t m = blk_map[j];
-        const int xrow = 4 * blk_col[j] + kk;
-        const double a = ((m >> eA) & 1u) ? double(val[voff + __popc(m & belowA)]) : 0.0;
-        double bx[NG];
+    int voff = val_ptr[br];
+    for (int c0 = p0; c0 < p1; c0 += 64) {
+        const int nb = min(64, p1 - c0);
+        uint32_t mj = 0u;
+        int cj = 0;
+        if (lane < nb) {
+            mj = blk_map[c0 + lane];
+            cj = blk_col[c0 + lane];
+        }
+        int inc = __popc(mj);
 #pragma unroll
-        for (int g = 0; g < NG; ++g) bx[g] = (xon[g] && xrow < ncols) ? x[xoff[g] + xrow] : 0.0;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int t = __shfl_up(inc, d);
+            if (lane >= d) inc += t;
+        }
+        const int oj = voff + inc - __popc(mj);
+        voff += __shfl(inc, 63);
+        for (int j0 = 0; j0 < nb; j0 += U) {
+            double a[U], bx[U][NG];
 #pragma unroll
-        for (int g = 0; g < NG; ++g) acc[g] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, bx[g], acc[g], 0, 0, 0);
-        voff += __popc(m);
+            for (int u = 0; u < U; ++u) {
+                const int j = j0 + u < nb ? j0 + u : nb - 1;
+                const uint32_t m = uint32_t(__shfl(int(mj), j));
+                const int xrow = 4 * __shfl(cj, j) + kk;
+                const int o = __shfl(oj, j);
+                const bool on = j0 + u < nb;
+                a[u] = (on && ((m >> eA) & 1u)) ? double(val[o + __popc(m & belowA)]) : 0.0;
+#pragma unroll
+                for (int g = 0; g < NG; ++g)
+                    bx[u][g] = (on && xon[g] && xrow < ncols) ? x[int64_t(xrow) * kx + vrel[g]] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int g = 0; g < NG; ++g)
+                    acc[g] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[u], bx[u][g], acc[g], 0, 0, 0);
+        }
     }
-    const int row = 4 * br + kk;  // D: row i = lane >> 4, right-hand side 4b + n
+    const int row = 4 * br + kk;
     if (row < nrows) {
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
             if (!xon[g]) continue;
-            const int v = k0 + 16 * g + 4 * bs + nn;
             double out = alpha * acc[g];
-            if (beta != 0.0) out = fma(beta, b[int64_t(v) * ldb + row], out);
-            y[int64_t(v) * ldy + row] = out;
+            if (beta != 0.0) out = fma(beta, b[int64_t(row) * kb + vrel[g]], out);
+            y[int64_t(row) * ky + vrel[g]] = out;
+        }
+    }
+}
+
+// Tiled transposes between the two multivector layouts (32 x 32 tiles in LDS,
+// both sides coalesced).  cm -> il: xi[i * k + v] = x[v * ldx + i].
+__global__ __launch_bounds__(256) void k_cm_to_il(int32_t n, int32_t k,
+                                                  const double *__restrict__ x, int64_t ldx,
+                                                  double *__restrict__ xi) {
+    __shared__ double tile[32][33];
+    const int i0 = blockIdx.x * 32, v0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int r = ty; r < 32; r += 8) {
+        const int v = v0 + r, i = i0 + tx;
+        tile[r][tx] = (v < k && i < n) ? x[int64_t(v) * ldx + i] : 0.0;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + r, v = v0 + tx;
+        if (i < n && v < k) xi[int64_t(i) * k + v] = tile[tx][r];
+    }
+}
+
+// il -> cm with the SpMM epilogue: y[v * ldy + i] = alpha yi[i * k + v] + beta b[v * ldb + i]
+__global__ __launch_bounds__(256) void k_il_to_cm(int32_t n, int32_t k,
+                                                  const double *__restrict__ yi, double alpha,
+                                                  double beta, const double *b, int64_t ldb,
+                                                  double *y, int64_t ldy) {
+    __shared__ double tile[32][33];
+    const int i0 = blockIdx.x * 32, v0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + r, v = v0 + tx;
+        tile[r][tx] = (i < n && v < k) ? yi[int64_t(i) * k + v] : 0.0;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int v = v0 + r, i = i0 + tx;
+        if (v < k && i < n) {
+            double out = alpha * tile[tx][r];
+            if (beta != 0.0) out = fma(beta, b[int64_t(v) * ldb + i], out);
+            y[int64_t(v) * ldy + i] = out;
         }
     }
 }
@@ -1000,32 +1073,62 @@ int launch_spmm_g(const Mbsr &A, const VT *val, int k, double alpha, const doubl
     return AMGT_OK;
 }
 
-// k >= 8: MFMA kernel, 32 right-hand sides per launch (16 for the last < 17)
+// interleaved multivectors: 32 right-hand sides per launch
 template <typename VT>
-int launch_spmm_mfma(const Mbsr &A, const VT *val, int k, double alpha, const double *x,
-                     int64_t ldx, double beta, const double *b, int64_t ldb, double *y,
-                     int64_t ldy, hipStream_t s) {
+int launch_spmm_il(const Mbsr &A, const VT *val, int k, double alpha, const double *x,
+                   int64_t kx, double beta, const double *b, int64_t kb, double *y, int64_t ky,
+                   hipStream_t s) {
     const int64_t grid = (int64_t(A.brow) + 3) / 4;
     for (int k0 = 0; k0 < k; k0 += 32) {
         const int kn = k - k0 < 32 ? k - k0 : 32;
         if (kn > 16)
-            hipLaunchKernelGGL((k_spmm_mfma<2, VT>), dim3(unsigned(grid)), dim3(256), 0, s, A.brow,
-                               A.nrows, A.ncols, A.blk_ptr, A.blk_col, A.blk_map, A.val_ptr, val,
-                               k0, kn, x, ldx, alpha, beta, b, ldb, y, ldy);
+            hipLaunchKernelGGL((k_spmm_mfma_il<2, VT>), dim3(unsigned(grid)), dim3(256), 0, s,
+                               A.brow, A.nrows, A.ncols, A.blk_ptr, A.blk_col, A.blk_map,
+                               A.val_ptr, val, k0, kn, x, kx, alpha, beta, b, kb, y, ky);
         else
-            hipLaunchKernelGGL((k_spmm_mfma<1, VT>), dim3(unsigned(grid)), dim3(256), 0, s, A.brow,
-                               A.nrows, A.ncols, A.blk_ptr, A.blk_col, A.blk_map, A.val_ptr, val,
-                               k0, kn, x, ldx, alpha, beta, b, ldb, y, ldy);
+            hipLaunchKernelGGL((k_spmm_mfma_il<1, VT>), dim3(unsigned(grid)), dim3(256), 0, s,
+                               A.brow, A.nrows, A.ncols, A.blk_ptr, A.blk_col, A.blk_map,
+                               A.val_ptr, val, k0, kn, x, kx, alpha, beta, b, kb, y, ky);
         AMGT_HIP(hipGetLastError());
     }
     return AMGT_OK;
+}
+
+// column-major multivectors with k >= 8: transpose X to the interleaved
+// layout, run the interleaved MFMA kernel, transpose back with the epilogue
+template <typename VT>
+int launch_spmm_cm_via_il(const Mbsr &A, const VT *val, int k, double alpha, const double *x,
+                          int64_t ldx, double beta, const double *b, int64_t ldb, double *y,
+                          int64_t ldy, hipStream_t s) {
+    double *xi = nullptr, *yi = nullptr;
+    AMGT_TRY(tmp_alloc_t(&xi, size_t(A.ncols) * size_t(k), s));
+    int st = tmp_alloc_t(&yi, size_t(A.nrows) * size_t(k), s);
+    if (st == AMGT_OK) {
+        const dim3 gx(unsigned((A.ncols + 31) / 32), unsigned((k + 31) / 32));
+        hipLaunchKernelGGL(k_cm_to_il, gx, dim3(256), 0, s, A.ncols, k, x, ldx, xi);
+        st = launch_spmm_il<VT>(A, val, k, 1.0, xi, k, 0.0, nullptr, k, yi, k, s);
+    }
+    if (st == AMGT_OK) {
+        const dim3 gy(unsigned((A.nrows + 31) / 32), unsigned((k + 31) / 32));
+        hipLaunchKernelGGL(k_il_to_cm, gy, dim3(256), 0, s, A.nrows, k, yi, alpha, beta, b, ldb,
+                           y, ldy);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            set_error(std::string("SpMM transpose: ") + hipGetErrorString(e));
+            st = AMGT_ERR_HIP;
+        }
+    }
+    tmp_free(xi, s);
+    tmp_free(yi, s);
+    return st;
 }
 
 template <typename VT, typename AT>
 int launch_spmm_t(const Mbsr &A, const VT *val, int lanes, int k, double alpha, const double *x,
                   int64_t ldx, double beta, const double *b, int64_t ldb, double *y, int64_t ldy,
                   hipStream_t s) {
-    if (k >= 8) return launch_spmm_mfma<VT>(A, val, k, alpha, x, ldx, beta, b, ldb, y, ldy, s);
+    if (k >= 8)
+        return launch_spmm_cm_via_il<VT>(A, val, k, alpha, x, ldx, beta, b, ldb, y, ldy, s);
     switch (lanes) {
         case 4: return launch_spmm_g<4, 8, VT, AT>(A, val, k, alpha, x, ldx, beta, b, ldb, y, ldy, s);
         case 8: return launch_spmm_g<8, 8, VT, AT>(A, val, k, alpha, x, ldx, beta, b, ldb, y, ldy, s);
@@ -1134,6 +1237,7 @@ int spmm(const Mbsr &A, int precision, int lanes, int k, double alpha, const dou
          int64_t ldx, double beta, const double *b, int64_t ldb, double *y, int64_t ldy,
          hipStream_t s) {
     if (A.nrows == 0) return AMGT_OK;
+    if (k == 1) return spmv(A, precision, lanes, alpha, x, beta, b, y, s);
     if (A.nblk == 0 || alpha == 0.0) {
         for (int v = 0; v < k; ++v) {
             hipLaunchKernelGGL(k_scale, dim3((A.nrows + 255) / 256), dim3(256), 0, s, A.nrows,
@@ -1152,6 +1256,22 @@ int spmm(const Mbsr &A, int precision, int lanes, int k, double alpha, const dou
         case AMGT_HALF:
             return launch_spmm_t<__half, float>(A, A.val16, lanes, k, alpha, x, ldx, beta, b, ldb,
                                                 y, ldy, s);
+        default:
+            set_error("unknown precision");
+            return AMGT_ERR_ARG;
+    }
+}
+
+int spmm_interleaved(const Mbsr &A, int precision, int k, double alpha, const double *x,
+                     double beta, const double *b, double *y, hipStream_t s) {
+    if (A.nrows == 0 || k == 0) return AMGT_OK;
+    switch (precision) {
+        case AMGT_DOUBLE:
+            return launch_spmm_il<double>(A, A.val, k, alpha, x, k, beta, b, k, y, k, s);
+        case AMGT_FLOAT:
+            return launch_spmm_il<float>(A, A.val32, k, alpha, x, k, beta, b, k, y, k, s);
+        case AMGT_HALF:
+            return launch_spmm_il<__half>(A, A.val16, k, alpha, x, k, beta, b, k, y, k, s);
         default:
             set_error("unknown precision");
             return AMGT_ERR_ARG;

@@ -218,6 +218,27 @@ class MBSRMatrix:
         L.check(st, "amgt_mbsr_spmm")
         return Y
 
+    def spmm_interleaved(self, X: torch.Tensor, Y: torch.Tensor | None = None,
+                         alpha: float = 1.0, beta: float = 0.0, B: torch.Tensor | None = None,
+                         trans: bool = False, stream=None) -> torch.Tensor:
+        """Y = alpha*op(A)*X + beta*B with X of shape (n, k), contiguous: the k
+        values of a row together (HYPRE multivector storage method 1)."""
+        nrows, ncols = self.shape
+        ny = ncols if trans else nrows
+        _need_cuda(X, "X", torch.float64)
+        k = X.shape[1]
+        if Y is None:
+            Y = torch.empty((ny, k), dtype=torch.float64, device=self.device)
+        _need_cuda(Y, "Y", torch.float64)
+        if B is None and beta != 0.0:
+            B = Y
+        _need_cuda(B, "B", torch.float64)
+        st = L.load().amgt_mbsr_spmm_interleaved(self.handle, int(bool(trans)), k, float(alpha),
+                                                 _ptr(X), float(beta), _ptr(B), _ptr(Y),
+                                                 _stream_ptr(stream))
+        L.check(st, "amgt_mbsr_spmm_interleaved")
+        return Y
+
     def spgemm(self, other: "MBSRMatrix", stream=None) -> "MBSRMatrix":
         h = ctypes.c_void_p()
         st = L.load().amgt_mbsr_spgemm(self.handle, other.handle, _stream_ptr(stream), ctypes.byref(h))

@@ -201,6 +201,19 @@ def spmm_timings(dev):
         out[f"k{k}"] = {"ms": round(ms, 4), "ms_per_rhs": round(ms / k, 5),
                         "alg_bytes": int(byt), "gbs": round(byt / (ms * 1e-3) / 1e9, 1),
                         "gflops": round(2 * inf["nnz"] * k / (ms * 1e-3) / 1e9, 1)}
+        if k > 1:
+            # interleaved storage (HYPRE multivector method 1): no layout transposes
+            Xi, Yi = X.t().contiguous(), torch.empty((n, k), dtype=torch.float64, device=dev)
+            for _ in range(5):
+                M.spmm_interleaved(Xi, Yi)
+            e0.record()
+            for _ in range(20):
+                M.spmm_interleaved(Xi, Yi)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / 20
+            out[f"k{k}_interleaved"] = {"ms": round(ms, 4), "ms_per_rhs": round(ms / k, 5),
+                                        "gbs": round(byt / (ms * 1e-3) / 1e9, 1)}
     return out
 
 

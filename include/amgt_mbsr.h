@@ -152,6 +152,13 @@ int amgt_mbsr_diagonal(amgt_mbsr_t A, int kind, double *d_diag, void *stream);
 int amgt_mbsr_relax_jacobi(amgt_mbsr_t A, double omega, const double *d_dinv, const double *d_f,
                            const double *d_u, double *d_u_new, void *stream);
 
+/* k right-hand sides interleaved (HYPRE multivector storage method 1): entry (row i,
+ * vector v) at x[i*k + v], likewise b and y.  MFMA kernel: one v_mfma_f64_4x4x4f64
+ * advances 16 right-hand sides of a 4x4 block; amgt_mbsr_spmm with k >= 8 transposes
+ * to this layout and back. */
+int amgt_mbsr_spmm_interleaved(amgt_mbsr_t A, int trans, int k, double alpha, const double *d_x,
+                               double beta, const double *d_b, double *d_y, void *stream);
+
 /* ---- SpGEMM: C = A*B on mBSR (spgemm_amgT_fp64 csr_spgemm_device.c:1527-1846) ----
  * Operand precision follows A (level policy in mixed mode, or A's explicit
  * precision): FP64, or FP32 / FP16 copies of both operands (spgemm_amgT_fp32

@@ -7,6 +7,7 @@ FP32 / FP16 value storage within 2e-6 / 2e-3 of the same scale (FP32
 accumulation; FP16 values round at 2^-11 relative).
 """
 import numpy as np
+import scipy.sparse as sp
 import pytest
 import torch
 
@@ -235,6 +236,33 @@ def test_spmm(dev, name, k):
         exp = O.mbsr_spmv(ref, 0.5, X[v, :n], 2.0, B[v, :m])
         scale = 0.5 * row_scale(m, n, I, J, A, X[v, :n], 0.0, None) + 2.0 * np.abs(B[v, :m])
         assert np.all(np.abs(Yh[v, :m] - exp) <= 1e-12 * scale), v
+
+
+@pytest.mark.parametrize("name", ["rand_rect", "27pt_10", "skewed"])
+@pytest.mark.parametrize("k", [1, 5, 16, 17, 40])
+@pytest.mark.parametrize("trans", [False, True])
+def test_spmm_interleaved(dev, name, k, trans):
+    """Interleaved multivectors (entry (i, v) at x[i*k + v]) through the MFMA kernel;
+    b aliasing y."""
+    print(f"[spmm-il] {name} k={k} trans={trans}", flush=True)
+    (m, n), I, J, A = CASES[name]()
+    ref = O.csr2mbsr(m, n, I, J, A)
+    M = make(dev, m, n, I, J, A)
+    nx, ny = (m, n) if trans else (n, m)
+    rng = np.random.default_rng(k + 100)
+    X, B = rng.standard_normal((nx, k)), rng.standard_normal((ny, k))
+    Yb = T(B, dev)
+    M.spmm_interleaved(T(X, dev), Yb, alpha=-1.5, beta=0.5, B=Yb, trans=trans)
+    Y = Yb.cpu().numpy()
+    S = sp.csr_matrix((np.abs(A), J, I), shape=(m, n))
+    for v in range(k):
+        if trans:
+            exp = O.mbsr_spmv_t(ref, -1.5, X[:, v], 0.5, B[:, v])
+            scale = 1.5 * (S.T @ np.abs(X[:, v])) + 0.5 * np.abs(B[:, v])
+        else:
+            exp = O.mbsr_spmv(ref, -1.5, X[:, v], 0.5, B[:, v])
+            scale = 1.5 * (S @ np.abs(X[:, v])) + 0.5 * np.abs(B[:, v])
+        assert np.all(np.abs(Y[:, v] - exp) <= 1e-12 * scale + 1e-300), v
 
 
 @pytest.mark.parametrize("prec,tol", [(L.AMGT_FLOAT, 2e-6), (L.AMGT_HALF, 2e-3)])
