@@ -14,6 +14,7 @@
 #include "internal.hpp"
 
 #include <string>
+#include <type_traits>
 
 namespace amgt {
 
@@ -335,7 +336,32 @@ __global__ __launch_bounds__(256) void k_spmv_blk(
             c4[u] = 4 * blk_col[pb + q];
         }
     }
-    {
+    bool vec2 = false;
+    if constexpr (std::is_same<VT, double>::value && std::is_same<AT, double>::value)
+        vec2 = (reinterpret_cast<uintptr_t>(val) & 15u) == 0;
+    if (vec2) {
+        // 16-byte loads from the first even value index on, the odd head and
+        // tail element singly
+        const int head = vb & 1;
+        const int npair = (nv - head) >> 1;
+        const double *dval = reinterpret_cast<const double *>(val);
+        double2 tv[VPT / 2];
+#pragma unroll
+        for (int u = 0; u < VPT / 2; ++u) {
+            const int p = u * 256 + tid;
+            if (p < npair) tv[u] = *reinterpret_cast<const double2 *>(dval + vb + head + 2 * p);
+        }
+        if (tid == 0 && head && nv > 0) s_buf[0] = AT(dval[vb]);
+        if (tid == 1 && head + 2 * npair < nv) s_buf[nv - 1] = AT(dval[vb + nv - 1]);
+#pragma unroll
+        for (int u = 0; u < VPT / 2; ++u) {
+            const int p = u * 256 + tid;
+            if (p < npair) {
+                s_buf[head + 2 * p] = AT(tv[u].x);
+                s_buf[head + 2 * p + 1] = AT(tv[u].y);
+            }
+        }
+    } else {
         VT tv[VPT];
 #pragma unroll
         for (int u = 0; u < VPT; ++u) {
@@ -414,9 +440,7 @@ __global__ __launch_bounds__(256) void k_spmv_blk(
         AT acc = AT(0);
         for (int j = s_bp[lr] - pb; j < s_bp[lr + 1] - pb; ++j) acc += s_buf[4 * j + r];
         const int row = 4 * (br0 + lr) + r;
-        if (row < nrows) {
-            y[row] = ep(row, double(acc));
-        }
+        if (row < nrows) y[row] = ep(row, double(acc));
     }
 }
 
