@@ -97,6 +97,25 @@ def trilinear_interpolation_device(grid, device):
     return I.to(torch.int32), C[M].to(torch.int32), V[M], ncx * ncy * ncz
 
 
+def interpolation_blockdiag(grids, device):
+    """Block-diagonal trilinear P for a batch of independent grids (problem i's
+    rows and columns follow problem i-1's), built on the device.  Returns
+    (I, J, A, ncoarse, coarse grids)."""
+    Is, Js, As = [torch.zeros(1, dtype=torch.int64, device=device)], [], []
+    nnz = col = 0
+    cgrids = []
+    for g in grids:
+        PI, PJ, PA, nc = trilinear_interpolation_device(g, device)
+        Is.append(PI[1:].to(torch.int64) + nnz)
+        Js.append(PJ + col)
+        As.append(PA)
+        nnz += int(PA.numel())
+        col += nc
+        cgrids.append(tuple((x + 1) // 2 for x in g))
+    return (torch.cat(Is).to(torch.int32), torch.cat(Js).to(torch.int32), torch.cat(As), col,
+            cgrids)
+
+
 def fp64_policy(level: int) -> int:
     return L.AMGT_DOUBLE
 
@@ -116,7 +135,7 @@ def mixed_spgemm_policy(level: int) -> int:
 @dataclass
 class Level:
     n: int
-    grid: tuple
+    grid: list  # the grid of each problem on this level
     A: MBSRMatrix
     dinv: torch.Tensor
     P: MBSRMatrix | None = None
@@ -136,7 +155,11 @@ class SetupTimes:
 
 
 class GeometricAMG:
-    """Galerkin hierarchy on a structured nx x ny x nz grid and its V(1,1) cycle."""
+    """Galerkin hierarchy on a structured nx x ny x nz grid, or on a batch of
+    independent grids held as one block-diagonal system (a list of grids), and
+    its V(1,1) cycle.  In a batch every grid coarsens by 2 per level, so each
+    level is one block-diagonal operator: one launch per step for all the
+    problems."""
 
     def __init__(self, I, J, V, grid, device, omega: float = 6.0 / 7.0, coarse_size: int = 512,
                  spmv_policy=fp64_policy, spgemm_policy=fp64_policy, max_levels: int = 25):
@@ -150,16 +173,20 @@ class GeometricAMG:
         self.coarse_inv: MBSRMatrix | None = None
         self.times = SetupTimes()
         self._graph = None
-        self._setup(I, J, V, tuple(grid))
+        grids = [tuple(g) for g in grid] if isinstance(grid, (list, tuple)) and \
+            len(grid) > 0 and isinstance(grid[0], (list, tuple)) else [tuple(grid)]
+        self.batch = len(grids) > 1
+        self._setup(I, J, V, grids)
 
     # ---- setup -------------------------------------------------------------------------
     def _t(self, a):
         return torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
 
-    def _setup(self, I, J, V, grid):
+    def _setup(self, I, J, V, grids):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        n = int(np.prod(grid))
+        grid = grids
+        n = int(sum(np.prod(g) for g in grids))
         A = MBSRMatrix.from_csr(I, J, V, n, n)
         lvl = 0
         while True:
@@ -168,9 +195,14 @@ class GeometricAMG:
             dinv = A.diagonal(inverse_abs=True)
             L_ = Level(n=n, grid=grid, A=A, dinv=dinv)
             self.levels.append(L_)
-            if n <= self.coarse_size or lvl + 1 >= self.max_levels or min(grid) <= 1:
+            if n <= self.coarse_size or lvl + 1 >= self.max_levels or \
+                    all(max(g) <= 1 for g in grid):
                 break
-            PI, PJ, PA, nc = trilinear_interpolation_device(grid, self.device)
+            if len(grid) == 1:
+                PI, PJ, PA, nc = trilinear_interpolation_device(grid[0], self.device)
+                cgrid = [tuple((x + 1) // 2 for x in grid[0])]
+            else:
+                PI, PJ, PA, nc, cgrid = interpolation_blockdiag(grid, self.device)
             P = MBSRMatrix.from_csr(PI, PJ, PA, n, nc)
             R = P.transpose()
             del PI, PJ, PA
@@ -191,7 +223,7 @@ class GeometricAMG:
             del AP
             L_.P, L_.R = P, R
             A, n = Ac, nc
-            grid = tuple((g + 1) // 2 for g in grid)
+            grid = cgrid
             lvl += 1
         # coarsest: dense inverse as an mBSR matrix with full blocks, applied in FP64
         torch.cuda.synchronize()
@@ -266,16 +298,33 @@ class GeometricAMG:
         return g
 
     # ---- Krylov ------------------------------------------------------------------------
-    def pcg(self, b: torch.Tensor, tol: float = 1e-8, max_iter: int = 200):
-        """Preconditioned CG (two-norm relative residual, as ij -tol with PCG)."""
+    def pcg(self, b: torch.Tensor, tol: float = 1e-8, max_iter: int = 200,
+            segments: torch.Tensor | None = None):
+        """Preconditioned CG (two-norm relative residual, as ij -tol with PCG).
+        With `segments` (problem index of every row of a batch), the test is
+        every problem's own relative residual."""
         A = self.levels[0].A
         x = torch.zeros_like(b)
         r = b.clone()
         z = torch.empty_like(b)
         Ap = torch.empty_like(b)
-        bnorm = torch.linalg.vector_norm(b).item()
-        if bnorm == 0.0:
-            return x, 0, [0.0]
+        if segments is not None:
+            nseg = int(segments.max().item()) + 1
+            bseg = torch.zeros(nseg, dtype=torch.float64, device=b.device).index_add_(
+                0, segments, b * b).sqrt_()
+            bseg = torch.where(bseg > 0, bseg, torch.ones_like(bseg))
+
+            def relres():
+                rs = torch.zeros(nseg, dtype=torch.float64, device=b.device).index_add_(
+                    0, segments, r * r).sqrt_()
+                return (rs / bseg).max().item()
+        else:
+            bn = torch.linalg.vector_norm(b).item()
+
+            def relres():
+                return torch.linalg.vector_norm(r).item() / bn
+            if bn == 0.0:
+                return x, 0, [0.0]
         self.precondition(r, z)
         p = z.clone()
         rz = torch.dot(r, z)
@@ -287,7 +336,7 @@ class GeometricAMG:
             x.add_(alpha * p)
             r.sub_(alpha * Ap)
             it += 1
-            rel = torch.linalg.vector_norm(r).item() / bnorm
+            rel = relres()
             hist.append(rel)
             if rel < tol:
                 break
@@ -330,3 +379,72 @@ def solve_poisson(nx: int, ny: int, nz: int, b: np.ndarray, device, mixed: bool 
     solve_s = time.perf_counter() - t0
     return {"amg": amg, "x": x, "iters": iters, "hist": hist, "setup": amg.times,
             "solve_s": solve_s}
+
+
+def batch_poisson2d(sizes, seed: int, device):
+    """Block-diagonal batch of 2D 5-pt Poisson problems (side n_i each, diag 4,
+    off-diagonal -1) with right-hand sides b_i = A_i x*_i, x* ~ N(0,1) from one
+    Park-Miller stream (seed).  Returns (I, J, V, b, x*, grids, segments) on the
+    host; row blocks follow the order of `sizes`."""
+    Is, Js, Vs, segs = [np.zeros(1, np.int64)], [], [], []
+    nnz = row = 0
+    for k, n in enumerate(sizes):
+        I, J, V = W.stencil(7, n, n, 1)
+        Is.append(I[1:].astype(np.int64) + nnz)
+        Js.append(J.astype(np.int64) + row)
+        Vs.append(V)
+        segs.append(np.full(n * n, k, np.int64))
+        nnz += len(J)
+        row += n * n
+    I = np.concatenate(Is).astype(np.int32)
+    J = np.concatenate(Js).astype(np.int32)
+    V = np.concatenate(Vs)
+    xs = W.rand_normal(seed, row)
+    import scipy.sparse as sp
+    b = sp.csr_matrix((V, J, I), shape=(row, row)) @ xs
+    return I, J, V, b, xs, [(n, n, 1) for n in sizes], np.concatenate(segs)
+
+
+def solve_batch(sizes, seed, device, tol: float = 1e-8, capture: bool = True,
+                coarse_size: int = 512, omega: float = 6.0 / 7.0):
+    """All problems of a batch in one block-diagonal setup and PCG/V(1,1) solve."""
+    I, J, V, b, xs, grids, seg = batch_poisson2d(sizes, seed, device)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
+    dI, dJ, dV, db, dseg = t(I), t(J), t(V), t(b), t(seg)
+    amg = GeometricAMG(dI, dJ, dV, grids, device, omega=omega, coarse_size=coarse_size)
+    if capture:
+        amg.capture()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, iters, hist = amg.pcg(db, tol=tol, segments=dseg)
+    torch.cuda.synchronize()
+    return {"amg": amg, "x": x, "xs": xs, "iters": iters, "hist": hist, "setup": amg.times,
+            "solve_s": time.perf_counter() - t0, "segments": seg}
+
+
+def batch_sizes(count: int = 48, seed: int = 4, lo: int = 32, hi: int = 512):
+    """Sides n_i ~ U{lo..hi} from the Park-Miller stream (seed)."""
+    u = W.pm_stream(seed, count).astype(np.float64) / 2147483647.0
+    return [int(min(hi, lo + int(v * (hi - lo + 1)))) for v in u]
+
+
+def solve_one_by_one(sizes, seed, device, tol: float = 1e-8, coarse_size: int = 512,
+                     omega: float = 6.0 / 7.0):
+    """The same problems solved one after the other (one hierarchy each)."""
+    I, J, V, b, xs, grids, seg = batch_poisson2d(sizes, seed, device)
+    out = []
+    row = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for n in sizes:
+        m = n * n
+        Ii, Ji, Vi = W.stencil(7, n, n, 1)
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
+        amg = GeometricAMG(t(Ii), t(Ji), t(Vi), (n, n, 1), device, omega=omega,
+                           coarse_size=coarse_size)
+        amg.capture()
+        x, iters, hist = amg.pcg(t(b[row:row + m]), tol=tol)
+        out.append((iters, hist[-1]))
+        row += m
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, out
