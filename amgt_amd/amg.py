@@ -50,43 +50,55 @@ from . import workloads as W
 from .mbsr import MBSRMatrix
 
 
-def _interp_1d_torch(n: int, device):
-    """Coarsening by 2 in one direction (coarse points at even fine indices,
-    linear weights, zero Dirichlet values outside): per fine index the low and
-    high coarse neighbours, their weights, and whether the high one exists."""
-    f = torch.arange(n, device=device)
-    nc = (n + 1) // 2
-    even = (f % 2) == 0
-    lo = f // 2
-    hi = lo + 1
-    w_lo = torch.where(even, 1.0, 0.5).to(torch.float64)
-    has_hi = (~even) & (hi < nc)
-    w_hi = torch.where(has_hi, 0.5, 0.0).to(torch.float64)
-    return nc, lo, hi, w_lo, w_hi, has_hi
+def _excl_cumsum(v: torch.Tensor) -> torch.Tensor:
+    out = torch.zeros_like(v)
+    out[1:] = torch.cumsum(v[:-1], 0)
+    return out
 
 
-def trilinear_interpolation_device(grid, device):
-    """CSR (I, J, A) of the trilinear prolongation from the even-index grid to
-    the full nx x ny x nz grid, built on the device (same arrays as
-    workloads.trilinear_interpolation: up to 8 entries per row, ascending
-    columns, no stored zeros).  Returns (I, J, A, ncoarse)."""
-    nx, ny, nz = grid
-    (ncx, lx, hx, wlx, whx, okx), (ncy, ly, hy, wly, why, oky), (ncz, lz, hz, wlz, whz, okz) = (
-        _interp_1d_torch(n, device) for n in (nx, ny, nz))
-    n = nx * ny * nz
-    idx = torch.arange(n, device=device)
-    x, y, z = idx % nx, (idx // nx) % ny, idx // (nx * ny)
+def interpolation_batched(grids, device):
+    """CSR (I, J, A) of the block-diagonal trilinear prolongation of a batch of
+    independent grids (problem k's rows and coarse columns follow problem
+    k-1's), built on the device in one pass over all rows.
+
+    Per grid and direction the coarse points are the even fine indices
+    (nc = (n + 1) // 2); a fine index takes weight 1 from its even point, or
+    1/2 from each even neighbour, a neighbour beyond the last coarse point
+    being a zero Dirichlet value.  Up to 8 entries per row, ascending columns,
+    no stored zeros.  Returns (I, J, A, ncoarse, coarse grids)."""
+    g = torch.tensor([tuple(x) for x in grids], dtype=torch.int64, device=device)
+    nx, ny, nz = g[:, 0], g[:, 1], g[:, 2]
+    ncx, ncy, ncz = (nx + 1) // 2, (ny + 1) // 2, (nz + 1) // 2
+    rows = nx * ny * nz
+    cgrids = [tuple((x + 1) // 2 for x in gr) for gr in grids]
+    n = int(sum(int(np.prod(gr)) for gr in grids))
+    nc = int(sum(int(np.prod(cg)) for cg in cgrids))
+    pid = torch.repeat_interleave(torch.arange(len(grids), device=device), rows, output_size=n)
+    loc = torch.arange(n, device=device) - _excl_cumsum(rows)[pid]
+    px, py, pcx, pcy, pcz = nx[pid], ny[pid], ncx[pid], ncy[pid], ncz[pid]
+    x, y, z = loc % px, (loc // px) % py, loc // (px * py)
+
+    def axis(f, ncf):
+        even = (f % 2) == 0
+        lo = f // 2
+        hi = lo + 1
+        w_lo = torch.where(even, 1.0, 0.5).to(torch.float64)
+        ok = (~even) & (hi < ncf)
+        w_hi = torch.where(ok, 0.5, 0.0).to(torch.float64)
+        return ((lo, w_lo, None), (hi, w_hi, ok))
+
+    cbase = _excl_cumsum(ncx * ncy * ncz)[pid]
     cols, vals, mask = [], [], []
     # slots in (kz, ky, kx) order give ascending columns (high = low + 1)
-    for cz, wz, mz in ((lz, wlz, None), (hz, whz, okz)):
-        for cy, wy, my in ((ly, wly, None), (hy, why, oky)):
-            for cx, wx, mx in ((lx, wlx, None), (hx, whx, okx)):
-                v = wx[x] * wy[y] * wz[z]
+    for cz, wz, mz in axis(z, pcz):
+        for cy, wy, my in axis(y, pcy):
+            for cx, wx, mx in axis(x, pcx):
+                v = wx * wy * wz
                 m = v != 0.0
-                for mm, q in ((mx, x), (my, y), (mz, z)):
+                for mm in (mx, my, mz):
                     if mm is not None:
-                        m = m & mm[q]
-                cols.append((cz[z] * ncy + cy[y]) * ncx + cx[x])
+                        m = m & mm
+                cols.append(cbase + (cz * pcy + cy) * pcx + cx)
                 vals.append(v)
                 mask.append(m)
     C = torch.stack(cols, 1)
@@ -94,26 +106,21 @@ def trilinear_interpolation_device(grid, device):
     M = torch.stack(mask, 1)
     I = torch.zeros(n + 1, dtype=torch.int64, device=device)
     I[1:] = torch.cumsum(M.sum(1), 0)
-    return I.to(torch.int32), C[M].to(torch.int32), V[M], ncx * ncy * ncz
+    return I.to(torch.int32), C[M].to(torch.int32), V[M], nc, cgrids
+
+
+def trilinear_interpolation_device(grid, device):
+    """CSR (I, J, A) of the trilinear prolongation from the even-index grid to
+    the full nx x ny x nz grid, built on the device (same arrays as
+    workloads.trilinear_interpolation).  Returns (I, J, A, ncoarse)."""
+    I, J, A, nc, _ = interpolation_batched([tuple(grid)], device)
+    return I, J, A, nc
 
 
 def interpolation_blockdiag(grids, device):
-    """Block-diagonal trilinear P for a batch of independent grids (problem i's
-    rows and columns follow problem i-1's), built on the device.  Returns
-    (I, J, A, ncoarse, coarse grids)."""
-    Is, Js, As = [torch.zeros(1, dtype=torch.int64, device=device)], [], []
-    nnz = col = 0
-    cgrids = []
-    for g in grids:
-        PI, PJ, PA, nc = trilinear_interpolation_device(g, device)
-        Is.append(PI[1:].to(torch.int64) + nnz)
-        Js.append(PJ + col)
-        As.append(PA)
-        nnz += int(PA.numel())
-        col += nc
-        cgrids.append(tuple((x + 1) // 2 for x in g))
-    return (torch.cat(Is).to(torch.int32), torch.cat(Js).to(torch.int32), torch.cat(As), col,
-            cgrids)
+    """Block-diagonal trilinear P for a batch of independent grids; see
+    interpolation_batched.  Returns (I, J, A, ncoarse, coarse grids)."""
+    return interpolation_batched(grids, device)
 
 
 def fp64_policy(level: int) -> int:
@@ -152,6 +159,44 @@ class SetupTimes:
     device_s: float = 0.0        # P, R = P^T, conversions, Galerkin SpGEMMs, diagonals
     total_s: float = 0.0
     spgemm_s: list = field(default_factory=list)
+
+
+class SegmentNorms:
+    """Two-norms of the row segments of vectors: problem k of a batch owns the
+    rows with segments == k, which are contiguous and ascending in k.
+
+    Rows are gathered once per call into chunks that never straddle two
+    problems (each problem padded to a whole number of chunks with a zero), the
+    chunks are summed densely and the few chunk sums are added per problem, so
+    no reduction funnels millions of atomics into a handful of addresses."""
+
+    def __init__(self, segments: torch.Tensor, chunk: int = 2048):
+        seg = segments.to(torch.int64)
+        dev = seg.device
+        n = seg.numel()
+        if n and bool((seg[1:] < seg[:-1]).any().item()):
+            raise ValueError("segments must be ascending (contiguous problems)")
+        self.nseg = int(seg.max().item()) + 1 if n else 0
+        lengths = torch.bincount(seg, minlength=self.nseg)
+        nch = (lengths + chunk - 1) // chunk
+        total = int(nch.sum().item())
+        self.chunk_seg = torch.repeat_interleave(torch.arange(self.nseg, device=dev), nch,
+                                                 output_size=total)
+        slot = torch.arange(total * chunk, device=dev)
+        c = slot // chunk
+        k = self.chunk_seg[c]
+        local = (c - _excl_cumsum(nch)[k]) * chunk + slot % chunk
+        self.gidx = torch.where(local < lengths[k], _excl_cumsum(lengths)[k] + local,
+                                torch.full_like(local, n))
+        self.chunk = chunk
+        self.n = n
+
+    def __call__(self, v: torch.Tensor) -> torch.Tensor:
+        sq = torch.zeros(self.n + 1, dtype=v.dtype, device=v.device)
+        torch.mul(v, v, out=sq[:self.n])
+        part = sq[self.gidx].view(-1, self.chunk).sum(1)
+        return torch.zeros(self.nseg, dtype=v.dtype, device=v.device).index_add_(
+            0, self.chunk_seg, part).sqrt_()
 
 
 class GeometricAMG:
@@ -198,11 +243,7 @@ class GeometricAMG:
             if n <= self.coarse_size or lvl + 1 >= self.max_levels or \
                     all(max(g) <= 1 for g in grid):
                 break
-            if len(grid) == 1:
-                PI, PJ, PA, nc = trilinear_interpolation_device(grid[0], self.device)
-                cgrid = [tuple((x + 1) // 2 for x in grid[0])]
-            else:
-                PI, PJ, PA, nc, cgrid = interpolation_blockdiag(grid, self.device)
+            PI, PJ, PA, nc, cgrid = interpolation_batched(grid, self.device)
             P = MBSRMatrix.from_csr(PI, PJ, PA, n, nc)
             R = P.transpose()
             del PI, PJ, PA
@@ -309,15 +350,12 @@ class GeometricAMG:
         z = torch.empty_like(b)
         Ap = torch.empty_like(b)
         if segments is not None:
-            nseg = int(segments.max().item()) + 1
-            bseg = torch.zeros(nseg, dtype=torch.float64, device=b.device).index_add_(
-                0, segments, b * b).sqrt_()
+            norms = SegmentNorms(segments)
+            bseg = norms(b)
             bseg = torch.where(bseg > 0, bseg, torch.ones_like(bseg))
 
             def relres():
-                rs = torch.zeros(nseg, dtype=torch.float64, device=b.device).index_add_(
-                    0, segments, r * r).sqrt_()
-                return (rs / bseg).max().item()
+                return (norms(r) / bseg).max().item()
         else:
             bn = torch.linalg.vector_norm(b).item()
 
@@ -370,6 +408,7 @@ def solve_poisson(nx: int, ny: int, nz: int, b: np.ndarray, device, mixed: bool 
     amg = GeometricAMG(dI, dJ, dV, (nx, ny, nz), device, omega=omega, coarse_size=coarse_size,
                        spmv_policy=mixed_spmv_policy if mixed else fp64_policy,
                        spgemm_policy=mixed_spgemm_policy if mixed else fp64_policy)
+    tc = time.perf_counter()
     if capture:
         amg.capture()
     torch.cuda.synchronize()
@@ -378,7 +417,7 @@ def solve_poisson(nx: int, ny: int, nz: int, b: np.ndarray, device, mixed: bool 
     torch.cuda.synchronize()
     solve_s = time.perf_counter() - t0
     return {"amg": amg, "x": x, "iters": iters, "hist": hist, "setup": amg.times,
-            "solve_s": solve_s}
+            "capture_s": t0 - tc, "solve_s": solve_s}
 
 
 def batch_poisson2d(sizes, seed: int, device):
@@ -412,6 +451,7 @@ def solve_batch(sizes, seed, device, tol: float = 1e-8, capture: bool = True,
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
     dI, dJ, dV, db, dseg = t(I), t(J), t(V), t(b), t(seg)
     amg = GeometricAMG(dI, dJ, dV, grids, device, omega=omega, coarse_size=coarse_size)
+    tc = time.perf_counter()
     if capture:
         amg.capture()
     torch.cuda.synchronize()
@@ -419,7 +459,7 @@ def solve_batch(sizes, seed, device, tol: float = 1e-8, capture: bool = True,
     x, iters, hist = amg.pcg(db, tol=tol, segments=dseg)
     torch.cuda.synchronize()
     return {"amg": amg, "x": x, "xs": xs, "iters": iters, "hist": hist, "setup": amg.times,
-            "solve_s": time.perf_counter() - t0, "segments": seg}
+            "capture_s": t0 - tc, "solve_s": time.perf_counter() - t0, "segments": seg}
 
 
 def batch_sizes(count: int = 48, seed: int = 4, lo: int = 32, hi: int = 512):
@@ -432,19 +472,22 @@ def solve_one_by_one(sizes, seed, device, tol: float = 1e-8, coarse_size: int = 
                      omega: float = 6.0 / 7.0):
     """The same problems solved one after the other (one hierarchy each)."""
     I, J, V, b, xs, grids, seg = batch_poisson2d(sizes, seed, device)
-    out = []
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
+    # inputs resident on the device before the clock starts, as in solve_batch
+    inputs = []
     row = 0
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
     for n in sizes:
         m = n * n
         Ii, Ji, Vi = W.stencil(7, n, n, 1)
-        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
-        amg = GeometricAMG(t(Ii), t(Ji), t(Vi), (n, n, 1), device, omega=omega,
-                           coarse_size=coarse_size)
-        amg.capture()
-        x, iters, hist = amg.pcg(t(b[row:row + m]), tol=tol)
-        out.append((iters, hist[-1]))
+        inputs.append((t(Ii), t(Ji), t(Vi), t(b[row:row + m]), n))
         row += m
+    out = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for dI, dJ, dV, db, n in inputs:
+        amg = GeometricAMG(dI, dJ, dV, (n, n, 1), device, omega=omega, coarse_size=coarse_size)
+        amg.capture()
+        x, iters, hist = amg.pcg(db, tol=tol)
+        out.append((iters, hist[-1]))
     torch.cuda.synchronize()
     return time.perf_counter() - t0, out

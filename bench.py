@@ -139,8 +139,8 @@ def amg_timings(dev, n1=128):
         best = None
         for _ in range(2):  # the second run has warm code objects and allocator cache
             r = G.solve_poisson(n1, n1, n1, b, dev, mixed=mixed, capture=True)
-            if best is None or r["setup"].total_s + r["solve_s"] < \
-                    best["setup"].total_s + best["solve_s"]:
+            r["total_s"] = r["setup"].total_s + r["capture_s"] + r["solve_s"]
+            if best is None or r["total_s"] < best["total_s"]:
                 best = r
         amg = best["amg"]
         sols[name] = best["x"].cpu().numpy()
@@ -148,9 +148,10 @@ def amg_timings(dev, n1=128):
             "setup_device_s": round(best["setup"].device_s, 4),
             "setup_host_s": round(best["setup"].host_s, 4),
             "galerkin_spgemm_s": [round(v, 4) for v in best["setup"].spgemm_s],
+            "graph_capture_s": round(best["capture_s"], 4),
             "solve_s": round(best["solve_s"], 4),
             "setup_s": round(best["setup"].total_s, 4),
-            "setup_plus_solve_s": round(best["setup"].total_s + best["solve_s"], 4),
+            "setup_plus_solve_s": round(best["total_s"], 4),
             "iterations": best["iters"],
             "final_rel_residual": float(best["hist"][-1]),
             "levels": len(amg.levels),
@@ -180,14 +181,16 @@ def batch_timings(dev):
     best = None
     for _ in range(2):
         r = G.solve_batch(sizes, 4, dev)
-        tot = r["setup"].total_s + r["solve_s"]
+        tot = r["setup"].total_s + r["capture_s"] + r["solve_s"]
         if best is None or tot < best[0]:
             best = (tot, r)
     tot, r = best
     seq_s, seq = G.solve_one_by_one(sizes, 4, dev)
     return {"problems": len(sizes), "rows": int(sum(n * n for n in sizes)),
             "sides_min_max": [min(sizes), max(sizes)],
-            "batch_setup_s": round(r["setup"].total_s, 4), "batch_solve_s": round(r["solve_s"], 4),
+            "batch_setup_s": round(r["setup"].total_s, 4),
+            "batch_graph_capture_s": round(r["capture_s"], 4),
+            "batch_solve_s": round(r["solve_s"], 4),
             "batch_total_s": round(tot, 4), "batch_iterations": r["iters"],
             "batch_max_problem_rel_residual": float(r["hist"][-1]),
             "one_by_one_total_s": round(seq_s, 4),
