@@ -46,6 +46,10 @@ __device__ __forceinline__ float mul_acc<__half, float>(float acc, __half a, dou
 // XCD-aware workgroup remap: hardware hands out workgroups round-robin over
 // the 8 XCDs; give each XCD a contiguous range of block rows so that x reuse
 // between neighbouring block rows stays within one L2.
+// Word 3 of a raw buffer resource on gfx9 parts (32-bit data format; no
+// swizzle): loads past num_records return zeros instead of faulting.
+constexpr int kBufferDword3 = 0x00020000;
+
 __device__ __forceinline__ int remap_wg(int wg, int nwg) {
     const int xcd = wg & 7;
     const int idx = wg >> 3;
@@ -323,76 +327,116 @@ __global__ __launch_bounds__(256) void k_spmv_blk(
         }
         return;
     }
-    // this thread's BPT consecutive blocks
+    // Loads are issued in dependency order and consumed late, so a tile costs
+    // two memory round trips after its row pointers, not one per load: block
+    // maps and columns, then the tile's values, then the x segments (which
+    // need the columns) while the values are still in flight; the values go
+    // to LDS last.
+    // (1) this thread's BPT consecutive blocks; lanes past the tile's end load
+    // its last block (clamped index: an unconditional load nothing waits for)
+    if (nb == 0) {  // every row of the tile is empty
+        for (int rr = tid; rr < 4 * nr; rr += 256) {
+            const int row = 4 * (br0 + (rr >> 2)) + (rr & 3);
+            if (row < nrows) y[row] = ep(row, 0.0);
+        }
+        return;
+    }
+    uint32_t mr[BPT];
+    int cr[BPT];
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) {
+        const int q = min(BPT * tid + u, nb - 1);
+        mr[u] = blk_map[pb + q];
+        cr[u] = blk_col[pb + q];
+    }
+    // (2) the tile's values
+    constexpr bool kPair = std::is_same<VT, double>::value && std::is_same<AT, double>::value;
+    const int head = vb & 1;
+    const int npair = (nv - head) >> 1;
+    double2 tv[kPair ? VPT / 2 : 1];
+    double edge = 0.0;
+    VT ts[kPair ? 1 : VPT];
+    if constexpr (kPair) {
+        // 16-byte loads from the first even value index on, the odd head and
+        // tail element singly
+        // (val is 16-byte aligned: the launcher checks it).  Raw buffer loads
+        // over exactly the tile's pairs: a lane past the last pair gets zeros
+        // from the range check, so the loads need no branch.
+        const double *dval = reinterpret_cast<const double *>(val);
+        const __amdgpu_buffer_rsrc_t vrs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<double *>(dval + vb + head), short(0), npair * 16, kBufferDword3);
+#pragma unroll
+        for (int u = 0; u < VPT / 2; ++u) {
+            const auto q = __builtin_amdgcn_raw_buffer_load_b128(vrs, (u * 256 + tid) * 16, 0, 0);
+            __builtin_memcpy(&tv[u], &q, 16);
+        }
+        // the first (even threads) or last (odd threads) value of the tile,
+        // for threads 0 and 1 (a buffer load: zero if the tile had no values)
+        const __amdgpu_buffer_rsrc_t ers = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<double *>(dval + vb), short(0), nv * 8, kBufferDword3);
+        const auto e = __builtin_amdgcn_raw_buffer_load_b64(ers, ((tid & 1) ? nv - 1 : 0) * 8, 0, 0);
+        __builtin_memcpy(&edge, &e, 8);
+    } else {
+#pragma unroll
+        for (int u = 0; u < VPT; ++u) {
+            const int q = u * 256 + tid;
+            if (q < nv) ts[u] = val[vb + q];
+        }
+    }
+    // (3) x segments of the blocks
     uint32_t m[BPT];
     int c4[BPT];
 #pragma unroll
     for (int u = 0; u < BPT; ++u) {
-        const int q = BPT * tid + u;
-        m[u] = 0u;
-        c4[u] = 0;
-        if (q < nb) {
-            m[u] = blk_map[pb + q];
-            c4[u] = 4 * blk_col[pb + q];
+        const bool ok = BPT * tid + u < nb;
+        m[u] = ok ? mr[u] : 0u;
+        c4[u] = ok ? 4 * cr[u] : 0;
+    }
+    // Unconditional 16-byte loads (no branch for the compiler to wait at): a
+    // block reaching past the last column, and an empty lane, read x[0..3]
+    // instead; an empty lane's segment is never used (no bit is set), a
+    // partial block is reloaded element-wise below.
+    double xs[BPT][4];
+    const bool xvec = ncols >= 4;
+    int xb[BPT];
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) xb[u] = (c4[u] + 4 <= ncols) ? c4[u] : 0;
+    if (xvec) {
+#pragma unroll
+        for (int u = 0; u < BPT; ++u) {
+            const double2 lo2 = *reinterpret_cast<const double2 *>(x + xb[u]);
+            const double2 hi2 = *reinterpret_cast<const double2 *>(x + xb[u] + 2);
+            xs[u][0] = lo2.x;
+            xs[u][1] = lo2.y;
+            xs[u][2] = hi2.x;
+            xs[u][3] = hi2.y;
         }
     }
-    bool vec2 = false;
-    if constexpr (std::is_same<VT, double>::value && std::is_same<AT, double>::value)
-        vec2 = (reinterpret_cast<uintptr_t>(val) & 15u) == 0;
-    if (vec2) {
-        // 16-byte loads from the first even value index on, the odd head and
-        // tail element singly
-        const int head = vb & 1;
-        const int npair = (nv - head) >> 1;
-        const double *dval = reinterpret_cast<const double *>(val);
-        double2 tv[VPT / 2];
-#pragma unroll
-        for (int u = 0; u < VPT / 2; ++u) {
-            const int p = u * 256 + tid;
-            if (p < npair) tv[u] = *reinterpret_cast<const double2 *>(dval + vb + head + 2 * p);
-        }
-        if (tid == 0 && head && nv > 0) s_buf[0] = AT(dval[vb]);
-        if (tid == 1 && head + 2 * npair < nv) s_buf[nv - 1] = AT(dval[vb + nv - 1]);
+    // (4) the values into LDS
+    if constexpr (kPair) {
+        if (tid == 0 && head) s_buf[0] = edge;
+        if (tid == 1 && head + 2 * npair < nv) s_buf[nv - 1] = edge;
 #pragma unroll
         for (int u = 0; u < VPT / 2; ++u) {
             const int p = u * 256 + tid;
             if (p < npair) {
-                s_buf[head + 2 * p] = AT(tv[u].x);
-                s_buf[head + 2 * p + 1] = AT(tv[u].y);
+                s_buf[head + 2 * p] = tv[u].x;
+                s_buf[head + 2 * p + 1] = tv[u].y;
             }
         }
     } else {
-        VT tv[VPT];
 #pragma unroll
         for (int u = 0; u < VPT; ++u) {
             const int q = u * 256 + tid;
-            if (q < nv) tv[u] = val[vb + q];
-        }
-#pragma unroll
-        for (int u = 0; u < VPT; ++u) {
-            const int q = u * 256 + tid;
-            if (q < nv) s_buf[q] = AT(tv[u]);
+            if (q < nv) s_buf[q] = AT(ts[u]);
         }
     }
-    // x segments of the blocks (independent of LDS)
-    double xs[BPT][4];
+    // (5) blocks reaching past the last column: x element-wise
 #pragma unroll
     for (int u = 0; u < BPT; ++u) {
-        xs[u][0] = xs[u][1] = xs[u][2] = xs[u][3] = 0.0;
-        if (m[u]) {
-            const double *xp = x + c4[u];
-            if (c4[u] + 4 <= ncols) {
-                const double2 lo2 = *reinterpret_cast<const double2 *>(xp);
-                const double2 hi2 = *reinterpret_cast<const double2 *>(xp + 2);
-                xs[u][0] = lo2.x;
-                xs[u][1] = lo2.y;
-                xs[u][2] = hi2.x;
-                xs[u][3] = hi2.y;
-            } else {
+        if (m[u] && (!xvec || c4[u] + 4 > ncols)) {
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    if (c4[u] + c < ncols) xs[u][c] = xp[c];
-            }
+            for (int c = 0; c < 4; ++c) xs[u][c] = (c4[u] + c < ncols) ? x[c4[u] + c] : 0.0;
         }
     }
     // exclusive scan of the popcounts over the workgroup -> value offsets
@@ -1037,7 +1081,11 @@ int launch_spmv_t(const Mbsr &A, const VT *val, int lanes, const double *x, EP e
             default: return launch_spmv_bal_g<2, VT, AT>(A, val, x, ep, y, s);
         }
     }
-    if ((A.spmv_kernel == kSpmvBlock || A.spmv_kernel == kSpmvBalanced) && x_aligned) {
+    // FP64 values are staged in 16-byte pieces too (handle arrays always are)
+    const bool val_aligned =
+        sizeof(VT) != 8 || (reinterpret_cast<uintptr_t>(val) & 15u) == 0;
+    if ((A.spmv_kernel == kSpmvBlock || A.spmv_kernel == kSpmvBalanced) && x_aligned &&
+        val_aligned) {
         // lanes selects the tile: 4 -> 64 block rows (2 blocks per thread),
         // 8 -> 128 (4 per thread), 16 -> 32, 32 -> 16, 64 -> 8 block rows
         switch (lanes) {

@@ -45,6 +45,9 @@ CASES = {
                                                   empty_rows=(0, 7, 2001, 3994))),
     # block totals landing on multiples of 256 / 512 / 1024 (balanced tile edges)
     "tile_edges": lambda: _tile_edges(),
+    # long runs of empty rows: whole SpMV tiles without a block
+    "mostly_empty": lambda: ((3001, 2999), np.array([0] * 4 + [2] * 1500 + [3] * 1498, np.int32),
+                             np.array([7, 2998, 1], np.int32), np.array([1.25, -3.0, 0.5])),
 }
 
 
@@ -119,7 +122,7 @@ def row_scale(m, n, I, J, A, x, beta, b):
 
 @pytest.mark.parametrize("name", ["c1_5pt_128", "7pt_aniso", "27pt_10", "rand_rect",
                                   "rand_tall_empty", "rand_wide", "bin1_rows", "bin2_rows",
-                                  "skewed", "tile_edges"])
+                                  "skewed", "tile_edges", "one_row", "mostly_empty"])
 @pytest.mark.parametrize("lanes", [4, 8, 16, 32, 64])
 @pytest.mark.parametrize("kind", [0, 1, 2, 3])
 def test_spmv_fp64(dev, name, lanes, kind):
