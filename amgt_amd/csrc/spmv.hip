@@ -292,6 +292,7 @@ __global__ __launch_bounds__(256) void k_spmv_blk(
     const int32_t *__restrict__ val_ptr, const VT *__restrict__ val, const double *__restrict__ x,
     EP ep, double *y) {
     constexpr int CAPB = 256 * BPT, CAPV = 4 * CAPB, VPT = CAPV / 256;
+    static_assert(RPB < 256, "one thread per row offset");
     __shared__ AT s_buf[CAPV];
     __shared__ int32_t s_bp[RPB + 1], s_vp[RPB + 1];
     __shared__ int32_t s_sc[4];
@@ -300,15 +301,20 @@ __global__ __launch_bounds__(256) void k_spmv_blk(
     const int wg = remap_wg(blockIdx.x, gridDim.x);
     const int br0 = wg * RPB;
     const int nr = min(RPB, brow - br0);
-    for (int t = tid; t <= nr; t += 256) {
-        s_bp[t] = blk_ptr[br0 + t];
-        s_vp[t] = val_ptr[br0 + t];
-    }
-    __syncthreads();
-    const int pb = s_bp[0], pe = s_bp[nr];
-    const int vb = s_vp[0], ve = s_vp[nr];
+    // the tile's ends: wave-uniform addresses, so scalar loads; the value and
+    // block loads wait for nothing else
+    const int pb = blk_ptr[br0], pe = blk_ptr[br0 + nr];
+    const int vb = val_ptr[br0], ve = val_ptr[br0 + nr];
     const int nb = pe - pb, nv = ve - vb;
+    // this thread's row offset for the row sums at the end (clamped index: an
+    // unconditional load; stored to LDS once the tile's loads are in flight)
+    const int bp_t = blk_ptr[br0 + min(tid, nr)];
     if (nb > CAPB || nv > CAPV) {
+        for (int t = tid; t <= nr; t += 256) {
+            s_bp[t] = blk_ptr[br0 + t];
+            s_vp[t] = val_ptr[br0 + t];
+        }
+        __syncthreads();
         // oversized tile: 256/RPB lanes per block row walk global memory
         constexpr int G = 256 / RPB >= 4 ? (256 / RPB > 64 ? 64 : 256 / RPB) : 4;
         constexpr int RPW = 64 / G;
@@ -412,7 +418,8 @@ __global__ __launch_bounds__(256) void k_spmv_blk(
             xs[u][3] = hi2.y;
         }
     }
-    // (4) the values into LDS
+    // (4) the values and row offsets into LDS
+    if (tid <= nr) s_bp[tid] = bp_t;
     if constexpr (kPair) {
         if (tid == 0 && head) s_buf[0] = edge;
         if (tid == 1 && head + 2 * npair < nv) s_buf[nv - 1] = edge;
