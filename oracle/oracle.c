@@ -620,3 +620,324 @@ void orc_mbsr_spgemm_fill(int brow_a, const int *ap, const int *ac, const uint16
     free(aoff);
     free(boff);
 }
+
+/* ------------------------------------------------------------------------ */
+/* BoomerAMG setup on one process.  HYPRE keeps the diagonal first in every
+ * row of a ParCSR diag block and reads it as A_diag_i[i]; here it is found by
+ * column, so rows in any column order work.  Off-diagonal entries are visited
+ * in row order, as HYPRE visits A_diag_i[i]+1 .. A_diag_i[i+1]-1. */
+static int64_t amg_diag_pos(const int *AI, const int *AJ, int i)
+{
+    for (int64_t p = AI[i]; p < AI[i + 1]; p++)
+        if (AJ[p] == i) return p;
+    return -1;
+}
+
+/* par_strength.c:75-528, num_functions = 1, one process */
+int64_t orc_amg_strength(int n, const int *AI, const int *AJ, const double *AA, double theta,
+                         double max_row_sum, int *SI, int *SJ)
+{
+    int64_t cnt = 0;
+    for (int i = 0; i < n; i++) {
+        SI[i] = (int)cnt;
+        const int64_t d = amg_diag_pos(AI, AJ, i);
+        const double diag = d >= 0 ? AA[d] : 0.0;
+        double row_scale = 0.0, row_sum = diag;
+        for (int64_t p = AI[i]; p < AI[i + 1]; p++) {
+            if (p == d) continue;
+            const double a = AA[p];
+            if (diag < 0) row_scale = a > row_scale ? a : row_scale;
+            else row_scale = a < row_scale ? a : row_scale;
+            row_sum += a;
+        }
+        /* par_strength.c:325-339: a row that is too diagonally dominant has
+         * only weak dependencies */
+        if (fabs(row_sum) > fabs(diag) * max_row_sum && max_row_sum < 1.0) continue;
+        for (int64_t p = AI[i]; p < AI[i + 1]; p++) {
+            if (p == d) continue;
+            const double a = AA[p];
+            const int weak = diag < 0 ? (a <= theta * row_scale) : (a >= theta * row_scale);
+            if (!weak) SJ[cnt++] = AJ[p];
+        }
+    }
+    SI[n] = (int)cnt;
+    return cnt;
+}
+
+/* par_indepset.c (hypre_BoomerAMGIndepSetInit): seed 2747 (one process, and
+ * the sequential stream skips FirstRowIndex = 0 draws), one draw per row */
+void orc_amg_pmis_measures(int n, const int *SI, const int *SJ, double *measure)
+{
+    for (int i = 0; i < n; i++) measure[i] = 0.0;
+    for (int64_t p = 0; p < SI[n]; p++) measure[SJ[p]] += 1.0;
+    int32_t st = orc_pm_seed(2747);
+    for (int i = 0; i < n; i++) measure[i] += pm_rand(&st);
+}
+
+/* par_coarsen.c:2101-2813 on one process */
+int orc_amg_pmis(int n, const int *SI, const int *SJ, int cf_init, int *cf)
+{
+    double *measure = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    int *graph = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    int *graph2 = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    orc_amg_pmis_measures(n, SI, SJ, measure);
+    int gsize = 0;
+    for (int i = 0; i < n; i++) {
+        cf[i] = 0;
+        if (SI[i + 1] - SI[i] == 0) { /* :2385-2396, no strong dependence */
+            cf[i] = (cf_init == 3 || cf_init == 4) ? 1 : -3;
+            measure[i] = 0.0;
+        } else {
+            graph[gsize++] = i;
+        }
+    }
+    int iter = 0;
+    while (gsize > 0) {
+        if (!cf_init || iter) { /* :2488-2573 independent set */
+            for (int g = 0; g < gsize; g++) {
+                const int i = graph[g];
+                if (measure[i] > 1) cf[i] = 1;
+            }
+            for (int g = 0; g < gsize; g++) {
+                const int i = graph[g];
+                if (!(measure[i] > 1)) continue;
+                for (int p = SI[i]; p < SI[i + 1]; p++) {
+                    const int j = SJ[p];
+                    if (measure[j] > 1) {
+                        if (measure[i] > measure[j]) cf[j] = 0;
+                        else if (measure[j] > measure[i]) cf[i] = 0;
+                    }
+                }
+            }
+        }
+        iter++;
+        for (int g = 0; g < gsize; g++) { /* :2583-2666 C and F points */
+            const int i = graph[g];
+            if (measure[i] < 1) cf[i] = -1;
+            if (cf[i] > 0) {
+                cf[i] = 1;
+            } else {
+                for (int p = SI[i]; p < SI[i + 1]; p++)
+                    if (cf[SJ[p]] > 0) cf[i] = -1;
+            }
+        }
+        int ng = 0; /* :2690-2760 subgraph update */
+        for (int g = 0; g < gsize; g++) {
+            const int i = graph[g];
+            if (cf[i] != 0) measure[i] = 0.0;
+            else graph2[ng++] = i;
+        }
+        int *t = graph;
+        graph = graph2;
+        graph2 = t;
+        gsize = ng;
+    }
+    int nc = 0;
+    for (int i = 0; i < n; i++) nc += cf[i] >= 0;
+    free(measure);
+    free(graph);
+    free(graph2);
+    return nc;
+}
+
+/* par_lr_interp.c:1025-1928 on one process: the structure (C points of S row
+ * i, then the strong C points of each strong F neighbour, first-touch order)
+ * and the weights of extended+i interpolation. */
+static int amg_pass(int n, const int *AI, const int *AJ, const double *AA, const int *SI,
+                    const int *SJ, const int *cf, const int *f2c, int64_t *pm, const int *PI,
+                    int *PJ, double *PA, int fill)
+{
+    int64_t counter = 0;
+    int64_t strong_f_marker = -2;
+    for (int i = 0; i < n; i++) pm[i] = -1;
+    for (int i = 0; i < n; i++) {
+        const int64_t begin = fill ? PI[i] : counter;
+        if (fill) counter = begin;
+        if (cf[i] >= 0) { /* :1488-1495 C point: identity */
+            if (fill) {
+                PJ[counter] = f2c[i];
+                PA[counter] = 1.0;
+            }
+            counter++;
+            continue;
+        }
+        if (cf[i] == -3) continue;
+        strong_f_marker--;
+        for (int p = SI[i]; p < SI[i + 1]; p++) { /* :1501-1560 */
+            const int i1 = SJ[p];
+            if (cf[i1] >= 0) {
+                if (pm[i1] < begin) {
+                    pm[i1] = counter;
+                    if (fill) {
+                        PJ[counter] = f2c[i1];
+                        PA[counter] = 0.0;
+                    }
+                    counter++;
+                }
+            } else if (cf[i1] != -3) {
+                pm[i1] = strong_f_marker;
+                for (int q = SI[i1]; q < SI[i1 + 1]; q++) {
+                    const int k1 = SJ[q];
+                    if (cf[k1] >= 0 && pm[k1] < begin) {
+                        pm[k1] = counter;
+                        if (fill) {
+                            PJ[counter] = f2c[k1];
+                            PA[counter] = 0.0;
+                        }
+                        counter++;
+                    }
+                }
+            }
+        }
+        if (!fill) continue;
+        /* :1630-1708 weights */
+        const int64_t di = amg_diag_pos(AI, AJ, i);
+        double diagonal = di >= 0 ? AA[di] : 0.0;
+        for (int64_t jj = AI[i]; jj < AI[i + 1]; jj++) {
+            if (jj == di) continue;
+            const int i1 = AJ[jj];
+            if (pm[i1] >= begin) {
+                PA[pm[i1]] += AA[jj];
+            } else if (pm[i1] == strong_f_marker) {
+                const int64_t d1 = amg_diag_pos(AI, AJ, i1);
+                const int sgn = (d1 >= 0 && AA[d1] < 0) ? -1 : 1;
+                double sum = 0.0;
+                for (int64_t q = AI[i1]; q < AI[i1 + 1]; q++) {
+                    if (q == d1) continue;
+                    const int i2 = AJ[q];
+                    if ((pm[i2] >= begin || i2 == i) && (sgn * AA[q]) < 0) sum += AA[q];
+                }
+                if (sum != 0) {
+                    const double distribute = AA[jj] / sum;
+                    for (int64_t q = AI[i1]; q < AI[i1 + 1]; q++) {
+                        if (q == d1) continue;
+                        const int i2 = AJ[q];
+                        if (pm[i2] >= begin && (sgn * AA[q]) < 0) PA[pm[i2]] += distribute * AA[q];
+                        if (i2 == i && (sgn * AA[q]) < 0) diagonal += distribute * AA[q];
+                    }
+                } else {
+                    diagonal += AA[jj];
+                }
+            } else if (cf[i1] != -3) {
+                diagonal += AA[jj];
+            }
+        }
+        if (diagonal != 0) /* :1790-1800 */
+            for (int64_t jj = begin; jj < counter; jj++) PA[jj] /= -diagonal;
+    }
+    return (int)counter;
+}
+
+static int *amg_fine_to_coarse(int n, const int *cf)
+{
+    int *f2c = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    int c = 0;
+    for (int i = 0; i < n; i++) f2c[i] = cf[i] >= 0 ? c++ : -1;
+    return f2c;
+}
+
+int64_t orc_amg_extpi_count(int n, const int *AI, const int *AJ, const int *SI, const int *SJ,
+                            const int *cf, int *PI)
+{
+    int *f2c = amg_fine_to_coarse(n, cf);
+    int64_t *pm = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    /* count row by row: a pass that records each row's start */
+    int64_t total = 0;
+    for (int i = 0; i < n; i++) pm[i] = -1;
+    int64_t strong_f_marker = -2;
+    for (int i = 0; i < n; i++) {
+        PI[i] = (int)total;
+        const int64_t begin = total;
+        if (cf[i] >= 0) { total++; continue; }
+        if (cf[i] == -3) continue;
+        strong_f_marker--;
+        for (int p = SI[i]; p < SI[i + 1]; p++) {
+            const int i1 = SJ[p];
+            if (cf[i1] >= 0) {
+                if (pm[i1] < begin) { pm[i1] = total; total++; }
+            } else if (cf[i1] != -3) {
+                pm[i1] = strong_f_marker;
+                for (int q = SI[i1]; q < SI[i1 + 1]; q++) {
+                    const int k1 = SJ[q];
+                    if (cf[k1] >= 0 && pm[k1] < begin) { pm[k1] = total; total++; }
+                }
+            }
+        }
+    }
+    PI[n] = (int)total;
+    free(pm);
+    free(f2c);
+    return total;
+}
+
+void orc_amg_extpi_fill(int n, const int *AI, const int *AJ, const double *AA, const int *SI,
+                        const int *SJ, const int *cf, const int *PI, int *PJ, double *PA)
+{
+    int *f2c = amg_fine_to_coarse(n, cf);
+    int64_t *pm = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    amg_pass(n, AI, AJ, AA, SI, SJ, cf, f2c, pm, PI, PJ, PA, 1);
+    free(pm);
+    free(f2c);
+}
+
+/* qsort.c:395-417, descending |w| */
+static void amg_swap2(int *v, double *w, int i, int j)
+{
+    int t = v[i];
+    v[i] = v[j];
+    v[j] = t;
+    double t2 = w[i];
+    w[i] = w[j];
+    w[j] = t2;
+}
+
+static void amg_qsort2_abs(int *v, double *w, int left, int right)
+{
+    if (left >= right) return;
+    amg_swap2(v, w, left, (left + right) / 2);
+    int last = left;
+    for (int i = left + 1; i <= right; i++)
+        if (fabs(w[i]) > fabs(w[left])) amg_swap2(v, w, ++last, i);
+    amg_swap2(v, w, left, last);
+    amg_qsort2_abs(v, w, left, last - 1);
+    amg_qsort2_abs(v, w, last + 1, right);
+}
+
+/* par_csr_matrix.c:3085-3230 (tol = 0, rescale = 1, one process) */
+int64_t orc_amg_truncate(int n, int *PI, int *PJ, double *PA, int max_elmts)
+{
+    if (max_elmts <= 0) return PI[n];
+    int64_t out = 0;
+    for (int i = 0; i < n; i++) {
+        const int64_t b = PI[i], e = PI[i + 1];
+        const int cnt = (int)(e - b);
+        PI[i] = (int)out;
+        if (cnt > max_elmts) {
+            double row_sum = 0.0;
+            for (int64_t p = b; p < e; p++) row_sum += PA[p];
+            /* sort a copy in place: entries b.. are not read again */
+            int *vj = PJ + b;
+            double *vw = PA + b;
+            amg_qsort2_abs(vj, vw, 0, cnt - 1);
+            double scale = 0.0;
+            for (int k = 0; k < max_elmts; k++) scale += vw[k];
+            for (int k = 0; k < max_elmts; k++) {
+                PJ[out + k] = vj[k];
+                PA[out + k] = vw[k];
+            }
+            if (scale != 0. && scale != row_sum) {
+                scale = row_sum / scale;
+                for (int k = 0; k < max_elmts; k++) PA[out + k] *= scale;
+            }
+            out += max_elmts;
+        } else {
+            for (int64_t p = b; p < e; p++) {
+                PJ[out] = PJ[p];
+                PA[out] = PA[p];
+                out++;
+            }
+        }
+    }
+    PI[n] = (int)out;
+    return out;
+}

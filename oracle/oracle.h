@@ -105,6 +105,30 @@ void orc_mbsr_spgemm_fill(int brow_a, const int *ap, const int *ac, const uint16
                           const int *bvp, const double *bv,
                           const int *cp, int *cc, uint16_t *cm, int *cvp, double *cv);
 
+/* ---- BoomerAMG setup on one process (CSR, any column order; the diagonal is
+ *      found by column, HYPRE keeps it first) ---- */
+/* strength, hypre_BoomerAMGCreateSHost par_strength.c:75-528 (one function):
+ * SI[n+1], SJ[nnz(A)] (strong columns in row order, no diagonal); returns nnz(S) */
+int64_t orc_amg_strength(int n, const int *AI, const int *AJ, const double *AA, double theta,
+                         double max_row_sum, int *SI, int *SJ);
+/* PMIS, hypre_BoomerAMGCoarsenPMISHost par_coarsen.c:2101-2813 with measures
+ * from hypre_BoomerAMGIndepSetInit par_indepset.c (seed 2747): cf = 1 (C),
+ * -1 (F), -3 (no strong dependence); returns the number of C points */
+int orc_amg_pmis(int n, const int *SI, const int *SJ, int cf_init, int *cf);
+/* measures |S^T row| + hypre_Rand() (seed 2747), for checking */
+void orc_amg_pmis_measures(int n, const int *SI, const int *SJ, double *measure);
+/* extended+i interpolation, hypre_BoomerAMGBuildExtPIInterpHost
+ * par_lr_interp.c:1025-1928 (one process): pass 1 fills PI[n+1] and returns
+ * nnz, pass 2 fills PJ/PA (row entries in discovery order) */
+int64_t orc_amg_extpi_count(int n, const int *AI, const int *AJ, const int *SI, const int *SJ,
+                            const int *cf, int *PI);
+void orc_amg_extpi_fill(int n, const int *AI, const int *AJ, const double *AA, const int *SI,
+                        const int *SJ, const int *cf, const int *PI, int *PJ, double *PA);
+/* keep the max_elmts largest |p_ij| per row and rescale to the row sum,
+ * hypre_ParCSRMatrixTruncate par_csr_matrix.c:2866-3260 (tol 0, rescale, the
+ * row's order from hypre_qsort2_abs qsort.c:395-417); in place, returns nnz */
+int64_t orc_amg_truncate(int n, int *PI, int *PJ, double *PA, int max_elmts);
+
 #ifdef __cplusplus
 }
 #endif

@@ -70,6 +70,13 @@ def lib():
             "orc_mbsr_spgemm_fill": (None, [_i32, _i32p, _i32p, _u16p, _i32p, _f64p,
                                             _i32, _i32p, _i32p, _u16p, _i32p, _f64p,
                                             _i32p, _i32p, _u16p, _i32p, _f64p]),
+            "orc_amg_strength": (_i64, [_i32, _i32p, _i32p, _f64p, _f64, _f64, _i32p, _i32p]),
+            "orc_amg_pmis": (_i32, [_i32, _i32p, _i32p, _i32, _i32p]),
+            "orc_amg_pmis_measures": (None, [_i32, _i32p, _i32p, _f64p]),
+            "orc_amg_extpi_count": (_i64, [_i32, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p]),
+            "orc_amg_extpi_fill": (None, [_i32, _i32p, _i32p, _f64p, _i32p, _i32p, _i32p, _i32p,
+                                          _i32p, _f64p]),
+            "orc_amg_truncate": (_i64, [_i32, _i32p, _i32p, _f64p, _i32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -280,3 +287,78 @@ def mbsr_spgemm(MA, MB):
     return dict(nrows=MA["nrows"], ncols=MB["ncols"], brow=brow, bcol=MB["bcol"], nblk=int(nblk),
                 nnz=nnz, blk_ptr=cp, blk_col=cc[:nblk], blk_map=cm[:nblk], val_ptr=cvp,
                 val=cv[:nnz])
+
+
+# --------------------------------------------------------------------------
+# BoomerAMG setup on one process (par_strength.c, par_coarsen.c,
+# par_lr_interp.c, par_csr_matrix.c truncation)
+
+def amg_strength(n, I, J, A, theta=0.25, max_row_sum=1.0):
+    """S (SI, SJ): strong columns per row in row order, no diagonal."""
+    I, J, A = _c32(I), _c32(J), _cf(A)
+    SI = np.zeros(n + 1, np.int32)
+    SJ = np.zeros(max(len(J), 1), np.int32)
+    nnz = lib().orc_amg_strength(n, I, J, A, float(theta), float(max_row_sum), SI, SJ)
+    return SI, SJ[:nnz].copy()
+
+
+def amg_pmis_measures(n, SI, SJ):
+    m = np.zeros(max(n, 1), np.float64)
+    lib().orc_amg_pmis_measures(n, _c32(SI), _c32(SJ), m)
+    return m[:n]
+
+
+def amg_pmis(n, SI, SJ, cf_init=0):
+    """CF marker: 1 C, -1 F, -3 no strong dependence; and the C count."""
+    cf = np.zeros(max(n, 1), np.int32)
+    nc = lib().orc_amg_pmis(n, _c32(SI), _c32(SJ), int(cf_init), cf)
+    return cf[:n], int(nc)
+
+
+def amg_extpi(n, I, J, A, SI, SJ, cf, max_elmts=0):
+    """Extended+i P (PI, PJ, PA, ncoarse), row entries in HYPRE's order,
+    truncated to max_elmts per row when max_elmts > 0."""
+    I, J, A, SI, SJ, cf = _c32(I), _c32(J), _cf(A), _c32(SI), _c32(SJ), _c32(cf)
+    PI = np.zeros(n + 1, np.int32)
+    nnz = lib().orc_amg_extpi_count(n, I, J, SI, SJ, cf, PI)
+    PJ = np.zeros(max(nnz, 1), np.int32)
+    PA = np.zeros(max(nnz, 1), np.float64)
+    lib().orc_amg_extpi_fill(n, I, J, A, SI, SJ, cf, PI, PJ, PA)
+    if max_elmts > 0:
+        nnz = lib().orc_amg_truncate(n, PI, PJ, PA, int(max_elmts))
+    return PI, PJ[:nnz].copy(), PA[:nnz].copy(), int((cf >= 0).sum())
+
+
+def csr_transpose(m, n, I, J, A):
+    """Rows of the transpose in ascending row order, entries in column order
+    of the input (a counting sort)."""
+    I, J, A = np.asarray(I), np.asarray(J), np.asarray(A)
+    cnt = np.bincount(J, minlength=n)
+    TI = np.zeros(n + 1, np.int64)
+    np.cumsum(cnt, out=TI[1:])
+    order = np.argsort(J, kind="stable")
+    rows = np.repeat(np.arange(m), np.diff(I))
+    return TI.astype(np.int32), rows[order].astype(np.int32), A[order].copy()
+
+
+def amg_hierarchy(n, I, J, A, theta=0.25, max_row_sum=1.0, cf_init=0, max_elmts=0,
+                  max_coarse_size=9, max_levels=25):
+    """The one-process BoomerAMG setup loop (par_amg_setup.c:994-3182) with
+    strength -> PMIS -> extended+i -> R = P^T -> R (A P) (host SpGEMM, the
+    diagonal first).  Returns the levels as dicts (n, I, J, A, cf, P...)."""
+    levels = [{"n": n, "I": _c32(I), "J": _c32(J), "A": _cf(A)}]
+    while True:
+        L = levels[-1]
+        if L["n"] <= max_coarse_size or len(levels) >= max_levels:
+            break
+        SI, SJ = amg_strength(L["n"], L["I"], L["J"], L["A"], theta, max_row_sum)
+        cf, nc = amg_pmis(L["n"], SI, SJ, cf_init)
+        if nc == 0 or nc == L["n"]:
+            break
+        PI, PJ, PA, nc = amg_extpi(L["n"], L["I"], L["J"], L["A"], SI, SJ, cf, max_elmts)
+        RI, RJ, RA = csr_transpose(L["n"], nc, PI, PJ, PA)
+        API, APJ, APA = csr_spgemm(L["n"], nc, L["I"], L["J"], L["A"], PI, PJ, PA, False)
+        CI, CJ, CA = csr_spgemm(nc, nc, RI, RJ, RA, API, APJ, APA, True)
+        L.update(S=(SI, SJ), cf=cf, P=(PI, PJ, PA), nc=nc)
+        levels.append({"n": nc, "I": CI, "J": CJ, "A": CA})
+    return levels

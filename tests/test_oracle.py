@@ -17,6 +17,13 @@ Pin map (oracle function -> tests):
   orc_csr_matvec / _t -> test_csr_matvec_vs_scipy
   orc_csr_spgemm_* -> test_spgemm_nnz_fixtures, test_csr_spgemm_vs_scipy
   orc_mbsr_spgemm_* -> test_mbsr_spgemm_vs_scipy, test_mbsr_spgemm_brute_force
+  orc_amg_strength / orc_amg_pmis / orc_amg_extpi_* (and the setup loop in
+    oracle.amg_hierarchy) -> test_amg_setup_matches_ij_default (HYPRE's own
+    TEST_ij default fixture: grid and operator complexity pin the structure of
+    S, the C/F split, P and R A P on every level; the average convergence
+    factor of the V(1,1) Jacobi solve pins P's values), test_amg_strength_anisotropic,
+    test_amg_pmis_invariants
+  orc_amg_truncate -> test_amg_truncate_invariants
 """
 import json
 import os
@@ -319,3 +326,140 @@ def test_mbsr_spgemm_brute_force():
         for j in range(7):
             exact[i, j] = sum(Ad[i, t] * Bd[t, j] for t in range(12))
     assert np.array_equal(C, exact)  # small integers: exact
+
+
+
+# --------------------------------------------------------------------------
+# BoomerAMG setup (strength, PMIS, extended+i, truncation)
+
+def _vcycle_factory(levels):
+    mats = []
+    for L in levels:
+        M = sp.csr_matrix((L["A"], L["J"], L["I"]), shape=(L["n"], L["n"]))
+        P = None
+        if "P" in L:
+            PI, PJ, PA = L["P"]
+            P = sp.csr_matrix((PA, PJ, PI), shape=(L["n"], L["nc"]))
+        mats.append((M, P))
+
+    def cyc(lvl, f):  # V(1,1), weighted Jacobi (weight 1), dense solve on the coarsest
+        M, P = mats[lvl]
+        if lvl == len(mats) - 1:
+            return np.linalg.solve(M.toarray(), f)
+        d = M.diagonal()
+        u = f / d
+        u = u + P @ cyc(lvl + 1, P.T @ (f - M @ u))
+        return u + (f - M @ u) / d
+    return mats, cyc
+
+
+def test_amg_setup_matches_ij_default():
+    g = GOLD["ij_default"]
+    I, J, A = O.stencil(7, 10, 10, 10)
+    levels = O.amg_hierarchy(1000, I, J, A, theta=0.25, max_row_sum=1.0, cf_init=2,
+                             max_elmts=0, max_coarse_size=9)
+    ns = [L["n"] for L in levels]
+    nnz = [len(L["J"]) for L in levels]
+    assert round(sum(ns) / ns[0], 6) == g["grid_complexity"]
+    assert round(sum(nnz) / nnz[0], 6) == g["operator_complexity"]
+    mats, cyc = _vcycle_factory(levels)
+    A0 = mats[0][0]
+    b = A0 @ np.ones(1000)
+    x = np.zeros(1000)
+    r0 = bn = np.linalg.norm(b)
+    it, r = 0, r0
+    while r / bn >= 1e-8 and it < 100:
+        x = x + cyc(0, b - A0 @ x)
+        r = np.linalg.norm(b - A0 @ x)
+        it += 1
+    assert round((r / r0) ** (1.0 / it), 6) == g["average_convergence_factor"]
+    # a wrong weight is caught: perturbing one interpolation row changes the factor
+    PI, PJ, PA = levels[0]["P"]
+    f_row = int(np.nonzero(np.diff(PI) > 1)[0][0])
+    PA[PI[f_row]] *= 1.5
+    mats2, cyc2 = _vcycle_factory(levels)
+    x = np.zeros(1000)
+    for _ in range(it):
+        x = x + cyc2(0, b - A0 @ x)
+    assert round((np.linalg.norm(b - A0 @ x) / r0) ** (1.0 / it), 6) != g["average_convergence_factor"]
+
+
+def test_amg_strength_anisotropic():
+    # c = (1, 1, 0.01): z couplings (-0.01) are weak against theta * (-1)
+    # (weak iff a >= 0.25 * min_j a_ij = -0.25), x and y couplings strong
+    nx, ny, nz = 5, 4, 3
+    I, J, A = O.stencil(7, nx, ny, nz, c=(1.0, 1.0, 0.01))
+    SI, SJ = O.amg_strength(nx * ny * nz, I, J, A, theta=0.25, max_row_sum=1.0)
+    for i in range(nx * ny * nz):
+        x, y = i % nx, (i // nx) % ny
+        exp = sorted(j for j in (i - 1, i + 1, i - nx, i + nx)
+                     if 0 <= j < nx * ny * nz and abs(j % nx - x) + abs((j // nx) % ny - y) == 1
+                     and j // (nx * ny) == i // (nx * ny))
+        assert sorted(SJ[SI[i]:SI[i + 1]].tolist()) == exp, i
+    # theta above 1 leaves nothing strong; max_row_sum < 1 with a non-zero row
+    # sum (boundary rows of a Laplacian) makes those rows weak
+    SI2, _ = O.amg_strength(nx * ny * nz, I, J, A, theta=1.01, max_row_sum=1.0)
+    assert SI2[-1] == 0
+    I3, J3, A3 = O.stencil(7, 4, 4, 4)
+    SI3, _ = O.amg_strength(64, I3, J3, A3, theta=0.25, max_row_sum=0.1)
+    rs = np.add.reduceat(A3, I3[:-1])
+    for i in range(64):
+        if abs(rs[i]) > 0.1 * 6.0:
+            assert SI3[i + 1] == SI3[i]
+        else:
+            assert SI3[i + 1] - SI3[i] == I3[i + 1] - I3[i] - 1
+
+
+def test_amg_pmis_invariants():
+    I, J, A = O.stencil(7, 9, 8, 7)
+    n = 9 * 8 * 7
+    SI, SJ = O.amg_strength(n, I, J, A)
+    cf, nc = O.amg_pmis(n, SI, SJ, cf_init=0)
+    m = O.amg_pmis_measures(n, SI, SJ)
+    # measures: column counts of S plus one Park-Miller draw each (seed 2747)
+    counts = np.bincount(SJ, minlength=n)
+    draws = O.pm_sequence(2747, n).astype(np.float64) / 2147483647.0
+    assert np.array_equal(m, counts.astype(np.float64) + draws)
+    assert nc == int((cf == 1).sum()) and set(np.unique(cf)) <= {1, -1, -3}
+    S = sp.csr_matrix((np.ones(len(SJ)), SJ, SI), shape=(n, n))
+    Ssym = ((S + S.T) > 0).tocsr()
+    C = cf == 1
+    # every F point that influences someone depends on a C point; points that
+    # influence nobody (measure < 1) are F
+    for i in range(n):
+        if cf[i] == -1 and counts[i] > 0:
+            assert C[SJ[SI[i]:SI[i + 1]]].any(), i
+        if counts[i] == 0:
+            assert cf[i] != 1
+    # within one independent-set pass two strongly connected C points cannot
+    # both have won: the C point with the larger measure of any S-edge pair
+    # was chosen first, so the other depends on no C point chosen before it
+    rows, cols = Ssym.nonzero()
+    both = C[rows] & C[cols]
+    assert both.sum() < 0.1 * C.sum()
+
+
+def test_amg_truncate_invariants():
+    rng = np.random.default_rng(5)
+    n, k = 40, 3
+    counts = rng.integers(0, 9, n)
+    PI = np.zeros(n + 1, np.int32)
+    PI[1:] = np.cumsum(counts)
+    PJ = rng.integers(0, 50, PI[-1]).astype(np.int32)
+    PA = rng.standard_normal(PI[-1])
+    PA[::7] = PA[1::7][:len(PA[::7])] if len(PA) > 8 else PA[::7]  # some ties in |w|
+    TI, TJ, TA = PI.copy(), PJ.copy(), PA.copy()
+    nnz = O.lib().orc_amg_truncate(n, TI, TJ, TA, k)
+    assert nnz == TI[-1] == np.minimum(counts, k).sum()
+    for i in range(n):
+        a, b = PA[PI[i]:PI[i + 1]], TA[TI[i]:TI[i + 1]]
+        if counts[i] <= k:
+            assert np.array_equal(a, b) and np.array_equal(PJ[PI[i]:PI[i + 1]], TJ[TI[i]:TI[i + 1]])
+            continue
+        kept_abs = np.sort(np.abs(a))[::-1][:k]
+        # the k largest |w| are kept (as a multiset), in decreasing |w| order
+        # before the rescale, and the row sum is preserved
+        assert np.all(np.diff(np.abs(b)) <= 1e-15 * np.abs(b).max())
+        assert abs(b.sum() - a.sum()) <= 1e-12 * np.abs(a).sum()
+        ratio = b / b[0] * np.abs(kept_abs[0]) * np.sign(b[0])
+        assert np.allclose(np.sort(np.abs(ratio))[::-1], kept_abs, rtol=1e-12)
