@@ -42,6 +42,7 @@ EXPORTS = (
     "amgt_mbsr_to_csr", "amgt_free", "amgt_csr_spgemm", "amgt_csr_matvec",
     "amgt_mbsr_diagonal", "amgt_mbsr_relax_jacobi", "amgt_mbsr_transpose",
     "amgt_read_ij", "amgt_read_mm", "amgt_host_free", "amgt_mbsr_spmm_interleaved",
+    "amgt_amg_strength", "amgt_amg_pmis", "amgt_amg_extpi_interp",
 )
 
 
@@ -131,6 +132,10 @@ def load():
                                 _pi64, _pvp, _pvp, _pvp]),
         "amgt_host_free": (None, [_vp]),
         "amgt_mbsr_relax_jacobi": (_int, [_vp, _dbl, _vp, _vp, _vp, _vp, _vp]),
+        "amgt_amg_strength": (_int, [_i32, _vp, _vp, _vp, _dbl, _dbl, _pi64, _pvp, _pvp, _vp]),
+        "amgt_amg_pmis": (_int, [_i32, _vp, _vp, _int, _vp, ctypes.POINTER(_i32), _vp]),
+        "amgt_amg_extpi_interp": (_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _int,
+                                         ctypes.POINTER(_i32), _pi64, _pvp, _pvp, _pvp, _vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -159,6 +159,7 @@ class SetupTimes:
     device_s: float = 0.0        # P, R = P^T, conversions, Galerkin SpGEMMs, diagonals
     total_s: float = 0.0
     spgemm_s: list = field(default_factory=list)
+    coarsen_s: list = field(default_factory=list)  # strength + PMIS + interpolation per level
 
 
 class SegmentNorms:
@@ -247,25 +248,33 @@ class GeometricAMG:
             P = MBSRMatrix.from_csr(PI, PJ, PA, n, nc)
             R = P.transpose()
             del PI, PJ, PA
-            for M in (P, R):
-                M.set_level(lvl)
-                M.set_precision(self.spmv_policy(lvl))
-            # Galerkin product in the level's SpGEMM precision (follows the left operand)
-            torch.cuda.synchronize()
-            tg = time.perf_counter()
-            A.set_precision(self.spgemm_policy(lvl))
-            AP = A.spgemm(P)
-            R.set_precision(self.spgemm_policy(lvl))
-            Ac = R.spgemm(AP)
-            torch.cuda.synchronize()
-            self.times.spgemm_s.append(time.perf_counter() - tg)
-            A.set_precision(self.spmv_policy(lvl))
-            R.set_precision(self.spmv_policy(lvl))
-            del AP
+            Ac = self._galerkin(lvl, A, P, R)
             L_.P, L_.R = P, R
             A, n = Ac, nc
             grid = cgrid
             lvl += 1
+        self._finish(t0)
+
+    def _galerkin(self, lvl, A, P, R):
+        """R (A P) in the level's SpGEMM precision (it follows the left
+        operand); the operands return to the level's SpMV precision."""
+        for M in (P, R):
+            M.set_level(lvl)
+            M.set_precision(self.spmv_policy(lvl))
+        torch.cuda.synchronize()
+        tg = time.perf_counter()
+        A.set_precision(self.spgemm_policy(lvl))
+        AP = A.spgemm(P)
+        R.set_precision(self.spgemm_policy(lvl))
+        Ac = R.spgemm(AP)
+        torch.cuda.synchronize()
+        self.times.spgemm_s.append(time.perf_counter() - tg)
+        A.set_precision(self.spmv_policy(lvl))
+        R.set_precision(self.spmv_policy(lvl))
+        return Ac
+
+    def _finish(self, t0):
+        """Coarsest level's dense inverse, work vectors, setup times."""
         # coarsest: dense inverse as an mBSR matrix with full blocks, applied in FP64
         torch.cuda.synchronize()
         th = time.perf_counter()
@@ -393,6 +402,106 @@ class GeometricAMG:
                  "spmv_precision": self.spmv_policy(i),
                  "spgemm_precision": self.spgemm_policy(i) if L_.P is not None else None}
                 for i, L_ in enumerate(self.levels)]
+
+
+class BoomerAMG(GeometricAMG):
+    """Algebraic Galerkin hierarchy built on the device the way BoomerAMG builds
+    it on one process (par_amg_setup.c:994-3182): per level the strength matrix
+    (theta, max row sum), PMIS coarsening (CF_init 0: coarsen type 8, 2: type
+    9), extended+i interpolation truncated to max_elmts entries per row, R =
+    P^T (mBSR transpose) and R (A P) on the mBSR SpGEMM.  Coarsening stops at
+    n <= coarse_size, at max_levels, or when a level would not shrink (no C
+    point or all C points); the coarsest level is solved with its dense inverse.
+    The smoother is l1-Jacobi (relax type 18; on one process the l1 norm of a
+    row is |a_ii|) with weight omega; the V-cycle and PCG are the same as for
+    the geometric hierarchy."""
+
+    def __init__(self, I, J, V, n, device, theta: float = 0.25, max_row_sum: float = 0.9,
+                 cf_init: int = 0, max_elmts: int = 4, omega: float = 1.0, coarse_size: int = 9,
+                 spmv_policy=fp64_policy, spgemm_policy=fp64_policy, max_levels: int = 25):
+        self.device = device
+        self.omega = float(omega)
+        self.coarse_size = int(coarse_size)
+        self.spmv_policy = spmv_policy
+        self.spgemm_policy = spgemm_policy
+        self.max_levels = max_levels
+        self.theta, self.max_row_sum = float(theta), float(max_row_sum)
+        self.cf_init, self.max_elmts = int(cf_init), int(max_elmts)
+        self.levels: list[Level] = []
+        self.coarse_inv: MBSRMatrix | None = None
+        self.times = SetupTimes()
+        self._graph = None
+        self.batch = False
+        self.cf: list[torch.Tensor] = []
+        self._setup_algebraic(I, J, V, int(n))
+
+    def _setup_algebraic(self, I, J, V, n):
+        from . import coarsen as C
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        A = MBSRMatrix.from_csr(I, J, V, n, n)
+        CI, CJ, CV = I, J, V
+        lvl = 0
+        while True:
+            A.set_level(lvl)
+            A.set_precision(self.spmv_policy(lvl))
+            dinv = A.diagonal(inverse_abs=True)
+            L_ = Level(n=n, grid=[], A=A, dinv=dinv)
+            self.levels.append(L_)
+            if n <= self.coarse_size or lvl + 1 >= self.max_levels:
+                break
+            torch.cuda.synchronize()
+            tc = time.perf_counter()
+            SI, SJ = C.strength(n, CI, CJ, CV, self.theta, self.max_row_sum)
+            cf, nc = C.pmis(n, SI, SJ, self.cf_init)
+            if nc == 0 or nc == n:
+                break
+            PI, PJ, PA, nc = C.extpi_interp(n, CI, CJ, CV, SI, SJ, cf, self.max_elmts)
+            del SI, SJ
+            torch.cuda.synchronize()
+            self.times.coarsen_s.append(time.perf_counter() - tc)
+            self.cf.append(cf)
+            P = MBSRMatrix.from_csr(PI, PJ, PA, n, nc)
+            R = P.transpose()
+            del PI, PJ, PA
+            Ac = self._galerkin(lvl, A, P, R)
+            L_.P, L_.R = P, R
+            A, n = Ac, nc
+            CI, CJ, CV = A.to_csr()
+            lvl += 1
+        self._finish(t0)
+
+    def complexities(self):
+        """(grid complexity, operator complexity) as BoomerAMG reports them."""
+        ns = [L_.n for L_ in self.levels]
+        nnz = [L_.A.info()["nnz"] for L_ in self.levels]
+        return sum(ns) / ns[0], sum(nnz) / nnz[0]
+
+
+def solve_poisson_boomer(nx: int, ny: int, nz: int, b: np.ndarray, device, mixed: bool = False,
+                         tol: float = 1e-8, capture: bool = True, max_elmts: int = 4,
+                         coarse_size: int = 512, theta: float = 0.25, max_row_sum: float = 0.9,
+                         omega: float = 1.0):
+    """7-pt Poisson with the algebraic hierarchy (device strength, PMIS,
+    extended+i): setup + PCG/V(1,1) solve; timings as solve_poisson."""
+    I, J, V = W.stencil(7, nx, ny, nz)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
+    dI, dJ, dV = t(I), t(J), t(V)
+    db = t(b)
+    torch.cuda.synchronize()
+    amg = BoomerAMG(dI, dJ, dV, nx * ny * nz, device, theta=theta, max_row_sum=max_row_sum,
+                    cf_init=0, max_elmts=max_elmts, omega=omega, coarse_size=coarse_size,
+                    spmv_policy=mixed_spmv_policy if mixed else fp64_policy,
+                    spgemm_policy=mixed_spgemm_policy if mixed else fp64_policy)
+    tc = time.perf_counter()
+    if capture:
+        amg.capture()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, iters, hist = amg.pcg(db, tol=tol)
+    torch.cuda.synchronize()
+    return {"amg": amg, "x": x, "iters": iters, "hist": hist, "setup": amg.times,
+            "capture_s": t0 - tc, "solve_s": time.perf_counter() - t0}
 
 
 def solve_poisson(nx: int, ny: int, nz: int, b: np.ndarray, device, mixed: bool = False,

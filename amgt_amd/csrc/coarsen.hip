@@ -1,0 +1,678 @@
+// BoomerAMG setup on the device, one process: strength of connection, PMIS
+// coarsening and extended+i interpolation (with max-elements truncation) on
+// CSR operators.
+//
+// Counterparts: hypre_BoomerAMGCreateSDevice (par_strength_device.c:34),
+// hypre_BoomerAMGCoarsenPMISDevice (par_coarsen_device.c:30),
+// hypre_BoomerAMGBuildExtPIInterpDevice (par_lr_interp_device.c:991) and
+// hypre_BoomerAMGInterpTruncationDevice (par_interp_trunc_device.c:731).  The
+// results follow HYPRE's host algorithms on one process (par_strength.c:75,
+// par_coarsen.c:2101, par_lr_interp.c:1025, par_csr_matrix.c:2866 with
+// qsort.c:395), which the reference's own regression outputs pin, so the
+// device hierarchy equals the CPU one entry for entry:
+//  * S: strong columns in row order, no diagonal;
+//  * PMIS measures |S^T row| + hypre_Rand() from seed 2747 (the Park-Miller
+//    stream jumped to each row's draw), the independent set of each pass taken
+//    exactly as the host loop leaves it;
+//  * P rows in the host's discovery order, weights accumulated in the host's
+//    order with contraction off (no fused multiply-add), truncated rows
+//    ordered by the host's quicksort.
+// The diagonal of a row is found by column (HYPRE keeps it first; exports of
+// the mBSR handles have ascending columns).
+#include "internal.hpp"
+
+#include <vector>
+
+namespace amgt {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+inline unsigned grid_of(int64_t n) { return unsigned((n + kThreads - 1) / kThreads); }
+
+__device__ __forceinline__ int64_t diag_pos(const int32_t *__restrict__ I,
+                                            const int32_t *__restrict__ J, int i) {
+    for (int64_t p = I[i]; p < I[i + 1]; ++p)
+        if (J[p] == i) return p;
+    return -1;
+}
+
+// ---- strength (par_strength.c:75-528, one function) ---------------------------
+#pragma clang fp contract(off)
+template <bool kFill>
+__global__ __launch_bounds__(kThreads) void k_strength(int32_t n, const int32_t *__restrict__ I,
+                                                      const int32_t *__restrict__ J,
+                                                      const double *__restrict__ A, double theta,
+                                                      double max_row_sum, int32_t *__restrict__ cnt,
+                                                      const int32_t *__restrict__ SI,
+                                                      int32_t *__restrict__ SJ) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const int64_t d = diag_pos(I, J, i);
+    const double diag = d >= 0 ? A[d] : 0.0;
+    double row_scale = 0.0, row_sum = diag;
+    for (int64_t p = I[i]; p < I[i + 1]; ++p) {
+        if (p == d) continue;
+        const double a = A[p];
+        if (diag < 0) row_scale = a > row_scale ? a : row_scale;
+        else row_scale = a < row_scale ? a : row_scale;
+        row_sum += a;
+    }
+    int c = 0;
+    int64_t out = kFill ? SI[i] : 0;
+    // a row too dominant by the row-sum test keeps only weak dependencies
+    if (!(fabs(row_sum) > fabs(diag) * max_row_sum && max_row_sum < 1.0)) {
+        const double thr = theta * row_scale;
+        for (int64_t p = I[i]; p < I[i + 1]; ++p) {
+            if (p == d) continue;
+            const double a = A[p];
+            const bool weak = diag < 0 ? (a <= thr) : (a >= thr);
+            if (!weak) {
+                if (kFill) SJ[out++] = J[p];
+                ++c;
+            }
+        }
+    }
+    if (!kFill) cnt[i] = c;
+}
+
+// ---- PMIS (par_coarsen.c:2101-2813 on one process) ------------------------------
+__global__ __launch_bounds__(kThreads) void k_col_count(int32_t n, const int32_t *__restrict__ SI,
+                                                       const int32_t *__restrict__ SJ,
+                                                       int32_t *__restrict__ cnt) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    for (int p = SI[i]; p < SI[i + 1]; ++p) atomicAdd(&cnt[SJ[p]], 1);
+}
+
+// x_{k} = a^k x_0 mod m of the Park-Miller generator (random.c:43-101); all
+// products stay below 2^62
+__device__ __forceinline__ uint64_t pm_jump(uint64_t x0, uint64_t k) {
+    constexpr uint64_t kA = 16807u, kM = 2147483647u;
+    uint64_t r = 1u, b = kA;
+    while (k) {
+        if (k & 1u) r = (r * b) % kM;
+        b = (b * b) % kM;
+        k >>= 1;
+    }
+    return (r * x0) % kM;
+}
+
+// measures and the initial graph (par_coarsen.c:2230-2400, IndepSetInit with
+// seed 2747); a row with no strong dependence is marked -3 and left out
+__global__ __launch_bounds__(kThreads) void k_pmis_init(int32_t n, const int32_t *__restrict__ SI,
+                                                       const int32_t *__restrict__ cnt,
+                                                       double *__restrict__ measure,
+                                                       int32_t *__restrict__ cf,
+                                                       int32_t *__restrict__ any) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const double r = double(pm_jump(2747u, uint64_t(i) + 1u)) / 2147483647.0;
+    const double m = double(cnt[i]) + r;
+    if (SI[i + 1] == SI[i]) {
+        cf[i] = -3;
+        measure[i] = 0.0;
+    } else {
+        cf[i] = 0;
+        measure[i] = m;
+        *any = 1;
+    }
+}
+
+// independent set, first half (par_coarsen.c:2510-2556): a node of the graph
+// with measure > 1 loses against any S neighbour (either direction) with a
+// larger measure > 1; the host's clearing stores are idempotent, so plain
+// stores of 1 reproduce them
+__global__ __launch_bounds__(kThreads) void k_pmis_clear(int32_t n, const int32_t *__restrict__ SI,
+                                                        const int32_t *__restrict__ SJ,
+                                                        const double *__restrict__ measure,
+                                                        int32_t *__restrict__ lost) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const double mi = measure[i];
+    if (!(mi > 1)) return;
+    for (int p = SI[i]; p < SI[i + 1]; ++p) {
+        const int j = SJ[p];
+        const double mj = measure[j];
+        if (mj > 1) {
+            if (mi > mj) lost[j] = 1;
+            else if (mj > mi) lost[i] = 1;
+        }
+    }
+}
+
+// graph nodes are the ones with a positive measure (decided and isolated nodes
+// have measure 0; a random draw is > 0)
+__global__ __launch_bounds__(kThreads) void k_pmis_set(int32_t n, const double *__restrict__ measure,
+                                                      const int32_t *__restrict__ lost,
+                                                      int32_t *__restrict__ cf) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    if (measure[i] > 1 && !lost[i]) cf[i] = 1;
+}
+
+// C and F points (par_coarsen.c:2583-2666): a graph node of measure < 1 is F;
+// a node outside the set that depends on a C point is F
+__global__ __launch_bounds__(kThreads) void k_pmis_cf(int32_t n, const int32_t *__restrict__ SI,
+                                                     const int32_t *__restrict__ SJ,
+                                                     const double *__restrict__ measure,
+                                                     int32_t *__restrict__ cf) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const double mi = measure[i];
+    if (!(mi > 0)) return;
+    if (mi < 1) cf[i] = -1;
+    if (cf[i] > 0) return;
+    for (int p = SI[i]; p < SI[i + 1]; ++p)
+        if (cf[SJ[p]] > 0) {
+            cf[i] = -1;
+            break;
+        }
+}
+
+// subgraph update (par_coarsen.c:2690-2760)
+__global__ __launch_bounds__(kThreads) void k_pmis_update(int32_t n, double *__restrict__ measure,
+                                                         const int32_t *__restrict__ cf,
+                                                         int32_t *__restrict__ any) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    if (!(measure[i] > 0)) return;
+    if (cf[i] != 0) measure[i] = 0.0;
+    else *any = 1;
+}
+
+__global__ __launch_bounds__(kThreads) void k_is_coarse(int32_t n, const int32_t *__restrict__ cf,
+                                                       int32_t *__restrict__ flag) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) flag[i] = cf[i] >= 0 ? 1 : 0;
+}
+
+// ---- extended+i interpolation (par_lr_interp.c:1025-1928, one process) -----------
+// upper bound of a row's entries: 1 for C, 0 for isolated, else the strong C
+// neighbours plus the strong dependencies of the strong F neighbours
+__global__ __launch_bounds__(kThreads) void k_extpi_ub(int32_t n, const int32_t *__restrict__ SI,
+                                                      const int32_t *__restrict__ SJ,
+                                                      const int32_t *__restrict__ cf,
+                                                      int64_t *__restrict__ ub) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    int64_t u = 0;
+    if (cf[i] >= 0) {
+        u = 1;
+    } else if (cf[i] != -3) {
+        for (int p = SI[i]; p < SI[i + 1]; ++p) {
+            const int i1 = SJ[p];
+            if (cf[i1] >= 0) u += 1;
+            else if (cf[i1] != -3) u += SI[i1 + 1] - SI[i1];
+        }
+    }
+    ub[i] = u;
+}
+
+// Per-row open-addressing table (linear probing, power-of-two size, at least
+// twice the entries it can receive): fine column -> position in the row (>= 0)
+// or kStrongF for a strong F neighbour; keys start at -1.  It stands for the
+// host's P_marker array, so every membership test is O(1).
+constexpr int32_t kStrongF = -2;
+
+__device__ __forceinline__ uint32_t h_slot(int k, uint32_t mask) {
+    return (uint32_t(k) * 2654435761u) & mask;
+}
+
+__device__ __forceinline__ int h_find(const int32_t *__restrict__ keys,
+                                      const int32_t *__restrict__ vals, uint32_t mask, int k) {
+    for (uint32_t h = h_slot(k, mask);; h = (h + 1) & mask) {
+        const int32_t kk = keys[h];
+        if (kk == k) return vals[h];
+        if (kk == -1) return -1;
+    }
+}
+
+__device__ __forceinline__ void h_put(int32_t *__restrict__ keys, int32_t *__restrict__ vals,
+                                      uint32_t mask, int k, int v) {
+    for (uint32_t h = h_slot(k, mask);; h = (h + 1) & mask) {
+        const int32_t kk = keys[h];
+        if (kk == -1 || kk == k) {
+            keys[h] = k;
+            vals[h] = v;
+            return;
+        }
+    }
+}
+
+// table size of a row: 0 for C and isolated rows, else the power of two at or
+// above 2 * (upper bound of its entries + its strong dependencies)
+__global__ __launch_bounds__(kThreads) void k_extpi_hsize(int32_t n, const int32_t *__restrict__ SI,
+                                                         const int32_t *__restrict__ cf,
+                                                         const int64_t *__restrict__ ub,
+                                                         int64_t *__restrict__ hs) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    int64_t z = 0;
+    if (cf[i] < 0 && cf[i] != -3) {
+        const int64_t need = 2 * (ub[i] + (SI[i + 1] - SI[i]));
+        z = 1;
+        while (z < need) z <<= 1;
+    }
+    hs[i] = z;
+}
+
+// qsort.c:395-417 (descending |w|) with an explicit stack in place of the
+// recursion.  Each partition step is the host's; the two subranges it leaves
+// are disjoint and sorted independently, so the order in which they are
+// sorted does not change the result: the smaller one goes first, which keeps
+// the stack within log2(length) entries.
+__device__ void qsort2_abs(int32_t *v, double *w, int left, int right) {
+    int stack[2 * 40];
+    int top = 0;
+    stack[top++] = left;
+    stack[top++] = right;
+    while (top > 0) {
+        const int r = stack[--top];
+        const int l = stack[--top];
+        if (l >= r) continue;
+        const int mid = (l + r) / 2;
+        {
+            const int32_t tv = v[l];
+            v[l] = v[mid];
+            v[mid] = tv;
+            const double tw = w[l];
+            w[l] = w[mid];
+            w[mid] = tw;
+        }
+        int last = l;
+        for (int i = l + 1; i <= r; ++i) {
+            if (fabs(w[i]) > fabs(w[l])) {
+                ++last;
+                const int32_t tv = v[last];
+                v[last] = v[i];
+                v[i] = tv;
+                const double tw = w[last];
+                w[last] = w[i];
+                w[i] = tw;
+            }
+        }
+        {
+            const int32_t tv = v[l];
+            v[l] = v[last];
+            v[last] = tv;
+            const double tw = w[l];
+            w[l] = w[last];
+            w[last] = tw;
+        }
+        // larger subrange pushed first, smaller one popped next
+        if (last - 1 - l > r - (last + 1)) {
+            stack[top++] = l;
+            stack[top++] = last - 1;
+            stack[top++] = last + 1;
+            stack[top++] = r;
+        } else {
+            stack[top++] = last + 1;
+            stack[top++] = r;
+            stack[top++] = l;
+            stack[top++] = last - 1;
+        }
+    }
+}
+
+// One thread per row: the row's columns (fine indices) and weights in the
+// host's discovery order into scratch at off[i]; truncated in place; cnt[i] =
+// the row's final length.
+__global__ __launch_bounds__(kThreads) void k_extpi_row(
+    int32_t n, const int32_t *__restrict__ I, const int32_t *__restrict__ J,
+    const double *__restrict__ A, const int32_t *__restrict__ SI, const int32_t *__restrict__ SJ,
+    const int32_t *__restrict__ cf, const int64_t *__restrict__ off, int max_elmts,
+    const int64_t *__restrict__ hoff, int32_t *__restrict__ hkeys, int32_t *__restrict__ hvals,
+    int32_t *__restrict__ sj, double *__restrict__ sw, int32_t *__restrict__ cnt) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    int32_t *cols = sj + off[i];
+    double *w = sw + off[i];
+    int32_t *keys = hkeys + hoff[i];
+    int32_t *vals = hvals + hoff[i];
+    const uint32_t mask = uint32_t(hoff[i + 1] - hoff[i]) - 1u;
+    if (cf[i] >= 0) {  // C point: identity (par_lr_interp.c:1488-1495)
+        cols[0] = i;
+        w[0] = 1.0;
+        cnt[i] = 1;
+        return;
+    }
+    if (cf[i] == -3) {
+        cnt[i] = 0;
+        return;
+    }
+    // structure (:1501-1560)
+    int c = 0;
+    for (int p = SI[i]; p < SI[i + 1]; ++p) {
+        const int i1 = SJ[p];
+        if (cf[i1] >= 0) {
+            if (h_find(keys, vals, mask, i1) < 0) {
+                h_put(keys, vals, mask, i1, c);
+                cols[c++] = i1;
+            }
+        } else if (cf[i1] != -3) {
+            h_put(keys, vals, mask, i1, kStrongF);
+            for (int q = SI[i1]; q < SI[i1 + 1]; ++q) {
+                const int k1 = SJ[q];
+                if (cf[k1] >= 0 && h_find(keys, vals, mask, k1) < 0) {
+                    h_put(keys, vals, mask, k1, c);
+                    cols[c++] = k1;
+                }
+            }
+        }
+    }
+    for (int q = 0; q < c; ++q) w[q] = 0.0;
+    // weights (:1630-1708)
+    const int64_t di = diag_pos(I, J, i);
+    double diagonal = di >= 0 ? A[di] : 0.0;
+    for (int64_t jj = I[i]; jj < I[i + 1]; ++jj) {
+        if (jj == di) continue;
+        const int i1 = J[jj];
+        const int pos = h_find(keys, vals, mask, i1);
+        if (pos >= 0) {
+            w[pos] += A[jj];
+            continue;
+        }
+        if (pos == kStrongF) {
+            const int64_t d1 = diag_pos(I, J, i1);
+            const int sgn = (d1 >= 0 && A[d1] < 0) ? -1 : 1;
+            double sum = 0.0;
+            for (int64_t q = I[i1]; q < I[i1 + 1]; ++q) {
+                if (q == d1) continue;
+                const int i2 = J[q];
+                if ((i2 == i || h_find(keys, vals, mask, i2) >= 0) && (sgn * A[q]) < 0)
+                    sum += A[q];
+            }
+            if (sum != 0) {
+                const double distribute = A[jj] / sum;
+                for (int64_t q = I[i1]; q < I[i1 + 1]; ++q) {
+                    if (q == d1) continue;
+                    const int i2 = J[q];
+                    if ((sgn * A[q]) < 0) {
+                        const int p2 = h_find(keys, vals, mask, i2);
+                        if (p2 >= 0) w[p2] += distribute * A[q];
+                        if (i2 == i) diagonal += distribute * A[q];
+                    }
+                }
+            } else {
+                diagonal += A[jj];
+            }
+        } else if (cf[i1] != -3) {
+            diagonal += A[jj];
+        }
+    }
+    if (diagonal != 0)
+        for (int q = 0; q < c; ++q) w[q] /= -diagonal;
+    // truncation (par_csr_matrix.c:3130-3215: tol 0, rescale)
+    if (max_elmts > 0 && c > max_elmts) {
+        double row_sum = 0.0;
+        for (int q = 0; q < c; ++q) row_sum += w[q];
+        qsort2_abs(cols, w, 0, c - 1);
+        double scale = 0.0;
+        for (int q = 0; q < max_elmts; ++q) scale += w[q];
+        if (scale != 0. && scale != row_sum) {
+            scale = row_sum / scale;
+            for (int q = 0; q < max_elmts; ++q) w[q] *= scale;
+        }
+        c = max_elmts;
+    }
+    cnt[i] = c;
+}
+#pragma clang fp contract(fast)
+
+__global__ __launch_bounds__(kThreads) void k_extpi_out(
+    int32_t n, const int64_t *__restrict__ off, const int32_t *__restrict__ PI,
+    const int32_t *__restrict__ sj, const double *__restrict__ sw, const int32_t *__restrict__ f2c,
+    int32_t *__restrict__ PJ, double *__restrict__ PA) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const int64_t s = off[i];
+    const int len = PI[i + 1] - PI[i];
+    for (int q = 0; q < len; ++q) {
+        PJ[PI[i] + q] = f2c[sj[s + q]];
+        PA[PI[i] + q] = sw[s + q];
+    }
+}
+
+struct Tmp {
+    hipStream_t s;
+    std::vector<void *> ptrs;
+    explicit Tmp(hipStream_t st) : s(st) {}
+    ~Tmp() {
+        for (void *p : ptrs) tmp_free(p, s);
+    }
+    template <typename T>
+    int get(T **p, size_t n) {
+        const int st = tmp_alloc_t(p, n, s);
+        if (st == AMGT_OK) ptrs.push_back(*p);
+        return st;
+    }
+};
+
+}  // namespace
+
+}  // namespace amgt
+
+using namespace amgt;
+
+extern "C" {
+
+int amgt_amg_strength(int32_t n, const int32_t *A_i, const int32_t *A_j, const double *A_data,
+                      double theta, double max_row_sum, int64_t *nnzS, int32_t **S_i,
+                      int32_t **S_j, void *stream) {
+    AMGT_CHECK_ARG(nnzS && S_i && S_j, "NULL output argument");
+    AMGT_CHECK_ARG(n >= 0, "negative size");
+    // column and value arrays may be NULL when the matrix has no entries
+    AMGT_CHECK_ARG(n == 0 || A_i, "NULL row pointer");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    *nnzS = 0;
+    *S_i = nullptr;
+    *S_j = nullptr;
+    Tmp tmp(s);
+    int32_t *cnt = nullptr;
+    AMGT_TRY(tmp.get(&cnt, size_t(n) + 1));
+    int32_t *si = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(n) + 1), s));
+    if (!si) {
+        set_error("allocation of S row pointer failed");
+        return AMGT_ERR_ALLOC;
+    }
+    if (n > 0) {
+        hipLaunchKernelGGL((k_strength<false>), dim3(grid_of(n)), dim3(kThreads), 0, s, n, A_i, A_j,
+                           A_data, theta, max_row_sum, cnt, nullptr, nullptr);
+        AMGT_HIP(hipGetLastError());
+    }
+    int st = exclusive_scan_i32(cnt, si, n, s);
+    if (st != AMGT_OK) {
+        dev_free(si, s);
+        return st;
+    }
+    int32_t total = 0;
+    AMGT_HIP(hipMemcpyAsync(&total, si + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    AMGT_HIP(hipStreamSynchronize(s));
+    int32_t *sjj = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(total) + 1), s));
+    if (!sjj) {
+        dev_free(si, s);
+        set_error("allocation of S columns failed");
+        return AMGT_ERR_ALLOC;
+    }
+    if (n > 0) {
+        hipLaunchKernelGGL((k_strength<true>), dim3(grid_of(n)), dim3(kThreads), 0, s, n, A_i, A_j,
+                           A_data, theta, max_row_sum, nullptr, si, sjj);
+        AMGT_HIP(hipGetLastError());
+    }
+    *nnzS = total;
+    *S_i = si;
+    *S_j = sjj;
+    return AMGT_OK;
+}
+
+int amgt_amg_pmis(int32_t n, const int32_t *S_i, const int32_t *S_j, int cf_init,
+                  int32_t *cf_marker, int32_t *n_coarse, void *stream) {
+    AMGT_CHECK_ARG(n_coarse, "NULL output argument");
+    AMGT_CHECK_ARG(n >= 0, "negative size");
+    AMGT_CHECK_ARG(n == 0 || (S_i && cf_marker), "NULL row pointer or marker array");
+    if (cf_init != 0 && cf_init != 2) {
+        set_error("PMIS: only CF_init 0 and 2 (coarsen types 8 and 9) are supported");
+        return AMGT_ERR_UNSUPPORTED;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    *n_coarse = 0;
+    if (n == 0) return AMGT_OK;
+    Tmp tmp(s);
+    int32_t *cnt = nullptr, *lost = nullptr, *any = nullptr, *flag = nullptr, *scan = nullptr;
+    double *measure = nullptr;
+    AMGT_TRY(tmp.get(&cnt, n));
+    AMGT_TRY(tmp.get(&lost, n));
+    AMGT_TRY(tmp.get(&any, 1));
+    AMGT_TRY(tmp.get(&measure, n));
+    AMGT_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * size_t(n), s));
+    AMGT_HIP(hipMemsetAsync(any, 0, sizeof(int32_t), s));
+    const dim3 g(grid_of(n)), b(kThreads);
+    hipLaunchKernelGGL(k_col_count, g, b, 0, s, n, S_i, S_j, cnt);
+    hipLaunchKernelGGL(k_pmis_init, g, b, 0, s, n, S_i, cnt, measure, cf_marker, any);
+    AMGT_HIP(hipGetLastError());
+    int32_t h_any = 0;
+    AMGT_HIP(hipMemcpyAsync(&h_any, any, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    AMGT_HIP(hipStreamSynchronize(s));
+    // every pass decides at least the nodes of largest measure, so the loop
+    // ends within n passes
+    for (int iter = 0; h_any && iter <= n; ++iter) {
+        if (!cf_init || iter) {
+            AMGT_HIP(hipMemsetAsync(lost, 0, sizeof(int32_t) * size_t(n), s));
+            hipLaunchKernelGGL(k_pmis_clear, g, b, 0, s, n, S_i, S_j, measure, lost);
+            hipLaunchKernelGGL(k_pmis_set, g, b, 0, s, n, measure, lost, cf_marker);
+        }
+        hipLaunchKernelGGL(k_pmis_cf, g, b, 0, s, n, S_i, S_j, measure, cf_marker);
+        AMGT_HIP(hipMemsetAsync(any, 0, sizeof(int32_t), s));
+        hipLaunchKernelGGL(k_pmis_update, g, b, 0, s, n, measure, cf_marker, any);
+        AMGT_HIP(hipGetLastError());
+        AMGT_HIP(hipMemcpyAsync(&h_any, any, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        AMGT_HIP(hipStreamSynchronize(s));
+    }
+    if (h_any) {
+        set_error("PMIS did not converge");
+        return AMGT_ERR_INPUT;
+    }
+    AMGT_TRY(tmp.get(&flag, n));
+    AMGT_TRY(tmp.get(&scan, size_t(n) + 1));
+    hipLaunchKernelGGL(k_is_coarse, g, b, 0, s, n, cf_marker, flag);
+    AMGT_HIP(hipGetLastError());
+    AMGT_TRY(exclusive_scan_i32(flag, scan, n, s));
+    AMGT_HIP(hipMemcpyAsync(n_coarse, scan + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    AMGT_HIP(hipStreamSynchronize(s));
+    return AMGT_OK;
+}
+
+int amgt_amg_extpi_interp(int32_t n, const int32_t *A_i, const int32_t *A_j,
+                          const double *A_data, const int32_t *S_i, const int32_t *S_j,
+                          const int32_t *cf_marker, int max_elmts, int32_t *n_coarse,
+                          int64_t *nnzP, int32_t **P_i, int32_t **P_j, double **P_data,
+                          void *stream) {
+    AMGT_CHECK_ARG(n_coarse && nnzP && P_i && P_j && P_data, "NULL output argument");
+    AMGT_CHECK_ARG(n >= 0 && max_elmts >= 0, "negative size");
+    AMGT_CHECK_ARG(n == 0 || (A_i && S_i && cf_marker), "NULL row pointer or marker array");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    *n_coarse = 0;
+    *nnzP = 0;
+    *P_i = nullptr;
+    *P_j = nullptr;
+    *P_data = nullptr;
+    Tmp tmp(s);
+    int64_t *ub = nullptr, *off = nullptr;
+    int32_t *flag = nullptr, *f2c = nullptr, *cnt = nullptr, *sj = nullptr;
+    double *sw = nullptr;
+    const dim3 g(grid_of(n)), b(kThreads);
+    AMGT_TRY(tmp.get(&ub, size_t(n) + 1));
+    AMGT_TRY(tmp.get(&off, size_t(n) + 1));
+    AMGT_TRY(tmp.get(&flag, size_t(n) + 1));
+    AMGT_TRY(tmp.get(&f2c, size_t(n) + 1));
+    AMGT_TRY(tmp.get(&cnt, size_t(n) + 1));
+    int32_t *pi = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(n) + 1), s));
+    if (!pi) {
+        set_error("allocation of P row pointer failed");
+        return AMGT_ERR_ALLOC;
+    }
+    int st = AMGT_OK;
+    int64_t total_ub = 0;
+    int32_t nc = 0, nnz = 0;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_is_coarse, g, b, 0, s, n, cf_marker, flag);
+        hipLaunchKernelGGL(k_extpi_ub, g, b, 0, s, n, S_i, S_j, cf_marker, ub);
+        if (hipGetLastError() != hipSuccess) st = AMGT_ERR_HIP;
+    }
+    if (st == AMGT_OK) st = exclusive_scan_i32(flag, f2c, n, s);
+    if (st == AMGT_OK) st = exclusive_scan_i64(ub, off, n, s);
+    if (st == AMGT_OK) {
+        if (hipMemcpyAsync(&total_ub, off + n, sizeof(int64_t), hipMemcpyDeviceToHost, s) !=
+                hipSuccess ||
+            hipMemcpyAsync(&nc, f2c + n, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            st = AMGT_ERR_HIP;
+    }
+    if (st == AMGT_OK) st = tmp.get(&sj, size_t(total_ub));
+    if (st == AMGT_OK) st = tmp.get(&sw, size_t(total_ub));
+    // per-row tables
+    int64_t *hs = nullptr, *hoff = nullptr;
+    int32_t *hkeys = nullptr, *hvals = nullptr;
+    int64_t total_h = 0;
+    if (st == AMGT_OK) st = tmp.get(&hs, size_t(n) + 1);
+    if (st == AMGT_OK) st = tmp.get(&hoff, size_t(n) + 1);
+    if (st == AMGT_OK && n > 0) {
+        hipLaunchKernelGGL(k_extpi_hsize, g, b, 0, s, n, S_i, cf_marker, ub, hs);
+        if (hipGetLastError() != hipSuccess) st = AMGT_ERR_HIP;
+    }
+    if (st == AMGT_OK) st = exclusive_scan_i64(hs, hoff, n, s);
+    if (st == AMGT_OK) {
+        if (hipMemcpyAsync(&total_h, hoff + n, sizeof(int64_t), hipMemcpyDeviceToHost, s) !=
+                hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            st = AMGT_ERR_HIP;
+    }
+    if (st == AMGT_OK) st = tmp.get(&hkeys, size_t(total_h));
+    if (st == AMGT_OK) st = tmp.get(&hvals, size_t(total_h));
+    if (st == AMGT_OK && total_h > 0 &&
+        hipMemsetAsync(hkeys, 0xff, sizeof(int32_t) * size_t(total_h), s) != hipSuccess)
+        st = AMGT_ERR_HIP;
+    if (st == AMGT_OK && n > 0) {
+        hipLaunchKernelGGL(k_extpi_row, g, b, 0, s, n, A_i, A_j, A_data, S_i, S_j, cf_marker, off,
+                           max_elmts, hoff, hkeys, hvals, sj, sw, cnt);
+        if (hipGetLastError() != hipSuccess) st = AMGT_ERR_HIP;
+    }
+    if (st == AMGT_OK) st = exclusive_scan_i32(cnt, pi, n, s);
+    if (st == AMGT_OK) {
+        if (hipMemcpyAsync(&nnz, pi + n, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
+                hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            st = AMGT_ERR_HIP;
+    }
+    int32_t *pj = nullptr;
+    double *pa = nullptr;
+    if (st == AMGT_OK) {
+        pj = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(nnz) + 1), s));
+        pa = static_cast<double *>(dev_alloc(sizeof(double) * (size_t(nnz) + 1), s));
+        if (!pj || !pa) {
+            set_error("allocation of P failed");
+            st = AMGT_ERR_ALLOC;
+        }
+    }
+    if (st == AMGT_OK && n > 0) {
+        hipLaunchKernelGGL(k_extpi_out, g, b, 0, s, n, off, pi, sj, sw, f2c, pj, pa);
+        if (hipGetLastError() != hipSuccess) st = AMGT_ERR_HIP;
+    }
+    if (st != AMGT_OK) {
+        if (st == AMGT_ERR_HIP) set_error("extended+i interpolation: kernel launch or copy failed");
+        dev_free(pi, s);
+        if (pj) dev_free(pj, s);
+        if (pa) dev_free(pa, s);
+        return st;
+    }
+    *n_coarse = nc;
+    *nnzP = nnz;
+    *P_i = pi;
+    *P_j = pj;
+    *P_data = pa;
+    return AMGT_OK;
+}
+
+}  // extern "C"

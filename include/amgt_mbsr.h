@@ -205,6 +205,30 @@ int amgt_csr_matvec(int trans, double alpha, int32_t nrows, int32_t ncols, int64
                     amgt_mbsr_t *A_cache, const double *d_x, double beta, const double *d_b,
                     double *d_y, void *stream);
 
+/* ---- BoomerAMG setup on the device, one process (CSR in, device pointers;
+ *      results equal HYPRE's host algorithms: par_strength.c:75,
+ *      par_coarsen.c:2101, par_lr_interp.c:1025, par_csr_matrix.c:2866).
+ *      The diagonal of a row is found by column.  Outputs allocated by the
+ *      library are freed with amgt_free. ---- */
+/* hypre_BoomerAMGCreateSDevice, par_strength_device.c:34 (num_functions 1):
+ * S_i[n+1], S_j[nnzS] strong columns in row order, no diagonal. */
+int amgt_amg_strength(int32_t n, const int32_t *A_i, const int32_t *A_j, const double *A_data,
+                      double theta, double max_row_sum, int64_t *nnzS, int32_t **S_i,
+                      int32_t **S_j, void *stream);
+/* hypre_BoomerAMGCoarsenPMISDevice, par_coarsen_device.c:30 (CF_init 0 or 2,
+ * coarsen types 8/9): cf_marker[n] (caller's device array) = 1 C, -1 F,
+ * -3 no strong dependence; measures |S^T row| + hypre_Rand() from seed 2747. */
+int amgt_amg_pmis(int32_t n, const int32_t *S_i, const int32_t *S_j, int cf_init,
+                  int32_t *cf_marker, int32_t *n_coarse, void *stream);
+/* hypre_BoomerAMGBuildExtPIInterpDevice, par_lr_interp_device.c:991, with
+ * hypre_BoomerAMGInterpTruncationDevice (max_elmts > 0, trunc_factor 0):
+ * P (n x n_coarse) in CSR, row entries in HYPRE's host order. */
+int amgt_amg_extpi_interp(int32_t n, const int32_t *A_i, const int32_t *A_j,
+                          const double *A_data, const int32_t *S_i, const int32_t *S_j,
+                          const int32_t *cf_marker, int max_elmts, int32_t *n_coarse,
+                          int64_t *nnzP, int32_t **P_i, int32_t **P_j, double **P_data,
+                          void *stream);
+
 #ifdef __cplusplus
 }
 #endif
