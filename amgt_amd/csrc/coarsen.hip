@@ -28,6 +28,12 @@ namespace amgt {
 namespace {
 
 constexpr int kThreads = 256;
+// levels with few rows (<= kWaveMaxRows) that hold at least kWaveMinRowLen
+// entries per row on average use the wave kernel of extended+i for their F
+// rows (one wave per row holds ~38 KB of LDS: worth it only when the thread
+// kernel would leave the chip idle behind a few long serial rows)
+constexpr int kWaveMinRowLen = 12;
+constexpr int kWaveMaxRows = 1 << 18;
 
 inline unsigned grid_of(int64_t n) { return unsigned((n + kThreads - 1) / kThreads); }
 
@@ -324,9 +330,11 @@ __global__ __launch_bounds__(kThreads) void k_extpi_row(
     const double *__restrict__ A, const int32_t *__restrict__ SI, const int32_t *__restrict__ SJ,
     const int32_t *__restrict__ cf, const int64_t *__restrict__ off, int max_elmts,
     const int64_t *__restrict__ hoff, int32_t *__restrict__ hkeys, int32_t *__restrict__ hvals,
-    int32_t *__restrict__ sj, double *__restrict__ sw, int32_t *__restrict__ cnt) {
+    const int32_t *__restrict__ skip, int32_t *__restrict__ sj, double *__restrict__ sw,
+    int32_t *__restrict__ cnt) {
     const int i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
+    if (skip && skip[i]) return;  // a row of the wave kernel
     int32_t *cols = sj + off[i];
     double *w = sw + off[i];
     int32_t *keys = hkeys + hoff[i];
@@ -419,7 +427,313 @@ __global__ __launch_bounds__(kThreads) void k_extpi_row(
     }
     cnt[i] = c;
 }
+
+// ---- extended+i, one wave per row (long rows of coarse levels) ------------------
+// The same results as k_extpi_row for rows whose columns are ascending (every
+// level exported from an mBSR handle): the host's order of discovery and of
+// accumulation is kept, the independent parts run across the lanes.
+//  * discovery: the candidate columns in the host's order (lane per strong
+//    dependency, offsets by a scan), first occurrences found with an LDS
+//    table (atomicMin of the candidate index), positions by a prefix count;
+//  * per entry jj of row i (lane per entry): its kind, and for a strong F
+//    neighbour i1 the sum over row i1 in row order, the distribution factor
+//    and its term for the diagonal;
+//  * per P column p (lane per column): w[p] folded over jj in row order, the
+//    entry (i1, p) found by binary search in the sorted row i1;
+//  * the diagonal, the truncation sort and rescale on lane 0, in order.
+constexpr int kWA = 256;   // entries of row i (A and S)
+constexpr int kWT = 768;   // distinct P columns of the row kept in LDS
+constexpr int kWH = 2048;  // table slots (>= 2 * (kWT + kWA) + 64)
+
+struct WaveRow {
+    int32_t acol[kWA];
+    double aval[kWA];
+    int32_t scol[kWA];
+    int32_t hkey[kWH];
+    int32_t hval[kWH];
+    int32_t pcol[kWT];
+    double pw[kWT];
+    double jdist[kWA];
+    double jdiag[kWA];
+    int32_t jinfo[kWA];  // >= 0: P position; -1 skip/diag; -2 strong F (distribute);
+                         // -3 add a_ij to the diagonal
+    int32_t jsgn[kWA];
+    int32_t sc[3];
+};
+
+__device__ __forceinline__ int64_t find_in_row(const int32_t *__restrict__ J, int64_t b, int64_t e,
+                                               int col) {
+    while (b < e) {
+        const int64_t m = (b + e) >> 1;
+        const int v = J[m];
+        if (v == col) return m;
+        if (v < col) b = m + 1;
+        else e = m;
+    }
+    return -1;
+}
+
+__device__ __forceinline__ int lds_find(const int32_t *hkey, const int32_t *hval, int k) {
+    for (uint32_t h = h_slot(k, kWH - 1);; h = (h + 1) & (kWH - 1)) {
+        const int32_t kk = hkey[h];
+        if (kk == k) return hval[h];
+        if (kk == -1) return -1;
+    }
+}
+
+// slot of key k (inserted if absent) in the wave's table
+__device__ __forceinline__ uint32_t lds_slot(int32_t *hkey, int k) {
+    for (uint32_t h = h_slot(k, kWH - 1);; h = (h + 1) & (kWH - 1)) {
+        const int32_t prev = atomicCAS(&hkey[h], -1, k);
+        if (prev == -1 || prev == k) return h;
+    }
+}
+
+// exclusive prefix sum over the wave's 64 lanes; *tot = total
+__device__ __forceinline__ int wave_excl(int v, int *tot) {
+    const int lane = threadIdx.x & 63;
+    int incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(incl, d);
+        if (lane >= d) incl += t;
+    }
+    *tot = __shfl(incl, 63);
+    return incl - v;
+}
+
+__global__ __launch_bounds__(64) void k_extpi_wave(
+    int32_t count, const int32_t *__restrict__ rows, const int32_t *__restrict__ I,
+    const int32_t *__restrict__ J, const double *__restrict__ A, const int32_t *__restrict__ SI,
+    const int32_t *__restrict__ SJ, const int32_t *__restrict__ cf,
+    const int64_t *__restrict__ off, int max_elmts, int32_t *__restrict__ sj,
+    double *__restrict__ sw, int32_t *__restrict__ cnt, int32_t *__restrict__ use) {
+    __shared__ WaveRow s;
+    if (blockIdx.x >= unsigned(count)) return;
+    const int lane = threadIdx.x;
+    const int i = rows[blockIdx.x];
+    int32_t *cand = sj + off[i];  // the row's candidates, then (by the fallback) its entries
+    const int a0 = I[i], la = I[i + 1] - a0;
+    const int s0 = SI[i], ls = SI[i + 1] - s0;
+    for (int t = lane; t < la; t += 64) {
+        s.acol[t] = J[a0 + t];
+        s.aval[t] = A[a0 + t];
+    }
+    for (int t = lane; t < ls; t += 64) s.scol[t] = SJ[s0 + t];
+    for (int t = lane; t < kWH; t += 64) {
+        s.hkey[t] = -1;
+        s.hval[t] = INT_MAX;
+    }
+    if (lane == 0) s.sc[2] = 0;  // distinct candidate columns
+    __syncthreads();
+    // candidates in the host's order (par_lr_interp.c:1501-1560)
+    int T = 0;
+    for (int b0 = 0; b0 < ls; b0 += 64) {
+        const int p = b0 + lane;
+        int i1 = 0, k = 0;
+        if (p < ls) {
+            i1 = s.scol[p];
+            if (cf[i1] >= 0) {
+                k = 1;
+            } else if (cf[i1] != -3) {
+                for (int q = SI[i1]; q < SI[i1 + 1]; ++q) k += cf[SJ[q]] >= 0;
+            }
+        }
+        int tot;
+        int o = T + wave_excl(k, &tot);
+        if (p < ls && k > 0) {
+            if (cf[i1] >= 0) {
+                cand[o] = i1;
+            } else {
+                for (int q = SI[i1]; q < SI[i1 + 1]; ++q) {
+                    const int k1 = SJ[q];
+                    if (cf[k1] >= 0) cand[o++] = k1;
+                }
+            }
+        }
+        T += tot;
+    }
+    __threadfence_block();
+    __syncthreads();
+    // first occurrences; a row with more distinct columns than kWT is left to
+    // the thread kernel (the insertions stop early enough that the table never
+    // fills: at most kWT + 64 keys before every lane sees the count)
+    for (int t = lane; t < T; t += 64) {
+        if (s.sc[2] > kWT) break;
+        const int col = cand[t];
+        uint32_t h = h_slot(col, kWH - 1);
+        for (;; h = (h + 1) & (kWH - 1)) {
+            const int32_t prev = atomicCAS(&s.hkey[h], -1, col);
+            if (prev == -1) {
+                atomicAdd(&s.sc[2], 1);
+                break;
+            }
+            if (prev == col) break;
+        }
+        atomicMin(&s.hval[h], t);
+    }
+    __syncthreads();
+    if (s.sc[2] > kWT) {
+        if (lane == 0) use[i] = 0;
+        return;
+    }
+    int c = 0;
+    for (int b0 = 0; b0 < T; b0 += 64) {
+        const int t = b0 + lane;
+        bool first = false;
+        int col = 0;
+        if (t < T) {
+            col = cand[t];
+            first = lds_find(s.hkey, s.hval, col) == t;
+        }
+        int tot;
+        const int pos = c + wave_excl(first ? 1 : 0, &tot);
+        if (first) s.pcol[pos] = col;
+        c += tot;
+    }
+    __syncthreads();
+    // table: column -> position, strong F neighbours -> kStrongF
+    for (int p = lane; p < c; p += 64) s.hval[lds_slot(s.hkey, s.pcol[p])] = p;
+    for (int p = lane; p < ls; p += 64) {
+        const int i1 = s.scol[p];
+        if (cf[i1] < 0 && cf[i1] != -3) s.hval[lds_slot(s.hkey, i1)] = kStrongF;
+    }
+    __syncthreads();
+    // per entry of row i
+    for (int jj = lane; jj < la; jj += 64) {
+        const int i1 = s.acol[jj];
+        int info = -1, sgn = 1;
+        double dist = 0.0, dterm = 0.0;
+        if (i1 != i) {
+            const int v = lds_find(s.hkey, s.hval, i1);
+            if (v >= 0) {
+                info = v;
+            } else if (v == kStrongF) {
+                const int64_t b1 = I[i1], e1 = I[i1 + 1];
+                const int64_t d1 = find_in_row(J, b1, e1, i1);
+                sgn = (d1 >= 0 && A[d1] < 0) ? -1 : 1;
+                double sum = 0.0;
+                for (int64_t q = b1; q < e1; ++q) {
+                    if (q == d1) continue;
+                    const int i2 = J[q];
+                    if ((i2 == i || lds_find(s.hkey, s.hval, i2) >= 0) && (sgn * A[q]) < 0)
+                        sum += A[q];
+                }
+                if (sum != 0) {
+                    info = -2;
+                    dist = s.aval[jj] / sum;
+                    const int64_t qi = find_in_row(J, b1, e1, i);
+                    if (qi >= 0 && (sgn * A[qi]) < 0) dterm = dist * A[qi];
+                } else {
+                    info = -3;
+                }
+            } else if (cf[i1] != -3) {
+                info = -3;
+            }
+        }
+        s.jinfo[jj] = info;
+        s.jsgn[jj] = sgn;
+        s.jdist[jj] = dist;
+        s.jdiag[jj] = dterm;
+    }
+    __syncthreads();
+    // weights: term of entry jj for column p, in row order
+    for (int p = lane; p < c; p += 64) {
+        const int col = s.pcol[p];
+        double acc = 0.0;
+        for (int jj = 0; jj < la; ++jj) {
+            const int info = s.jinfo[jj];
+            if (info == p) {
+                acc += s.aval[jj];
+            } else if (info == -2) {
+                const int i1 = s.acol[jj];
+                const int64_t q = find_in_row(J, I[i1], I[i1 + 1], col);
+                if (q >= 0 && (s.jsgn[jj] * A[q]) < 0) acc += s.jdist[jj] * A[q];
+            }
+        }
+        s.pw[p] = acc;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        double diagonal = 0.0;
+        for (int jj = 0; jj < la; ++jj)
+            if (s.acol[jj] == i) {
+                diagonal = s.aval[jj];
+                break;
+            }
+        for (int jj = 0; jj < la; ++jj) {
+            const int info = s.jinfo[jj];
+            if (info == -2) {
+                diagonal += s.jdiag[jj];
+            } else if (info == -3) {
+                diagonal += s.aval[jj];
+            }
+        }
+        s.sc[0] = 0;
+        if (diagonal != 0)
+            for (int q = 0; q < c; ++q) s.pw[q] /= -diagonal;
+        int cc = c;
+        if (max_elmts > 0 && c > max_elmts) {
+            double row_sum = 0.0;
+            for (int q = 0; q < c; ++q) row_sum += s.pw[q];
+            qsort2_abs(s.pcol, s.pw, 0, c - 1);
+            double scale = 0.0;
+            for (int q = 0; q < max_elmts; ++q) scale += s.pw[q];
+            if (scale != 0. && scale != row_sum) {
+                scale = row_sum / scale;
+                for (int q = 0; q < max_elmts; ++q) s.pw[q] *= scale;
+            }
+            cc = max_elmts;
+        }
+        s.sc[1] = cc;
+        cnt[i] = cc;
+    }
+    __syncthreads();
+    const int cc = s.sc[1];
+    for (int q = lane; q < cc; q += 64) {
+        sj[off[i] + q] = s.pcol[q];
+        sw[off[i] + q] = s.pw[q];
+    }
+}
 #pragma clang fp contract(fast)
+
+// rows whose columns are not ascending (the diagonal may sit anywhere)
+__global__ __launch_bounds__(kThreads) void k_rows_unsorted(int32_t n, const int32_t *__restrict__ I,
+                                                           const int32_t *__restrict__ J,
+                                                           int32_t *__restrict__ flag) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    int prev = -1;
+    for (int64_t p = I[i]; p < I[i + 1]; ++p) {
+        const int c = J[p];
+        if (c == i) continue;
+        if (c <= prev) {
+            *flag = 1;
+            return;
+        }
+        prev = c;
+    }
+}
+
+// rows for the wave kernel: F rows within the LDS capacities
+__global__ __launch_bounds__(kThreads) void k_wave_rows(int32_t n, const int32_t *__restrict__ I,
+                                                       const int32_t *__restrict__ SI,
+                                                       const int32_t *__restrict__ cf,
+                                                       const int64_t *__restrict__ ub,
+                                                       int32_t *__restrict__ use) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    use[i] = (cf[i] < 0 && cf[i] != -3 && I[i + 1] - I[i] <= kWA && SI[i + 1] - SI[i] <= kWA)
+                 ? 1
+                 : 0;
+}
+
+__global__ __launch_bounds__(kThreads) void k_list_rows(int32_t n, const int32_t *__restrict__ use,
+                                                       const int32_t *__restrict__ pos,
+                                                       int32_t *__restrict__ list) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < n && use[i]) list[pos[i]] = i;
+}
 
 __global__ __launch_bounds__(kThreads) void k_extpi_out(
     int32_t n, const int64_t *__restrict__ off, const int32_t *__restrict__ PI,
@@ -634,9 +948,59 @@ int amgt_amg_extpi_interp(int32_t n, const int32_t *A_i, const int32_t *A_j,
     if (st == AMGT_OK && total_h > 0 &&
         hipMemsetAsync(hkeys, 0xff, sizeof(int32_t) * size_t(total_h), s) != hipSuccess)
         st = AMGT_ERR_HIP;
+    // long rows with ascending columns (coarse levels): a wave per row
+    int32_t *use = nullptr, *upos = nullptr, *wlist = nullptr;
+    int32_t nwave = 0;
+    if (st == AMGT_OK && n > 0) {
+        int64_t nnzA = 0;
+        int32_t h_unsorted = 0;
+        int32_t *unsorted = nullptr;
+        int32_t lastI = 0;
+        if (hipMemcpyAsync(&lastI, A_i + n, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
+                hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            st = AMGT_ERR_HIP;
+        nnzA = lastI;
+        if (st == AMGT_OK && n <= kWaveMaxRows && nnzA >= int64_t(kWaveMinRowLen) * n) {
+            st = tmp.get(&unsorted, 1);
+            if (st == AMGT_OK &&
+                (hipMemsetAsync(unsorted, 0, sizeof(int32_t), s) != hipSuccess))
+                st = AMGT_ERR_HIP;
+            if (st == AMGT_OK) {
+                hipLaunchKernelGGL(k_rows_unsorted, g, b, 0, s, n, A_i, A_j, unsorted);
+                if (hipGetLastError() != hipSuccess ||
+                    hipMemcpyAsync(&h_unsorted, unsorted, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                   s) != hipSuccess ||
+                    hipStreamSynchronize(s) != hipSuccess)
+                    st = AMGT_ERR_HIP;
+            }
+            if (st == AMGT_OK && !h_unsorted) {
+                st = tmp.get(&use, size_t(n) + 1);
+                if (st == AMGT_OK) st = tmp.get(&upos, size_t(n) + 1);
+                if (st == AMGT_OK) {
+                    hipLaunchKernelGGL(k_wave_rows, g, b, 0, s, n, A_i, S_i, cf_marker, ub, use);
+                    if (hipGetLastError() != hipSuccess) st = AMGT_ERR_HIP;
+                }
+                if (st == AMGT_OK) st = exclusive_scan_i32(use, upos, n, s);
+                if (st == AMGT_OK &&
+                    (hipMemcpyAsync(&nwave, upos + n, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
+                         hipSuccess ||
+                     hipStreamSynchronize(s) != hipSuccess))
+                    st = AMGT_ERR_HIP;
+                if (st == AMGT_OK && nwave > 0) st = tmp.get(&wlist, size_t(nwave));
+                if (st == AMGT_OK && nwave > 0) {
+                    hipLaunchKernelGGL(k_list_rows, g, b, 0, s, n, use, upos, wlist);
+                    hipLaunchKernelGGL(k_extpi_wave, dim3(unsigned(nwave)), dim3(64), 0, s, nwave,
+                                       wlist, A_i, A_j, A_data, S_i, S_j, cf_marker, off, max_elmts,
+                                       sj, sw, cnt, use);
+                    if (hipGetLastError() != hipSuccess) st = AMGT_ERR_HIP;
+                }
+            }
+        }
+    }
     if (st == AMGT_OK && n > 0) {
         hipLaunchKernelGGL(k_extpi_row, g, b, 0, s, n, A_i, A_j, A_data, S_i, S_j, cf_marker, off,
-                           max_elmts, hoff, hkeys, hvals, sj, sw, cnt);
+                           max_elmts, hoff, hkeys, hvals, nwave > 0 ? use : nullptr, sj, sw, cnt);
         if (hipGetLastError() != hipSuccess) st = AMGT_ERR_HIP;
     }
     if (st == AMGT_OK) st = exclusive_scan_i32(cnt, pi, n, s);
