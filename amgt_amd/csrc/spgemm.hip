@@ -34,6 +34,7 @@ namespace {
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr int kBin1Ub = 2048;   // workgroup per row, LDS table of 4096
 constexpr int kNumCap = 64;     // C blocks per row kept in LDS by the numeric kernel
+constexpr int kBigCap = 128;    // ... and by the long-row kernel before its global slab
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -611,13 +612,20 @@ __global__ __launch_bounds__(256) void k_numeric_global(int count, const int32_t
                                                         const int32_t *__restrict__ cvp,
                                                         double *__restrict__ cval, double *slab,
                                                         int64_t slab_elems) {
+    // rows up to kBigCap blocks accumulate in LDS, longer ones in the wave's slab
+    __shared__ double s_acc[4][kBigCap * 16];
+    __shared__ int32_t s_key[4][kBigCap];
     __shared__ NumStage s_st[4];
     const int w = threadIdx.x >> 6;
     const int wave = blockIdx.x * 4 + w;
     const int nwaves = gridDim.x * 4;
     double *acc = slab + int64_t(wave) * slab_elems;
     for (int r = wave; r < count; r += nwaves) {
-        numeric_row<VT>(rows[r], g, boff, cptr, ccol, cmap, cvp, cval, nullptr, acc, s_st[w]);
+        const int i = rows[r];
+        if (cptr[i + 1] - cptr[i] <= kBigCap)
+            numeric_row<VT>(i, g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w], s_st[w]);
+        else
+            numeric_row<VT>(i, g, boff, cptr, ccol, cmap, cvp, cval, nullptr, acc, s_st[w]);
     }
 }
 

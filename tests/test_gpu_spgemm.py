@@ -97,6 +97,17 @@ def test_spgemm_bins_and_big_rows(dev):
     AI, AJ, AA = dense_rows_csr(8, 3000, 400, 9)
     BI, BJ, BA = rand_csr(3000, 5000, 0.004, 10)
     check_product(dev, 8, 3000, 5000, AI, AJ, AA, BI, BJ, BA, "bin2")
+    # C block rows of 65..128 blocks (LDS accumulators of the long-row kernel)
+    # next to rows over 128 (its global slab)
+    AI, AJ, AA = dense_rows_csr(16, 600, 60, 11)
+    BI, BJ, BA = rand_csr(600, 480, 0.05, 12)
+    C = sp.csr_matrix((AA, AJ, AI), shape=(16, 600)) @ sp.csr_matrix((BA, BJ, BI), shape=(600, 480))
+    blocks = {(r // 4, c // 4) for r, c in zip(*C.nonzero())}
+    per_row = np.bincount([b[0] for b in blocks], minlength=4)
+    assert per_row.max() > 64
+    check_product(dev, 16, 600, 480, AI, AJ, AA, BI, BJ, BA, "bin2_lds")
+    BI, BJ, BA = rand_csr(600, 2000, 0.05, 13)
+    check_product(dev, 16, 600, 2000, AI, AJ, AA, BI, BJ, BA, "bin2_slab")
 
 
 def test_spgemm_rap(dev):
