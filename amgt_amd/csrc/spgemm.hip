@@ -34,7 +34,7 @@ namespace {
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr int kBin1Ub = 2048;   // workgroup per row, LDS table of 4096
 constexpr int kNumCap = 64;     // C blocks per row kept in LDS by the numeric kernel
-constexpr int kBigCap = 128;    // ... and by the long-row kernel before its global slab
+constexpr int kBigCap = 256;    // ... and by the long-row kernel before its global slab
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
