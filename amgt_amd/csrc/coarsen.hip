@@ -33,7 +33,8 @@ constexpr int kThreads = 256;
 // rows (one wave per row holds ~38 KB of LDS: worth it only when the thread
 // kernel would leave the chip idle behind a few long serial rows)
 constexpr int kWaveMinRowLen = 12;
-constexpr int kWaveMaxRows = 1 << 18;
+constexpr int kWaveMaxRows = 1 << 18;       // the kernel of rows up to 256 entries
+constexpr int kWaveMaxRowsSmall = 1 << 21;  // the kernel of rows up to 64 entries
 
 inline unsigned grid_of(int64_t n) { return unsigned((n + kThreads - 1) / kThreads); }
 
@@ -441,23 +442,26 @@ __global__ __launch_bounds__(kThreads) void k_extpi_row(
 //  * per P column p (lane per column): w[p] folded over jj in row order, the
 //    entry (i1, p) found by binary search in the sorted row i1;
 //  * the diagonal, the truncation sort and rescale on lane 0, in order.
-constexpr int kWA = 256;   // entries of row i (A and S)
-constexpr int kWT = 768;   // distinct P columns of the row kept in LDS
-constexpr int kWH = 2048;  // table slots (>= 2 * (kWT + kWA) + 64)
-
+// capacities: WA entries of row i (A and S), WT distinct P columns kept in
+// LDS, WH table slots: at most WT + 64 candidate columns (the insertions stop
+// once WT is passed) plus WA strong F neighbours ever enter the table, which
+// stays at most about half full.  Two sets: rows of up to 64 entries (about
+// 15 KB of LDS, many rows per CU) and of up to 256.
+template <int WA, int WT, int WH>
 struct WaveRow {
-    int32_t acol[kWA];
-    double aval[kWA];
-    int32_t scol[kWA];
-    int32_t hkey[kWH];
-    int32_t hval[kWH];
-    int32_t pcol[kWT];
-    double pw[kWT];
-    double jdist[kWA];
-    double jdiag[kWA];
-    int32_t jinfo[kWA];  // >= 0: P position; -1 skip/diag; -2 strong F (distribute);
+    static_assert(2 * (WT + 64 + WA) <= WH + 128 && (WH & (WH - 1)) == 0, "table size");
+    int32_t acol[WA];
+    double aval[WA];
+    int32_t scol[WA];
+    int32_t hkey[WH];
+    int32_t hval[WH];
+    int32_t pcol[WT];
+    double pw[WT];
+    double jdist[WA];
+    double jdiag[WA];
+    int32_t jinfo[WA];  // >= 0: P position; -1 skip/diag; -2 strong F (distribute);
                          // -3 add a_ij to the diagonal
-    int32_t jsgn[kWA];
+    int32_t jsgn[WA];
     int32_t sc[3];
 };
 
@@ -473,8 +477,9 @@ __device__ __forceinline__ int64_t find_in_row(const int32_t *__restrict__ J, in
     return -1;
 }
 
+template <int WH>
 __device__ __forceinline__ int lds_find(const int32_t *hkey, const int32_t *hval, int k) {
-    for (uint32_t h = h_slot(k, kWH - 1);; h = (h + 1) & (kWH - 1)) {
+    for (uint32_t h = h_slot(k, WH - 1);; h = (h + 1) & (WH - 1)) {
         const int32_t kk = hkey[h];
         if (kk == k) return hval[h];
         if (kk == -1) return -1;
@@ -482,8 +487,9 @@ __device__ __forceinline__ int lds_find(const int32_t *hkey, const int32_t *hval
 }
 
 // slot of key k (inserted if absent) in the wave's table
+template <int WH>
 __device__ __forceinline__ uint32_t lds_slot(int32_t *hkey, int k) {
-    for (uint32_t h = h_slot(k, kWH - 1);; h = (h + 1) & (kWH - 1)) {
+    for (uint32_t h = h_slot(k, WH - 1);; h = (h + 1) & (WH - 1)) {
         const int32_t prev = atomicCAS(&hkey[h], -1, k);
         if (prev == -1 || prev == k) return h;
     }
@@ -502,13 +508,14 @@ __device__ __forceinline__ int wave_excl(int v, int *tot) {
     return incl - v;
 }
 
+template <int WA, int WT, int WH>
 __global__ __launch_bounds__(64) void k_extpi_wave(
     int32_t count, const int32_t *__restrict__ rows, const int32_t *__restrict__ I,
     const int32_t *__restrict__ J, const double *__restrict__ A, const int32_t *__restrict__ SI,
     const int32_t *__restrict__ SJ, const int32_t *__restrict__ cf,
     const int64_t *__restrict__ off, int max_elmts, int32_t *__restrict__ sj,
     double *__restrict__ sw, int32_t *__restrict__ cnt, int32_t *__restrict__ use) {
-    __shared__ WaveRow s;
+    __shared__ WaveRow<WA, WT, WH> s;
     if (blockIdx.x >= unsigned(count)) return;
     const int lane = threadIdx.x;
     const int i = rows[blockIdx.x];
@@ -520,7 +527,7 @@ __global__ __launch_bounds__(64) void k_extpi_wave(
         s.aval[t] = A[a0 + t];
     }
     for (int t = lane; t < ls; t += 64) s.scol[t] = SJ[s0 + t];
-    for (int t = lane; t < kWH; t += 64) {
+    for (int t = lane; t < WH; t += 64) {
         s.hkey[t] = -1;
         s.hval[t] = INT_MAX;
     }
@@ -555,14 +562,14 @@ __global__ __launch_bounds__(64) void k_extpi_wave(
     }
     __threadfence_block();
     __syncthreads();
-    // first occurrences; a row with more distinct columns than kWT is left to
+    // first occurrences; a row with more distinct columns than WT is left to
     // the thread kernel (the insertions stop early enough that the table never
-    // fills: at most kWT + 64 keys before every lane sees the count)
+    // fills: at most WT + 64 keys before every lane sees the count)
     for (int t = lane; t < T; t += 64) {
-        if (s.sc[2] > kWT) break;
+        if (s.sc[2] > WT) break;
         const int col = cand[t];
-        uint32_t h = h_slot(col, kWH - 1);
-        for (;; h = (h + 1) & (kWH - 1)) {
+        uint32_t h = h_slot(col, WH - 1);
+        for (;; h = (h + 1) & (WH - 1)) {
             const int32_t prev = atomicCAS(&s.hkey[h], -1, col);
             if (prev == -1) {
                 atomicAdd(&s.sc[2], 1);
@@ -573,7 +580,7 @@ __global__ __launch_bounds__(64) void k_extpi_wave(
         atomicMin(&s.hval[h], t);
     }
     __syncthreads();
-    if (s.sc[2] > kWT) {
+    if (s.sc[2] > WT) {
         if (lane == 0) use[i] = 0;
         return;
     }
@@ -584,7 +591,7 @@ __global__ __launch_bounds__(64) void k_extpi_wave(
         int col = 0;
         if (t < T) {
             col = cand[t];
-            first = lds_find(s.hkey, s.hval, col) == t;
+            first = lds_find<WH>(s.hkey, s.hval, col) == t;
         }
         int tot;
         const int pos = c + wave_excl(first ? 1 : 0, &tot);
@@ -593,10 +600,10 @@ __global__ __launch_bounds__(64) void k_extpi_wave(
     }
     __syncthreads();
     // table: column -> position, strong F neighbours -> kStrongF
-    for (int p = lane; p < c; p += 64) s.hval[lds_slot(s.hkey, s.pcol[p])] = p;
+    for (int p = lane; p < c; p += 64) s.hval[lds_slot<WH>(s.hkey, s.pcol[p])] = p;
     for (int p = lane; p < ls; p += 64) {
         const int i1 = s.scol[p];
-        if (cf[i1] < 0 && cf[i1] != -3) s.hval[lds_slot(s.hkey, i1)] = kStrongF;
+        if (cf[i1] < 0 && cf[i1] != -3) s.hval[lds_slot<WH>(s.hkey, i1)] = kStrongF;
     }
     __syncthreads();
     // per entry of row i
@@ -605,7 +612,7 @@ __global__ __launch_bounds__(64) void k_extpi_wave(
         int info = -1, sgn = 1;
         double dist = 0.0, dterm = 0.0;
         if (i1 != i) {
-            const int v = lds_find(s.hkey, s.hval, i1);
+            const int v = lds_find<WH>(s.hkey, s.hval, i1);
             if (v >= 0) {
                 info = v;
             } else if (v == kStrongF) {
@@ -616,7 +623,7 @@ __global__ __launch_bounds__(64) void k_extpi_wave(
                 for (int64_t q = b1; q < e1; ++q) {
                     if (q == d1) continue;
                     const int i2 = J[q];
-                    if ((i2 == i || lds_find(s.hkey, s.hval, i2) >= 0) && (sgn * A[q]) < 0)
+                    if ((i2 == i || lds_find<WH>(s.hkey, s.hval, i2) >= 0) && (sgn * A[q]) < 0)
                         sum += A[q];
                 }
                 if (sum != 0) {
@@ -715,24 +722,34 @@ __global__ __launch_bounds__(kThreads) void k_rows_unsorted(int32_t n, const int
     }
 }
 
-// rows for the wave kernel: F rows within the LDS capacities
+// rows for the wave kernels: 1 = rows of up to 64 entries, 2 = up to 256
+// (when allowed), 0 = the thread kernel
 __global__ __launch_bounds__(kThreads) void k_wave_rows(int32_t n, const int32_t *__restrict__ I,
                                                        const int32_t *__restrict__ SI,
                                                        const int32_t *__restrict__ cf,
-                                                       const int64_t *__restrict__ ub,
-                                                       int32_t *__restrict__ use) {
+                                                       int allow_big, int32_t *__restrict__ use) {
     const int i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
-    use[i] = (cf[i] < 0 && cf[i] != -3 && I[i + 1] - I[i] <= kWA && SI[i + 1] - SI[i] <= kWA)
-                 ? 1
-                 : 0;
+    const int la = I[i + 1] - I[i], ls = SI[i + 1] - SI[i];
+    int u = 0;
+    if (cf[i] < 0 && cf[i] != -3) {
+        if (la <= 64 && ls <= 64) u = 1;
+        else if (allow_big && la <= 256 && ls <= 256) u = 2;
+    }
+    use[i] = u;
 }
 
-__global__ __launch_bounds__(kThreads) void k_list_rows(int32_t n, const int32_t *__restrict__ use,
+__global__ __launch_bounds__(kThreads) void k_flag_eq(int32_t n, const int32_t *__restrict__ use,
+                                                     int which, int32_t *__restrict__ flag) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) flag[i] = use[i] == which ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kThreads) void k_list_rows(int32_t n, const int32_t *__restrict__ flag,
                                                        const int32_t *__restrict__ pos,
                                                        int32_t *__restrict__ list) {
     const int i = blockIdx.x * kThreads + threadIdx.x;
-    if (i < n && use[i]) list[pos[i]] = i;
+    if (i < n && flag[i]) list[pos[i]] = i;
 }
 
 __global__ __launch_bounds__(kThreads) void k_extpi_out(
@@ -949,7 +966,7 @@ int amgt_amg_extpi_interp(int32_t n, const int32_t *A_i, const int32_t *A_j,
         hipMemsetAsync(hkeys, 0xff, sizeof(int32_t) * size_t(total_h), s) != hipSuccess)
         st = AMGT_ERR_HIP;
     // long rows with ascending columns (coarse levels): a wave per row
-    int32_t *use = nullptr, *upos = nullptr, *wlist = nullptr;
+    int32_t *use = nullptr, *upos = nullptr;
     int32_t nwave = 0;
     if (st == AMGT_OK && n > 0) {
         int64_t nnzA = 0;
@@ -961,7 +978,7 @@ int amgt_amg_extpi_interp(int32_t n, const int32_t *A_i, const int32_t *A_j,
             hipStreamSynchronize(s) != hipSuccess)
             st = AMGT_ERR_HIP;
         nnzA = lastI;
-        if (st == AMGT_OK && n <= kWaveMaxRows && nnzA >= int64_t(kWaveMinRowLen) * n) {
+        if (st == AMGT_OK && n <= kWaveMaxRowsSmall && nnzA >= int64_t(kWaveMinRowLen) * n) {
             st = tmp.get(&unsorted, 1);
             if (st == AMGT_OK &&
                 (hipMemsetAsync(unsorted, 0, sizeof(int32_t), s) != hipSuccess))
@@ -975,25 +992,40 @@ int amgt_amg_extpi_interp(int32_t n, const int32_t *A_i, const int32_t *A_j,
                     st = AMGT_ERR_HIP;
             }
             if (st == AMGT_OK && !h_unsorted) {
+                int32_t *flag2 = nullptr;
                 st = tmp.get(&use, size_t(n) + 1);
                 if (st == AMGT_OK) st = tmp.get(&upos, size_t(n) + 1);
+                if (st == AMGT_OK) st = tmp.get(&flag2, size_t(n) + 1);
                 if (st == AMGT_OK) {
-                    hipLaunchKernelGGL(k_wave_rows, g, b, 0, s, n, A_i, S_i, cf_marker, ub, use);
+                    hipLaunchKernelGGL(k_wave_rows, g, b, 0, s, n, A_i, S_i, cf_marker,
+                                       n <= kWaveMaxRows ? 1 : 0, use);
                     if (hipGetLastError() != hipSuccess) st = AMGT_ERR_HIP;
                 }
-                if (st == AMGT_OK) st = exclusive_scan_i32(use, upos, n, s);
-                if (st == AMGT_OK &&
-                    (hipMemcpyAsync(&nwave, upos + n, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
-                         hipSuccess ||
-                     hipStreamSynchronize(s) != hipSuccess))
-                    st = AMGT_ERR_HIP;
-                if (st == AMGT_OK && nwave > 0) st = tmp.get(&wlist, size_t(nwave));
-                if (st == AMGT_OK && nwave > 0) {
-                    hipLaunchKernelGGL(k_list_rows, g, b, 0, s, n, use, upos, wlist);
-                    hipLaunchKernelGGL(k_extpi_wave, dim3(unsigned(nwave)), dim3(64), 0, s, nwave,
-                                       wlist, A_i, A_j, A_data, S_i, S_j, cf_marker, off, max_elmts,
-                                       sj, sw, cnt, use);
+                for (int which = 1; which <= 2 && st == AMGT_OK; ++which) {
+                    int32_t cntw = 0;
+                    hipLaunchKernelGGL(k_flag_eq, g, b, 0, s, n, use, which, flag2);
                     if (hipGetLastError() != hipSuccess) st = AMGT_ERR_HIP;
+                    if (st == AMGT_OK) st = exclusive_scan_i32(flag2, upos, n, s);
+                    if (st == AMGT_OK &&
+                        (hipMemcpyAsync(&cntw, upos + n, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                        s) != hipSuccess ||
+                         hipStreamSynchronize(s) != hipSuccess))
+                        st = AMGT_ERR_HIP;
+                    if (st != AMGT_OK || cntw == 0) continue;
+                    int32_t *wl = nullptr;
+                    st = tmp.get(&wl, size_t(cntw));
+                    if (st != AMGT_OK) break;
+                    hipLaunchKernelGGL(k_list_rows, g, b, 0, s, n, flag2, upos, wl);
+                    if (which == 1)
+                        hipLaunchKernelGGL((k_extpi_wave<64, 384, 1024>), dim3(unsigned(cntw)),
+                                           dim3(64), 0, s, cntw, wl, A_i, A_j, A_data, S_i, S_j,
+                                           cf_marker, off, max_elmts, sj, sw, cnt, use);
+                    else
+                        hipLaunchKernelGGL((k_extpi_wave<256, 768, 2048>), dim3(unsigned(cntw)),
+                                           dim3(64), 0, s, cntw, wl, A_i, A_j, A_data, S_i, S_j,
+                                           cf_marker, off, max_elmts, sj, sw, cnt, use);
+                    if (hipGetLastError() != hipSuccess) st = AMGT_ERR_HIP;
+                    nwave += cntw;
                 }
             }
         }
