@@ -333,8 +333,11 @@ class GeometricAMG:
         z.copy_(out)
         return z
 
-    def capture(self, warmup: int = 2):
-        """Captures one V-cycle (on the fixed work vectors) in a hipGraph."""
+    def capture(self, warmup: int = 1):
+        """Captures one V-cycle (on the fixed work vectors) in a hipGraph.  The
+        warm-up cycles run first, eagerly, so that everything a first call
+        allocates (low-precision copies, SpMV tile tables) exists before the
+        capture."""
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):
@@ -480,7 +483,7 @@ class BoomerAMG(GeometricAMG):
 
 def solve_poisson_boomer(nx: int, ny: int, nz: int, b: np.ndarray, device, mixed: bool = False,
                          tol: float = 1e-8, capture: bool = True, max_elmts: int = 4,
-                         coarse_size: int = 512, theta: float = 0.25, max_row_sum: float = 0.9,
+                         coarse_size: int = 64, theta: float = 0.25, max_row_sum: float = 0.9,
                          omega: float = 1.0):
     """7-pt Poisson with the algebraic hierarchy (device strength, PMIS,
     extended+i): setup + PCG/V(1,1) solve; timings as solve_poisson."""
@@ -506,7 +509,7 @@ def solve_poisson_boomer(nx: int, ny: int, nz: int, b: np.ndarray, device, mixed
 
 def solve_poisson(nx: int, ny: int, nz: int, b: np.ndarray, device, mixed: bool = False,
                   tol: float = 1e-8, capture: bool = True, omega: float = 6.0 / 7.0,
-                  coarse_size: int = 512):
+                  coarse_size: int = 64):
     """7-pt Poisson on nx x ny x nz: setup + PCG/V(1,1) solve; returns a dict of
     timings and the solution (the timed solve includes the residual checks)."""
     I, J, V = W.stencil(7, nx, ny, nz)

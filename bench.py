@@ -121,7 +121,7 @@ def amg_timings(dev, n1=128):
     FP32/FP16 coarse-level smoothing, FP64 Galerkin products on the two finest
     levels).  b = A x*, x* ~ N(0,1) (Park-Miller seed 1).  Setup is split into
     the device part (P, R = P^T, conversions, Galerkin SpGEMMs, diagonals) and
-    the host part (dense inverse of the coarsest operator, <= 512 rows)."""
+    the host part (dense inverse of the coarsest operator, <= 64 rows)."""
     import numpy as np
     import scipy.sparse as sp
     import torch
@@ -217,7 +217,7 @@ def boomer_timings(dev, n1=128):
                                          out["mixed"]["setup_plus_solve_s"], 3)
     out["hierarchy"] = ("algebraic on the device: strength, PMIS (coarsen type 8), extended+i "
                         "(P max 4), R = P^T, Galerkin R(AP) on the mBSR SpGEMM; dense inverse "
-                        "at <= 512 rows")
+                        "at <= 64 rows")
     return out
 
 
