@@ -241,9 +241,12 @@ void mbsr_release(Mbsr *A, hipStream_t s) {
 }
 
 static int effective_precision(const Mbsr &A) {
-    if (A.precision >= 0) return A.precision;
-    if (!mixed_precision_enabled()) return AMGT_DOUBLE;
-    return amgt_precision_for_level(A.level);
+    int p = AMGT_DOUBLE;
+    if (A.precision >= 0) p = A.precision;
+    else if (mixed_precision_enabled()) p = amgt_precision_for_level(A.level);
+    // values known not to fit FP16 run in FP32 (mbsr_ensure_lowp)
+    if (p == AMGT_HALF && A.half_fits == 0) p = AMGT_FLOAT;
+    return p;
 }
 
 }  // namespace amgt
@@ -396,7 +399,7 @@ int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream) {
     if (m.nblk == 0 || m.pattern_only) return AMGT_OK;
     hipStream_t s = as_stream(stream);
     int prec = effective_precision(m);
-    AMGT_TRY(mbsr_ensure_lowp(&m, prec, s));
+    AMGT_TRY(mbsr_ensure_lowp(&m, &prec, s));
     double *x = nullptr, *y = nullptr;
     AMGT_TRY(tmp_alloc_t(&x, size_t(m.ncols) + 4, s));
     AMGT_TRY(tmp_alloc_t(&y, size_t(m.nrows) + 4, s));
@@ -488,7 +491,7 @@ int amgt_mbsr_spmv(amgt_mbsr_t A, int trans, double alpha, const double *d_x, do
     Mbsr *op = &m;
     if (trans) AMGT_TRY(get_transpose(m, s, &op));
     int prec = effective_precision(m);
-    AMGT_TRY(mbsr_ensure_lowp(op, prec, s));
+    AMGT_TRY(mbsr_ensure_lowp(op, &prec, s));
     AMGT_TRY(spmv_prepare(op, lanes_of(*op), s));
     return spmv(*op, prec, lanes_of(*op), alpha, d_x, beta, d_b, d_y, s);
 }
@@ -509,7 +512,7 @@ int amgt_mbsr_spmm(amgt_mbsr_t A, int trans, int k, double alpha, const double *
     Mbsr *op = &m;
     if (trans) AMGT_TRY(get_transpose(m, s, &op));
     int prec = effective_precision(m);
-    AMGT_TRY(mbsr_ensure_lowp(op, prec, s));
+    AMGT_TRY(mbsr_ensure_lowp(op, &prec, s));
     AMGT_TRY(spmv_prepare(op, lanes_of(*op), s));  // k == 1 runs the SpMV kernels
     return spmm(*op, prec, lanes_of(*op), k, alpha, d_x, ldx, beta, d_b, ldb, d_y, ldy, s);
 }
@@ -527,8 +530,8 @@ int amgt_mbsr_spmm_interleaved(amgt_mbsr_t A, int trans, int k, double alpha, co
     AMGT_CHECK_ARG(!m.pattern_only, "pattern-only matrix has no values");
     Mbsr *op = &m;
     if (trans) AMGT_TRY(get_transpose(m, s, &op));
-    const int prec = effective_precision(m);
-    AMGT_TRY(mbsr_ensure_lowp(op, prec, s));
+    int prec = effective_precision(m);
+    AMGT_TRY(mbsr_ensure_lowp(op, &prec, s));
     if (op->nblk == 0 || alpha == 0.0) {
         // y = beta b over all k*ny entries
         return scale_vector(int32_t(int64_t(ny) * k), beta, d_b, d_y, s);
@@ -570,8 +573,8 @@ int amgt_mbsr_relax_jacobi(amgt_mbsr_t A, double omega, const double *d_dinv, co
     AMGT_CHECK_ARG(m.nrows == 0 || (d_dinv && d_f && d_u && d_u_new), "NULL vector");
     AMGT_CHECK_ARG(d_u != d_u_new || m.nrows == 0, "u_new must not alias u");
     hipStream_t s = as_stream(stream);
-    const int prec = effective_precision(m);
-    AMGT_TRY(mbsr_ensure_lowp(&m, prec, s));
+    int prec = effective_precision(m);
+    AMGT_TRY(mbsr_ensure_lowp(&m, &prec, s));
     AMGT_TRY(spmv_prepare(&m, lanes_of(m), s));
     return relax_jacobi(m, prec, lanes_of(m), omega, d_dinv, d_f, d_u, d_u_new, s);
 }
@@ -586,10 +589,16 @@ int amgt_mbsr_spgemm(amgt_mbsr_t A, amgt_mbsr_t B, void *stream, amgt_mbsr_t *C)
     hipStream_t s = as_stream(stream);
     // operand precision of A's level (get_calculationPrecison(hypre_Level(A)),
     // csr_spgemm_device.c:2592) or A's explicit precision
-    const int prec = effective_precision(A->m);
+    int prec = effective_precision(A->m);
     if (!A->m.pattern_only && !B->m.pattern_only) {
-        AMGT_TRY(mbsr_ensure_lowp(&A->m, prec, s));
-        AMGT_TRY(mbsr_ensure_lowp(&B->m, prec, s));
+        int pa = prec, pb = prec;
+        AMGT_TRY(mbsr_ensure_lowp(&A->m, &pa, s));
+        AMGT_TRY(mbsr_ensure_lowp(&B->m, &pb, s));
+        if (pa != prec || pb != prec) {  // an operand outside the FP16 range: both in FP32
+            prec = AMGT_FLOAT;
+            AMGT_TRY(mbsr_ensure_lowp(&A->m, &prec, s));
+            AMGT_TRY(mbsr_ensure_lowp(&B->m, &prec, s));
+        }
     }
     auto *h = new amgt_mbsr();
     int st = spgemm(A->m, B->m, prec, s, &h->m);

@@ -525,6 +525,7 @@ int mbsr_refresh_values(Mbsr *Ap, const int32_t *rowptr, const int32_t *colidx,
     m.val32 = nullptr;
     dev_free(m.val16, s);
     m.val16 = nullptr;
+    m.half_fits = -1;
     if (m.trans) {
         mbsr_release(m.trans, s);
         delete m.trans;

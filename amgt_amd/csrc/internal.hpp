@@ -69,6 +69,7 @@ struct Mbsr {
     double *val = nullptr;
     float *val32 = nullptr;   // lazily made FP32 copy
     __half *val16 = nullptr;  // lazily made FP16 copy
+    int8_t half_fits = -1;    // every |value| <= 65504 (1), not (0), not checked yet (-1)
     bool pattern_only = false;
     double stand = 0.0, avg_nnz = 0.0;
     int32_t max_row_blocks = 0;
@@ -109,7 +110,10 @@ int csr_to_mbsr(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *rowptr
 int mbsr_refresh_values(Mbsr *A, const int32_t *rowptr, const int32_t *colidx,
                         const double *val, hipStream_t s);
 int mbsr_transpose(const Mbsr &A, hipStream_t s, Mbsr *out);
-int mbsr_ensure_lowp(Mbsr *A, int precision, hipStream_t s);
+// Makes the value copy that *precision needs.  FP16 is used only when every
+// value fits its range (max |a| <= 65504, checked once per value set); else
+// *precision becomes FP32 and the FP32 copy is made.
+int mbsr_ensure_lowp(Mbsr *A, int *precision, hipStream_t s);
 // builds what the handle's SpMV kernel kind needs for this lanes setting (the
 // balanced kernel's tile tables); no-op for the other kinds
 int spmv_prepare(Mbsr *A, int lanes, hipStream_t s);

@@ -1008,6 +1008,29 @@ __global__ void k_cast_f32(int64_t n, const double *__restrict__ in, float *__re
     if (i < n) out[i] = float(in[i]);
 }
 
+// per-workgroup maximum of |in[i]| over a grid-stride range (NaN propagates
+// as a failed "fits" test by the caller)
+__global__ __launch_bounds__(256) void k_absmax_part(int64_t n, const double *__restrict__ in,
+                                                     double *__restrict__ out) {
+    __shared__ double s_m[4];
+    double m = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+        const double v = fabs(in[i]);
+        m = (v > m || v != v) ? v : m;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off);
+        m = (o > m || o != o) ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = s_m[0];
+        for (int w = 1; w < 4; ++w) r = (s_m[w] > r || s_m[w] != s_m[w]) ? s_m[w] : r;
+        out[blockIdx.x] = r;
+    }
+}
+
 __global__ void k_cast_f16(int64_t n, const double *__restrict__ in, __half *__restrict__ out) {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) out[i] = __float2half(float(in[i]));
@@ -1269,10 +1292,31 @@ int spmv_prepare(Mbsr *A, int lanes, hipStream_t s) {
     return AMGT_OK;
 }
 
-int mbsr_ensure_lowp(Mbsr *A, int precision, hipStream_t s) {
-    if (precision == AMGT_DOUBLE || A->nnz == 0 || A->pattern_only) return AMGT_OK;
+int mbsr_ensure_lowp(Mbsr *A, int *prec, hipStream_t s) {
+    if (*prec == AMGT_DOUBLE || A->nnz == 0 || A->pattern_only) return AMGT_OK;
     const int64_t n = A->nnz;
     const unsigned grid = unsigned((n + 255) / 256);
+    if (*prec == AMGT_HALF && A->half_fits < 0) {
+        // max |value| in two stages (per-workgroup partials, one workgroup)
+        const int parts = int(std::min<int64_t>((n + 255) / 256, 1024));
+        double *part = nullptr, h_max = 0.0;
+        AMGT_TRY(tmp_alloc_t(&part, size_t(parts) + 1, s));
+        hipLaunchKernelGGL(k_absmax_part, dim3(unsigned(parts)), dim3(256), 0, s, n, A->val, part);
+        hipLaunchKernelGGL(k_absmax_part, dim3(1), dim3(256), 0, s, int64_t(parts), part,
+                           part + parts);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(&h_max, part + parts, sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        tmp_free(part, s);
+        if (e != hipSuccess) {
+            set_error(std::string("FP16 range check: ") + hipGetErrorString(e));
+            return AMGT_ERR_HIP;
+        }
+        A->half_fits = h_max <= 65504.0 ? 1 : 0;  // NaN compares false: not FP16
+    }
+    if (*prec == AMGT_HALF && !A->half_fits) *prec = AMGT_FLOAT;
+    const int precision = *prec;
     if (precision == AMGT_FLOAT && !A->val32) {
         A->val32 = static_cast<float *>(dev_alloc(sizeof(float) * size_t(n), s));
         if (!A->val32) {

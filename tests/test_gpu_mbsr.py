@@ -219,6 +219,32 @@ def test_spmv_low_precision(dev, prec, tol):
     assert err.max() > 0  # really ran in reduced precision
 
 
+def test_half_range_fallback(dev):
+    """Values beyond the FP16 range (|a| > 65504) run in FP32 instead of
+    overflowing; the reported precision follows; new values re-check."""
+    (m, n), I, J, A = CASES["7pt_aniso"]()
+    big = A * 3.0e5
+    ref = O.csr2mbsr(m, n, I, J, big)
+    M = make(dev, m, n, I, J, big)
+    M.set_precision(L.AMGT_HALF)
+    x = np.random.default_rng(12).standard_normal(n)
+    y = M.spmv(T(x, dev)).cpu().numpy()
+    exp = O.mbsr_spmv(ref, 1.0, x)
+    err = np.abs(y - exp) / row_scale(m, n, I, J, big, x, 0.0, None)
+    assert np.all(np.isfinite(y)) and err.max() <= 2e-6
+    assert M.info()["precision"] == L.AMGT_FLOAT
+    # SpGEMM with an out-of-range operand falls back to FP32 for both
+    C = M.spgemm(M)
+    CI, CJ, CV = C.to_csr()
+    assert np.all(np.isfinite(CV.cpu().numpy()))
+    # values back in range: FP16 again
+    M.update_values(T(I, dev), T(J, dev), T(A, dev))
+    y2 = M.spmv(T(x, dev)).cpu().numpy()
+    assert M.info()["precision"] == L.AMGT_HALF
+    err2 = np.abs(y2 - O.mbsr_spmv(O.csr2mbsr(m, n, I, J, A), 1.0, x)) / row_scale(m, n, I, J, A, x, 0.0, None)
+    assert err2.max() <= 2e-3 and err2.max() > 2e-6
+
+
 @pytest.mark.parametrize("name", ["rand_rect", "27pt_10", "skewed", "rand_tall_empty"])
 @pytest.mark.parametrize("k", [1, 3, 8, 9, 16, 17, 32, 40])
 def test_spmm(dev, name, k):
