@@ -1,4 +1,5 @@
-"""ctypes binding of libamgt_mbsr.so (the C-ABI declared in include/amgt_mbsr.h).
+"""ctypes binding of libamgt_mbsr.so (the C-ABI declared in include/amgt_mbsr.h and
+include/amgt_policy.h).
 
 The library must be built in-tree (``python -m amgt_amd.build``).  There is no
 CPU fallback: every entry point raises when the HIP library is missing.
@@ -31,7 +32,11 @@ AMGT_DOUBLE = 0
 AMGT_FLOAT = 1
 AMGT_HALF = 2
 
-# every symbol include/amgt_mbsr.h declares
+POLICY_FEATURES = 12
+POLICY_OUTPUTS = 4
+POLICY_PARAMS = 5252
+
+# every symbol include/amgt_mbsr.h and include/amgt_policy.h declare
 EXPORTS = (
     "amgt_status_string", "amgt_last_error", "amgt_set_allocator", "amgt_precision_for_level",
     "amgt_set_mixed_precision", "amgt_mbsr_create_from_csr", "amgt_mbsr_destroy",
@@ -43,6 +48,8 @@ EXPORTS = (
     "amgt_mbsr_diagonal", "amgt_mbsr_relax_jacobi", "amgt_mbsr_transpose",
     "amgt_read_ij", "amgt_read_mm", "amgt_host_free", "amgt_mbsr_spmm_interleaved",
     "amgt_amg_strength", "amgt_amg_pmis", "amgt_amg_extpi_interp",
+    "amgt_policy_work_bytes", "amgt_policy_forward", "amgt_policy_loss_grad", "amgt_policy_adam",
+    "amgt_level_stats_work_bytes", "amgt_level_stats",
 )
 
 
@@ -136,6 +143,15 @@ def load():
         "amgt_amg_pmis": (_int, [_i32, _vp, _vp, _int, _vp, ctypes.POINTER(_i32), _vp]),
         "amgt_amg_extpi_interp": (_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _int,
                                          ctypes.POINTER(_i32), _pi64, _pvp, _pvp, _pvp, _vp]),
+        "amgt_policy_work_bytes": (ctypes.c_size_t, [_i64]),
+        "amgt_policy_forward": (_int, [_vp, _vp, _i64, _vp, _vp, ctypes.c_size_t, _vp]),
+        "amgt_policy_loss_grad": (_int, [_vp, _vp, _vp, _vp, _i64, ctypes.c_float, _vp, _vp, _vp,
+                                         ctypes.c_size_t, _vp]),
+        "amgt_level_stats_work_bytes": (ctypes.c_size_t, [_i32]),
+        "amgt_level_stats": (_int, [_i32, _vp, _vp, _vp, ctypes.POINTER(_dbl), _vp, ctypes.c_size_t,
+                                    _vp]),
+        "amgt_policy_adam": (_int, [_vp, _vp, _vp, _vp, _i64, _i64, ctypes.c_float, ctypes.c_float,
+                                    ctypes.c_float, ctypes.c_float, _vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

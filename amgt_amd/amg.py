@@ -417,15 +417,24 @@ class BoomerAMG(GeometricAMG):
     point or all C points); the coarsest level is solved with its dense inverse.
     The smoother is l1-Jacobi (relax type 18; on one process the l1 norm of a
     row is |a_ii|) with weight omega; the V-cycle and PCG are the same as for
-    the geometric hierarchy."""
+    the geometric hierarchy.
+
+    `level_hook(lvl, n_prev, A, CI, CJ, CV) -> (theta, precision)`, when given,
+    is called for every level before it is coarsened and sets that level's
+    strength threshold and SpMV/smoother precision (replacing `theta` and
+    `spmv_policy`); the Galerkin products keep `spgemm_policy`."""
 
     def __init__(self, I, J, V, n, device, theta: float = 0.25, max_row_sum: float = 0.9,
                  cf_init: int = 0, max_elmts: int = 4, omega: float = 1.0, coarse_size: int = 9,
-                 spmv_policy=fp64_policy, spgemm_policy=fp64_policy, max_levels: int = 25):
+                 spmv_policy=fp64_policy, spgemm_policy=fp64_policy, max_levels: int = 25,
+                 level_hook=None):
         self.device = device
         self.omega = float(omega)
         self.coarse_size = int(coarse_size)
-        self.spmv_policy = spmv_policy
+        self.level_hook = level_hook
+        self.level_theta: list[float] = []
+        self.level_prec: list[int] = []
+        self.spmv_policy = (lambda lvl: self.level_prec[lvl]) if level_hook else spmv_policy
         self.spgemm_policy = spgemm_policy
         self.max_levels = max_levels
         self.theta, self.max_row_sum = float(theta), float(max_row_sum)
@@ -445,8 +454,16 @@ class BoomerAMG(GeometricAMG):
         A = MBSRMatrix.from_csr(I, J, V, n, n)
         CI, CJ, CV = I, J, V
         lvl = 0
+        n_prev = n
         while True:
             A.set_level(lvl)
+            if self.level_hook is not None:
+                theta, prec = self.level_hook(lvl, n_prev, A, CI, CJ, CV)
+                self.level_theta.append(float(theta))
+                self.level_prec.append(int(prec))
+            else:
+                theta = self.theta
+                self.level_theta.append(theta)
             A.set_precision(self.spmv_policy(lvl))
             dinv = A.diagonal(inverse_abs=True)
             L_ = Level(n=n, grid=[], A=A, dinv=dinv)
@@ -455,7 +472,7 @@ class BoomerAMG(GeometricAMG):
                 break
             torch.cuda.synchronize()
             tc = time.perf_counter()
-            SI, SJ = C.strength(n, CI, CJ, CV, self.theta, self.max_row_sum)
+            SI, SJ = C.strength(n, CI, CJ, CV, theta, self.max_row_sum)
             cf, nc = C.pmis(n, SI, SJ, self.cf_init)
             if nc == 0 or nc == n:
                 break
@@ -469,7 +486,7 @@ class BoomerAMG(GeometricAMG):
             del PI, PJ, PA
             Ac = self._galerkin(lvl, A, P, R)
             L_.P, L_.R = P, R
-            A, n = Ac, nc
+            A, n_prev, n = Ac, n, nc
             CI, CJ, CV = A.to_csr()
             lvl += 1
         self._finish(t0)

@@ -9,6 +9,7 @@
 #include <string>
 
 #include "../../include/amgt_mbsr.h"
+#include "../../include/amgt_policy.h"
 
 namespace amgt {
 

@@ -179,3 +179,95 @@ def csr_transpose(nrows: int, ncols: int, I, J, A):
     np.cumsum(counts, out=TI[1:])
     assert TI[-1] == nnz
     return TI.astype(np.int32), TJ, TA
+
+
+# ---- Gaussian-random-field diffusion (learned-policy problems) ----------------------------
+# Park-Miller x_k = a^k x_0 mod m is one cycle and a seed is only a position on it, so
+# seeds drawn from one stream would give shifted copies of one stream.  Problem i of a
+# split starts at its own position a^p (p = base + i * stride) and uses at most `stride`
+# draws: training problems at [0, 1500 * 2^18), held-out problems at 2^30 + j * 2^22.
+GRF_SPLITS = {"train": (0, 1 << 18, 1500), "test": (1 << 30, 1 << 22, 50)}
+
+
+def problem_seed(split: str, i: int, side: int) -> int:
+    """Seed of problem i of a split ("train" or "test"); the side^2 problem
+    must fit its stride (2 side^2 + 2 draws)."""
+    base, stride, count = GRF_SPLITS[split]
+    if not 0 <= i < count:
+        raise ValueError(f"{split} problem index {i} outside [0, {count})")
+    if 2 * side * side + 2 > stride:
+        raise ValueError(f"a {side}^2 problem needs more than the {split} stride of draws")
+    return pow(PM_A, base + i * stride, PM_M)
+
+
+def grf(side: int, sigma: float, ell: float, noise: np.ndarray) -> np.ndarray:
+    """Stationary Gaussian random field on the side x side cell centres of the
+    unit square (periodic distance), covariance sigma^2 exp(-|r|^2 / (2 ell^2)),
+    by spectral synthesis: g = ifft2(fft2(w) * sqrt(Lambda)) with w ~ N(0,1)
+    white noise and Lambda = fft2(c) the eigenvalues of the circulant
+    covariance (negative round-off eigenvalues clipped to 0), so that
+    E[g_x g_(x+r)] = c(r) exactly."""
+    d = np.minimum(np.arange(side), side - np.arange(side)) / side
+    r2 = d[:, None] ** 2 + d[None, :] ** 2
+    c = sigma * sigma * np.exp(-r2 / (2.0 * ell * ell))
+    lam = np.maximum(np.fft.rfft2(c).real, 0.0)
+    w = noise.reshape(side, side)
+    return np.fft.irfft2(np.fft.rfft2(w) * np.sqrt(lam), s=(side, side))
+
+
+def fv_diffusion(k: np.ndarray):
+    """5-point finite-volume matrix of -div(k grad u) on the unit square with
+    u = 0 on the boundary, one cell per entry of k (side x side, cell h = 1/side,
+    row index i + side*j for cell (i, j), i along x).  Interior face
+    transmissibility = harmonic mean 2 k1 k2 / (k1 + k2), boundary face 2 k
+    (centre-to-wall distance h/2); the face length h and the 1/h of the gradient
+    cancel.  Rows diagonal first, then ascending columns."""
+    ny, nx = k.shape
+    n = nx * ny
+    idx = np.arange(n).reshape(ny, nx)
+    tx = 2.0 * k[:, :-1] * k[:, 1:] / (k[:, :-1] + k[:, 1:])   # face (i, j)-(i+1, j)
+    ty = 2.0 * k[:-1, :] * k[1:, :] / (k[:-1, :] + k[1:, :])   # face (i, j)-(i, j+1)
+    diag = np.zeros((ny, nx))
+    diag[:, :-1] += tx
+    diag[:, 1:] += tx
+    diag[:-1, :] += ty
+    diag[1:, :] += ty
+    diag[:, 0] += 2.0 * k[:, 0]
+    diag[:, -1] += 2.0 * k[:, -1]
+    diag[0, :] += 2.0 * k[0, :]
+    diag[-1, :] += 2.0 * k[-1, :]
+    # (row, col, value) of the off-diagonals, then one stable sort per row
+    r = np.concatenate([idx[:, :-1].ravel(), idx[:, 1:].ravel(), idx[:-1, :].ravel(), idx[1:, :].ravel()])
+    c = np.concatenate([idx[:, 1:].ravel(), idx[:, :-1].ravel(), idx[1:, :].ravel(), idx[:-1, :].ravel()])
+    v = -np.concatenate([tx.ravel(), tx.ravel(), ty.ravel(), ty.ravel()])
+    order = np.lexsort((c, r))
+    r, c, v = r[order], c[order], v[order]
+    counts = np.bincount(r, minlength=n) + 1
+    I = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(counts, out=I[1:])
+    J = np.empty(I[-1], dtype=np.int32)
+    A = np.empty(I[-1], dtype=np.float64)
+    J[I[:-1]] = np.arange(n)
+    A[I[:-1]] = diag.reshape(-1)
+    rest = np.ones(I[-1], dtype=bool)
+    rest[I[:-1]] = False
+    J[rest] = c
+    A[rest] = v
+    return I.astype(np.int32), J, A
+
+
+def grf_problem(side: int, seed: int):
+    """One learned-policy problem: -div(k grad u) = f on side^2 cells with
+    k = exp(g), g a GRF with sigma ~ U[0.5, 3] and correlation length
+    ell ~ U[0.01, 0.2] of the unit domain, f ~ N(0,1) (cell integrals f h^2).
+    From the problem's seed: the first two Park-Miller draws give sigma and ell,
+    the stream continued gives the field's white noise and then f.  Returns
+    (I, J, A, b, sigma, ell)."""
+    u = pm_stream(seed, 2)
+    sigma = 0.5 + 2.5 * float(u[0]) / PM_M
+    ell = 0.01 + 0.19 * float(u[1]) / PM_M
+    z = rand_normal(int(u[1]), 2 * side * side)
+    g = grf(side, sigma, ell, z[:side * side])
+    I, J, A = fv_diffusion(np.exp(g))
+    b = z[side * side:] / float(side * side)
+    return I, J, A, b, sigma, ell

@@ -941,3 +941,101 @@ int64_t orc_amg_truncate(int n, int *PI, int *PJ, double *PA, int max_elmts)
     PI[n] = (int)out;
     return out;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Per-level policy network: plain loops over the layers, double precision.
+ * Layer l maps h_(l-1) to z_l = W_l h_(l-1) + b_l, h_l = tanh(z_l) for the two
+ * hidden layers and o = z_3 for the output.  Backward by the chain rule:
+ * do = d loss / d o, dW_l = dz_l h_(l-1)^T, db_l = dz_l, and
+ * dz_(l-1) = (W_l^T dz_l) * (1 - h_(l-1)^2). */
+#define POL_IN 12
+#define POL_HID 64
+#define POL_OUT 4
+#define POL_W1 0
+#define POL_B1 (POL_W1 + POL_HID * POL_IN)
+#define POL_W2 (POL_B1 + POL_HID)
+#define POL_B2 (POL_W2 + POL_HID * POL_HID)
+#define POL_W3 (POL_B2 + POL_HID)
+#define POL_B3 (POL_W3 + POL_OUT * POL_HID)
+
+static void pol_layer(const double *W, const double *b, const double *in, int nin, int nout,
+                      int act, double *out)
+{
+    for (int o = 0; o < nout; ++o) {
+        double acc = b[o];
+        for (int i = 0; i < nin; ++i) acc += W[o * nin + i] * in[i];
+        out[o] = act ? tanh(acc) : acc;
+    }
+}
+
+static void pol_sample(const double *p, const double *x, double *h1, double *h2, double *o)
+{
+    pol_layer(p + POL_W1, p + POL_B1, x, POL_IN, POL_HID, 1, h1);
+    pol_layer(p + POL_W2, p + POL_B2, h1, POL_HID, POL_HID, 1, h2);
+    pol_layer(p + POL_W3, p + POL_B3, h2, POL_HID, POL_OUT, 0, o);
+}
+
+void orc_policy_forward(const double *params, const double *x, int64_t n, double *out)
+{
+    double h1[POL_HID], h2[POL_HID];
+    for (int64_t s = 0; s < n; ++s) pol_sample(params, x + s * POL_IN, h1, h2, out + s * POL_OUT);
+}
+
+double orc_policy_loss_grad(const double *params, const double *x, const int *cls,
+                            const double *target, int64_t n, double lambda, double *grad)
+{
+    double h1[POL_HID], h2[POL_HID], o[POL_OUT], dout[POL_OUT], dz2[POL_HID], dz1[POL_HID];
+    double loss = 0.0;
+    memset(grad, 0, sizeof(double) * (POL_B3 + POL_OUT));
+    for (int64_t s = 0; s < n; ++s) {
+        const double *xs = x + s * POL_IN;
+        pol_sample(params, xs, h1, h2, o);
+        /* softmax cross-entropy over the three precision logits */
+        double m = o[0];
+        for (int c = 1; c < 3; ++c) m = o[c] > m ? o[c] : m;
+        double sum = 0.0, e[3];
+        for (int c = 0; c < 3; ++c) { e[c] = exp(o[c] - m); sum += e[c]; }
+        const int y = cls[s];
+        const double r = o[3] - target[s];
+        loss += log(sum) + m - o[y] + lambda * r * r;
+        for (int c = 0; c < 3; ++c) dout[c] = (e[c] / sum - (c == y)) / (double)n;
+        dout[3] = 2.0 * lambda * r / (double)n;
+        /* output layer */
+        for (int k = 0; k < POL_OUT; ++k) {
+            for (int i = 0; i < POL_HID; ++i) grad[POL_W3 + k * POL_HID + i] += dout[k] * h2[i];
+            grad[POL_B3 + k] += dout[k];
+        }
+        for (int i = 0; i < POL_HID; ++i) {
+            double a = 0.0;
+            for (int k = 0; k < POL_OUT; ++k) a += params[POL_W3 + k * POL_HID + i] * dout[k];
+            dz2[i] = a * (1.0 - h2[i] * h2[i]);
+        }
+        /* second hidden layer */
+        for (int k = 0; k < POL_HID; ++k) {
+            for (int i = 0; i < POL_HID; ++i) grad[POL_W2 + k * POL_HID + i] += dz2[k] * h1[i];
+            grad[POL_B2 + k] += dz2[k];
+        }
+        for (int i = 0; i < POL_HID; ++i) {
+            double a = 0.0;
+            for (int k = 0; k < POL_HID; ++k) a += params[POL_W2 + k * POL_HID + i] * dz2[k];
+            dz1[i] = a * (1.0 - h1[i] * h1[i]);
+        }
+        /* first hidden layer */
+        for (int k = 0; k < POL_HID; ++k) {
+            for (int i = 0; i < POL_IN; ++i) grad[POL_W1 + k * POL_IN + i] += dz1[k] * xs[i];
+            grad[POL_B1 + k] += dz1[k];
+        }
+    }
+    return loss / (double)n;
+}
+
+void orc_policy_adam(double *params, const double *grad, double *m, double *v, int64_t np,
+                     int64_t step, double lr, double b1, double b2, double eps)
+{
+    const double bc1 = 1.0 - pow(b1, (double)step), bc2 = 1.0 - pow(b2, (double)step);
+    for (int64_t i = 0; i < np; ++i) {
+        m[i] = b1 * m[i] + (1.0 - b1) * grad[i];
+        v[i] = b2 * v[i] + (1.0 - b2) * grad[i] * grad[i];
+        params[i] -= lr * (m[i] / bc1) / (sqrt(v[i] / bc2) + eps);
+    }
+}

@@ -129,6 +129,20 @@ void orc_amg_extpi_fill(int n, const int *AI, const int *AJ, const double *AA, c
  * row's order from hypre_qsort2_abs qsort.c:395-417); in place, returns nnz */
 int64_t orc_amg_truncate(int n, int *PI, int *PJ, double *PA, int max_elmts);
 
+/* ---- per-level policy network (no counterpart in the reference: it replaces
+ *      the fixed rule of get_calculationPrecison, csr_spgemm_device.c:1847-1862).
+ *      12 -> 64 tanh -> 64 tanh -> 4, parameters W1[64][12] b1 W2[64][64] b2
+ *      W3[4][64] b3 (5252 values), in double. ---- */
+/* out[n][4] */
+void orc_policy_forward(const double *params, const double *x, int64_t n, double *out);
+/* returns the mean over the samples of -log softmax(o[0..2])[cls] +
+ * lambda (o[3] - target)^2 and writes its gradient to grad[5252] */
+double orc_policy_loss_grad(const double *params, const double *x, const int *cls,
+                            const double *target, int64_t n, double lambda, double *grad);
+/* one bias-corrected Adam step (step >= 1), in place */
+void orc_policy_adam(double *params, const double *grad, double *m, double *v, int64_t np,
+                     int64_t step, double lr, double b1, double b2, double eps);
+
 #ifdef __cplusplus
 }
 #endif

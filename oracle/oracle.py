@@ -77,6 +77,10 @@ def lib():
             "orc_amg_extpi_fill": (None, [_i32, _i32p, _i32p, _f64p, _i32p, _i32p, _i32p, _i32p,
                                           _i32p, _f64p]),
             "orc_amg_truncate": (_i64, [_i32, _i32p, _i32p, _f64p, _i32]),
+            "orc_policy_forward": (None, [_f64p, _f64p, _i64, _f64p]),
+            "orc_policy_loss_grad": (_f64, [_f64p, _f64p, _i32p, _f64p, _i64, _f64, _f64p]),
+            "orc_policy_adam": (None, [_f64p, _f64p, _f64p, _f64p, _i64, _i64, _f64, _f64, _f64,
+                                       _f64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -362,3 +366,27 @@ def amg_hierarchy(n, I, J, A, theta=0.25, max_row_sum=1.0, cf_init=0, max_elmts=
         L.update(S=(SI, SJ), cf=cf, P=(PI, PJ, PA), nc=nc)
         levels.append({"n": nc, "I": CI, "J": CJ, "A": CA})
     return levels
+
+
+# ---- per-level policy network ---------------------------------------------------------------
+POLICY_PARAMS = 5252
+
+
+def policy_forward(params, X):
+    X = _cf(X).reshape(-1, 12)
+    out = np.empty((X.shape[0], 4))
+    lib().orc_policy_forward(_cf(params), X, X.shape[0], out)
+    return out
+
+
+def policy_loss_grad(params, X, cls, target, lam):
+    X = _cf(X).reshape(-1, 12)
+    grad = np.empty(POLICY_PARAMS)
+    loss = lib().orc_policy_loss_grad(_cf(params), X, _c32(cls), _cf(target), X.shape[0],
+                                      float(lam), grad)
+    return loss, grad
+
+
+def policy_adam(params, grad, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """In place on params, m, v (float64 arrays)."""
+    lib().orc_policy_adam(params, _cf(grad), m, v, params.size, int(step), lr, b1, b2, eps)

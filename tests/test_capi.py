@@ -1,5 +1,5 @@
-"""C-ABI library: builds, loads, exports every symbol include/amgt_mbsr.h declares,
-and its host-only helpers behave (no device calls here)."""
+"""C-ABI library: builds, loads, exports every symbol include/*.h declares, and
+its host-only helpers behave (no device calls here)."""
 import ctypes
 import os
 import re
@@ -8,11 +8,13 @@ import subprocess
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "amgt_mbsr.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("amgt_mbsr.h", "amgt_policy.h")]
 
 
 def declared_symbols():
-    text = open(HEADER).read()
+    assert sorted(HEADERS) == sorted(os.path.join(ROOT, "include", h)
+                                     for h in os.listdir(os.path.join(ROOT, "include")))
+    text = "\n".join(open(h).read() for h in HEADERS)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = set(re.findall(r"\b(amgt_[a-z0-9_]+)\s*\(", text))
     # typedef'd callback types are not functions

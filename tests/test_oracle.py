@@ -463,3 +463,86 @@ def test_amg_truncate_invariants():
         assert abs(b.sum() - a.sum()) <= 1e-12 * np.abs(a).sum()
         ratio = b / b[0] * np.abs(kept_abs[0]) * np.sign(b[0])
         assert np.allclose(np.sort(np.abs(ratio))[::-1], kept_abs, rtol=1e-12)
+
+
+# ---- policy network ---------------------------------------------------------------------------
+def _policy_params(seed, scale=0.3):
+    return np.random.default_rng(seed).normal(scale=scale, size=O.POLICY_PARAMS)
+
+
+def test_policy_gradient_matches_finite_differences():
+    """d loss / d p from the oracle's backward pass against central differences
+    of its own loss, on parameters of every block (W1 b1 W2 b2 W3 b3)."""
+    rng = np.random.default_rng(1)
+    p = _policy_params(2)
+    X = rng.normal(size=(9, 12))
+    cls = rng.integers(0, 3, 9)
+    tgt = rng.normal(size=9)
+    loss, g = O.policy_loss_grad(p, X, cls, tgt, 0.7)
+    blocks = [(0, 768), (768, 832), (832, 4928), (4928, 4992), (4992, 5248), (5248, 5252)]
+    idx = np.concatenate([rng.choice(np.arange(a, b), min(40, b - a), replace=False)
+                          for a, b in blocks])
+    h = 1e-6
+    for k in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (O.policy_loss_grad(pp, X, cls, tgt, 0.7)[0] - O.policy_loss_grad(pm, X, cls, tgt, 0.7)[0]) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-7 + 1e-5 * abs(fd), (k, fd, g[k])
+
+
+def test_policy_forward_closed_forms():
+    """Hand-set parameters: with W1 = W2 = 0 the output is W3 tanh(b2) + b3; a
+    single path x_j -> h1_0 -> h2_0 -> o_3 gives tanh(tanh(x_j + b) + c) + d,
+    which a transposed W1 (12 x 64 read as 64 x 12) would not select."""
+    p = np.zeros(O.POLICY_PARAMS)
+    b2 = np.linspace(-1, 1, 64)
+    W3 = np.random.default_rng(3).normal(size=(4, 64))
+    b3 = np.array([0.1, -0.2, 0.3, 0.4])
+    p[4928:4992] = b2
+    p[4992:5248] = W3.ravel()
+    p[5248:5252] = b3
+    X = np.random.default_rng(4).normal(size=(3, 12))
+    out = O.policy_forward(p, X)
+    assert np.allclose(out, np.tile(W3 @ np.tanh(b2) + b3, (3, 1)), rtol=1e-14, atol=1e-14)
+    p = np.zeros(O.POLICY_PARAMS)
+    j = 7
+    p[0 * 12 + j] = 1.0          # W1[0][j]
+    p[768 + 0] = 0.25            # b1[0]
+    p[832 + 0 * 64 + 0] = 1.5    # W2[0][0]
+    p[4928 + 0] = -0.1           # b2[0]
+    p[4992 + 3 * 64 + 0] = 2.0   # W3[3][0]
+    p[5248 + 3] = 0.05           # b3[3]
+    out = O.policy_forward(p, X)
+    ref = 2.0 * np.tanh(1.5 * np.tanh(X[:, j] + 0.25) - 0.1) + 0.05
+    assert np.allclose(out[:, 3], ref, rtol=1e-14, atol=1e-14)
+    assert np.all(out[:, :3] == 0.0)
+
+
+def test_policy_loss_closed_form():
+    """Zero weights: the logits are b3[0..2] for every sample, so the loss is
+    log(sum e^b) - b[cls] + lambda (b3[3] - t)^2 averaged."""
+    p = np.zeros(O.POLICY_PARAMS)
+    p[5248:5252] = [1.0, 2.0, 3.0, 0.5]
+    X = np.random.default_rng(5).normal(size=(4, 12))
+    cls = np.array([0, 1, 2, 2])
+    tgt = np.array([0.0, 1.0, 0.5, -1.0])
+    loss, g = O.policy_loss_grad(p, X, cls, tgt, 0.3)
+    lse = np.log(np.exp(1.0) + np.exp(2.0) + np.exp(3.0))
+    ref = np.mean([lse - [1.0, 2.0, 3.0][c] for c in cls]) + 0.3 * np.mean((0.5 - tgt) ** 2)
+    assert abs(loss - ref) < 1e-14
+    # gradient of b3: mean softmax - one-hot, and 2 lambda mean(o3 - t)
+    sm = np.exp([1.0, 2.0, 3.0]) / np.exp([1.0, 2.0, 3.0]).sum()
+    assert np.allclose(g[5248:5251], sm - np.bincount(cls, minlength=3) / 4.0, atol=1e-15)
+    assert abs(g[5251] - 2 * 0.3 * np.mean(0.5 - tgt)) < 1e-15
+
+
+def test_policy_adam_first_step():
+    """From zero moments the first bias-corrected step is lr g / (|g| + eps)."""
+    g = np.array([1e-3, -2.0, 0.0, 5.0])
+    p = np.ones(4)
+    m = np.zeros(4)
+    v = np.zeros(4)
+    O.policy_adam(p, g, m, v, 1, 0.01, eps=1e-8)
+    assert np.allclose(p, 1.0 - 0.01 * g / (np.abs(g) + 1e-8), rtol=1e-12, atol=1e-15)
+    assert np.allclose(m, 0.1 * g) and np.allclose(v, 0.001 * g * g)
