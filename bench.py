@@ -221,6 +221,28 @@ def boomer_timings(dev, n1=128):
     return out
 
 
+def policy_timings(dev, count=5, side=1024):
+    """Learned per-level policy on held-out GRF diffusion problems (side^2
+    cells; the first `count` of the 50 held-out problems): plain FP64 (theta
+    0.25), the fixed mixed schedule and the trained network with its
+    convergence check and FP64 fallback; means per problem."""
+    from amgt_amd import policy as P
+
+    net = P.PolicyNet.load(dev)
+    rows, summary = P.evaluate(net, count, side, dev, log=None)
+    for name in ("fp64", "mixed", "learned"):
+        summary[name] = {k: (round(v, 5) if isinstance(v, float) else v)
+                         for k, v in summary[name].items()}
+    summary["learned"]["check_s"] = round(float(sum(r["learned"]["check_s"] for r in rows) /
+                                                len(rows)), 5)
+    summary["problems"] = count
+    summary["side"] = side
+    summary["network"] = {"train_accuracy": round(net.meta.get("train_accuracy", 0.0), 4),
+                          "train_theta_mae": round(net.meta.get("train_theta_mae", 0.0), 4),
+                          "samples": net.meta.get("samples")}
+    return summary
+
+
 def batch_timings(dev):
     """48 independent 2D 5-pt Poisson problems, sides n ~ U{32..512} (Park-Miller
     seed 4), b_i = A_i x*_i with x* ~ N(0,1) (seed 4): one block-diagonal setup
@@ -439,6 +461,7 @@ def main():
     spmm = spmm_timings(dev)
     batch = None if args.no_amg else batch_timings(dev)
     boomer = None if args.no_amg else boomer_timings(dev)
+    policy = None if args.no_amg else policy_timings(dev)
 
     B = spmv_bytes(info)
     value = B / (ms_per_step * 1e-3) / 1e9
@@ -481,6 +504,8 @@ def main():
         out["batch_48_poisson2d"] = batch
     if boomer is not None:
         out["amg_boomer_7pt128"] = boomer
+    if policy is not None:
+        out["learned_policy_grf1024"] = policy
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(I, J, A, x_h, B)
     print(json.dumps(out), flush=True)

@@ -176,3 +176,27 @@ def test_level_stats_match_numpy(dev, case):
     ref = [dom.sum(), (dom < 1 - 1e-12).sum(), ((A > 0) & ~dmask).sum(), np.abs(A).max(),
            d[pos].min(), np.log10(d[pos]).sum(), (np.log10(d[pos]) ** 2).sum(), pos.sum()]
     assert np.allclose(st, ref, rtol=1e-12, atol=1e-12), (st, ref)
+
+
+def test_trained_policy_on_heldout(dev):
+    """The committed network on three held-out 1024^2 problems against plain
+    FP64 and the fixed mixed schedule: every result reaches a true FP64
+    relative residual below 1e-8, the learned solution is within 1e-6 of the
+    FP64 one (max norm, relative), it needs at most one PCG iteration more per
+    problem on average, and the times are printed (the network's gain or
+    loss: on these problems it trades the 2-4 ms of its convergence check for
+    about 1 ms of FP16 coarse levels)."""
+    import json
+    net = P.PolicyNet.load(dev)
+    assert net.meta["samples"] > 5000 and net.meta["train_accuracy"] > 0.95
+    rows, summary = P.evaluate(net, 3, 1024, dev, log=None)
+    print("[policy] held-out 1024^2:", json.dumps(summary), flush=True)
+    for name in ("fp64", "mixed", "learned"):
+        assert summary[name]["max_true_rel_res"] < 1e-8, name
+    assert summary["learned"]["max_diff_vs_fp64"] < 1e-6
+    assert summary["learned"]["iters"] <= summary["fp64"]["iters"] + 1
+    # the network's thresholds stay in the swept range and the finest level is FP64
+    for r in rows:
+        lr = r["learned"]
+        assert all(P.THETA_RANGE[0] <= t <= P.THETA_RANGE[1] for t in lr["thetas"])
+        assert lr["precisions"][0] == L.AMGT_DOUBLE
