@@ -231,7 +231,7 @@ def policy_timings(dev, count=5, side=1024):
     net = P.PolicyNet.load(dev)
     rows, summary = P.evaluate(net, count, side, dev, log=None)
     for name in ("fp64", "mixed", "learned"):
-        summary[name] = {k: (round(v, 5) if isinstance(v, float) else v)
+        summary[name] = {k: (float(f"{v:.5g}") if isinstance(v, float) else v)
                          for k, v in summary[name].items()}
     summary["learned"]["check_s"] = round(float(sum(r["learned"]["check_s"] for r in rows) /
                                                 len(rows)), 5)
