@@ -244,7 +244,7 @@ static int effective_precision(const Mbsr &A) {
     int p = AMGT_DOUBLE;
     if (A.precision >= 0) p = A.precision;
     else if (mixed_precision_enabled()) p = amgt_precision_for_level(A.level);
-    // values known not to fit FP16 run in FP32 (mbsr_ensure_lowp)
+    // values known to have no FP16 copy (non-finite) run in FP32 (mbsr_ensure_lowp)
     if (p == AMGT_HALF && A.half_fits == 0) p = AMGT_FLOAT;
     return p;
 }
@@ -594,7 +594,7 @@ int amgt_mbsr_spgemm(amgt_mbsr_t A, amgt_mbsr_t B, void *stream, amgt_mbsr_t *C)
         int pa = prec, pb = prec;
         AMGT_TRY(mbsr_ensure_lowp(&A->m, &pa, s));
         AMGT_TRY(mbsr_ensure_lowp(&B->m, &pb, s));
-        if (pa != prec || pb != prec) {  // an operand outside the FP16 range: both in FP32
+        if (pa != prec || pb != prec) {  // an operand without an FP16 copy: both in FP32
             prec = AMGT_FLOAT;
             AMGT_TRY(mbsr_ensure_lowp(&A->m, &prec, s));
             AMGT_TRY(mbsr_ensure_lowp(&B->m, &prec, s));

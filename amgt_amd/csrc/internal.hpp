@@ -70,7 +70,8 @@ struct Mbsr {
     double *val = nullptr;
     float *val32 = nullptr;   // lazily made FP32 copy
     __half *val16 = nullptr;  // lazily made FP16 copy
-    int8_t half_fits = -1;    // every |value| <= 65504 (1), not (0), not checked yet (-1)
+    int8_t half_fits = -1;    // values finite, FP16 copy possible (1), not (0), not checked yet (-1)
+    double half_scale = 1.0;  // val16 = half_scale * val (a power of two: max |val16| near 2^14)
     bool pattern_only = false;
     double stand = 0.0, avg_nnz = 0.0;
     int32_t max_row_blocks = 0;
@@ -111,9 +112,11 @@ int csr_to_mbsr(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *rowptr
 int mbsr_refresh_values(Mbsr *A, const int32_t *rowptr, const int32_t *colidx,
                         const double *val, hipStream_t s);
 int mbsr_transpose(const Mbsr &A, hipStream_t s, Mbsr *out);
-// Makes the value copy that *precision needs.  FP16 is used only when every
-// value fits its range (max |a| <= 65504, checked once per value set); else
-// *precision becomes FP32 and the FP32 copy is made.
+// Makes the value copy that *precision needs.  The FP16 copy holds the values
+// times a power of two (half_scale) that puts max |a| in [2^14, 2^15), so that
+// small values stay out of FP16's subnormal range and large ones fit (max |a|
+// is measured once per value set); every consumer divides by it exactly.  A
+// matrix with non-finite values gets no FP16 copy: *precision becomes FP32.
 int mbsr_ensure_lowp(Mbsr *A, int *precision, hipStream_t s);
 // builds what the handle's SpMV kernel kind needs for this lanes setting (the
 // balanced kernel's tile tables); no-op for the other kinds

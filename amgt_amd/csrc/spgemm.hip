@@ -758,6 +758,11 @@ __global__ __launch_bounds__(256) void k_block_popc(int64_t nblk, const uint16_t
     if (q < nblk) out[q] = __popc(uint32_t(map[q]));
 }
 
+__global__ __launch_bounds__(256) void k_scale_values(int64_t n, double f, double *__restrict__ v) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) v[i] *= f;
+}
+
 // numeric kernels of one value precision; *nslab receives the global slab of
 // the rows over kNumCap blocks (freed by the caller)
 template <typename VT>
@@ -961,6 +966,10 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
                 break;
             case AMGT_HALF:
                 st = launch_numeric<__half>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, &nslab, s);
+                // the FP16 copies hold A and B times half_scale: undo both (powers of two, exact)
+                if (st == AMGT_OK && C.nnz > 0)
+                    hipLaunchKernelGGL(k_scale_values, dim3(unsigned((C.nnz + 255) / 256)), dim3(256),
+                                       0, s, C.nnz, 1.0 / (A.half_scale * B.half_scale), C.val);
                 break;
             default:
                 st = launch_numeric<double>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, &nslab, s);

@@ -68,6 +68,14 @@ struct EpAxpby {  // alpha (A x) + beta b; beta == 0 never reads b
         return v;
     }
 };
+template <typename EP>
+struct EpScaled {  // the inner epilogue of sc (A x): A's values were stored times 1/sc
+    EP inner;
+    double sc;
+    __device__ __forceinline__ double operator()(int row, double ax) const {
+        return inner(row, ax * sc);
+    }
+};
 struct EpJacobi {  // one weighted Jacobi sweep: u + omega dinv (f - A u), with x = u
     double omega;
     const double *dinv, *f, *u;
@@ -1031,9 +1039,10 @@ __global__ __launch_bounds__(256) void k_absmax_part(int64_t n, const double *__
     }
 }
 
-__global__ void k_cast_f16(int64_t n, const double *__restrict__ in, __half *__restrict__ out) {
+__global__ void k_cast_f16(int64_t n, const double *__restrict__ in, double scale,
+                           __half *__restrict__ out) {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = __float2half(float(in[i]));
+    if (i < n) out[i] = __float2half(float(in[i] * scale));
 }
 
 template <int G, typename VT, typename AT, typename EP>
@@ -1151,7 +1160,9 @@ int launch_spmv_prec(const Mbsr &A, int precision, int lanes, const double *x, E
     switch (precision) {
         case AMGT_DOUBLE: return launch_spmv_t<double, double>(A, A.val, lanes, x, ep, y, s);
         case AMGT_FLOAT: return launch_spmv_t<float, float>(A, A.val32, lanes, x, ep, y, s);
-        case AMGT_HALF: return launch_spmv_t<__half, float>(A, A.val16, lanes, x, ep, y, s);
+        case AMGT_HALF:
+            return launch_spmv_t<__half, float>(A, A.val16, lanes, x,
+                                                EpScaled<EP>{ep, 1.0 / A.half_scale}, y, s);
         default:
             set_error("unknown precision");
             return AMGT_ERR_ARG;
@@ -1313,7 +1324,12 @@ int mbsr_ensure_lowp(Mbsr *A, int *prec, hipStream_t s) {
             set_error(std::string("FP16 range check: ") + hipGetErrorString(e));
             return AMGT_ERR_HIP;
         }
-        A->half_fits = h_max <= 65504.0 ? 1 : 0;  // NaN compares false: not FP16
+        A->half_fits = std::isfinite(h_max) ? 1 : 0;
+        // 2^e with max |a| * 2^e in [2^14, 2^15): ilogb gives max |a| = f 2^k, f in [1, 2)
+        const int ex = h_max > 0.0 && A->half_fits
+                           ? std::max(-64, std::min(64, 14 - std::ilogb(h_max)))
+                           : 0;
+        A->half_scale = std::ldexp(1.0, ex);
     }
     if (*prec == AMGT_HALF && !A->half_fits) *prec = AMGT_FLOAT;
     const int precision = *prec;
@@ -1332,7 +1348,8 @@ int mbsr_ensure_lowp(Mbsr *A, int *prec, hipStream_t s) {
             set_error("FP16 value copy allocation failed");
             return AMGT_ERR_ALLOC;
         }
-        hipLaunchKernelGGL(k_cast_f16, dim3(grid), dim3(256), 0, s, n, A->val, A->val16);
+        hipLaunchKernelGGL(k_cast_f16, dim3(grid), dim3(256), 0, s, n, A->val, A->half_scale,
+                           A->val16);
         AMGT_HIP(hipGetLastError());
     }
     return AMGT_OK;
@@ -1377,8 +1394,8 @@ int spmm(const Mbsr &A, int precision, int lanes, int k, double alpha, const dou
             return launch_spmm_t<float, float>(A, A.val32, lanes, k, alpha, x, ldx, beta, b, ldb,
                                                y, ldy, s);
         case AMGT_HALF:
-            return launch_spmm_t<__half, float>(A, A.val16, lanes, k, alpha, x, ldx, beta, b, ldb,
-                                                y, ldy, s);
+            return launch_spmm_t<__half, float>(A, A.val16, lanes, k, alpha / A.half_scale, x,
+                                                ldx, beta, b, ldb, y, ldy, s);
         default:
             set_error("unknown precision");
             return AMGT_ERR_ARG;
@@ -1394,7 +1411,8 @@ int spmm_interleaved(const Mbsr &A, int precision, int k, double alpha, const do
         case AMGT_FLOAT:
             return launch_spmm_il<float>(A, A.val32, k, alpha, x, k, beta, b, k, y, k, s);
         case AMGT_HALF:
-            return launch_spmm_il<__half>(A, A.val16, k, alpha, x, k, beta, b, k, y, k, s);
+            return launch_spmm_il<__half>(A, A.val16, k, alpha / A.half_scale, x, k, beta, b, k,
+                                          y, k, s);
         default:
             set_error("unknown precision");
             return AMGT_ERR_ARG;

@@ -219,25 +219,63 @@ def test_spmv_low_precision(dev, prec, tol):
     assert err.max() > 0  # really ran in reduced precision
 
 
-def test_half_range_fallback(dev):
-    """Values beyond the FP16 range (|a| > 65504) run in FP32 instead of
-    overflowing; the reported precision follows; new values re-check."""
+def half_round(v):
+    """The library's FP16 copy read back: values times 2^e (max |v| 2^e in
+    [2^14, 2^15)), rounded through FP32 to FP16, divided by 2^e."""
+    m = float(np.max(np.abs(v))) if v.size else 0.0
+    if m == 0.0:
+        return v.astype(np.float32).astype(np.float16).astype(np.float64)
+    sc = 2.0 ** (15 - np.frexp(m)[1])
+    return (v * sc).astype(np.float32).astype(np.float16).astype(np.float64) / sc
+
+
+@pytest.mark.parametrize("factor", [3.0e5, 1.0e-9, 1.0])
+def test_half_scaled_copy(dev, factor):
+    """FP16 copies are scaled by a power of two: values far beyond FP16's range
+    (x 3e5, max 1.9e6) or below its smallest subnormal (x 1e-9) run in FP16
+    with FP16's relative precision; the result equals the FP64 product of the
+    scaled-and-rounded values (FP32 accumulation) within 2e-6 of the row
+    scale; the SpGEMM of two scaled copies is rescaled exactly."""
     (m, n), I, J, A = CASES["7pt_aniso"]()
-    big = A * 3.0e5
-    ref = O.csr2mbsr(m, n, I, J, big)
-    M = make(dev, m, n, I, J, big)
+    V = A * factor
+    M = make(dev, m, n, I, J, V)
     M.set_precision(L.AMGT_HALF)
     x = np.random.default_rng(12).standard_normal(n)
     y = M.spmv(T(x, dev)).cpu().numpy()
-    exp = O.mbsr_spmv(ref, 1.0, x)
-    err = np.abs(y - exp) / row_scale(m, n, I, J, big, x, 0.0, None)
-    assert np.all(np.isfinite(y)) and err.max() <= 2e-6
-    assert M.info()["precision"] == L.AMGT_FLOAT
-    # SpGEMM with an out-of-range operand falls back to FP32 for both
+    assert M.info()["precision"] == L.AMGT_HALF
+    rs = row_scale(m, n, I, J, V, x, 0.0, None)
+    exp16 = O.mbsr_spmv(O.csr2mbsr(m, n, I, J, half_round(V)), 1.0, x)
+    assert np.all(np.abs(y - exp16) <= 2e-6 * rs)
+    exp = O.mbsr_spmv(O.csr2mbsr(m, n, I, J, V), 1.0, x)
+    assert np.all(np.abs(y - exp) <= 2e-3 * rs)
+    # Jacobi sweep through the same scaled copy
+    d = np.abs(V[np.asarray(J) == np.repeat(np.arange(m), np.diff(I))])
+    dinv = 1.0 / d
+    f, u = np.random.default_rng(13).standard_normal(m), np.random.default_rng(14).standard_normal(n)
+    un = M.relax_jacobi(T(f, dev), T(u, dev), T(dinv, dev), 0.8).cpu().numpy()
+    ref = u + 0.8 * dinv * (f - O.mbsr_spmv(O.csr2mbsr(m, n, I, J, half_round(V)), 1.0, u))
+    assert np.all(np.abs(un - ref) <= 2e-6 * 0.8 * dinv * (np.abs(f) + row_scale(m, n, I, J, V, u, 0.0, None)) + 1e-300)
+    # SpGEMM with both operands' scaled FP16 copies
     C = M.spgemm(M)
     CI, CJ, CV = C.to_csr()
-    assert np.all(np.isfinite(CV.cpu().numpy()))
-    # values back in range: FP16 again
+    ref = O.mbsr_spgemm(O.csr2mbsr(m, n, I, J, half_round(V)), O.csr2mbsr(m, n, I, J, half_round(V)))
+    RI, RJ, RV = O.mbsr2csr(ref)
+    cv = CV.cpu().numpy()
+    assert np.array_equal(CI.cpu().numpy(), RI) and np.array_equal(CJ.cpu().numpy(), RJ)
+    assert np.all(np.abs(cv - RV) <= 2.0 ** -22 * np.abs(RV) + 1e-300 + 1e-7 * np.abs(RV).max())
+
+
+def test_half_nonfinite_values_use_fp32(dev):
+    """A matrix with an infinite value gets no FP16 copy: FP32 instead (the
+    reported precision follows); new finite values make FP16 possible again."""
+    (m, n), I, J, A = CASES["7pt_aniso"]()
+    bad = A.copy()
+    bad[5] = np.inf
+    M = make(dev, m, n, I, J, bad)
+    M.set_precision(L.AMGT_HALF)
+    x = np.random.default_rng(12).standard_normal(n)
+    M.spmv(T(x, dev))
+    assert M.info()["precision"] == L.AMGT_FLOAT
     M.update_values(T(I, dev), T(J, dev), T(A, dev))
     y2 = M.spmv(T(x, dev)).cpu().numpy()
     assert M.info()["precision"] == L.AMGT_HALF
