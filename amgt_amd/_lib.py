@@ -49,7 +49,7 @@ EXPORTS = (
     "amgt_read_ij", "amgt_read_mm", "amgt_host_free", "amgt_mbsr_spmm_interleaved",
     "amgt_amg_strength", "amgt_amg_pmis", "amgt_amg_extpi_interp",
     "amgt_policy_work_bytes", "amgt_policy_forward", "amgt_policy_loss_grad", "amgt_policy_adam",
-    "amgt_level_stats_work_bytes", "amgt_level_stats",
+    "amgt_level_stats_work_bytes", "amgt_level_stats", "amgt_interp_trilinear",
 )
 
 
@@ -148,6 +148,8 @@ def load():
         "amgt_policy_loss_grad": (_int, [_vp, _vp, _vp, _vp, _i64, ctypes.c_float, _vp, _vp, _vp,
                                          ctypes.c_size_t, _vp]),
         "amgt_level_stats_work_bytes": (ctypes.c_size_t, [_i32]),
+        "amgt_interp_trilinear": (_int, [_i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32),
+                                         ctypes.POINTER(_i32), _pi64, _pvp, _pvp, _pvp, _vp]),
         "amgt_level_stats": (_int, [_i32, _vp, _vp, _vp, ctypes.POINTER(_dbl), _vp, ctypes.c_size_t,
                                     _vp]),
         "amgt_policy_adam": (_int, [_vp, _vp, _vp, _vp, _i64, _i64, ctypes.c_float, ctypes.c_float,

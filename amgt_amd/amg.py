@@ -59,54 +59,28 @@ def _excl_cumsum(v: torch.Tensor) -> torch.Tensor:
 def interpolation_batched(grids, device):
     """CSR (I, J, A) of the block-diagonal trilinear prolongation of a batch of
     independent grids (problem k's rows and coarse columns follow problem
-    k-1's), built on the device in one pass over all rows.
+    k-1's), built on the device by amgt_interp_trilinear in one pass over all
+    rows.
 
     Per grid and direction the coarse points are the even fine indices
     (nc = (n + 1) // 2); a fine index takes weight 1 from its even point, or
     1/2 from each even neighbour, a neighbour beyond the last coarse point
     being a zero Dirichlet value.  Up to 8 entries per row, ascending columns,
     no stored zeros.  Returns (I, J, A, ncoarse, coarse grids)."""
-    g = torch.tensor([tuple(x) for x in grids], dtype=torch.int64, device=device)
-    nx, ny, nz = g[:, 0], g[:, 1], g[:, 2]
-    ncx, ncy, ncz = (nx + 1) // 2, (ny + 1) // 2, (nz + 1) // 2
-    rows = nx * ny * nz
-    cgrids = [tuple((x + 1) // 2 for x in gr) for gr in grids]
-    n = int(sum(int(np.prod(gr)) for gr in grids))
-    nc = int(sum(int(np.prod(cg)) for cg in cgrids))
-    pid = torch.repeat_interleave(torch.arange(len(grids), device=device), rows, output_size=n)
-    loc = torch.arange(n, device=device) - _excl_cumsum(rows)[pid]
-    px, py, pcx, pcy, pcz = nx[pid], ny[pid], ncx[pid], ncy[pid], ncz[pid]
-    x, y, z = loc % px, (loc // px) % py, loc // (px * py)
-
-    def axis(f, ncf):
-        even = (f % 2) == 0
-        lo = f // 2
-        hi = lo + 1
-        w_lo = torch.where(even, 1.0, 0.5).to(torch.float64)
-        ok = (~even) & (hi < ncf)
-        w_hi = torch.where(ok, 0.5, 0.0).to(torch.float64)
-        return ((lo, w_lo, None), (hi, w_hi, ok))
-
-    cbase = _excl_cumsum(ncx * ncy * ncz)[pid]
-    cols, vals, mask = [], [], []
-    # slots in (kz, ky, kx) order give ascending columns (high = low + 1)
-    for cz, wz, mz in axis(z, pcz):
-        for cy, wy, my in axis(y, pcy):
-            for cx, wx, mx in axis(x, pcx):
-                v = wx * wy * wz
-                m = v != 0.0
-                for mm in (mx, my, mz):
-                    if mm is not None:
-                        m = m & mm
-                cols.append(cbase + (cz * pcy + cy) * pcx + cx)
-                vals.append(v)
-                mask.append(m)
-    C = torch.stack(cols, 1)
-    V = torch.stack(vals, 1)
-    M = torch.stack(mask, 1)
-    I = torch.zeros(n + 1, dtype=torch.int64, device=device)
-    I[1:] = torch.cumsum(M.sum(1), 0)
-    return I.to(torch.int32), C[M].to(torch.int32), V[M], nc, cgrids
+    import ctypes
+    from .mbsr import _stream_ptr, copy_from_device_ptr
+    flat = (ctypes.c_int32 * (3 * len(grids)))(*[int(v) for g in grids for v in g])
+    nf, nc, nnz = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+    pi, pj, pa = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    L.check(L.load().amgt_interp_trilinear(len(grids), flat, ctypes.byref(nf), ctypes.byref(nc),
+                                           ctypes.byref(nnz), ctypes.byref(pi), ctypes.byref(pj),
+                                           ctypes.byref(pa), _stream_ptr(None)),
+            "amgt_interp_trilinear")
+    I = copy_from_device_ptr(pi.value, nf.value + 1, torch.int32, device)
+    J = copy_from_device_ptr(pj.value, nnz.value, torch.int32, device)
+    A = copy_from_device_ptr(pa.value, nnz.value, torch.float64, device)
+    cgrids = [tuple((int(x) + 1) // 2 for x in gr) for gr in grids]
+    return I, J, A, int(nc.value), cgrids
 
 
 def trilinear_interpolation_device(grid, device):

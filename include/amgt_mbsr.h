@@ -229,6 +229,18 @@ int amgt_amg_extpi_interp(int32_t n, const int32_t *A_i, const int32_t *A_j,
                           int64_t *nnzP, int32_t **P_i, int32_t **P_j, double **P_data,
                           void *stream);
 
+/* ---- geometric prolongation (the V-cycle driver's structured hierarchy; no
+ *      counterpart in the reference, whose BoomerAMG builds P algebraically,
+ *      amgt_amg_extpi_interp).  Outputs freed with amgt_free. ---- */
+/* Block-diagonal trilinear P of nprob independent grids (grids: HOST array of
+ * nprob (nx, ny, nz) triples, x fastest): coarse points at even indices,
+ * weights 1 / (1/2, 1/2) per direction, zero Dirichlet beyond the last coarse
+ * point; rows in (z, y, x)-major order with ascending columns, at most 8
+ * entries.  Synchronises the stream. */
+int amgt_interp_trilinear(int32_t nprob, const int32_t *grids, int32_t *n_fine,
+                          int32_t *n_coarse, int64_t *nnzP, int32_t **P_i, int32_t **P_j,
+                          double **P_data, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

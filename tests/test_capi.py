@@ -61,3 +61,19 @@ def test_package_fails_loudly_without_library(tmp_path, monkeypatch):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(_lib.AmgtLibraryMissing):
         _lib.load()
+
+
+def test_trilinear_interp_rejects_bad_grids_before_device_work():
+    """Argument errors return AMGT_ERR_ARG before any HIP call (runs without a GPU)."""
+    from amgt_amd import _lib
+    L = _lib.load()
+    g = (ctypes.c_int32 * 3)(4, 0, 2)
+    out = [ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()]
+    ptrs = [ctypes.c_void_p() for _ in range(3)]
+    rc = L.amgt_interp_trilinear(1, g, ctypes.byref(out[0]), ctypes.byref(out[1]),
+                                 ctypes.byref(out[2]), *[ctypes.byref(p) for p in ptrs], None)
+    assert rc == _lib.AMGT_ERR_ARG
+    assert b"positive" in L.amgt_last_error()
+    rc = L.amgt_interp_trilinear(0, g, ctypes.byref(out[0]), ctypes.byref(out[1]),
+                                 ctypes.byref(out[2]), *[ctypes.byref(p) for p in ptrs], None)
+    assert rc == _lib.AMGT_ERR_ARG

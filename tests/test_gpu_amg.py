@@ -128,6 +128,24 @@ def test_pcg_fp64_and_mixed(dev):
     assert pol[0] == L.AMGT_DOUBLE and L.AMGT_FLOAT in pol
 
 
+def test_batched_interpolation_is_block_diagonal(dev):
+    """amgt_interp_trilinear on a batch equals the block diagonal of the
+    per-grid numpy construction (row pointers, columns and values exactly)."""
+    import scipy.sparse as sp
+    grids = [(5, 4, 3), (2, 3, 1), (7, 1, 1), (1, 1, 1), (9, 8, 2)]
+    I, J, A, nc, cg = G.interpolation_batched(grids, dev)
+    blocks = []
+    for g in grids:
+        PI, PJ, PA, pnc = W.trilinear_interpolation(*g)
+        blocks.append(sp.csr_matrix((PA, PJ, PI), shape=(int(np.prod(g)), pnc)))
+    ref = sp.block_diag(blocks, format="csr")
+    ref.sort_indices()
+    assert nc == ref.shape[1] and cg == [tuple((x + 1) // 2 for x in g) for g in grids]
+    assert np.array_equal(I.cpu().numpy(), ref.indptr)
+    assert np.array_equal(J.cpu().numpy(), ref.indices)
+    assert np.array_equal(A.cpu().numpy(), ref.data)
+
+
 @pytest.mark.parametrize("grid", [(5, 4, 3), (12, 10, 9), (33, 17, 5), (2, 3, 1)])
 def test_device_interpolation_and_transpose(dev, grid):
     """The device-built trilinear P equals workloads' numpy construction; the
