@@ -351,6 +351,8 @@ def main():
                     help="profiling mode: tuned SpMV launches plus a calibration copy, no report")
     ap.add_argument("--copy-mb", type=int, default=256,
                     help="size of the calibration copy in --kernel-only mode")
+    ap.add_argument("--spgemm-only", action="store_true",
+                    help="profiling mode: only the SpGEMM timings (A^2 and the Galerkin pair)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the mBSR path is "
@@ -380,6 +382,10 @@ def main():
     conv_ms = (time.perf_counter() - tc) * 1e3
     info = M.info()
 
+    if args.spgemm_only:
+        print(json.dumps({"spgemm_only": True,
+                          "spgemm": spgemm_timings(W, MBSRMatrix, M, I, J, A, dev)}), flush=True)
+        return
     if args.kernel_only:
         # counter runs: the tuned SpMV kernel alone, then a plain device copy of
         # known size whose FETCH_SIZE / WRITE_SIZE calibrate the counters
