@@ -38,9 +38,23 @@ constexpr int kWaveMaxRowsSmall = 1 << 21;  // the kernel of rows up to 64 entri
 
 inline unsigned grid_of(int64_t n) { return unsigned((n + kThreads - 1) / kThreads); }
 
+// position of a_ii in row i, or -1: the first entry (HYPRE's diagonal-first
+// rows), else a binary search (rows with ascending columns, the exports of
+// mBSR handles), else a scan.  A search can only return an entry of column i,
+// and a CSR row holds at most one, so any row order gives the same answer.
 __device__ __forceinline__ int64_t diag_pos(const int32_t *__restrict__ I,
                                             const int32_t *__restrict__ J, int i) {
-    for (int64_t p = I[i]; p < I[i + 1]; ++p)
+    const int64_t b = I[i], e = I[i + 1];
+    if (b == e) return -1;
+    if (J[b] == i) return b;
+    int64_t lo = b, hi = e - 1;
+    while (lo <= hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        const int c = J[mid];
+        if (c == i) return mid;
+        if (c < i) lo = mid + 1; else hi = mid - 1;
+    }
+    for (int64_t p = b; p < e; ++p)
         if (J[p] == i) return p;
     return -1;
 }
