@@ -134,6 +134,7 @@ class PolicyNet:
         self.mean = np.zeros(NF)
         self.std = np.ones(NF)
         self.theta_mid, self.theta_half = 0.4, 0.3
+        self.meta: dict = {}
         self._work = None
 
     # standardisation -------------------------------------------------------------------
@@ -152,8 +153,18 @@ class PolicyNet:
         return self._work
 
     # kernels ---------------------------------------------------------------------------
+    def _check(self, t: torch.Tensor, shape: tuple, dtype, name: str):
+        """The kernels index these arrays by the sizes given: anything else is
+        rejected before a launch."""
+        if tuple(t.shape) != shape or t.dtype != dtype or not t.is_contiguous() or \
+                t.device != self.params.device:
+            raise ValueError(f"{name}: expected a contiguous {dtype} tensor of shape {shape} on "
+                             f"{self.params.device}, got {tuple(t.shape)} {t.dtype} on {t.device}")
+
     def forward_raw(self, Xd: torch.Tensor) -> torch.Tensor:
         n = Xd.shape[0]
+        self._check(Xd, (n, NF), torch.float32, "features")
+        self._check(self.params, (NPARAMS,), torch.float32, "params")
         out = torch.empty((n, NOUT), dtype=torch.float32, device=self.device)
         w = self._workspace(n)
         L.check(L.load().amgt_policy_forward(
@@ -164,6 +175,10 @@ class PolicyNet:
 
     def loss_grad_raw(self, Xd, cls_d, tgt_d, lam: float):
         n = Xd.shape[0]
+        self._check(Xd, (n, NF), torch.float32, "features")
+        self._check(cls_d, (n,), torch.int32, "labels")
+        self._check(tgt_d, (n,), torch.float32, "targets")
+        self._check(self.params, (NPARAMS,), torch.float32, "params")
         grad = torch.empty(NPARAMS, dtype=torch.float32, device=self.device)
         loss = torch.empty(1, dtype=torch.float32, device=self.device)
         w = self._workspace(n)
@@ -175,6 +190,8 @@ class PolicyNet:
         return loss, grad
 
     def adam_raw(self, grad, m, v, step: int, lr: float, b1=0.9, b2=0.999, eps=1e-8):
+        for t, name in ((self.params, "params"), (grad, "grad"), (m, "m"), (v, "v")):
+            self._check(t, (NPARAMS,), torch.float32, name)
         L.check(L.load().amgt_policy_adam(
             ctypes.c_void_p(self.params.data_ptr()), ctypes.c_void_p(grad.data_ptr()),
             ctypes.c_void_p(m.data_ptr()), ctypes.c_void_p(v.data_ptr()), NPARAMS, int(step),
@@ -239,11 +256,6 @@ def set_level_precision(amg, lvl: int, prec: int):
     for M in (L_.A, L_.P, L_.R):
         if M is not None:
             M.set_precision(prec)
-
-
-def conv_factor(iters: int, hist: list) -> float:
-    """PCG average convergence factor (r_k / r_0)^(1/k)."""
-    return (hist[-1] / hist[0]) ** (1.0 / max(iters, 1))
 
 
 def cycle_contraction(amg, cycles: int = 3, seed: int = 7) -> float:

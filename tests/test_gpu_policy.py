@@ -200,3 +200,19 @@ def test_trained_policy_on_heldout(dev):
         lr = r["learned"]
         assert all(P.THETA_RANGE[0] <= t <= P.THETA_RANGE[1] for t in lr["thetas"])
         assert lr["precisions"][0] == L.AMGT_DOUBLE
+
+
+def test_wrappers_reject_bad_arrays(dev):
+    """Arrays the kernels would index out of bounds never reach a launch."""
+    net = P.PolicyNet(dev)
+    with pytest.raises(ValueError):
+        net.forward_raw(torch.zeros((4, 11), dtype=torch.float32, device=dev))
+    with pytest.raises(ValueError):
+        net.forward_raw(torch.zeros((4, 12), dtype=torch.float64, device=dev))
+    X = torch.zeros((4, 12), dtype=torch.float32, device=dev)
+    with pytest.raises(ValueError):
+        net.loss_grad_raw(X, torch.zeros(3, dtype=torch.int32, device=dev),
+                          torch.zeros(4, dtype=torch.float32, device=dev), 1.0)
+    g = torch.zeros(P.NPARAMS, dtype=torch.float32, device=dev)
+    with pytest.raises(ValueError):
+        net.adam_raw(g, g, torch.zeros(10, dtype=torch.float32, device=dev), 1, 1e-3)
