@@ -35,6 +35,10 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr int kBin1Ub = 2048;   // workgroup per row, LDS table of 4096
 constexpr int kNumCap = 64;     // C blocks per row kept in LDS by the numeric kernel
 constexpr int kBigCap = 256;    // ... and by the long-row kernel before its global slab
+// long rows whose C values (compressed, as the symbolic phase fixed them) fit
+// kCmpVal accumulators and whose blocks fit kCmpBlk: LDS per wave ~16 KB
+constexpr int kCmpBlk = 512;
+constexpr int kCmpVal = 1024;
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -451,12 +455,16 @@ struct NumStage {
 // may hit the same C block, so their results are added one block slot after
 // the other (fixed order: deterministic, no atomics).  boff[q] is the value
 // offset of B block q (values are stored in block order).
-template <typename VT>
+// kCmp: acc holds only the row's C values, in their final (block, bit) order:
+// entry e of block t at scvo[t] + popc(scm[t] below e); an entry outside the C
+// bitmap only ever receives +0.0 products and is not stored.  The additions
+// into stored entries are the same, in the same order, as with dense blocks.
+template <typename VT, bool kCmp = false>
 __device__ void numeric_row(int i, const SpgemmArgs &g, const int32_t *__restrict__ boff,
                             const int32_t *__restrict__ cptr, const int32_t *__restrict__ ccol,
                             const uint16_t *__restrict__ cmap, const int32_t *__restrict__ cvp,
                             double *__restrict__ cval, int32_t *skey, double *acc,
-                            NumStage &st) {
+                            NumStage &st, int32_t *scvo = nullptr, uint32_t *scm = nullptr) {
     const int lane = threadIdx.x & 63;
     // MFMA operand roles of this lane (see block_product)
     const int bs = (lane >> 2) & 3;                       // block slot
@@ -470,7 +478,29 @@ __device__ void numeric_row(int i, const SpgemmArgs &g, const int32_t *__restric
         for (int t = lane; t < cnt; t += 64) skey[t] = keys[t];
         keys = skey;
     }
-    for (int x = lane; x < cnt * 16; x += 64) acc[x] = 0.0;
+    int nacc = cnt * 16;
+    if constexpr (kCmp) {
+        // value offset of every C block of the row: a scan of the bitmap popcounts
+        int run = 0;
+        for (int t0 = 0; t0 < cnt; t0 += 64) {
+            const int t = t0 + lane;
+            const uint32_t m = t < cnt ? uint32_t(cmap[c0 + t]) : 0u;
+            const int pc = __popc(m);
+            int inc = pc;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int v = __shfl_up(inc, d);
+                if (lane >= d) inc += v;
+            }
+            if (t < cnt) {
+                scm[t] = m;
+                scvo[t] = run + inc - pc;
+            }
+            run += __shfl(inc, 63);
+        }
+        nacc = run;
+    }
+    for (int x = lane; x < nacc; x += 64) acc[x] = 0.0;
     wave_sync();
     int aoff_run = g.avp[i];
     const int a_beg = g.ap[i], a_end = g.ap[i + 1];
@@ -560,7 +590,14 @@ __device__ void numeric_row(int i, const SpgemmArgs &g, const int32_t *__restric
                     const double d = block_product(av[u], bv[u]);
 #pragma unroll
                     for (int b2 = 0; b2 < 4; ++b2) {
-                        if (bs == b2 && on[u]) acc[sl[u] * 16 + eB] += d;
+                        if (bs == b2 && on[u]) {
+                            if constexpr (kCmp) {
+                                const uint32_t m = scm[sl[u]];
+                                if ((m >> eB) & 1u) acc[scvo[sl[u]] + __popc(m & belowB)] += d;
+                            } else {
+                                acc[sl[u] * 16 + eB] += d;
+                            }
+                        }
                         wave_sync();
                     }
                 }
@@ -568,6 +605,11 @@ __device__ void numeric_row(int i, const SpgemmArgs &g, const int32_t *__restric
         }
     }
     wave_sync();
+    if constexpr (kCmp) {  // already in the value order of the row
+        const int v0 = cvp[i];
+        for (int x = lane; x < nacc; x += 64) cval[v0 + x] = round_out<VT>(acc[x]);
+        return;
+    }
     // compress: four C blocks per step, slot gs takes block t0 + gs
     const int gs = lane >> 4, e = lane & 15;
     const uint32_t below = (1u << e) - 1u;
@@ -602,6 +644,35 @@ __global__ __launch_bounds__(256) void k_numeric_lds(int count, const int32_t *_
     numeric_row<VT>(rows[r], g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w], s_st[w]);
 }
 
+// rows of the long bin that fit the compressed accumulators: one wave per row
+// (about 16 KB of LDS per wave, two workgroups per CU)
+__device__ __forceinline__ bool fits_cmp(const int32_t *cptr, const int32_t *cvp, int i) {
+    return cptr[i + 1] - cptr[i] <= kCmpBlk && cvp[i + 1] - cvp[i] <= kCmpVal;
+}
+
+template <typename VT>
+__global__ __launch_bounds__(256) void k_numeric_cmp(int count, const int32_t *__restrict__ rows,
+                                                     SpgemmArgs g,
+                                                     const int32_t *__restrict__ boff,
+                                                     const int32_t *__restrict__ cptr,
+                                                     const int32_t *__restrict__ ccol,
+                                                     const uint16_t *__restrict__ cmap,
+                                                     const int32_t *__restrict__ cvp,
+                                                     double *__restrict__ cval) {
+    __shared__ double s_acc[4][kCmpVal];
+    __shared__ int32_t s_key[4][kCmpBlk];
+    __shared__ int32_t s_cvo[4][kCmpBlk];
+    __shared__ uint32_t s_cm[4][kCmpBlk];
+    __shared__ NumStage s_st[4];
+    const int w = threadIdx.x >> 6;
+    const int r = blockIdx.x * 4 + w;
+    if (r >= count) return;  // whole wave
+    const int i = rows[r];
+    if (!fits_cmp(cptr, cvp, i)) return;  // k_numeric_global's row
+    numeric_row<VT, true>(i, g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w], s_st[w],
+                          s_cvo[w], s_cm[w]);
+}
+
 template <typename VT>
 __global__ __launch_bounds__(256) void k_numeric_global(int count, const int32_t *__restrict__ rows,
                                                         SpgemmArgs g,
@@ -622,6 +693,7 @@ __global__ __launch_bounds__(256) void k_numeric_global(int count, const int32_t
     double *acc = slab + int64_t(wave) * slab_elems;
     for (int r = wave; r < count; r += nwaves) {
         const int i = rows[r];
+        if (fits_cmp(cptr, cvp, i)) continue;  // k_numeric_cmp's row
         if (cptr[i + 1] - cptr[i] <= kBigCap)
             numeric_row<VT>(i, g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w], s_st[w]);
         else
@@ -778,6 +850,9 @@ int launch_numeric(const SpgemmArgs &g, const int32_t *boff, const Mbsr &C, int3
                            lds_rows, list + brow, g, boff, C.blk_ptr, C.blk_col, C.blk_map,
                            C.val_ptr, C.val);
     if (big_rows > 0) {
+        hipLaunchKernelGGL((k_numeric_cmp<VT>), dim3((big_rows + 3) / 4), dim3(256), 0, s,
+                           big_rows, list + 2 * int64_t(brow), g, boff, C.blk_ptr, C.blk_col,
+                           C.blk_map, C.val_ptr, C.val);
         const int64_t elems = int64_t(C.max_row_blocks) * 16;
         int nwaves = std::min(big_rows, 1024);
         while (nwaves > 4 && elems * nwaves * 8 > (int64_t(1) << 31)) nwaves >>= 1;

@@ -110,6 +110,40 @@ def test_spgemm_bins_and_big_rows(dev):
     check_product(dev, 16, 600, 2000, AI, AJ, AA, BI, BJ, BA, "bin2_slab")
 
 
+def test_spgemm_long_sparse_rows(dev):
+    """C block rows over 64 blocks with sparse blocks (coarse Galerkin rows):
+    rows of at most 512 blocks and 1024 values accumulate compressed in LDS,
+    the others (more blocks, or more values) in the long-row kernel, in the
+    same launch; both against the oracle."""
+    rng = np.random.default_rng(21)
+    k = n = 4000
+    rows = []
+    for r in range(16):
+        br = r // 4
+        if br == 0:      # ~320 blocks, ~380 values: compressed accumulators
+            cols = rng.choice(k, 80, replace=False)
+        elif br == 1:    # ~540 blocks: long-row kernel (slab)
+            cols = rng.choice(k, 160, replace=False)
+        elif br == 2:    # short rows
+            cols = rng.choice(k, 12, replace=False)
+        else:            # ~300 blocks but ~1200 values: long-row kernel
+            cols = rng.choice(1200, 300, replace=False)
+        rows.append(np.sort(cols))
+    AI = np.concatenate([[0], np.cumsum([len(c) for c in rows])]).astype(np.int32)
+    AJ = np.concatenate(rows).astype(np.int32)
+    AA = rng.standard_normal(AJ.size)
+    Bm = (sp.identity(k, format="csr") * 2.0 + sp.random(k, n, density=0.00005, random_state=22,
+                                                           format="csr")).tocsr()
+    Bm.sort_indices()
+    BI, BJ, BA = Bm.indptr.astype(np.int32), Bm.indices.astype(np.int32), Bm.data.copy()
+    C = (sp.csr_matrix((AA, AJ, AI), shape=(16, k)) @ Bm).tocoo()
+    nblk = [len({c // 4 for r, c in zip(C.row, C.col) if r // 4 == b}) for b in range(4)]
+    nval = [int(np.sum(C.row // 4 == b)) for b in range(4)]
+    assert 64 < nblk[0] <= 512 and nval[0] <= 1024
+    assert nblk[1] > 512 and nval[3] > 1024
+    check_product(dev, 16, k, n, AI, AJ, AA, BI, BJ, BA, "long_sparse")
+
+
 def test_spgemm_rap(dev):
     nx, ny, nz = 17, 14, 11
     AI, AJ, AA = W.stencil(7, nx, ny, nz)
