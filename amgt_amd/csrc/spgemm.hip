@@ -348,7 +348,16 @@ struct SymbolicBin {
     }
 };
 
-// ---- bin 1: one workgroup per row, LDS table of 4096
+// hash slots for one row: a power of two >= twice the row's upper bound (the
+// distinct keys never fill more than half of it), at least 256, at most cap
+__device__ __forceinline__ int row_table(const int64_t *__restrict__ ub_off, int i, int cap) {
+    const int64_t want = 2 * (ub_off[i + 1] - ub_off[i]);
+    int hs = 256;
+    while (hs < want && hs < cap) hs <<= 1;
+    return hs < cap ? hs : cap;
+}
+
+// ---- bin 1: one workgroup per row, LDS table of up to 4096
 __global__ __launch_bounds__(256) void k_symbolic_block(int count, const int32_t *__restrict__ rows,
                                                         SpgemmArgs g,
                                                         const int64_t *__restrict__ ub_off,
@@ -360,11 +369,12 @@ __global__ __launch_bounds__(256) void k_symbolic_block(int count, const int32_t
     __shared__ int s_excl[256], s_q0[256], s_ma[256], s_sc[4];
     if (int(blockIdx.x) >= count) return;
     const int i = rows[blockIdx.x];
-    symbolic_row<256>(i, threadIdx.x, s_key, s_map, HS, s_excl, s_q0, s_ma, s_sc, g.ap, g.ac, g.am,
+    symbolic_row<256>(i, threadIdx.x, s_key, s_map, row_table(ub_off, i, HS), s_excl, s_q0, s_ma,
+                      s_sc, g.ap, g.ac, g.am,
                       g.bp, g.bc, g.bm, ub_off[i], tcol, tmap, cnt_out, nnz_out);
 }
 
-// ---- bin 2: workgroups loop over rows, each with a global slab of hs slots
+// ---- bin 2: workgroups loop over rows, each with a global slab of up to hs slots
 __global__ __launch_bounds__(256) void k_symbolic_global(int count, const int32_t *__restrict__ rows,
                                                          SpgemmArgs g,
                                                          const int64_t *__restrict__ ub_off,
@@ -376,7 +386,8 @@ __global__ __launch_bounds__(256) void k_symbolic_global(int count, const int32_
     uint32_t *hmap = hkey + hs;
     for (int r = blockIdx.x; r < count; r += gridDim.x) {
         const int i = rows[r];
-        symbolic_row<256>(i, threadIdx.x, hkey, hmap, hs, s_excl, s_q0, s_ma, s_sc, g.ap, g.ac,
+        symbolic_row<256>(i, threadIdx.x, hkey, hmap, row_table(ub_off, i, hs), s_excl, s_q0,
+                          s_ma, s_sc, g.ap, g.ac,
                           g.am, g.bp, g.bc, g.bm, ub_off[i], tcol, tmap, cnt_out, nnz_out);
     }
 }
