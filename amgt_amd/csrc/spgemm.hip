@@ -91,17 +91,28 @@ __device__ __forceinline__ uint32_t hash_slot(uint32_t key, int hs) {
 // Symbolic work for one C block row by a group of NT threads (tid in [0, NT)).
 // hkey/hmap: hash table of hs slots (LDS or global), initialised here.
 // s_excl/s_q0/s_ma: NT-entry LDS staging for the product enumeration.
-template <int NT>
+// kTry (a row whose upper bound exceeds the table): insertion gives up once
+// more than hs/2 distinct keys arrived or a probe sequence passed the whole
+// table; the row then writes nothing but retry[i] = 1 (retry[i] = 0 when it
+// completes), for the global-table kernel.  s_flag: two LDS ints.
+template <int NT, bool kTry = false>
 __device__ void symbolic_row(int i, int tid, uint32_t *hkey, uint32_t *hmap, int hs,
                              int *s_excl, int *s_q0, int *s_ma, int *s_sc,
                              const int32_t *__restrict__ ap, const int32_t *__restrict__ ac,
                              const uint16_t *__restrict__ am, const int32_t *__restrict__ bp,
                              const int32_t *__restrict__ bc, const uint16_t *__restrict__ bm,
                              int64_t slot, int32_t *__restrict__ tcol, uint32_t *__restrict__ tmap,
-                             int32_t *__restrict__ cnt_out, int32_t *__restrict__ nnz_out) {
+                             int32_t *__restrict__ cnt_out, int32_t *__restrict__ nnz_out,
+                             int *s_flag = nullptr, int32_t *__restrict__ retry = nullptr) {
     for (int s = tid; s < hs; s += NT) {
         hkey[s] = kEmpty;
         hmap[s] = 0u;
+    }
+    if constexpr (kTry) {
+        if (tid == 0) {
+            s_flag[0] = 0;  // distinct keys inserted
+            s_flag[1] = 0;  // given up
+        }
     }
     grp_sync<NT>();
     const int a_beg = ap[i], a_end = ap[i + 1];
@@ -133,17 +144,42 @@ __device__ void symbolic_row(int i, int tid, uint32_t *hkey, uint32_t *hmap, int
             if (mc) {
                 const uint32_t key = uint32_t(bc[q]);
                 uint32_t h = hash_slot(key, hs);
-                while (true) {
-                    const uint32_t old = atomicCAS(&hkey[h], kEmpty, key);
-                    if (old == kEmpty || old == key) {
-                        atomicOr(&hmap[h], mc);
-                        break;
+                if constexpr (kTry) {
+                    if (__atomic_load_n(&s_flag[1], __ATOMIC_RELAXED)) continue;
+                    int probe = 0;
+                    for (; probe < hs; ++probe) {
+                        const uint32_t old = atomicCAS(&hkey[h], kEmpty, key);
+                        if (old == kEmpty || old == key) {
+                            atomicOr(&hmap[h], mc);
+                            if (old == kEmpty && atomicAdd(&s_flag[0], 1) >= hs / 2)
+                                atomicExch(&s_flag[1], 1);
+                            break;
+                        }
+                        h = (h + 1) & uint32_t(hs - 1);
                     }
-                    h = (h + 1) & uint32_t(hs - 1);
+                    if (probe == hs) atomicExch(&s_flag[1], 1);
+                } else {
+                    while (true) {
+                        const uint32_t old = atomicCAS(&hkey[h], kEmpty, key);
+                        if (old == kEmpty || old == key) {
+                            atomicOr(&hmap[h], mc);
+                            break;
+                        }
+                        h = (h + 1) & uint32_t(hs - 1);
+                    }
                 }
             }
         }
         grp_sync<NT>();
+        if constexpr (kTry) {
+            if (__atomic_load_n(&s_flag[1], __ATOMIC_RELAXED)) break;  // uniform after the sync
+        }
+    }
+    if constexpr (kTry) {
+        const int gave_up = __atomic_load_n(&s_flag[1], __ATOMIC_RELAXED);
+        grp_sync<NT>();
+        if (tid == 0) retry[i] = gave_up;
+        if (gave_up) return;  // uniform: every thread read the flag after the last sync
     }
     // compact occupied slots to the front (in place: destinations never pass sources)
     int base = 0;
@@ -374,18 +410,39 @@ __global__ __launch_bounds__(256) void k_symbolic_block(int count, const int32_t
                       g.bp, g.bc, g.bm, ub_off[i], tcol, tmap, cnt_out, nnz_out);
 }
 
+// ---- bin 2, first try: one workgroup per row, the LDS table of bin 1 (the
+// rows' distinct C blocks are usually far fewer than their upper bound)
+__global__ __launch_bounds__(256) void k_symbolic_try(int count, const int32_t *__restrict__ rows,
+                                                      SpgemmArgs g,
+                                                      const int64_t *__restrict__ ub_off,
+                                                      int32_t *tcol, uint32_t *tmap,
+                                                      int32_t *cnt_out, int32_t *nnz_out,
+                                                      int32_t *retry) {
+    constexpr int HS = 2 * kBin1Ub;
+    __shared__ uint32_t s_key[HS];
+    __shared__ uint32_t s_map[HS];
+    __shared__ int s_excl[256], s_q0[256], s_ma[256], s_sc[4], s_flag[2];
+    if (int(blockIdx.x) >= count) return;
+    const int i = rows[blockIdx.x];
+    symbolic_row<256, true>(i, threadIdx.x, s_key, s_map, HS, s_excl, s_q0, s_ma, s_sc, g.ap,
+                            g.ac, g.am, g.bp, g.bc, g.bm, ub_off[i], tcol, tmap, cnt_out,
+                            nnz_out, s_flag, retry);
+}
+
 // ---- bin 2: workgroups loop over rows, each with a global slab of up to hs slots
 __global__ __launch_bounds__(256) void k_symbolic_global(int count, const int32_t *__restrict__ rows,
                                                          SpgemmArgs g,
                                                          const int64_t *__restrict__ ub_off,
                                                          int32_t *tcol, uint32_t *tmap,
                                                          int32_t *cnt_out, int32_t *nnz_out,
-                                                         uint32_t *slab, int hs) {
+                                                         uint32_t *slab, int hs,
+                                                         const int32_t *__restrict__ retry) {
     __shared__ int s_excl[256], s_q0[256], s_ma[256], s_sc[4];
     uint32_t *hkey = slab + int64_t(blockIdx.x) * 2 * hs;
     uint32_t *hmap = hkey + hs;
     for (int r = blockIdx.x; r < count; r += gridDim.x) {
         const int i = rows[r];
+        if (!retry[i]) continue;  // completed in LDS by k_symbolic_try
         symbolic_row<256>(i, threadIdx.x, hkey, hmap, row_table(ub_off, i, hs), s_excl, s_q0,
                           s_ma, s_sc, g.ap, g.ac,
                           g.am, g.bp, g.bc, g.bm, ub_off[i], tcol, tmap, cnt_out, nnz_out);
@@ -921,6 +978,7 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
     int32_t *list = nullptr, *ccnt = nullptr, *cnnz = nullptr;
     int32_t *tcol = nullptr;
     uint32_t *tmap = nullptr, *slab = nullptr;
+    int32_t *retry = nullptr;
     double *nslab = nullptr;
     int32_t *bpop = nullptr, *boff = nullptr;
     int st = AMGT_OK;
@@ -935,6 +993,7 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
         tmp_free(tcol, s);
         tmp_free(tmap, s);
         tmp_free(slab, s);
+        tmp_free(retry, s);
         tmp_free(nslab, s);
         tmp_free(bpop, s);
         tmp_free(boff, s);
@@ -1001,8 +1060,12 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
             return AMGT_ERR_UNSUPPORTED;
         }
         SG_TRY(tmp_alloc_t(&slab, size_t(per_wg * nwg), s));
+        SG_TRY(tmp_alloc_t(&retry, size_t(brow) + 1, s));
+        hipLaunchKernelGGL(k_symbolic_try, dim3(bin[2]), dim3(256), 0, s, bin[2],
+                           list + 2 * int64_t(brow), g, ub_off, tcol, tmap, ccnt, cnnz, retry);
         hipLaunchKernelGGL(k_symbolic_global, dim3(nwg), dim3(256), 0, s, bin[2],
-                           list + 2 * int64_t(brow), g, ub_off, tcol, tmap, ccnt, cnnz, slab, hs);
+                           list + 2 * int64_t(brow), g, ub_off, tcol, tmap, ccnt, cnnz, slab, hs,
+                           retry);
     }
     SG_HIP(hipGetLastError());
     SG_TRY(exclusive_scan_i32(ccnt, C.blk_ptr, brow, s));

@@ -97,6 +97,12 @@ def test_spgemm_bins_and_big_rows(dev):
     AI, AJ, AA = dense_rows_csr(8, 3000, 400, 9)
     BI, BJ, BA = rand_csr(3000, 5000, 0.004, 10)
     check_product(dev, 8, 3000, 5000, AI, AJ, AA, BI, BJ, BA, "bin2")
+    # over 2048 distinct C blocks per block row: the LDS try gives up, the
+    # global table redoes the row
+    BI, BJ, BA = rand_csr(3000, 40000, 0.004, 15)
+    C = sp.csr_matrix((AA, AJ, AI), shape=(8, 3000)) @ sp.csr_matrix((BA, BJ, BI), shape=(3000, 40000))
+    assert len({c // 4 for r, c in zip(*C.nonzero()) if r < 4}) > 2048
+    check_product(dev, 8, 3000, 40000, AI, AJ, AA, BI, BJ, BA, "bin2_retry")
     # C block rows of 65..128 blocks (LDS accumulators of the long-row kernel)
     # next to rows over 128 (its global slab)
     AI, AJ, AA = dense_rows_csr(16, 600, 60, 11)
