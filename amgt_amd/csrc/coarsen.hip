@@ -528,7 +528,8 @@ __global__ __launch_bounds__(64) void k_extpi_wave(
     const int32_t *__restrict__ J, const double *__restrict__ A, const int32_t *__restrict__ SI,
     const int32_t *__restrict__ SJ, const int32_t *__restrict__ cf,
     const int64_t *__restrict__ off, int max_elmts, int32_t *__restrict__ sj,
-    double *__restrict__ sw, int32_t *__restrict__ cnt, int32_t *__restrict__ use) {
+    double *__restrict__ sw, int32_t *__restrict__ cnt, int32_t *__restrict__ use,
+    int overflow_code) {
     __shared__ WaveRow<WA, WT, WH> s;
     if (blockIdx.x >= unsigned(count)) return;
     const int lane = threadIdx.x;
@@ -594,8 +595,8 @@ __global__ __launch_bounds__(64) void k_extpi_wave(
         atomicMin(&s.hval[h], t);
     }
     __syncthreads();
-    if (s.sc[2] > WT) {
-        if (lane == 0) use[i] = 0;
+    if (s.sc[2] > WT) {  // to the next capacity (overflow_code), or the thread kernel (0)
+        if (lane == 0) use[i] = overflow_code;
         return;
     }
     int c = 0;
@@ -736,8 +737,12 @@ __global__ __launch_bounds__(kThreads) void k_rows_unsorted(int32_t n, const int
     }
 }
 
-// rows for the wave kernels: 1 = rows of up to 64 entries, 2 = up to 256
-// (when allowed), 0 = the thread kernel
+// rows for the wave kernels (0 = the thread kernel): kTier64 = rows of up to 64
+// entries, kTier128 / kTier256 = up to 128 / 256 (when allowed).  Each tier
+// first runs with a small table of P columns; a row with more distinct columns
+// moves on (use[i] = the next code) to the tier's large table, and from there
+// to the thread kernel.
+constexpr int kTier64 = 1, kTier64Big = 2, kTier128 = 3, kTier128Big = 4, kTier256 = 5;
 __global__ __launch_bounds__(kThreads) void k_wave_rows(int32_t n, const int32_t *__restrict__ I,
                                                        const int32_t *__restrict__ SI,
                                                        const int32_t *__restrict__ cf,
@@ -747,8 +752,9 @@ __global__ __launch_bounds__(kThreads) void k_wave_rows(int32_t n, const int32_t
     const int la = I[i + 1] - I[i], ls = SI[i + 1] - SI[i];
     int u = 0;
     if (cf[i] < 0 && cf[i] != -3) {
-        if (la <= 64 && ls <= 64) u = 1;
-        else if (allow_big && la <= 256 && ls <= 256) u = 2;
+        if (la <= 64 && ls <= 64) u = kTier64;
+        else if (allow_big && la <= 128 && ls <= 128) u = kTier128;
+        else if (allow_big && la <= 256 && ls <= 256) u = kTier256;
     }
     use[i] = u;
 }
@@ -1015,7 +1021,12 @@ int amgt_amg_extpi_interp(int32_t n, const int32_t *A_i, const int32_t *A_j,
                                        n <= kWaveMaxRows ? 1 : 0, use);
                     if (hipGetLastError() != hipSuccess) st = AMGT_ERR_HIP;
                 }
-                for (int which = 1; which <= 2 && st == AMGT_OK; ++which) {
+                // (rows with this code, kernel, code for the rows that overflow it)
+                const int stages[5][2] = {{kTier64, kTier64Big}, {kTier64Big, 0},
+                                          {kTier128, kTier128Big}, {kTier128Big, 0},
+                                          {kTier256, 0}};
+                for (int sg = 0; sg < 5 && st == AMGT_OK; ++sg) {
+                    const int which = stages[sg][0], next = stages[sg][1];
                     int32_t cntw = 0;
                     hipLaunchKernelGGL(k_flag_eq, g, b, 0, s, n, use, which, flag2);
                     if (hipGetLastError() != hipSuccess) st = AMGT_ERR_HIP;
@@ -1030,14 +1041,17 @@ int amgt_amg_extpi_interp(int32_t n, const int32_t *A_i, const int32_t *A_j,
                     st = tmp.get(&wl, size_t(cntw));
                     if (st != AMGT_OK) break;
                     hipLaunchKernelGGL(k_list_rows, g, b, 0, s, n, flag2, upos, wl);
-                    if (which == 1)
-                        hipLaunchKernelGGL((k_extpi_wave<64, 384, 1024>), dim3(unsigned(cntw)),
-                                           dim3(64), 0, s, cntw, wl, A_i, A_j, A_data, S_i, S_j,
-                                           cf_marker, off, max_elmts, sj, sw, cnt, use);
-                    else
-                        hipLaunchKernelGGL((k_extpi_wave<256, 768, 2048>), dim3(unsigned(cntw)),
-                                           dim3(64), 0, s, cntw, wl, A_i, A_j, A_data, S_i, S_j,
-                                           cf_marker, off, max_elmts, sj, sw, cnt, use);
+#define AMGT_EXTPI_WAVE(WA_, WT_, WH_)                                                     \
+    hipLaunchKernelGGL((k_extpi_wave<WA_, WT_, WH_>), dim3(unsigned(cntw)), dim3(64), 0, s, cntw, \
+                       wl, A_i, A_j, A_data, S_i, S_j, cf_marker, off, max_elmts, sj, sw, cnt, \
+                       use, next)
+                    switch (which) {
+                        case kTier64: AMGT_EXTPI_WAVE(64, 128, 512); break;      // ~8.4 KB
+                        case kTier64Big: AMGT_EXTPI_WAVE(64, 384, 1024); break;  // ~15.6 KB
+                        case kTier128: AMGT_EXTPI_WAVE(128, 256, 1024); break;   // ~16 KB
+                        default: AMGT_EXTPI_WAVE(256, 768, 2048); break;         // ~35 KB
+                    }
+#undef AMGT_EXTPI_WAVE
                     if (hipGetLastError() != hipSuccess) st = AMGT_ERR_HIP;
                     nwave += cntw;
                 }

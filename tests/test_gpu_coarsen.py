@@ -94,6 +94,48 @@ def test_strength_pmis_extpi_match_oracle(dev, name, theta, mrs):
         assert np.array_equal(PA.cpu().numpy(), rPA), pmx
 
 
+def _rand_long(n, per_row, seed):
+    """Random M-matrix rows of about per_row negative off-diagonals (sorted
+    columns, dominant diagonal): long rows with many distance-two C points."""
+    rng = np.random.default_rng(seed)
+    c = rng.integers(0, n - 1, size=(n, per_row + 8))
+    rows, cols = [], []
+    for i in range(n):
+        cc = np.unique(c[i])[:per_row]
+        cc = np.where(cc >= i, cc + 1, cc)
+        rows.append(np.full(len(cc), i))
+        cols.append(cc)
+    r = np.concatenate(rows)
+    cl = np.concatenate(cols)
+    M = sp.csr_matrix((-rng.uniform(0.5, 1.5, r.size), (r, cl)), shape=(n, n))
+    d = np.asarray(abs(M).sum(axis=1)).ravel() + 0.5
+    M = (M + sp.diags(d)).tocsr()
+    M.sort_indices()
+    return M.indptr.astype(np.int32), M.indices.astype(np.int32), M.data.copy()
+
+
+@pytest.mark.parametrize("n,per_row", [(6000, 40), (20000, 100)])
+def test_extpi_wave_tiers_match_oracle(dev, n, per_row):
+    """Rows whose distinct P columns overflow the wave kernels' small tables
+    (6000 x 40: a few rows over 128 columns of the 64-entry tier; 20000 x 100:
+    every row over the 256 of the 128-entry tier) are redone with the large
+    tables: S, the C/F split and P (P max 4) equal the oracle's exactly."""
+    print(f"[extpi tiers] n={n} per_row={per_row}", flush=True)
+    I, J, A = _rand_long(n, per_row, 7)
+    dI, dJ, dA = T(I, dev), T(J, dev), T(A, dev)
+    SI, SJ = C.strength(n, dI, dJ, dA, 0.25, 0.9)
+    rSI, rSJ = O.amg_strength(n, I, J, A, 0.25, 0.9)
+    assert np.array_equal(SI.cpu().numpy(), rSI) and np.array_equal(SJ.cpu().numpy(), rSJ)
+    cf, nc = C.pmis(n, SI, SJ, 0)
+    rcf, rnc = O.amg_pmis(n, rSI, rSJ, 0)
+    assert nc == rnc and np.array_equal(cf.cpu().numpy(), rcf)
+    PI, PJ, PA, _ = C.extpi_interp(n, dI, dJ, dA, SI, SJ, cf, 4)
+    rPI, rPJ, rPA, _ = O.amg_extpi(n, I, J, A, rSI, rSJ, rcf, 4)
+    assert np.array_equal(PI.cpu().numpy(), rPI)
+    assert np.array_equal(PJ.cpu().numpy(), rPJ)
+    assert np.array_equal(PA.cpu().numpy(), rPA)
+
+
 def test_pmis_cf_init_rejected(dev):
     n, I, J, A = CASES["7pt_10"]()
     SI, SJ = C.strength(n, T(I, dev), T(J, dev), T(A, dev))
