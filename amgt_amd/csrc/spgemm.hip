@@ -39,6 +39,13 @@ constexpr int kBigCap = 256;    // ... and by the long-row kernel before its glo
 // kCmpVal accumulators and whose blocks fit kCmpBlk: LDS per wave ~16 KB
 constexpr int kCmpBlk = 512;
 constexpr int kCmpVal = 1024;
+constexpr int kCmpValShort = 256;  // ... and rows of at most kNumCap blocks
+
+// a C block row of at most BLK blocks and VAL values (row pointers of C)
+template <int BLK, int VAL>
+__device__ __forceinline__ bool fits_cmp(const int32_t *cptr, const int32_t *cvp, int i) {
+    return cptr[i + 1] - cptr[i] <= BLK && cvp[i + 1] - cvp[i] <= VAL;
+}
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -709,16 +716,15 @@ __global__ __launch_bounds__(256) void k_numeric_lds(int count, const int32_t *_
     const int w = threadIdx.x >> 6;
     const int r = blockIdx.x * 4 + w;
     if (r >= count) return;
+    if (fits_cmp<kNumCap, kCmpValShort>(cptr, cvp, rows[r])) return;  // k_numeric_cmp's row
     numeric_row<VT>(rows[r], g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w], s_st[w]);
 }
 
-// rows of the long bin that fit the compressed accumulators: one wave per row
-// (about 16 KB of LDS per wave, two workgroups per CU)
-__device__ __forceinline__ bool fits_cmp(const int32_t *cptr, const int32_t *cvp, int i) {
-    return cptr[i + 1] - cptr[i] <= kCmpBlk && cvp[i + 1] - cvp[i] <= kCmpVal;
-}
-
-template <typename VT>
+// rows that fit compressed accumulators of BLK blocks and VAL values: one wave
+// per row.  <kCmpBlk, kCmpVal> for the long bin (about 16 KB of LDS per wave,
+// two workgroups per CU); <kNumCap, kCmpValShort> for the bin of at most 64
+// blocks (about 5 KB per wave, against 10 KB for dense blocks)
+template <typename VT, int BLK, int VAL>
 __global__ __launch_bounds__(256) void k_numeric_cmp(int count, const int32_t *__restrict__ rows,
                                                      SpgemmArgs g,
                                                      const int32_t *__restrict__ boff,
@@ -727,16 +733,16 @@ __global__ __launch_bounds__(256) void k_numeric_cmp(int count, const int32_t *_
                                                      const uint16_t *__restrict__ cmap,
                                                      const int32_t *__restrict__ cvp,
                                                      double *__restrict__ cval) {
-    __shared__ double s_acc[4][kCmpVal];
-    __shared__ int32_t s_key[4][kCmpBlk];
-    __shared__ int32_t s_cvo[4][kCmpBlk];
-    __shared__ uint32_t s_cm[4][kCmpBlk];
+    __shared__ double s_acc[4][VAL];
+    __shared__ int32_t s_key[4][BLK];
+    __shared__ int32_t s_cvo[4][BLK];
+    __shared__ uint32_t s_cm[4][BLK];
     __shared__ NumStage s_st[4];
     const int w = threadIdx.x >> 6;
     const int r = blockIdx.x * 4 + w;
     if (r >= count) return;  // whole wave
     const int i = rows[r];
-    if (!fits_cmp(cptr, cvp, i)) return;  // k_numeric_global's row
+    if (!fits_cmp<BLK, VAL>(cptr, cvp, i)) return;  // the dense kernel's row
     numeric_row<VT, true>(i, g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w], s_st[w],
                           s_cvo[w], s_cm[w]);
 }
@@ -761,7 +767,7 @@ __global__ __launch_bounds__(256) void k_numeric_global(int count, const int32_t
     double *acc = slab + int64_t(wave) * slab_elems;
     for (int r = wave; r < count; r += nwaves) {
         const int i = rows[r];
-        if (fits_cmp(cptr, cvp, i)) continue;  // k_numeric_cmp's row
+        if (fits_cmp<kCmpBlk, kCmpVal>(cptr, cvp, i)) continue;  // k_numeric_cmp's row
         if (cptr[i + 1] - cptr[i] <= kBigCap)
             numeric_row<VT>(i, g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w], s_st[w]);
         else
@@ -913,12 +919,16 @@ int launch_numeric(const SpgemmArgs &g, const int32_t *boff, const Mbsr &C, int3
         hipLaunchKernelGGL((k_numeric_reg<VT>), dim3((reg_rows + 3) / 4), dim3(256), 0, s,
                            reg_rows, list, g, boff, C.blk_ptr, C.blk_col, C.blk_map, C.val_ptr,
                            C.val);
-    if (lds_rows > 0)
+    if (lds_rows > 0) {
+        hipLaunchKernelGGL((k_numeric_cmp<VT, kNumCap, kCmpValShort>), dim3((lds_rows + 3) / 4),
+                           dim3(256), 0, s, lds_rows, list + brow, g, boff, C.blk_ptr, C.blk_col,
+                           C.blk_map, C.val_ptr, C.val);
         hipLaunchKernelGGL((k_numeric_lds<VT>), dim3((lds_rows + 3) / 4), dim3(256), 0, s,
                            lds_rows, list + brow, g, boff, C.blk_ptr, C.blk_col, C.blk_map,
                            C.val_ptr, C.val);
+    }
     if (big_rows > 0) {
-        hipLaunchKernelGGL((k_numeric_cmp<VT>), dim3((big_rows + 3) / 4), dim3(256), 0, s,
+        hipLaunchKernelGGL((k_numeric_cmp<VT, kCmpBlk, kCmpVal>), dim3((big_rows + 3) / 4), dim3(256), 0, s,
                            big_rows, list + 2 * int64_t(brow), g, boff, C.blk_ptr, C.blk_col,
                            C.blk_map, C.val_ptr, C.val);
         const int64_t elems = int64_t(C.max_row_blocks) * 16;
