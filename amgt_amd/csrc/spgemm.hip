@@ -400,6 +400,28 @@ __device__ __forceinline__ int row_table(const int64_t *__restrict__ ub_off, int
     return hs < cap ? hs : cap;
 }
 
+// ---- bin 1, rows with ub <= kWaveUb: one wave per row, each with its own LDS
+// table of up to 2 kWaveUb slots (four rows per workgroup)
+constexpr int kWaveUb = 512;
+__global__ __launch_bounds__(256) void k_symbolic_wave(int count, const int32_t *__restrict__ rows,
+                                                       SpgemmArgs g,
+                                                       const int64_t *__restrict__ ub_off,
+                                                       int32_t *tcol, uint32_t *tmap,
+                                                       int32_t *cnt_out, int32_t *nnz_out) {
+    constexpr int HS = 2 * kWaveUb;
+    __shared__ uint32_t s_key[4][HS];
+    __shared__ uint32_t s_map[4][HS];
+    __shared__ int s_excl[4][64], s_q0[4][64], s_ma[4][64];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + w;
+    if (r >= count) return;  // whole wave
+    const int i = rows[r];
+    if (ub_off[i + 1] - ub_off[i] > kWaveUb) return;  // k_symbolic_block's row
+    symbolic_row<64>(i, lane, s_key[w], s_map[w], row_table(ub_off, i, HS), s_excl[w], s_q0[w],
+                     s_ma[w], nullptr, g.ap, g.ac, g.am, g.bp, g.bc, g.bm, ub_off[i], tcol, tmap,
+                     cnt_out, nnz_out);
+}
+
 // ---- bin 1: one workgroup per row, LDS table of up to 4096
 __global__ __launch_bounds__(256) void k_symbolic_block(int count, const int32_t *__restrict__ rows,
                                                         SpgemmArgs g,
@@ -412,6 +434,7 @@ __global__ __launch_bounds__(256) void k_symbolic_block(int count, const int32_t
     __shared__ int s_excl[256], s_q0[256], s_ma[256], s_sc[4];
     if (int(blockIdx.x) >= count) return;
     const int i = rows[blockIdx.x];
+    if (ub_off[i + 1] - ub_off[i] <= kWaveUb) return;  // k_symbolic_wave's row
     symbolic_row<256>(i, threadIdx.x, s_key, s_map, row_table(ub_off, i, HS), s_excl, s_q0, s_ma,
                       s_sc, g.ap, g.ac, g.am,
                       g.bp, g.bc, g.bm, ub_off[i], tcol, tmap, cnt_out, nnz_out);
@@ -1053,9 +1076,12 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
     if (bin[0] > 0)
         hipLaunchKernelGGL(k_symbolic_reg, dim3((bin[0] + 3) / 4), dim3(256), 0, s, bin[0],
                            list, g, ub_off, tcol, tmap, ccnt, cnnz);
-    if (bin[1] > 0)
+    if (bin[1] > 0) {
+        hipLaunchKernelGGL(k_symbolic_wave, dim3((bin[1] + 3) / 4), dim3(256), 0, s, bin[1],
+                           list + brow, g, ub_off, tcol, tmap, ccnt, cnnz);
         hipLaunchKernelGGL(k_symbolic_block, dim3(bin[1]), dim3(256), 0, s, bin[1],
                            list + brow, g, ub_off, tcol, tmap, ccnt, cnnz);
+    }
     if (bin[2] > 0) {
         int hs = 1;
         const int64_t want = std::min<int64_t>(2 * int64_t(ht[0].max_blocks), 2 * int64_t(B.bcol) + 2);
