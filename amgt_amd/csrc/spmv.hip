@@ -494,12 +494,32 @@ __global__ __launch_bounds__(256) void k_spmv_blk(
         }
     }
     __syncthreads();
-    for (int rr = tid; rr < 4 * nr; rr += 256) {
-        const int lr = rr >> 2, r = rr & 3;
+    // row sums: LPR lanes per row when the tile has fewer than 256 rows (long
+    // block rows: lane `sub` adds blocks sub, sub + LPR, ... in ascending
+    // order, then a fixed xor tree combines the lanes: deterministic)
+    constexpr int LPR = 4 * RPB >= 256 ? 1 : 256 / (4 * RPB);
+    if constexpr (LPR == 1) {
+        for (int rr = tid; rr < 4 * nr; rr += 256) {
+            const int lr = rr >> 2, r = rr & 3;
+            AT acc = AT(0);
+            for (int j = s_bp[lr] - pb; j < s_bp[lr + 1] - pb; ++j) acc += s_buf[4 * j + r];
+            const int row = 4 * (br0 + lr) + r;
+            if (row < nrows) y[row] = ep(row, double(acc));
+        }
+    } else {
+        const int rr = tid / LPR, sub = tid % LPR;  // rr < 4 RPB: every thread has a row slot
         AT acc = AT(0);
-        for (int j = s_bp[lr] - pb; j < s_bp[lr + 1] - pb; ++j) acc += s_buf[4 * j + r];
-        const int row = 4 * (br0 + lr) + r;
-        if (row < nrows) y[row] = ep(row, double(acc));
+        if (rr < 4 * nr) {
+            const int lr = rr >> 2, r = rr & 3;
+            for (int j = s_bp[lr] - pb + sub; j < s_bp[lr + 1] - pb; j += LPR)
+                acc += s_buf[4 * j + r];
+        }
+#pragma unroll
+        for (int d = LPR / 2; d >= 1; d >>= 1) acc += __shfl_xor(acc, d);
+        if (sub == 0 && rr < 4 * nr) {
+            const int row = 4 * (br0 + (rr >> 2)) + (rr & 3);
+            if (row < nrows) y[row] = ep(row, double(acc));
+        }
     }
 }
 
@@ -1254,8 +1274,17 @@ int launch_spmm_t(const Mbsr &A, const VT *val, int lanes, int k, double alpha, 
 }  // namespace
 
 int spmv_default_lanes(const Mbsr &A) {
-    // enough slots that a typical block row takes about two block steps
     const double avg = A.brow ? double(A.nblk) / double(A.brow) : 0.0;
+    if (A.spmv_kernel == kSpmvBlock) {
+        // block-lane tiles of 512 blocks: the most block rows per tile (64, 32,
+        // 16 or 8, lanes 4 / 16 / 32 / 64) whose average load stays within 90 %
+        // of the capacity, so that few tiles overflow to the global-memory walk
+        if (avg * 64.0 <= 460.0) return 4;
+        if (avg * 32.0 <= 460.0) return 16;
+        if (avg * 16.0 <= 460.0) return 32;
+        return 64;
+    }
+    // enough slots that a typical block row takes about two block steps
     if (avg <= 8.0) return 4;
     if (avg <= 16.0) return 8;
     if (avg <= 32.0) return 16;
