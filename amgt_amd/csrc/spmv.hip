@@ -647,21 +647,37 @@ __global__ __launch_bounds__(256) void k_spmv_bal(
     const int head = tile_head[t];
     const int nh = head >= 0 ? 1 : 0;
     const int nrow_t = nh + (hi - lo);
-    for (int rr = tid; rr < 4 * nrow_t; rr += 256) {
-        const int idx = rr >> 2, r = rr & 3;
-        const int R = (idx < nh) ? head : lo + idx - nh;
-        const int rb = blk_ptr[R], re = blk_ptr[R + 1];
-        const int j0 = max(rb, q0) - q0, j1 = min(re, q1) - q0;
+    // lanes per row: as many as 256 threads allow for the tile's rows (a tile
+    // of a few long block rows), a power of two; lane `sub` adds the row's
+    // blocks sub, sub + lpr, ... in ascending order, a fixed xor tree combines
+    // the lanes (deterministic).  lpr is the same for the whole workgroup.
+    int lpr = 1;
+    while (lpr < 64 && 8 * nrow_t * lpr <= 256) lpr <<= 1;
+    for (int rr0 = 0; rr0 < 4 * nrow_t * lpr; rr0 += 256) {  // uniform trip count
+        const int rr = (rr0 + tid) / lpr, sub = (rr0 + tid) % lpr;
+        const bool live = rr < 4 * nrow_t;
+        int R = 0, rb = 0, re = 0, r = 0;
         AT acc = AT(0);
-        for (int j = j0; j < j1; ++j) acc += s_buf[4 * j + r];
-        if (rb < q0) {
-            carry[8 * int64_t(t) + r] = double(acc);  // row started in an earlier tile
-        } else if (re > q1) {
-            carry[8 * int64_t(t) + 4 + r] = double(acc);  // row continues in later tiles
-        } else {
-            const int row = 4 * R + r;
-            if (row < nrows) {
-                y[row] = ep(row, double(acc));
+        if (live) {
+            const int idx = rr >> 2;
+            r = rr & 3;
+            R = (idx < nh) ? head : lo + idx - nh;
+            rb = blk_ptr[R];
+            re = blk_ptr[R + 1];
+            const int j0 = max(rb, q0) - q0, j1 = min(re, q1) - q0;
+            for (int j = j0 + sub; j < j1; j += lpr) acc += s_buf[4 * j + r];
+        }
+        for (int d = lpr / 2; d >= 1; d >>= 1) acc += __shfl_xor(acc, d);
+        if (live && sub == 0) {
+            if (rb < q0) {
+                carry[8 * int64_t(t) + r] = double(acc);  // row started in an earlier tile
+            } else if (re > q1) {
+                carry[8 * int64_t(t) + 4 + r] = double(acc);  // row continues in later tiles
+            } else {
+                const int row = 4 * R + r;
+                if (row < nrows) {
+                    y[row] = ep(row, double(acc));
+                }
             }
         }
     }
