@@ -369,7 +369,13 @@ __global__ __launch_bounds__(256) void k_spmv_blk(
     const int npair = (nv - head) >> 1;
     double2 tv[kPair ? VPT / 2 : 1];
     double edge = 0.0;
-    VT ts[kPair ? 1 : VPT];
+    // FP32 / FP16 values: E per 16-byte load from the first index vb + head
+    // that is a multiple of E, the head and tail elements singly
+    constexpr int E = kPair ? 1 : int(16 / sizeof(VT));
+    constexpr int NVPT = kPair ? 1 : (VPT / E > 0 ? VPT / E : 1);
+    uint4 tq[NVPT];
+    VT tsingle = VT(0.0f);
+    int vhead = 0, nvec = 0;
     if constexpr (kPair) {
         // 16-byte loads from the first even value index on, the odd head and
         // tail element singly
@@ -391,11 +397,22 @@ __global__ __launch_bounds__(256) void k_spmv_blk(
         const auto e = __builtin_amdgcn_raw_buffer_load_b64(ers, ((tid & 1) ? nv - 1 : 0) * 8, 0, 0);
         __builtin_memcpy(&edge, &e, 8);
     } else {
+        // (val is 16-byte aligned: handle arrays are).  Raw buffer loads over
+        // exactly the tile's vectors: lanes past the last one read zeros.
+        vhead = (E - (vb % E)) % E;
+        if (vhead > nv) vhead = nv;
+        nvec = (nv - vhead) / E;
+        const __amdgpu_buffer_rsrc_t vrs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<VT *>(val + vb + vhead), short(0), nvec * 16, kBufferDword3);
 #pragma unroll
-        for (int u = 0; u < VPT; ++u) {
-            const int q = u * 256 + tid;
-            if (q < nv) ts[u] = val[vb + q];
+        for (int u = 0; u < NVPT; ++u) {
+            const auto q = __builtin_amdgcn_raw_buffer_load_b128(vrs, (u * 256 + tid) * 16, 0, 0);
+            __builtin_memcpy(&tq[u], &q, 16);
         }
+        // head (threads 0 .. vhead-1) and tail (threads 8 .. 8 + tail - 1) elements
+        const int ntail = nv - vhead - nvec * E;
+        if (tid < vhead) tsingle = val[vb + tid];
+        else if (tid >= 8 && tid < 8 + ntail) tsingle = val[vb + vhead + nvec * E + (tid - 8)];
     }
     // (3) x segments of the blocks
     uint32_t m[BPT];
@@ -441,10 +458,18 @@ __global__ __launch_bounds__(256) void k_spmv_blk(
         }
     } else {
 #pragma unroll
-        for (int u = 0; u < VPT; ++u) {
-            const int q = u * 256 + tid;
-            if (q < nv) s_buf[q] = AT(ts[u]);
+        for (int u = 0; u < NVPT; ++u) {
+            const int p = u * 256 + tid;
+            if (p < nvec) {
+                VT e[E];
+                __builtin_memcpy(e, &tq[u], 16);
+#pragma unroll
+                for (int k = 0; k < E; ++k) s_buf[vhead + E * p + k] = AT(e[k]);
+            }
         }
+        const int ntail = nv - vhead - nvec * E;
+        if (tid < vhead) s_buf[tid] = AT(tsingle);
+        else if (tid >= 8 && tid < 8 + ntail) s_buf[vhead + nvec * E + (tid - 8)] = AT(tsingle);
     }
     // (5) blocks reaching past the last column: x element-wise
 #pragma unroll
@@ -1156,9 +1181,8 @@ int launch_spmv_t(const Mbsr &A, const VT *val, int lanes, const double *x, EP e
             default: return launch_spmv_bal_g<2, VT, AT>(A, val, x, ep, y, s);
         }
     }
-    // FP64 values are staged in 16-byte pieces too (handle arrays always are)
-    const bool val_aligned =
-        sizeof(VT) != 8 || (reinterpret_cast<uintptr_t>(val) & 15u) == 0;
+    // values are staged in 16-byte pieces (handle arrays always are aligned)
+    const bool val_aligned = (reinterpret_cast<uintptr_t>(val) & 15u) == 0;
     if ((A.spmv_kernel == kSpmvBlock || A.spmv_kernel == kSpmvBalanced) && x_aligned &&
         val_aligned) {
         // lanes selects the tile: 4 -> 64 block rows (2 blocks per thread),
