@@ -241,22 +241,20 @@ __device__ __forceinline__ uint32_t h_slot(int k, uint32_t mask) {
     return (uint32_t(k) * 2654435761u) & mask;
 }
 
-__device__ __forceinline__ int h_find(const int32_t *__restrict__ keys,
-                                      const int32_t *__restrict__ vals, uint32_t mask, int k) {
+// slots are (key, value) pairs: one 8-byte load per probe
+__device__ __forceinline__ int h_find(const int2 *__restrict__ kv, uint32_t mask, int k) {
     for (uint32_t h = h_slot(k, mask);; h = (h + 1) & mask) {
-        const int32_t kk = keys[h];
-        if (kk == k) return vals[h];
-        if (kk == -1) return -1;
+        const int2 e = kv[h];
+        if (e.x == k) return e.y;
+        if (e.x == -1) return -1;
     }
 }
 
-__device__ __forceinline__ void h_put(int32_t *__restrict__ keys, int32_t *__restrict__ vals,
-                                      uint32_t mask, int k, int v) {
+__device__ __forceinline__ void h_put(int2 *__restrict__ kv, uint32_t mask, int k, int v) {
     for (uint32_t h = h_slot(k, mask);; h = (h + 1) & mask) {
-        const int32_t kk = keys[h];
+        const int32_t kk = kv[h].x;
         if (kk == -1 || kk == k) {
-            keys[h] = k;
-            vals[h] = v;
+            kv[h] = make_int2(k, v);
             return;
         }
     }
@@ -344,7 +342,7 @@ __global__ __launch_bounds__(kThreads) void k_extpi_row(
     int32_t n, const int32_t *__restrict__ I, const int32_t *__restrict__ J,
     const double *__restrict__ A, const int32_t *__restrict__ SI, const int32_t *__restrict__ SJ,
     const int32_t *__restrict__ cf, const int64_t *__restrict__ off, int max_elmts,
-    const int64_t *__restrict__ hoff, int32_t *__restrict__ hkeys, int32_t *__restrict__ hvals,
+    const int64_t *__restrict__ hoff, int2 *__restrict__ hkv,
     const int32_t *__restrict__ skip, int32_t *__restrict__ sj, double *__restrict__ sw,
     int32_t *__restrict__ cnt) {
     const int i = blockIdx.x * kThreads + threadIdx.x;
@@ -352,8 +350,7 @@ __global__ __launch_bounds__(kThreads) void k_extpi_row(
     if (skip && skip[i]) return;  // a row of the wave kernel
     int32_t *cols = sj + off[i];
     double *w = sw + off[i];
-    int32_t *keys = hkeys + hoff[i];
-    int32_t *vals = hvals + hoff[i];
+    int2 *kv = hkv + hoff[i];
     const uint32_t mask = uint32_t(hoff[i + 1] - hoff[i]) - 1u;
     if (cf[i] >= 0) {  // C point: identity (par_lr_interp.c:1488-1495)
         cols[0] = i;
@@ -370,16 +367,16 @@ __global__ __launch_bounds__(kThreads) void k_extpi_row(
     for (int p = SI[i]; p < SI[i + 1]; ++p) {
         const int i1 = SJ[p];
         if (cf[i1] >= 0) {
-            if (h_find(keys, vals, mask, i1) < 0) {
-                h_put(keys, vals, mask, i1, c);
+            if (h_find(kv, mask, i1) < 0) {
+                h_put(kv, mask, i1, c);
                 cols[c++] = i1;
             }
         } else if (cf[i1] != -3) {
-            h_put(keys, vals, mask, i1, kStrongF);
+            h_put(kv, mask, i1, kStrongF);
             for (int q = SI[i1]; q < SI[i1 + 1]; ++q) {
                 const int k1 = SJ[q];
-                if (cf[k1] >= 0 && h_find(keys, vals, mask, k1) < 0) {
-                    h_put(keys, vals, mask, k1, c);
+                if (cf[k1] >= 0 && h_find(kv, mask, k1) < 0) {
+                    h_put(kv, mask, k1, c);
                     cols[c++] = k1;
                 }
             }
@@ -392,7 +389,7 @@ __global__ __launch_bounds__(kThreads) void k_extpi_row(
     for (int64_t jj = I[i]; jj < I[i + 1]; ++jj) {
         if (jj == di) continue;
         const int i1 = J[jj];
-        const int pos = h_find(keys, vals, mask, i1);
+        const int pos = h_find(kv, mask, i1);
         if (pos >= 0) {
             w[pos] += A[jj];
             continue;
@@ -404,7 +401,7 @@ __global__ __launch_bounds__(kThreads) void k_extpi_row(
             for (int64_t q = I[i1]; q < I[i1 + 1]; ++q) {
                 if (q == d1) continue;
                 const int i2 = J[q];
-                if ((i2 == i || h_find(keys, vals, mask, i2) >= 0) && (sgn * A[q]) < 0)
+                if ((i2 == i || h_find(kv, mask, i2) >= 0) && (sgn * A[q]) < 0)
                     sum += A[q];
             }
             if (sum != 0) {
@@ -413,7 +410,7 @@ __global__ __launch_bounds__(kThreads) void k_extpi_row(
                     if (q == d1) continue;
                     const int i2 = J[q];
                     if ((sgn * A[q]) < 0) {
-                        const int p2 = h_find(keys, vals, mask, i2);
+                        const int p2 = h_find(kv, mask, i2);
                         if (p2 >= 0) w[p2] += distribute * A[q];
                         if (i2 == i) diagonal += distribute * A[q];
                     }
@@ -965,7 +962,7 @@ int amgt_amg_extpi_interp(int32_t n, const int32_t *A_i, const int32_t *A_j,
     if (st == AMGT_OK) st = tmp.get(&sw, size_t(total_ub));
     // per-row tables
     int64_t *hs = nullptr, *hoff = nullptr;
-    int32_t *hkeys = nullptr, *hvals = nullptr;
+    int2 *hkv = nullptr;
     int64_t total_h = 0;
     if (st == AMGT_OK) st = tmp.get(&hs, size_t(n) + 1);
     if (st == AMGT_OK) st = tmp.get(&hoff, size_t(n) + 1);
@@ -980,10 +977,9 @@ int amgt_amg_extpi_interp(int32_t n, const int32_t *A_i, const int32_t *A_j,
             hipStreamSynchronize(s) != hipSuccess)
             st = AMGT_ERR_HIP;
     }
-    if (st == AMGT_OK) st = tmp.get(&hkeys, size_t(total_h));
-    if (st == AMGT_OK) st = tmp.get(&hvals, size_t(total_h));
+    if (st == AMGT_OK) st = tmp.get(&hkv, size_t(total_h));
     if (st == AMGT_OK && total_h > 0 &&
-        hipMemsetAsync(hkeys, 0xff, sizeof(int32_t) * size_t(total_h), s) != hipSuccess)
+        hipMemsetAsync(hkv, 0xff, sizeof(int2) * size_t(total_h), s) != hipSuccess)
         st = AMGT_ERR_HIP;
     // long rows with ascending columns (coarse levels): a wave per row
     int32_t *use = nullptr, *upos = nullptr;
@@ -1060,7 +1056,7 @@ int amgt_amg_extpi_interp(int32_t n, const int32_t *A_i, const int32_t *A_j,
     }
     if (st == AMGT_OK && n > 0) {
         hipLaunchKernelGGL(k_extpi_row, g, b, 0, s, n, A_i, A_j, A_data, S_i, S_j, cf_marker, off,
-                           max_elmts, hoff, hkeys, hvals, nwave > 0 ? use : nullptr, sj, sw, cnt);
+                           max_elmts, hoff, hkv, nwave > 0 ? use : nullptr, sj, sw, cnt);
         if (hipGetLastError() != hipSuccess) st = AMGT_ERR_HIP;
     }
     if (st == AMGT_OK) st = exclusive_scan_i32(cnt, pi, n, s);
