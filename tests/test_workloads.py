@@ -98,3 +98,19 @@ def test_problem_seeds_disjoint():
         W.problem_seed("train", 0, 1024)
     with pytest.raises(ValueError):
         W.problem_seed("test", 50, 1024)
+
+
+def test_grf_problem_regression():
+    """The committed policy labels and weights were produced from these
+    problems: the generator's outputs for two of them stay as recorded
+    (sigma and ell exactly; sums of the matrix and right-hand side to 1e-12)."""
+    I, J, A, b, sigma, ell = W.grf_problem(64, W.problem_seed("train", 0, 64))
+    assert (sigma, ell) == (0.5000195659231486, 0.03499217974720159)
+    assert int(I[-1]) == 20224
+    assert np.isclose(A.sum(), 604.7348820216386, rtol=1e-12)
+    assert np.isclose(np.abs(A).sum(), 36982.36730485499, rtol=1e-12)
+    assert np.isclose(b.sum(), -0.0029478719881590324, rtol=1e-10, atol=1e-15)
+    I, J, A, b, sigma, ell = W.grf_problem(256, W.problem_seed("test", 3, 256))
+    assert (sigma, ell) == (1.975880250323508, 0.019071906227186276)
+    assert np.isclose(np.abs(A).sum(), 3511498.009229469, rtol=1e-12)
+    assert np.isclose(b.sum(), -0.001801997472194001, rtol=1e-10, atol=1e-15)
