@@ -221,6 +221,61 @@ def boomer_timings(dev, n1=128):
     return out
 
 
+def c3_timings(dev, n1=160):
+    """Mixed-precision configuration: 3D anisotropic 7-pt diffusion on n1^3
+    (couplings 1, 1, 0.01), b ~ N(0,1) (Park-Miller seed 2), the algebraic
+    hierarchy (theta 0.25, P max 4, l1-Jacobi 6/7) in plain FP64 and with the
+    mixed schedule (FP64 finest level and Krylov vectors, FP32 levels 1-2,
+    scaled FP16 below, FP64 Galerkin products on the two finest levels), PCG
+    to a 1e-10 relative residual; the true residual ||b - A x|| / ||b|| is
+    recomputed in FP64 on the device, and the mixed solution is compared with
+    the FP64 one.  Best of two runs each."""
+    import numpy as np
+    import torch
+
+    from amgt_amd import amg as G
+    from amgt_amd import workloads as W
+
+    I, J, V = W.stencil(7, n1, n1, n1, c=(1.0, 1.0, 0.01))
+    n = n1 ** 3
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    dI, dJ, dV = t(I), t(J), t(V)
+    db = t(W.rand_normal(2, n))
+    out, sols = {}, {}
+    for name, pv, pg in (("fp64", G.fp64_policy, G.fp64_policy),
+                         ("mixed", G.mixed_spmv_policy, G.mixed_spgemm_policy)):
+        best = None
+        for _ in range(2):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            amg = G.BoomerAMG(dI, dJ, dV, n, dev, max_elmts=4, coarse_size=64, omega=6.0 / 7.0,
+                              spmv_policy=pv, spgemm_policy=pg)
+            amg.capture()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            x, iters, hist = amg.pcg(db, tol=1e-10, max_iter=200)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            r = {"setup_s": t1 - t0, "solve_s": t2 - t1, "total_s": t2 - t0, "iters": iters,
+                 "x": x, "levels": len(amg.levels)}
+            if best is None or r["total_s"] < best["total_s"]:
+                best = r
+            del amg
+        A0 = G.MBSRMatrix.from_csr(dI, dJ, dV, n, n)
+        res = A0.spmv(best["x"], alpha=-1.0, beta=1.0, b=db)
+        true = (torch.linalg.vector_norm(res) / torch.linalg.vector_norm(db)).item()
+        sols[name] = best["x"]
+        out[name] = {"setup_s": round(best["setup_s"], 4), "solve_s": round(best["solve_s"], 4),
+                     "setup_plus_solve_s": round(best["total_s"], 4), "iterations": best["iters"],
+                     "true_rel_residual": true, "levels": best["levels"]}
+    out["mixed"]["max_diff_vs_fp64_solution"] = float(
+        (sols["mixed"] - sols["fp64"]).abs().max() / sols["fp64"].abs().max())
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["setup_plus_solve_s"] /
+                                         out["mixed"]["setup_plus_solve_s"], 3)
+    out["rows"] = n
+    return out
+
+
 def policy_timings(dev, count=5, side=1024):
     """Learned per-level policy on held-out GRF diffusion problems (side^2
     cells; the first `count` of the 50 held-out problems): plain FP64 (theta
@@ -468,6 +523,7 @@ def main():
     batch = None if args.no_amg else batch_timings(dev)
     boomer = None if args.no_amg else boomer_timings(dev)
     policy = None if args.no_amg else policy_timings(dev)
+    c3 = None if args.no_amg else c3_timings(dev)
 
     B = spmv_bytes(info)
     value = B / (ms_per_step * 1e-3) / 1e9
@@ -512,6 +568,8 @@ def main():
         out["amg_boomer_7pt128"] = boomer
     if policy is not None:
         out["learned_policy_grf1024"] = policy
+    if c3 is not None:
+        out["c3_aniso160_mixed"] = c3
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(I, J, A, x_h, B)
     print(json.dumps(out), flush=True)
