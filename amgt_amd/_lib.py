@@ -22,6 +22,8 @@ AMGT_ERR_UNSUPPORTED = 6
 
 AMGT_DIAG_VALUE = 0
 AMGT_DIAG_INV_ABS = 1
+AMGT_DIAG_INV_L1 = 2
+AMGT_DIAG_INV = 3
 AMGT_SPMV_DIRECT = 0
 AMGT_SPMV_STAGED = 1
 AMGT_SPMV_BLOCK = 2
@@ -47,7 +49,8 @@ EXPORTS = (
     "amgt_mbsr_to_csr", "amgt_free", "amgt_csr_spgemm", "amgt_csr_matvec",
     "amgt_mbsr_diagonal", "amgt_mbsr_relax_jacobi", "amgt_mbsr_transpose",
     "amgt_read_ij", "amgt_read_mm", "amgt_host_free", "amgt_mbsr_spmm_interleaved",
-    "amgt_amg_strength", "amgt_amg_pmis", "amgt_amg_extpi_interp",
+    "amgt_amg_strength", "amgt_amg_pmis", "amgt_amg_extpi_interp", "amgt_amg_ruge_first_pass",
+    "amgt_amg_hmis",
     "amgt_policy_work_bytes", "amgt_policy_forward", "amgt_policy_loss_grad", "amgt_policy_adam",
     "amgt_level_stats_work_bytes", "amgt_level_stats", "amgt_interp_trilinear",
 )
@@ -141,6 +144,8 @@ def load():
         "amgt_mbsr_relax_jacobi": (_int, [_vp, _dbl, _vp, _vp, _vp, _vp, _vp]),
         "amgt_amg_strength": (_int, [_i32, _vp, _vp, _vp, _dbl, _dbl, _pi64, _pvp, _pvp, _vp]),
         "amgt_amg_pmis": (_int, [_i32, _vp, _vp, _int, _vp, ctypes.POINTER(_i32), _vp]),
+        "amgt_amg_ruge_first_pass": (_int, [_i32, _vp, _vp, _vp]),
+        "amgt_amg_hmis": (_int, [_i32, _vp, _vp, _vp, ctypes.POINTER(_i32), _vp]),
         "amgt_amg_extpi_interp": (_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _int,
                                          ctypes.POINTER(_i32), _pi64, _pvp, _pvp, _pvp, _vp]),
         "amgt_policy_work_bytes": (ctypes.c_size_t, [_i64]),

@@ -12,24 +12,23 @@ hypre_BoomerAMGBuildCoarseOperatorKT / RAPKT does on the GPU
 (par_csr_triplemat_device.c:844).  HMIS coarsening and ext+i interpolation
 (BoomerAMG's own setup) are not rebuilt here.
 
-Per level: A, P, R as mBSR handles tagged with the level, 1/|a_ii| for the
-smoother (inverse l1 norms of relax 18 on one process), and fixed work
-vectors.  V(1,1):
+Per level: A, P, R as mBSR handles tagged with the level, the inverse of the
+smoother's l1 vector, and fixed work vectors.  V(1,1):
     u  = w D^-1 f                      (first sweep from u = 0: no matvec, as HYPRE)
     r  = f - A u                       (SpMV, residual form)
     f' = R r,  coarse V-cycle -> u'    (SpMV)
     u += P u'                          (SpMV with b aliasing y)
     u2 = u + w D^-1 (f - A u)          (one fused SpMV launch, amgt_mbsr_relax_jacobi)
 The coarsest level (<= coarse_size rows) applies its dense inverse, held as
-an mBSR matrix with full blocks, with one SpMV.  Every step is a library
-kernel on the current stream, so one V-cycle can be captured in a hipGraph
-(torch.cuda.CUDAGraph) and replayed.
+an mBSR matrix with full blocks, with one SpMV (HYPRE: Gaussian elimination,
+relax type 9).  Every step is a library kernel on the current stream, so one
+V-cycle can be captured in a hipGraph (torch.cuda.CUDAGraph) and replayed.
 
-Smoother weight: HYPRE's relax weight is a user setting (default 1); with
-l1 norms reduced to |a_ii| on one process, weight 1 is plain Jacobi, which
-does not damp the highest-frequency mode of the 7-point operator (PCG took 44
-iterations on 24x20x18, 115 on 64^3).  The default here is 6/7, Jacobi's
-optimal smoothing weight for the 3D 7-point Laplacian (11 iterations on both).
+Smoother: relax type 18 (l1-Jacobi, HYPRE's GPU default) or 7 (Jacobi), weight
+w = 1 by default as in HYPRE.  D is what hypre_ParCSRComputeL1Norms gives
+(ams.c:527-800): for relax 18 the l1 norm of the whole row, sum_j |a_ij|
+(option 1, par_amg_setup.c:3393-3406), for relax 7 the diagonal (option 5,
+:3521-3536).
 
 Precision (mixed mode): per-level policies for SpMV/smoothing and for the
 Galerkin SpGEMM; the defaults follow the mixed configuration of the workload
@@ -95,6 +94,10 @@ def interpolation_blockdiag(grids, device):
     """Block-diagonal trilinear P for a batch of independent grids; see
     interpolation_batched.  Returns (I, J, A, ncoarse, coarse grids)."""
     return interpolation_batched(grids, device)
+
+
+# smoother diagonal per relax type (amgt_mbsr_diagonal kinds)
+RELAX_DIAG = {18: L.AMGT_DIAG_INV_L1, 7: L.AMGT_DIAG_INV}
 
 
 def fp64_policy(level: int) -> int:
@@ -174,6 +177,12 @@ class SegmentNorms:
             0, self.chunk_seg, part).sqrt_()
 
 
+def _check_relax(relax_type: int) -> int:
+    if relax_type not in RELAX_DIAG:
+        raise ValueError(f"relax type {relax_type}: 7 (Jacobi) or 18 (l1-Jacobi)")
+    return int(relax_type)
+
+
 class GeometricAMG:
     """Galerkin hierarchy on a structured nx x ny x nz grid, or on a batch of
     independent grids held as one block-diagonal system (a list of grids), and
@@ -181,10 +190,12 @@ class GeometricAMG:
     level is one block-diagonal operator: one launch per step for all the
     problems."""
 
-    def __init__(self, I, J, V, grid, device, omega: float = 6.0 / 7.0, coarse_size: int = 512,
-                 spmv_policy=fp64_policy, spgemm_policy=fp64_policy, max_levels: int = 25):
+    def __init__(self, I, J, V, grid, device, omega: float = 1.0, coarse_size: int = 512,
+                 spmv_policy=fp64_policy, spgemm_policy=fp64_policy, max_levels: int = 25,
+                 relax_type: int = 18):
         self.device = device
         self.omega = float(omega)
+        self.relax_type = _check_relax(relax_type)
         self.coarse_size = int(coarse_size)
         self.spmv_policy = spmv_policy
         self.spgemm_policy = spgemm_policy
@@ -211,8 +222,8 @@ class GeometricAMG:
         lvl = 0
         while True:
             A.set_level(lvl)
-            A.set_precision(self.spmv_policy(lvl))
-            dinv = A.diagonal(inverse_abs=True)
+            A.set_precision(self.level_precision(lvl))
+            dinv = A.diagonal(kind=RELAX_DIAG[self.relax_type])
             L_ = Level(n=n, grid=grid, A=A, dinv=dinv)
             self.levels.append(L_)
             if n <= self.coarse_size or lvl + 1 >= self.max_levels or \
@@ -229,12 +240,16 @@ class GeometricAMG:
             lvl += 1
         self._finish(t0)
 
+    def level_precision(self, lvl: int) -> int:
+        """SpMV / smoother precision of level lvl."""
+        return self.spmv_policy(lvl)
+
     def _galerkin(self, lvl, A, P, R):
         """R (A P) in the level's SpGEMM precision (it follows the left
         operand); the operands return to the level's SpMV precision."""
         for M in (P, R):
             M.set_level(lvl)
-            M.set_precision(self.spmv_policy(lvl))
+            M.set_precision(self.level_precision(lvl))
         torch.cuda.synchronize()
         tg = time.perf_counter()
         A.set_precision(self.spgemm_policy(lvl))
@@ -243,8 +258,8 @@ class GeometricAMG:
         Ac = R.spgemm(AP)
         torch.cuda.synchronize()
         self.times.spgemm_s.append(time.perf_counter() - tg)
-        A.set_precision(self.spmv_policy(lvl))
-        R.set_precision(self.spmv_policy(lvl))
+        A.set_precision(self.level_precision(lvl))
+        R.set_precision(self.level_precision(lvl))
         return Ac
 
     def _finish(self, t0):
@@ -315,17 +330,26 @@ class GeometricAMG:
         torch.cuda.graph's context manager also runs gc.collect() and
         empty_cache() (10-40 ms here, against 0.9 ms for capture and
         instantiation of the cycle), and the cycle allocates nothing."""
+        import gc
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         g = torch.cuda.CUDAGraph()
         with torch.cuda.stream(s):
             for _ in range(warmup):
                 self._vcycle(0)
-            g.capture_begin()
+            # no collection during the capture: a handle freed then would
+            # synchronise the device inside it
+            gc_was_on = gc.isenabled()
+            gc.disable()
             try:
-                out = self._vcycle(0)
+                g.capture_begin()
+                try:
+                    out = self._vcycle(0)
+                finally:
+                    g.capture_end()
             finally:
-                g.capture_end()
+                if gc_was_on:
+                    gc.enable()
         torch.cuda.current_stream().wait_stream(s)
         self._graph, self._graph_out = g, out
         return g
@@ -382,7 +406,7 @@ class GeometricAMG:
 
     def describe(self) -> list:
         return [{"level": i, "rows": L_.n, "nnz": L_.A.info()["nnz"],
-                 "spmv_precision": self.spmv_policy(i),
+                 "spmv_precision": self.level_precision(i),
                  "spgemm_precision": self.spgemm_policy(i) if L_.P is not None else None}
                 for i, L_ in enumerate(self.levels)]
 
@@ -390,31 +414,37 @@ class GeometricAMG:
 class BoomerAMG(GeometricAMG):
     """Algebraic Galerkin hierarchy built on the device the way BoomerAMG builds
     it on one process (par_amg_setup.c:994-3182): per level the strength matrix
-    (theta, max row sum), PMIS coarsening (CF_init 0: coarsen type 8, 2: type
-    9), extended+i interpolation truncated to max_elmts entries per row, R =
-    P^T (mBSR transpose) and R (A P) on the mBSR SpGEMM.  Coarsening stops at
-    n <= coarse_size, at max_levels, or when a level would not shrink (no C
-    point or all C points); the coarsest level is solved with its dense inverse.
-    The smoother is l1-Jacobi (relax type 18; on one process the l1 norm of a
-    row is |a_ii|) with weight omega; the V-cycle and PCG are the same as for
-    the geometric hierarchy.
+    (theta, max row sum), HMIS coarsening (coarsen type 10, HYPRE's default:
+    the Ruge first pass on the host, PMIS with CF_init 1 on the device) or PMIS
+    (coarsen="pmis"; CF_init 0: coarsen type 8, 2: type 9), extended+i
+    interpolation truncated to max_elmts entries per row, R = P^T (mBSR
+    transpose) and R (A P) on the mBSR SpGEMM.  Coarsening stops at n <=
+    coarse_size, at max_levels, or when a level would not shrink (no C point or
+    all C points); the coarsest level is solved with its dense inverse.  The
+    defaults are ij's (test/ij.c:244-290, 356): theta 0.25, max row sum 1.0,
+    P max 4, coarse size 9, relax type 18 with weight 1.  The V-cycle and PCG
+    are the same as for the geometric hierarchy.
 
     `level_hook(lvl, n_prev, A, CI, CJ, CV) -> (theta, precision)`, when given,
     is called for every level before it is coarsened and sets that level's
     strength threshold and SpMV/smoother precision (replacing `theta` and
     `spmv_policy`); the Galerkin products keep `spgemm_policy`."""
 
-    def __init__(self, I, J, V, n, device, theta: float = 0.25, max_row_sum: float = 0.9,
+    def __init__(self, I, J, V, n, device, theta: float = 0.25, max_row_sum: float = 1.0,
                  cf_init: int = 0, max_elmts: int = 4, omega: float = 1.0, coarse_size: int = 9,
                  spmv_policy=fp64_policy, spgemm_policy=fp64_policy, max_levels: int = 25,
-                 level_hook=None):
+                 level_hook=None, coarsen: str = "hmis", relax_type: int = 18):
+        if coarsen not in ("hmis", "pmis"):
+            raise ValueError(f"coarsen {coarsen!r}: 'hmis' or 'pmis'")
         self.device = device
         self.omega = float(omega)
+        self.relax_type = _check_relax(relax_type)
+        self.coarsen = coarsen
         self.coarse_size = int(coarse_size)
         self.level_hook = level_hook
         self.level_theta: list[float] = []
         self.level_prec: list[int] = []
-        self.spmv_policy = (lambda lvl: self.level_prec[lvl]) if level_hook else spmv_policy
+        self.spmv_policy = spmv_policy
         self.spgemm_policy = spgemm_policy
         self.max_levels = max_levels
         self.theta, self.max_row_sum = float(theta), float(max_row_sum)
@@ -444,8 +474,8 @@ class BoomerAMG(GeometricAMG):
             else:
                 theta = self.theta
                 self.level_theta.append(theta)
-            A.set_precision(self.spmv_policy(lvl))
-            dinv = A.diagonal(inverse_abs=True)
+            A.set_precision(self.level_precision(lvl))
+            dinv = A.diagonal(kind=RELAX_DIAG[self.relax_type])
             L_ = Level(n=n, grid=[], A=A, dinv=dinv)
             self.levels.append(L_)
             if n <= self.coarse_size or lvl + 1 >= self.max_levels:
@@ -453,7 +483,10 @@ class BoomerAMG(GeometricAMG):
             torch.cuda.synchronize()
             tc = time.perf_counter()
             SI, SJ = C.strength(n, CI, CJ, CV, theta, self.max_row_sum)
-            cf, nc = C.pmis(n, SI, SJ, self.cf_init)
+            if self.coarsen == "hmis":
+                cf, nc = C.hmis(n, SI, SJ)
+            else:
+                cf, nc = C.pmis(n, SI, SJ, self.cf_init)
             if nc == 0 or nc == n:
                 break
             PI, PJ, PA, nc = C.extpi_interp(n, CI, CJ, CV, SI, SJ, cf, self.max_elmts)
@@ -471,6 +504,11 @@ class BoomerAMG(GeometricAMG):
             lvl += 1
         self._finish(t0)
 
+    def level_precision(self, lvl: int) -> int:
+        if self.level_hook is not None:
+            return self.level_prec[lvl]
+        return self.spmv_policy(lvl)
+
     def complexities(self):
         """(grid complexity, operator complexity) as BoomerAMG reports them."""
         ns = [L_.n for L_ in self.levels]
@@ -480,10 +518,11 @@ class BoomerAMG(GeometricAMG):
 
 def solve_poisson_boomer(nx: int, ny: int, nz: int, b: np.ndarray, device, mixed: bool = False,
                          tol: float = 1e-8, capture: bool = True, max_elmts: int = 4,
-                         coarse_size: int = 64, theta: float = 0.25, max_row_sum: float = 0.9,
-                         omega: float = 1.0):
-    """7-pt Poisson with the algebraic hierarchy (device strength, PMIS,
-    extended+i): setup + PCG/V(1,1) solve; timings as solve_poisson."""
+                         coarse_size: int = 9, theta: float = 0.25, max_row_sum: float = 1.0,
+                         omega: float = 1.0, coarsen: str = "hmis", relax_type: int = 18):
+    """7-pt Poisson with the algebraic hierarchy (device strength, HMIS or
+    PMIS, extended+i; ij's defaults): setup + PCG/V(1,1) solve; timings as
+    solve_poisson."""
     I, J, V = W.stencil(7, nx, ny, nz)
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
     dI, dJ, dV = t(I), t(J), t(V)
@@ -492,7 +531,8 @@ def solve_poisson_boomer(nx: int, ny: int, nz: int, b: np.ndarray, device, mixed
     amg = BoomerAMG(dI, dJ, dV, nx * ny * nz, device, theta=theta, max_row_sum=max_row_sum,
                     cf_init=0, max_elmts=max_elmts, omega=omega, coarse_size=coarse_size,
                     spmv_policy=mixed_spmv_policy if mixed else fp64_policy,
-                    spgemm_policy=mixed_spgemm_policy if mixed else fp64_policy)
+                    spgemm_policy=mixed_spgemm_policy if mixed else fp64_policy,
+                    coarsen=coarsen, relax_type=relax_type)
     tc = time.perf_counter()
     if capture:
         amg.capture()
@@ -505,7 +545,7 @@ def solve_poisson_boomer(nx: int, ny: int, nz: int, b: np.ndarray, device, mixed
 
 
 def solve_poisson(nx: int, ny: int, nz: int, b: np.ndarray, device, mixed: bool = False,
-                  tol: float = 1e-8, capture: bool = True, omega: float = 6.0 / 7.0,
+                  tol: float = 1e-8, capture: bool = True, omega: float = 1.0,
                   coarse_size: int = 64):
     """7-pt Poisson on nx x ny x nz: setup + PCG/V(1,1) solve; returns a dict of
     timings and the solution (the timed solve includes the residual checks)."""
@@ -554,7 +594,7 @@ def batch_poisson2d(sizes, seed: int, device):
 
 
 def solve_batch(sizes, seed, device, tol: float = 1e-8, capture: bool = True,
-                coarse_size: int = 512, omega: float = 6.0 / 7.0):
+                coarse_size: int = 512, omega: float = 1.0):
     """All problems of a batch in one block-diagonal setup and PCG/V(1,1) solve."""
     I, J, V, b, xs, grids, seg = batch_poisson2d(sizes, seed, device)
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
@@ -578,7 +618,7 @@ def batch_sizes(count: int = 48, seed: int = 4, lo: int = 32, hi: int = 512):
 
 
 def solve_one_by_one(sizes, seed, device, tol: float = 1e-8, coarse_size: int = 512,
-                     omega: float = 6.0 / 7.0):
+                     omega: float = 1.0):
     """The same problems solved one after the other (one hierarchy each)."""
     I, J, V, b, xs, grids, seg = batch_poisson2d(sizes, seed, device)
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731

@@ -1,5 +1,5 @@
-"""BoomerAMG setup on the device: strength, PMIS coarsening and extended+i
-interpolation (libamgt_mbsr.so's amgt_amg_* entry points).
+"""BoomerAMG setup on the device: strength, PMIS and HMIS coarsening and
+extended+i interpolation (libamgt_mbsr.so's amgt_amg_* entry points).
 
 Counterparts of hypre_BoomerAMGCreateS (par_strength.c:531, device version
 par_strength_device.c:34), hypre_BoomerAMGCoarsenPMIS (par_coarsen.c:2816,
@@ -53,11 +53,42 @@ def strength(n: int, I: torch.Tensor, J: torch.Tensor, A: torch.Tensor, theta: f
 def pmis(n: int, SI: torch.Tensor, SJ: torch.Tensor, cf_init: int = 0, stream=None):
     """C/F splitting (1 C, -1 F, -3 no strong dependence) and the C count
     (par_coarsen.c:2101; CF_init 0 or 2)."""
+    if cf_init not in (0, 2):
+        raise ValueError("pmis: CF_init 0 or 2 (use hmis for coarsen type 10)")
     _check_csr(n, SI, SJ)
     cf = torch.empty(max(n, 1), dtype=torch.int32, device=SI.device)
     nc = ctypes.c_int32()
     L.check(L.load().amgt_amg_pmis(n, _p(SI), _p(SJ), int(cf_init), _p(cf), ctypes.byref(nc),
                                    _stream_ptr(stream)), "amgt_amg_pmis")
+    return cf[:n], int(nc.value)
+
+
+def ruge_first_pass(n: int, SI, SJ):
+    """Markers of the Ruge-Stueben first pass as HMIS runs it (coarsen type 10,
+    par_coarsen.c:911): 1 C, -1 F, -2 Z, -3 no strong dependence.  Host
+    arrays (numpy int32) in and out."""
+    import numpy as np
+    SI = np.ascontiguousarray(SI, dtype=np.int32)
+    SJ = np.ascontiguousarray(SJ, dtype=np.int32)
+    if SI.size != n + 1:
+        raise ValueError(f"row pointer has {SI.size} entries, expected {n + 1}")
+    if SJ.size < int(SI[n]):
+        raise ValueError("column array shorter than the row pointer says")
+    cf = np.zeros(max(n, 1), np.int32)
+    vp = lambda a: ctypes.c_void_p(a.ctypes.data if a.size else 0)  # noqa: E731
+    L.check(L.load().amgt_amg_ruge_first_pass(n, vp(SI), vp(SJ), vp(cf)),
+            "amgt_amg_ruge_first_pass")
+    return cf[:n]
+
+
+def hmis(n: int, SI: torch.Tensor, SJ: torch.Tensor, stream=None):
+    """HMIS C/F splitting (coarsen type 10, par_coarsen.c:2845): the Ruge first
+    pass on the host, PMIS with CF_init 1 on the device.  (cf, C count)."""
+    _check_csr(n, SI, SJ)
+    cf = torch.empty(max(n, 1), dtype=torch.int32, device=SI.device)
+    nc = ctypes.c_int32()
+    L.check(L.load().amgt_amg_hmis(n, _p(SI), _p(SJ), _p(cf), ctypes.byref(nc),
+                                   _stream_ptr(stream)), "amgt_amg_hmis")
     return cf[:n], int(nc.value)
 
 

@@ -141,6 +141,42 @@ __global__ __launch_bounds__(kThreads) void k_pmis_init(int32_t n, const int32_t
     }
 }
 
+// CF_init 1 (HMIS, par_coarsen.c:2353-2386): cf holds the markers of the Ruge
+// first pass.  C points stay C and join the graph; F points are undecided
+// again; a Z point (it influences nobody) joins the graph as undecided if its
+// measure reaches 1 or it depends on some point, else it is F; points without
+// strong dependence stay out.  Graph membership is a positive measure, so the
+// points left out get measure 0 (the host leaves a Z point's measure below 1,
+// which no comparison with > 1 can see).
+__global__ __launch_bounds__(kThreads) void k_pmis_init_hmis(int32_t n,
+                                                            const int32_t *__restrict__ SI,
+                                                            const int32_t *__restrict__ cnt,
+                                                            double *__restrict__ measure,
+                                                            int32_t *__restrict__ cf,
+                                                            int32_t *__restrict__ any) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const double r = double(pm_jump(2747u, uint64_t(i) + 1u)) / 2147483647.0;
+    const double m = double(cnt[i]) + r;
+    int c = cf[i];
+    if (c == -3) {
+        measure[i] = 0.0;
+        return;
+    }
+    if (c == -1) c = 0;
+    if (c == -2) {
+        if (!(m >= 1.0) && SI[i + 1] == SI[i]) {
+            cf[i] = -1;
+            measure[i] = 0.0;
+            return;
+        }
+        c = 0;
+    }
+    cf[i] = c;
+    measure[i] = m;
+    *any = 1;
+}
+
 // independent set, first half (par_coarsen.c:2510-2556): a node of the graph
 // with measure > 1 loses against any S neighbour (either direction) with a
 // larger measure > 1; the host's clearing stores are idempotent, so plain
@@ -488,6 +524,14 @@ __device__ __forceinline__ int64_t find_in_row(const int32_t *__restrict__ J, in
     return -1;
 }
 
+// position of column col (!= r) in row r of a row that k_rows_unsorted let
+// through: ascending, or the diagonal first and the rest ascending
+__device__ __forceinline__ int64_t find_off(const int32_t *__restrict__ J, int64_t b, int64_t e,
+                                            int r, int col) {
+    if (b < e && J[b] == r) ++b;
+    return find_in_row(J, b, e, col);
+}
+
 template <int WH>
 __device__ __forceinline__ int lds_find(const int32_t *hkey, const int32_t *hval, int k) {
     for (uint32_t h = h_slot(k, WH - 1);; h = (h + 1) & (WH - 1)) {
@@ -629,7 +673,7 @@ __global__ __launch_bounds__(64) void k_extpi_wave(
                 info = v;
             } else if (v == kStrongF) {
                 const int64_t b1 = I[i1], e1 = I[i1 + 1];
-                const int64_t d1 = find_in_row(J, b1, e1, i1);
+                const int64_t d1 = diag_pos(I, J, i1);
                 sgn = (d1 >= 0 && A[d1] < 0) ? -1 : 1;
                 double sum = 0.0;
                 for (int64_t q = b1; q < e1; ++q) {
@@ -641,7 +685,7 @@ __global__ __launch_bounds__(64) void k_extpi_wave(
                 if (sum != 0) {
                     info = -2;
                     dist = s.aval[jj] / sum;
-                    const int64_t qi = find_in_row(J, b1, e1, i);
+                    const int64_t qi = find_off(J, b1, e1, i1, i);
                     if (qi >= 0 && (sgn * A[qi]) < 0) dterm = dist * A[qi];
                 } else {
                     info = -3;
@@ -666,7 +710,7 @@ __global__ __launch_bounds__(64) void k_extpi_wave(
                 acc += s.aval[jj];
             } else if (info == -2) {
                 const int i1 = s.acol[jj];
-                const int64_t q = find_in_row(J, I[i1], I[i1 + 1], col);
+                const int64_t q = find_off(J, I[i1], I[i1 + 1], i1, col);
                 if (q >= 0 && (s.jsgn[jj] * A[q]) < 0) acc += s.jdist[jj] * A[q];
             }
         }
@@ -716,16 +760,20 @@ __global__ __launch_bounds__(64) void k_extpi_wave(
 }
 #pragma clang fp contract(fast)
 
-// rows whose columns are not ascending (the diagonal may sit anywhere)
+// rows the wave kernels cannot search: a row must be ascending, or hold its
+// diagonal first and the rest ascending (HYPRE's layout); find_off and
+// diag_pos rely on exactly that
 __global__ __launch_bounds__(kThreads) void k_rows_unsorted(int32_t n, const int32_t *__restrict__ I,
                                                            const int32_t *__restrict__ J,
                                                            int32_t *__restrict__ flag) {
     const int i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
+    int64_t p = I[i];
+    const int64_t e = I[i + 1];
+    if (p < e && J[p] == i) ++p;
     int prev = -1;
-    for (int64_t p = I[i]; p < I[i + 1]; ++p) {
+    for (; p < e; ++p) {
         const int c = J[p];
-        if (c == i) continue;
         if (c <= prev) {
             *flag = 1;
             return;
@@ -860,8 +908,8 @@ int amgt_amg_pmis(int32_t n, const int32_t *S_i, const int32_t *S_j, int cf_init
     AMGT_CHECK_ARG(n_coarse, "NULL output argument");
     AMGT_CHECK_ARG(n >= 0, "negative size");
     AMGT_CHECK_ARG(n == 0 || (S_i && cf_marker), "NULL row pointer or marker array");
-    if (cf_init != 0 && cf_init != 2) {
-        set_error("PMIS: only CF_init 0 and 2 (coarsen types 8 and 9) are supported");
+    if (cf_init != 0 && cf_init != 1 && cf_init != 2) {
+        set_error("PMIS: only CF_init 0, 1 (HMIS) and 2 (coarsen types 8, 10 and 9) are supported");
         return AMGT_ERR_UNSUPPORTED;
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -878,7 +926,10 @@ int amgt_amg_pmis(int32_t n, const int32_t *S_i, const int32_t *S_j, int cf_init
     AMGT_HIP(hipMemsetAsync(any, 0, sizeof(int32_t), s));
     const dim3 g(grid_of(n)), b(kThreads);
     hipLaunchKernelGGL(k_col_count, g, b, 0, s, n, S_i, S_j, cnt);
-    hipLaunchKernelGGL(k_pmis_init, g, b, 0, s, n, S_i, cnt, measure, cf_marker, any);
+    if (cf_init == 1)
+        hipLaunchKernelGGL(k_pmis_init_hmis, g, b, 0, s, n, S_i, cnt, measure, cf_marker, any);
+    else
+        hipLaunchKernelGGL(k_pmis_init, g, b, 0, s, n, S_i, cnt, measure, cf_marker, any);
     AMGT_HIP(hipGetLastError());
     int32_t h_any = 0;
     AMGT_HIP(hipMemcpyAsync(&h_any, any, sizeof(int32_t), hipMemcpyDeviceToHost, s));

@@ -557,7 +557,9 @@ int amgt_mbsr_transpose(amgt_mbsr_t A, void *stream, amgt_mbsr_t *AT) {
 
 int amgt_mbsr_diagonal(amgt_mbsr_t A, int kind, double *d_diag, void *stream) {
     AMGT_CHECK_ARG(A != nullptr, "handle is NULL");
-    AMGT_CHECK_ARG(kind == AMGT_DIAG_VALUE || kind == AMGT_DIAG_INV_ABS, "unknown diagonal kind");
+    AMGT_CHECK_ARG(kind == AMGT_DIAG_VALUE || kind == AMGT_DIAG_INV_ABS ||
+                       kind == AMGT_DIAG_INV_L1 || kind == AMGT_DIAG_INV,
+                   "unknown diagonal kind");
     AMGT_CHECK_ARG(!A->m.pattern_only, "pattern-only matrix has no values");
     const int32_t n = A->m.nrows < A->m.ncols ? A->m.nrows : A->m.ncols;
     AMGT_CHECK_ARG(d_diag != nullptr || n == 0, "output is NULL");

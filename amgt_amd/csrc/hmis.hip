@@ -1,0 +1,237 @@
+// HMIS coarsening (hypre_BoomerAMGCoarsenHMIS, par_coarsen.c:2845-2866): the
+// first pass of Ruge-Stueben coarsening on the host, then PMIS with CF_init 1
+// on the device (amgt_amg_pmis).
+//
+// The first pass (hypre_BoomerAMGCoarsenRuge par_coarsen.c:911-1398 with
+// coarsen_type 10, measure_type 0, one process) is a greedy sweep: the
+// undecided point of largest measure (number of points it influences plus
+// the F points that depend on it) becomes C, the points it influences become
+// F, and measures are updated around them.  Which point is "largest" among
+// equals decides the whole splitting, so the order is HYPRE's: points wait
+// in one first-in-first-out list per measure, a point whose measure changes
+// goes to the end of its new list, and the head of the largest non-empty list
+// is taken (the behaviour of HYPRE's list of lists, utilities/amg_linklist.c).
+// HYPRE keeps the lists in a linked list searched from the largest measure;
+// here they are an array indexed by measure, so every step is O(1).  Points
+// that influence nobody are marked Z (-2) and left for PMIS, which re-examines
+// them, keeps the C points and decides the F points again
+// (par_coarsen.c:2353-2386, first pass of its loop at :2494).
+//
+// The sweep is sequential by nature, as in HYPRE (the GPU build of HYPRE has
+// no HMIS).  S goes to pinned host memory, the markers come back, and the
+// rest runs on the device.
+#include "internal.hpp"
+
+#include <mutex>
+#include <vector>
+
+namespace amgt {
+
+namespace {
+
+constexpr int32_t kC = 1, kF = -1, kZ = -2, kSF = -3, kUndecided = 0;
+
+// first-in-first-out lists of points, one per measure (measures > 0)
+class MeasureLists {
+  public:
+    explicit MeasureLists(int32_t n) : next_(static_cast<size_t>(n)), prev_(static_cast<size_t>(n)) {}
+
+    void enter(int32_t m, int32_t p) {
+        if (size_t(m) >= head_.size()) {
+            head_.resize(size_t(m) * 2 + 16, -1);
+            tail_.resize(head_.size(), -1);
+        }
+        next_[p] = -1;
+        if (tail_[m] < 0) {
+            head_[m] = p;
+            prev_[p] = -1;
+        } else {
+            next_[tail_[m]] = p;
+            prev_[p] = tail_[m];
+        }
+        tail_[m] = p;
+        if (m > top_) top_ = m;
+    }
+
+    void remove(int32_t m, int32_t p) {
+        if (prev_[p] >= 0) next_[prev_[p]] = next_[p]; else head_[m] = next_[p];
+        if (next_[p] >= 0) prev_[next_[p]] = prev_[p]; else tail_[m] = prev_[p];
+    }
+
+    // oldest point of the largest measure, or -1 when every list is empty
+    int32_t first() {
+        while (top_ > 0 && head_[top_] < 0) --top_;
+        return top_ > 0 ? head_[top_] : -1;
+    }
+
+  private:
+    std::vector<int32_t> head_, tail_, next_, prev_;
+    int32_t top_ = 0;
+};
+
+void ruge_first_pass(int32_t n, const int32_t *SI, const int32_t *SJ, int32_t *cf) {
+    // S^T: the points each point influences, in ascending order
+    std::vector<int32_t> ti(static_cast<size_t>(n) + 1, 0), tj(static_cast<size_t>(SI[n]));
+    for (int64_t p = 0; p < SI[n]; ++p) ++ti[size_t(SJ[p]) + 1];
+    for (int32_t i = 0; i < n; ++i) ti[i + 1] += ti[i];
+    {
+        std::vector<int32_t> pos(ti.begin(), ti.end() - 1);
+        for (int32_t i = 0; i < n; ++i)
+            for (int32_t p = SI[i]; p < SI[i + 1]; ++p) tj[size_t(pos[SJ[p]]++)] = i;
+    }
+    std::vector<int32_t> meas(static_cast<size_t>(n));
+    int64_t left = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        meas[i] = ti[i + 1] - ti[i];
+        if (SI[i + 1] == SI[i]) {
+            cf[i] = kSF;
+            meas[i] = 0;
+        } else {
+            cf[i] = kUndecided;
+            ++left;
+        }
+    }
+    MeasureLists L(n);
+    for (int32_t j = 0; j < n; ++j) {
+        if (cf[j] == kSF) continue;
+        if (meas[j] > 0) {
+            L.enter(meas[j], j);
+            continue;
+        }
+        cf[j] = kZ;
+        for (int32_t k = SI[j]; k < SI[j + 1]; ++k) {
+            const int32_t nb = SJ[k];
+            if (cf[nb] == kSF) continue;
+            if (nb < j) {
+                if (meas[nb] > 0) L.remove(meas[nb], nb);
+                L.enter(++meas[nb], nb);
+            } else {
+                ++meas[nb];
+            }
+        }
+        --left;
+    }
+    // an undecided point gains one for each point that depends on it and turned F
+    auto bump_dependencies = [&](int32_t f) {
+        for (int32_t k = SI[f]; k < SI[f + 1]; ++k) {
+            const int32_t q = SJ[k];
+            if (cf[q] != kUndecided) continue;
+            L.remove(meas[q], q);
+            L.enter(++meas[q], q);
+        }
+    };
+    while (left > 0) {
+        const int32_t c = L.first();
+        if (c < 0) break;
+        cf[c] = kC;
+        L.remove(meas[c], c);
+        meas[c] = 0;
+        --left;
+        for (int32_t p = ti[c]; p < ti[c + 1]; ++p) {
+            const int32_t f = tj[p];
+            if (cf[f] != kUndecided) continue;
+            cf[f] = kF;
+            L.remove(meas[f], f);
+            --left;
+            bump_dependencies(f);
+        }
+        for (int32_t p = SI[c]; p < SI[c + 1]; ++p) {
+            const int32_t q = SJ[p];
+            if (cf[q] != kUndecided) continue;
+            L.remove(meas[q], q);
+            if (--meas[q] > 0) {
+                L.enter(meas[q], q);
+            } else {
+                cf[q] = kF;
+                --left;
+                bump_dependencies(q);
+            }
+        }
+    }
+}
+
+// pinned staging for S and the markers, grown as needed, kept for the process
+struct Staging {
+    std::mutex mu;
+    void *buf = nullptr;
+    size_t bytes = 0;
+    int get(size_t need, void **out) {
+        if (need > bytes) {
+            if (buf) (void)hipHostFree(buf);
+            buf = nullptr;
+            bytes = 0;
+            const size_t sz = need + need / 4;
+            AMGT_HIP(hipHostMalloc(&buf, sz, hipHostMallocDefault));
+            bytes = sz;
+        }
+        *out = buf;
+        return AMGT_OK;
+    }
+};
+
+Staging &staging() {
+    static Staging s;
+    return s;
+}
+
+}  // namespace
+
+}  // namespace amgt
+
+using namespace amgt;
+
+extern "C" {
+
+int amgt_amg_ruge_first_pass(int32_t n, const int32_t *S_i, const int32_t *S_j,
+                             int32_t *cf_marker) {
+    AMGT_CHECK_ARG(n >= 0, "negative size");
+    AMGT_CHECK_ARG(n == 0 || (S_i && cf_marker), "NULL row pointer or marker array");
+    if (n == 0) return AMGT_OK;
+    AMGT_CHECK_ARG(S_i[n] == 0 || S_j, "NULL column array");
+    for (int32_t i = 0; i < n; ++i)
+        if (S_i[i + 1] < S_i[i]) {
+            set_error("Ruge first pass: row pointer not ascending");
+            return AMGT_ERR_INPUT;
+        }
+    for (int64_t p = 0; p < S_i[n]; ++p)
+        if (S_j[p] < 0 || S_j[p] >= n) {
+            set_error("Ruge first pass: column out of range");
+            return AMGT_ERR_INPUT;
+        }
+    ruge_first_pass(n, S_i, S_j, cf_marker);
+    return AMGT_OK;
+}
+
+int amgt_amg_hmis(int32_t n, const int32_t *S_i, const int32_t *S_j, int32_t *cf_marker,
+                  int32_t *n_coarse, void *stream) {
+    AMGT_CHECK_ARG(n_coarse, "NULL output argument");
+    AMGT_CHECK_ARG(n >= 0, "negative size");
+    AMGT_CHECK_ARG(n == 0 || (S_i && cf_marker), "NULL row pointer or marker array");
+    *n_coarse = 0;
+    if (n == 0) return AMGT_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    Staging &st = staging();
+    std::lock_guard<std::mutex> lock(st.mu);
+    int32_t nnz = 0;
+    AMGT_HIP(hipMemcpyAsync(&nnz, S_i + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    AMGT_HIP(hipStreamSynchronize(s));
+    AMGT_CHECK_ARG(nnz == 0 || S_j, "NULL column array");
+    void *buf = nullptr;
+    AMGT_TRY(st.get(sizeof(int32_t) * (2 * size_t(n) + 1 + size_t(nnz)), &buf));
+    int32_t *hi = static_cast<int32_t *>(buf);
+    int32_t *hj = hi + n + 1;
+    int32_t *hcf = hj + nnz;
+    AMGT_HIP(hipMemcpyAsync(hi, S_i, sizeof(int32_t) * (size_t(n) + 1), hipMemcpyDeviceToHost, s));
+    if (nnz) AMGT_HIP(hipMemcpyAsync(hj, S_j, sizeof(int32_t) * size_t(nnz), hipMemcpyDeviceToHost, s));
+    AMGT_HIP(hipStreamSynchronize(s));
+    AMGT_TRY(amgt_amg_ruge_first_pass(n, hi, hj, hcf));
+    AMGT_HIP(hipMemcpyAsync(cf_marker, hcf, sizeof(int32_t) * size_t(n), hipMemcpyHostToDevice, s));
+    // amgt_amg_pmis synchronises the stream before it returns successfully; on
+    // an error the copy out of the staging buffer is waited for here, so the
+    // buffer is free again when the lock is released
+    const int status = amgt_amg_pmis(n, S_i, S_j, 1, cf_marker, n_coarse, stream);
+    if (status != AMGT_OK) (void)hipStreamSynchronize(s);
+    return status;
+}
+
+}  // extern "C"

@@ -1,13 +1,17 @@
-// Smoother support for the V-cycle: the matrix diagonal and the weighted
-// Jacobi sweep fused into the SpMV epilogue (spmv.hip, EpJacobi).
+// Smoother support for the V-cycle: the matrix diagonal, the l1 norms of the
+// rows, and the weighted Jacobi sweep fused into the SpMV epilogue
+// (spmv.hip, EpJacobi).
 //
-// HYPRE's relax type 18 on the GPU (hypre_BoomerAMGRelax18WeightedL1Jacobi ->
-// hypre_BoomerAMGRelax7Jacobi, par_relax.c:338-355, 1178-1250) is three
-// kernels: Vtemp = f, Vtemp = w f - w A u (matvec), u += Vtemp ./ l1_norms.
-// Here one SpMV launch writes u_new = u + w (f - A u) ./ l1 directly.  The l1
-// norms of relax 18 (hypre_ParCSRComputeL1Norms option 4, ams.c:659-690) are
-// |a_ii| plus half the l1 norm of the off-processor part; on one GPU there is
-// no off-processor part, so l1_i = |a_ii|.
+// HYPRE's relax types 7 and 18 on one process (hypre_BoomerAMGRelax7Jacobi,
+// par_relax.c:1178-1250; relax 18 calls it too, :338-355) are three steps:
+// Vtemp = f, Vtemp = w f - w A u (matvec), u += Vtemp ./ l1.  Here one SpMV
+// launch writes u_new = u + w (f - A u) .* (1 ./ l1) directly.  The l1 vector
+// comes from hypre_ParCSRComputeL1Norms (ams.c:527-800) as par_amg_setup.c
+// selects it:
+//  * relax 18: option 1 (par_amg_setup.c:3393-3406), the l1 norm of the whole
+//    row, sum_j |a_ij| (ams.c:593-602), negated where a_ii < 0 (:786-799);
+//  * relax 7: option 5 (par_amg_setup.c:3521-3536), the diagonal a_ii, with 1
+//    where it is 0 (ams.c:700-725).
 #include "internal.hpp"
 
 namespace amgt {
@@ -28,6 +32,19 @@ __global__ __launch_bounds__(256) void k_diagonal(int32_t n, int kind,
     const int br = row >> 2, rr = row & 3;
     const int p0 = blk_ptr[br], p1 = blk_ptr[br + 1];
     double d = 0.0;
+    double l1 = 0.0;
+    if (kind == AMGT_DIAG_INV_L1) {
+        // sum of |a_ij| over row `row`: the row's nibble of every block
+        int off = val_ptr[br];
+        for (int q = p0; q < p1; ++q) {
+            const uint32_t m = blk_map[q];
+            const uint32_t nib = (m >> (4 * rr)) & 0xFu;
+            int v = off + __popc(m & ((1u << (4 * rr)) - 1u));
+            for (int c = 0; c < 4; ++c)
+                if ((nib >> c) & 1u) l1 += fabs(val[v++]);
+            off += __popc(m);
+        }
+    }
     int lo = p0, hi = p1;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
@@ -42,7 +59,14 @@ __global__ __launch_bounds__(256) void k_diagonal(int32_t n, int kind,
             d = val[off + __popc(m & ((1u << bit) - 1u))];
         }
     }
-    if (kind == AMGT_DIAG_INV_ABS) d = (d != 0.0) ? 1.0 / fabs(d) : 0.0;
+    if (kind == AMGT_DIAG_INV_ABS) {
+        d = (d != 0.0) ? 1.0 / fabs(d) : 0.0;
+    } else if (kind == AMGT_DIAG_INV_L1) {
+        if (d < 0.0) l1 = -l1;
+        d = (l1 != 0.0) ? 1.0 / l1 : 0.0;
+    } else if (kind == AMGT_DIAG_INV) {
+        d = (d != 0.0) ? 1.0 / d : 1.0;
+    }
     out[row] = d;
 }
 

@@ -919,6 +919,12 @@ struct NumericBin {
     }
 };
 
+__global__ __launch_bounds__(256) void k_widen(int32_t n, const int32_t *__restrict__ in,
+                                               int64_t *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i];
+}
+
 // popcount of every block bitmap (values are stored in block order, so the
 // exclusive scan of these is each block's value offset)
 __global__ __launch_bounds__(256) void k_block_popc(int64_t nblk, const uint16_t *__restrict__ map,
@@ -1104,6 +1110,32 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
                            retry);
     }
     SG_HIP(hipGetLastError());
+    // C's block and value offsets are int32: nblk <= total_ub and nnz <= 16 nblk,
+    // so only a product whose bound reaches 2^31 values needs its exact totals,
+    // summed in 64 bits before anything of C is allocated
+    if (16 * total_ub >= (int64_t(1) << 31)) {
+        int64_t *wide = nullptr, *wscan = nullptr;
+        SG_TRY(tmp_alloc_t(&wide, size_t(brow) + 1, s));
+        SG_TRY(tmp_alloc_t(&wscan, size_t(brow) + 1, s));
+        int64_t tot[2] = {0, 0};
+        const int32_t *cnts[2] = {ccnt, cnnz};
+        for (int w = 0; w < 2; ++w) {
+            if (brow > 0)
+                hipLaunchKernelGGL(k_widen, dim3((brow + 255) / 256), dim3(256), 0, s, brow,
+                                   cnts[w], wide);
+            SG_HIP(hipGetLastError());
+            SG_TRY(exclusive_scan_i64(wide, wscan, brow, s));
+            SG_HIP(hipMemcpyAsync(&tot[w], wscan + brow, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        }
+        SG_HIP(hipStreamSynchronize(s));
+        tmp_free(wide, s);
+        tmp_free(wscan, s);
+        if (tot[0] >= (int64_t(1) << 31) || tot[1] >= (int64_t(1) << 31)) {
+            set_error("SpGEMM: the product has 2^31 or more blocks or values (int32 offsets)");
+            cleanup();
+            return AMGT_ERR_UNSUPPORTED;
+        }
+    }
     SG_TRY(exclusive_scan_i32(ccnt, C.blk_ptr, brow, s));
     SG_TRY(exclusive_scan_i32(cnnz, C.val_ptr, brow, s));
     // C block counts: rows over kNumCap go to the global-memory numeric kernel;

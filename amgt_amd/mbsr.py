@@ -44,6 +44,33 @@ def _need_cuda(t, name, dtype=None):
         raise ValueError(f"{name} must be contiguous")
 
 
+def _dev_key(d: torch.device):
+    if d.type == "cuda" and d.index is None:
+        return d.type, torch.cuda.current_device()
+    return d.type, d.index
+
+
+def _need_len(t, name, n, device):
+    """t (already type-checked) is on `device` and holds at least n entries
+    (the kernels index n of them): a shorter or foreign tensor would be read or
+    written out of bounds, so it is refused before any launch."""
+    if t is None:
+        return
+    if _dev_key(t.device) != _dev_key(device):
+        raise ValueError(f"{name} is on {t.device}, the matrix on {device}")
+    if t.numel() < n:
+        raise ValueError(f"{name} holds {t.numel()} entries, {n} needed")
+
+
+def _need_rows_len(t, name, k, n, device):
+    if t is None:
+        return
+    if _dev_key(t.device) != _dev_key(device):
+        raise ValueError(f"{name} is on {t.device}, the matrix on {device}")
+    if t.shape[0] != k or (k > 0 and t.shape[1] < n):
+        raise ValueError(f"{name} has shape {tuple(t.shape)}, ({k}, >= {n}) needed")
+
+
 def copy_from_device_ptr(ptr: int, n: int, dtype, device, stream=None, free=True):
     """Copies n elements at a library-owned device pointer into a new tensor and
     (optionally) frees the pointer through the library allocator."""
@@ -156,6 +183,7 @@ class MBSRMatrix:
         """y = alpha*op(A)*x + beta*b (b defaults to y)."""
         nrows, ncols = self.shape
         ny = ncols if trans else nrows
+        nx = nrows if trans else ncols
         _need_cuda(x, "x", torch.float64)
         if y is None:
             y = torch.empty(ny, dtype=torch.float64, device=self.device)
@@ -163,19 +191,27 @@ class MBSRMatrix:
         if b is None and beta != 0.0:
             b = y
         _need_cuda(b, "b", torch.float64)
+        _need_len(x, "x", nx, self.device)
+        _need_len(y, "y", ny, self.device)
+        if beta != 0.0:
+            _need_len(b, "b", ny, self.device)
         st = L.load().amgt_mbsr_spmv(self.handle, int(bool(trans)), float(alpha), _ptr(x),
                                      float(beta), _ptr(b), _ptr(y), _stream_ptr(stream))
         L.check(st, "amgt_mbsr_spmv")
         return y
 
     def diagonal(self, inverse_abs: bool = False, out: torch.Tensor | None = None,
-                 stream=None) -> torch.Tensor:
-        """a_ii, or 1/|a_ii| (the inverse l1 norms of relax type 18 on one process)."""
+                 stream=None, kind: int | None = None) -> torch.Tensor:
+        """a_ii; 1/|a_ii| with inverse_abs; or the given amgt_mbsr_diagonal kind
+        (L.AMGT_DIAG_INV_L1: inverse l1 norms of relax type 18, L.AMGT_DIAG_INV:
+        inverse diagonal of relax type 7)."""
         n = min(self.shape)
         if out is None:
             out = torch.empty(n, dtype=torch.float64, device=self.device)
         _need_cuda(out, "out", torch.float64)
-        kind = L.AMGT_DIAG_INV_ABS if inverse_abs else L.AMGT_DIAG_VALUE
+        _need_len(out, "out", n, self.device)
+        if kind is None:
+            kind = L.AMGT_DIAG_INV_ABS if inverse_abs else L.AMGT_DIAG_VALUE
         L.check(L.load().amgt_mbsr_diagonal(self.handle, kind, _ptr(out), _stream_ptr(stream)),
                 "amgt_mbsr_diagonal")
         return out
@@ -190,6 +226,8 @@ class MBSRMatrix:
         if out is None:
             out = torch.empty(n, dtype=torch.float64, device=self.device)
         _need_cuda(out, "out", torch.float64)
+        for t, name in ((f, "f"), (u, "u"), (dinv, "dinv"), (out, "out")):
+            _need_len(t, name, n, self.device)
         L.check(L.load().amgt_mbsr_relax_jacobi(self.handle, float(omega), _ptr(dinv), _ptr(f),
                                                  _ptr(u), _ptr(out), _stream_ptr(stream)),
                 "amgt_mbsr_relax_jacobi")
@@ -211,6 +249,10 @@ class MBSRMatrix:
         if B is None and beta != 0.0:
             B = Y
         _need_rows(B, "B")
+        _need_rows_len(X, "X", k, nrows if trans else ncols, self.device)
+        _need_rows_len(Y, "Y", k, ny, self.device)
+        if beta != 0.0:
+            _need_rows_len(B, "B", k, ny, self.device)
         st = L.load().amgt_mbsr_spmm(self.handle, int(bool(trans)), k, float(alpha), _ptr(X),
                                      X.stride(0), float(beta), _ptr(B),
                                      B.stride(0) if B is not None else 0, _ptr(Y), Y.stride(0),
@@ -226,6 +268,8 @@ class MBSRMatrix:
         nrows, ncols = self.shape
         ny = ncols if trans else nrows
         _need_cuda(X, "X", torch.float64)
+        if X.dim() != 2:
+            raise ValueError("X must be (n, k)")
         k = X.shape[1]
         if Y is None:
             Y = torch.empty((ny, k), dtype=torch.float64, device=self.device)
@@ -233,6 +277,13 @@ class MBSRMatrix:
         if B is None and beta != 0.0:
             B = Y
         _need_cuda(B, "B", torch.float64)
+        nx = nrows if trans else ncols
+        for t, name, rows in ((X, "X", nx), (Y, "Y", ny), (B if beta != 0.0 else None, "B", ny)):
+            if t is None:
+                continue
+            if t.dim() != 2 or t.shape[1] != k or t.shape[0] < rows:
+                raise ValueError(f"{name} has shape {tuple(t.shape)}, (>= {rows}, {k}) needed")
+            _need_len(t, name, rows * k, self.device)
         st = L.load().amgt_mbsr_spmm_interleaved(self.handle, int(bool(trans)), k, float(alpha),
                                                  _ptr(X), float(beta), _ptr(B), _ptr(Y),
                                                  _stream_ptr(stream))

@@ -50,7 +50,8 @@ FEATURE_NAMES = ("log10_rows_rel", "log10_nnz_rel", "nnz_per_row", "block_fill",
                  "level", "coarsening_ratio", "mean_diag_dominance", "frac_rows_not_dominant",
                  "frac_positive_offdiag", "log10_value_range", "std_log10_diag")
 WEIGHTS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "policy_weights.json")
-AMG_ARGS = dict(max_row_sum=0.9, cf_init=0, max_elmts=4, omega=6.0 / 7.0, coarse_size=64)
+AMG_ARGS = dict(max_row_sum=0.9, cf_init=0, max_elmts=4, omega=6.0 / 7.0, coarse_size=64,
+                coarsen="pmis", relax_type=7)
 
 
 # ---- features ------------------------------------------------------------------------------

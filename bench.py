@@ -189,9 +189,10 @@ def amg_timings(dev, n1=128):
 
 def boomer_timings(dev, n1=128):
     """The same 3D 7-pt Poisson 128^3 solve with BoomerAMG's algebraic hierarchy
-    built on the device: strength (theta 0.25, max row sum 0.9), PMIS, extended+i
-    with at most 4 entries per row, R = P^T, R (A P); l1-Jacobi weight 6/7
-    V(1,1), PCG to 1e-8; FP64 against mixed precision."""
+    built on the device with ij's defaults: strength (theta 0.25, max row sum
+    1), HMIS, extended+i with at most 4 entries per row, R = P^T, R (A P);
+    l1-Jacobi (relax 18) weight 1 V(1,1), Gaussian elimination below 10 rows,
+    PCG to 1e-8; FP64 against mixed precision."""
     import numpy as np
     import scipy.sparse as sp
 
@@ -206,7 +207,7 @@ def boomer_timings(dev, n1=128):
     sols = {}
 
     def run(name):
-        r = G.solve_poisson_boomer(n1, n1, n1, b, dev, mixed=name == "mixed", omega=6.0 / 7.0)
+        r = G.solve_poisson_boomer(n1, n1, n1, b, dev, mixed=name == "mixed")
         r["total_s"] = r["setup"].total_s + r["capture_s"] + r["solve_s"]
         return r
 
@@ -236,16 +237,18 @@ def boomer_timings(dev, n1=128):
         np.max(np.abs(sols["mixed"] - sols["fp64"])) / np.max(np.abs(sols["fp64"])))
     out["speedup_mixed_vs_fp64"] = round(out["fp64"]["setup_plus_solve_s"] /
                                          out["mixed"]["setup_plus_solve_s"], 3)
-    out["hierarchy"] = ("algebraic on the device: strength, PMIS (coarsen type 8), extended+i "
-                        "(P max 4), R = P^T, Galerkin R(AP) on the mBSR SpGEMM; dense inverse "
-                        "at <= 64 rows")
+    out["hierarchy"] = ("algebraic on the device: strength, HMIS (coarsen type 10: Ruge first "
+                        "pass on the host, PMIS on the device), extended+i (P max 4), R = P^T, "
+                        "Galerkin R(AP) on the mBSR SpGEMM, l1-Jacobi weight 1; dense inverse "
+                        "at <= 9 rows")
     return out
 
 
 def c3_timings(dev, n1=160):
     """Mixed-precision configuration: 3D anisotropic 7-pt diffusion on n1^3
     (couplings 1, 1, 0.01), b ~ N(0,1) (Park-Miller seed 2), the algebraic
-    hierarchy (theta 0.25, P max 4, l1-Jacobi 6/7) in plain FP64 and with the
+    hierarchy (ij's defaults: HMIS, theta 0.25, P max 4, l1-Jacobi weight 1) in
+    plain FP64 and with the
     mixed schedule (FP64 finest level and Krylov vectors, FP32 levels 1-2,
     scaled FP16 below, FP64 Galerkin products on the two finest levels), PCG
     to a 1e-10 relative residual; the true residual ||b - A x|| / ||b|| is
@@ -271,8 +274,7 @@ def c3_timings(dev, n1=160):
         pv, pg = policies[name]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        amg = G.BoomerAMG(dI, dJ, dV, n, dev, max_elmts=4, coarse_size=64, omega=6.0 / 7.0,
-                          spmv_policy=pv, spgemm_policy=pg)
+        amg = G.BoomerAMG(dI, dJ, dV, n, dev, spmv_policy=pv, spgemm_policy=pg)
         amg.capture()
         torch.cuda.synchronize()
         t1 = time.perf_counter()

@@ -139,10 +139,16 @@ int amgt_mbsr_spmm(amgt_mbsr_t A, int trans, int k, double alpha, const double *
 int amgt_mbsr_transpose(amgt_mbsr_t A, void *stream, amgt_mbsr_t *AT);
 
 /* ---- smoother support (V-cycle) ---- */
-enum { AMGT_DIAG_VALUE = 0, AMGT_DIAG_INV_ABS = 1 };
-/* d[i] = a_ii (AMGT_DIAG_VALUE) or 1/|a_ii| (AMGT_DIAG_INV_ABS, 0 where a_ii = 0) for
- * i < min(nrows, ncols).  1/|a_ii| is the inverse l1 norm of relax type 18 on one
- * process (hypre_ParCSRComputeL1Norms option 4, ams.c:659-690). */
+enum { AMGT_DIAG_VALUE = 0, AMGT_DIAG_INV_ABS = 1, AMGT_DIAG_INV_L1 = 2, AMGT_DIAG_INV = 3 };
+/* For i < min(nrows, ncols):
+ *   AMGT_DIAG_VALUE    d[i] = a_ii
+ *   AMGT_DIAG_INV_ABS  d[i] = 1/|a_ii| (0 where a_ii = 0)
+ *   AMGT_DIAG_INV_L1   d[i] = 1/l1_i with l1_i = sum_j |a_ij|, negated where a_ii < 0 (0
+ *                      where the row is empty): the inverse l1 norms of relax type 18
+ *                      (hypre_ParCSRComputeL1Norms option 1, ams.c:593-602, 786-799, as
+ *                      par_amg_setup.c:3393-3406 selects it)
+ *   AMGT_DIAG_INV      d[i] = 1/a_ii (1 where a_ii = 0): the diagonal of relax type 7
+ *                      (option 5, ams.c:700-725, par_amg_setup.c:3521-3536) */
 int amgt_mbsr_diagonal(amgt_mbsr_t A, int kind, double *d_diag, void *stream);
 /* One weighted l1-Jacobi sweep, u_new = u + omega * dinv .* (f - A u), fused into a
  * single SpMV launch (relax type 18: hypre_BoomerAMGRelax18WeightedL1Jacobi ->
@@ -216,10 +222,26 @@ int amgt_amg_strength(int32_t n, const int32_t *A_i, const int32_t *A_j, const d
                       double theta, double max_row_sum, int64_t *nnzS, int32_t **S_i,
                       int32_t **S_j, void *stream);
 /* hypre_BoomerAMGCoarsenPMISDevice, par_coarsen_device.c:30 (CF_init 0 or 2,
- * coarsen types 8/9): cf_marker[n] (caller's device array) = 1 C, -1 F,
- * -3 no strong dependence; measures |S^T row| + hypre_Rand() from seed 2747. */
+ * coarsen types 8/9; CF_init 1 below): cf_marker[n] (caller's device array) = 1 C,
+ * -1 F, -3 no strong dependence; measures |S^T row| + hypre_Rand() from seed 2747. */
 int amgt_amg_pmis(int32_t n, const int32_t *S_i, const int32_t *S_j, int cf_init,
                   int32_t *cf_marker, int32_t *n_coarse, void *stream);
+/* PMIS with CF_init 1 (hypre_BoomerAMGCoarsenPMISHost, par_coarsen.c:2353-2386, 2494)
+ * reads cf_marker as the markers of amgt_amg_ruge_first_pass (1 C, -1 F, -2 Z,
+ * -3 no strong dependence) and overwrites it with the splitting. */
+/* First pass of Ruge-Stueben coarsening as HMIS runs it, on HOST arrays
+ * (hypre_BoomerAMGCoarsenRuge, par_coarsen.c:911-1398, coarsen_type 10, measure_type 0,
+ * one process): cf_marker[n] = 1 C, -1 F, -2 Z (influences no point), -3 no strong
+ * dependence.  The undecided point of largest measure is taken first, the oldest among
+ * equals, as HYPRE's list of lists does (utilities/amg_linklist.c). */
+int amgt_amg_ruge_first_pass(int32_t n, const int32_t *S_i, const int32_t *S_j,
+                             int32_t *cf_marker);
+/* hypre_BoomerAMGCoarsenHMIS, par_coarsen.c:2845-2866 (coarsen type 10; HYPRE's default):
+ * the first pass on the host (S is copied to pinned host memory and the markers back),
+ * then amgt_amg_pmis with CF_init 1 on the device.  Device S and cf_marker[n]; result
+ * markers as amgt_amg_pmis.  Synchronises the stream. */
+int amgt_amg_hmis(int32_t n, const int32_t *S_i, const int32_t *S_j, int32_t *cf_marker,
+                  int32_t *n_coarse, void *stream);
 /* hypre_BoomerAMGBuildExtPIInterpDevice, par_lr_interp_device.c:991, with
  * hypre_BoomerAMGInterpTruncationDevice (max_elmts > 0, trunc_factor 0):
  * P (n x n_coarse) in CSR, row entries in HYPRE's host order. */

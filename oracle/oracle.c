@@ -682,13 +682,37 @@ int orc_amg_pmis(int n, const int *SI, const int *SJ, int cf_init, int *cf)
     int *graph2 = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
     orc_amg_pmis_measures(n, SI, SJ, measure);
     int gsize = 0;
-    for (int i = 0; i < n; i++) {
-        cf[i] = 0;
-        if (SI[i + 1] - SI[i] == 0) { /* :2385-2396, no strong dependence */
-            cf[i] = (cf_init == 3 || cf_init == 4) ? 1 : -3;
-            measure[i] = 0.0;
-        } else {
-            graph[gsize++] = i;
+    if (cf_init == 1) {
+        /* :2353-2386, markers of the Ruge first pass (HMIS): F points are
+         * undecided again, C points stay C, a Z point (no influence) goes back
+         * to the graph if its measure reaches 1 or it depends on some point,
+         * else it is F; special F points stay out of the graph */
+        for (int i = 0; i < n; i++) {
+            if (cf[i] != -3) {
+                if (cf[i] == -1) cf[i] = 0;
+                if (cf[i] == -2) {
+                    if (measure[i] >= 1.0 || SI[i + 1] - SI[i] > 0) {
+                        cf[i] = 0;
+                        graph[gsize++] = i;
+                    } else {
+                        cf[i] = -1;
+                    }
+                } else {
+                    graph[gsize++] = i;
+                }
+            } else {
+                measure[i] = 0.0;
+            }
+        }
+    } else {
+        for (int i = 0; i < n; i++) {
+            cf[i] = 0;
+            if (SI[i + 1] - SI[i] == 0) { /* :2385-2396, no strong dependence */
+                cf[i] = (cf_init == 3 || cf_init == 4) ? 1 : -3;
+                measure[i] = 0.0;
+            } else {
+                graph[gsize++] = i;
+            }
         }
     }
     int iter = 0;
@@ -738,6 +762,303 @@ int orc_amg_pmis(int n, const int *SI, const int *SJ, int cf_init, int *cf)
     free(graph);
     free(graph2);
     return nc;
+}
+
+/* Points kept in lists by measure, with the behaviour of hypre_LinkList
+ * (utilities/amg_linklist.c:40-260): the lists are chained in descending
+ * measure, a point entering a list goes to its tail, a list that loses its
+ * last point is dropped.  next[p] / prev[p] link the points of one list
+ * (-2 ends a list, -1 starts it). */
+typedef struct amg_list {
+    int measure, head, tail;
+    struct amg_list *up, *down; /* larger / smaller measure */
+} amg_list;
+
+typedef struct {
+    amg_list *top, *bottom;
+    int *next, *prev;
+} amg_lists;
+
+static void amg_lists_remove(amg_lists *L, int measure, int p)
+{
+    for (amg_list *l = L->top; l; l = l->down) {
+        if (l->measure != measure) continue;
+        if (l->head == p && l->tail == p) {
+            if (l->up) l->up->down = l->down; else L->top = l->down;
+            if (l->down) l->down->up = l->up; else L->bottom = l->up;
+            free(l);
+        } else if (l->head == p) {
+            l->head = L->next[p];
+            L->prev[L->next[p]] = -1;
+        } else if (l->tail == p) {
+            l->tail = L->prev[p];
+            L->next[L->prev[p]] = -2;
+        } else {
+            L->next[L->prev[p]] = L->next[p];
+            L->prev[L->next[p]] = L->prev[p];
+        }
+        return;
+    }
+}
+
+static amg_list *amg_list_new(int measure, int p, amg_lists *L)
+{
+    amg_list *l = (amg_list *)calloc(1, sizeof(amg_list));
+    l->measure = measure;
+    l->head = l->tail = p;
+    L->next[p] = -2;
+    L->prev[p] = -1;
+    return l;
+}
+
+static void amg_lists_enter(amg_lists *L, int measure, int p)
+{
+    for (amg_list *l = L->top; l; l = l->down) {
+        if (measure > l->measure) { /* a new list above l */
+            amg_list *e = amg_list_new(measure, p, L);
+            e->up = l->up;
+            e->down = l;
+            if (l->up) l->up->down = e; else L->top = e;
+            l->up = e;
+            return;
+        }
+        if (measure == l->measure) { /* to the tail of l */
+            L->next[l->tail] = p;
+            L->prev[p] = l->tail;
+            L->next[p] = -2;
+            l->tail = p;
+            return;
+        }
+    }
+    amg_list *e = amg_list_new(measure, p, L); /* below every list */
+    e->up = L->bottom;
+    if (L->bottom) L->bottom->down = e; else L->top = e;
+    L->bottom = e;
+}
+
+/* The first pass of Ruge-Stueben coarsening as HMIS runs it:
+ * hypre_BoomerAMGCoarsenRuge par_coarsen.c:911-1398 with coarsen_type 10
+ * (:1119-1123: points that influence nobody become Z points, -2, and the
+ * routine returns after the first pass, :1391-1398), measure_type 0,
+ * cut_factor 0, one process, a fresh marker array.  cf[i] = 1 (C), -1 (F),
+ * -2 (Z), -3 (no strong dependence). */
+void orc_amg_ruge_first_pass(int n, const int *SI, const int *SJ, int *cf)
+{
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    int *STI = (int *)calloc(nn + 1, sizeof(int));
+    int *STJ = (int *)malloc(sizeof(int) * (size_t)(SI[n] > 0 ? SI[n] : 1));
+    int *meas = (int *)malloc(sizeof(int) * nn);
+    amg_lists L = {NULL, NULL, (int *)malloc(sizeof(int) * nn), (int *)malloc(sizeof(int) * nn)};
+    /* S^T (:1069-1094): rows of S^T in ascending order of the rows of S */
+    for (int64_t p = 0; p < SI[n]; p++) STI[SJ[p] + 1]++;
+    for (int i = 0; i < n; i++) STI[i + 1] += STI[i];
+    {
+        int *pos = (int *)malloc(sizeof(int) * nn);
+        for (int i = 0; i < n; i++) pos[i] = STI[i];
+        for (int i = 0; i < n; i++)
+            for (int p = SI[i]; p < SI[i + 1]; p++) STJ[pos[SJ[p]]++] = i;
+        free(pos);
+    }
+    for (int i = 0; i < n; i++) meas[i] = STI[i + 1] - STI[i]; /* :1108-1111 */
+    int num_left = 0;                                            /* :1176-1201 */
+    for (int j = 0; j < n; j++) {
+        if (SI[j + 1] - SI[j] == 0) {
+            cf[j] = -3;
+            meas[j] = 0;
+        } else {
+            cf[j] = 0;
+            num_left++;
+        }
+    }
+    for (int j = 0; j < n; j++) { /* :1222-1266 */
+        if (cf[j] == -3) continue;
+        if (meas[j] > 0) {
+            amg_lists_enter(&L, meas[j], j);
+            continue;
+        }
+        cf[j] = -2;
+        for (int k = SI[j]; k < SI[j + 1]; k++) {
+            const int nb = SJ[k];
+            if (cf[nb] == -3) continue;
+            if (nb < j) {
+                if (meas[nb] > 0) amg_lists_remove(&L, meas[nb], nb);
+                amg_lists_enter(&L, ++meas[nb], nb);
+            } else {
+                ++meas[nb];
+            }
+        }
+        --num_left;
+    }
+    while (num_left > 0 && L.top) { /* :1288-1366 */
+        const int index = L.top->head;
+        cf[index] = 1;
+        const int m = meas[index];
+        meas[index] = 0;
+        --num_left;
+        amg_lists_remove(&L, m, index);
+        for (int p = STI[index]; p < STI[index + 1]; p++) {
+            const int nb = STJ[p];
+            if (cf[nb] != 0) continue;
+            cf[nb] = -1;
+            amg_lists_remove(&L, meas[nb], nb);
+            --num_left;
+            for (int k = SI[nb]; k < SI[nb + 1]; k++) {
+                const int n2 = SJ[k];
+                if (cf[n2] != 0) continue;
+                amg_lists_remove(&L, meas[n2], n2);
+                amg_lists_enter(&L, ++meas[n2], n2);
+            }
+        }
+        for (int p = SI[index]; p < SI[index + 1]; p++) {
+            const int nb = SJ[p];
+            if (cf[nb] != 0) continue;
+            amg_lists_remove(&L, meas[nb], nb);
+            const int mm = --meas[nb];
+            if (mm > 0) {
+                amg_lists_enter(&L, mm, nb);
+            } else {
+                cf[nb] = -1;
+                --num_left;
+                for (int k = SI[nb]; k < SI[nb + 1]; k++) {
+                    const int n2 = SJ[k];
+                    if (cf[n2] != 0) continue;
+                    amg_lists_remove(&L, meas[n2], n2);
+                    amg_lists_enter(&L, ++meas[n2], n2);
+                }
+            }
+        }
+    }
+    while (L.top) {
+        amg_list *d = L.top->down;
+        free(L.top);
+        L.top = d;
+    }
+    free(L.next);
+    free(L.prev);
+    free(meas);
+    free(STI);
+    free(STJ);
+}
+
+/* hypre_BoomerAMGCoarsenHMIS par_coarsen.c:2845-2866: the Ruge first pass,
+ * then PMIS with CF_init 1 on its markers; returns the number of C points */
+int orc_amg_hmis(int n, const int *SI, const int *SJ, int *cf)
+{
+    orc_amg_ruge_first_pass(n, SI, SJ, cf);
+    return orc_amg_pmis(n, SI, SJ, 1, cf);
+}
+
+/* The Galerkin product as HYPRE's host setup forms it,
+ * hypre_BoomerAMGBuildCoarseOperatorKT par_rap.c:1525-1930 on one process
+ * (square RAP, R = P^T by hypre_CSRMatrixTranspose, so a row of R lists its
+ * fine rows in ascending order): row ic starts with its diagonal (zero,
+ * :1607-1613); row ic of R*A is accumulated over R's row in order and each A
+ * row in its stored order, a new column taking the next slot (:1759-1788);
+ * then the RA row's entries, in that order, are multiplied into P's rows in
+ * their stored order, a new column taking the next slot of the RAP row
+ * (:1855-1890).  Products r*a and (r*a)*p are rounded separately (no fused
+ * multiply-add).  Pass 1 (fill = 0) counts RAP_i[nc+1] and returns nnz; pass 2
+ * fills RAP_j/RAP_a. */
+#pragma GCC push_options
+#pragma GCC optimize("fp-contract=off")
+static int64_t amg_rap_kt(int n, int nc, const int *AI, const int *AJ, const double *AA,
+                          const int *PI, const int *PJ, const double *PA, int *RAPI, int *RAPJ,
+                          double *RAPA, int fill)
+{
+    /* R = P^T */
+    int *RI = (int *)calloc((size_t)nc + 1, sizeof(int));
+    const int64_t nnzP = PI[n];
+    int *RJ = (int *)malloc(sizeof(int) * (size_t)(nnzP > 0 ? nnzP : 1));
+    double *RA = (double *)malloc(sizeof(double) * (size_t)(nnzP > 0 ? nnzP : 1));
+    for (int64_t p = 0; p < nnzP; p++) RI[PJ[p] + 1]++;
+    for (int c = 0; c < nc; c++) RI[c + 1] += RI[c];
+    int *pos = (int *)malloc(sizeof(int) * (size_t)(nc > 0 ? nc : 1));
+    for (int c = 0; c < nc; c++) pos[c] = RI[c];
+    for (int i = 0; i < n; i++)
+        for (int p = PI[i]; p < PI[i + 1]; p++) {
+            RJ[pos[PJ[p]]] = i;
+            RA[pos[PJ[p]]++] = fill ? PA[p] : 0.0;
+        }
+    free(pos);
+    int64_t *amark = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    int64_t *pmark = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nc > 0 ? nc : 1));
+    int *raj = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    double *rav = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    for (int i = 0; i < n; i++) amark[i] = -1;
+    for (int c = 0; c < nc; c++) pmark[c] = -1;
+    int64_t cnt = 0;
+    for (int ic = 0; ic < nc; ic++) {
+        const int64_t row_begin = cnt;
+        if (!fill) RAPI[ic] = (int)cnt;
+        else cnt = RAPI[ic];
+        const int64_t rb = fill ? RAPI[ic] : row_begin;
+        pmark[ic] = cnt;
+        if (fill) {
+            RAPJ[cnt] = ic;
+            RAPA[cnt] = 0.0;
+        }
+        cnt++;
+        /* row ic of R*A */
+        int nra = 0;
+        for (int p1 = RI[ic]; p1 < RI[ic + 1]; p1++) {
+            const int i1 = RJ[p1];
+            const double r = RA[p1];
+            for (int p2 = AI[i1]; p2 < AI[i1 + 1]; p2++) {
+                const int i2 = AJ[p2];
+                const double ra = fill ? r * AA[p2] : 0.0;
+                if (amark[i2] < 0) {
+                    amark[i2] = nra;
+                    raj[nra] = i2;
+                    rav[nra++] = ra;
+                } else {
+                    rav[amark[i2]] += ra;
+                }
+            }
+        }
+        /* (R*A)*P */
+        for (int q = 0; q < nra; q++) {
+            const int i1 = raj[q];
+            const double ra = rav[q];
+            amark[i1] = -1;
+            for (int p2 = PI[i1]; p2 < PI[i1 + 1]; p2++) {
+                const int i2 = PJ[p2];
+                const double v = fill ? ra * PA[p2] : 0.0;
+                if (pmark[i2] < rb) {
+                    pmark[i2] = cnt;
+                    if (fill) {
+                        RAPJ[cnt] = i2;
+                        RAPA[cnt] = v;
+                    }
+                    cnt++;
+                } else if (fill) {
+                    RAPA[pmark[i2]] += v;
+                }
+            }
+        }
+    }
+    if (!fill) RAPI[nc] = (int)cnt;
+    free(RI);
+    free(RJ);
+    free(RA);
+    free(amark);
+    free(pmark);
+    free(raj);
+    free(rav);
+    return cnt;
+}
+#pragma GCC pop_options
+
+int64_t orc_amg_rap_kt_count(int n, int nc, const int *AI, const int *AJ, const int *PI,
+                             const int *PJ, int *RAPI)
+{
+    return amg_rap_kt(n, nc, AI, AJ, NULL, PI, PJ, NULL, RAPI, NULL, NULL, 0);
+}
+
+void orc_amg_rap_kt_fill(int n, int nc, const int *AI, const int *AJ, const double *AA,
+                         const int *PI, const int *PJ, const double *PA, const int *RAPI,
+                         int *RAPJ, double *RAPA)
+{
+    amg_rap_kt(n, nc, AI, AJ, AA, PI, PJ, PA, (int *)RAPI, RAPJ, RAPA, 1);
 }
 
 /* par_lr_interp.c:1025-1928 on one process: the structure (C points of S row

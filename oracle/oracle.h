@@ -113,8 +113,16 @@ int64_t orc_amg_strength(int n, const int *AI, const int *AJ, const double *AA, 
                          double max_row_sum, int *SI, int *SJ);
 /* PMIS, hypre_BoomerAMGCoarsenPMISHost par_coarsen.c:2101-2813 with measures
  * from hypre_BoomerAMGIndepSetInit par_indepset.c (seed 2747): cf = 1 (C),
- * -1 (F), -3 (no strong dependence); returns the number of C points */
+ * -1 (F), -3 (no strong dependence); returns the number of C points.
+ * cf_init 1 (HMIS) reads cf as the markers of orc_amg_ruge_first_pass. */
 int orc_amg_pmis(int n, const int *SI, const int *SJ, int cf_init, int *cf);
+/* Ruge-Stueben first pass as HMIS runs it, hypre_BoomerAMGCoarsenRuge
+ * par_coarsen.c:911-1398 (coarsen_type 10, measure_type 0, one process):
+ * cf = 1 (C), -1 (F), -2 (Z: influences no point), -3 (no strong dependence) */
+void orc_amg_ruge_first_pass(int n, const int *SI, const int *SJ, int *cf);
+/* HMIS, hypre_BoomerAMGCoarsenHMIS par_coarsen.c:2845-2866: the first pass,
+ * then PMIS with CF_init 1; returns the number of C points */
+int orc_amg_hmis(int n, const int *SI, const int *SJ, int *cf);
 /* measures |S^T row| + hypre_Rand() (seed 2747), for checking */
 void orc_amg_pmis_measures(int n, const int *SI, const int *SJ, double *measure);
 /* extended+i interpolation, hypre_BoomerAMGBuildExtPIInterpHost
@@ -124,6 +132,15 @@ int64_t orc_amg_extpi_count(int n, const int *AI, const int *AJ, const int *SI, 
                             const int *cf, int *PI);
 void orc_amg_extpi_fill(int n, const int *AI, const int *AJ, const double *AA, const int *SI,
                         const int *SJ, const int *cf, const int *PI, int *PJ, double *PA);
+/* Galerkin product P^T A P in the order of HYPRE's host setup,
+ * hypre_BoomerAMGBuildCoarseOperatorKT par_rap.c:1525-1930 (one process):
+ * (R*A)*P row by row with first-touch slots, the diagonal first.  Pass 1
+ * fills RAPI[nc+1] and returns nnz; pass 2 fills RAPJ/RAPA. */
+int64_t orc_amg_rap_kt_count(int n, int nc, const int *AI, const int *AJ, const int *PI,
+                             const int *PJ, int *RAPI);
+void orc_amg_rap_kt_fill(int n, int nc, const int *AI, const int *AJ, const double *AA,
+                         const int *PI, const int *PJ, const double *PA, const int *RAPI,
+                         int *RAPJ, double *RAPA);
 /* keep the max_elmts largest |p_ij| per row and rescale to the row sum,
  * hypre_ParCSRMatrixTruncate par_csr_matrix.c:2866-3260 (tol 0, rescale, the
  * row's order from hypre_qsort2_abs qsort.c:395-417); in place, returns nnz */

@@ -72,11 +72,16 @@ def lib():
                                             _i32p, _i32p, _u16p, _i32p, _f64p]),
             "orc_amg_strength": (_i64, [_i32, _i32p, _i32p, _f64p, _f64, _f64, _i32p, _i32p]),
             "orc_amg_pmis": (_i32, [_i32, _i32p, _i32p, _i32, _i32p]),
+            "orc_amg_ruge_first_pass": (None, [_i32, _i32p, _i32p, _i32p]),
+            "orc_amg_hmis": (_i32, [_i32, _i32p, _i32p, _i32p]),
             "orc_amg_pmis_measures": (None, [_i32, _i32p, _i32p, _f64p]),
             "orc_amg_extpi_count": (_i64, [_i32, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p]),
             "orc_amg_extpi_fill": (None, [_i32, _i32p, _i32p, _f64p, _i32p, _i32p, _i32p, _i32p,
                                           _i32p, _f64p]),
             "orc_amg_truncate": (_i64, [_i32, _i32p, _i32p, _f64p, _i32]),
+            "orc_amg_rap_kt_count": (_i64, [_i32, _i32, _i32p, _i32p, _i32p, _i32p, _i32p]),
+            "orc_amg_rap_kt_fill": (None, [_i32, _i32, _i32p, _i32p, _f64p, _i32p, _i32p, _f64p,
+                                           _i32p, _i32p, _f64p]),
             "orc_policy_forward": (None, [_f64p, _f64p, _i64, _f64p]),
             "orc_policy_loss_grad": (_f64, [_f64p, _f64p, _i32p, _f64p, _i64, _f64, _f64p]),
             "orc_policy_adam": (None, [_f64p, _f64p, _f64p, _f64p, _i64, _i64, _f64, _f64, _f64,
@@ -312,10 +317,28 @@ def amg_pmis_measures(n, SI, SJ):
     return m[:n]
 
 
-def amg_pmis(n, SI, SJ, cf_init=0):
-    """CF marker: 1 C, -1 F, -3 no strong dependence; and the C count."""
+def amg_pmis(n, SI, SJ, cf_init=0, cf_in=None):
+    """CF marker: 1 C, -1 F, -3 no strong dependence; and the C count.
+    cf_init 1 starts from cf_in, the markers of amg_ruge_first_pass."""
     cf = np.zeros(max(n, 1), np.int32)
+    if cf_init == 1:
+        cf[:n] = np.asarray(cf_in, np.int32)
     nc = lib().orc_amg_pmis(n, _c32(SI), _c32(SJ), int(cf_init), cf)
+    return cf[:n], int(nc)
+
+
+def amg_ruge_first_pass(n, SI, SJ):
+    """Markers of the Ruge-Stueben first pass (coarsen type 10): 1 C, -1 F,
+    -2 Z (influences no point), -3 no strong dependence."""
+    cf = np.zeros(max(n, 1), np.int32)
+    lib().orc_amg_ruge_first_pass(n, _c32(SI), _c32(SJ), cf)
+    return cf[:n]
+
+
+def amg_hmis(n, SI, SJ):
+    """HMIS: the Ruge first pass, then PMIS with CF_init 1."""
+    cf = np.zeros(max(n, 1), np.int32)
+    nc = lib().orc_amg_hmis(n, _c32(SI), _c32(SJ), cf)
     return cf[:n], int(nc)
 
 
@@ -345,10 +368,23 @@ def csr_transpose(m, n, I, J, A):
     return TI.astype(np.int32), rows[order].astype(np.int32), A[order].copy()
 
 
+def amg_rap_kt(n, nc, I, J, A, PI, PJ, PA):
+    """P^T A P in HYPRE's host order ((R A) P, first-touch slots, diagonal
+    first): (RI, RJ, RA)."""
+    I, J, A, PI, PJ, PA = _c32(I), _c32(J), _cf(A), _c32(PI), _c32(PJ), _cf(PA)
+    RI = np.zeros(nc + 1, np.int32)
+    nnz = lib().orc_amg_rap_kt_count(n, nc, I, J, PI, PJ, RI)
+    RJ = np.zeros(max(nnz, 1), np.int32)
+    RA = np.zeros(max(nnz, 1), np.float64)
+    lib().orc_amg_rap_kt_fill(n, nc, I, J, A, PI, PJ, PA, RI, RJ, RA)
+    return RI, RJ[:nnz].copy(), RA[:nnz].copy()
+
+
 def amg_hierarchy(n, I, J, A, theta=0.25, max_row_sum=1.0, cf_init=0, max_elmts=0,
-                  max_coarse_size=9, max_levels=25):
+                  max_coarse_size=9, max_levels=25, coarsen="pmis", rap="two_step"):
     """The one-process BoomerAMG setup loop (par_amg_setup.c:994-3182) with
-    strength -> PMIS -> extended+i -> R = P^T -> R (A P) (host SpGEMM, the
+    strength -> PMIS (coarsen types 8/9, CF_init 0/2) or HMIS (coarsen type
+    10, coarsen="hmis") -> extended+i -> R = P^T -> R (A P) (host SpGEMM, the
     diagonal first).  Returns the levels as dicts (n, I, J, A, cf, P...)."""
     levels = [{"n": n, "I": _c32(I), "J": _c32(J), "A": _cf(A)}]
     while True:
@@ -356,13 +392,19 @@ def amg_hierarchy(n, I, J, A, theta=0.25, max_row_sum=1.0, cf_init=0, max_elmts=
         if L["n"] <= max_coarse_size or len(levels) >= max_levels:
             break
         SI, SJ = amg_strength(L["n"], L["I"], L["J"], L["A"], theta, max_row_sum)
-        cf, nc = amg_pmis(L["n"], SI, SJ, cf_init)
+        if coarsen == "hmis":
+            cf, nc = amg_hmis(L["n"], SI, SJ)
+        else:
+            cf, nc = amg_pmis(L["n"], SI, SJ, cf_init)
         if nc == 0 or nc == L["n"]:
             break
         PI, PJ, PA, nc = amg_extpi(L["n"], L["I"], L["J"], L["A"], SI, SJ, cf, max_elmts)
-        RI, RJ, RA = csr_transpose(L["n"], nc, PI, PJ, PA)
-        API, APJ, APA = csr_spgemm(L["n"], nc, L["I"], L["J"], L["A"], PI, PJ, PA, False)
-        CI, CJ, CA = csr_spgemm(nc, nc, RI, RJ, RA, API, APJ, APA, True)
+        if rap == "kt":
+            CI, CJ, CA = amg_rap_kt(L["n"], nc, L["I"], L["J"], L["A"], PI, PJ, PA)
+        else:
+            RI, RJ, RA = csr_transpose(L["n"], nc, PI, PJ, PA)
+            API, APJ, APA = csr_spgemm(L["n"], nc, L["I"], L["J"], L["A"], PI, PJ, PA, False)
+            CI, CJ, CA = csr_spgemm(nc, nc, RI, RJ, RA, API, APJ, APA, True)
         L.update(S=(SI, SJ), cf=cf, P=(PI, PJ, PA), nc=nc)
         levels.append({"n": nc, "I": CI, "J": CJ, "A": CA})
     return levels

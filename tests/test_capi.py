@@ -77,3 +77,25 @@ def test_trilinear_interp_rejects_bad_grids_before_device_work():
     rc = L.amgt_interp_trilinear(0, g, ctypes.byref(out[0]), ctypes.byref(out[1]),
                                  ctypes.byref(out[2]), *[ctypes.byref(p) for p in ptrs], None)
     assert rc == _lib.AMGT_ERR_ARG
+
+
+def test_wrapper_length_and_device_checks():
+    """The MBSRMatrix wrappers refuse short or foreign tensors before any
+    launch (the kernels would index past their ends)."""
+    import torch
+
+    from amgt_amd import mbsr as M
+
+    cpu = torch.device("cpu")
+    M._need_len(torch.zeros(5), "x", 5, cpu)
+    with pytest.raises(ValueError, match="holds 4 entries, 5 needed"):
+        M._need_len(torch.zeros(4), "x", 5, cpu)
+    with pytest.raises(ValueError, match="is on"):
+        M._need_len(torch.zeros(5, device="meta"), "x", 5, cpu)
+    M._need_rows_len(torch.zeros(3, 7), "X", 3, 7, cpu)
+    with pytest.raises(ValueError):
+        M._need_rows_len(torch.zeros(3, 6), "X", 3, 7, cpu)
+    with pytest.raises(ValueError):
+        M._need_rows_len(torch.zeros(2, 7), "X", 3, 7, cpu)
+    with pytest.raises(TypeError):
+        M._need_cuda(torch.zeros(3), "x", torch.float64)

@@ -11,7 +11,15 @@ of HYPRE's host algorithms.
   V(1,1) Jacobi convergence factor 0.678738 of default.saved;
 * every level of a truncated (P max 4) hierarchy against the oracle fed with
   the device's own A_l, and R (A P) against the oracle's host SpGEMM;
-* PCG with the algebraic V-cycle reaches 1e-8.
+* PCG with the algebraic V-cycle reaches 1e-8;
+* HMIS (coarsen type 10, HYPRE's default): the device splitting equals the
+  oracle's on every case; HYPRE's TEST_ij fsai.out.103 job (10^3, HMIS, relax 7,
+  PCG) gives the fixture's 22 iterations and final relative residual
+  7.480945e-09 on the device; the C1 problem (5-pt 128^2) at full size equals
+  HYPRE's CPU path in levels, complexities, iterations and final residual, and
+  every level's S, C/F split and P equal the oracle's bit for bit with R (A P)
+  within 1e-12; the C2 problem (7-pt 128^3) at full size, level by level
+  against the oracle fed with the device's own A_l.
 """
 import json
 import os
@@ -24,6 +32,7 @@ import torch
 from amgt_amd import amg as G
 from amgt_amd import coarsen as C
 from oracle import oracle as O
+from tests import util_amg as UA
 from tests.util_matrices import rand_csr
 
 pytestmark = pytest.mark.gpu
@@ -62,8 +71,51 @@ def _rand_mmatrix(n, density, seed):
     return Ii, Jj, Aa
 
 
+def _neg(case):
+    n, I, J, A = case
+    return n, I, J, -A
+
+
+def _ascending(case):
+    n, I, J, A = case
+    return (n,) + _sorted_csr(n, I, J, A)
+
+
+def _diag_first_long(n, seed):
+    """Random M-matrix rows of 12-40 entries, stored as HYPRE stores them: the
+    diagonal first, the rest ascending; diagonal negative (the matrix negated)."""
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for i in range(n):
+        k = int(rng.integers(12, 41))
+        c = rng.choice(n, size=k, replace=False)
+        c = c[c != i]
+        rows.append(np.full(c.size, i))
+        cols.append(c)
+    M = sp.csr_matrix((-rng.random(sum(c.size for c in cols)) - 0.1,
+                       (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+    M = (M + M.T).tocsr()
+    d = np.asarray(abs(M).sum(axis=1)).ravel() + 0.5
+    M = -(M + sp.diags(d)).tocsr()
+    M.sort_indices()
+    I, J, A = [0], [], []
+    for i in range(n):
+        s0, e0 = M.indptr[i], M.indptr[i + 1]
+        cj, ca = M.indices[s0:e0], M.data[s0:e0]
+        m = cj != i
+        J += [i] + list(cj[m])
+        A += [ca[~m][0]] + list(ca[m])
+        I.append(len(J))
+    return np.array(I, np.int32), np.array(J, np.int32), np.array(A)
+
+
 CASES = {
     "7pt_10": lambda: (1000,) + tuple(O.stencil(7, 10, 10, 10)),
+    # 27 entries per row, diagonal first (the stencil's layout): the wave kernel
+    # of extended+i runs; negated, the diagonal is negative
+    "27pt_neg": lambda: _neg((512,) + tuple(O.stencil(27, 8, 8, 8))),
+    "27pt_ascending": lambda: _ascending((600,) + tuple(O.stencil(27, 10, 6, 10))),
+    "dfirst_long": lambda: (900,) + _diag_first_long(900, 11),
     "7pt_aniso": lambda: (6 * 7 * 9,) + tuple(O.stencil(7, 6, 7, 9, c=(1, 1, 0.01))),
     "27pt_8": lambda: (512,) + tuple(O.stencil(27, 8, 8, 8)),
     "5pt_33": lambda: (33 * 33,) + tuple(O.stencil(7, 33, 33, 1)),
@@ -137,17 +189,128 @@ def test_extpi_wave_tiers_match_oracle(dev, n, per_row):
 
 
 def test_pmis_cf_init_rejected(dev):
+    import ctypes
+    from amgt_amd import _lib as L
     n, I, J, A = CASES["7pt_10"]()
     SI, SJ = C.strength(n, T(I, dev), T(J, dev), T(A, dev))
-    with pytest.raises(RuntimeError):
+    with pytest.raises(ValueError):
         C.pmis(n, SI, SJ, cf_init=1)
+    cf = torch.zeros(n, dtype=torch.int32, device=dev)
+    nc = ctypes.c_int32()
+    st = L.load().amgt_amg_pmis(n, ctypes.c_void_p(SI.data_ptr()), ctypes.c_void_p(SJ.data_ptr()),
+                                3, ctypes.c_void_p(cf.data_ptr()), ctypes.byref(nc), None)
+    assert st == L.AMGT_ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("theta", [0.25, 0.5])
+def test_hmis_matches_oracle(dev, name, theta):
+    print(f"[hmis] {name} theta={theta}", flush=True)
+    n, I, J, A = CASES[name]()
+    SI, SJ = C.strength(n, T(I, dev), T(J, dev), T(A, dev), theta, 1.0)
+    rSI, rSJ = O.amg_strength(n, I, J, A, theta, 1.0)
+    cf, nc = C.hmis(n, SI, SJ)
+    rcf, rnc = O.amg_hmis(n, rSI, rSJ)
+    assert nc == rnc and np.array_equal(cf.cpu().numpy(), rcf)
+    # the device PMIS pass with CF_init 1 on the oracle's first-pass markers
+    first = T(O.amg_ruge_first_pass(n, rSI, rSJ), dev)
+    import ctypes
+    from amgt_amd import _lib as L
+    nc2 = ctypes.c_int32()
+    L.check(L.load().amgt_amg_pmis(n, ctypes.c_void_p(SI.data_ptr()),
+                                   ctypes.c_void_p(SJ.data_ptr() if SJ.numel() else 0), 1,
+                                   ctypes.c_void_p(first.data_ptr()), ctypes.byref(nc2), None),
+            "amgt_amg_pmis")
+    assert nc2.value == rnc and np.array_equal(first.cpu().numpy(), rcf)
+
+
+def _levels_match_oracle(amg, theta, mrs, coarsen, pmx, levels=None, cf_init=0):
+    """Every level of a device hierarchy against the oracle fed with the device's
+    own A_l: S, the C/F split and P bit for bit, R (A P) structure exact and
+    values within 1e-12 of |R||A||P|."""
+    todo = range(len(amg.levels) - 1) if levels is None else levels
+    for lvl in todo:
+        Lf, Lc = amg.levels[lvl], amg.levels[lvl + 1]
+        print(f"[levels] {coarsen} level {lvl} n={Lf.n}", flush=True)
+        dI, dJ, dA = Lf.A.to_csr()
+        hI, hJ, hA = dI.cpu().numpy(), dJ.cpu().numpy(), dA.cpu().numpy()
+        rSI, rSJ = O.amg_strength(Lf.n, hI, hJ, hA, theta, mrs)
+        if coarsen == "hmis":
+            rcf, rnc = O.amg_hmis(Lf.n, rSI, rSJ)
+        else:
+            rcf, rnc = O.amg_pmis(Lf.n, rSI, rSJ, cf_init)
+        assert np.array_equal(amg.cf[lvl].cpu().numpy(), rcf), lvl
+        rPI, rPJ, rPA, _ = O.amg_extpi(Lf.n, hI, hJ, hA, rSI, rSJ, rcf, pmx)
+        P = sp.csr_matrix((rPA, rPJ, rPI), shape=(Lf.n, rnc))
+        P.sort_indices()
+        gI, gJ, gA = Lf.P.to_csr()
+        assert np.array_equal(gI.cpu().numpy(), P.indptr) and np.array_equal(gJ.cpu().numpy(), P.indices)
+        assert np.array_equal(gA.cpu().numpy(), P.data), lvl
+        RI, RJ, RA = O.csr_transpose(Lf.n, rnc, P.indptr, P.indices, P.data)
+        AP = O.csr_spgemm(Lf.n, rnc, hI, hJ, hA, P.indptr, P.indices, P.data, False)
+        CI, CJ, CA = O.csr_spgemm(rnc, rnc, RI, RJ, RA, *AP, True)
+        ref = sp.csr_matrix((CA, CJ, CI), shape=(rnc, rnc))
+        ref.sort_indices()
+        cI, cJ, cA = Lc.A.to_csr()
+        assert np.array_equal(cI.cpu().numpy(), ref.indptr)
+        assert np.array_equal(cJ.cpu().numpy(), ref.indices)
+        scale = (abs(sp.csr_matrix((RA, RJ, RI), shape=(rnc, Lf.n))) @
+                 abs(sp.csr_matrix((hA, hJ, hI), shape=(Lf.n, Lf.n))) @ abs(P)).tocsr()
+        scale.sort_indices()
+        assert np.all(np.abs(cA.cpu().numpy() - ref.data) <= 1e-12 * scale.data + 1e-300)
+
+
+def _pcg_ones_rhs(amg, dev, ones_solution):
+    n = amg.levels[0].n
+    one = torch.ones(n, dtype=torch.float64, device=dev)
+    b = amg.levels[0].A.spmv(one) if ones_solution else one
+    amg.capture()
+    return amg.pcg(b, tol=1e-8, max_iter=1000)
+
+
+def test_hmis_matches_ij_fsai_fixture(dev):
+    """TEST_ij fsai.out.103: ij -n 10 10 10 -solver 1 -rlx 7 (ij's defaults:
+    HMIS, extended+i, P max 4, theta 0.25, max row sum 1, coarse size 9)."""
+    g = GOLD["ij_fsai_103"]
+    n, I, J, A = CASES["7pt_10"]()
+    amg = G.BoomerAMG(T(I, dev), T(J, dev), T(A, dev), n, dev, coarsen="hmis", relax_type=7)
+    _levels_match_oracle(amg, 0.25, 1.0, "hmis", 4)
+    x, it, hist = _pcg_ones_rhs(amg, dev, ones_solution=False)
+    print(f"[fsai] iterations {it} final {hist[-1]:e}", flush=True)
+    assert it == g["iterations"]
+    assert "%e" % hist[-1] == g["final_relative_residual"]
+
+
+def test_hmis_c1_full_size_matches_hypre_cpu_path(dev):
+    """C1: 2D 5-pt 128^2, HMIS, theta 0.25, extended+i, P max 4, l1-Jacobi
+    (relax 18) weight 1, b = A*1: HYPRE's CPU path gives 7 levels, grid /
+    operator complexity 1.666626 / 2.590556, 9 iterations, 3.829e-09."""
+    g = GOLD["cpu_probe"]["c1_5pt_128"]
+    I, J, A = O.stencil(7, 128, 128, 1)
+    n = 128 * 128
+    amg = G.BoomerAMG(T(I, dev), T(J, dev), T(A, dev), n, dev)
+    _levels_match_oracle(amg, 0.25, 1.0, "hmis", 4)
+    gc, oc = amg.complexities()
+    assert len(amg.levels) == g["levels"]
+    assert round(gc, 6) == g["grid_complexity"] and round(oc, 6) == g["operator_complexity"]
+    x, it, hist = _pcg_ones_rhs(amg, dev, ones_solution=True)
+    print(f"[c1] iterations {it} final {hist[-1]:e}", flush=True)
+    assert it == g["iterations"] and "%.3e" % hist[-1] == g["final_relative_residual"]
+    # the oracle's hierarchy and solve give the same
+    levels = O.amg_hierarchy(n, I, J, A, theta=0.25, max_row_sum=1.0, max_elmts=4,
+                             max_coarse_size=9, coarsen="hmis")
+    assert [L_["n"] for L_ in levels] == [L_.n for L_ in amg.levels]
+    mats = UA.level_mats(levels)
+    _, it_o, rel_o = UA.pcg(mats[0][0], mats[0][0] @ np.ones(n), UA.VCycle(mats, relax=18))
+    assert it_o == it and abs(rel_o - hist[-1]) <= 1e-6 * rel_o
 
 
 def test_hierarchy_matches_ij_default(dev):
     g = GOLD["ij_default"]
     n, I, J, A = CASES["7pt_10"]()
     amg = G.BoomerAMG(T(I, dev), T(J, dev), T(A, dev), n, dev, theta=0.25, max_row_sum=1.0,
-                      cf_init=2, max_elmts=0, omega=1.0, coarse_size=9)
+                      cf_init=2, max_elmts=0, omega=1.0, coarse_size=9, coarsen="pmis",
+                      relax_type=7)
     gc, oc = amg.complexities()
     assert [L_.n for L_ in amg.levels] == [1000, 354, 47, 6]
     assert round(gc, 6) == g["grid_complexity"] and round(oc, 6) == g["operator_complexity"]
@@ -168,39 +331,31 @@ def test_hierarchy_matches_ij_default(dev):
     assert round((rn / r0) ** (1.0 / it), 6) == g["average_convergence_factor"]
 
 
-def test_truncated_hierarchy_levels_match_oracle(dev):
+@pytest.mark.parametrize("coarsen", ["hmis", "pmis"])
+def test_truncated_hierarchy_levels_match_oracle(dev, coarsen):
     nx = 24
     I, J, A = O.stencil(7, nx, nx, nx)
     n = nx ** 3
     amg = G.BoomerAMG(T(I, dev), T(J, dev), T(A, dev), n, dev, theta=0.25, max_row_sum=0.9,
-                      cf_init=0, max_elmts=4, coarse_size=20)
+                      cf_init=0, max_elmts=4, coarse_size=20, coarsen=coarsen)
     assert len(amg.levels) >= 4
-    for lvl in range(len(amg.levels) - 1):
-        Lf, Lc = amg.levels[lvl], amg.levels[lvl + 1]
-        dI, dJ, dA = Lf.A.to_csr()
-        hI, hJ, hA = dI.cpu().numpy(), dJ.cpu().numpy(), dA.cpu().numpy()
-        rSI, rSJ = O.amg_strength(Lf.n, hI, hJ, hA, 0.25, 0.9)
-        rcf, rnc = O.amg_pmis(Lf.n, rSI, rSJ, 0)
-        assert np.array_equal(amg.cf[lvl].cpu().numpy(), rcf), lvl
-        rPI, rPJ, rPA, _ = O.amg_extpi(Lf.n, hI, hJ, hA, rSI, rSJ, rcf, 4)
-        P = sp.csr_matrix((rPA, rPJ, rPI), shape=(Lf.n, rnc))
-        P.sort_indices()
-        gI, gJ, gA = Lf.P.to_csr()
-        assert np.array_equal(gI.cpu().numpy(), P.indptr) and np.array_equal(gJ.cpu().numpy(), P.indices)
-        assert np.array_equal(gA.cpu().numpy(), P.data), lvl
-        # R (A P) against the oracle's host SpGEMM on the same operands
-        RI, RJ, RA = O.csr_transpose(Lf.n, rnc, P.indptr, P.indices, P.data)
-        AP = O.csr_spgemm(Lf.n, rnc, hI, hJ, hA, P.indptr, P.indices, P.data, False)
-        CI, CJ, CA = O.csr_spgemm(rnc, rnc, RI, RJ, RA, *AP, True)
-        ref = sp.csr_matrix((CA, CJ, CI), shape=(rnc, rnc))
-        ref.sort_indices()
-        cI, cJ, cA = Lc.A.to_csr()
-        assert np.array_equal(cI.cpu().numpy(), ref.indptr)
-        assert np.array_equal(cJ.cpu().numpy(), ref.indices)
-        scale = (abs(sp.csr_matrix((RA, RJ, RI), shape=(rnc, Lf.n))) @
-                 abs(sp.csr_matrix((hA, hJ, hI), shape=(Lf.n, Lf.n))) @ abs(P)).tocsr()
-        scale.sort_indices()
-        assert np.all(np.abs(cA.cpu().numpy() - ref.data) <= 1e-12 * scale.data + 1e-300)
+    _levels_match_oracle(amg, 0.25, 0.9, coarsen, 4)
+
+
+def test_c2_full_size_levels_match_oracle(dev):
+    """C2 (7-pt 128^3, ij's defaults) on the device: every level against the
+    oracle fed with the device's own A_l, then PCG (b = A*1) to 1e-8."""
+    g = GOLD["cpu_probe"]["c2_7pt_128"]
+    nx = 128
+    I, J, A = O.stencil(7, nx, nx, nx)
+    n = nx ** 3
+    amg = G.BoomerAMG(T(I, dev), T(J, dev), T(A, dev), n, dev)
+    _levels_match_oracle(amg, 0.25, 1.0, "hmis", 4)
+    gc, oc = amg.complexities()
+    x, it, hist = _pcg_ones_rhs(amg, dev, ones_solution=True)
+    print(f"[c2] levels {[L_.n for L_ in amg.levels]} complexities {gc:.6f} {oc:.6f} "
+          f"iterations {it} final {hist[-1]:e}; HYPRE CPU: {g}", flush=True)
+    assert hist[-1] < 1e-8 and it <= g["iterations"] + 1
 
 
 def test_pcg_with_algebraic_vcycle(dev):
@@ -212,6 +367,6 @@ def test_pcg_with_algebraic_vcycle(dev):
     xs = O.rand_normal(1, n)
     b = T(sp.csr_matrix((A, J, I), shape=(n, n)) @ xs, dev)
     x, iters, hist = amg.pcg(b, tol=1e-8)
-    # PMIS + extended+i (P max 4) + Jacobi weight 1: 27 iterations measured
-    assert hist[-1] < 1e-8 and iters <= 35, (iters, hist[-1])
+    # HMIS + extended+i (P max 4) + l1-Jacobi weight 1
+    assert hist[-1] < 1e-8 and iters <= 14, (iters, hist[-1])
     assert np.max(np.abs(x.cpu().numpy() - xs)) < 1e-5 * np.max(np.abs(xs))

@@ -3,7 +3,7 @@ not the bench line itself.
 
 * C3, mixed precision: 3D anisotropic 7-pt diffusion on 160^3 (couplings 1,
   1, 0.01; 4,096,000 rows), b ~ N(0,1) (Park-Miller seed 2), the algebraic
-  hierarchy with FP64 on the finest level, FP32 on levels 1-2 and FP16 below
+  hierarchy (HMIS, extended+i, P max 4, l1-Jacobi) with FP64 on the finest level, FP32 on levels 1-2 and FP16 below
   (FP32 accumulation), FP64 Krylov vectors: the true relative residual
   ||b - A x|| / ||b||, recomputed on the host in FP64, is below 1e-10 for the
   FP64 and the mixed preconditioner, and the two solutions agree to 1e-6
@@ -40,8 +40,8 @@ def test_c3_aniso_mixed_precision(dev):
     for name, pol_v, pol_g in (("fp64", G.fp64_policy, G.fp64_policy),
                                ("mixed", G.mixed_spmv_policy, G.mixed_spgemm_policy)):
         print(f"[c3] {name}", flush=True)
-        amg = G.BoomerAMG(T(I, dev), T(J, dev), T(V, dev), n, dev, max_elmts=4, coarse_size=512,
-                          omega=6.0 / 7.0, spmv_policy=pol_v, spgemm_policy=pol_g)
+        amg = G.BoomerAMG(T(I, dev), T(J, dev), T(V, dev), n, dev, spmv_policy=pol_v,
+                          spgemm_policy=pol_g)
         if name == "mixed":
             precs = [L_.A.info()["precision"] for L_ in amg.levels]
             assert precs[0] == 0 and all(p == 1 for p in precs[1:3]) and all(p == 2 for p in precs[3:])

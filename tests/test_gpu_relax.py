@@ -48,6 +48,17 @@ def test_diagonal(dev, name):
     assert np.array_equal(M.diagonal().cpu().numpy(), d)
     inv = np.where(d != 0, 1.0 / np.where(d != 0, np.abs(d), 1.0), 0.0)
     assert np.array_equal(M.diagonal(inverse_abs=True).cpu().numpy(), inv)
+    # relax 7: 1/a_ii, 1 where a_ii = 0 (hypre_ParCSRComputeL1Norms option 5)
+    inv7 = 1.0 / np.where(d != 0, d, 1.0)
+    assert np.array_equal(M.diagonal(kind=L.AMGT_DIAG_INV).cpu().numpy(), inv7)
+    # relax 18: 1/l1 with l1 = sum |a_ij| over the row, negated where a_ii < 0
+    # (option 1); the sum's order differs from the row's, so within 1 ulp-ish
+    l1 = np.asarray(abs(sp.csr_matrix((A, J, I), shape=(n, n))).sum(axis=1)).ravel()
+    l1 = np.where(d < 0, -l1, l1)
+    inv18 = np.where(l1 != 0, 1.0 / np.where(l1 != 0, l1, 1.0), 0.0)
+    got = M.diagonal(kind=L.AMGT_DIAG_INV_L1).cpu().numpy()
+    assert np.all((got == 0) == (inv18 == 0))
+    assert np.all(np.abs(got - inv18) <= 1e-13 * np.abs(inv18))
 
 
 @pytest.mark.parametrize("name", list(CASES))

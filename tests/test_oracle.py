@@ -24,6 +24,15 @@ Pin map (oracle function -> tests):
     factor of the V(1,1) Jacobi solve pins P's values), test_amg_strength_anisotropic,
     test_amg_pmis_invariants
   orc_amg_truncate -> test_amg_truncate_invariants
+  orc_amg_ruge_first_pass / orc_amg_hmis / PMIS with CF_init 1 ->
+    test_hmis_matches_ij_fsai_job (HYPRE's TEST_ij fsai.out.103 fixture: 22 PCG
+    iterations and the final relative residual to 7 digits pin the whole HMIS
+    hierarchy and relax type 7), test_hmis_c1_matches_hypre_cpu_path,
+    test_hmis_c2_matches_hypre_cpu_path (HYPRE's CPU path on the C1/C2 problems:
+    levels, complexities to 6 digits, iterations), test_ruge_first_pass_1d_closed_form
+  orc_amg_rap_kt_* -> test_rap_kt_vs_scipy (values and the diagonal-first row layout),
+    test_hmis_c2_matches_hypre_cpu_path (its summation order: with the order of
+    R (A P) the coarse levels below level 3 differ from HYPRE's)
 """
 import json
 import os
@@ -33,6 +42,7 @@ import pytest
 import scipy.sparse as sp
 
 from oracle import oracle as O
+from tests import util_amg as UA
 from tests.util_matrices import rand_csr, to_scipy
 
 GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "hypre_fixtures.json")))
@@ -463,6 +473,96 @@ def test_amg_truncate_invariants():
         assert abs(b.sum() - a.sum()) <= 1e-12 * np.abs(a).sum()
         ratio = b / b[0] * np.abs(kept_abs[0]) * np.sign(b[0])
         assert np.allclose(np.sort(np.abs(ratio))[::-1], kept_abs, rtol=1e-12)
+
+
+# ---- HMIS (Ruge first pass + PMIS with CF_init 1) and HYPRE's host RAP ------------------------
+def _hmis_solve(I, J, A, n, relax, b, rap="two_step"):
+    levels = O.amg_hierarchy(n, I, J, A, theta=0.25, max_row_sum=1.0, max_elmts=4,
+                             max_coarse_size=9, coarsen="hmis", rap=rap)
+    mats = UA.level_mats(levels)
+    x, it, rel = UA.pcg(mats[0][0], b(mats[0][0]), UA.VCycle(mats, relax=relax))
+    ns = [L["n"] for L in levels]
+    nnz = [len(L["J"]) for L in levels]
+    return levels, it, rel, UA.complexities(ns, nnz)
+
+
+def test_hmis_matches_ij_fsai_job():
+    g = GOLD["ij_fsai_103"]
+    I, J, A = O.stencil(7, 10, 10, 10)
+    levels, it, rel, _ = _hmis_solve(I, J, A, 1000, 7, lambda M: np.ones(M.shape[0]))
+    assert it == g["iterations"]
+    assert "%e" % rel == g["final_relative_residual"]
+    # l1-Jacobi (relax 18) on the same hierarchy converges differently: the
+    # fixture pins relax 7's diagonal, not the l1 norm
+    mats = UA.level_mats(levels)
+    _, it18, rel18 = UA.pcg(mats[0][0], np.ones(1000), UA.VCycle(mats, relax=18))
+    assert (it18, "%e" % rel18) != (g["iterations"], g["final_relative_residual"])
+
+
+def test_hmis_c1_matches_hypre_cpu_path():
+    g = GOLD["cpu_probe"]["c1_5pt_128"]
+    I, J, A = O.stencil(7, 128, 128, 1)
+    levels, it, rel, (gc, oc) = _hmis_solve(I, J, A, 128 * 128, 18,
+                                            lambda M: M @ np.ones(M.shape[0]))
+    assert len(levels) == g["levels"]
+    assert round(gc, 6) == g["grid_complexity"] and round(oc, 6) == g["operator_complexity"]
+    assert it == g["iterations"] and "%.3e" % rel == g["final_relative_residual"]
+
+
+@pytest.mark.slow
+def test_hmis_c2_matches_hypre_cpu_path():
+    g = GOLD["cpu_probe"]["c2_7pt_128"]
+    I, J, A = O.stencil(7, 128, 128, 128)
+    n = 128 ** 3
+    levels, it, rel, (gc, oc) = _hmis_solve(I, J, A, n, 18, lambda M: M @ np.ones(n), rap="kt")
+    assert len(levels) == g["levels"]
+    assert round(gc, 6) == g["grid_complexity"] and round(oc, 6) == g["operator_complexity"]
+    assert it == g["iterations"] and "%.3e" % rel == g["final_relative_residual"]
+
+
+def test_ruge_first_pass_1d_closed_form():
+    # 1D Laplacian chain: every point influences its neighbours (measures 1 at
+    # the ends, 2 inside); the oldest point of measure 2 (point 1) is taken
+    # first, its neighbours turn F, which raises point 3 above the rest, and so
+    # on: C points are the odd indices.  With two points both have measure 1
+    # and the older one, point 0, is taken.
+    SI, SJ = np.array([0, 1, 2], np.int32), np.array([1, 0], np.int32)
+    assert np.array_equal(O.amg_ruge_first_pass(2, SI, SJ), [1, -1])
+    for n in (5, 6, 11):
+        M = sp.diags([-np.ones(n - 1), 2 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1], format="csr")
+        SI, SJ = O.amg_strength(n, M.indptr, M.indices, M.data, 0.25, 1.0)
+        cf = O.amg_ruge_first_pass(n, SI, SJ)
+        assert np.array_equal(cf, np.where(np.arange(n) % 2 == 1, 1, -1)), (n, cf)
+        cfh, nc = O.amg_hmis(n, SI, SJ)
+        assert np.array_equal(cfh, cf) and nc == n // 2
+    # a point without strong dependence is -3; one that influences nobody is Z (-2)
+    SI = np.array([0, 0, 1, 2], np.int32)  # 0: none, 1 depends on 0, 2 depends on 1
+    SJ = np.array([0, 1], np.int32)
+    cf = O.amg_ruge_first_pass(3, SI, SJ)
+    assert cf[0] == -3 and cf[2] == -2
+
+
+def test_rap_kt_vs_scipy():
+    for seed in range(3):
+        I, J, A = O.stencil(7, 9, 8, 7) if seed == 0 else rand_csr(60, 60, 0.08, seed)
+        n = len(I) - 1
+        M = to_scipy(n, n, I, J, A)
+        M = (M + M.T + sp.identity(n) * 8).tocsr()
+        SI, SJ = O.amg_strength(n, M.indptr, M.indices, M.data, 0.25, 1.0)
+        cf, nc = O.amg_hmis(n, SI, SJ)
+        PI, PJ, PA, nc = O.amg_extpi(n, M.indptr, M.indices, M.data, SI, SJ, cf, 4)
+        P = to_scipy(n, nc, PI, PJ, PA)
+        RI, RJ, RA = O.amg_rap_kt(n, nc, M.indptr, M.indices, M.data, PI, PJ, PA)
+        ref = (P.T @ M @ P).tocsr()
+        got = to_scipy(nc, nc, RI, RJ, RA)
+        assert np.all(RJ[RI[:-1]] == np.arange(nc))  # diagonal first
+        assert abs(got - ref).max() <= 1e-13 * abs(ref).max()
+        # the same pattern (structural zeros included) as the two-step product
+        CI, CJ, CA = O.csr_spgemm(nc, nc, *O.csr_transpose(n, nc, PI, PJ, PA),
+                                  *O.csr_spgemm(n, nc, M.indptr, M.indices, M.data, PI, PJ, PA,
+                                                False), True)
+        for r in range(nc):
+            assert sorted(RJ[RI[r]:RI[r + 1]]) == sorted(CJ[CI[r]:CI[r + 1]])
 
 
 # ---- policy network ---------------------------------------------------------------------------
