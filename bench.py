@@ -35,25 +35,64 @@ def spmv_bytes(info):
             + 8 * info["nrows"])
 
 
+def host_cpu():
+    """(model name, logical CPUs, threads to use): the box's CPU share is 16
+    threads per GPU, whatever the host reports."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    ncpu = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    return model, ncpu, max(1, min(threads, 16, ncpu))
+
+
 def cpu_baseline(I, J, A, x, bytes_per_call, budget_s=10.0):
+    import numpy as np
+
     from oracle import oracle as O
 
     n = len(I) - 1
-    O.csr_matvec(n, I, J, A, 1.0, x)  # warm
+    model, ncpu, threads = host_cpu()
+    y = np.empty(n)
+    O.csr_matvec_omp(threads, n, I, J, A, 1.0, x, out=y)  # warm
     t0 = time.perf_counter()
     reps = 0
     while True:
-        O.csr_matvec(n, I, J, A, 1.0, x)
+        O.csr_matvec_omp(threads, n, I, J, A, 1.0, x, out=y)
         reps += 1
         el = time.perf_counter() - t0
         if el >= budget_s:
             break
     per = el / reps
-    return {"value": round(bytes_per_call / per / 1e9, 3), "unit": "GB/s", "cores": 1,
+    return {"value": round(bytes_per_call / per / 1e9, 3), "unit": "GB/s", "cores": threads,
             "kind": "port",
             "sample": f"{reps} HYPRE host CSR SpMVs (oracle restatement of "
-                      f"hypre_CSRMatrixMatvecOutOfPlaceHost, 1 thread) on the same 128^3 7-pt "
-                      f"matrix, {el:.1f} s; {per * 1e3:.2f} ms per SpMV"}
+                      f"hypre_CSRMatrixMatvecOutOfPlaceHost, rows split over {threads} OpenMP "
+                      f"threads as HYPRE's OpenMP build splits them) on the same 128^3 7-pt "
+                      f"matrix, {el:.1f} s; {per * 1e3:.2f} ms per SpMV; host: {model}, "
+                      f"{ncpu} logical CPUs, {threads} used (the box's share per GPU)"}
+
+
+def cpu_amg_baseline(I, J, V, b):
+    """HYPRE's CPU BoomerAMG-PCG on the C2 problem as the oracle restates it
+    (ij's defaults with relax 18, the host RAP order), one thread: setup and
+    solve seconds beside the GPU ones."""
+    from oracle import solve as S
+
+    n = len(I) - 1
+    model, _, _ = host_cpu()
+    x, it, rel, levels, ts, tsol = S.setup_and_solve(n, I, J, V, b, relax=18)
+    return {"setup_s": round(ts, 3), "solve_s": round(tsol, 3),
+            "setup_plus_solve_s": round(ts + tsol, 3), "iterations": it,
+            "final_rel_residual": rel, "levels": levels, "cores": 1, "kind": "port",
+            "sample": "the oracle's restatement of HYPRE's host setup (strength, HMIS, "
+                      "extended+i P max 4, RAP in HYPRE's host order; C functions, 1 thread) and "
+                      f"PCG with the V(1,1) l1-Jacobi cycle (scipy, 1 thread); host: {model}"}
 
 
 def csr_flops(AI, AJ, BI):
@@ -111,6 +150,65 @@ def spgemm_timings(W, MBSRMatrix, M, I, J, A, dev, reps=3):
         "total_ms": round(ap_ms + rap_ms, 3), "coarse_rows": nc,
         "coarse_nnz": int(RAP.info()["nnz"]), "flops": f_ap + f_rap,
         "gflops": round((f_ap + f_rap) / ((ap_ms + rap_ms) * 1e-3) / 1e9, 2)}
+    return out
+
+
+def spmv_cold(dev, M, x, steps=200):
+    """SpMV figures whose matrix bytes cannot stay in the 256 MB last-level
+    cache (MALL) between launches: (a) two copies of the 7-pt 128^3 matrix
+    (2 x 174 MB), launches alternating between them; (b) the 7-pt anisotropic
+    160^3 matrix of the mixed-precision configuration (340 MB).  HIP events
+    around `steps` launches on the launch stream."""
+    import numpy as np
+    import torch
+
+    from amgt_amd import workloads as W
+    from amgt_amd.mbsr import MBSRMatrix
+
+    def timed(calls):
+        for f in calls[:4]:
+            f()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for k in range(steps):
+            calls[k % len(calls)]()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    out = {}
+    n = M.shape[0]
+    I, J, V = (t.to(dev) for t in M.to_csr())
+    M2 = MBSRMatrix.from_csr(I, J, V, n, n)
+    inf = M.info()
+    M2.set_spmv_kernel(inf["spmv_kernel"])
+    M2.set_spmv_lanes(inf["spmv_lanes"])
+    del I, J, V
+    y1 = torch.empty(n, dtype=torch.float64, device=dev)
+    y2 = torch.empty(n, dtype=torch.float64, device=dev)
+    x2 = x.clone()
+    B = spmv_bytes(inf)
+    ms = timed([lambda: M.spmv(x, y1), lambda: M2.spmv(x2, y2)])
+    out["alternating_two_7pt128"] = {
+        "ms": round(ms, 5), "bytes_per_call": int(B), "gbs": round(B / (ms * 1e-3) / 1e9, 1),
+        "frac_of_hbm_peak": round(B / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "working_set_mb": round(2 * B / 1e6, 1)}
+    del M2, x2, y2
+    n1 = 160
+    Ih, Jh, Vh = W.stencil(7, n1, n1, n1, c=(1.0, 1.0, 0.01))
+    n3 = n1 ** 3
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    M3 = MBSRMatrix.from_csr(t(Ih), t(Jh), t(Vh), n3, n3)
+    M3.autotune()
+    x3 = t(W.rand_normal(2, n3))
+    y3 = torch.empty(n3, dtype=torch.float64, device=dev)
+    B3 = spmv_bytes(M3.info())
+    ms3 = timed([lambda: M3.spmv(x3, y3)])
+    out["aniso_160"] = {
+        "ms": round(ms3, 5), "bytes_per_call": int(B3), "gbs": round(B3 / (ms3 * 1e-3) / 1e9, 1),
+        "frac_of_hbm_peak": round(B3 / (ms3 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "lanes": M3.info()["spmv_lanes"]}
     return out
 
 
@@ -187,7 +285,7 @@ def amg_timings(dev, n1=128):
     return out
 
 
-def boomer_timings(dev, n1=128):
+def boomer_timings(dev, n1=128, cpu=True):
     """The same 3D 7-pt Poisson 128^3 solve with BoomerAMG's algebraic hierarchy
     built on the device with ij's defaults: strength (theta 0.25, max row sum
     1), HMIS, extended+i with at most 4 entries per row, R = P^T, R (A P);
@@ -237,6 +335,8 @@ def boomer_timings(dev, n1=128):
         np.max(np.abs(sols["mixed"] - sols["fp64"])) / np.max(np.abs(sols["fp64"])))
     out["speedup_mixed_vs_fp64"] = round(out["fp64"]["setup_plus_solve_s"] /
                                          out["mixed"]["setup_plus_solve_s"], 3)
+    if cpu:
+        out["cpu_baseline"] = cpu_amg_baseline(I, J, V, b)
     out["hierarchy"] = ("algebraic on the device: strength, HMIS (coarsen type 10: Ruge first "
                         "pass on the host, PMIS on the device), extended+i (P max 4), R = P^T, "
                         "Galerkin R(AP) on the mBSR SpGEMM, l1-Jacobi weight 1; dense inverse "
@@ -537,6 +637,7 @@ def main():
     torch.cuda.synchronize()
     resid_ms = e0.elapsed_time(e1) / 50
 
+    cold = spmv_cold(dev, M, x)
     spgemm = None if args.no_spgemm else spgemm_timings(W, MBSRMatrix, M, I, J, A, dev)
     # conversion again, warm (code objects loaded, device blocks cached)
     tc = time.perf_counter()
@@ -547,7 +648,7 @@ def main():
     amg = None if args.no_amg else amg_timings(dev)
     spmm = spmm_timings(dev)
     batch = None if args.no_amg else batch_timings(dev)
-    boomer = None if args.no_amg else boomer_timings(dev)
+    boomer = None if args.no_amg else boomer_timings(dev, cpu=not args.no_cpu_baseline)
     policy = None if args.no_amg else policy_timings(dev)
     c3 = None if args.no_amg else c3_timings(dev)
 
@@ -577,6 +678,7 @@ def main():
         "spmv_alg_bytes": int(B),
         "kernel_ms": round(kern_ms, 5),
         "residual_form_ms": round(resid_ms, 5),
+        "spmv_cold_cache": cold,
         "spmv_kernel": SPMV_KERNEL_NAMES[kind],
         "variant_ms": {str(k): round(v, 5) for k, v in variants.items()},
         "conversion_ms": round(conv_ms, 2),

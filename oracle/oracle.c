@@ -398,6 +398,35 @@ void orc_csr_matvec(int nrows, const int *I, const int *J, const double *A, doub
     }
 }
 
+/* The same product with the rows split over nthreads OpenMP threads in
+ * contiguous static chunks, as HYPRE's OpenMP build of
+ * hypre_CSRMatrixMatvecOutOfPlaceHost splits them (csr_matvec.c:22-700 under
+ * HYPRE_USING_OPENMP); every row is summed as in orc_csr_matvec, so the result
+ * is the same bit for bit.  The CPU baseline of the SpMV timing. */
+void orc_csr_matvec_omp(int nthreads, int nrows, const int *I, const int *J, const double *A,
+                        double alpha, const double *x, double beta, const double *b, double *y)
+{
+    const double temp = alpha == 0.0 ? 0.0 : beta / alpha;
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (int i = 0; i < nrows; i++) {
+        if (alpha == 0.0) {
+            y[i] = beta * b[i];
+            continue;
+        }
+        double t = 0.0;
+        for (int p = I[i]; p < I[i + 1]; p++) t += A[p] * x[J[p]];
+        if (temp == 0.0) {
+            y[i] = (alpha == 1.0) ? t : ((alpha == -1.0) ? -t : alpha * t);
+        } else if (alpha == 1.0) {
+            y[i] = beta * b[i] + t;
+        } else if (alpha == -1.0) {
+            y[i] = -temp * b[i] - t;
+        } else {
+            y[i] = beta * b[i] + alpha * t;
+        }
+    }
+}
+
 /* hypre_CSRMatrixMatvecTHost, csr_matvec.c:914: y = alpha*A^T*x + beta*b,
  * scattered in row order. */
 void orc_csr_matvec_t(int nrows, int ncols, const int *I, const int *J, const double *A,

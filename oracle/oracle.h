@@ -79,6 +79,9 @@ void orc_mbsr_spmv_t(int nrows, int ncols, int brow, const int *blk_ptr, const i
  *      (single vector, no rownnz) and MatvecTHost csr_matvec.c:914 ---- */
 void orc_csr_matvec(int nrows, const int *I, const int *J, const double *A, double alpha,
                     const double *x, double beta, const double *b, double *y);
+/* orc_csr_matvec with the rows split over nthreads OpenMP threads (same results) */
+void orc_csr_matvec_omp(int nthreads, int nrows, const int *I, const int *J, const double *A,
+                        double alpha, const double *x, double beta, const double *b, double *y);
 void orc_csr_matvec_t(int nrows, int ncols, const int *I, const int *J, const double *A,
                       double alpha, const double *x, double beta, const double *b, double *y);
 

@@ -59,6 +59,8 @@ def lib():
             "orc_mbsr_spmv_t": (None, [_i32, _i32, _i32, _i32p, _i32p, _u16p, _i32p, _f64p, _f64,
                                        _f64p, _f64, _f64p, _f64p]),
             "orc_csr_matvec": (None, [_i32, _i32p, _i32p, _f64p, _f64, _f64p, _f64, _f64p, _f64p]),
+            "orc_csr_matvec_omp": (None, [_i32, _i32, _i32p, _i32p, _f64p, _f64, _f64p, _f64,
+                                          _f64p, _f64p]),
             "orc_csr_matvec_t": (None, [_i32, _i32, _i32p, _i32p, _f64p, _f64, _f64p, _f64, _f64p,
                                         _f64p]),
             "orc_csr_spgemm_count": (_i64, [_i32, _i32, _i32p, _i32p, _i32p, _i32p, _i32, _i32p]),
@@ -245,6 +247,15 @@ def csr_matvec(nrows, I, J, A, alpha, x, beta=0.0, b=None):
     y = np.empty(nrows, dtype=np.float64)
     bb = _cf(b) if b is not None else np.zeros(max(nrows, 1))
     lib().orc_csr_matvec(nrows, _c32(I), _nz32(J), _nzf(A), alpha, _nzf(x), beta, bb, y)
+    return y
+
+
+def csr_matvec_omp(nthreads, nrows, I, J, A, alpha, x, beta=0.0, b=None, out=None):
+    """csr_matvec with the rows split over nthreads OpenMP threads."""
+    y = np.empty(nrows, np.float64) if out is None else out
+    bb = _cf(b) if b is not None else np.zeros(max(nrows, 1))
+    lib().orc_csr_matvec_omp(int(nthreads), nrows, _c32(I), _c32(J), _cf(A), float(alpha),
+                             _cf(x), float(beta), bb, y)
     return y
 
 

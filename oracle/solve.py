@@ -1,5 +1,6 @@
-"""BoomerAMG's one-process solve phase restated with scipy, for checking
-hierarchies (the oracle's and the device's) against HYPRE's own outputs.
+"""BoomerAMG's one-process solve phase restated with scipy -- TEST
+INFRASTRUCTURE ONLY (the checker of hierarchies against HYPRE's own outputs,
+and bench.py's CPU baseline of setup + solve).
 
 * l1_norms: the smoother's diagonal, hypre_ParCSRComputeL1Norms
   (parcsr_ls/ams.c:527-800) as par_amg_setup.c selects it: relax type 7 uses
@@ -98,3 +99,22 @@ def pcg(A, b, precond, tol=1e-8, max_iter=1000):
 
 def complexities(levels_n, levels_nnz):
     return sum(levels_n) / levels_n[0], sum(levels_nnz) / levels_nnz[0]
+
+
+def setup_and_solve(n, I, J, A, b, relax=18, tol=1e-8, rap="kt"):
+    """HYPRE's CPU BoomerAMG-PCG on one process as the oracle restates it (ij's
+    defaults: HMIS, theta 0.25, max row sum 1, extended+i P max 4, coarse size
+    9, the host RAP order), timed: (x, iterations, final relative residual,
+    levels, setup seconds, solve seconds)."""
+    import time
+
+    from . import oracle as O
+    t0 = time.perf_counter()
+    levels = O.amg_hierarchy(n, I, J, A, theta=0.25, max_row_sum=1.0, max_elmts=4,
+                             max_coarse_size=9, coarsen="hmis", rap=rap)
+    mats = level_mats(levels)
+    M = VCycle(mats, relax=relax)
+    t1 = time.perf_counter()
+    x, it, rel = pcg(mats[0][0], b, M, tol=tol)
+    t2 = time.perf_counter()
+    return x, it, rel, [L["n"] for L in levels], t1 - t0, t2 - t1

@@ -173,13 +173,16 @@ def test_batch_blockdiag(dev):
     r = G.solve_batch(sizes, 4, dev, capture=True, coarse_size=8)
     amg = r["amg"]
     assert len(amg.levels) >= 3
-    # level 1 = block-diagonal of the standalone level-1 operators
+    # level 1 = block-diagonal of the problems' own Galerkin operators, each
+    # from the oracle's host SpGEMM on the host restatement of the trilinear P
     blocks = []
     for n in sizes:
         I, J, V = W.stencil(7, n, n, 1)
-        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
-        one = G.GeometricAMG(t(I), t(J), t(V), (n, n, 1), dev, coarse_size=0, max_levels=2)
-        blocks.append(to_sp(one.levels[1].A) if len(one.levels) > 1 else to_sp(one.levels[0].A))
+        PI, PJ, PA, nc = W.trilinear_interpolation(n, n, 1)
+        RI, RJ, RA = O.csr_transpose(n * n, nc, PI, PJ, PA)
+        AP = O.csr_spgemm(n * n, nc, I, J, V, PI, PJ, PA, False)
+        CI, CJ, CA = O.csr_spgemm(nc, nc, RI, RJ, RA, *AP, True)
+        blocks.append(sp.csr_matrix((CA, CJ, CI), shape=(nc, nc)))
     ref = sp.block_diag(blocks, format="csr")
     ref.sort_indices()
     got = to_sp(amg.levels[1].A)
@@ -201,3 +204,57 @@ def test_batch_blockdiag(dev):
     for k in range(len(sizes)):
         m = seg == k
         assert np.linalg.norm(res[m]) <= 1e-8 * np.linalg.norm(b[m]), k
+
+
+def test_batch_c4b_full_size(dev):
+    """C4(b) at its configured size: the 48 problems with sides drawn from
+    U{32..512} (Park-Miller seed 4), one block-diagonal hierarchy, one PCG.
+    For the smallest, the largest and three random problems, the problem's
+    block of the level-1 operator equals the oracle's R (A P) (HYPRE's host
+    SpGEMM) on that problem's own A and P (the host restatement of the
+    trilinear P): structure exact, values within 1e-12 of |R||A||P|.  Every
+    problem's true relative residual, recomputed on the host, is below 1e-8."""
+    sizes = G.batch_sizes(48, 4)
+    assert len(sizes) == 48 and min(sizes) >= 32 and max(sizes) <= 512
+    print(f"[c4b] {len(sizes)} problems, {sum(n * n for n in sizes)} rows", flush=True)
+    r = G.solve_batch(sizes, 4, dev, capture=True)
+    amg = r["amg"]
+    L1 = to_sp(amg.levels[1].A).tocsr()
+    L1.sort_indices()
+    coarse = [((n + 1) // 2) ** 2 for n in sizes]
+    offs = np.concatenate([[0], np.cumsum(coarse)])
+    rng = np.random.default_rng(4)
+    pick = sorted({int(np.argmin(sizes)), int(np.argmax(sizes))} |
+                  {int(k) for k in rng.choice(len(sizes), 3, replace=False)})
+    for k in pick:
+        n = sizes[k]
+        I, J, V = W.stencil(7, n, n, 1)
+        PI, PJ, PA, nc = W.trilinear_interpolation(n, n, 1)
+        assert nc == coarse[k]
+        RI, RJ, RA = O.csr_transpose(n * n, nc, PI, PJ, PA)
+        AP = O.csr_spgemm(n * n, nc, I, J, V, PI, PJ, PA, False)
+        CI, CJ, CA = O.csr_spgemm(nc, nc, RI, RJ, RA, *AP, True)
+        ref = sp.csr_matrix((CA, CJ, CI), shape=(nc, nc))
+        ref.sort_indices()
+        got = L1[offs[k]:offs[k + 1], offs[k]:offs[k + 1]].tocsr()
+        got.sort_indices()
+        # nothing of problem k's rows lies outside its diagonal block
+        assert L1[offs[k]:offs[k + 1]].nnz == got.nnz, k
+        assert np.array_equal(got.indptr, ref.indptr) and np.array_equal(got.indices, ref.indices), k
+        A_ = sp.csr_matrix((V, J, I), shape=(n * n, n * n))
+        P_ = sp.csr_matrix((PA, PJ, PI), shape=(n * n, nc))
+        scale = (abs(P_.T) @ abs(A_) @ abs(P_)).tocsr()
+        scale.sort_indices()
+        sc = np.asarray(scale[np.repeat(np.arange(nc), np.diff(ref.indptr)), ref.indices]).ravel()
+        assert np.all(np.abs(got.data - ref.data) <= 1e-12 * sc + 1e-300), k
+    I, J, V, b, xs, grids, seg = G.batch_poisson2d(sizes, 4, dev)
+    A = sp.csr_matrix((V, J, I), shape=(len(seg), len(seg)))
+    x = r["x"].cpu().numpy()
+    res = b - A @ x
+    worst = 0.0
+    for k in range(len(sizes)):
+        m = seg == k
+        rel = np.linalg.norm(res[m]) / np.linalg.norm(b[m])
+        worst = max(worst, rel)
+        assert rel < 1e-8, (k, sizes[k], rel)
+    print(f"[c4b] {r['iters']} iterations, worst problem residual {worst:.2e}", flush=True)

@@ -32,7 +32,7 @@ import torch
 from amgt_amd import amg as G
 from amgt_amd import coarsen as C
 from oracle import oracle as O
-from tests import util_amg as UA
+from oracle import solve as UA
 from tests.util_matrices import rand_csr
 
 pytestmark = pytest.mark.gpu

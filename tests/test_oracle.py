@@ -14,7 +14,8 @@ Pin map (oracle function -> tests):
   orc_mbsr_stats -> test_mbsr_stats
   orc_mbsr2csr_* -> test_mbsr_roundtrip
   orc_mbsr_spmv / _t -> test_mbsr_spmv_vs_scipy, test_spmv_closed_form
-  orc_csr_matvec / _t -> test_csr_matvec_vs_scipy
+  orc_csr_matvec / _t -> test_csr_matvec_vs_scipy; orc_csr_matvec_omp ->
+    test_csr_matvec_omp_equals_serial
   orc_csr_spgemm_* -> test_spgemm_nnz_fixtures, test_csr_spgemm_vs_scipy
   orc_mbsr_spgemm_* -> test_mbsr_spgemm_vs_scipy, test_mbsr_spgemm_brute_force
   orc_amg_strength / orc_amg_pmis / orc_amg_extpi_* (and the setup loop in
@@ -42,7 +43,7 @@ import pytest
 import scipy.sparse as sp
 
 from oracle import oracle as O
-from tests import util_amg as UA
+from oracle import solve as UA
 from tests.util_matrices import rand_csr, to_scipy
 
 GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "hypre_fixtures.json")))
@@ -244,6 +245,18 @@ def test_spmv_closed_form():
     x, yy = idx % 128, idx // 128
     exp = (x == 0).astype(int) + (x == 127) + (yy == 0) + (yy == 127)
     assert np.array_equal(y, exp.astype(float))  # exact: small integers
+
+
+def test_csr_matvec_omp_equals_serial():
+    # the threaded restatement (the SpMV CPU baseline) sums every row as the
+    # serial one: equal bit for bit, whatever the thread count
+    I, J, A = rand_csr(3001, 2500, 0.01, 17)
+    x = np.random.default_rng(4).standard_normal(2500)
+    b = np.random.default_rng(5).standard_normal(3001)
+    for alpha, beta in ((1.0, 0.0), (-1.0, 1.0), (2.5, -0.5), (0.0, 3.0)):
+        ref = O.csr_matvec(3001, I, J, A, alpha, x, beta, b)
+        for t in (1, 3, 8):
+            assert np.array_equal(O.csr_matvec_omp(t, 3001, I, J, A, alpha, x, beta, b), ref)
 
 
 def test_csr_matvec_vs_scipy():
