@@ -33,6 +33,7 @@ SPMV_KERNEL_NAMES = {0: "direct", 1: "staged", 2: "block", 3: "balanced"}
 AMGT_DOUBLE = 0
 AMGT_FLOAT = 1
 AMGT_HALF = 2
+AMGT_NOALLOC_SCRATCH_BYTES = 8 * 1024 * 1024
 
 POLICY_FEATURES = 12
 POLICY_OUTPUTS = 4
@@ -45,7 +46,8 @@ EXPORTS = (
     "amgt_mbsr_update_values", "amgt_mbsr_get_info", "amgt_mbsr_get_arrays",
     "amgt_mbsr_set_level", "amgt_mbsr_set_precision", "amgt_mbsr_set_spmv_lanes",
     "amgt_mbsr_set_spmv_kernel",
-    "amgt_mbsr_autotune", "amgt_mbsr_spmv", "amgt_mbsr_spmm", "amgt_mbsr_spgemm",
+    "amgt_mbsr_autotune", "amgt_mbsr_spmv", "amgt_mbsr_spmv_noalloc", "amgt_mbsr_spmm",
+    "amgt_mbsr_spgemm",
     "amgt_mbsr_to_csr", "amgt_free", "amgt_csr_spgemm", "amgt_csr_matvec",
     "amgt_mbsr_diagonal", "amgt_mbsr_relax_jacobi", "amgt_mbsr_transpose",
     "amgt_read_ij", "amgt_read_mm", "amgt_host_free", "amgt_mbsr_spmm_interleaved",
@@ -124,6 +126,7 @@ def load():
         "amgt_mbsr_set_spmv_kernel": (_int, [_vp, _int]),
         "amgt_mbsr_autotune": (_int, [_vp, _vp]),
         "amgt_mbsr_spmv": (_int, [_vp, _int, _dbl, _vp, _dbl, _vp, _vp, _vp]),
+        "amgt_mbsr_spmv_noalloc": (_int, [_vp, _dbl, _vp, _dbl, _vp, _vp, _vp]),
         "amgt_mbsr_spmm": (_int, [_vp, _int, _int, _dbl, _vp, _i64, _dbl, _vp, _i64, _vp, _i64,
                                   _vp]),
         "amgt_mbsr_spgemm": (_int, [_vp, _vp, _vp, _pvp]),

@@ -125,6 +125,14 @@ int balanced_bpt(int lanes);
 void mbsr_release_balanced(Mbsr *A, hipStream_t s);
 int spmv(const Mbsr &A, int precision, int lanes, double alpha, const double *x, double beta,
          const double *b, double *y, hipStream_t s);
+// y = alpha*A*x + beta*b (FP64 values) with the balanced kernel over a caller's
+// workspace of ws_bytes instead of the handle's tile tables: nothing is
+// allocated, the tables are rebuilt on every call.  AMGT_ERR_UNSUPPORTED when
+// the workspace is smaller than spmv_balanced_ws_bytes(A, lanes).  x must be
+// 16-byte aligned.
+size_t spmv_balanced_ws_bytes(const Mbsr &A, int lanes);
+int spmv_balanced_ws(const Mbsr &A, int lanes, double alpha, const double *x, double beta,
+                     const double *b, double *y, void *ws, size_t ws_bytes, hipStream_t s);
 int spmm(const Mbsr &A, int precision, int lanes, int k, double alpha, const double *x,
          int64_t ldx, double beta, const double *b, int64_t ldb, double *y, int64_t ldy,
          hipStream_t s);

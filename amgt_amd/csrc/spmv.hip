@@ -772,6 +772,33 @@ __global__ __launch_bounds__(256) void k_tile_popc(int32_t nblk, int32_t capb,
     if (threadIdx.x == 0) tile_cnt[t] = s_part[0] + s_part[1] + s_part[2] + s_part[3];
 }
 
+// Exclusive scan of v[0..n-1] in place into v[0..n] (v[n] = total) by one
+// workgroup of 1024 threads, each owning a contiguous chunk: the balanced
+// kernel's value offsets per tile when no scan temporaries may be allocated.
+__global__ __launch_bounds__(1024) void k_scan_excl_1wg(int32_t n, int32_t *__restrict__ v) {
+    __shared__ int32_t s_w[16];
+    const int tid = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int i0 = min(n, tid * per), i1 = min(n, i0 + per);
+    int sum = 0;
+    for (int i = i0; i < i1; ++i) sum += v[i];
+    int incl = sum;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(incl, d);
+        if ((tid & 63) >= d) incl += t;
+    }
+    if ((tid & 63) == 63) s_w[tid >> 6] = incl;
+    __syncthreads();
+    int off = incl - sum;
+    for (int w = 0; w < (tid >> 6); ++w) off += s_w[w];
+    for (int i = i0; i < i1; ++i) {
+        const int c = v[i];
+        v[i] = off;
+        off += c;
+    }
+    if (tid == 1023) v[n] = off;  // the last chunk ends at the total
+}
+
 template <int G, typename VT, typename AT, int CAPV, int CAPB, int MAXB, bool XSEG, typename EP>
 __global__ __launch_bounds__(256) void k_spmv_staged(
     int32_t brow, int32_t nrows, int32_t ncols, const int32_t *__restrict__ blk_ptr,
@@ -1147,17 +1174,29 @@ int launch_spmv_blk_g(const Mbsr &A, const VT *val, const double *x, EP ep, doub
     return AMGT_OK;
 }
 
+// Tile tables and carry buffer of the balanced kernel: the handle's own
+// (spmv_prepare) or a caller's workspace (spmv_balanced_ws).
+struct BalTables {
+    int32_t ntiles, capb;
+    const int32_t *lo, *head, *vb;
+    double *carry;
+};
+
+inline BalTables handle_tables(const Mbsr &A) {
+    return BalTables{A.bal_ntiles, A.bal_capb, A.bal_lo, A.bal_head, A.bal_vb, A.bal_carry};
+}
+
 template <int BPT, typename VT, typename AT, typename EP>
-int launch_spmv_bal_g(const Mbsr &A, const VT *val, const double *x, EP ep, double *y,
-                      hipStream_t s) {
-    const int nt = A.bal_ntiles;
+int launch_spmv_bal_g(const Mbsr &A, const BalTables &tb, const VT *val, const double *x, EP ep,
+                      double *y, hipStream_t s) {
+    const int nt = tb.ntiles;
     hipLaunchKernelGGL((k_spmv_bal<BPT, VT, AT, EP>), dim3(unsigned(nt)), dim3(256), 0, s, nt,
                        int32_t(A.nblk), A.nrows, A.ncols, A.blk_ptr, A.blk_col, A.blk_map, val,
-                       A.bal_lo, A.bal_head, A.bal_vb, A.bal_carry, x, ep, y);
+                       tb.lo, tb.head, tb.vb, tb.carry, x, ep, y);
     AMGT_HIP(hipGetLastError());
     hipLaunchKernelGGL((k_spmv_bal_fix<EP>), dim3(unsigned((4 * int64_t(nt) + 255) / 256)),
-                       dim3(256), 0, s, nt, A.bal_capb, int32_t(A.nblk), A.nrows, A.blk_ptr,
-                       A.bal_lo, A.bal_head, A.bal_carry, ep, y);
+                       dim3(256), 0, s, nt, tb.capb, int32_t(A.nblk), A.nrows, A.blk_ptr,
+                       tb.lo, tb.head, tb.carry, ep, y);
     AMGT_HIP(hipGetLastError());
     return AMGT_OK;
 }
@@ -1175,10 +1214,11 @@ int launch_spmv_t(const Mbsr &A, const VT *val, int lanes, const double *x, EP e
     const bool x_aligned = (reinterpret_cast<uintptr_t>(x) & 15u) == 0;
     if (A.spmv_kernel == kSpmvBalanced && x_aligned && A.bal_ntiles > 0 &&
         A.bal_capb == 256 * balanced_bpt(lanes)) {
+        const BalTables tb = handle_tables(A);
         switch (balanced_bpt(lanes)) {
-            case 1: return launch_spmv_bal_g<1, VT, AT>(A, val, x, ep, y, s);
-            case 4: return launch_spmv_bal_g<4, VT, AT>(A, val, x, ep, y, s);
-            default: return launch_spmv_bal_g<2, VT, AT>(A, val, x, ep, y, s);
+            case 1: return launch_spmv_bal_g<1, VT, AT>(A, tb, val, x, ep, y, s);
+            case 4: return launch_spmv_bal_g<4, VT, AT>(A, tb, val, x, ep, y, s);
+            default: return launch_spmv_bal_g<2, VT, AT>(A, tb, val, x, ep, y, s);
         }
     }
     // values are staged in 16-byte pieces (handle arrays always are aligned)
@@ -1370,6 +1410,65 @@ int spmv_prepare(Mbsr *A, int lanes, hipStream_t s) {
     A->bal_capb = capb;
     A->bal_ntiles = int32_t(nt);
     return AMGT_OK;
+}
+
+// Workspace of the balanced kernel for nt tiles: tile_lo[nt+1], tile_head[nt],
+// tile_vb[nt+1] (16-byte aligned pieces), then carry[8 nt] doubles.
+struct BalWsLayout {
+    size_t head, vb, carry, bytes;
+};
+
+static size_t up16(size_t b) { return (b + 15) & ~size_t(15); }
+
+static BalWsLayout bal_ws_layout(int64_t nt) {
+    BalWsLayout w;
+    w.head = up16(4 * size_t(nt + 1));
+    w.vb = w.head + up16(4 * size_t(nt));
+    w.carry = w.vb + up16(4 * size_t(nt + 1));
+    w.bytes = w.carry + 64 * size_t(nt);
+    return w;
+}
+
+size_t spmv_balanced_ws_bytes(const Mbsr &A, int lanes) {
+    if (A.nblk == 0) return 0;
+    const int64_t capb = 256 * balanced_bpt(lanes);
+    return bal_ws_layout((A.nblk + capb - 1) / capb).bytes;
+}
+
+int spmv_balanced_ws(const Mbsr &A, int lanes, double alpha, const double *x, double beta,
+                     const double *b, double *y, void *ws, size_t ws_bytes, hipStream_t s) {
+    if (lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64) {
+        set_error("lanes per block row must be 4, 8, 16, 32 or 64");
+        return AMGT_ERR_ARG;
+    }
+    if (A.nrows == 0) return AMGT_OK;
+    if (A.nblk == 0 || alpha == 0.0) return scale_vector(A.nrows, beta, b, y, s);
+    const int capb = 256 * balanced_bpt(lanes);
+    const int64_t nt = (A.nblk + capb - 1) / capb;
+    const BalWsLayout L = bal_ws_layout(nt);
+    if (L.bytes > ws_bytes) {
+        set_error("balanced SpMV workspace: " + std::to_string(L.bytes) + " bytes needed for " +
+                  std::to_string(nt) + " tiles, " + std::to_string(ws_bytes) + " available");
+        return AMGT_ERR_UNSUPPORTED;
+    }
+    char *base = static_cast<char *>(ws);
+    int32_t *lo = reinterpret_cast<int32_t *>(base);
+    int32_t *head = reinterpret_cast<int32_t *>(base + L.head);
+    int32_t *vb = reinterpret_cast<int32_t *>(base + L.vb);
+    double *carry = reinterpret_cast<double *>(base + L.carry);
+    hipLaunchKernelGGL(k_tile_rows, dim3(unsigned((nt + 1 + 255) / 256)), dim3(256), 0, s,
+                       int32_t(nt), capb, A.brow, A.blk_ptr, lo, head);
+    hipLaunchKernelGGL(k_tile_popc, dim3(unsigned(nt)), dim3(256), 0, s, int32_t(A.nblk), capb,
+                       A.blk_map, vb);
+    hipLaunchKernelGGL(k_scan_excl_1wg, dim3(1), dim3(1024), 0, s, int32_t(nt), vb);
+    AMGT_HIP(hipGetLastError());
+    const BalTables tb{int32_t(nt), capb, lo, head, vb, carry};
+    const EpAxpby ep{alpha, beta, b};
+    switch (balanced_bpt(lanes)) {
+        case 1: return launch_spmv_bal_g<1, double, double>(A, tb, A.val, x, ep, y, s);
+        case 4: return launch_spmv_bal_g<4, double, double>(A, tb, A.val, x, ep, y, s);
+        default: return launch_spmv_bal_g<2, double, double>(A, tb, A.val, x, ep, y, s);
+    }
 }
 
 int mbsr_ensure_lowp(Mbsr *A, int *prec, hipStream_t s) {

@@ -127,6 +127,22 @@ int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream);
  * FP64. */
 int amgt_mbsr_spmv(amgt_mbsr_t A, int trans, double alpha, const double *d_x, double beta,
                    const double *d_b, double *d_y, void *stream);
+/* ---- SpMV that allocates nothing ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  y = alpha*A*x + beta*b over the handle's FP64 values with
+ * the balanced kernel (AMGT_SPMV_BALANCED at the handle's lanes setting; the
+ * same results as amgt_mbsr_spmv with that kernel and FP64).  Its tile tables
+ * and carry sums live in one library-owned device array of
+ * AMGT_NOALLOC_SCRATCH_BYTES per GPU (76 bytes per tile of 256/512/1024
+ * blocks: up to about 28M/56M/112M blocks), rebuilt on every call by three
+ * small launches; larger matrices return AMGT_ERR_UNSUPPORTED.  Calls that
+ * overlap on one GPU (two streams or host threads) share the array and race:
+ * serialise them.  x must be 16-byte aligned; b may equal y; x must not alias
+ * y; beta == 0 never reads b. */
+#define AMGT_NOALLOC_SCRATCH_BYTES (8u * 1024u * 1024u)
+int amgt_mbsr_spmv_noalloc(amgt_mbsr_t A, double alpha, const double *d_x, double beta,
+                           const double *d_b, double *d_y, void *stream);
 /* k right-hand sides, column-major with leading dimensions ldx/ldb/ldy
  * (HYPRE multivector storage method 0, vector.c) */
 int amgt_mbsr_spmm(amgt_mbsr_t A, int trans, int k, double alpha, const double *d_x, int64_t ldx,
