@@ -52,7 +52,7 @@ EXPORTS = (
     "amgt_mbsr_diagonal", "amgt_mbsr_relax_jacobi", "amgt_mbsr_transpose",
     "amgt_read_ij", "amgt_read_mm", "amgt_host_free", "amgt_mbsr_spmm_interleaved",
     "amgt_amg_strength", "amgt_amg_pmis", "amgt_amg_extpi_interp", "amgt_amg_ruge_first_pass",
-    "amgt_amg_hmis",
+    "amgt_amg_hmis", "amgt_amg_rap_host_order",
     "amgt_policy_work_bytes", "amgt_policy_forward", "amgt_policy_loss_grad", "amgt_policy_adam",
     "amgt_level_stats_work_bytes", "amgt_level_stats", "amgt_interp_trilinear",
 )
@@ -149,6 +149,8 @@ def load():
         "amgt_amg_pmis": (_int, [_i32, _vp, _vp, _int, _vp, ctypes.POINTER(_i32), _vp]),
         "amgt_amg_ruge_first_pass": (_int, [_i32, _vp, _vp, _vp]),
         "amgt_amg_hmis": (_int, [_i32, _vp, _vp, _vp, ctypes.POINTER(_i32), _vp]),
+        "amgt_amg_rap_host_order": (_int, [_i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _pi64, _pvp,
+                                           _pvp, _pvp, _vp]),
         "amgt_amg_extpi_interp": (_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _int,
                                          ctypes.POINTER(_i32), _pi64, _pvp, _pvp, _pvp, _vp]),
         "amgt_policy_work_bytes": (ctypes.c_size_t, [_i64]),

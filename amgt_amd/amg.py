@@ -428,14 +428,30 @@ class BoomerAMG(GeometricAMG):
     `level_hook(lvl, n_prev, A, CI, CJ, CV) -> (theta, precision)`, when given,
     is called for every level before it is coarsened and sets that level's
     strength threshold and SpMV/smoother precision (replacing `theta` and
-    `spmv_policy`); the Galerkin products keep `spgemm_policy`."""
+    `spmv_policy`); the Galerkin products keep `spgemm_policy`.
+
+    `galerkin` picks the coarse-operator product: "mbsr" (default) is the
+    mBSR SpGEMM R (A P), the product hypre_CSRMatrixMultiplyDevice runs; its
+    sums run in another order than HYPRE's host RAP, so coarse levels agree
+    with HYPRE's CPU hierarchy to round-off, and where round-off decides a tie
+    (strength threshold, truncation order) the coarsening can differ.
+    "host_order" forms R A P in HYPRE's host order in FP64
+    (amgt_amg_rap_host_order) and keeps every level's CSR in HYPRE's layout
+    (`csr_levels`), so the whole hierarchy equals HYPRE's CPU one bit for
+    bit; `spgemm_policy` does not apply to it."""
 
     def __init__(self, I, J, V, n, device, theta: float = 0.25, max_row_sum: float = 1.0,
                  cf_init: int = 0, max_elmts: int = 4, omega: float = 1.0, coarse_size: int = 9,
                  spmv_policy=fp64_policy, spgemm_policy=fp64_policy, max_levels: int = 25,
-                 level_hook=None, coarsen: str = "hmis", relax_type: int = 18):
+                 level_hook=None, coarsen: str = "hmis", relax_type: int = 18,
+                 galerkin: str = "mbsr"):
         if coarsen not in ("hmis", "pmis"):
             raise ValueError(f"coarsen {coarsen!r}: 'hmis' or 'pmis'")
+        if galerkin not in ("mbsr", "host_order"):
+            raise ValueError(f"galerkin {galerkin!r}: 'mbsr' or 'host_order'")
+        self.galerkin = galerkin
+        self.csr_levels: list = []
+        self.p_csr: list = []
         self.device = device
         self.omega = float(omega)
         self.relax_type = _check_relax(relax_type)
@@ -496,12 +512,31 @@ class BoomerAMG(GeometricAMG):
             self.cf.append(cf)
             P = MBSRMatrix.from_csr(PI, PJ, PA, n, nc)
             R = P.transpose()
+            if self.galerkin == "host_order":
+                self.csr_levels.append((CI, CJ, CV))
+                self.p_csr.append((PI, PJ, PA))
+                for M in (P, R):
+                    M.set_level(lvl)
+                    M.set_precision(self.level_precision(lvl))
+                torch.cuda.synchronize()
+                tg = time.perf_counter()
+                RI_, RJ_, RA_ = C.rap_host_order(n, nc, CI, CJ, CV, PI, PJ, PA)
+                Ac = MBSRMatrix.from_csr(RI_, RJ_, RA_, nc, nc)
+                torch.cuda.synchronize()
+                self.times.spgemm_s.append(time.perf_counter() - tg)
+                L_.P, L_.R = P, R
+                A, n_prev, n = Ac, n, nc
+                CI, CJ, CV = RI_, RJ_, RA_
+                lvl += 1
+                continue
             del PI, PJ, PA
             Ac = self._galerkin(lvl, A, P, R)
             L_.P, L_.R = P, R
             A, n_prev, n = Ac, n, nc
             CI, CJ, CV = A.to_csr()
             lvl += 1
+        if self.galerkin == "host_order":
+            self.csr_levels.append((CI, CJ, CV))
         self._finish(t0)
 
     def level_precision(self, lvl: int) -> int:
@@ -519,10 +554,11 @@ class BoomerAMG(GeometricAMG):
 def solve_poisson_boomer(nx: int, ny: int, nz: int, b: np.ndarray, device, mixed: bool = False,
                          tol: float = 1e-8, capture: bool = True, max_elmts: int = 4,
                          coarse_size: int = 9, theta: float = 0.25, max_row_sum: float = 1.0,
-                         omega: float = 1.0, coarsen: str = "hmis", relax_type: int = 18):
+                         omega: float = 1.0, coarsen: str = "hmis", relax_type: int = 18,
+                         galerkin: str = "mbsr"):
     """7-pt Poisson with the algebraic hierarchy (device strength, HMIS or
     PMIS, extended+i; ij's defaults): setup + PCG/V(1,1) solve; timings as
-    solve_poisson."""
+    solve_poisson.  `galerkin` as for BoomerAMG."""
     I, J, V = W.stencil(7, nx, ny, nz)
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
     dI, dJ, dV = t(I), t(J), t(V)
@@ -532,7 +568,7 @@ def solve_poisson_boomer(nx: int, ny: int, nz: int, b: np.ndarray, device, mixed
                     cf_init=0, max_elmts=max_elmts, omega=omega, coarse_size=coarse_size,
                     spmv_policy=mixed_spmv_policy if mixed else fp64_policy,
                     spgemm_policy=mixed_spgemm_policy if mixed else fp64_policy,
-                    coarsen=coarsen, relax_type=relax_type)
+                    coarsen=coarsen, relax_type=relax_type, galerkin=galerkin)
     tc = time.perf_counter()
     if capture:
         amg.capture()

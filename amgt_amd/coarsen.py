@@ -112,3 +112,23 @@ def extpi_interp(n: int, I: torch.Tensor, J: torch.Tensor, A: torch.Tensor, SI: 
     PJ = copy_from_device_ptr(pj.value, nnz.value, torch.int32, I.device, stream)
     PA = copy_from_device_ptr(pa.value, nnz.value, torch.float64, I.device, stream)
     return PI, PJ, PA, int(nc.value)
+
+
+def rap_host_order(n: int, nc: int, I: torch.Tensor, J: torch.Tensor, A: torch.Tensor,
+                   PI: torch.Tensor, PJ: torch.Tensor, PA: torch.Tensor, stream=None):
+    """R A P with R = P^T in HYPRE's host order (par_rap.c:1525-1930): the
+    coarse operator (RAP_i, RAP_j, RAP_a) bit for bit as HYPRE's CPU setup
+    forms it, rows diagonal-first in HYPRE's layout."""
+    _check_csr(n, I, J, A)
+    _check_csr(n, PI, PJ, PA)
+    lib = L.load()
+    nnz = ctypes.c_int64()
+    ci, cj, ca = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    L.check(lib.amgt_amg_rap_host_order(n, nc, _p(I), _p(J), _p(A), _p(PI), _p(PJ), _p(PA),
+                                        ctypes.byref(nnz), ctypes.byref(ci), ctypes.byref(cj),
+                                        ctypes.byref(ca), _stream_ptr(stream)),
+            "amgt_amg_rap_host_order")
+    CI = copy_from_device_ptr(ci.value, nc + 1, torch.int32, I.device, stream)
+    CJ = copy_from_device_ptr(cj.value, nnz.value, torch.int32, I.device, stream)
+    CA = copy_from_device_ptr(ca.value, nnz.value, torch.float64, I.device, stream)
+    return CI, CJ, CA

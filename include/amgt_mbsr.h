@@ -267,6 +267,19 @@ int amgt_amg_extpi_interp(int32_t n, const int32_t *A_i, const int32_t *A_j,
                           int64_t *nnzP, int32_t **P_i, int32_t **P_j, double **P_data,
                           void *stream);
 
+/* The Galerkin product R A P (R = P^T) in HYPRE's host order,
+ * hypre_BoomerAMGBuildCoarseOperatorKT par_rap.c:1525-1930 on one process: R A with
+ * first-touch column slots over R's rows (fine rows ascending) and A's rows in their
+ * stored order, then (R A) P with the diagonal first and first-touch slots; products and
+ * sums rounded one at a time.  The coarse operator (n_coarse x n_coarse, device CSR, rows
+ * in HYPRE's layout) equals HYPRE's host one bit for bit, so a hierarchy built with it
+ * equals HYPRE's CPU hierarchy level by level.  Device inputs (A n x n, P n x n_coarse,
+ * any row order); outputs freed with amgt_free.  Synchronises the stream. */
+int amgt_amg_rap_host_order(int32_t n, int32_t n_coarse, const int32_t *A_i, const int32_t *A_j,
+                            const double *A_data, const int32_t *P_i, const int32_t *P_j,
+                            const double *P_data, int64_t *nnz, int32_t **RAP_i,
+                            int32_t **RAP_j, double **RAP_data, void *stream);
+
 /* ---- geometric prolongation (the V-cycle driver's structured hierarchy; no
  *      counterpart in the reference, whose BoomerAMG builds P algebraically,
  *      amgt_amg_extpi_interp).  Outputs freed with amgt_free. ---- */

@@ -376,3 +376,93 @@ def test_pcg_with_algebraic_vcycle(dev):
     # HMIS + extended+i (P max 4) + l1-Jacobi weight 1
     assert hist[-1] < 1e-8 and iters <= 14, (iters, hist[-1])
     assert np.max(np.abs(x.cpu().numpy() - xs)) < 1e-5 * np.max(np.abs(xs))
+
+
+# ---- the Galerkin product in HYPRE's host order and bit-exact hierarchies ----
+def _rand_p(n, nc, seed):
+    """Random P-like rectangular CSR: 1-5 entries per row in random order."""
+    rng = np.random.default_rng(seed)
+    rows = [rng.permutation(nc)[:rng.integers(1, 6)] for _ in range(n)]
+    PI = np.zeros(n + 1, np.int32)
+    PI[1:] = np.cumsum([len(r) for r in rows])
+    PJ = np.concatenate(rows).astype(np.int32)
+    return PI, PJ, rng.standard_normal(len(PJ))
+
+
+@pytest.mark.parametrize("name", ["7pt_10", "27pt_neg", "dfirst_long", "7pt_aniso", "rand_m",
+                                  "single"])
+def test_rap_host_order_matches_oracle(dev, name):
+    """Device R A P in HYPRE's host order against the oracle's restatement of
+    hypre_BoomerAMGBuildCoarseOperatorKT: row pointers, columns in row order
+    and values bit for bit, on P from extended+i and on a random P."""
+    print(f"[rap_host] {name}", flush=True)
+    n, I, J, A = CASES[name]()
+    rSI, rSJ = O.amg_strength(n, I, J, A, 0.25, 1.0)
+    rcf, rnc = O.amg_hmis(n, rSI, rSJ)
+    rPI, rPJ, rPA, _ = O.amg_extpi(n, I, J, A, rSI, rSJ, rcf, 4)
+    nc_rand = max(1, n // 3)
+    for PI, PJ, PA, nc in ((rPI, rPJ, rPA, rnc), _rand_p(n, nc_rand, 5) + (nc_rand,)):
+        if nc == 0:
+            continue
+        ref = O.amg_rap_kt(n, nc, I, J, A, PI, PJ, PA)
+        got = C.rap_host_order(n, nc, T(I, dev), T(J, dev), T(A, dev), T(PI, dev), T(PJ, dev),
+                               T(PA, dev))
+        for g, r in zip(got, ref):
+            assert np.array_equal(g.cpu().numpy(), r)
+
+
+def _hierarchy_equals_oracle(amg, levels):
+    """Every level of a host-order device hierarchy equal to the oracle's
+    (HYPRE's host setup): A_l in HYPRE's row layout, C/F markers and P, bit for
+    bit."""
+    assert [L_.n for L_ in amg.levels] == [L_["n"] for L_ in levels]
+    for lvl, ref in enumerate(levels):
+        CI, CJ, CV = (t.cpu().numpy() for t in amg.csr_levels[lvl])
+        assert np.array_equal(CI, ref["I"]) and np.array_equal(CJ, ref["J"]), lvl
+        assert np.array_equal(CV, ref["A"]), lvl
+        if "P" in ref:
+            assert np.array_equal(amg.cf[lvl].cpu().numpy(), ref["cf"]), lvl
+            for g, r in zip(amg.p_csr[lvl], ref["P"]):
+                assert np.array_equal(g.cpu().numpy(), r), lvl
+
+
+def test_host_order_hierarchy_reproduces_ij_fsai_fixture(dev):
+    """TEST_ij fsai.out.103 with the host-order Galerkin product: the device
+    hierarchy equals HYPRE's host one bit for bit, and PCG gives the fixture's
+    22 iterations and final relative residual 7.480945e-09 to every printed
+    digit."""
+    g = GOLD["ij_fsai_103"]
+    n, I, J, A = CASES["7pt_10"]()
+    amg = G.BoomerAMG(T(I, dev), T(J, dev), T(A, dev), n, dev, coarsen="hmis", relax_type=7,
+                      galerkin="host_order")
+    levels = O.amg_hierarchy(n, I, J, A, theta=0.25, max_row_sum=1.0, max_elmts=4,
+                             max_coarse_size=9, coarsen="hmis", rap="kt")
+    _hierarchy_equals_oracle(amg, levels)
+    x, it, hist = _pcg_ones_rhs(amg, dev, ones_solution=False)
+    print(f"[fsai host order] iterations {it} final {hist[-1]:e}", flush=True)
+    assert it == g["iterations"] and "%e" % hist[-1] == g["final_relative_residual"]
+
+
+@pytest.mark.parametrize("cfg", ["c1_5pt_128", "c2_7pt_128"])
+def test_host_order_hierarchy_matches_hypre_cpu_path(dev, cfg):
+    """C1 (5-pt 128^2) and C2 (7-pt 128^3) at full size with ij's defaults
+    (HMIS, theta 0.25, extended+i P max 4, relax 18 weight 1, b = A*1): every
+    level equals HYPRE's host hierarchy bit for bit; levels, complexities (6
+    digits), iterations and final residual (4 digits) are HYPRE's CPU
+    path's."""
+    g = GOLD["cpu_probe"][cfg]
+    nz = 1 if cfg.startswith("c1") else 128
+    I, J, A = O.stencil(7, 128, 128, nz)
+    n = 128 * 128 * nz
+    amg = G.BoomerAMG(T(I, dev), T(J, dev), T(A, dev), n, dev, galerkin="host_order")
+    print(f"[{cfg} host order] levels {[L_.n for L_ in amg.levels]} "
+          f"setup {amg.times.total_s:.3f} s", flush=True)
+    levels = O.amg_hierarchy(n, I, J, A, theta=0.25, max_row_sum=1.0, max_elmts=4,
+                             max_coarse_size=9, coarsen="hmis", rap="kt")
+    _hierarchy_equals_oracle(amg, levels)
+    gc, oc = amg.complexities()
+    assert len(amg.levels) == g["levels"]
+    assert round(gc, 6) == g["grid_complexity"] and round(oc, 6) == g["operator_complexity"]
+    x, it, hist = _pcg_ones_rhs(amg, dev, ones_solution=True)
+    print(f"[{cfg} host order] iterations {it} final {hist[-1]:e}", flush=True)
+    assert it == g["iterations"] and "%.3e" % hist[-1] == g["final_relative_residual"]
