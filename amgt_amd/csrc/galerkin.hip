@@ -187,6 +187,15 @@ __global__ __launch_bounds__(kThreads) void k_compact(int32_t m, const int64_t *
     }
 }
 
+__global__ __launch_bounds__(kThreads) void k_gather_pattern(int32_t nnz,
+                                                             const int32_t *__restrict__ perm,
+                                                             const int32_t *__restrict__ row_of,
+                                                             int32_t *__restrict__ tj) {
+    const int q = blockIdx.x * kThreads + threadIdx.x;
+    if (q >= nnz) return;
+    tj[q] = row_of[perm[q]];
+}
+
 int d2h_sync(void *dst, const void *src, size_t bytes, hipStream_t s) {
     AMGT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
     AMGT_HIP(hipStreamSynchronize(s));
@@ -271,6 +280,44 @@ int first_touch_product(int32_t m, const int32_t *XI, const int32_t *XJ, const d
 
 }  // namespace
 
+int csr_transpose_device(int32_t nrows, int32_t ncols, const int32_t *I, const int32_t *J,
+                         const double *V, int32_t nnz, int32_t *TI, int32_t *TJ, double *TV,
+                         hipStream_t s) {
+    Tmp tmp(s);
+    int32_t *row_of = nullptr, *idx = nullptr, *perm = nullptr, *keys = nullptr;
+    AMGT_TRY(tmp.get(&row_of, size_t(nnz)));
+    AMGT_TRY(tmp.get(&idx, size_t(nnz)));
+    AMGT_TRY(tmp.get(&perm, size_t(nnz)));
+    AMGT_TRY(tmp.get(&keys, size_t(nnz)));
+    if (nnz > 0) {
+        hipLaunchKernelGGL(k_row_of, dim3(grid_of(nrows)), dim3(kThreads), 0, s, nrows, I, row_of);
+        hipLaunchKernelGGL(k_iota, dim3(grid_of(nnz)), dim3(kThreads), 0, s, int64_t(nnz), idx);
+        AMGT_HIP(hipGetLastError());
+        int bits = 1;
+        while (bits < 31 && (int64_t(1) << bits) < int64_t(ncols)) ++bits;
+        size_t bytes = 0;
+        AMGT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, J, keys, idx, perm, nnz, 0,
+                                                    bits, s));
+        char *sort_tmp = nullptr;
+        AMGT_TRY(tmp.get(&sort_tmp, bytes));
+        AMGT_HIP(hipcub::DeviceRadixSort::SortPairs(sort_tmp, bytes, J, keys, idx, perm, nnz, 0,
+                                                    bits, s));
+        if (V) {
+            hipLaunchKernelGGL(k_gather_t, dim3(grid_of(nnz)), dim3(kThreads), 0, s, nnz, perm,
+                               row_of, V, TJ, TV);
+        } else {
+            hipLaunchKernelGGL(k_gather_pattern, dim3(grid_of(nnz)), dim3(kThreads), 0, s, nnz,
+                               perm, row_of, TJ);
+        }
+        AMGT_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_rowptr_sorted, dim3(grid_of(int64_t(ncols) + 1)), dim3(kThreads), 0, s,
+                       ncols, nnz, keys, TI);
+    AMGT_HIP(hipGetLastError());
+    // the temporaries go back to the pool stream-ordered: no wait needed
+    return AMGT_OK;
+}
+
 }  // namespace amgt
 
 using namespace amgt;
@@ -297,37 +344,13 @@ int amgt_amg_rap_host_order(int32_t n, int32_t nc, const int32_t *A_i, const int
     }
     AMGT_CHECK_ARG(nnzP == 0 || (P_j && P_data), "NULL P column or value array");
     AMGT_CHECK_ARG(nnzA == 0 || (A_j && A_data), "NULL A column or value array");
-    // R = P^T, rows ascending in the fine index (a stable sort by column)
-    int32_t *row_of = nullptr, *idx = nullptr, *perm = nullptr, *keys = nullptr;
+    // R = P^T, rows ascending in the fine index
     int32_t *ri = nullptr, *rj = nullptr;
     double *ra = nullptr;
-    AMGT_TRY(tmp.get(&row_of, size_t(nnzP)));
-    AMGT_TRY(tmp.get(&idx, size_t(nnzP)));
-    AMGT_TRY(tmp.get(&perm, size_t(nnzP)));
-    AMGT_TRY(tmp.get(&keys, size_t(nnzP)));
     AMGT_TRY(tmp.get(&ri, size_t(nc) + 1));
     AMGT_TRY(tmp.get(&rj, size_t(nnzP)));
     AMGT_TRY(tmp.get(&ra, size_t(nnzP)));
-    if (nnzP > 0) {
-        hipLaunchKernelGGL(k_row_of, dim3(grid_of(n)), dim3(kThreads), 0, s, n, P_i, row_of);
-        hipLaunchKernelGGL(k_iota, dim3(grid_of(nnzP)), dim3(kThreads), 0, s, int64_t(nnzP), idx);
-        AMGT_HIP(hipGetLastError());
-        int bits = 1;
-        while (bits < 31 && (int64_t(1) << bits) < int64_t(nc)) ++bits;
-        size_t bytes = 0;
-        AMGT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, P_j, keys, idx, perm, nnzP, 0,
-                                                    bits, s));
-        void *sort_tmp = nullptr;
-        AMGT_TRY(tmp.get(reinterpret_cast<char **>(&sort_tmp), bytes));
-        AMGT_HIP(hipcub::DeviceRadixSort::SortPairs(sort_tmp, bytes, P_j, keys, idx, perm, nnzP, 0,
-                                                    bits, s));
-        hipLaunchKernelGGL(k_gather_t, dim3(grid_of(nnzP)), dim3(kThreads), 0, s, nnzP, perm,
-                           row_of, P_data, rj, ra);
-        AMGT_HIP(hipGetLastError());
-    }
-    hipLaunchKernelGGL(k_rowptr_sorted, dim3(grid_of(int64_t(nc) + 1)), dim3(kThreads), 0, s, nc,
-                       nnzP, keys, ri);
-    AMGT_HIP(hipGetLastError());
+    AMGT_TRY(csr_transpose_device(n, nc, P_i, P_j, P_data, nnzP, ri, rj, ra, s));
     // R A, then (R A) P with the diagonal first
     int32_t *rai = nullptr, *raj = nullptr;
     double *rav = nullptr;

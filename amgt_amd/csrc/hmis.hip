@@ -18,8 +18,8 @@
 // (par_coarsen.c:2353-2386, first pass of its loop at :2494).
 //
 // The sweep is sequential by nature, as in HYPRE (the GPU build of HYPRE has
-// no HMIS).  S goes to pinned host memory, the markers come back, and the
-// rest runs on the device.
+// no HMIS).  S and its transpose (built on the device) go to pinned host
+// memory, the markers come back, and the rest runs on the device.
 #include "internal.hpp"
 
 #include <mutex>
@@ -69,16 +69,21 @@ class MeasureLists {
     int32_t top_ = 0;
 };
 
-void ruge_first_pass(int32_t n, const int32_t *SI, const int32_t *SJ, int32_t *cf) {
-    // S^T: the points each point influences, in ascending order
-    std::vector<int32_t> ti(static_cast<size_t>(n) + 1, 0), tj(static_cast<size_t>(SI[n]));
+// S^T on the host: the points each point influences, in ascending order
+void transpose_host(int32_t n, const int32_t *SI, const int32_t *SJ, std::vector<int32_t> &ti,
+                    std::vector<int32_t> &tj) {
+    ti.assign(static_cast<size_t>(n) + 1, 0);
+    tj.resize(static_cast<size_t>(SI[n]));
     for (int64_t p = 0; p < SI[n]; ++p) ++ti[size_t(SJ[p]) + 1];
     for (int32_t i = 0; i < n; ++i) ti[i + 1] += ti[i];
-    {
-        std::vector<int32_t> pos(ti.begin(), ti.end() - 1);
-        for (int32_t i = 0; i < n; ++i)
-            for (int32_t p = SI[i]; p < SI[i + 1]; ++p) tj[size_t(pos[SJ[p]]++)] = i;
-    }
+    std::vector<int32_t> pos(ti.begin(), ti.end() - 1);
+    for (int32_t i = 0; i < n; ++i)
+        for (int32_t p = SI[i]; p < SI[i + 1]; ++p) tj[size_t(pos[SJ[p]]++)] = i;
+}
+
+// the sweep, given S and S^T (rows of S^T ascending)
+void ruge_first_pass(int32_t n, const int32_t *SI, const int32_t *SJ, const int32_t *ti,
+                     const int32_t *tj, int32_t *cf) {
     std::vector<int32_t> meas(static_cast<size_t>(n));
     int64_t left = 0;
     for (int32_t i = 0; i < n; ++i) {
@@ -150,6 +155,19 @@ void ruge_first_pass(int32_t n, const int32_t *SI, const int32_t *SJ, int32_t *c
     }
 }
 
+// flags an S whose row pointer descends or whose columns leave [0, n): the
+// host sweep indexes with them (plain stores of 1, idempotent)
+__global__ __launch_bounds__(256) void k_check_s(int32_t n, const int32_t *__restrict__ SI,
+                                                 const int32_t *__restrict__ SJ,
+                                                 int32_t *__restrict__ bad) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int p0 = SI[i], p1 = SI[i + 1];
+    bool ok = p1 >= p0;
+    for (int p = p0; ok && p < p1; ++p) ok = uint32_t(SJ[p]) < uint32_t(n);
+    if (!ok) *bad = 1;
+}
+
 // pinned staging for S and the markers, grown as needed, kept for the process
 struct Staging {
     std::mutex mu;
@@ -198,7 +216,9 @@ int amgt_amg_ruge_first_pass(int32_t n, const int32_t *S_i, const int32_t *S_j,
             set_error("Ruge first pass: column out of range");
             return AMGT_ERR_INPUT;
         }
-    ruge_first_pass(n, S_i, S_j, cf_marker);
+    std::vector<int32_t> ti, tj;
+    transpose_host(n, S_i, S_j, ti, tj);
+    ruge_first_pass(n, S_i, S_j, ti.data(), tj.data(), cf_marker);
     return AMGT_OK;
 }
 
@@ -216,20 +236,59 @@ int amgt_amg_hmis(int32_t n, const int32_t *S_i, const int32_t *S_j, int32_t *cf
     AMGT_HIP(hipMemcpyAsync(&nnz, S_i + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     AMGT_HIP(hipStreamSynchronize(s));
     AMGT_CHECK_ARG(nnz == 0 || S_j, "NULL column array");
+    AMGT_CHECK_ARG(nnz >= 0, "negative nonzero count in S");
+    // S^T on the device (rows ascending, as the host sweep reads them)
+    int32_t *dti = nullptr, *dtj = nullptr, *dbad = nullptr, hbad = 0;
+    AMGT_TRY(tmp_alloc_t(&dti, size_t(n) + 1, s));
+    int status = tmp_alloc_t(&dtj, size_t(nnz), s);
+    if (status == AMGT_OK) status = tmp_alloc_t(&dbad, 1, s);
+    if (status == AMGT_OK) {
+        if (hipMemsetAsync(dbad, 0, sizeof(int32_t), s) != hipSuccess) status = AMGT_ERR_HIP;
+        hipLaunchKernelGGL(k_check_s, dim3((n + 255) / 256), dim3(256), 0, s, n, S_i, S_j, dbad);
+        if (hipGetLastError() != hipSuccess) status = AMGT_ERR_HIP;
+    }
+    if (status == AMGT_OK)
+        status = csr_transpose_device(n, n, S_i, S_j, nullptr, nnz, dti, dtj, nullptr, s);
     void *buf = nullptr;
-    AMGT_TRY(st.get(sizeof(int32_t) * (2 * size_t(n) + 1 + size_t(nnz)), &buf));
+    if (status == AMGT_OK)
+        status = st.get(sizeof(int32_t) * (3 * size_t(n) + 2 + 2 * size_t(nnz)), &buf);
     int32_t *hi = static_cast<int32_t *>(buf);
     int32_t *hj = hi + n + 1;
-    int32_t *hcf = hj + nnz;
-    AMGT_HIP(hipMemcpyAsync(hi, S_i, sizeof(int32_t) * (size_t(n) + 1), hipMemcpyDeviceToHost, s));
-    if (nnz) AMGT_HIP(hipMemcpyAsync(hj, S_j, sizeof(int32_t) * size_t(nnz), hipMemcpyDeviceToHost, s));
-    AMGT_HIP(hipStreamSynchronize(s));
-    AMGT_TRY(amgt_amg_ruge_first_pass(n, hi, hj, hcf));
+    int32_t *hti = hj + nnz;
+    int32_t *htj = hti + n + 1;
+    int32_t *hcf = htj + nnz;
+    if (status == AMGT_OK) {
+        hipError_t e = hipMemcpyAsync(hi, S_i, sizeof(int32_t) * (size_t(n) + 1),
+                                      hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(hti, dti, sizeof(int32_t) * (size_t(n) + 1), hipMemcpyDeviceToHost,
+                               s);
+        if (e == hipSuccess && nnz)
+            e = hipMemcpyAsync(hj, S_j, sizeof(int32_t) * size_t(nnz), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess && nnz)
+            e = hipMemcpyAsync(htj, dtj, sizeof(int32_t) * size_t(nnz), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(&hbad, dbad, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            set_error(std::string("HMIS: copy of S to the host: ") + hipGetErrorString(e));
+            status = AMGT_ERR_HIP;
+        }
+    }
+    tmp_free(dti, s);
+    tmp_free(dtj, s);
+    tmp_free(dbad, s);
+    AMGT_TRY(status);
+    if (hbad) {
+        set_error("HMIS: S row pointer not ascending or column out of range");
+        return AMGT_ERR_INPUT;
+    }
+    ruge_first_pass(n, hi, hj, hti, htj, hcf);
     AMGT_HIP(hipMemcpyAsync(cf_marker, hcf, sizeof(int32_t) * size_t(n), hipMemcpyHostToDevice, s));
     // amgt_amg_pmis synchronises the stream before it returns successfully; on
     // an error the copy out of the staging buffer is waited for here, so the
     // buffer is free again when the lock is released
-    const int status = amgt_amg_pmis(n, S_i, S_j, 1, cf_marker, n_coarse, stream);
+    status = amgt_amg_pmis(n, S_i, S_j, 1, cf_marker, n_coarse, stream);
     if (status != AMGT_OK) (void)hipStreamSynchronize(s);
     return status;
 }

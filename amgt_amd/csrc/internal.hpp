@@ -170,6 +170,13 @@ double stand_from(const RowStats &h, int64_t nblk, int32_t brow);
 // is returned through *total if non-null (synchronises the stream)
 int exclusive_scan_i32(const int32_t *in, int32_t *out, int64_t n, hipStream_t s);
 int exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t s);
+// CSR transpose on the device (nrows x ncols -> ncols x nrows): every row of
+// the transpose lists its columns (the input rows) in ascending order, values
+// (when V and TV are given) moved with them; a stable radix sort by column.
+// TI[ncols+1], TJ[nnz], TV[nnz] are the caller's.
+int csr_transpose_device(int32_t nrows, int32_t ncols, const int32_t *I, const int32_t *J,
+                         const double *V, int32_t nnz, int32_t *TI, int32_t *TJ, double *TV,
+                         hipStream_t s);
 
 // ---- device helpers ----------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
