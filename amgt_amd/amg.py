@@ -9,14 +9,15 @@ BoomerAMG's: P is trilinear interpolation to the grid of even-index points
 mBSR transpose, and every coarse operator is the
 Galerkin product R (A P) computed by the library's mBSR SpGEMM, as
 hypre_BoomerAMGBuildCoarseOperatorKT / RAPKT does on the GPU
-(par_csr_triplemat_device.c:844).  HMIS coarsening and ext+i interpolation
-(BoomerAMG's own setup) are not rebuilt here.
+(par_csr_triplemat_device.c:844).  BoomerAMG below builds the algebraic
+hierarchy instead (strength, HMIS/PMIS, extended+i on the device).
 
 Per level: A, P, R as mBSR handles tagged with the level, the inverse of the
 smoother's l1 vector, and fixed work vectors.  V(1,1):
     u  = w D^-1 f                      (first sweep from u = 0: no matvec, as HYPRE)
-    r  = f - A u                       (SpMV, residual form)
-    f' = R r,  coarse V-cycle -> u'    (SpMV)
+    r  = f - A u,  f' = R r            (one launch, amgt_mbsr_residual_restrict;
+                                        two SpMVs with fused_restrict=False)
+    coarse V-cycle -> u'
     u += P u'                          (SpMV with b aliasing y)
     u2 = u + w D^-1 (f - A u)          (one fused SpMV launch, amgt_mbsr_relax_jacobi)
 The coarsest level (<= coarse_size rows) applies its dense inverse, held as
@@ -46,7 +47,7 @@ import torch
 
 from . import _lib as L
 from . import workloads as W
-from .mbsr import MBSRMatrix
+from .mbsr import MBSRMatrix, RestrictPlan
 
 
 def _excl_cumsum(v: torch.Tensor) -> torch.Tensor:
@@ -128,6 +129,7 @@ class Level:
     u: torch.Tensor | None = None
     u2: torch.Tensor | None = None
     r: torch.Tensor | None = None
+    plan: object = None  # RestrictPlan of the fused residual + restriction
 
 
 @dataclass
@@ -192,8 +194,9 @@ class GeometricAMG:
 
     def __init__(self, I, J, V, grid, device, omega: float = 1.0, coarse_size: int = 512,
                  spmv_policy=fp64_policy, spgemm_policy=fp64_policy, max_levels: int = 25,
-                 relax_type: int = 18):
+                 relax_type: int = 18, fused_restrict: bool = True):
         self.device = device
+        self.fused_restrict = bool(fused_restrict)
         self.omega = float(omega)
         self.relax_type = _check_relax(relax_type)
         self.coarse_size = int(coarse_size)
@@ -281,6 +284,10 @@ class GeometricAMG:
         self.coarse_inv.set_precision(L.AMGT_DOUBLE)
         torch.cuda.synchronize()
         host = time.perf_counter() - th
+        # tables of the one-launch residual + restriction of every level with a P
+        if getattr(self, "fused_restrict", True):
+            for L_ in self.levels[:-1]:
+                L_.plan = RestrictPlan(L_.A, L_.R)
         # work vectors (fixed addresses, so a captured V-cycle stays valid)
         for L_ in self.levels:
             z = lambda: torch.zeros(L_.n, dtype=torch.float64, device=self.device)  # noqa: E731
@@ -300,10 +307,13 @@ class GeometricAMG:
         torch.mul(L_.dinv, L_.f, out=L_.u)
         if self.omega != 1.0:
             L_.u.mul_(self.omega)
-        # residual and restriction
-        L_.A.spmv(L_.u, L_.r, alpha=-1.0, beta=1.0, b=L_.f)
+        # residual and restriction (one launch with the level's plan)
         Lc = self.levels[lvl + 1]
-        L_.R.spmv(L_.r, Lc.f)
+        if L_.plan is not None:
+            L_.A.residual_restrict(L_.plan, L_.f, L_.u, L_.r, Lc.f)
+        else:
+            L_.A.spmv(L_.u, L_.r, alpha=-1.0, beta=1.0, b=L_.f)
+            L_.R.spmv(L_.r, Lc.f)
         uc = self._vcycle(lvl + 1)
         # coarse-grid correction u += P uc, then post-smoothing into u2
         L_.P.spmv(uc, L_.u, alpha=1.0, beta=1.0, b=L_.u)
@@ -444,12 +454,13 @@ class BoomerAMG(GeometricAMG):
                  cf_init: int = 0, max_elmts: int = 4, omega: float = 1.0, coarse_size: int = 9,
                  spmv_policy=fp64_policy, spgemm_policy=fp64_policy, max_levels: int = 25,
                  level_hook=None, coarsen: str = "hmis", relax_type: int = 18,
-                 galerkin: str = "mbsr"):
+                 galerkin: str = "mbsr", fused_restrict: bool = True):
         if coarsen not in ("hmis", "pmis"):
             raise ValueError(f"coarsen {coarsen!r}: 'hmis' or 'pmis'")
         if galerkin not in ("mbsr", "host_order"):
             raise ValueError(f"galerkin {galerkin!r}: 'mbsr' or 'host_order'")
         self.galerkin = galerkin
+        self.fused_restrict = bool(fused_restrict)
         self.csr_levels: list = []
         self.p_csr: list = []
         self.device = device

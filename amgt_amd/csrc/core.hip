@@ -240,7 +240,7 @@ void mbsr_release(Mbsr *A, hipStream_t s) {
     *A = Mbsr();
 }
 
-static int effective_precision(const Mbsr &A) {
+int effective_precision(const Mbsr &A) {
     int p = AMGT_DOUBLE;
     if (A.precision >= 0) p = A.precision;
     else if (mixed_precision_enabled()) p = amgt_precision_for_level(A.level);

@@ -95,6 +95,9 @@ struct amgt_mbsr {
 namespace amgt {
 
 bool mixed_precision_enabled();
+// precision the SpMV of A runs in: its explicit one, else the level policy
+// when mixed precision is on, else FP64 (FP32 for values without an FP16 copy)
+int effective_precision(const Mbsr &A);
 
 // SpMV kernel kinds (amgt_mbsr_set_spmv_kernel)
 enum : int { kSpmvDirect = 0, kSpmvStaged = 1, kSpmvBlock = 2, kSpmvBalanced = 3 };
@@ -140,6 +143,23 @@ int spmv_default_lanes(const Mbsr &A);
 // k right-hand sides interleaved (entry (row i, vector v) at x[i * k + v])
 int spmm_interleaved(const Mbsr &A, int precision, int k, double alpha, const double *x,
                      double beta, const double *b, double *y, hipStream_t s);
+// Residual with its restriction in one launch (the block SpMV kernel with
+// tiles of rpb block rows): r = f - A u, and fc = R r, each coarse row summed
+// over R's row c in order by the last tile of A that touches it.  The tables
+// come from amgt_restrict_plan_create: per tile its coarse rows, per coarse
+// row the number of tiles touching it and an arrival counter (zero between
+// calls), R in CSR with ascending fine rows.  u 16-byte aligned.
+struct RestrictTables {
+    int32_t rpb;
+    const int32_t *tile_ptr, *tile_c, *nspan;
+    int32_t *cnt;
+    const int32_t *ri, *rj;
+    const double *rv;
+};
+int residual_restrict(const Mbsr &A, int precision, const RestrictTables &t, const double *f,
+                      const double *u, double *r, double *fc, hipStream_t s);
+// block rows per tile of the block SpMV kernel for a lanes setting
+int blk_tile_rows(int lanes);
 // u_new = u + omega * dinv .* (f - A u) in one SpMV launch (square A)
 int relax_jacobi(const Mbsr &A, int precision, int lanes, double omega, const double *dinv,
                  const double *f, const double *u, double *u_new, hipStream_t s);

@@ -83,6 +83,47 @@ struct EpJacobi {  // one weighted Jacobi sweep: u + omega dinv (f - A u), with 
         return fma(omega * dinv[row], f[row] - ax, u[row]);
     }
 };
+// The residual r = f - A u of a V-cycle level (x = u, y = r) whose tiles also
+// restrict it, fc = R r: once a tile has stored its rows of r it arrives at
+// every coarse row its fine rows touch (tile_c[tile_ptr[t] .. tile_ptr[t+1])),
+// and the last tile to arrive at coarse row c (nspan[c] arrivals) forms
+// fc[c] over R's row c in R's order -- the same sum whichever tile is last --
+// and puts the counter back to zero for the next call.  No tile waits for
+// another, so nothing can hang; agent-scope fences order the r stores before
+// the arrivals and the last tile's loads after its arrival.
+struct EpResidRestrict {
+    const double *f;
+    const int32_t *tile_ptr, *tile_c, *nspan;
+    int32_t *cnt;
+    const int32_t *ri, *rj;
+    const double *rv;
+    const double *r;
+    double *fc;
+    __device__ __forceinline__ double operator()(int row, double ax) const { return f[row] - ax; }
+};
+
+// Work of a whole tile after its rows are stored (every thread of the
+// workgroup calls it): nothing for the plain epilogues.
+template <typename EP>
+__device__ __forceinline__ void tile_post(const EP &, int) {}
+template <typename EP>
+__device__ __forceinline__ void tile_post(const EpScaled<EP> &ep, int t) {
+    tile_post(ep.inner, t);
+}
+__device__ __forceinline__ void tile_post(const EpResidRestrict &ep, int t) {
+    __threadfence();  // this thread's rows of r before the tile's arrivals
+    __syncthreads();
+    for (int k = ep.tile_ptr[t] + int(threadIdx.x); k < ep.tile_ptr[t + 1]; k += int(blockDim.x)) {
+        const int c = ep.tile_c[k];
+        if (atomicAdd(&ep.cnt[c], 1) == ep.nspan[c] - 1) {
+            __threadfence();  // every other tile's rows of r after its arrival
+            double acc = 0.0;
+            for (int p = ep.ri[c]; p < ep.ri[c + 1]; ++p) acc = fma(ep.rv[p], ep.r[ep.rj[p]], acc);
+            ep.fc[c] = acc;
+            ep.cnt[c] = 0;
+        }
+    }
+}
 
 template <int G, typename VT, typename AT, typename EP>
 __global__ __launch_bounds__(256) void k_spmv(
@@ -339,6 +380,7 @@ __global__ __launch_bounds__(256) void k_spmv_blk(
                 }
             }
         }
+        tile_post(ep, wg);
         return;
     }
     // Loads are issued in dependency order and consumed late, so a tile costs
@@ -353,6 +395,7 @@ __global__ __launch_bounds__(256) void k_spmv_blk(
             const int row = 4 * (br0 + (rr >> 2)) + (rr & 3);
             if (row < nrows) y[row] = ep(row, 0.0);
         }
+        tile_post(ep, wg);
         return;
     }
     uint32_t mr[BPT];
@@ -546,6 +589,7 @@ __global__ __launch_bounds__(256) void k_spmv_blk(
             if (row < nrows) y[row] = ep(row, double(acc));
         }
     }
+    tile_post(ep, wg);
 }
 
 // Balanced variant (counterpart of bsr_spmv_balanced_tc_fp64,
@@ -1533,6 +1577,42 @@ int spmv(const Mbsr &A, int precision, int lanes, double alpha, const double *x,
         return AMGT_OK;
     }
     return launch_spmv_prec(A, precision, lanes, x, EpAxpby{alpha, beta, b}, y, s);
+}
+
+int residual_restrict(const Mbsr &A, int precision, const RestrictTables &t, const double *f,
+                      const double *u, double *r, double *fc, hipStream_t s) {
+    if (A.nrows == 0) return AMGT_OK;
+    const EpResidRestrict ep{f, t.tile_ptr, t.tile_c, t.nspan, t.cnt, t.ri, t.rj, t.rv, r, fc};
+    auto go = [&](auto vt_tag, auto at_tag, const auto *val, auto epx) -> int {
+        using VT = decltype(vt_tag);
+        using AT = decltype(at_tag);
+        switch (t.rpb) {
+            case 64: return launch_spmv_blk_g<64, 2, VT, AT>(A, val, u, epx, r, s);
+            case 128: return launch_spmv_blk_g<128, 4, VT, AT>(A, val, u, epx, r, s);
+            case 32: return launch_spmv_blk_g<32, 2, VT, AT>(A, val, u, epx, r, s);
+            case 16: return launch_spmv_blk_g<16, 2, VT, AT>(A, val, u, epx, r, s);
+            default: return launch_spmv_blk_g<8, 2, VT, AT>(A, val, u, epx, r, s);
+        }
+    };
+    switch (precision) {
+        case AMGT_DOUBLE: return go(double(), double(), A.val, ep);
+        case AMGT_FLOAT: return go(float(), float(), A.val32, ep);
+        case AMGT_HALF:
+            return go(__half(), float(), A.val16, EpScaled<EpResidRestrict>{ep, 1.0 / A.half_scale});
+        default:
+            set_error("unknown precision");
+            return AMGT_ERR_ARG;
+    }
+}
+
+int blk_tile_rows(int lanes) {
+    switch (lanes) {
+        case 4: return 64;
+        case 8: return 128;
+        case 16: return 32;
+        case 32: return 16;
+        default: return 8;
+    }
 }
 
 int relax_jacobi(const Mbsr &A, int precision, int lanes, double omega, const double *dinv,

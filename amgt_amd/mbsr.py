@@ -296,6 +296,21 @@ class MBSRMatrix:
         L.check(st, "amgt_mbsr_spgemm")
         return MBSRMatrix(h, self.device)
 
+    def residual_restrict(self, plan: "RestrictPlan", f: torch.Tensor, u: torch.Tensor,
+                          r: torch.Tensor, fc: torch.Tensor, stream=None) -> torch.Tensor:
+        """r = f - A u and fc = R r in one launch (amgt_mbsr_residual_restrict);
+        `plan` comes from RestrictPlan(A, R).  Returns fc."""
+        n = self.shape[0]
+        for t, name in ((f, "f"), (u, "u"), (r, "r"), (fc, "fc")):
+            _need_cuda(t, name, torch.float64)
+        for t, name in ((f, "f"), (u, "u"), (r, "r")):
+            _need_len(t, name, n, self.device)
+        _need_len(fc, "fc", plan.n_coarse, self.device)
+        st = L.load().amgt_mbsr_residual_restrict(self.handle, plan.handle, _ptr(f), _ptr(u),
+                                                  _ptr(r), _ptr(fc), _stream_ptr(stream))
+        L.check(st, "amgt_mbsr_residual_restrict")
+        return fc
+
     def transpose(self, stream=None) -> "MBSRMatrix":
         """A new handle holding A^T (amgt_mbsr_transpose)."""
         h = ctypes.c_void_p()
@@ -330,3 +345,33 @@ def set_mixed_precision(enable: bool) -> None:
 
 def precision_for_level(level: int) -> int:
     return int(L.load().amgt_precision_for_level(level))
+
+
+class RestrictPlan:
+    """Tables of the one-launch residual + restriction of a V-cycle level
+    (amgt_restrict_plan_create): A (square) and R = P^T as mBSR handles; built
+    once for A's lanes setting."""
+
+    def __init__(self, A: MBSRMatrix, R: MBSRMatrix, stream=None):
+        h = ctypes.c_void_p()
+        L.check(L.load().amgt_restrict_plan_create(A.handle, R.handle, _stream_ptr(stream),
+                                                   ctypes.byref(h)), "amgt_restrict_plan_create")
+        self._h = h
+        self.n_coarse = R.shape[0]
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        if self._h is None:
+            raise ValueError("plan already destroyed")
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.load().amgt_restrict_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -44,6 +44,7 @@ enum {
 enum { AMGT_DOUBLE = 0, AMGT_FLOAT = 1, AMGT_HALF = 2 };
 
 typedef struct amgt_mbsr *amgt_mbsr_t;
+typedef struct amgt_restrict_plan *amgt_restrict_plan_t;
 
 typedef struct {
     int32_t nrows, ncols, brow, bcol;
@@ -127,6 +128,22 @@ int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream);
  * FP64. */
 int amgt_mbsr_spmv(amgt_mbsr_t A, int trans, double alpha, const double *d_x, double beta,
                    const double *d_b, double *d_y, void *stream);
+/* ---- residual and restriction in one launch ----
+ * The V-cycle's r = f - A u and f_c = R r (par_cycle.c:443 and :704: a matvec, then
+ * the restriction matvec with R = P^T) as one launch of the block SpMV kernel: each
+ * tile stores its rows of r, then arrives at the coarse rows its fine rows touch; the
+ * last tile to arrive at coarse row c forms f_c[c] over R's row c in order (the same
+ * sum whichever tile is last, so results are deterministic).  The plan (per level:
+ * R in CSR, the tiles' coarse rows, arrival counters kept at zero between calls) is
+ * built once for A's lanes setting; the call allocates nothing.  A square, R's
+ * columns = A's rows; values in A's precision, R in FP64; u 16-byte aligned; r must
+ * not alias f or u.  Calls with one plan must not overlap (shared counters). */
+int amgt_restrict_plan_create(amgt_mbsr_t A, amgt_mbsr_t R, void *stream,
+                              amgt_restrict_plan_t *plan);
+int amgt_restrict_plan_destroy(amgt_restrict_plan_t plan);
+int amgt_mbsr_residual_restrict(amgt_mbsr_t A, amgt_restrict_plan_t plan, const double *d_f,
+                                const double *d_u, double *d_r, double *d_fc, void *stream);
+
 /* ---- SpMV that allocates nothing ----
  * No counterpart in the reference.  For callers that cannot allocate device
  * memory: code under HIP graph capture, tight serving loops, harnesses that
