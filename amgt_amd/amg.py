@@ -15,8 +15,11 @@ hierarchy instead (strength, HMIS/PMIS, extended+i on the device).
 Per level: A, P, R as mBSR handles tagged with the level, the inverse of the
 smoother's l1 vector, and fixed work vectors.  V(1,1):
     u  = w D^-1 f                      (first sweep from u = 0: no matvec, as HYPRE)
-    r  = f - A u,  f' = R r            (one launch, amgt_mbsr_residual_restrict;
-                                        two SpMVs with fused_restrict=False)
+    r  = f - A u,  f' = R r            (two SpMVs; one launch of
+                                        amgt_mbsr_residual_restrict with
+                                        fused_restrict=True, correct but slower
+                                        on MI355X: its cross-tile arrivals cost
+                                        more than the second launch)
     coarse V-cycle -> u'
     u += P u'                          (SpMV with b aliasing y)
     u2 = u + w D^-1 (f - A u)          (one fused SpMV launch, amgt_mbsr_relax_jacobi)
@@ -194,7 +197,7 @@ class GeometricAMG:
 
     def __init__(self, I, J, V, grid, device, omega: float = 1.0, coarse_size: int = 512,
                  spmv_policy=fp64_policy, spgemm_policy=fp64_policy, max_levels: int = 25,
-                 relax_type: int = 18, fused_restrict: bool = True):
+                 relax_type: int = 18, fused_restrict: bool = False):
         self.device = device
         self.fused_restrict = bool(fused_restrict)
         self.omega = float(omega)
@@ -285,7 +288,7 @@ class GeometricAMG:
         torch.cuda.synchronize()
         host = time.perf_counter() - th
         # tables of the one-launch residual + restriction of every level with a P
-        if getattr(self, "fused_restrict", True):
+        if getattr(self, "fused_restrict", False):
             for L_ in self.levels[:-1]:
                 L_.plan = RestrictPlan(L_.A, L_.R)
         # work vectors (fixed addresses, so a captured V-cycle stays valid)
@@ -454,7 +457,7 @@ class BoomerAMG(GeometricAMG):
                  cf_init: int = 0, max_elmts: int = 4, omega: float = 1.0, coarse_size: int = 9,
                  spmv_policy=fp64_policy, spgemm_policy=fp64_policy, max_levels: int = 25,
                  level_hook=None, coarsen: str = "hmis", relax_type: int = 18,
-                 galerkin: str = "mbsr", fused_restrict: bool = True):
+                 galerkin: str = "mbsr", fused_restrict: bool = False):
         if coarsen not in ("hmis", "pmis"):
             raise ValueError(f"coarsen {coarsen!r}: 'hmis' or 'pmis'")
         if galerkin not in ("mbsr", "host_order"):

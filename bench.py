@@ -344,6 +344,67 @@ def boomer_timings(dev, n1=128, cpu=True):
     return out
 
 
+def vcycle_variants(dev, n1=128):
+    """The residual + restriction of every level in one launch
+    (amgt_mbsr_residual_restrict) against two SpMV launches, on the geometric
+    and the algebraic 7-pt 128^3 hierarchies: V-cycle time (hipGraph replay,
+    mean of 50) and PCG to 1e-8; plus the algebraic hierarchy with the Galerkin
+    product in HYPRE's host order (the parity mode whose hierarchy equals
+    HYPRE's CPU one): its setup time, levels and operator complexity."""
+    import numpy as np
+    import scipy.sparse as sp
+    import torch
+
+    from amgt_amd import amg as G
+    from amgt_amd import workloads as W
+
+    I, J, V = W.stencil(7, n1, n1, n1)
+    n = n1 ** 3
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    dI, dJ, dV = t(I), t(J), t(V)
+    b = t(sp.csr_matrix((V, J, I), shape=(n, n)) @ W.rand_normal(1, n))
+    out = {}
+    for hier in ("geometric", "algebraic"):
+        res = {}
+        amgs = {}
+        for fused in (False, True):
+            if hier == "geometric":
+                amgs[fused] = G.GeometricAMG(dI, dJ, dV, (n1, n1, n1), dev, fused_restrict=fused)
+            else:
+                amgs[fused] = G.BoomerAMG(dI, dJ, dV, n, dev, fused_restrict=fused)
+            amgs[fused].capture()
+        z = torch.empty_like(b)
+        for rep in range(2):  # the modes alternate; the second round is kept
+            for fused in (False, True):
+                amg = amgs[fused]
+                for _ in range(5):
+                    amg.precondition(b, z)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(50):
+                    amg.precondition(b, z)
+                torch.cuda.synchronize()
+                cyc = (time.perf_counter() - t0) / 50
+                t0 = time.perf_counter()
+                _, it, hist = amg.pcg(b, tol=1e-8)
+                torch.cuda.synchronize()
+                res["one_launch" if fused else "two_launches"] = {
+                    "vcycle_ms": round(cyc * 1e3, 4),
+                    "pcg_ms": round((time.perf_counter() - t0) * 1e3, 3), "iterations": it,
+                    "final_rel_residual": float(hist[-1])}
+        del amgs
+        out[hier] = res
+    torch.cuda.synchronize()
+    host = G.BoomerAMG(dI, dJ, dV, n, dev, galerkin="host_order")
+    gc, oc = host.complexities()
+    out["algebraic_host_order_galerkin"] = {
+        "setup_s": round(host.times.total_s, 4),
+        "galerkin_s": [round(v, 4) for v in host.times.spgemm_s],
+        "levels": [L_.n for L_ in host.levels], "grid_complexity": round(gc, 6),
+        "operator_complexity": round(oc, 6)}
+    return out
+
+
 def c3_timings(dev, n1=160):
     """Mixed-precision configuration: 3D anisotropic 7-pt diffusion on n1^3
     (couplings 1, 1, 0.01), b ~ N(0,1) (Park-Miller seed 2), the algebraic
@@ -649,6 +710,7 @@ def main():
     spmm = spmm_timings(dev)
     batch = None if args.no_amg else batch_timings(dev)
     boomer = None if args.no_amg else boomer_timings(dev, cpu=not args.no_cpu_baseline)
+    vcyc = None if args.no_amg else vcycle_variants(dev)
     policy = None if args.no_amg else policy_timings(dev)
     c3 = None if args.no_amg else c3_timings(dev)
 
@@ -694,6 +756,8 @@ def main():
         out["batch_48_poisson2d"] = batch
     if boomer is not None:
         out["amg_boomer_7pt128"] = boomer
+    if vcyc is not None:
+        out["vcycle_variants_7pt128"] = vcyc
     if policy is not None:
         out["learned_policy_grf1024"] = policy
     if c3 is not None:
