@@ -46,7 +46,8 @@ EXPORTS = (
     "amgt_mbsr_update_values", "amgt_mbsr_get_info", "amgt_mbsr_get_arrays",
     "amgt_mbsr_set_level", "amgt_mbsr_set_precision", "amgt_mbsr_set_spmv_lanes",
     "amgt_mbsr_set_spmv_kernel",
-    "amgt_mbsr_autotune", "amgt_mbsr_spmv", "amgt_mbsr_spmv_noalloc", "amgt_mbsr_spmm",
+    "amgt_mbsr_autotune", "amgt_mbsr_spmv_sweep", "amgt_mbsr_tune_from_stats",
+    "amgt_mbsr_spmv", "amgt_mbsr_spmv_noalloc", "amgt_mbsr_spmm",
     "amgt_restrict_plan_create", "amgt_restrict_plan_destroy", "amgt_mbsr_residual_restrict",
     "amgt_mbsr_spgemm",
     "amgt_mbsr_to_csr", "amgt_free", "amgt_csr_spgemm", "amgt_csr_matvec",
@@ -126,6 +127,8 @@ def load():
         "amgt_mbsr_set_spmv_lanes": (_int, [_vp, _int]),
         "amgt_mbsr_set_spmv_kernel": (_int, [_vp, _int]),
         "amgt_mbsr_autotune": (_int, [_vp, _vp]),
+        "amgt_mbsr_spmv_sweep": (_int, [_vp, _vp, ctypes.POINTER(ctypes.c_float)]),
+        "amgt_mbsr_tune_from_stats": (_int, [_vp]),
         "amgt_mbsr_spmv": (_int, [_vp, _int, _dbl, _vp, _dbl, _vp, _vp, _vp]),
         "amgt_mbsr_spmv_noalloc": (_int, [_vp, _dbl, _vp, _dbl, _vp, _vp, _vp]),
         "amgt_restrict_plan_create": (_int, [_vp, _vp, _vp, _pvp]),

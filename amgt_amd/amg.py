@@ -418,9 +418,15 @@ class GeometricAMG:
         return sum(nnz) / nnz[0]
 
     def describe(self) -> list:
+        """Per level: rows, nnz, precisions and the SpMV kernel/lanes of A, P
+        and R (chosen from each operator's statistics when it was built,
+        amgt_mbsr_tune_from_stats)."""
         return [{"level": i, "rows": L_.n, "nnz": L_.A.info()["nnz"],
                  "spmv_precision": self.level_precision(i),
-                 "spgemm_precision": self.spgemm_policy(i) if L_.P is not None else None}
+                 "spgemm_precision": self.spgemm_policy(i) if L_.P is not None else None,
+                 "spmv_config": {tag: "%s_%d" % M.spmv_config()
+                                 for tag, M in (("A", L_.A), ("P", L_.P), ("R", L_.R))
+                                 if M is not None}}
                 for i, L_ in enumerate(self.levels)]
 
 

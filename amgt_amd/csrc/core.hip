@@ -393,11 +393,17 @@ int amgt_mbsr_set_spmv_lanes(amgt_mbsr_t A, int lanes) {
     return AMGT_OK;
 }
 
-int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream) {
-    AMGT_CHECK_ARG(A != nullptr, "handle is NULL");
-    Mbsr &m = A->m;
+// Times y = A x with every timed kernel kind x lanes setting (two sweeps,
+// clocks warmed, best of each): t[k * 5 + c] for kind kinds[k] and lanes
+// cand[c], 1e30 where a pair does not exist.  Leaves the handle's kernel and
+// lanes as they were.
+static const int kTuneLanes[5] = {4, 8, 16, 32, 64};
+static const int kTuneKinds[4] = {kSpmvBlock, kSpmvBalanced, kSpmvStaged, kSpmvDirect};
+
+static int spmv_sweep(Mbsr &m, hipStream_t s, float *t_min) {
+    for (int q = 0; q < 20; ++q) t_min[q] = 1e30f;
     if (m.nblk == 0 || m.pattern_only) return AMGT_OK;
-    hipStream_t s = as_stream(stream);
+    const int kind0 = m.spmv_kernel, lanes0 = m.spmv_lanes;
     int prec = effective_precision(m);
     AMGT_TRY(mbsr_ensure_lowp(&m, &prec, s));
     double *x = nullptr, *y = nullptr;
@@ -407,50 +413,68 @@ int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream) {
     hipEvent_t e0, e1;
     AMGT_HIP(hipEventCreate(&e0));
     AMGT_HIP(hipEventCreate(&e1));
-    const int cand[5] = {4, 8, 16, 32, 64};
-    // kernels worth timing (the others stay selectable by hand)
-    const int kinds[4] = {kSpmvBlock, kSpmvBalanced, kSpmvStaged, kSpmvDirect};
-    float t_min[4][5];
-    for (auto &row : t_min)
-        for (float &v : row) v = 1e30f;
     // bring the clocks up before the first measurement
     AMGT_TRY(spmv_prepare(&m, spmv_default_lanes(m), s));
     for (int r = 0; r < 30; ++r) AMGT_TRY(spmv(m, prec, spmv_default_lanes(m), 1.0, x, 0.0, nullptr, y, s));
-    // two sweeps, keep each candidate's best time
     for (int sweep = 0; sweep < 2; ++sweep) {
         for (int kk = 0; kk < 4; ++kk) {
-            m.spmv_kernel = kinds[kk];
+            m.spmv_kernel = kTuneKinds[kk];
             for (int c = 0; c < 5; ++c) {
                 // the balanced kernel has three tile sizes (lanes 4, 8, 16)
-                if (kinds[kk] == kSpmvBalanced && cand[c] > 16) continue;
-                int st = spmv_prepare(&m, cand[c], s);
+                if (kTuneKinds[kk] == kSpmvBalanced && kTuneLanes[c] > 16) continue;
+                int st = spmv_prepare(&m, kTuneLanes[c], s);
                 for (int r = 0; r < 3 && st == AMGT_OK; ++r)  // warm-up
-                    st = spmv(m, prec, cand[c], 1.0, x, 0.0, nullptr, y, s);
+                    st = spmv(m, prec, kTuneLanes[c], 1.0, x, 0.0, nullptr, y, s);
                 if (st != AMGT_OK) continue;
                 AMGT_HIP(hipEventRecord(e0, s));
                 const int reps = 10;
-                for (int r = 0; r < reps; ++r) spmv(m, prec, cand[c], 1.0, x, 0.0, nullptr, y, s);
+                for (int r = 0; r < reps; ++r)
+                    spmv(m, prec, kTuneLanes[c], 1.0, x, 0.0, nullptr, y, s);
                 AMGT_HIP(hipEventRecord(e1, s));
                 AMGT_HIP(hipEventSynchronize(e1));
                 float ms = 0.f;
                 AMGT_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (ms < t_min[kk][c]) t_min[kk][c] = ms;
+                ms /= float(reps);
+                if (ms < t_min[kk * 5 + c]) t_min[kk * 5 + c] = ms;
             }
         }
     }
-    float best = 1e30f;
-    int best_lanes = spmv_default_lanes(m), best_kind = kinds[0];
-    for (int kk = 0; kk < 4; ++kk)
-        for (int c = 0; c < 5; ++c)
-            if (t_min[kk][c] < best) {
-                best = t_min[kk][c];
-                best_lanes = cand[c];
-                best_kind = kinds[kk];
-            }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     tmp_free(x, s);
     tmp_free(y, s);
+    m.spmv_kernel = kind0;
+    m.spmv_lanes = lanes0;
+    return spmv_prepare(&m, m.spmv_lanes ? m.spmv_lanes : spmv_default_lanes(m), s);
+}
+
+int amgt_mbsr_spmv_sweep(amgt_mbsr_t A, void *stream, float *times_ms) {
+    AMGT_CHECK_ARG(A != nullptr && times_ms != nullptr, "NULL argument");
+    return spmv_sweep(A->m, as_stream(stream), times_ms);
+}
+
+int amgt_mbsr_tune_from_stats(amgt_mbsr_t A) {
+    AMGT_CHECK_ARG(A != nullptr, "handle is NULL");
+    apply_spmv_defaults(A->m);
+    return AMGT_OK;
+}
+
+int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream) {
+    AMGT_CHECK_ARG(A != nullptr, "handle is NULL");
+    Mbsr &m = A->m;
+    if (m.nblk == 0 || m.pattern_only) return AMGT_OK;
+    hipStream_t s = as_stream(stream);
+    float t[20];
+    AMGT_TRY(spmv_sweep(m, s, t));
+    float best = 1e30f;
+    int best_lanes = spmv_default_lanes(m), best_kind = kTuneKinds[0];
+    for (int kk = 0; kk < 4; ++kk)
+        for (int c = 0; c < 5; ++c)
+            if (t[kk * 5 + c] < best) {
+                best = t[kk * 5 + c];
+                best_lanes = kTuneLanes[c];
+                best_kind = kTuneKinds[kk];
+            }
     m.spmv_lanes = best_lanes;
     m.spmv_kernel = best_kind;
     return spmv_prepare(&m, best_lanes, s);

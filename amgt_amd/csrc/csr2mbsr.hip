@@ -480,7 +480,7 @@ int csr_to_mbsr(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *rowptr
             m.stand = stand_from(hrs, nblk, m.brow);
             m.avg_nnz = nblk ? double(nnz) / double(nblk) : 0.0;
             m.max_row_blocks = hrs.max_blocks;
-            m.spmv_kernel = default_spmv_kernel(m.stand);
+            m.spmv_kernel = default_spmv_kernel(m);
             if (hc.err & kErrCol) {
                 set_error("CSR->mBSR: column index out of range");
                 st = AMGT_ERR_INPUT;

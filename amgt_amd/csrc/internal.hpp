@@ -103,10 +103,17 @@ int effective_precision(const Mbsr &A);
 enum : int { kSpmvDirect = 0, kSpmvStaged = 1, kSpmvBlock = 2, kSpmvBalanced = 3 };
 
 // Spread of blocks per block row above which the balanced SpMV is the default
-// (the reference's switch, csr_matvec_device.c:1336)
+// for matrices too large for the direct walk (the reference's switch,
+// csr_matvec_device.c:1336)
 constexpr double kBalancedStand = 12.0;
-inline int default_spmv_kernel(double stand) {
-    return stand >= kBalancedStand ? kSpmvBalanced : kSpmvBlock;
+// SpMV kernel kind chosen from the matrix statistics (spmv.hip): the direct
+// walk for small operators and short-row ones, the balanced kernel for a large
+// spread of blocks per row, else the block-lane kernel.
+int default_spmv_kernel(const Mbsr &A);
+// applies the statistics choice to a handle (kind; lanes left to the rule)
+inline void apply_spmv_defaults(Mbsr &A) {
+    A.spmv_kernel = default_spmv_kernel(A);
+    A.spmv_lanes = 0;
 }
 
 // ---- component entry points (each in its own translation unit) -------------

@@ -65,7 +65,9 @@ int amgt_mbsr_spmv_noalloc(amgt_mbsr_t A, double alpha, const double *d_x, doubl
     AMGT_CHECK_ARG(!m.pattern_only, "pattern-only matrix has no values");
     AMGT_CHECK_ARG((reinterpret_cast<uintptr_t>(d_x) & 15u) == 0,
                    "x must be 16-byte aligned for the balanced kernel");
-    const int lanes = m.spmv_lanes ? m.spmv_lanes : spmv_default_lanes(m);
+    // the handle's lanes setting, else the balanced kernel's default (tiles of
+    // 256 blocks), as amgt_mbsr_spmv runs it with AMGT_SPMV_BALANCED
+    const int lanes = m.spmv_lanes ? m.spmv_lanes : 16;
     const size_t need = spmv_balanced_ws_bytes(m, lanes);
     if (need > kScratchBytes) {
         set_error("no-alloc SpMV: the matrix needs " + std::to_string(need) +

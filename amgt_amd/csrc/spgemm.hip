@@ -1154,7 +1154,7 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
     C.max_row_blocks = ht[1].max_blocks;
     C.avg_nnz = nblk ? double(nnz) / double(nblk) : 0.0;
     C.stand = stand_from(ht[1], nblk, brow);
-    C.spmv_kernel = default_spmv_kernel(C.stand);
+    C.spmv_kernel = default_spmv_kernel(C);
     C.blk_col = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * size_t(nblk ? nblk : 1), s));
     C.blk_map = static_cast<uint16_t *>(dev_alloc(sizeof(uint16_t) * size_t(nblk ? nblk : 1), s));
     if (!C.pattern_only)

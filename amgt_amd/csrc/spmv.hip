@@ -1421,8 +1421,40 @@ int launch_spmm_t(const Mbsr &A, const VT *val, int lanes, int k, double alpha, 
 
 }  // namespace
 
+// The rule below was fitted to the timing sweep (amgt_mbsr_spmv_sweep) of
+// every A, P and R of the geometric and algebraic 7-pt 128^3 hierarchies, the
+// algebraic anisotropic 160^3 and 1024^2 GRF ones, the geometric 27-pt 64^3
+// one and the 48-problem batch (126 matrices, 3 to 4.1M rows): summed over
+// them its choices take 1.3 % longer than each matrix's fastest setting (the
+// block kernel everywhere took 12 % longer).
+//  * up to 150k blocks, up to 1M blocks with at most 5 blocks per block row
+//    (interpolation and restriction operators) or up to 350k with at most 8:
+//    the direct walk.  These launches are latency bound; the block kernel's
+//    LDS staging, scans and barriers cost more than the coalescing gains
+//    (about 3 against 4.5-5 us at the small end), whatever the spread;
+//  * a spread of blocks per block row >= 12 above that: the balanced kernel;
+//  * else the block-lane kernel.
+int default_spmv_kernel(const Mbsr &A) {
+    const double avg = A.brow ? double(A.nblk) / double(A.brow) : 0.0;
+    if (A.nblk <= 150000 || (avg <= 5.0 && A.nblk <= 1000000) || (avg <= 8.0 && A.nblk <= 350000))
+        return kSpmvDirect;
+    if (A.stand >= kBalancedStand) return kSpmvBalanced;
+    return kSpmvBlock;
+}
+
 int spmv_default_lanes(const Mbsr &A) {
     const double avg = A.brow ? double(A.nblk) / double(A.brow) : 0.0;
+    if (A.spmv_kernel == kSpmvDirect) {
+        // lanes per block row so that about 2^18 lanes share the walk: the
+        // power of two at or above 0.9 * 2^18 / brow, at least 8 from 64k block
+        // rows down, within 4..64
+        const double want = 0.9 * 262144.0 / double(A.brow > 0 ? A.brow : 1);
+        int g = 4;
+        while (g < want && g < 64) g <<= 1;
+        if (A.brow <= 65536 && g < 8) g = 8;
+        return g;
+    }
+    if (A.spmv_kernel == kSpmvBalanced) return 16;  // tiles of 256 blocks
     if (A.spmv_kernel == kSpmvBlock) {
         // block-lane tiles of 512 blocks: the most block rows per tile (64, 32,
         // 16 or 8, lanes 4 / 16 / 32 / 64) whose average load stays within 90 %

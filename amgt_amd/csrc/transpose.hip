@@ -170,7 +170,7 @@ int mbsr_transpose(const Mbsr &A, hipStream_t s, Mbsr *out) {
     T.max_row_blocks = hrs.max_blocks;
     T.avg_nnz = nb ? double(A.nnz) / double(nb) : 0.0;
     T.stand = stand_from(hrs, nb, T.brow);
-    T.spmv_kernel = default_spmv_kernel(T.stand);
+    T.spmv_kernel = default_spmv_kernel(T);
     return AMGT_OK;
 }
 

@@ -171,6 +171,30 @@ class MBSRMatrix:
     def autotune(self, stream=None):
         L.check(L.load().amgt_mbsr_autotune(self.handle, _stream_ptr(stream)), "amgt_mbsr_autotune")
 
+    def tune_from_stats(self):
+        """SpMV kernel kind and lanes from the matrix statistics
+        (amgt_mbsr_tune_from_stats; what a new handle starts with)."""
+        L.check(L.load().amgt_mbsr_tune_from_stats(self.handle), "amgt_mbsr_tune_from_stats")
+
+    def spmv_config(self) -> tuple:
+        """(kernel kind name, lanes) the SpMV runs with."""
+        inf = self.info()
+        return L.SPMV_KERNEL_NAMES[inf["spmv_kernel"]], inf["spmv_lanes"]
+
+    def spmv_sweep(self, stream=None) -> dict:
+        """Measured SpMV time (ms) of every (kernel kind, lanes) pair, the
+        handle unchanged: {(kind name, lanes): ms}."""
+        t = (ctypes.c_float * 20)()
+        L.check(L.load().amgt_mbsr_spmv_sweep(self.handle, _stream_ptr(stream), t),
+                "amgt_mbsr_spmv_sweep")
+        kinds = (L.AMGT_SPMV_BLOCK, L.AMGT_SPMV_BALANCED, L.AMGT_SPMV_STAGED, L.AMGT_SPMV_DIRECT)
+        out = {}
+        for k, kind in enumerate(kinds):
+            for c, lanes in enumerate((4, 8, 16, 32, 64)):
+                if t[k * 5 + c] < 1e29:
+                    out[(L.SPMV_KERNEL_NAMES[kind], lanes)] = float(t[k * 5 + c])
+        return out
+
     def update_values(self, rowptr, colidx, values, stream=None):
         L.check(L.load().amgt_mbsr_update_values(self.handle, _ptr(rowptr), _ptr(colidx),
                                                  _ptr(values), _stream_ptr(stream)),

@@ -405,6 +405,55 @@ def vcycle_variants(dev, n1=128):
     return out
 
 
+def spmv_choice_report(dev):
+    """SpMV kernel and lanes chosen from the matrix statistics against the
+    timing sweep (amgt_mbsr_spmv_sweep), for every operator (A, P, R) of the
+    geometric and algebraic 7-pt 128^3 hierarchies and of the 48-problem batch:
+    per operator [rows, blocks, blocks per block row, spread, chosen kind/lanes,
+    its us, fastest kind/lanes, its us], and the sums."""
+    import numpy as np
+    import torch
+
+    from amgt_amd import amg as G
+    from amgt_amd import workloads as W
+
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    I, J, V = W.stencil(7, 128, 128, 128)
+    sizes = G.batch_sizes()
+    bI, bJ, bV, _, _, grids, _ = G.batch_poisson2d(sizes, 4, dev)
+    hiers = {
+        "geometric_7pt128": lambda: G.GeometricAMG(t(I), t(J), t(V), (128, 128, 128), dev),
+        "algebraic_7pt128": lambda: G.BoomerAMG(t(I), t(J), t(V), 128 ** 3, dev),
+        "batch_48_poisson2d": lambda: G.GeometricAMG(t(bI), t(bJ), t(bV), grids, dev),
+    }
+    out = {}
+    for name, make in hiers.items():
+        amg = make()
+        rows, tot_c, tot_b = [], 0.0, 0.0
+        for lvl, L_ in enumerate(amg.levels):
+            for tag, M in (("A", L_.A), ("P", L_.P), ("R", L_.R)):
+                if M is None:
+                    continue
+                inf = M.info()
+                kind, lanes = M.spmv_config()
+                sw = M.spmv_sweep()
+                chosen = sw.get((kind, lanes))
+                if chosen is None:  # a pair the sweep does not time
+                    continue
+                best = min(sw, key=sw.get)
+                tot_c += chosen
+                tot_b += sw[best]
+                rows.append([f"{tag}{lvl}", inf["nrows"], inf["nblk"],
+                             round(inf["nblk"] / max(inf["brow"], 1), 2), round(inf["stand"], 1),
+                             f"{kind}_{lanes}", round(chosen * 1e3, 1), f"{best[0]}_{best[1]}",
+                             round(sw[best] * 1e3, 1)])
+        out[name] = {"operators": rows, "chosen_us": round(tot_c * 1e3, 1),
+                     "fastest_us": round(tot_b * 1e3, 1),
+                     "chosen_over_fastest": round(tot_c / tot_b, 4) if tot_b else None}
+        del amg
+    return out
+
+
 def c3_timings(dev, n1=160):
     """Mixed-precision configuration: 3D anisotropic 7-pt diffusion on n1^3
     (couplings 1, 1, 0.01), b ~ N(0,1) (Park-Miller seed 2), the algebraic
@@ -711,6 +760,7 @@ def main():
     batch = None if args.no_amg else batch_timings(dev)
     boomer = None if args.no_amg else boomer_timings(dev, cpu=not args.no_cpu_baseline)
     vcyc = None if args.no_amg else vcycle_variants(dev)
+    choice = None if args.no_amg else spmv_choice_report(dev)
     policy = None if args.no_amg else policy_timings(dev)
     c3 = None if args.no_amg else c3_timings(dev)
 
@@ -758,6 +808,8 @@ def main():
         out["amg_boomer_7pt128"] = boomer
     if vcyc is not None:
         out["vcycle_variants_7pt128"] = vcyc
+    if choice is not None:
+        out["spmv_choice_from_statistics"] = choice
     if policy is not None:
         out["learned_policy_grf1024"] = policy
     if c3 is not None:

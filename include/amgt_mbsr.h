@@ -119,6 +119,20 @@ enum { AMGT_SPMV_DIRECT = 0, AMGT_SPMV_STAGED = 1, AMGT_SPMV_BLOCK = 2, AMGT_SPM
 int amgt_mbsr_set_spmv_kernel(amgt_mbsr_t A, int kind);
 /* times every lanes-per-block-row variant on this matrix and keeps the fastest */
 int amgt_mbsr_autotune(amgt_mbsr_t A, void *stream);
+/* The kernel kind and lanes chosen from the matrix statistics (what a new handle gets):
+ * the direct walk up to 150k blocks, up to 1M blocks with <= 5 blocks per block row or
+ * up to 350k with <= 8, with lanes per block row so that about 2^18 lanes share the
+ * walk; else the balanced kernel when the spread of blocks per block row is >= 12
+ * (tiles of 256 blocks), else the block-lane kernel with the largest tile whose average
+ * load fits 90 % of its 512 blocks.  Fitted to amgt_mbsr_spmv_sweep over every
+ * operator of the AMG hierarchies of the benchmark configurations.  Resets settings made
+ * with amgt_mbsr_set_spmv_kernel / _lanes or amgt_mbsr_autotune. */
+int amgt_mbsr_tune_from_stats(amgt_mbsr_t A);
+/* The timing sweep behind amgt_mbsr_autotune without changing the handle:
+ * times_ms[k * 5 + c] = mean time of y = A x with kernel kind k (AMGT_SPMV_BLOCK,
+ * _BALANCED, _STAGED, _DIRECT in that order) and lanes {4, 8, 16, 32, 64}[c] (best of
+ * two sweeps of 10 launches; 1e30 where the pair does not exist).  Synchronises. */
+int amgt_mbsr_spmv_sweep(amgt_mbsr_t A, void *stream, float *times_ms);
 
 /* ---- SpMV: y = alpha*op(A)*x + beta*b ----
  * spmv_amgT_fp64 csr_matvec_device.c:1302-1411 (+ fp32 2205-2309, fp16

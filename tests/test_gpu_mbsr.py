@@ -161,16 +161,54 @@ def test_spmv_deterministic(dev, name, kind):
     assert torch.equal(y1, y2)
 
 
-def test_default_kernel_follows_stand(dev):
-    """stand >= 12 selects the balanced kernel (csr_matvec_device.c:1336)."""
-    for name, want in [("skewed", L.AMGT_SPMV_BALANCED), ("27pt_10", L.AMGT_SPMV_BLOCK),
-                       ("c1_5pt_128", L.AMGT_SPMV_BLOCK)]:
-        (m, n), I, J, A = CASES[name]()
+def _big_skewed():
+    """200,000 rows in a band of 8 entries each, plus 4 block rows of 2,000
+    entries per row: 50,000 block rows, about 600k blocks (about 12 per block
+    row), a spread of blocks per block row far above 12."""
+    rng = np.random.default_rng(17)
+    m = 200000
+    band = np.array([0, 1, 7, 301, 4099, 9973, 30011, 77777])
+    rows = []
+    for r in range(m):
+        if r // 4 in (3, 12000, 30000, 49999):
+            rows.append(np.sort(rng.permutation(m)[:2000]))
+        else:
+            rows.append(np.sort((r + band) % m))
+    I = np.zeros(m + 1, np.int32)
+    I[1:] = np.cumsum([len(c) for c in rows])
+    J = np.concatenate(rows).astype(np.int32)
+    return (m, m), I, J, rng.standard_normal(len(J))
+
+
+def test_default_kernel_from_statistics(dev):
+    """The statistics choice of a new handle (amgt_mbsr_tune_from_stats): the
+    direct walk up to 150k blocks (whatever the spread), the balanced kernel
+    for a spread >= 12 above that (csr_matvec_device.c:1336's switch), else the
+    block-lane kernel; lanes: direct ~2^18 lanes in all, balanced 16."""
+    I7, J7, A7 = O.stencil(7, 64, 64, 64)
+    cases = [("skewed", CASES["skewed"](), L.AMGT_SPMV_DIRECT),
+             ("27pt_10", CASES["27pt_10"](), L.AMGT_SPMV_DIRECT),
+             ("c1_5pt_128", CASES["c1_5pt_128"](), L.AMGT_SPMV_DIRECT),
+             ("big_skewed", _big_skewed(), L.AMGT_SPMV_BALANCED),
+             ("7pt_64", ((64 ** 3, 64 ** 3), I7, J7, A7), L.AMGT_SPMV_BLOCK)]
+    for name, ((m, n), I, J, A), want in cases:
         ref = O.csr2mbsr(m, n, I, J, A)
         stand, _ = O.mbsr_stats(ref)
-        inf = make(dev, m, n, I, J, A).info()
-        assert (stand >= 12.0) == (want == L.AMGT_SPMV_BALANCED)
-        assert inf["spmv_kernel"] == want, name
+        M = make(dev, m, n, I, J, A)
+        inf = M.info()
+        nblk, brow = inf["nblk"], inf["brow"]
+        assert inf["spmv_kernel"] == want, (name, nblk, stand)
+        if want == L.AMGT_SPMV_BALANCED:
+            assert stand >= 12.0 and nblk > 150000 and inf["spmv_lanes"] == 16
+        if want == L.AMGT_SPMV_DIRECT:
+            g = inf["spmv_lanes"]
+            assert g in (4, 8, 16, 32, 64)
+            assert g == 64 or g * brow >= 0.9 * 262144
+        # overrides are undone by tune_from_stats
+        M.set_spmv_kernel(L.AMGT_SPMV_STAGED)
+        M.set_spmv_lanes(8)
+        M.tune_from_stats()
+        assert M.info()["spmv_kernel"] == want
 
 
 @pytest.mark.parametrize("prec,tol", [(L.AMGT_FLOAT, 2e-6), (L.AMGT_HALF, 2e-3)])
@@ -179,7 +217,7 @@ def test_spmv_balanced_low_precision(dev, prec, tol):
     ref = O.csr2mbsr(m, n, I, J, A)
     M = make(dev, m, n, I, J, A)
     M.set_precision(prec)
-    assert M.info()["spmv_kernel"] == L.AMGT_SPMV_BALANCED
+    M.set_spmv_kernel(L.AMGT_SPMV_BALANCED)
     rng = np.random.default_rng(3)
     x, b = rng.standard_normal(n), rng.standard_normal(m)
     exp = O.mbsr_spmv(ref, -1.0, x, 1.0, b)
