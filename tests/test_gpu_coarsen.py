@@ -466,3 +466,27 @@ def test_host_order_hierarchy_matches_hypre_cpu_path(dev, cfg):
     x, it, hist = _pcg_ones_rhs(amg, dev, ones_solution=True)
     print(f"[{cfg} host order] iterations {it} final {hist[-1]:e}", flush=True)
     assert it == g["iterations"] and "%.3e" % hist[-1] == g["final_relative_residual"]
+
+
+def test_host_order_hierarchy_matches_committed_level_fixtures(dev, tmp_path):
+    """The device hierarchy of the TEST_ij fsai job (host-order Galerkin
+    product) against the committed per-level operators in HYPRE's
+    DEBUG_SAVE_ALL_OPS layout (tests/golden/hierarchy_ij_fsai103): every A_l
+    and P_l entry in HYPRE's stored order, values to the 14 printed
+    decimals."""
+    from tests.test_matrix_io import load_level_fixture
+    n, I, J, A = CASES["7pt_10"]()
+    amg = G.BoomerAMG(T(I, dev), T(J, dev), T(A, dev), n, dev, coarsen="hmis", relax_type=7,
+                      galerkin="host_order")
+    assert [L_.n for L_ in amg.levels] == [1000, 500, 99, 14, 4]
+    for lvl in range(len(amg.levels)):
+        mats = [("A", amg.csr_levels[lvl])]
+        if lvl < len(amg.p_csr):
+            mats.append(("P", amg.p_csr[lvl]))
+        for tag, (MI, MJ, MV) in mats:
+            MI, MJ, MV = MI.cpu().numpy(), MJ.cpu().numpy(), MV.cpu().numpy()
+            (nr, nc, r, c, v), _ = load_level_fixture(tmp_path, f"{tag}_{lvl:02d}")
+            assert nr == len(MI) - 1
+            assert np.array_equal(r, np.repeat(np.arange(nr), np.diff(MI))), (tag, lvl)
+            assert np.array_equal(c, MJ), (tag, lvl)
+            assert np.all(np.abs(v - MV) <= 1e-14 * np.abs(MV)), (tag, lvl)

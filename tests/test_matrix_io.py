@@ -95,12 +95,13 @@ def test_ij_round_trip(tmp_path, nfiles):
     same(*MIO.read_ij(pre, nfiles), sp.csr_matrix((A, J, I), shape=(61, 47)))
 
 
-REF_AMS = "/root/reference/AmgT_HYPRE/src/test/TEST_ams/mfem.A"
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+REF_AMS = os.path.join(GOLDEN, "ij_ams", "mfem.Aalpha")
 
 
-@pytest.mark.skipif(not os.path.exists(REF_AMS + ".00000"), reason="reference tree absent")
 def test_reference_ij_fixture():
-    """HYPRE's own IJ matrix (4 rank files) against a plain parse of the lines."""
+    """HYPRE's own IJ matrix (TEST_ams/mfem.Aalpha, 4 rank files, copied from
+    the reference's test data) against a plain parse of the lines."""
     rows, cols, vals, hi_r, hi_c = [], [], [], 0, 0
     for f in range(4):
         with open(f"{REF_AMS}.{f:05d}") as fh:
@@ -114,3 +115,42 @@ def test_reference_ij_fixture():
     assert len(np.unique(r * (hi_c + 1) + c)) == len(r)  # no repeated entries in the fixture
     S = sp.csr_matrix((v, (r, c)), shape=(hi_r + 1, hi_c + 1))
     same(*MIO.read_ij(REF_AMS, 4), S)
+
+
+def load_level_fixture(tmp_path, name):
+    """A DEBUG_SAVE_ALL_OPS file of the fsai job's hierarchy: (n_rows, n_cols,
+    rows, cols, values) in the stored order, and the library reader's CSR."""
+    import gzip
+    import shutil
+    src = os.path.join(GOLDEN, "hierarchy_ij_fsai103", name + ".IJ.out.00000.gz")
+    dst = tmp_path / (name + ".IJ.out.00000")
+    with gzip.open(src, "rb") as fi, open(dst, "wb") as fo:
+        shutil.copyfileobj(fi, fo)
+    with open(dst) as fh:
+        ilo, ihi, jlo, jhi = map(int, fh.readline().split())
+        d = np.loadtxt(fh, ndmin=2)
+    raw = (ihi + 1, jhi + 1, d[:, 0].astype(np.int64), d[:, 1].astype(np.int64), d[:, 2])
+    return raw, MIO.read_ij(str(tmp_path / (name + ".IJ.out")), 1)
+
+
+def test_hierarchy_fixtures_match_oracle(tmp_path):
+    """The committed per-level operators (HYPRE's DEBUG_SAVE_ALL_OPS layout,
+    tests/golden/make_hierarchy_fixtures.py) against the oracle's host-order
+    hierarchy of the same job: entries in HYPRE's stored order, values to the
+    14 printed decimals; read back by amgt_read_ij."""
+    from oracle import oracle as O
+    I, J, A = O.stencil(7, 10, 10, 10)
+    levels = O.amg_hierarchy(1000, I, J, A, theta=0.25, max_row_sum=1.0, max_elmts=4,
+                             max_coarse_size=9, coarsen="hmis", rap="kt")
+    assert [L_["n"] for L_ in levels] == [1000, 500, 99, 14, 4]
+    for lvl, L_ in enumerate(levels):
+        mats = [("A", L_["n"], L_["I"], L_["J"], L_["A"])]
+        if "P" in L_:
+            mats.append(("P", L_["nc"], *L_["P"]))
+        for tag, ncols, MI, MJ, MV in mats:
+            (nr, nc, r, c, v), rd = load_level_fixture(tmp_path, f"{tag}_{lvl:02d}")
+            assert (nr, nc) == (L_["n"], ncols)
+            assert np.array_equal(r, np.repeat(np.arange(L_["n"]), np.diff(MI)))
+            assert np.array_equal(c, MJ)
+            assert np.all(np.abs(v - MV) <= 1e-14 * np.abs(MV))
+            same(*rd, sp.csr_matrix((v, (r, c)), shape=(nr, nc)))
