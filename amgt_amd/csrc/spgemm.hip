@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <string>
+#include <type_traits>
 
 namespace amgt {
 
@@ -269,10 +270,10 @@ __device__ __forceinline__ double ldv<__half>(const void *p, int64_t idx) {
 }
 
 // C entry as stored: the FP64 product for FP64 operands; below FP64 the
-// entry is rounded to FP32 (the reference keeps C in the operands' precision,
+// entry is an FP32 number (the reference keeps C in the operands' precision,
 // csr_spgemm_device.c:938-1140 TF32 and 753-937 FP16 with FP16 accumulation;
-// here products of the rounded operands accumulate in FP64 and the entry is
-// rounded once to FP32)
+// here FP32 / FP16 operands run on the 16-block MFMAs with FP32 accumulation,
+// and the entry is stored as FP32)
 template <typename VT>
 __device__ __forceinline__ double round_out(double x) {
     return double(float(x));
@@ -528,6 +529,62 @@ __device__ __forceinline__ double block_product(double a, double b) {
     return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, 0.0, 0, 0, 0);
 }
 
+// FP32 / FP16 operands: the 16-block MFMAs, FP32 accumulation.  Operand layout
+// (measured on the box): lane 4b + r holds row r of block b's A (a[k], k =
+// 0..3) and column r of block b's B (b[k]); the result entry (v, r) of block b
+// is register v of lane 4b + r.  FP16: one v_mfma_f32_4x4x4_16b_f16 per 16
+// block products; FP32: four v_mfma_f32_4x4x1_16b_f32 (one per k), chained.
+typedef _Float16 mf_half4 __attribute__((ext_vector_type(4)));
+typedef float mf_float4 __attribute__((ext_vector_type(4)));
+
+// operand element type of the 16-block MFMA for value type VT, and the
+// value's bits read as that type (no conversion: the copies hold FP16 / FP32
+// numbers)
+template <typename VT> struct Mf16;
+template <> struct Mf16<__half> {
+    using T = _Float16;
+    using V = mf_half4;
+    __device__ static T load(const void *p, int64_t idx) {
+        return static_cast<const _Float16 *>(p)[idx];
+    }
+};
+template <> struct Mf16<float> {
+    using T = float;
+    using V = mf_float4;
+    __device__ static T load(const void *p, int64_t idx) {
+        return static_cast<const float *>(p)[idx];
+    }
+};
+
+template <typename VT>
+__device__ __forceinline__ mf_float4 mfma16(typename Mf16<VT>::V a, typename Mf16<VT>::V b,
+                                            mf_float4 c);
+template <>
+__device__ __forceinline__ mf_float4 mfma16<__half>(mf_half4 a, mf_half4 b, mf_float4 c) {
+    return __builtin_amdgcn_mfma_f32_4x4x4f16(a, b, c, 0, 0, 0);
+}
+template <>
+__device__ __forceinline__ mf_float4 mfma16<float>(mf_float4 a, mf_float4 b, mf_float4 c) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) c = __builtin_amdgcn_mfma_f32_4x4x1f32(a[k], b[k], c, 0, 0, 0);
+    return c;
+}
+
+// this lane's operands of a product (A block map xa at value offset oa, B
+// block map xb at ob) for the 16-block MFMAs: row r of A, column r of B
+template <typename VT>
+__device__ __forceinline__ void gather16(const SpgemmArgs &g, uint32_t xa, int oa, uint32_t xb,
+                                         int ob, int r, typename Mf16<VT>::V &a,
+                                         typename Mf16<VT>::V &b) {
+    using T = typename Mf16<VT>::T;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int ea = 4 * r + k, eb = 4 * k + r;
+        a[k] = ((xa >> ea) & 1u) ? Mf16<VT>::load(g.av, oa + __popc(xa & ((1u << ea) - 1u))) : T(0);
+        b[k] = ((xb >> eb) & 1u) ? Mf16<VT>::load(g.bv, ob + __popc(xb & ((1u << eb) - 1u))) : T(0);
+    }
+}
+
 __device__ __forceinline__ int lower_bound_key(const int32_t *keys, int n, int key) {
     int lo = 0, hi = n;
     while (lo < hi) {
@@ -663,6 +720,46 @@ __device__ void numeric_row(int i, const SpgemmArgs &g, const int32_t *__restric
                 st.p_slot[r] = slot;
             }
             wave_sync();
+            if constexpr (!std::is_same<VT, double>::value) {
+                // FP32 / FP16: 16 products per 16-block MFMA, slot b16 = lane / 4
+                // takes product g0 + b16; their results are added one slot
+                // after the other (fixed order: deterministic, no atomics)
+                const int b16 = lane >> 2, r4 = lane & 3;
+                for (int g0 = 0; g0 < np; g0 += 16) {
+                    const int idx = g0 + b16;
+                    const bool on16 = idx < np;
+                    typename Mf16<VT>::V fa = {}, fb = {};
+                    int sl = 0;
+                    if (on16) {
+                        sl = st.p_slot[idx];
+                        gather16<VT>(g, uint32_t(st.p_ma[idx]), st.p_aoff[idx],
+                                     uint32_t(st.p_mb[idx]), st.p_boff[idx], r4, fa, fb);
+                    }
+                    const mf_float4 d = mfma16<VT>(fa, fb, mf_float4{0.f, 0.f, 0.f, 0.f});
+                    // slot b2's 16 entries go to lanes 0..15 (entry e = lane, from
+                    // register e / 4 of lane 4 b2 + e % 4), which add them
+                    const int e16 = lane & 15, v16 = e16 >> 2;
+                    const int nslot = min(16, np - g0);  // wave-uniform
+                    for (int b2 = 0; b2 < nslot; ++b2) {
+                        const int from = 4 * b2 + (e16 & 3);
+                        const float t0 = __shfl(d[0], from), t1 = __shfl(d[1], from),
+                                    t2 = __shfl(d[2], from), t3 = __shfl(d[3], from);
+                        const float val = v16 == 0 ? t0 : (v16 == 1 ? t1 : (v16 == 2 ? t2 : t3));
+                        const int sl2 = __shfl(sl, 4 * b2);
+                        if (lane < 16) {
+                            if constexpr (kCmp) {
+                                const uint32_t m = scm[sl2];
+                                if ((m >> e16) & 1u)
+                                    acc[scvo[sl2] + __popc(m & ((1u << e16) - 1u))] += double(val);
+                            } else {
+                                acc[sl2 * 16 + e16] += double(val);
+                            }
+                        }
+                        wave_sync();
+                    }
+                }
+                continue;
+            }
             // batches of four MFMA groups (16 products): all gathers first
             for (int g0 = 0; g0 < np; g0 += 16) {
                 double av[4], bv[4];
@@ -907,6 +1004,126 @@ __global__ __launch_bounds__(256) void k_numeric_reg(int count, const int32_t *_
     }
 }
 
+// The register kernel for FP32 / FP16 operands: the same product and slot
+// assignment as k_numeric_reg, but the 16-block MFMAs (FP32 accumulation)
+// take 16 C blocks at a time: lane 4b + r works for C block g0 + b, gathers
+// row r of A and column r of B of that block's next product and chains the
+// block's products (in product order) through the MFMA accumulator.  Lane
+// 4b + r then holds entries (v, r), v = 0..3, of the finished block.
+template <typename VT>
+__global__ __launch_bounds__(256) void k_numeric_reg16(int count, const int32_t *__restrict__ rows,
+                                                       SpgemmArgs g,
+                                                       const int32_t *__restrict__ boff,
+                                                       const int32_t *__restrict__ cptr,
+                                                       const int32_t *__restrict__ ccol,
+                                                       const uint16_t *__restrict__ cmap,
+                                                       const int32_t *__restrict__ cvp,
+                                                       double *__restrict__ cval) {
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= count) return;  // whole wave
+    const int i = rows[r];
+    const int b16 = lane >> 2, r4 = lane & 3;
+    const int c0 = cptr[i], cnt = cptr[i + 1] - c0;
+    const int key = lane < cnt ? ccol[c0 + lane] : INT_MAX;
+    const int a_beg = g.ap[i], na = g.ap[i + 1] - a_beg;
+    int ma = 0, bq0 = 0, blen = 0;
+    if (lane < na) {
+        const int k = g.ac[a_beg + lane];
+        ma = g.am[a_beg + lane];
+        bq0 = g.bp[k];
+        blen = g.bp[k + 1] - bq0;
+    }
+    const int pc = __popc(uint32_t(ma));
+    int pinc = pc, linc = blen;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t1 = __shfl_up(pinc, d), t2 = __shfl_up(linc, d);
+        if (lane >= d) {
+            pinc += t1;
+            linc += t2;
+        }
+    }
+    const int aoff = g.avp[i] + pinc - pc;
+    const int pst = linc - blen;
+    const int ptot = __shfl(linc, 63);
+    // product t = lane: owner = last A block j < na with pst[j] <= t
+    const int t = lane;
+    int own = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1) {
+        const int cand = own + step;
+        const int v = __shfl(pst, cand < 64 ? cand : 63);
+        if (cand < na && v <= t) own = cand;
+    }
+    const int q = __shfl(bq0, own) + (t - __shfl(pst, own));
+    const int pma = __shfl(ma, own);
+    const int paoff = __shfl(aoff, own);
+    int pmb = 0, pboff = 0, bcol = 0;
+    if (t < ptot) {
+        pmb = g.bm[q];
+        bcol = g.bc[q];
+        pboff = boff[q];
+    }
+    const bool valid = (t < ptot) && bitmap_mul(uint32_t(pma), uint32_t(pmb)) != 0u;
+    // C slot: number of C block columns below bcol
+    int slot = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1) {
+        const int cand = slot + step;
+        const int kv = __shfl(key, cand - 1);
+        if (cand <= cnt && kv < bcol) slot = cand;
+    }
+    int voff = cvp[i];
+    for (int g0 = 0; g0 < cnt; g0 += 16) {
+        uint64_t mine = 0ull;
+        int iters = 0;
+#pragma unroll
+        for (int sb = 0; sb < 16; ++sb) {
+            const uint64_t m = __ballot(valid && slot == g0 + sb);
+            if (sb == b16) mine = m;
+            iters = max(iters, __popcll(m));
+        }
+        mf_float4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int it = 0; it < iters; it += 2) {
+            // two products per step: both gathers before the MFMAs
+            typename Mf16<VT>::V fa[2] = {}, fb[2] = {};
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const bool on = mine != 0ull;
+                const int src = on ? __builtin_ctzll(mine) : 0;
+                if (on) mine &= mine - 1ull;
+                const uint32_t xa = uint32_t(__shfl(pma, src)), xb = uint32_t(__shfl(pmb, src));
+                const int oa = __shfl(paoff, src), ob = __shfl(pboff, src);
+                if (on) gather16<VT>(g, xa, oa, xb, ob, r4, fa[u], fb[u]);
+            }
+            acc = mfma16<VT>(fa[0], fb[0], acc);
+            if (it + 1 < iters) acc = mfma16<VT>(fa[1], fb[1], acc);
+        }
+        // lane (b16, r4): entries (v, r4) of C block g0 + b16
+        const int tb = g0 + b16;
+        const uint32_t cm = tb < cnt ? uint32_t(cmap[c0 + tb]) : 0u;
+        const int cpc = __popc(cm);
+        // exclusive prefix of the 16 blocks' value counts (lanes 4b carry them)
+        const int x = r4 == 0 ? cpc : 0;
+        int inc = x;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int v = __shfl_up(inc, d);
+            if (lane >= d) inc += v;
+        }
+        const int excl = __shfl(inc, 4 * b16) - cpc;
+        const int tot = __shfl(inc, 63);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int e = 4 * v + r4;
+            if ((cm >> e) & 1u)
+                cval[voff + excl + __popc(cm & ((1u << e) - 1u))] = round_out<VT>(double(acc[v]));
+        }
+        voff += tot;
+    }
+}
+
 // Numeric bin of a C block row: 0 = register kernel (<= 64 products and A
 // blocks), 1 = LDS accumulators (<= kNumCap C blocks), 2 = global slab.
 struct NumericBin {
@@ -944,10 +1161,16 @@ template <typename VT>
 int launch_numeric(const SpgemmArgs &g, const int32_t *boff, const Mbsr &C, int32_t brow,
                    const int32_t *list, int32_t reg_rows, int32_t lds_rows, int32_t big_rows,
                    double **nslab, hipStream_t s) {
-    if (reg_rows > 0)
-        hipLaunchKernelGGL((k_numeric_reg<VT>), dim3((reg_rows + 3) / 4), dim3(256), 0, s,
-                           reg_rows, list, g, boff, C.blk_ptr, C.blk_col, C.blk_map, C.val_ptr,
-                           C.val);
+    if (reg_rows > 0) {
+        if constexpr (std::is_same<VT, double>::value)
+            hipLaunchKernelGGL((k_numeric_reg<VT>), dim3((reg_rows + 3) / 4), dim3(256), 0, s,
+                               reg_rows, list, g, boff, C.blk_ptr, C.blk_col, C.blk_map,
+                               C.val_ptr, C.val);
+        else
+            hipLaunchKernelGGL((k_numeric_reg16<VT>), dim3((reg_rows + 3) / 4), dim3(256), 0, s,
+                               reg_rows, list, g, boff, C.blk_ptr, C.blk_col, C.blk_map,
+                               C.val_ptr, C.val);
+    }
     if (lds_rows > 0) {
         hipLaunchKernelGGL((k_numeric_cmp<VT, kNumCap, kCmpValShort>), dim3((lds_rows + 3) / 4),
                            dim3(256), 0, s, lds_rows, list + brow, g, boff, C.blk_ptr, C.blk_col,

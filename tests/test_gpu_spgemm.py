@@ -217,11 +217,14 @@ def test_hypre_spgemm_boundary(dev):
 @pytest.mark.parametrize("case", ["random", "stencil_27", "big_rows"])
 def test_spgemm_reduced_precision(dev, prec, dt, case):
     """spgemm_amgT_fp32 / _fp16 (csr_spgemm_device.c:2224-2578, 1863-2223):
-    operands rounded to FP32 / FP16, products of the rounded operands
-    accumulated at least in FP32 and C entries rounded to FP32.  Oracle: the
-    FP64 product of the rounded operands, rounded to FP32; bar: 2^-22 of the
-    (|A| |B|) scale of the rounded operands (one FP32 rounding of the entry plus
-    the FP64 accumulation order)."""
+    operands rounded to FP32 / FP16, block products on the 16-block FP32 /
+    FP16 MFMAs with FP32 accumulation, C entries stored as FP32 numbers.
+    Oracle: the FP64 product of the rounded operands.  Bar: the FP32 summation
+    bound, (2 c + 2) 2^-24 of the (|A| |B|) scale of the rounded operands for
+    an entry with c products (each product and each addition rounded once,
+    plus the final FP32 rounding); and within the input rounding (2^-22 for
+    FP32, 2^-9 for FP16) plus that bound of the FP64 product of the unrounded
+    operands."""
     if case == "random":
         m, k, n = 61, 47, 53
         AI, AJ, AA = rand_csr(m, k, 0.2, 11)
@@ -239,7 +242,6 @@ def test_spgemm_reduced_precision(dev, prec, dt, case):
     rb = BA.astype(dt).astype(np.float64)
     ref = O.mbsr_spgemm(O.csr2mbsr(m, k, AI, AJ, ra), O.csr2mbsr(k, n, BI, BJ, rb))
     RI, RJ, RV = O.mbsr2csr(ref)
-    RV32 = RV.astype(np.float32).astype(np.float64)
     GA, GB = make(dev, m, k, AI, AJ, AA), make(dev, k, n, BI, BJ, BA)
     GA.set_precision(prec)
     GC = GA.spgemm(GB)
@@ -250,10 +252,14 @@ def test_spgemm_reduced_precision(dev, prec, dt, case):
     assert np.array_equal(cv, cv.astype(np.float32).astype(np.float64))
     scale = abs_product_scale(m, k, n, AI, AJ, np.abs(ra), BI, BJ, np.abs(rb))
     scale.sort_indices()
-    assert np.all(np.abs(cv - RV32) <= 2.0 ** -22 * scale.data + 1e-300)
+    # products per C entry: the product of the patterns
+    cnt = abs_product_scale(m, k, n, AI, AJ, np.ones_like(ra), BI, BJ, np.ones_like(rb))
+    cnt.sort_indices()
+    bound = (2.0 * cnt.data + 2.0) * 2.0 ** -24 * scale.data
+    assert np.all(np.abs(cv - RV) <= bound + 1e-300)
     # and the FP64 product stays within FP32 / FP16 input rounding of the exact one
     GA.set_precision(-1)
     exact = GA.spgemm(GB).to_csr()[2].cpu().numpy()
     tol = 2.0 ** -22 if dt == np.float32 else 2.0 ** -9
-    assert np.all(np.abs(cv - exact) <= tol * abs_product_scale(
-        m, k, n, AI, AJ, AA, BI, BJ, BA).sorted_indices().data + 1e-300)
+    full = abs_product_scale(m, k, n, AI, AJ, AA, BI, BJ, BA).sorted_indices().data
+    assert np.all(np.abs(cv - exact) <= tol * full + bound + 1e-300)
