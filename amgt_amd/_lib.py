@@ -49,6 +49,7 @@ EXPORTS = (
     "amgt_mbsr_autotune", "amgt_mbsr_spmv_sweep", "amgt_mbsr_tune_from_stats",
     "amgt_mbsr_spmv", "amgt_mbsr_spmv_noalloc", "amgt_mbsr_spmm",
     "amgt_restrict_plan_create", "amgt_restrict_plan_destroy", "amgt_mbsr_residual_restrict",
+    "amgt_pcg_work_bytes", "amgt_dot", "amgt_pcg_update_xr", "amgt_pcg_update_p",
     "amgt_mbsr_spgemm",
     "amgt_mbsr_to_csr", "amgt_free", "amgt_csr_spgemm", "amgt_csr_matvec",
     "amgt_mbsr_diagonal", "amgt_mbsr_relax_jacobi", "amgt_mbsr_transpose",
@@ -133,6 +134,10 @@ def load():
         "amgt_mbsr_spmv_noalloc": (_int, [_vp, _dbl, _vp, _dbl, _vp, _vp, _vp]),
         "amgt_restrict_plan_create": (_int, [_vp, _vp, _vp, _pvp]),
         "amgt_restrict_plan_destroy": (_int, [_vp]),
+        "amgt_pcg_work_bytes": (ctypes.c_size_t, []),
+        "amgt_dot": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
+        "amgt_pcg_update_xr": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "amgt_pcg_update_p": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
         "amgt_mbsr_residual_restrict": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "amgt_mbsr_spmm": (_int, [_vp, _int, _int, _dbl, _vp, _i64, _dbl, _vp, _i64, _vp, _i64,
                                   _vp]),

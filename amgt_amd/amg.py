@@ -42,6 +42,7 @@ FP64.  PCG itself (level 0 vectors, dot products, updates) is FP64.
 """
 from __future__ import annotations
 
+import ctypes
 import time
 from dataclasses import dataclass, field
 
@@ -370,9 +371,80 @@ class GeometricAMG:
     # ---- Krylov ------------------------------------------------------------------------
     def pcg(self, b: torch.Tensor, tol: float = 1e-8, max_iter: int = 200,
             segments: torch.Tensor | None = None):
-        """Preconditioned CG (two-norm relative residual, as ij -tol with PCG).
-        With `segments` (problem index of every row of a batch), the test is
-        every problem's own relative residual."""
+        """Preconditioned CG (hypre_PCGSolve, krylov/pcg.c:320-900: two-norm
+        relative residual, as ij -tol with PCG).  With `segments` (problem index
+        of every row of a batch), the test is every problem's own relative
+        residual.  With a captured V-cycle (capture()) an iteration is the
+        library's kernels with every scalar in device memory: Ap = A p, p.Ap,
+        one launch for x += alpha p, r -= alpha Ap and the partials of r.r,
+        (the host reads r.r for the stopping test), the V-cycle replay z = M r,
+        r.z and one launch for p = z + beta p.  r and z are the V-cycle's own
+        input and output vectors (no copies).  Without a captured cycle:
+        pcg_eager."""
+        if self._graph is None:
+            return self.pcg_eager(b, tol, max_iter, segments)
+        L0 = self.levels[0]
+        n = L0.n
+        st = self._pcg_state()
+        lib = L.load()
+        # z: the cycle's output vector (u2 of the finest level, u of a
+        # one-level hierarchy)
+        x, r, z, p, ap = st["x"], L0.f, self._graph_out, st["p"], st["ap"]
+        sc, work = st["sc"], st["work"]
+        ptr = lambda t, k=0: ctypes.c_void_p(t.data_ptr() + 8 * k)  # noqa: E731
+        sp_ = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        # sc: 0 rz, 1 pAp, 2 rr, 3 rz_new, 4 bb
+        x.zero_()
+        r.copy_(b)
+        L.check(lib.amgt_dot(n, ptr(b), ptr(b), ptr(sc, 4), ptr(work), sp_), "amgt_dot")
+        if segments is not None:
+            norms = SegmentNorms(segments)
+            bseg = norms(b)
+            bseg = torch.where(bseg > 0, bseg, torch.ones_like(bseg))
+        bb = sc[4].item()
+        if bb == 0.0:
+            return x.clone(), 0, [0.0]
+        self._graph.replay()
+        p.copy_(z)
+        L.check(lib.amgt_dot(n, ptr(r), ptr(z), ptr(sc, 0), ptr(work), sp_), "amgt_dot")
+        hist = [1.0]
+        it = 0
+        while it < max_iter:
+            L0.A.spmv(p, ap)
+            L.check(lib.amgt_dot(n, ptr(p), ptr(ap), ptr(sc, 1), ptr(work), sp_), "amgt_dot")
+            L.check(lib.amgt_pcg_update_xr(n, ptr(sc, 0), ptr(sc, 1), ptr(p), ptr(ap), ptr(x),
+                                           ptr(r), ptr(sc, 2), ptr(work), sp_),
+                    "amgt_pcg_update_xr")
+            it += 1
+            if segments is not None:
+                rel = (norms(r) / bseg).max().item()
+            else:
+                rel = (max(sc[2].item(), 0.0) / bb) ** 0.5
+            hist.append(rel)
+            if rel < tol:
+                break
+            self._graph.replay()
+            L.check(lib.amgt_dot(n, ptr(r), ptr(z), ptr(sc, 3), ptr(work), sp_), "amgt_dot")
+            L.check(lib.amgt_pcg_update_p(n, ptr(sc, 3), ptr(sc, 0), ptr(z), ptr(p), sp_),
+                    "amgt_pcg_update_p")
+        return x.clone(), it, hist
+
+    def _pcg_state(self) -> dict:
+        """PCG vectors and device scalars of this hierarchy (allocated once)."""
+        if getattr(self, "_pcg_st", None) is None:
+            n = self.levels[0].n
+            dev = self.device
+            z = lambda: torch.zeros(n, dtype=torch.float64, device=dev)  # noqa: E731
+            nw = int(L.load().amgt_pcg_work_bytes()) // 8
+            self._pcg_st = {"x": z(), "p": z(), "ap": z(),
+                            "sc": torch.zeros(8, dtype=torch.float64, device=dev),
+                            "work": torch.zeros(nw, dtype=torch.float64, device=dev)}
+        return self._pcg_st
+
+    def pcg_eager(self, b: torch.Tensor, tol: float = 1e-8, max_iter: int = 200,
+                  segments: torch.Tensor | None = None):
+        """PCG with torch vector operations around precondition() (the form
+        used before capture(); the reference for the graph-replayed pcg)."""
         A = self.levels[0].A
         x = torch.zeros_like(b)
         r = b.clone()

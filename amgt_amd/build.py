@@ -19,7 +19,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = ["core.hip", "csr2mbsr.hip", "spmv.hip", "transpose.hip", "mbsr2csr.hip", "spgemm.hip",
            "relax.hip", "io.hip", "coarsen.hip", "hmis.hip", "policy.hip", "interp.hip",
            "noalloc.hip", "galerkin.hip",
-           "restrict.hip"]
+           "restrict.hip", "pcg.hip"]
 
 CFLAGS = [
     "--offload-arch=gfx950",

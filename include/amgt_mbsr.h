@@ -142,6 +142,24 @@ int amgt_mbsr_spmv_sweep(amgt_mbsr_t A, void *stream, float *times_ms);
  * FP64. */
 int amgt_mbsr_spmv(amgt_mbsr_t A, int trans, double alpha, const double *d_x, double beta,
                    const double *d_b, double *d_y, void *stream);
+/* ---- PCG vector kernels (hypre_PCGSolve, krylov/pcg.c:320-900, two-norm form) ----
+ * For the solve driver: every scalar stays in device memory (d_*: one double each),
+ * so one iteration is a fixed sequence of launches that a graph can capture.  Dots are
+ * deterministic (per-workgroup partials over fixed ranges, summed in order by one
+ * workgroup).  d_work: amgt_pcg_work_bytes() bytes of device memory, shared by the calls
+ * of one stream.
+ *   amgt_dot            d_out = x . y
+ *   amgt_pcg_update_xr  alpha = rz / pAp; x += alpha p; r -= alpha Ap; d_rr = r . r
+ *   amgt_pcg_update_p   beta = rz_new / rz; p = z + beta p; then rz = rz_new */
+size_t amgt_pcg_work_bytes(void);
+int amgt_dot(int64_t n, const double *d_x, const double *d_y, double *d_out, void *d_work,
+             void *stream);
+int amgt_pcg_update_xr(int64_t n, const double *d_rz, const double *d_pap, const double *d_p,
+                       const double *d_ap, double *d_x, double *d_r, double *d_rr, void *d_work,
+                       void *stream);
+int amgt_pcg_update_p(int64_t n, const double *d_rz_new, double *d_rz, const double *d_z,
+                      double *d_p, void *stream);
+
 /* ---- residual and restriction in one launch ----
  * The V-cycle's r = f - A u and f_c = R r (par_cycle.c:443 and :704: a matvec, then
  * the restriction matvec with R = P^T) as one launch of the block SpMV kernel: each
