@@ -787,7 +787,8 @@ __global__ __launch_bounds__(kThreads) void k_rows_unsorted(int32_t n, const int
 // first runs with a small table of P columns; a row with more distinct columns
 // moves on (use[i] = the next code) to the tier's large table, and from there
 // to the thread kernel.
-constexpr int kTier64 = 1, kTier64Big = 2, kTier128 = 3, kTier128Big = 4, kTier256 = 5;
+constexpr int kTier64 = 1, kTier64Big = 2, kTier128 = 3, kTier128Big = 4, kTier256 = 5,
+              kTier256Big = 6, kTier512 = 7;
 __global__ __launch_bounds__(kThreads) void k_wave_rows(int32_t n, const int32_t *__restrict__ I,
                                                        const int32_t *__restrict__ SI,
                                                        const int32_t *__restrict__ cf,
@@ -800,6 +801,7 @@ __global__ __launch_bounds__(kThreads) void k_wave_rows(int32_t n, const int32_t
         if (la <= 64 && ls <= 64) u = kTier64;
         else if (allow_big && la <= 128 && ls <= 128) u = kTier128;
         else if (allow_big && la <= 256 && ls <= 256) u = kTier256;
+        else if (allow_big && la <= 512 && ls <= 512) u = kTier512;
     }
     use[i] = u;
 }
@@ -1069,10 +1071,11 @@ int amgt_amg_extpi_interp(int32_t n, const int32_t *A_i, const int32_t *A_j,
                     if (hipGetLastError() != hipSuccess) st = AMGT_ERR_HIP;
                 }
                 // (rows with this code, kernel, code for the rows that overflow it)
-                const int stages[5][2] = {{kTier64, kTier64Big}, {kTier64Big, 0},
+                const int stages[7][2] = {{kTier64, kTier64Big}, {kTier64Big, 0},
                                           {kTier128, kTier128Big}, {kTier128Big, 0},
-                                          {kTier256, 0}};
-                for (int sg = 0; sg < 5 && st == AMGT_OK; ++sg) {
+                                          {kTier256, kTier256Big}, {kTier256Big, 0},
+                                          {kTier512, 0}};
+                for (int sg = 0; sg < 7 && st == AMGT_OK; ++sg) {
                     const int which = stages[sg][0], next = stages[sg][1];
                     int32_t cntw = 0;
                     hipLaunchKernelGGL(k_flag_eq, g, b, 0, s, n, use, which, flag2);
@@ -1096,6 +1099,8 @@ int amgt_amg_extpi_interp(int32_t n, const int32_t *A_i, const int32_t *A_j,
                         case kTier64: AMGT_EXTPI_WAVE(64, 128, 512); break;      // ~8.4 KB
                         case kTier64Big: AMGT_EXTPI_WAVE(64, 384, 1024); break;  // ~15.6 KB
                         case kTier128: AMGT_EXTPI_WAVE(128, 256, 1024); break;   // ~16 KB
+                        case kTier256Big: AMGT_EXTPI_WAVE(256, 1536, 4096); break;  // ~60 KB
+                        case kTier512: AMGT_EXTPI_WAVE(512, 1536, 4096); break;   // ~70 KB
                         default: AMGT_EXTPI_WAVE(256, 768, 2048); break;         // ~35 KB
                     }
 #undef AMGT_EXTPI_WAVE

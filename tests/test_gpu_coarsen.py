@@ -166,12 +166,15 @@ def _rand_long(n, per_row, seed):
     return M.indptr.astype(np.int32), M.indices.astype(np.int32), M.data.copy()
 
 
-@pytest.mark.parametrize("n,per_row", [(6000, 40), (20000, 100)])
+@pytest.mark.parametrize("n,per_row", [(6000, 40), (20000, 100), (4000, 200), (3000, 400)])
 def test_extpi_wave_tiers_match_oracle(dev, n, per_row):
     """Rows whose distinct P columns overflow the wave kernels' small tables
     (6000 x 40: a few rows over 128 columns of the 64-entry tier; 20000 x 100:
     every row over the 256 of the 128-entry tier) are redone with the large
-    tables: S, the C/F split and P (P max 4) equal the oracle's exactly."""
+    tables; rows of about 200 entries overflow the 256-entry tier's 768
+    columns into its 1536-column table, rows of about 400 entries take the
+    512-entry tier: S, the C/F split and P (P max 4) equal the oracle's
+    exactly."""
     print(f"[extpi tiers] n={n} per_row={per_row}", flush=True)
     I, J, A = _rand_long(n, per_row, 7)
     dI, dJ, dA = T(I, dev), T(J, dev), T(A, dev)
