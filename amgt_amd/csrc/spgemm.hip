@@ -501,21 +501,57 @@ __global__ __launch_bounds__(256) void k_upper_bound(int32_t brow, const int32_t
     rowpass::write_part(i < brow, vb, v, 0, kBin1Ub, parts);
 }
 
-__global__ __launch_bounds__(256) void k_compact_c(int32_t brow, const int64_t *__restrict__ ub_off,
+// (rows: the block rows to copy, or every row 0 .. count-1 when NULL)
+__global__ __launch_bounds__(256) void k_compact_c(int32_t count, const int32_t *__restrict__ rows,
+                                                   const int64_t *__restrict__ ub_off,
                                                    const int32_t *__restrict__ cptr,
                                                    const int32_t *__restrict__ tcol,
                                                    const uint32_t *__restrict__ tmap,
                                                    int32_t *__restrict__ ccol,
                                                    uint16_t *__restrict__ cmap) {
     const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= brow) return;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= count) return;
+    const int i = rows ? rows[r] : r;
     const int64_t s = ub_off[i];
     const int p0 = cptr[i], n = cptr[i + 1] - p0;
     for (int t = lane; t < n; t += 64) {
         ccol[p0 + t] = tcol[s + t];
         cmap[p0 + t] = uint16_t(tmap[s + t]);
     }
+}
+
+// Inclusive sum / max over the 64 lanes of a wave from DPP moves: row shifts
+// by 1, 2, 4, 8 within each row of 16 lanes, then the last lane of a row
+// broadcast into the next row (row_bcast:15) and lane 31 into rows 2 and 3
+// (row_bcast:31).  Lanes a move does not reach add 0 (the `old` operand).
+// One VALU operation per step, no LDS traffic.
+// Orders the LDS accesses of a wave's lanes: what one lane stored before it
+// is what another lane loads after it (the wave's LDS operations execute in
+// program order; this keeps the compiler from moving or forwarding them).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ int wave_incl_sum(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
+    return v;
+}
+// (v >= 0)
+__device__ __forceinline__ int wave_incl_max(int v) {
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));
+    return v;
 }
 
 // ---- numeric -------------------------------------------------------------------------
@@ -895,32 +931,59 @@ __global__ __launch_bounds__(256) void k_numeric_global(int count, const int32_t
     }
 }
 
-// Rows with at most 64 products and 64 A blocks: one wave per row, nothing in
-// LDS.  Lane j holds A block j, lane t product t, lane c the block column of C
-// block c (owner and slot found with branchless shuffle searches).  Products
-// are grouped by C block: block slot b of the MFMA accumulates C block
-// g0 + b in registers, chaining its products (in product order) through the
-// MFMA's accumulator input, and the finished blocks go straight to the
-// compressed value array.  Deterministic, no atomics.
-template <typename VT>
-__global__ __launch_bounds__(256) void k_numeric_reg(int count, const int32_t *__restrict__ rows,
-                                                     SpgemmArgs g,
-                                                     const int32_t *__restrict__ boff,
-                                                     const int32_t *__restrict__ cptr,
-                                                     const int32_t *__restrict__ ccol,
-                                                     const uint16_t *__restrict__ cmap,
-                                                     const int32_t *__restrict__ cvp,
-                                                     double *__restrict__ cval) {
+// The register bin in FP64, outer-product order: for each A block of the row
+// (in stored order) its products with the blocks of B's row, four per
+// v_mfma_f64_4x4x4f64: the A block in every block slot, four consecutive
+// blocks of the B row in the slots.  A B row never repeats a block column, so
+// the four results belong to four distinct C blocks; each is added to its C
+// block's dense 4x4 accumulator in LDS.  One wave owns a row and its LDS
+// accumulators, and its LDS accesses execute in program order, so every C
+// entry sums its products in A-block order: deterministic, no atomics.
+// (Grouping the products by C block instead, one C block per block slot with
+// the products chained through the MFMA accumulator, pads every group of four
+// slots to its longest one and needs per-slot ballots and four shuffles per
+// product: 0.85 against 0.70 ms for the numeric pass of A^2 on 7-pt 128^3.)
+// Lane t of the prologue holds product t (owner A block and C slot found
+// with branchless shuffle searches).  CAPB: C blocks per row the wave's
+// accumulators hold (at least the bin's longest C row).
+template <int CAPB>
+__global__ __launch_bounds__(256) void k_numeric_outer(int count, const int32_t *__restrict__ rows,
+                                                       SpgemmArgs g,
+                                                       const int32_t *__restrict__ boff,
+                                                       const int32_t *__restrict__ cptr,
+                                                       const int64_t *__restrict__ ub_off,
+                                                       const int32_t *__restrict__ tcol,
+                                                       const uint32_t *__restrict__ tmap,
+                                                       int32_t *__restrict__ ccol,
+                                                       uint16_t *__restrict__ cmap,
+                                                       const int32_t *__restrict__ cvp,
+                                                       double *__restrict__ cval) {
+    __shared__ double s_acc[4][CAPB * 16];
+    __shared__ int32_t s_own[4][64];
     const int lane = threadIdx.x & 63;
-    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (r >= count) return;  // whole wave
+    const int w = threadIdx.x >> 6;
+    const int r = blockIdx.x * 4 + w;
+    if (r >= count) return;  // whole wave; the waves share no LDS
+    double *acc = s_acc[w];
     const int i = rows[r];
     const int bs = (lane >> 2) & 3;
     const int eA = 4 * (lane & 3) + (lane >> 4);
     const int eB = 4 * (lane >> 4) + (lane & 3);
     const uint32_t belowA = (1u << eA) - 1u, belowB = (1u << eB) - 1u;
+    // C's block columns and bitmaps as the symbolic pass left them in the
+    // row's slot of the temporaries: copied to C here (cnt <= min(64, CAPB):
+    // the bin's rule)
     const int c0 = cptr[i], cnt = cptr[i + 1] - c0;
-    const int key = lane < cnt ? ccol[c0 + lane] : INT_MAX;
+    const int64_t so = ub_off[i];
+    const int key = lane < cnt ? tcol[so + lane] : INT_MAX;
+    const uint32_t cmy = lane < cnt ? tmap[so + lane] : 0u;
+    if (lane < cnt) {
+        ccol[c0 + lane] = key;
+        cmap[c0 + lane] = uint16_t(cmy);
+    }
+    for (int e = lane; e < cnt * 16; e += 64) acc[e] = 0.0;
+    int32_t *own_of = s_own[w];
+    own_of[lane] = 0;
     const int a_beg = g.ap[i], na = g.ap[i + 1] - a_beg;
     int ma = 0, bq0 = 0, blen = 0;
     if (lane < na) {
@@ -930,30 +993,19 @@ __global__ __launch_bounds__(256) void k_numeric_reg(int count, const int32_t *_
         blen = g.bp[k + 1] - bq0;
     }
     const int pc = __popc(uint32_t(ma));
-    int pinc = pc, linc = blen;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int t1 = __shfl_up(pinc, d), t2 = __shfl_up(linc, d);
-        if (lane >= d) {
-            pinc += t1;
-            linc += t2;
-        }
-    }
-    const int aoff = g.avp[i] + pinc - pc;
+    const int aoff = g.avp[i] + wave_incl_sum(pc) - pc;
+    const int linc = wave_incl_sum(blen);
     const int pst = linc - blen;
-    const int ptot = __shfl(linc, 63);
-    // product t = lane: owner = last A block j < na with pst[j] <= t
+    const int ptot = __builtin_amdgcn_readlane(linc, 63);
+    // product t = lane: owner = last A block j < na with pst[j] <= t: every
+    // A block with products marks its first product, a max-scan spreads it
+    wave_lds_sync();
+    if (blen > 0) own_of[pst] = lane;  // pst < ptot <= 64, distinct
+    wave_lds_sync();
     const int t = lane;
-    int own = 0;
-#pragma unroll
-    for (int step = 32; step >= 1; step >>= 1) {
-        const int cand = own + step;
-        const int v = __shfl(pst, cand < 64 ? cand : 63);
-        if (cand < na && v <= t) own = cand;
-    }
+    const int own = wave_incl_max(own_of[lane]);
     const int q = __shfl(bq0, own) + (t - __shfl(pst, own));
     const int pma = __shfl(ma, own);
-    const int paoff = __shfl(aoff, own);
     int pmb = 0, pboff = 0, bcol = 0;
     if (t < ptot) {
         pmb = g.bm[q];
@@ -961,7 +1013,6 @@ __global__ __launch_bounds__(256) void k_numeric_reg(int count, const int32_t *_
         pboff = boff[q];
     }
     const bool valid = (t < ptot) && bitmap_mul(uint32_t(pma), uint32_t(pmb)) != 0u;
-    // C slot: number of C block columns below bcol
     int slot = 0;
 #pragma unroll
     for (int step = 32; step >= 1; step >>= 1) {
@@ -969,43 +1020,59 @@ __global__ __launch_bounds__(256) void k_numeric_reg(int count, const int32_t *_
         const int kv = __shfl(key, cand - 1);
         if (cand <= cnt && kv < bcol) slot = cand;
     }
-    int voff = cvp[i];
-    for (int g0 = 0; g0 < cnt; g0 += 4) {
-        const uint64_t m0 = __ballot(valid && slot == g0), m1 = __ballot(valid && slot == g0 + 1),
-                       m2 = __ballot(valid && slot == g0 + 2), m3 = __ballot(valid && slot == g0 + 3);
-        uint64_t mine = bs == 0 ? m0 : (bs == 1 ? m1 : (bs == 2 ? m2 : m3));
-        const int iters = max(max(__popcll(m0), __popcll(m1)), max(__popcll(m2), __popcll(m3)));
-        double acc = 0.0;
-        for (int it = 0; it < iters; it += 2) {
-            // two products per step: both gathers before the MFMAs
-            double a[2], b[2];
+    // product record: B bitmap and C slot, or -1 for a product with an empty
+    // bitmap product (it adds nothing)
+    const int pinfo = valid ? (pmb | (slot << 16)) : -1;
+    wave_lds_sync();  // the zeroed accumulators, before other lanes add to them
+    for (int j = 0; j < na; ++j) {  // na is wave-uniform
+        const uint32_t xa = uint32_t(__builtin_amdgcn_readlane(ma, j));
+        const int oa = __builtin_amdgcn_readlane(aoff, j);
+        const int st = __builtin_amdgcn_readlane(pst, j);
+        const int bl = __builtin_amdgcn_readlane(blen, j);
+        const double av = ((xa >> eA) & 1u) ? ldv<double>(g.av, oa + __popc(xa & belowA)) : 0.0;
+        for (int s0 = 0; s0 < bl; s0 += 8) {
+            // two steps of four products: both gathers before the MFMAs
+            double bv[2];
+            int info[2];
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
-                const bool on = mine != 0ull;
-                const int src = on ? __builtin_ctzll(mine) : 0;
-                if (on) mine &= mine - 1ull;
-                const uint32_t xa = uint32_t(__shfl(pma, src)), xb = uint32_t(__shfl(pmb, src));
-                const int oa = __shfl(paoff, src), ob = __shfl(pboff, src);
-                a[u] = (on && ((xa >> eA) & 1u)) ? ldv<VT>(g.av, oa + __popc(xa & belowA)) : 0.0;
-                b[u] = (on && ((xb >> eB) & 1u)) ? ldv<VT>(g.bv, ob + __popc(xb & belowB)) : 0.0;
+                const bool in = s0 + 4 * u + bs < bl;
+                const int src = in ? st + s0 + 4 * u + bs : 0;  // < ptot <= 64
+                const int inf = __shfl(pinfo, src);
+                const int ob = __shfl(pboff, src);
+                info[u] = in ? inf : -1;
+                const uint32_t xb = uint32_t(info[u]) & 0xFFFFu;
+                bv[u] = (info[u] >= 0 && ((xb >> eB) & 1u))
+                            ? ldv<double>(g.bv, ob + __popc(xb & belowB)) : 0.0;
             }
-            acc = __builtin_amdgcn_mfma_f64_4x4x4f64(a[0], b[0], acc, 0, 0, 0);
-            if (it + 1 < iters) acc = __builtin_amdgcn_mfma_f64_4x4x4f64(a[1], b[1], acc, 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                if (s0 + 4 * u < bl) {  // uniform: the MFMA runs on the whole wave
+                    const double d = __builtin_amdgcn_mfma_f64_4x4x4f64(av, bv[u], 0.0, 0, 0, 0);
+                    // lane (slot bs, entry eB) holds entry eB of its product's C block
+                    if (info[u] >= 0) acc[(info[u] >> 16) * 16 + eB] += d;
+                }
+            }
+            wave_lds_sync();  // a C entry's next product may come from another lane
         }
-        // lane (slot bs, bit eB) holds entry eB of C block g0 + bs
-        const int tb = g0 + bs;
-        const uint32_t cm = tb < cnt ? uint32_t(cmap[c0 + tb]) : 0u;
-        const int cpc = __popc(cm);
-        const int p0c = __shfl(cpc, 0), p1c = __shfl(cpc, 4), p2c = __shfl(cpc, 8),
-                  p3c = __shfl(cpc, 12);
-        const int excl = (bs > 0 ? p0c : 0) + (bs > 1 ? p1c : 0) + (bs > 2 ? p2c : 0);
-        if ((cm >> eB) & 1u) cval[voff + excl + __popc(cm & belowB)] = round_out<VT>(acc);
-        voff += p0c + p1c + p2c + p3c;
+    }
+    // compress by the C bitmaps: block t's values start at the exclusive scan
+    // of the row's popcounts
+    const int cpc = __popc(cmy);
+    const int cexcl = wave_incl_sum(cpc) - cpc;
+    const int vbase = cvp[i];
+    for (int e0 = 0; e0 < cnt * 16; e0 += 64) {
+        const int e = e0 + lane;
+        const int tb = min(e >> 4, 63), bit = e & 15;
+        const uint32_t cm = uint32_t(__shfl(int(cmy), tb));
+        const int ex = __shfl(cexcl, tb);
+        if (e < cnt * 16 && ((cm >> bit) & 1u))
+            cval[vbase + ex + __popc(cm & ((1u << bit) - 1u))] = acc[e];
     }
 }
 
-// The register kernel for FP32 / FP16 operands: the same product and slot
-// assignment as k_numeric_reg, but the 16-block MFMAs (FP32 accumulation)
+// The register bin for FP32 / FP16 operands: the product and slot assignment
+// of k_numeric_outer's prologue; the 16-block MFMAs (FP32 accumulation)
 // take 16 C blocks at a time: lane 4b + r works for C block g0 + b, gathers
 // row r of A and column r of B of that block's next product and chains the
 // block's products (in product order) through the MFMA accumulator.  Lane
@@ -1157,16 +1224,31 @@ __global__ __launch_bounds__(256) void k_scale_values(int64_t n, double f, doubl
 
 // numeric kernels of one value precision; *nslab receives the global slab of
 // the rows over kNumCap blocks (freed by the caller)
+// The FP64 register-bin kernel also copies its rows' block columns and
+// bitmaps from the symbolic temporaries (ub_off, tcol, tmap) to C; the other
+// kernels read them from C.
 template <typename VT>
 int launch_numeric(const SpgemmArgs &g, const int32_t *boff, const Mbsr &C, int32_t brow,
                    const int32_t *list, int32_t reg_rows, int32_t lds_rows, int32_t big_rows,
+                   const int64_t *ub_off, const int32_t *tcol, const uint32_t *tmap,
                    double **nslab, hipStream_t s) {
     if (reg_rows > 0) {
-        if constexpr (std::is_same<VT, double>::value)
-            hipLaunchKernelGGL((k_numeric_reg<VT>), dim3((reg_rows + 3) / 4), dim3(256), 0, s,
-                               reg_rows, list, g, boff, C.blk_ptr, C.blk_col, C.blk_map,
-                               C.val_ptr, C.val);
-        else
+        if constexpr (std::is_same<VT, double>::value) {
+            // accumulators for the bin's longest C row (at most 64 blocks)
+            const dim3 grid((reg_rows + 3) / 4);
+            if (C.max_row_blocks <= 16)
+                hipLaunchKernelGGL((k_numeric_outer<16>), grid, dim3(256), 0, s, reg_rows, list,
+                                   g, boff, C.blk_ptr, ub_off, tcol, tmap, C.blk_col,
+                                   C.blk_map, C.val_ptr, C.val);
+            else if (C.max_row_blocks <= 32)
+                hipLaunchKernelGGL((k_numeric_outer<32>), grid, dim3(256), 0, s, reg_rows, list,
+                                   g, boff, C.blk_ptr, ub_off, tcol, tmap, C.blk_col,
+                                   C.blk_map, C.val_ptr, C.val);
+            else
+                hipLaunchKernelGGL((k_numeric_outer<64>), grid, dim3(256), 0, s, reg_rows, list,
+                                   g, boff, C.blk_ptr, ub_off, tcol, tmap, C.blk_col,
+                                   C.blk_map, C.val_ptr, C.val);
+        } else
             hipLaunchKernelGGL((k_numeric_reg16<VT>), dim3((reg_rows + 3) / 4), dim3(256), 0, s,
                                reg_rows, list, g, boff, C.blk_ptr, C.blk_col, C.blk_map,
                                C.val_ptr, C.val);
@@ -1387,9 +1469,20 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
         cleanup();
         return AMGT_ERR_ALLOC;
     }
-    if (brow > 0)
-        hipLaunchKernelGGL(k_compact_c, dim3((brow + 3) / 4), dim3(256), 0, s, brow, ub_off,
-                           C.blk_ptr, tcol, tmap, C.blk_col, C.blk_map);
+    // C's block columns and bitmaps out of the symbolic temporaries; in FP64
+    // the register-bin numeric kernel copies its own rows
+    const bool fp64_numeric = !C.pattern_only && precision != AMGT_FLOAT && precision != AMGT_HALF;
+    if (brow > 0 && !fp64_numeric)
+        hipLaunchKernelGGL(k_compact_c, dim3((brow + 3) / 4), dim3(256), 0, s, brow,
+                           static_cast<const int32_t *>(nullptr), ub_off, C.blk_ptr, tcol, tmap,
+                           C.blk_col, C.blk_map);
+    if (fp64_numeric && lds_rows > 0)
+        hipLaunchKernelGGL(k_compact_c, dim3((lds_rows + 3) / 4), dim3(256), 0, s, lds_rows,
+                           list + brow, ub_off, C.blk_ptr, tcol, tmap, C.blk_col, C.blk_map);
+    if (fp64_numeric && big_rows > 0)
+        hipLaunchKernelGGL(k_compact_c, dim3((big_rows + 3) / 4), dim3(256), 0, s, big_rows,
+                           list + 2 * int64_t(brow), ub_off, C.blk_ptr, tcol, tmap, C.blk_col,
+                           C.blk_map);
     SG_HIP(hipGetLastError());
     if (!C.pattern_only && brow > 0) {
         // value offset of every B block
@@ -1402,17 +1495,17 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
         SG_TRY(exclusive_scan_i32(bpop, boff, B.nblk, s));
         switch (precision) {
             case AMGT_FLOAT:
-                st = launch_numeric<float>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, &nslab, s);
+                st = launch_numeric<float>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, ub_off, tcol, tmap, &nslab, s);
                 break;
             case AMGT_HALF:
-                st = launch_numeric<__half>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, &nslab, s);
+                st = launch_numeric<__half>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, ub_off, tcol, tmap, &nslab, s);
                 // the FP16 copies hold A and B times half_scale: undo both (powers of two, exact)
                 if (st == AMGT_OK && C.nnz > 0)
                     hipLaunchKernelGGL(k_scale_values, dim3(unsigned((C.nnz + 255) / 256)), dim3(256),
                                        0, s, C.nnz, 1.0 / (A.half_scale * B.half_scale), C.val);
                 break;
             default:
-                st = launch_numeric<double>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, &nslab, s);
+                st = launch_numeric<double>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, ub_off, tcol, tmap, &nslab, s);
         }
         if (st != AMGT_OK) {
             cleanup();
