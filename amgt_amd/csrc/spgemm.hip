@@ -53,6 +53,31 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// Inclusive sum / max over the 64 lanes of a wave from DPP moves: row shifts
+// by 1, 2, 4, 8 within each row of 16 lanes, then the last lane of a row
+// broadcast into the next row (row_bcast:15) and lane 31 into rows 2 and 3
+// (row_bcast:31).  Lanes a move does not reach add 0 (the `old` operand).
+// One VALU operation per step, no LDS traffic.
+__device__ __forceinline__ int wave_incl_sum(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
+    return v;
+}
+// (v >= 0)
+__device__ __forceinline__ int wave_incl_max(int v) {
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));
+    return v;
+}
+
 template <int NT>
 __device__ __forceinline__ void grp_sync() {
     if constexpr (NT == 64) {
@@ -284,12 +309,14 @@ __device__ __forceinline__ double round_out<double>(double x) {
 }
 
 // ---- bin 0: rows with at most 64 products and 64 A blocks, one wave per row.
-// Lane j holds A block j, lane t product t (owner found with a branchless
-// shuffle search).  The block columns of the products with a non-zero bitmap
-// product are rank-sorted (each lane counts the keys below its own over an LDS
-// copy of the 64 keys, ties by lane), placed in sorted order, and equal
-// columns are merged by a segmented OR of their bitmap products; the last lane
-// of every run writes the C block.  No hashing, no atomics.
+// Lane j holds A block j, lane t product t (its owner A block: every A block
+// with products marks its first one, a max-scan spreads the marks).  The
+// block columns of the products with a non-zero bitmap product are
+// rank-sorted (each lane counts the keys below its own over an LDS copy of
+// the 64 keys, ties by lane) and placed in sorted order; the first lane of
+// every run of equal columns opens a C block, and every product ORs its
+// bitmap product into its C block's bitmap in LDS (OR does not depend on the
+// order).  No hashing, deterministic.
 __global__ __launch_bounds__(256) void k_symbolic_reg(int count, const int32_t *__restrict__ rows,
                                                       SpgemmArgs g,
                                                       const int64_t *__restrict__ ub_off,
@@ -298,11 +325,11 @@ __global__ __launch_bounds__(256) void k_symbolic_reg(int count, const int32_t *
                                                       int32_t *__restrict__ cnt_out,
                                                       int32_t *__restrict__ nnz_out) {
     __shared__ __align__(16) int32_t s_key[4][64];
-    __shared__ int32_t s_sk[4][64];
-    __shared__ uint32_t s_map[4][64], s_sm[4][64];
+    __shared__ int32_t s_sk[4][64];  // owner marks, then the sorted keys
+    __shared__ uint32_t s_sm[4][64], s_cm[4][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r = blockIdx.x * 4 + w;
-    if (r >= count) return;  // whole wave
+    if (r >= count) return;  // whole wave; the waves share no LDS
     const int i = rows[r];
     const int a_beg = g.ap[i], na = g.ap[i + 1] - a_beg;
     int ma = 0, bq0 = 0, blen = 0;
@@ -312,21 +339,15 @@ __global__ __launch_bounds__(256) void k_symbolic_reg(int count, const int32_t *
         bq0 = g.bp[k];
         blen = g.bp[k + 1] - bq0;
     }
-    int linc = blen;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int t2 = __shfl_up(linc, d);
-        if (lane >= d) linc += t2;
-    }
+    const int linc = wave_incl_sum(blen);
     const int pst = linc - blen;
-    const int ptot = __shfl(linc, 63);
-    int own = 0;
-#pragma unroll
-    for (int step = 32; step >= 1; step >>= 1) {
-        const int cand = own + step;
-        const int v = __shfl(pst, cand < 64 ? cand : 63);
-        if (cand < na && v <= lane) own = cand;
-    }
+    const int ptot = __builtin_amdgcn_readlane(linc, 63);
+    s_sk[w][lane] = 0;
+    s_cm[w][lane] = 0u;
+    wave_sync();
+    if (blen > 0) s_sk[w][pst] = lane;  // pst < ptot <= 64, distinct
+    wave_sync();
+    const int own = wave_incl_max(s_sk[w][lane]);
     const int q = __shfl(bq0, own) + (lane - __shfl(pst, own));
     const uint32_t pma = uint32_t(__shfl(ma, own));
     int key = INT_MAX;
@@ -337,7 +358,6 @@ __global__ __launch_bounds__(256) void k_symbolic_reg(int count, const int32_t *
     }
     const int nvalid = __popcll(__ballot(key != INT_MAX));
     s_key[w][lane] = key;
-    s_map[w][lane] = mc;
     wave_sync();
     int rank = 0;
     for (int j = 0; j < ptot; j += 4) {
@@ -353,28 +373,21 @@ __global__ __launch_bounds__(256) void k_symbolic_reg(int count, const int32_t *
         s_sm[w][rank] = mc;
     }
     wave_sync();
-    int sk = lane < nvalid ? s_sk[w][lane] : INT_MAX;
-    uint32_t sm = lane < nvalid ? s_sm[w][lane] : 0u;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int k2 = __shfl_up(sk, d);
-        const uint32_t m2 = uint32_t(__shfl_up(int(sm), d));
-        if (lane >= d && k2 == sk) sm |= m2;
-    }
-    const int nxt = __shfl_down(sk, 1);
-    const bool tail = lane < nvalid && (lane == 63 || nxt != sk);
-    const uint64_t tails = __ballot(tail);
-    const uint64_t lt = (1ull << lane) - 1ull;
+    const bool in = lane < nvalid;
+    const int sk = in ? s_sk[w][lane] : INT_MAX;
+    const bool head = in && (lane == 0 || s_sk[w][lane > 0 ? lane - 1 : 0] != sk);
+    const uint64_t heads = __ballot(head);
+    const int cslot = __popcll(heads & ((2ull << lane) - 1ull)) - 1;  // C block of sorted lane
+    if (in) atomicOr(&s_cm[w][cslot], s_sm[w][lane]);
+    wave_sync();
+    const int ncb = __popcll(heads);
     const int64_t slot = ub_off[i];
-    if (tail) {
-        const int rk = __popcll(tails & lt);
-        tcol[slot + rk] = sk;
-        tmap[slot + rk] = sm;
-    }
-    int nz = tail ? __popc(sm) : 0;
-    for (int off = 32; off > 0; off >>= 1) nz += __shfl_xor(nz, off);
+    if (head) tcol[slot + cslot] = sk;
+    const uint32_t cm = lane < ncb ? s_cm[w][lane] : 0u;
+    if (lane < ncb) tmap[slot + lane] = cm;
+    const int nz = __builtin_amdgcn_readlane(wave_incl_sum(__popc(cm)), 63);
     if (lane == 0) {
-        cnt_out[i] = __popcll(tails);
+        cnt_out[i] = ncb;
         nnz_out[i] = nz;
     }
 }
@@ -519,39 +532,6 @@ __global__ __launch_bounds__(256) void k_compact_c(int32_t count, const int32_t 
         ccol[p0 + t] = tcol[s + t];
         cmap[p0 + t] = uint16_t(tmap[s + t]);
     }
-}
-
-// Inclusive sum / max over the 64 lanes of a wave from DPP moves: row shifts
-// by 1, 2, 4, 8 within each row of 16 lanes, then the last lane of a row
-// broadcast into the next row (row_bcast:15) and lane 31 into rows 2 and 3
-// (row_bcast:31).  Lanes a move does not reach add 0 (the `old` operand).
-// One VALU operation per step, no LDS traffic.
-// Orders the LDS accesses of a wave's lanes: what one lane stored before it
-// is what another lane loads after it (the wave's LDS operations execute in
-// program order; this keeps the compiler from moving or forwarding them).
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-
-__device__ __forceinline__ int wave_incl_sum(int v) {
-    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
-    return v;
-}
-// (v >= 0)
-__device__ __forceinline__ int wave_incl_max(int v) {
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false));
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false));
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false));
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));
-    return v;
 }
 
 // ---- numeric -------------------------------------------------------------------------
@@ -962,29 +942,25 @@ __global__ __launch_bounds__(256) void k_numeric_outer(int count, const int32_t 
     __shared__ int32_t s_own[4][64];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
-    const int r = blockIdx.x * 4 + w;
+    const int r = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + w);
     if (r >= count) return;  // whole wave; the waves share no LDS
     double *acc = s_acc[w];
-    const int i = rows[r];
+    int32_t *own_of = s_own[w];
+    // the row's scalars first, all in flight together (wave-uniform: scalar loads)
+    const int i = __builtin_amdgcn_readfirstlane(rows[r]);
+    const int c0 = cptr[i], cnt = cptr[i + 1] - c0;  // cnt <= min(64, CAPB): the bin's rule
+    const int64_t so = ub_off[i];
+    const int a_beg = g.ap[i], na = g.ap[i + 1] - a_beg;
+    const int avp0 = g.avp[i];
+    const int vbase = cvp[i];
     const int bs = (lane >> 2) & 3;
     const int eA = 4 * (lane & 3) + (lane >> 4);
     const int eB = 4 * (lane >> 4) + (lane & 3);
     const uint32_t belowA = (1u << eA) - 1u, belowB = (1u << eB) - 1u;
     // C's block columns and bitmaps as the symbolic pass left them in the
-    // row's slot of the temporaries: copied to C here (cnt <= min(64, CAPB):
-    // the bin's rule)
-    const int c0 = cptr[i], cnt = cptr[i + 1] - c0;
-    const int64_t so = ub_off[i];
+    // row's slot of the temporaries, copied to C here
     const int key = lane < cnt ? tcol[so + lane] : INT_MAX;
     const uint32_t cmy = lane < cnt ? tmap[so + lane] : 0u;
-    if (lane < cnt) {
-        ccol[c0 + lane] = key;
-        cmap[c0 + lane] = uint16_t(cmy);
-    }
-    for (int e = lane; e < cnt * 16; e += 64) acc[e] = 0.0;
-    int32_t *own_of = s_own[w];
-    own_of[lane] = 0;
-    const int a_beg = g.ap[i], na = g.ap[i + 1] - a_beg;
     int ma = 0, bq0 = 0, blen = 0;
     if (lane < na) {
         const int k = g.ac[a_beg + lane];
@@ -992,16 +968,22 @@ __global__ __launch_bounds__(256) void k_numeric_outer(int count, const int32_t 
         bq0 = g.bp[k];
         blen = g.bp[k + 1] - bq0;
     }
+    if (lane < cnt) {
+        ccol[c0 + lane] = key;
+        cmap[c0 + lane] = uint16_t(cmy);
+    }
+    for (int e = lane; e < cnt * 16; e += 64) acc[e] = 0.0;
+    own_of[lane] = 0;
     const int pc = __popc(uint32_t(ma));
-    const int aoff = g.avp[i] + wave_incl_sum(pc) - pc;
+    const int aoff = avp0 + wave_incl_sum(pc) - pc;
     const int linc = wave_incl_sum(blen);
     const int pst = linc - blen;
     const int ptot = __builtin_amdgcn_readlane(linc, 63);
     // product t = lane: owner = last A block j < na with pst[j] <= t: every
     // A block with products marks its first product, a max-scan spreads it
-    wave_lds_sync();
+    wave_sync();
     if (blen > 0) own_of[pst] = lane;  // pst < ptot <= 64, distinct
-    wave_lds_sync();
+    wave_sync();
     const int t = lane;
     const int own = wave_incl_max(own_of[lane]);
     const int q = __shfl(bq0, own) + (t - __shfl(pst, own));
@@ -1023,51 +1005,71 @@ __global__ __launch_bounds__(256) void k_numeric_outer(int count, const int32_t 
     // product record: B bitmap and C slot, or -1 for a product with an empty
     // bitmap product (it adds nothing)
     const int pinfo = valid ? (pmb | (slot << 16)) : -1;
-    wave_lds_sync();  // the zeroed accumulators, before other lanes add to them
-    for (int j = 0; j < na; ++j) {  // na is wave-uniform
-        const uint32_t xa = uint32_t(__builtin_amdgcn_readlane(ma, j));
-        const int oa = __builtin_amdgcn_readlane(aoff, j);
-        const int st = __builtin_amdgcn_readlane(pst, j);
-        const int bl = __builtin_amdgcn_readlane(blen, j);
-        const double av = ((xa >> eA) & 1u) ? ldv<double>(g.av, oa + __popc(xa & belowA)) : 0.0;
-        for (int s0 = 0; s0 < bl; s0 += 8) {
-            // two steps of four products: both gathers before the MFMAs
-            double bv[2];
-            int info[2];
+    // lane's operands of step s of A block j: the A block's entry eA, and
+    // entry eB of B block s * 4 + bs of the A block's B row
+    auto gather_b = [&](int st, int bl, int s4, int &info) -> double {
+        const bool in = s4 + bs < bl;
+        const int src = in ? st + s4 + bs : 0;  // < ptot <= 64
+        const int inf = __shfl(pinfo, src);
+        const int ob = __shfl(pboff, src);
+        info = in ? inf : -1;
+        const uint32_t xb = uint32_t(info) & 0xFFFFu;
+        return (info >= 0 && ((xb >> eB) & 1u)) ? ldv<double>(g.bv, ob + __popc(xb & belowB))
+                                                 : 0.0;
+    };
+    auto add_step = [&](double av, double bv, int info) {
+        const double d = __builtin_amdgcn_mfma_f64_4x4x4f64(av, bv, 0.0, 0, 0, 0);
+        // lane (slot bs, entry eB) holds entry eB of its product's C block
+        if (info >= 0) acc[(info >> 16) * 16 + eB] += d;
+    };
+    wave_sync();  // the zeroed accumulators, before other lanes add to them
+    // four A blocks at a time: the operands of their first eight products
+    // each are gathered together, then multiplied in A-block order
+    for (int j0 = 0; j0 < na; j0 += 4) {  // na is wave-uniform
+        double av[4], bv[4][2];
+        int info[4][2], st[4], bl[4];
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                const bool in = s0 + 4 * u + bs < bl;
-                const int src = in ? st + s0 + 4 * u + bs : 0;  // < ptot <= 64
-                const int inf = __shfl(pinfo, src);
-                const int ob = __shfl(pboff, src);
-                info[u] = in ? inf : -1;
-                const uint32_t xb = uint32_t(info[u]) & 0xFFFFu;
-                bv[u] = (info[u] >= 0 && ((xb >> eB) & 1u))
-                            ? ldv<double>(g.bv, ob + __popc(xb & belowB)) : 0.0;
-            }
+        for (int u = 0; u < 4; ++u) {
+            const int j = j0 + u < na ? j0 + u : 0;
+            const uint32_t xa = uint32_t(__builtin_amdgcn_readlane(ma, j));
+            const int oa = __builtin_amdgcn_readlane(aoff, j);
+            st[u] = __builtin_amdgcn_readlane(pst, j);
+            bl[u] = j0 + u < na ? __builtin_amdgcn_readlane(blen, j) : 0;
+            av[u] = (bl[u] > 0 && ((xa >> eA) & 1u)) ? ldv<double>(g.av, oa + __popc(xa & belowA))
+                                                      : 0.0;
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                if (s0 + 4 * u < bl) {  // uniform: the MFMA runs on the whole wave
-                    const double d = __builtin_amdgcn_mfma_f64_4x4x4f64(av, bv[u], 0.0, 0, 0, 0);
-                    // lane (slot bs, entry eB) holds entry eB of its product's C block
-                    if (info[u] >= 0) acc[(info[u] >> 16) * 16 + eB] += d;
-                }
+            for (int v = 0; v < 2; ++v) bv[u][v] = gather_b(st[u], bl[u], 4 * v, info[u][v]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+#pragma unroll
+            for (int v = 0; v < 2; ++v)
+                if (4 * v < bl[u]) add_step(av[u], bv[u][v], info[u][v]);  // uniform
+            // B rows of more than eight blocks: the rest of this A block's products
+            for (int s4 = 8; s4 < bl[u]; s4 += 4) {
+                int inf;
+                const double b = gather_b(st[u], bl[u], s4, inf);
+                add_step(av[u], b, inf);
             }
-            wave_lds_sync();  // a C entry's next product may come from another lane
+            wave_sync();  // a C entry's next product may come from another lane
         }
     }
     // compress by the C bitmaps: block t's values start at the exclusive scan
-    // of the row's popcounts
+    // of the row's popcounts; four lanes per block, four bits each
     const int cpc = __popc(cmy);
     const int cexcl = wave_incl_sum(cpc) - cpc;
-    const int vbase = cvp[i];
-    for (int e0 = 0; e0 < cnt * 16; e0 += 64) {
-        const int e = e0 + lane;
-        const int tb = min(e >> 4, 63), bit = e & 15;
-        const uint32_t cm = uint32_t(__shfl(int(cmy), tb));
-        const int ex = __shfl(cexcl, tb);
-        if (e < cnt * 16 && ((cm >> bit) & 1u))
-            cval[vbase + ex + __popc(cm & ((1u << bit) - 1u))] = acc[e];
+    const int qd = lane & 3;
+    for (int t0 = 0; t0 < cnt; t0 += 16) {
+        const int tb = t0 + (lane >> 2);
+        const int tc = tb < 64 ? tb : 63;
+        const uint32_t cm = uint32_t(__shfl(int(cmy), tc));
+        const int ex = __shfl(cexcl, tc);
+        if (tb < cnt) {
+            int o = vbase + ex + __popc(cm & ((1u << (4 * qd)) - 1u));
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                if ((cm >> (4 * qd + b)) & 1u) cval[o++] = acc[tb * 16 + 4 * qd + b];
+        }
     }
 }
 
