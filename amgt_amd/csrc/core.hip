@@ -172,6 +172,33 @@ int tmp_alloc(void **p, size_t bytes, hipStream_t s) {
 
 void tmp_free(void *p, hipStream_t s) { pool_free(p, s); }
 
+int readback(const Readback *items, int n, hipStream_t s) {
+    // one pinned area per host thread, allocated on first use, kept for the process
+    thread_local unsigned char *area = nullptr;
+    if (!area) {
+        void *p = nullptr;
+        AMGT_HIP(hipHostMalloc(&p, kReadbackBytes, hipHostMallocDefault));
+        area = static_cast<unsigned char *>(p);
+    }
+    size_t off = 0;
+    for (int k = 0; k < n; ++k) {
+        const size_t b = items[k].bytes;
+        if (off + b > kReadbackBytes) {
+            set_error("readback: more than kReadbackBytes requested");
+            return AMGT_ERR_ARG;
+        }
+        AMGT_HIP(hipMemcpyAsync(area + off, items[k].src, b, hipMemcpyDeviceToHost, s));
+        off = (off + b + 15) & ~size_t(15);
+    }
+    AMGT_HIP(hipStreamSynchronize(s));
+    off = 0;
+    for (int k = 0; k < n; ++k) {
+        std::memcpy(items[k].dst, area + off, items[k].bytes);
+        off = (off + items[k].bytes + 15) & ~size_t(15);
+    }
+    return AMGT_OK;
+}
+
 template <typename T>
 __global__ void k_set_zero(T *p) {
     if (threadIdx.x == 0) p[0] = T(0);

@@ -45,6 +45,32 @@ struct Status {
         }                                         \
     } while (0)
 
+// ---- wave primitives ----------------------------------------------------------
+// Inclusive sum / max over the 64 lanes of a wave from DPP moves: row shifts
+// by 1, 2, 4, 8 within each row of 16 lanes, then the last lane of a row
+// broadcast into the next row (row_bcast:15) and lane 31 into rows 2 and 3
+// (row_bcast:31).  Lanes a move does not reach add 0 (the `old` operand).
+// One VALU operation per step, no LDS traffic.
+__device__ __forceinline__ int wave_incl_sum(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
+    return v;
+}
+// (v >= 0)
+__device__ __forceinline__ int wave_incl_max(int v) {
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));
+    return v;
+}
+
 // ---- allocation -------------------------------------------------------------
 // Long-lived arrays (handle arrays, CSR exports) go through the user hooks
 // when set, else the library's device pool, stream-ordered on s.
@@ -58,6 +84,21 @@ template <typename T>
 int tmp_alloc_t(T **p, size_t n, hipStream_t s) {
     return tmp_alloc(reinterpret_cast<void **>(p), sizeof(T) * (n ? n : 1), s);
 }
+
+// Device->host readbacks of sizes and counters, all at once: each item is
+// copied into a pinned per-thread staging area, one stream synchronisation
+// covers them, then they are copied to their destinations.  (A copy into
+// pageable memory waits for the stream and stages through a driver buffer
+// on its own: about 12 us each on MI355X, against one sync for a batch.)
+// At most kReadbackBytes in total; every caller synchronises before it
+// returns, so the area is free again.
+struct Readback {
+    void *dst;
+    const void *src;
+    size_t bytes;
+};
+constexpr size_t kReadbackBytes = 4096;
+int readback(const Readback *items, int n, hipStream_t s);
 
 // ---- handle -----------------------------------------------------------------
 struct Mbsr {

@@ -80,41 +80,45 @@ __global__ __launch_bounds__(256) void k_pass_count(int32_t n, FB fb, FS fs, int
     write_part(valid, valid ? fb(i) : 0, valid ? fs(i) : 0, t0, t1, parts);
 }
 
-// exclusive scan over the 256 threads of the workgroup; *tot = sum
-__device__ __forceinline__ int block_excl_scan(int v, int *tot) {
-    __shared__ int s_w[4];
+// One workgroup of kFinishThreads: partial counts -> list offsets per
+// workgroup and bin; totals.  Each pass scans kFinishThreads partials (DPP
+// scans within the waves, then the wave totals from LDS).
+constexpr int kFinishThreads = 1024;
+static __global__ __launch_bounds__(kFinishThreads) void k_pass_finish(int32_t ng, Part *parts,
+                                                                       Totals *tot) {
+    constexpr int NW = kFinishThreads / 64;
+    __shared__ int s_w[NW][3];
+    __shared__ int s_mx[NW];
+    __shared__ unsigned long long s_sq[NW];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int incl = v;
-    for (int d = 1; d < 64; d <<= 1) {
-        const int t = __shfl_up(incl, d);
-        if (lane >= d) incl += t;
-    }
-    __syncthreads();
-    if (lane == 63) s_w[w] = incl;
-    __syncthreads();
-    int before = 0;
-    for (int q = 0; q < w; ++q) before += s_w[q];
-    *tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
-    return before + incl - v;
-}
-
-// One workgroup: partial counts -> list offsets per workgroup and bin; totals.
-static __global__ __launch_bounds__(256) void k_pass_finish(int32_t ng, Part *parts,
-                                                            Totals *tot) {
     int run[3] = {0, 0, 0};
     int mx = 0;
     unsigned long long sq = 0ull;
-    for (int base = 0; base < ng; base += kThreads) {
+    for (int base = 0; base < ng; base += kFinishThreads) {
         const int g = base + threadIdx.x;
         Part p{};
         if (g < ng) p = parts[g];
-        int off[3];
+        int incl[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) incl[q] = wave_incl_sum(p.cnt[q]);
+        if (lane == 63) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) s_w[w][q] = incl[q];
+        }
+        __syncthreads();
+        int off[3], all[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            int t;
-            off[q] = run[q] + block_excl_scan(g < ng ? p.cnt[q] : 0, &t);
-            run[q] += t;
+            int before = 0, sum = 0;
+            for (int k = 0; k < NW; ++k) {
+                const int c = s_w[k][q];
+                before += k < w ? c : 0;
+                sum += c;
+            }
+            off[q] = run[q] + before + incl[q] - p.cnt[q];
+            all[q] = sum;
         }
+        __syncthreads();  // s_w is rewritten by the next pass
         if (g < ng) {
             parts[g].cnt[0] = off[0];
             parts[g].cnt[1] = off[1];
@@ -122,24 +126,30 @@ static __global__ __launch_bounds__(256) void k_pass_finish(int32_t ng, Part *pa
             mx = max(mx, p.mx);
             sq += p.sumsq;
         }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) run[q] += all[q];
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        mx = max(mx, __shfl_xor(mx, off));
-        sq += __shfl_xor(sq, off);
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = max(mx, __shfl_xor(mx, o));
+        sq += __shfl_xor(sq, o);
     }
-    __shared__ int s_mx[4];
-    __shared__ unsigned long long s_sq[4];
-    if ((threadIdx.x & 63) == 0) {
-        s_mx[threadIdx.x >> 6] = mx;
-        s_sq[threadIdx.x >> 6] = sq;
+    if (lane == 0) {
+        s_mx[w] = mx;
+        s_sq[w] = sq;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        int m = 0;
+        unsigned long long t = 0ull;
+        for (int k = 0; k < NW; ++k) {
+            m = max(m, s_mx[k]);
+            t += s_sq[k];
+        }
         tot->cnt[0] = run[0];
         tot->cnt[1] = run[1];
         tot->cnt[2] = run[2];
-        tot->max_blocks = max(max(s_mx[0], s_mx[1]), max(s_mx[2], s_mx[3]));
-        tot->sumsq = s_sq[0] + s_sq[1] + s_sq[2] + s_sq[3];
+        tot->max_blocks = m;
+        tot->sumsq = t;
     }
 }
 
@@ -195,7 +205,7 @@ int count(int32_t n, FB fb, FS fs, int64_t t0, int64_t t1, Part *parts, Totals *
     if (ng > 0)
         hipLaunchKernelGGL((k_pass_count<FB, FS>), dim3(unsigned(ng)), dim3(kThreads), 0, s, n, fb,
                            fs, t0, t1, parts);
-    hipLaunchKernelGGL(k_pass_finish, dim3(1), dim3(kThreads), 0, s, ng, parts, tot);
+    hipLaunchKernelGGL(k_pass_finish, dim3(1), dim3(kFinishThreads), 0, s, ng, parts, tot);
     AMGT_HIP(hipGetLastError());
     return AMGT_OK;
 }

@@ -53,31 +53,6 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// Inclusive sum / max over the 64 lanes of a wave from DPP moves: row shifts
-// by 1, 2, 4, 8 within each row of 16 lanes, then the last lane of a row
-// broadcast into the next row (row_bcast:15) and lane 31 into rows 2 and 3
-// (row_bcast:31).  Lanes a move does not reach add 0 (the `old` operand).
-// One VALU operation per step, no LDS traffic.
-__device__ __forceinline__ int wave_incl_sum(int v) {
-    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
-    return v;
-}
-// (v >= 0)
-__device__ __forceinline__ int wave_incl_max(int v) {
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false));
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false));
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false));
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));
-    return v;
-}
-
 template <int NT>
 __device__ __forceinline__ void grp_sync() {
     if constexpr (NT == 64) {
@@ -1211,6 +1186,23 @@ __global__ __launch_bounds__(256) void k_widen(int32_t n, const int32_t *__restr
     if (i < n) out[i] = in[i];
 }
 
+// value offset of every block of B: its block row's value offset plus the
+// popcounts of the row's blocks before it (one thread per block row; rows of
+// at most kVoffRowMax blocks, else k_block_popc and a scan)
+constexpr int kVoffRowMax = 256;
+__global__ __launch_bounds__(256) void k_block_voff(int32_t brow, const int32_t *__restrict__ bp,
+                                                    const int32_t *__restrict__ vp,
+                                                    const uint16_t *__restrict__ map,
+                                                    int32_t *__restrict__ off) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= brow) return;
+    int o = vp[r];
+    for (int q = bp[r]; q < bp[r + 1]; ++q) {
+        off[q] = o;
+        o += __popc(uint32_t(map[q]));
+    }
+}
+
 // popcount of every block bitmap (values are stored in block order, so the
 // exclusive scan of these is each block's value offset)
 __global__ __launch_bounds__(256) void k_block_popc(int64_t nblk, const uint16_t *__restrict__ map,
@@ -1373,16 +1365,19 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
     if (brow > 0)
         hipLaunchKernelGGL(k_upper_bound, dim3(gr), dim3(256), 0, s, brow, A.blk_ptr, A.blk_col,
                            B.blk_ptr, ub, parts);
-    hipLaunchKernelGGL(rowpass::k_pass_finish, dim3(1), dim3(256), 0, s, int32_t(gr), parts,
+    hipLaunchKernelGGL(rowpass::k_pass_finish, dim3(1), dim3(rowpass::kFinishThreads), 0, s,
+                       int32_t(gr), parts,
                        &tots[0]);
     SG_HIP(hipGetLastError());
     SG_TRY(rowpass::scatter(brow, SymbolicBin{ub, A.blk_ptr}, 0, kBin1Ub, parts, list,
                             int64_t(brow), 7u, s));
     SG_TRY(exclusive_scan_i64(ub, ub_off, brow, s));
     int64_t total_ub = 0;
-    SG_HIP(hipMemcpyAsync(&total_ub, ub_off + brow, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    SG_HIP(hipMemcpyAsync(&ht[0], &tots[0], sizeof(RowStats), hipMemcpyDeviceToHost, s));
-    SG_HIP(hipStreamSynchronize(s));  // sync 1: sizes of the temporaries and bins
+    {  // sync 1: sizes of the temporaries and bins
+        const Readback rb[2] = {{&total_ub, ub_off + brow, sizeof(int64_t)},
+                                {&ht[0], &tots[0], sizeof(RowStats)}};
+        SG_TRY(readback(rb, 2, s));
+    }
     const int32_t *bin = ht[0].cnt;
     SG_TRY(tmp_alloc_t(&tcol, size_t(total_ub), s));
     SG_TRY(tmp_alloc_t(&tmap, size_t(total_ub), s));
@@ -1421,9 +1416,9 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
     // so only a product whose bound reaches 2^31 values needs its exact totals,
     // summed in 64 bits before anything of C is allocated
     if (16 * total_ub >= (int64_t(1) << 31)) {
-        int64_t *wide = nullptr, *wscan = nullptr;
+        int64_t *wide = nullptr, *wscan = nullptr;  // wscan: one scan per count array
         SG_TRY(tmp_alloc_t(&wide, size_t(brow) + 1, s));
-        SG_TRY(tmp_alloc_t(&wscan, size_t(brow) + 1, s));
+        SG_TRY(tmp_alloc_t(&wscan, 2 * (size_t(brow) + 1), s));
         int64_t tot[2] = {0, 0};
         const int32_t *cnts[2] = {ccnt, cnnz};
         for (int w = 0; w < 2; ++w) {
@@ -1431,10 +1426,13 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
                 hipLaunchKernelGGL(k_widen, dim3((brow + 255) / 256), dim3(256), 0, s, brow,
                                    cnts[w], wide);
             SG_HIP(hipGetLastError());
-            SG_TRY(exclusive_scan_i64(wide, wscan, brow, s));
-            SG_HIP(hipMemcpyAsync(&tot[w], wscan + brow, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+            SG_TRY(exclusive_scan_i64(wide, wscan + w * (int64_t(brow) + 1), brow, s));
         }
-        SG_HIP(hipStreamSynchronize(s));
+        {
+            const Readback rb[2] = {{&tot[0], wscan + brow, sizeof(int64_t)},
+                                    {&tot[1], wscan + 2 * int64_t(brow) + 1, sizeof(int64_t)}};
+            SG_TRY(readback(rb, 2, s));
+        }
         tmp_free(wide, s);
         tmp_free(wscan, s);
         if (tot[0] >= (int64_t(1) << 31) || tot[1] >= (int64_t(1) << 31)) {
@@ -1451,10 +1449,12 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
     SG_TRY(rowpass::count(brow, nbin, rowpass::ArrayI32{ccnt}, 0, 1, parts, &tots[1], s));
     SG_TRY(rowpass::scatter(brow, nbin, 0, 1, parts, list, int64_t(brow), 7u, s));
     int32_t nblk = 0, nnz = 0;
-    SG_HIP(hipMemcpyAsync(&nblk, C.blk_ptr + brow, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    SG_HIP(hipMemcpyAsync(&nnz, C.val_ptr + brow, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    SG_HIP(hipMemcpyAsync(&ht[1], &tots[1], sizeof(RowStats), hipMemcpyDeviceToHost, s));
-    SG_HIP(hipStreamSynchronize(s));  // sync 2: sizes of C
+    {  // sync 2: sizes of C
+        const Readback rb[3] = {{&nblk, C.blk_ptr + brow, sizeof(int32_t)},
+                                {&nnz, C.val_ptr + brow, sizeof(int32_t)},
+                                {&ht[1], &tots[1], sizeof(RowStats)}};
+        SG_TRY(readback(rb, 3, s));
+    }
     const int32_t reg_rows = ht[1].cnt[0], lds_rows = ht[1].cnt[1], big_rows = ht[1].cnt[2];
     C.nblk = nblk;
     C.nnz = nnz;
@@ -1488,13 +1488,20 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
     SG_HIP(hipGetLastError());
     if (!C.pattern_only && brow > 0) {
         // value offset of every B block
-        SG_TRY(tmp_alloc_t(&bpop, size_t(B.nblk) + 1, s));
         SG_TRY(tmp_alloc_t(&boff, size_t(B.nblk) + 1, s));
-        if (B.nblk > 0)
-            hipLaunchKernelGGL(k_block_popc, dim3(unsigned((B.nblk + 255) / 256)), dim3(256), 0, s,
-                               B.nblk, B.blk_map, bpop);
+        if (B.max_row_blocks <= kVoffRowMax) {
+            if (B.brow > 0)
+                hipLaunchKernelGGL(k_block_voff, dim3(unsigned((B.brow + 255) / 256)), dim3(256),
+                                   0, s, B.brow, B.blk_ptr, B.val_ptr, B.blk_map, boff);
+        } else {
+            SG_TRY(tmp_alloc_t(&bpop, size_t(B.nblk) + 1, s));
+            if (B.nblk > 0)
+                hipLaunchKernelGGL(k_block_popc, dim3(unsigned((B.nblk + 255) / 256)), dim3(256),
+                                   0, s, B.nblk, B.blk_map, bpop);
+            SG_HIP(hipGetLastError());
+            SG_TRY(exclusive_scan_i32(bpop, boff, B.nblk, s));
+        }
         SG_HIP(hipGetLastError());
-        SG_TRY(exclusive_scan_i32(bpop, boff, B.nblk, s));
         switch (precision) {
             case AMGT_FLOAT:
                 st = launch_numeric<float>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, ub_off, tcol, tmap, &nslab, s);
