@@ -886,8 +886,10 @@ int amgt_amg_strength(int32_t n, const int32_t *A_i, const int32_t *A_j, const d
         return st;
     }
     int32_t total = 0;
-    AMGT_HIP(hipMemcpyAsync(&total, si + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    AMGT_HIP(hipStreamSynchronize(s));
+    {
+        const Readback rb[1] = {{&total, si + n, sizeof(int32_t)}};
+        AMGT_TRY(readback(rb, 1, s));
+    }
     int32_t *sjj = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(total) + 1), s));
     if (!sjj) {
         dev_free(si, s);
@@ -934,8 +936,8 @@ int amgt_amg_pmis(int32_t n, const int32_t *S_i, const int32_t *S_j, int cf_init
         hipLaunchKernelGGL(k_pmis_init, g, b, 0, s, n, S_i, cnt, measure, cf_marker, any);
     AMGT_HIP(hipGetLastError());
     int32_t h_any = 0;
-    AMGT_HIP(hipMemcpyAsync(&h_any, any, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    AMGT_HIP(hipStreamSynchronize(s));
+    const Readback rb_any[1] = {{&h_any, any, sizeof(int32_t)}};
+    AMGT_TRY(readback(rb_any, 1, s));
     // every pass decides at least the nodes of largest measure, so the loop
     // ends within n passes
     for (int iter = 0; h_any && iter <= n; ++iter) {
@@ -948,8 +950,7 @@ int amgt_amg_pmis(int32_t n, const int32_t *S_i, const int32_t *S_j, int cf_init
         AMGT_HIP(hipMemsetAsync(any, 0, sizeof(int32_t), s));
         hipLaunchKernelGGL(k_pmis_update, g, b, 0, s, n, measure, cf_marker, any);
         AMGT_HIP(hipGetLastError());
-        AMGT_HIP(hipMemcpyAsync(&h_any, any, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        AMGT_HIP(hipStreamSynchronize(s));
+        AMGT_TRY(readback(rb_any, 1, s));
     }
     if (h_any) {
         set_error("PMIS did not converge");
@@ -960,9 +961,8 @@ int amgt_amg_pmis(int32_t n, const int32_t *S_i, const int32_t *S_j, int cf_init
     hipLaunchKernelGGL(k_is_coarse, g, b, 0, s, n, cf_marker, flag);
     AMGT_HIP(hipGetLastError());
     AMGT_TRY(exclusive_scan_i32(flag, scan, n, s));
-    AMGT_HIP(hipMemcpyAsync(n_coarse, scan + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    AMGT_HIP(hipStreamSynchronize(s));
-    return AMGT_OK;
+    const Readback rb[1] = {{n_coarse, scan + n, sizeof(int32_t)}};
+    return readback(rb, 1, s);
 }
 
 int amgt_amg_extpi_interp(int32_t n, const int32_t *A_i, const int32_t *A_j,
@@ -1005,11 +1005,9 @@ int amgt_amg_extpi_interp(int32_t n, const int32_t *A_i, const int32_t *A_j,
     if (st == AMGT_OK) st = exclusive_scan_i32(flag, f2c, n, s);
     if (st == AMGT_OK) st = exclusive_scan_i64(ub, off, n, s);
     if (st == AMGT_OK) {
-        if (hipMemcpyAsync(&total_ub, off + n, sizeof(int64_t), hipMemcpyDeviceToHost, s) !=
-                hipSuccess ||
-            hipMemcpyAsync(&nc, f2c + n, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess)
-            st = AMGT_ERR_HIP;
+        const Readback rb[2] = {{&total_ub, off + n, sizeof(int64_t)},
+                                {&nc, f2c + n, sizeof(int32_t)}};
+        st = readback(rb, 2, s);
     }
     if (st == AMGT_OK) st = tmp.get(&sj, size_t(total_ub));
     if (st == AMGT_OK) st = tmp.get(&sw, size_t(total_ub));
@@ -1025,10 +1023,8 @@ int amgt_amg_extpi_interp(int32_t n, const int32_t *A_i, const int32_t *A_j,
     }
     if (st == AMGT_OK) st = exclusive_scan_i64(hs, hoff, n, s);
     if (st == AMGT_OK) {
-        if (hipMemcpyAsync(&total_h, hoff + n, sizeof(int64_t), hipMemcpyDeviceToHost, s) !=
-                hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess)
-            st = AMGT_ERR_HIP;
+        const Readback rb[1] = {{&total_h, hoff + n, sizeof(int64_t)}};
+        st = readback(rb, 1, s);
     }
     if (st == AMGT_OK) st = tmp.get(&hkv, size_t(total_h));
     if (st == AMGT_OK && total_h > 0 &&
@@ -1042,10 +1038,10 @@ int amgt_amg_extpi_interp(int32_t n, const int32_t *A_i, const int32_t *A_j,
         int32_t h_unsorted = 0;
         int32_t *unsorted = nullptr;
         int32_t lastI = 0;
-        if (hipMemcpyAsync(&lastI, A_i + n, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
-                hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess)
-            st = AMGT_ERR_HIP;
+        {
+            const Readback rb[1] = {{&lastI, A_i + n, sizeof(int32_t)}};
+            st = readback(rb, 1, s);
+        }
         nnzA = lastI;
         if (st == AMGT_OK && n <= kWaveMaxRowsSmall && nnzA >= int64_t(kWaveMinRowLen) * n) {
             st = tmp.get(&unsorted, 1);
@@ -1054,11 +1050,8 @@ int amgt_amg_extpi_interp(int32_t n, const int32_t *A_i, const int32_t *A_j,
                 st = AMGT_ERR_HIP;
             if (st == AMGT_OK) {
                 hipLaunchKernelGGL(k_rows_unsorted, g, b, 0, s, n, A_i, A_j, unsorted);
-                if (hipGetLastError() != hipSuccess ||
-                    hipMemcpyAsync(&h_unsorted, unsorted, sizeof(int32_t), hipMemcpyDeviceToHost,
-                                   s) != hipSuccess ||
-                    hipStreamSynchronize(s) != hipSuccess)
-                    st = AMGT_ERR_HIP;
+                const Readback rb[1] = {{&h_unsorted, unsorted, sizeof(int32_t)}};
+                st = hipGetLastError() != hipSuccess ? AMGT_ERR_HIP : readback(rb, 1, s);
             }
             if (st == AMGT_OK && !h_unsorted) {
                 int32_t *flag2 = nullptr;
@@ -1081,11 +1074,10 @@ int amgt_amg_extpi_interp(int32_t n, const int32_t *A_i, const int32_t *A_j,
                     hipLaunchKernelGGL(k_flag_eq, g, b, 0, s, n, use, which, flag2);
                     if (hipGetLastError() != hipSuccess) st = AMGT_ERR_HIP;
                     if (st == AMGT_OK) st = exclusive_scan_i32(flag2, upos, n, s);
-                    if (st == AMGT_OK &&
-                        (hipMemcpyAsync(&cntw, upos + n, sizeof(int32_t), hipMemcpyDeviceToHost,
-                                        s) != hipSuccess ||
-                         hipStreamSynchronize(s) != hipSuccess))
-                        st = AMGT_ERR_HIP;
+                    if (st == AMGT_OK) {
+                        const Readback rb[1] = {{&cntw, upos + n, sizeof(int32_t)}};
+                        st = readback(rb, 1, s);
+                    }
                     if (st != AMGT_OK || cntw == 0) continue;
                     int32_t *wl = nullptr;
                     st = tmp.get(&wl, size_t(cntw));
@@ -1117,10 +1109,8 @@ int amgt_amg_extpi_interp(int32_t n, const int32_t *A_i, const int32_t *A_j,
     }
     if (st == AMGT_OK) st = exclusive_scan_i32(cnt, pi, n, s);
     if (st == AMGT_OK) {
-        if (hipMemcpyAsync(&nnz, pi + n, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
-                hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess)
-            st = AMGT_ERR_HIP;
+        const Readback rb[1] = {{&nnz, pi + n, sizeof(int32_t)}};
+        st = readback(rb, 1, s);
     }
     int32_t *pj = nullptr;
     double *pa = nullptr;

@@ -344,8 +344,10 @@ struct HostHeader {
 // Bins block rows by entries: <= 64 (wave sort, not listed: it walks every
 // block row), <= 1024 and larger (listed at list + brow and list + 2*brow for
 // the workgroup sorts).  bin_count[3] flags a block row over 8192 entries.
+// hh (optional): I[0] and I[nrows], read back with the bins.
 static int bin_block_rows(int32_t nrows, int32_t brow, const int32_t *I, int32_t **list,
-                          ConvCounters *cnt, ConvCounters *h_cnt, hipStream_t s) {
+                          ConvCounters *cnt, ConvCounters *h_cnt, HostHeader *hh,
+                          hipStream_t s) {
     AMGT_TRY(tmp_alloc_t(list, size_t(brow) * 3 + 1, s));
     rowpass::Part *parts = nullptr;
     RowStats *tot = nullptr;
@@ -357,12 +359,10 @@ static int bin_block_rows(int32_t nrows, int32_t brow, const int32_t *I, int32_t
     int st = rowpass::count(brow, f, 64, 1024, parts, tot, s);
     RowStats ht{};
     if (st == AMGT_OK) {
-        hipError_t e = hipMemcpyAsync(&ht, tot, sizeof(RowStats), hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess) {
-            set_error(std::string("CSR->mBSR: ") + hipGetErrorString(e));
-            st = AMGT_ERR_HIP;
-        }
+        const Readback rb[3] = {{&ht, tot, sizeof(RowStats)},
+                                {hh ? &hh->i0 : nullptr, I, sizeof(int32_t)},
+                                {hh ? &hh->in : nullptr, I + nrows, sizeof(int32_t)}};
+        st = readback(rb, hh ? 3 : 1, s);
     }
     *h_cnt = ConvCounters{};
     h_cnt->bin_count[0] = ht.cnt[0];
@@ -396,10 +396,8 @@ int csr_to_mbsr(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *rowptr
         set_error("CSR->mBSR: ncols > 2^30 - 8 does not fit the 32-bit sort key");
         return AMGT_ERR_UNSUPPORTED;
     }
-    // verify I[0] == 0 and I[nrows] == nnz (one small D2H, the stream syncs below anyway)
+    // I[0] == 0 and I[nrows] == nnz are verified with the bins' readback below
     HostHeader hh{0, 0};
-    AMGT_HIP(hipMemcpyAsync(&hh.i0, rowptr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    AMGT_HIP(hipMemcpyAsync(&hh.in, rowptr + nrows, sizeof(int32_t), hipMemcpyDeviceToHost, s));
 
     m.blk_ptr = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(m.brow) + 1), s));
     m.val_ptr = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(m.brow) + 1), s));
@@ -418,7 +416,7 @@ int csr_to_mbsr(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *rowptr
     AMGT_TRY(tmp_alloc_t(&cnt, 1, s));
     ConvCounters hc;
     int32_t *list = nullptr;
-    int st = bin_block_rows(nrows, m.brow, rowptr, &list, cnt, &hc, s);
+    int st = bin_block_rows(nrows, m.brow, rowptr, &list, cnt, &hc, &hh, s);
     if (st != AMGT_OK) {
         tmp_free(cnt, s);
         return st;
@@ -446,13 +444,8 @@ int csr_to_mbsr(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *rowptr
     if (st == AMGT_OK) st = exclusive_scan_i32(blk_cnt, m.blk_ptr, m.brow, s);
     int32_t nblk = 0;
     if (st == AMGT_OK) {
-        hipError_t e = hipMemcpyAsync(&nblk, m.blk_ptr + m.brow, sizeof(int32_t),
-                                      hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess) {
-            set_error(std::string("CSR->mBSR: ") + hipGetErrorString(e));
-            st = AMGT_ERR_HIP;
-        }
+        const Readback rb[1] = {{&nblk, m.blk_ptr + m.brow, sizeof(int32_t)}};
+        st = readback(rb, 1, s);
     }
     if (st == AMGT_OK) {
         m.nblk = nblk;
@@ -468,15 +461,16 @@ int csr_to_mbsr(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *rowptr
                            rowptr, m.blk_ptr, tmp_col, tmp_map, m.blk_col, m.blk_map, m.val_ptr);
         st = row_stats_launch(m.blk_ptr, m.brow, rstats, s);
         hipError_t e = hipGetLastError();
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(&hc, cnt, sizeof(ConvCounters), hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(&hrs, rstats, sizeof(RowStats), hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (st == AMGT_OK && e != hipSuccess) {
             set_error(std::string("CSR->mBSR: ") + hipGetErrorString(e));
             st = AMGT_ERR_HIP;
-        } else if (st == AMGT_OK) {
+        }
+        if (st == AMGT_OK) {
+            const Readback rb[2] = {{&hc, cnt, sizeof(ConvCounters)},
+                                    {&hrs, rstats, sizeof(RowStats)}};
+            st = readback(rb, 2, s);
+        }
+        if (st == AMGT_OK) {
             m.stand = stand_from(hrs, nblk, m.brow);
             m.avg_nnz = nblk ? double(nnz) / double(nblk) : 0.0;
             m.max_row_blocks = hrs.max_blocks;
@@ -514,7 +508,7 @@ int mbsr_refresh_values(Mbsr *Ap, const int32_t *rowptr, const int32_t *colidx,
     AMGT_TRY(tmp_alloc_t(&cnt, 1, s));
     ConvCounters hc;
     int32_t *list = nullptr;
-    int st = bin_block_rows(m.nrows, m.brow, rowptr, &list, cnt, &hc, s);
+    int st = bin_block_rows(m.nrows, m.brow, rowptr, &list, cnt, &hc, nullptr, s);
     if (st == AMGT_OK && m.brow > 0)
         st = launch_sorts(m.nrows, m.ncols, rowptr, colidx, A, list, m.brow, hc.bin_count, m.val,
                           nullptr, nullptr, nullptr, cnt, 1, s);

@@ -156,7 +156,10 @@ int mbsr_transpose(const Mbsr &A, hipStream_t s, Mbsr *out) {
     AMGT_HIP(hipGetLastError());
     AMGT_TRY(row_stats_launch(T.blk_ptr, T.brow, rstats, s));
     RowStats hrs{};
-    AMGT_HIP(hipMemcpyAsync(&hrs, rstats, sizeof(RowStats), hipMemcpyDeviceToHost, s));
+    {
+        const Readback rb[1] = {{&hrs, rstats, sizeof(RowStats)}};
+        AMGT_TRY(readback(rb, 1, s));
+    }
     tmp_free(row_of, s);
     tmp_free(idx, s);
     tmp_free(perm, s);
@@ -166,7 +169,6 @@ int mbsr_transpose(const Mbsr &A, hipStream_t s, Mbsr *out) {
     tmp_free(src_voff, s);
     tmp_free(dst_voff, s);
     tmp_free(rstats, s);
-    AMGT_HIP(hipStreamSynchronize(s));
     T.max_row_blocks = hrs.max_blocks;
     T.avg_nnz = nb ? double(A.nnz) / double(nb) : 0.0;
     T.stand = stand_from(hrs, nb, T.brow);
