@@ -46,6 +46,10 @@ struct Status {
     } while (0)
 
 // ---- wave primitives ----------------------------------------------------------
+// Word 3 of a raw buffer resource on gfx9 parts (32-bit data format; no
+// swizzle): loads past num_records return zeros instead of faulting.
+constexpr int kBufferDword3 = 0x00020000;
+
 // Inclusive sum / max over the 64 lanes of a wave from DPP moves: row shifts
 // by 1, 2, 4, 8 within each row of 16 lanes, then the last lane of a row
 // broadcast into the next row (row_bcast:15) and lane 31 into rows 2 and 3

@@ -257,6 +257,7 @@ struct SpgemmArgs {
     const int32_t *bp, *bc, *bvp;
     const uint16_t *bm;
     const void *bv;   // B values, same type as av
+    int64_t bnnz;     // values of B
 };
 
 // value of type VT at index idx, widened to double
@@ -980,17 +981,32 @@ __global__ __launch_bounds__(256) void k_numeric_outer(int count, const int32_t 
     // product record: B bitmap and C slot, or -1 for a product with an empty
     // bitmap product (it adds nothing)
     const int pinfo = valid ? (pmb | (slot << 16)) : -1;
+    // Operand loads are raw buffer loads whose range check returns 0.0 for
+    // an offset past the resource, so an entry outside the bitmap (offset
+    // kNoEntry) needs no branch.  A values: the row's own, from avp0.
+    constexpr uint32_t kNoEntry = 0xFFFFFFF0u;
+    const double *av_row = static_cast<const double *>(g.av) + avp0;
+    const __amdgpu_buffer_rsrc_t a_rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<double *>(av_row), short(0), (g.avp[i + 1] - avp0) * 8, kBufferDword3);
     // lane's operands of step s of A block j: the A block's entry eA, and
-    // entry eB of B block s * 4 + bs of the A block's B row
-    auto gather_b = [&](int st, int bl, int s4, int &info) -> double {
+    // entry eB of B block s * 4 + bs of the A block's B row (bq: B's value
+    // offset the resource b_rs starts at).  The product's lane is addressed
+    // directly (ds_bpermute uses the lane bits only); a lane past the A
+    // block's products discards what it fetched.
+    auto gather_b = [&](int st, int bl, int s4, int bq, __amdgpu_buffer_rsrc_t b_rs,
+                        int &info) -> double {
         const bool in = s4 + bs < bl;
-        const int src = in ? st + s4 + bs : 0;  // < ptot <= 64
-        const int inf = __shfl(pinfo, src);
-        const int ob = __shfl(pboff, src);
+        const int addr = (st + s4 + bs) << 2;
+        const int inf = __builtin_amdgcn_ds_bpermute(addr, pinfo);
+        const int ob = __builtin_amdgcn_ds_bpermute(addr, pboff);
         info = in ? inf : -1;
         const uint32_t xb = uint32_t(info) & 0xFFFFu;
-        return (info >= 0 && ((xb >> eB) & 1u)) ? ldv<double>(g.bv, ob + __popc(xb & belowB))
-                                                 : 0.0;
+        const uint32_t idx = uint32_t(ob - bq + __popc(xb & belowB)) * 8u;  // no branch
+        const uint32_t off = ((info >= 0) & (((xb >> eB) & 1u) != 0u)) ? idx : kNoEntry;
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(b_rs, off, 0, 0);
+        double d;
+        __builtin_memcpy(&d, &v, 8);
+        return d;
     };
     auto add_step = [&](double av, double bv, int info) {
         const double d = __builtin_amdgcn_mfma_f64_4x4x4f64(av, bv, 0.0, 0, 0, 0);
@@ -1002,7 +1018,8 @@ __global__ __launch_bounds__(256) void k_numeric_outer(int count, const int32_t 
     // each are gathered together, then multiplied in A-block order
     for (int j0 = 0; j0 < na; j0 += 4) {  // na is wave-uniform
         double av[4], bv[4][2];
-        int info[4][2], st[4], bl[4];
+        int info[4][2], st[4], bl[4], bq[4];
+        __amdgpu_buffer_rsrc_t b_rs[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int j = j0 + u < na ? j0 + u : 0;
@@ -1010,10 +1027,19 @@ __global__ __launch_bounds__(256) void k_numeric_outer(int count, const int32_t 
             const int oa = __builtin_amdgcn_readlane(aoff, j);
             st[u] = __builtin_amdgcn_readlane(pst, j);
             bl[u] = j0 + u < na ? __builtin_amdgcn_readlane(blen, j) : 0;
-            av[u] = (bl[u] > 0 && ((xa >> eA) & 1u)) ? ldv<double>(g.av, oa + __popc(xa & belowA))
-                                                      : 0.0;
+            // the B row's first value: B's value offset of the block's first product
+            bq[u] = __builtin_amdgcn_readlane(pboff, st[u] < 64 ? st[u] : 63);
+            const int64_t brest = g.bnnz - bq[u];
+            b_rs[u] = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<double *>(static_cast<const double *>(g.bv) + bq[u]), short(0),
+                int(brest < 0x1FFFFFFE ? brest * 8 : 0xFFFFFFF0), kBufferDword3);
+            const uint32_t aidx = uint32_t(oa - avp0 + __popc(xa & belowA)) * 8u;
+            const uint32_t aoff_e = ((bl[u] > 0) & (((xa >> eA) & 1u) != 0u)) ? aidx : kNoEntry;
+            const auto a = __builtin_amdgcn_raw_buffer_load_b64(a_rs, aoff_e, 0, 0);
+            __builtin_memcpy(&av[u], &a, 8);
 #pragma unroll
-            for (int v = 0; v < 2; ++v) bv[u][v] = gather_b(st[u], bl[u], 4 * v, info[u][v]);
+            for (int v = 0; v < 2; ++v)
+                bv[u][v] = gather_b(st[u], bl[u], 4 * v, bq[u], b_rs[u], info[u][v]);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -1023,7 +1049,7 @@ __global__ __launch_bounds__(256) void k_numeric_outer(int count, const int32_t 
             // B rows of more than eight blocks: the rest of this A block's products
             for (int s4 = 8; s4 < bl[u]; s4 += 4) {
                 int inf;
-                const double b = gather_b(st[u], bl[u], s4, inf);
+                const double b = gather_b(st[u], bl[u], s4, bq[u], b_rs[u], inf);
                 add_step(av[u], b, inf);
             }
             wave_sync();  // a C entry's next product may come from another lane
@@ -1307,7 +1333,7 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
         return AMGT_ERR_ARG;
     }
     SpgemmArgs g{A.blk_ptr, A.blk_col, A.val_ptr, A.blk_map, avals,
-                 B.blk_ptr, B.blk_col, B.val_ptr, B.blk_map, bvals};
+                 B.blk_ptr, B.blk_col, B.val_ptr, B.blk_map, bvals, B.nnz};
 
     rowpass::Part *parts = nullptr;
     RowStats *tots = nullptr;  // [0]: upper-bound pass, [1]: C block counts

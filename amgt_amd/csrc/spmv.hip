@@ -46,10 +46,6 @@ __device__ __forceinline__ float mul_acc<__half, float>(float acc, __half a, dou
 // XCD-aware workgroup remap: hardware hands out workgroups round-robin over
 // the 8 XCDs; give each XCD a contiguous range of block rows so that x reuse
 // between neighbouring block rows stays within one L2.
-// Word 3 of a raw buffer resource on gfx9 parts (32-bit data format; no
-// swizzle): loads past num_records return zeros instead of faulting.
-constexpr int kBufferDword3 = 0x00020000;
-
 __device__ __forceinline__ int remap_wg(int wg, int nwg) {
     const int xcd = wg & 7;
     const int idx = wg >> 3;
