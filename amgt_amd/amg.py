@@ -145,6 +145,15 @@ class SetupTimes:
     coarsen_s: list = field(default_factory=list)  # strength + PMIS + interpolation per level
 
 
+def stalled(hist: list, stall) -> bool:
+    """stall = (window w, rate q): the last w iterations reduced the relative
+    residual by less than q^w (mean rate per iteration above q)."""
+    if stall is None:
+        return False
+    w, q = stall
+    return len(hist) > 2 * w and hist[-1] > hist[-1 - w] * q ** w
+
+
 class SegmentNorms:
     """Two-norms of the row segments of vectors: problem k of a batch owns the
     rows with segments == k, which are contiguous and ascending in k.
@@ -370,7 +379,7 @@ class GeometricAMG:
 
     # ---- Krylov ------------------------------------------------------------------------
     def pcg(self, b: torch.Tensor, tol: float = 1e-8, max_iter: int = 200,
-            segments: torch.Tensor | None = None):
+            segments: torch.Tensor | None = None, stall=None):
         """Preconditioned CG (hypre_PCGSolve, krylov/pcg.c:320-900: two-norm
         relative residual, as ij -tol with PCG).  With `segments` (problem index
         of every row of a batch), the test is every problem's own relative
@@ -380,9 +389,10 @@ class GeometricAMG:
         (the host reads r.r for the stopping test), the V-cycle replay z = M r,
         r.z and one launch for p = z + beta p.  r and z are the V-cycle's own
         input and output vectors (no copies).  Without a captured cycle:
-        pcg_eager."""
+        pcg_eager.  `stall` = (w, q) stops early, unconverged, when the last w
+        iterations reduced the residual by less than q^w (`stalled`)."""
         if self._graph is None:
-            return self.pcg_eager(b, tol, max_iter, segments)
+            return self.pcg_eager(b, tol, max_iter, segments, stall)
         L0 = self.levels[0]
         n = L0.n
         st = self._pcg_state()
@@ -421,7 +431,7 @@ class GeometricAMG:
             else:
                 rel = (max(sc[2].item(), 0.0) / bb) ** 0.5
             hist.append(rel)
-            if rel < tol:
+            if rel < tol or stalled(hist, stall):
                 break
             self._graph.replay()
             L.check(lib.amgt_dot(n, ptr(r), ptr(z), ptr(sc, 3), ptr(work), sp_), "amgt_dot")
@@ -442,7 +452,7 @@ class GeometricAMG:
         return self._pcg_st
 
     def pcg_eager(self, b: torch.Tensor, tol: float = 1e-8, max_iter: int = 200,
-                  segments: torch.Tensor | None = None):
+                  segments: torch.Tensor | None = None, stall=None):
         """PCG with torch vector operations around precondition() (the form
         used before capture(); the reference for the graph-replayed pcg)."""
         A = self.levels[0].A
@@ -477,7 +487,7 @@ class GeometricAMG:
             it += 1
             rel = relres()
             hist.append(rel)
-            if rel < tol:
+            if rel < tol or stalled(hist, stall):
                 break
             self.precondition(r, z)
             rz_new = torch.dot(r, z)

@@ -518,7 +518,7 @@ def policy_timings(dev, count=5, side=1024):
     """Learned per-level policy on held-out GRF diffusion problems (side^2
     cells; the first `count` of the 50 held-out problems): plain FP64 (theta
     0.25), the fixed mixed schedule and the trained network with its
-    convergence check and FP64 fallback; means per problem."""
+    in-solve check and FP64 fallback; means per problem."""
     from amgt_amd import policy as P
 
     net = P.PolicyNet.load(dev)
@@ -526,8 +526,6 @@ def policy_timings(dev, count=5, side=1024):
     for name in ("fp64", "mixed", "learned"):
         summary[name] = {k: (float(f"{v:.5g}") if isinstance(v, float) else v)
                          for k, v in summary[name].items()}
-    summary["learned"]["check_s"] = round(float(sum(r["learned"]["check_s"] for r in rows) /
-                                                len(rows)), 5)
     summary["problems"] = count
     summary["side"] = side
     summary["network"] = {"train_accuracy": round(net.meta.get("train_accuracy", 0.0), 4),

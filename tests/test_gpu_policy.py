@@ -122,25 +122,26 @@ def test_learned_setup_uses_network(dev):
 
 
 def test_fallback_to_fp64(dev):
-    """With no slack every learned hierarchy fails the convergence check: the
-    problem is solved again with the plain FP64 settings (threshold 0.25, FP64
-    on every level) and meets the tolerance.  With unlimited slack the
-    network's hierarchy is kept (here threshold 0.9 and FP16 on every level
-    but the finest)."""
+    """With no iterations allowed every learned solve fails the in-solve
+    check: the problem is solved again with the plain FP64 settings
+    (threshold 0.25, FP64 on every level) and meets the tolerance.  Without
+    the stall rule and with a large cap the network's hierarchy is kept (here
+    threshold 0.9 and FP16 on every level but the finest)."""
     net = P.PolicyNet(dev, seed=12)
     p = net.params.cpu().numpy()
     p[4992:5248] = 0.0
     p[5248:5252] = [-10.0, -10.0, 10.0, 10.0]
     net.params = torch.from_numpy(p).to(dev)
     h, (dI, dJ, dV, db) = _grf(256, 4, dev)
-    amg, x, iters, hist, t, info = P.solve_learned(net, dI, dJ, dV, 256 * 256, db, dev, slack=0.0)
-    assert info["fallback"] is not None
+    amg, x, iters, hist, t, info = P.solve_learned(net, dI, dJ, dV, 256 * 256, db, dev,
+                                                   iter_cap=0)
+    assert info["fallback"] is not None and info["learned_iters"] == 0
     assert all(th == P.THETA_RANGE[1] for th in info["thetas"])
     assert _true_res(h, x.cpu().numpy()) < 1e-8
     assert amg.level_theta == [0.25] * len(amg.levels)
     assert all(lv.A.info()["precision"] == L.AMGT_DOUBLE for lv in amg.levels)
     amg, x, iters, hist, t, info = P.solve_learned(net, dI, dJ, dV, 256 * 256, db, dev,
-                                                   slack=float("inf"))
+                                                   stall=None, iter_cap=200)
     assert info["fallback"] is None
     assert amg.level_theta == [P.THETA_RANGE[1]] * len(amg.levels)
     assert info["precisions"] == [L.AMGT_DOUBLE] + [L.AMGT_HALF] * (len(amg.levels) - 1)

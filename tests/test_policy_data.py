@@ -37,3 +37,18 @@ def test_weights_reproduce_labels():
     mae = float(np.abs(theta - T).mean())
     assert abs(acc - w["meta"]["train_accuracy"]) < 2e-3, (acc, w["meta"]["train_accuracy"])
     assert acc > 0.99 and mae < 0.05, (acc, mae)
+
+
+def test_stall_rule():
+    """The in-solve check of the learned policy: a solve stops when the last w
+    iterations reduced the residual by less than q^w, never before 2w+1
+    residuals exist, and not at all without a rule."""
+    from amgt_amd.amg import stalled
+    fast = [0.2 ** k for k in range(12)]
+    assert not any(stalled(fast[:k], (3, 0.7)) for k in range(1, 13))
+    slow = [0.9 ** k for k in range(12)]
+    assert [stalled(slow[:k], (3, 0.7)) for k in range(1, 9)] == [False] * 6 + [True, True]
+    assert not stalled(slow, None)
+    # a rate exactly at the bound is not a stall; just above it is
+    edge = [0.7 ** k for k in range(8)]
+    assert not stalled(edge, (3, 0.7 + 1e-12)) and stalled(edge, (3, 0.7 - 1e-3))
