@@ -261,17 +261,31 @@ __global__ __launch_bounds__(kThreads) void k_level_stats(int32_t n, const int32
     if (threadIdx.x < kStats) part[size_t(blockIdx.x) * kStats + threadIdx.x] = s_v[threadIdx.x][0];
 }
 
+// one workgroup: thread t combines partials t, t + kThreads, ... in order,
+// then the same fixed tree as k_level_stats (deterministic)
 __global__ __launch_bounds__(kThreads) void k_level_stats_final(int64_t nparts,
                                                                const double *__restrict__ part,
                                                                double *__restrict__ out) {
-    const int k = threadIdx.x;
-    if (k >= kStats) return;
-    double acc = k == 4 ? INFINITY : 0.0;
-    for (int64_t b = 0; b < nparts; ++b) {
-        const double x = part[b * kStats + k];
-        acc = k == 3 ? fmax(acc, x) : (k == 4 ? fmin(acc, x) : acc + x);
+    __shared__ double s_v[kStats][kThreads];
+    double v[kStats] = {0.0, 0.0, 0.0, 0.0, INFINITY, 0.0, 0.0, 0.0};
+    for (int64_t b = threadIdx.x; b < nparts; b += kThreads) {
+        for (int k = 0; k < kStats; ++k) {
+            const double x = part[b * kStats + k];
+            v[k] = k == 3 ? fmax(v[k], x) : (k == 4 ? fmin(v[k], x) : v[k] + x);
+        }
     }
-    out[k] = acc;
+    for (int k = 0; k < kStats; ++k) s_v[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) {
+            for (int k = 0; k < kStats; ++k) {
+                const double a = s_v[k][threadIdx.x], b = s_v[k][threadIdx.x + w];
+                s_v[k][threadIdx.x] = k == 3 ? fmax(a, b) : (k == 4 ? fmin(a, b) : a + b);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < kStats) out[threadIdx.x] = s_v[threadIdx.x][0];
 }
 
 int forward_into(const float *params, const float *x, int64_t n, const Work &w, float *out,
@@ -365,9 +379,8 @@ int amgt_level_stats(int32_t n, const int32_t *A_i, const int32_t *A_j, const do
     k_level_stats<<<nb, kThreads, 0, s>>>(n, A_i, A_j, A_data, part);
     k_level_stats_final<<<1, kThreads, 0, s>>>(nb, part, dev_out);
     AMGT_HIP(hipGetLastError());
-    AMGT_HIP(hipMemcpyAsync(stats, dev_out, kStats * sizeof(double), hipMemcpyDeviceToHost, s));
-    AMGT_HIP(hipStreamSynchronize(s));
-    return AMGT_OK;
+    const Readback rb[1] = {{stats, dev_out, kStats * sizeof(double)}};
+    return readback(rb, 1, s);
 }
 
 int amgt_policy_adam(float *params, const float *grad, float *m, float *v, int64_t nparams,

@@ -14,7 +14,9 @@ Workload: -div(k grad u) = f, 5-point finite volumes, k = exp(GRF)
 Labels come from the solver itself (`sweep_labels`):
 * threshold: every problem is set up and solved (capture + PCG to 1e-8) with
   each threshold of THETAS on every level, twice; the threshold with the
-  smallest measured setup + solve time is every level's target;
+  smallest measured setup + solve time is every level's target, except that
+  the default 0.25 is kept when it is within LABEL_TIE (5%) of the fastest
+  (the per-threshold times are stored with the labels);
 * precision: on that hierarchy, each level l >= 1 that has a coarser level is
   switched alone to FP16, then FP32 (the others FP64); the label is the lowest
   precision whose V-cycle contraction (`cycle_contraction`: three cycles of
@@ -50,6 +52,8 @@ PRECISIONS = (L.AMGT_DOUBLE, L.AMGT_FLOAT, L.AMGT_HALF)
 THETAS = (0.1, 0.25, 0.4, 0.55, 0.7)
 THETA_RANGE = (THETAS[0], THETAS[-1])   # predictions are clipped to the swept range
 FACTOR_SLACK = 1.10          # allowed convergence-factor ratio to FP64 (precision labels)
+DEFAULT_THETA = 0.25         # BoomerAMG's strong threshold (ij's default)
+LABEL_TIE = 0.05             # threshold labels: relative time difference taken as a tie
 STALL = (3, 0.7)             # in-solve check: window, largest mean reduction per iteration
 ITER_CAP = 40                # in-solve check: PCG iterations before the FP64 fallback
 FEATURE_NAMES = ("log10_rows_rel", "log10_nnz_rel", "nnz_per_row", "block_fill", "block_row_std",
@@ -366,6 +370,11 @@ def sweep_labels(I, J, A, b, n, device, tol=1e-8):
             best, best_t = theta, times[theta]
     if best is None:
         return None
+    # within LABEL_TIE of the fastest, BoomerAMG's default threshold wins: at
+    # the training size the differences are within the timing noise, while a
+    # lower threshold's larger strength graph costs setup time on larger grids
+    if times.get(DEFAULT_THETA, float("inf")) <= best_t * (1.0 + LABEL_TIE):
+        best = DEFAULT_THETA
     hook = _FeatureHook(lambda lvl, f: (best, L.AMGT_DOUBLE))
     amg = G.BoomerAMG(dI, dJ, dV, n, device, level_hook=hook, spgemm_policy=G.fp64_policy,
                       **AMG_ARGS)
@@ -388,12 +397,13 @@ def sweep_labels(I, J, A, b, n, device, tol=1e-8):
 
 def build_training_set(count: int, side: int, device, log=print):
     """Labels of the first `count` training problems (side^2 cells)."""
-    X, C, T = [], [], []
+    X, C, T, S = [], [], [], []
     for i in range(count):
         I, J, A, b, sigma, ell = W.grf_problem(side, W.problem_seed("train", i, side))
         r = sweep_labels(I, J, A, b, side * side, device)
         if r is None:
             continue
+        S.append([r["times"][th] for th in THETAS])
         # the coarsest level is solved directly: no precision or threshold to learn
         nl = r["levels"] - 1
         X.append(r["features"][:nl])
@@ -402,7 +412,7 @@ def build_training_set(count: int, side: int, device, log=print):
         if log and (i % 25 == 0 or i == count - 1):
             log(f"[labels] {i + 1}/{count} sigma {sigma:.2f} ell {ell:.3f} theta* {r['theta']} "
                 f"cls {r['cls'].tolist()}")
-    return np.concatenate(X), np.concatenate(C), np.concatenate(T)
+    return np.concatenate(X), np.concatenate(C), np.concatenate(T), np.array(S)
 
 
 # ---- evaluation ----------------------------------------------------------------------------
@@ -492,8 +502,9 @@ def _main(argv=None):
     dev = torch.device("cuda:0")
     if args.cmd == "labels":
         t0 = time.perf_counter()
-        X, C, T = build_training_set(args.count, args.side, dev)
+        X, C, T, S = build_training_set(args.count, args.side, dev)
         np.savez_compressed(args.out, features=X.astype(np.float32), cls=C.astype(np.int8),
+                            theta_times=S.astype(np.float32),
                             theta=T.astype(np.float32), side=args.side, count=args.count)
         print(f"[labels] {X.shape[0]} levels from {args.count} problems in "
               f"{time.perf_counter() - t0:.1f} s; class counts {np.bincount(C, minlength=3).tolist()}"

@@ -23,6 +23,18 @@ def test_labels_well_formed():
     fin = X[:, 5] == 0
     assert np.all(C[fin] == 0) and np.allclose(X[fin, :2], 0.0)
     assert int(d["count"]) == 1500 and int(d["side"]) == 256
+    # per-problem times of every threshold; the label rule picks from them
+    S = d["theta_times"].astype(np.float64)
+    assert S.shape[1] == len(P.THETAS) and S.shape[0] <= 1500 and np.all(S > 0)
+    best = S.min(axis=1)
+    dflt = S[:, P.THETAS.index(P.DEFAULT_THETA)]
+    want = np.where(dflt <= best * (1 + P.LABEL_TIE), P.DEFAULT_THETA,
+                    np.array(P.THETAS)[S.argmin(axis=1)])
+    # every level of a problem carries that problem's threshold; problems are
+    # consecutive runs of level samples that start at a finest level
+    starts = np.nonzero(X[:, 5] == 0)[0]
+    assert starts.size == S.shape[0]
+    assert np.allclose(T[starts].astype(np.float64), want, atol=1e-6)
 
 
 def test_weights_reproduce_labels():
