@@ -1407,6 +1407,24 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
     const int32_t *bin = ht[0].cnt;
     SG_TRY(tmp_alloc_t(&tcol, size_t(total_ub), s));
     SG_TRY(tmp_alloc_t(&tmap, size_t(total_ub), s));
+    if (!C.pattern_only && brow > 0) {
+        // value offset of every B block, for the numeric pass; enqueued here
+        // so that it does not follow the readback of C's sizes
+        SG_TRY(tmp_alloc_t(&boff, size_t(B.nblk) + 1, s));
+        if (B.max_row_blocks <= kVoffRowMax) {
+            if (B.brow > 0)
+                hipLaunchKernelGGL(k_block_voff, dim3(unsigned((B.brow + 255) / 256)), dim3(256),
+                                   0, s, B.brow, B.blk_ptr, B.val_ptr, B.blk_map, boff);
+        } else {
+            SG_TRY(tmp_alloc_t(&bpop, size_t(B.nblk) + 1, s));
+            if (B.nblk > 0)
+                hipLaunchKernelGGL(k_block_popc, dim3(unsigned((B.nblk + 255) / 256)), dim3(256),
+                                   0, s, B.nblk, B.blk_map, bpop);
+            SG_HIP(hipGetLastError());
+            SG_TRY(exclusive_scan_i32(bpop, boff, B.nblk, s));
+        }
+        SG_HIP(hipGetLastError());
+    }
     if (bin[0] > 0)
         hipLaunchKernelGGL(k_symbolic_reg, dim3((bin[0] + 3) / 4), dim3(256), 0, s, bin[0],
                            list, g, ub_off, tcol, tmap, ccnt, cnnz);
@@ -1513,21 +1531,6 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
                            C.blk_map);
     SG_HIP(hipGetLastError());
     if (!C.pattern_only && brow > 0) {
-        // value offset of every B block
-        SG_TRY(tmp_alloc_t(&boff, size_t(B.nblk) + 1, s));
-        if (B.max_row_blocks <= kVoffRowMax) {
-            if (B.brow > 0)
-                hipLaunchKernelGGL(k_block_voff, dim3(unsigned((B.brow + 255) / 256)), dim3(256),
-                                   0, s, B.brow, B.blk_ptr, B.val_ptr, B.blk_map, boff);
-        } else {
-            SG_TRY(tmp_alloc_t(&bpop, size_t(B.nblk) + 1, s));
-            if (B.nblk > 0)
-                hipLaunchKernelGGL(k_block_popc, dim3(unsigned((B.nblk + 255) / 256)), dim3(256),
-                                   0, s, B.nblk, B.blk_map, bpop);
-            SG_HIP(hipGetLastError());
-            SG_TRY(exclusive_scan_i32(bpop, boff, B.nblk, s));
-        }
-        SG_HIP(hipGetLastError());
         switch (precision) {
             case AMGT_FLOAT:
                 st = launch_numeric<float>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, ub_off, tcol, tmap, &nslab, s);
