@@ -832,6 +832,139 @@ __global__ __launch_bounds__(256) void k_numeric_lds(int count, const int32_t *_
     numeric_row<VT>(rows[r], g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w], s_st[w]);
 }
 
+// FP64 rows of the compressed-accumulator bins, four products per MFMA that
+// all come from one A block: each A block's products are padded to a multiple
+// of four positions, so the four block slots of an instruction hold products
+// of one B row, whose block columns are distinct, and the four results are
+// added to their C blocks' compressed entries in one step (numeric_row adds
+// the four slots one after the other, since its flattened products may share
+// a C block).  A chunk of 64 padded positions is staged at a time: lane t
+// finds its position's A block (binary search over the padded starts), the
+// product's bitmap, B value offset and C slot; then 16 MFMAs read their
+// operands' descriptions from LDS.  Every C entry sums its products in A-block
+// order: deterministic, no atomics.
+struct OuterStage {
+    int32_t a_bq0[64], a_pps[64], a_ma[64], a_aoff[64], a_blen[64];
+    int32_t p_info[64], p_boff[64], p_ma[64], p_aoff[64];
+};
+
+__device__ void numeric_row_cmp_outer(int i, const SpgemmArgs &g, const int32_t *__restrict__ boff,
+                                      const int32_t *__restrict__ cptr,
+                                      const int32_t *__restrict__ ccol,
+                                      const uint16_t *__restrict__ cmap,
+                                      const int32_t *__restrict__ cvp, double *__restrict__ cval,
+                                      int32_t *skey, double *acc, OuterStage &st, int32_t *scvo,
+                                      uint32_t *scm) {
+    const int lane = threadIdx.x & 63;
+    const int bs = (lane >> 2) & 3;
+    const int eA = 4 * (lane & 3) + (lane >> 4);
+    const int eB = 4 * (lane >> 4) + (lane & 3);
+    const uint32_t belowA = (1u << eA) - 1u, belowB = (1u << eB) - 1u;
+    const int c0 = cptr[i], cnt = cptr[i + 1] - c0;
+    for (int t = lane; t < cnt; t += 64) skey[t] = ccol[c0 + t];
+    // value offset of every C block of the row: a scan of the bitmap popcounts
+    int run = 0;
+    for (int t0 = 0; t0 < cnt; t0 += 64) {
+        const int t = t0 + lane;
+        const uint32_t m = t < cnt ? uint32_t(cmap[c0 + t]) : 0u;
+        const int pc = __popc(m);
+        const int inc = wave_incl_sum(pc);
+        if (t < cnt) {
+            scm[t] = m;
+            scvo[t] = run + inc - pc;
+        }
+        run += __builtin_amdgcn_readlane(inc, 63);
+    }
+    const int nacc = run;
+    for (int x = lane; x < nacc; x += 64) acc[x] = 0.0;
+    wave_sync();
+    int aoff_run = g.avp[i];
+    const int a_beg = g.ap[i], a_end = g.ap[i + 1];
+    for (int a0 = a_beg; a0 < a_end; a0 += 64) {
+        const int p = a0 + lane;
+        int ma = 0, bq0 = 0, blen = 0;
+        if (p < a_end) {
+            const int k = g.ac[p];
+            ma = g.am[p];
+            bq0 = g.bp[k];
+            blen = g.bp[k + 1] - bq0;
+        }
+        const int pc = __popc(uint32_t(ma));
+        const int pinc = wave_incl_sum(pc);
+        const int pad = (blen + 3) & ~3;
+        const int ppinc = wave_incl_sum(pad);
+        const int ptot = __builtin_amdgcn_readlane(ppinc, 63);
+        const int na = min(64, a_end - a0);
+        wave_sync();  // the previous chunk's staging is consumed
+        st.a_bq0[lane] = bq0;
+        st.a_pps[lane] = ppinc - pad;
+        st.a_ma[lane] = ma;
+        st.a_aoff[lane] = aoff_run + pinc - pc;
+        st.a_blen[lane] = blen;
+        aoff_run += __builtin_amdgcn_readlane(pinc, 63);
+        wave_sync();
+        for (int t0 = 0; t0 < ptot; t0 += 64) {
+            const int t = t0 + lane;
+            int info = -1, pboff = 0, pma = 0, paoff = 0;
+            if (t < ptot) {
+                // owner: the last A block of the chunk whose padded start is <= t
+                int lo = 0, hi = na - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (st.a_pps[mid] <= t) lo = mid; else hi = mid - 1;
+                }
+                const int r = t - st.a_pps[lo];
+                pma = st.a_ma[lo];
+                paoff = st.a_aoff[lo];
+                if (r < st.a_blen[lo]) {
+                    const int q = st.a_bq0[lo] + r;
+                    const uint32_t mb = g.bm[q];
+                    if (bitmap_mul(uint32_t(pma), mb)) {
+                        info = int(mb) | (lower_bound_key(skey, cnt, g.bc[q]) << 16);
+                        pboff = boff[q];
+                    }
+                }
+            }
+            wave_sync();  // the previous positions are consumed
+            st.p_info[lane] = info;
+            st.p_boff[lane] = pboff;
+            st.p_ma[lane] = pma;
+            st.p_aoff[lane] = paoff;
+            wave_sync();
+            const int ng = min(64, ptot - t0) >> 2;  // whole groups of four (padded)
+            for (int g0 = 0; g0 < ng; g0 += 4) {
+                double av[4], bv[4];
+                int inf[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int idx = 4 * (g0 + u) + bs;  // < 64
+                    inf[u] = g0 + u < ng ? st.p_info[idx] : -1;
+                    const uint32_t xa = uint32_t(st.p_ma[idx]);
+                    const uint32_t xb = uint32_t(inf[u]) & 0xFFFFu;
+                    av[u] = (inf[u] >= 0 && ((xa >> eA) & 1u))
+                                ? ldv<double>(g.av, st.p_aoff[idx] + __popc(xa & belowA)) : 0.0;
+                    bv[u] = (inf[u] >= 0 && ((xb >> eB) & 1u))
+                                ? ldv<double>(g.bv, st.p_boff[idx] + __popc(xb & belowB)) : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (g0 + u >= ng) break;  // wave-uniform: the MFMA runs with full EXEC
+                    const double d = block_product(av[u], bv[u]);
+                    if (inf[u] >= 0) {
+                        const int sl = inf[u] >> 16;
+                        const uint32_t m = scm[sl];
+                        if ((m >> eB) & 1u) acc[scvo[sl] + __popc(m & belowB)] += d;
+                    }
+                    wave_sync();  // a C entry's next product may come from another lane
+                }
+            }
+        }
+    }
+    wave_sync();
+    const int v0 = cvp[i];
+    for (int x = lane; x < nacc; x += 64) cval[v0 + x] = acc[x];
+}
+
 // rows that fit compressed accumulators of BLK blocks and VAL values: one wave
 // per row.  <kCmpBlk, kCmpVal> for the long bin (about 16 KB of LDS per wave,
 // two workgroups per CU); <kNumCap, kCmpValShort> for the bin of at most 64
@@ -849,14 +982,20 @@ __global__ __launch_bounds__(256) void k_numeric_cmp(int count, const int32_t *_
     __shared__ int32_t s_key[4][BLK];
     __shared__ int32_t s_cvo[4][BLK];
     __shared__ uint32_t s_cm[4][BLK];
-    __shared__ NumStage s_st[4];
+    using Stage = typename std::conditional<std::is_same<VT, double>::value, OuterStage,
+                                            NumStage>::type;
+    __shared__ Stage s_st[4];
     const int w = threadIdx.x >> 6;
-    const int r = blockIdx.x * 4 + w;
+    const int r = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + w);
     if (r >= count) return;  // whole wave
-    const int i = rows[r];
+    const int i = __builtin_amdgcn_readfirstlane(rows[r]);
     if (!fits_cmp<BLK, VAL>(cptr, cvp, i)) return;  // the dense kernel's row
-    numeric_row<VT, true>(i, g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w], s_st[w],
-                          s_cvo[w], s_cm[w]);
+    if constexpr (std::is_same<VT, double>::value)
+        numeric_row_cmp_outer(i, g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w],
+                              s_st[w], s_cvo[w], s_cm[w]);
+    else
+        numeric_row<VT, true>(i, g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w],
+                              s_st[w], s_cvo[w], s_cm[w]);
 }
 
 template <typename VT>
