@@ -813,27 +813,8 @@ __device__ void numeric_row(int i, const SpgemmArgs &g, const int32_t *__restric
     }
 }
 
-template <typename VT>
-__global__ __launch_bounds__(256) void k_numeric_lds(int count, const int32_t *__restrict__ rows,
-                                                     SpgemmArgs g,
-                                                     const int32_t *__restrict__ boff,
-                                                     const int32_t *__restrict__ cptr,
-                                                     const int32_t *__restrict__ ccol,
-                                                     const uint16_t *__restrict__ cmap,
-                                                     const int32_t *__restrict__ cvp,
-                                                     double *__restrict__ cval) {
-    __shared__ double s_acc[4][kNumCap * 16];
-    __shared__ int32_t s_key[4][kNumCap];
-    __shared__ NumStage s_st[4];
-    const int w = threadIdx.x >> 6;
-    const int r = blockIdx.x * 4 + w;
-    if (r >= count) return;
-    if (fits_cmp<kNumCap, kCmpValShort>(cptr, cvp, rows[r])) return;  // k_numeric_cmp's row
-    numeric_row<VT>(rows[r], g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w], s_st[w]);
-}
-
-// FP64 rows of the compressed-accumulator bins, four products per MFMA that
-// all come from one A block: each A block's products are padded to a multiple
+// FP64 rows of the LDS and global-slab bins (compressed or dense
+// accumulators), four products per MFMA that all come from one A block: each A block's products are padded to a multiple
 // of four positions, so the four block slots of an instruction hold products
 // of one B row, whose block columns are distinct, and the four results are
 // added to their C blocks' compressed entries in one step (numeric_row adds
@@ -848,34 +829,45 @@ struct OuterStage {
     int32_t p_info[64], p_boff[64], p_ma[64], p_aoff[64];
 };
 
-__device__ void numeric_row_cmp_outer(int i, const SpgemmArgs &g, const int32_t *__restrict__ boff,
-                                      const int32_t *__restrict__ cptr,
-                                      const int32_t *__restrict__ ccol,
-                                      const uint16_t *__restrict__ cmap,
-                                      const int32_t *__restrict__ cvp, double *__restrict__ cval,
-                                      int32_t *skey, double *acc, OuterStage &st, int32_t *scvo,
-                                      uint32_t *scm) {
+// kCmp: compressed accumulators (scvo / scm as in numeric_row); else dense
+// 4x4 blocks.  skey: LDS copy of the row's block columns, or NULL to search
+// them in C's column array; acc: LDS or the wave's global slab.
+template <bool kCmp>
+__device__ void numeric_row_outer(int i, const SpgemmArgs &g, const int32_t *__restrict__ boff,
+                                  const int32_t *__restrict__ cptr,
+                                  const int32_t *__restrict__ ccol,
+                                  const uint16_t *__restrict__ cmap,
+                                  const int32_t *__restrict__ cvp, double *__restrict__ cval,
+                                  int32_t *skey, double *acc, OuterStage &st,
+                                  int32_t *scvo = nullptr, uint32_t *scm = nullptr) {
     const int lane = threadIdx.x & 63;
     const int bs = (lane >> 2) & 3;
     const int eA = 4 * (lane & 3) + (lane >> 4);
     const int eB = 4 * (lane >> 4) + (lane & 3);
     const uint32_t belowA = (1u << eA) - 1u, belowB = (1u << eB) - 1u;
     const int c0 = cptr[i], cnt = cptr[i + 1] - c0;
-    for (int t = lane; t < cnt; t += 64) skey[t] = ccol[c0 + t];
-    // value offset of every C block of the row: a scan of the bitmap popcounts
-    int run = 0;
-    for (int t0 = 0; t0 < cnt; t0 += 64) {
-        const int t = t0 + lane;
-        const uint32_t m = t < cnt ? uint32_t(cmap[c0 + t]) : 0u;
-        const int pc = __popc(m);
-        const int inc = wave_incl_sum(pc);
-        if (t < cnt) {
-            scm[t] = m;
-            scvo[t] = run + inc - pc;
-        }
-        run += __builtin_amdgcn_readlane(inc, 63);
+    const int32_t *keys = ccol + c0;
+    if (skey) {
+        for (int t = lane; t < cnt; t += 64) skey[t] = keys[t];
+        keys = skey;
     }
-    const int nacc = run;
+    int nacc = cnt * 16;
+    if constexpr (kCmp) {
+        // value offset of every C block of the row: a scan of the bitmap popcounts
+        int run = 0;
+        for (int t0 = 0; t0 < cnt; t0 += 64) {
+            const int t = t0 + lane;
+            const uint32_t m = t < cnt ? uint32_t(cmap[c0 + t]) : 0u;
+            const int pc = __popc(m);
+            const int inc = wave_incl_sum(pc);
+            if (t < cnt) {
+                scm[t] = m;
+                scvo[t] = run + inc - pc;
+            }
+            run += __builtin_amdgcn_readlane(inc, 63);
+        }
+        nacc = run;
+    }
     for (int x = lane; x < nacc; x += 64) acc[x] = 0.0;
     wave_sync();
     int aoff_run = g.avp[i];
@@ -920,7 +912,7 @@ __device__ void numeric_row_cmp_outer(int i, const SpgemmArgs &g, const int32_t 
                     const int q = st.a_bq0[lo] + r;
                     const uint32_t mb = g.bm[q];
                     if (bitmap_mul(uint32_t(pma), mb)) {
-                        info = int(mb) | (lower_bound_key(skey, cnt, g.bc[q]) << 16);
+                        info = int(mb) | (lower_bound_key(keys, cnt, g.bc[q]) << 16);
                         pboff = boff[q];
                     }
                 }
@@ -952,8 +944,12 @@ __device__ void numeric_row_cmp_outer(int i, const SpgemmArgs &g, const int32_t 
                     const double d = block_product(av[u], bv[u]);
                     if (inf[u] >= 0) {
                         const int sl = inf[u] >> 16;
-                        const uint32_t m = scm[sl];
-                        if ((m >> eB) & 1u) acc[scvo[sl] + __popc(m & belowB)] += d;
+                        if constexpr (kCmp) {
+                            const uint32_t m = scm[sl];
+                            if ((m >> eB) & 1u) acc[scvo[sl] + __popc(m & belowB)] += d;
+                        } else {
+                            acc[sl * 16 + eB] += d;
+                        }
                     }
                     wave_sync();  // a C entry's next product may come from another lane
                 }
@@ -961,8 +957,51 @@ __device__ void numeric_row_cmp_outer(int i, const SpgemmArgs &g, const int32_t 
         }
     }
     wave_sync();
-    const int v0 = cvp[i];
-    for (int x = lane; x < nacc; x += 64) cval[v0 + x] = acc[x];
+    if constexpr (kCmp) {  // already in the value order of the row
+        const int v0 = cvp[i];
+        for (int x = lane; x < nacc; x += 64) cval[v0 + x] = acc[x];
+        return;
+    }
+    // compress: four C blocks per step, slot gs takes block t0 + gs
+    const int gs = lane >> 4, e = lane & 15;
+    const uint32_t below = (1u << e) - 1u;
+    int voff = cvp[i];
+    for (int t0 = 0; t0 < cnt; t0 += 4) {
+        const int t = t0 + gs;
+        const uint32_t m = (t < cnt) ? uint32_t(cmap[c0 + t]) : 0u;
+        const int pc = __popc(m);
+        const int p0c = __shfl(pc, 0), p1c = __shfl(pc, 16), p2c = __shfl(pc, 32),
+                  p3c = __shfl(pc, 48);
+        const int excl = (gs > 0 ? p0c : 0) + (gs > 1 ? p1c : 0) + (gs > 2 ? p2c : 0);
+        if ((m >> e) & 1u) cval[voff + excl + __popc(m & below)] = acc[t * 16 + e];
+        voff += p0c + p1c + p2c + p3c;
+    }
+}
+
+template <typename VT>
+__global__ __launch_bounds__(256) void k_numeric_lds(int count, const int32_t *__restrict__ rows,
+                                                     SpgemmArgs g,
+                                                     const int32_t *__restrict__ boff,
+                                                     const int32_t *__restrict__ cptr,
+                                                     const int32_t *__restrict__ ccol,
+                                                     const uint16_t *__restrict__ cmap,
+                                                     const int32_t *__restrict__ cvp,
+                                                     double *__restrict__ cval) {
+    __shared__ double s_acc[4][kNumCap * 16];
+    __shared__ int32_t s_key[4][kNumCap];
+    using Stage = typename std::conditional<std::is_same<VT, double>::value, OuterStage,
+                                            NumStage>::type;
+    __shared__ Stage s_st[4];
+    const int w = threadIdx.x >> 6;
+    const int r = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + w);
+    if (r >= count) return;
+    const int i = __builtin_amdgcn_readfirstlane(rows[r]);
+    if (fits_cmp<kNumCap, kCmpValShort>(cptr, cvp, i)) return;  // k_numeric_cmp's row
+    if constexpr (std::is_same<VT, double>::value)
+        numeric_row_outer<false>(i, g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w],
+                                 s_st[w]);
+    else
+        numeric_row<VT>(i, g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w], s_st[w]);
 }
 
 // rows that fit compressed accumulators of BLK blocks and VAL values: one wave
@@ -991,8 +1030,8 @@ __global__ __launch_bounds__(256) void k_numeric_cmp(int count, const int32_t *_
     const int i = __builtin_amdgcn_readfirstlane(rows[r]);
     if (!fits_cmp<BLK, VAL>(cptr, cvp, i)) return;  // the dense kernel's row
     if constexpr (std::is_same<VT, double>::value)
-        numeric_row_cmp_outer(i, g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w],
-                              s_st[w], s_cvo[w], s_cm[w]);
+        numeric_row_outer<true>(i, g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w],
+                                s_st[w], s_cvo[w], s_cm[w]);
     else
         numeric_row<VT, true>(i, g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w],
                               s_st[w], s_cvo[w], s_cm[w]);
@@ -1011,18 +1050,31 @@ __global__ __launch_bounds__(256) void k_numeric_global(int count, const int32_t
     // rows up to kBigCap blocks accumulate in LDS, longer ones in the wave's slab
     __shared__ double s_acc[4][kBigCap * 16];
     __shared__ int32_t s_key[4][kBigCap];
-    __shared__ NumStage s_st[4];
+    using Stage = typename std::conditional<std::is_same<VT, double>::value, OuterStage,
+                                            NumStage>::type;
+    __shared__ Stage s_st[4];
     const int w = threadIdx.x >> 6;
-    const int wave = blockIdx.x * 4 + w;
+    const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + w);
     const int nwaves = gridDim.x * 4;
     double *acc = slab + int64_t(wave) * slab_elems;
     for (int r = wave; r < count; r += nwaves) {
-        const int i = rows[r];
+        const int i = __builtin_amdgcn_readfirstlane(rows[r]);
         if (fits_cmp<kCmpBlk, kCmpVal>(cptr, cvp, i)) continue;  // k_numeric_cmp's row
-        if (cptr[i + 1] - cptr[i] <= kBigCap)
-            numeric_row<VT>(i, g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w], s_st[w]);
-        else
-            numeric_row<VT>(i, g, boff, cptr, ccol, cmap, cvp, cval, nullptr, acc, s_st[w]);
+        const bool in_lds = cptr[i + 1] - cptr[i] <= kBigCap;
+        if constexpr (std::is_same<VT, double>::value) {
+            if (in_lds)
+                numeric_row_outer<false>(i, g, boff, cptr, ccol, cmap, cvp, cval, s_key[w],
+                                         s_acc[w], s_st[w]);
+            else
+                numeric_row_outer<false>(i, g, boff, cptr, ccol, cmap, cvp, cval, nullptr, acc,
+                                         s_st[w]);
+        } else {
+            if (in_lds)
+                numeric_row<VT>(i, g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w],
+                                s_st[w]);
+            else
+                numeric_row<VT>(i, g, boff, cptr, ccol, cmap, cvp, cval, nullptr, acc, s_st[w]);
+        }
     }
 }
 
