@@ -593,14 +593,15 @@ struct NumStage {
     int32_t p_ma[64], p_aoff[64], p_mb[64], p_boff[64], p_slot[64];
 };
 
-// One wave per C block row; acc points to LDS (cnt <= kNumCap) or a global
-// slab, keys to the row's block columns (LDS copy or global).
-// The row's products (A block, B block) are flattened over the 64 lanes, so
-// the gathers of a chunk of 64 products are all in flight together; products
-// with an empty bitmap product are dropped; the rest run four at a time on
-// one v_mfma_f64_4x4x4f64 (block slot b = product 4g + b).  Four products
-// may hit the same C block, so their results are added one block slot after
-// the other (fixed order: deterministic, no atomics).  boff[q] is the value
+// FP32 / FP16 rows of the LDS and global-slab bins (FP64 rows:
+// numeric_row_outer).  One wave per C block row; acc points to LDS (cnt <=
+// kNumCap) or a global slab, keys to the row's block columns (LDS copy or
+// global).  The row's products (A block, B block) are flattened over the 64
+// lanes, so the gathers of a chunk of 64 products are all in flight together;
+// products with an empty bitmap product are dropped; the rest run 16 at a
+// time on one 16-block MFMA (block slot b16 = product g0 + b16).  Products
+// may share a C block, so their results are added one block slot after the
+// other (fixed order: deterministic, no atomics).  boff[q] is the value
 // offset of B block q (values are stored in block order).
 // kCmp: acc holds only the row's C values, in their final (block, bit) order:
 // entry e of block t at scvo[t] + popc(scm[t] below e); an entry outside the C
@@ -612,12 +613,8 @@ __device__ void numeric_row(int i, const SpgemmArgs &g, const int32_t *__restric
                             const uint16_t *__restrict__ cmap, const int32_t *__restrict__ cvp,
                             double *__restrict__ cval, int32_t *skey, double *acc,
                             NumStage &st, int32_t *scvo = nullptr, uint32_t *scm = nullptr) {
+    static_assert(!std::is_same<VT, double>::value, "FP64 rows: numeric_row_outer");
     const int lane = threadIdx.x & 63;
-    // MFMA operand roles of this lane (see block_product)
-    const int bs = (lane >> 2) & 3;                       // block slot
-    const int eA = 4 * (lane & 3) + (lane >> 4);          // A operand bit
-    const int eB = 4 * (lane >> 4) + (lane & 3);          // B operand bit = result bit
-    const uint32_t belowA = (1u << eA) - 1u, belowB = (1u << eB) - 1u;
     const uint64_t lt = (1ull << lane) - 1ull;
     const int c0 = cptr[i], cnt = cptr[i + 1] - c0;
     const int32_t *keys = ccol + c0;
@@ -712,81 +709,41 @@ __device__ void numeric_row(int i, const SpgemmArgs &g, const int32_t *__restric
                 st.p_slot[r] = slot;
             }
             wave_sync();
-            if constexpr (!std::is_same<VT, double>::value) {
-                // FP32 / FP16: 16 products per 16-block MFMA, slot b16 = lane / 4
-                // takes product g0 + b16; their results are added one slot
-                // after the other (fixed order: deterministic, no atomics)
-                const int b16 = lane >> 2, r4 = lane & 3;
-                for (int g0 = 0; g0 < np; g0 += 16) {
-                    const int idx = g0 + b16;
-                    const bool on16 = idx < np;
-                    typename Mf16<VT>::V fa = {}, fb = {};
-                    int sl = 0;
-                    if (on16) {
-                        sl = st.p_slot[idx];
-                        gather16<VT>(g, uint32_t(st.p_ma[idx]), st.p_aoff[idx],
-                                     uint32_t(st.p_mb[idx]), st.p_boff[idx], r4, fa, fb);
-                    }
-                    const mf_float4 d = mfma16<VT>(fa, fb, mf_float4{0.f, 0.f, 0.f, 0.f});
-                    // slot b2's 16 entries go to lanes 0..15 (entry e = lane, from
-                    // register e / 4 of lane 4 b2 + e % 4), which add them
-                    const int e16 = lane & 15, v16 = e16 >> 2;
-                    const int nslot = min(16, np - g0);  // wave-uniform
-                    for (int b2 = 0; b2 < nslot; ++b2) {
-                        const int from = 4 * b2 + (e16 & 3);
-                        const float t0 = __shfl(d[0], from), t1 = __shfl(d[1], from),
-                                    t2 = __shfl(d[2], from), t3 = __shfl(d[3], from);
-                        const float val = v16 == 0 ? t0 : (v16 == 1 ? t1 : (v16 == 2 ? t2 : t3));
-                        const int sl2 = __shfl(sl, 4 * b2);
-                        if (lane < 16) {
-                            if constexpr (kCmp) {
-                                const uint32_t m = scm[sl2];
-                                if ((m >> e16) & 1u)
-                                    acc[scvo[sl2] + __popc(m & ((1u << e16) - 1u))] += double(val);
-                            } else {
-                                acc[sl2 * 16 + e16] += double(val);
-                            }
-                        }
-                        wave_sync();
-                    }
-                }
-                continue;
-            }
-            // batches of four MFMA groups (16 products): all gathers first
+            // FP32 / FP16: 16 products per 16-block MFMA, slot b16 = lane / 4
+            // takes product g0 + b16; their results are added one slot
+            // after the other (fixed order: deterministic, no atomics)
+            const int b16 = lane >> 2, r4 = lane & 3;
             for (int g0 = 0; g0 < np; g0 += 16) {
-                double av[4], bv[4];
-                int sl[4];
-                bool on[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int idx = g0 + 4 * u + bs;
-                    on[u] = idx < np;
-                    av[u] = 0.0;
-                    bv[u] = 0.0;
-                    sl[u] = 0;
-                    if (on[u]) {
-                        const uint32_t xa = uint32_t(st.p_ma[idx]), xb = uint32_t(st.p_mb[idx]);
-                        sl[u] = st.p_slot[idx];
-                        if ((xa >> eA) & 1u) av[u] = ldv<VT>(g.av, st.p_aoff[idx] + __popc(xa & belowA));
-                        if ((xb >> eB) & 1u) bv[u] = ldv<VT>(g.bv, st.p_boff[idx] + __popc(xb & belowB));
-                    }
+                const int idx = g0 + b16;
+                const bool on16 = idx < np;
+                typename Mf16<VT>::V fa = {}, fb = {};
+                int sl = 0;
+                if (on16) {
+                    sl = st.p_slot[idx];
+                    gather16<VT>(g, uint32_t(st.p_ma[idx]), st.p_aoff[idx],
+                                 uint32_t(st.p_mb[idx]), st.p_boff[idx], r4, fa, fb);
                 }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (g0 + 4 * u >= np) break;  // wave-uniform: MFMA runs with full EXEC
-                    const double d = block_product(av[u], bv[u]);
-#pragma unroll
-                    for (int b2 = 0; b2 < 4; ++b2) {
-                        if (bs == b2 && on[u]) {
-                            if constexpr (kCmp) {
-                                const uint32_t m = scm[sl[u]];
-                                if ((m >> eB) & 1u) acc[scvo[sl[u]] + __popc(m & belowB)] += d;
-                            } else {
-                                acc[sl[u] * 16 + eB] += d;
-                            }
+                const mf_float4 d = mfma16<VT>(fa, fb, mf_float4{0.f, 0.f, 0.f, 0.f});
+                // slot b2's 16 entries go to lanes 0..15 (entry e = lane, from
+                // register e / 4 of lane 4 b2 + e % 4), which add them
+                const int e16 = lane & 15, v16 = e16 >> 2;
+                const int nslot = min(16, np - g0);  // wave-uniform
+                for (int b2 = 0; b2 < nslot; ++b2) {
+                    const int from = 4 * b2 + (e16 & 3);
+                    const float t0 = __shfl(d[0], from), t1 = __shfl(d[1], from),
+                                t2 = __shfl(d[2], from), t3 = __shfl(d[3], from);
+                    const float val = v16 == 0 ? t0 : (v16 == 1 ? t1 : (v16 == 2 ? t2 : t3));
+                    const int sl2 = __shfl(sl, 4 * b2);
+                    if (lane < 16) {
+                        if constexpr (kCmp) {
+                            const uint32_t m = scm[sl2];
+                            if ((m >> e16) & 1u)
+                                acc[scvo[sl2] + __popc(m & ((1u << e16) - 1u))] += double(val);
+                        } else {
+                            acc[sl2 * 16 + e16] += double(val);
                         }
-                        wave_sync();
                     }
+                    wave_sync();
                 }
             }
         }
