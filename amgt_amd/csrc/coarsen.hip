@@ -509,6 +509,8 @@ struct WaveRow {
     int32_t jinfo[WA];  // >= 0: P position; -1 skip/diag; -2 strong F (distribute);
                          // -3 add a_ij to the diagonal
     int32_t jsgn[WA];
+    int32_t jb1[WA];    // strong F neighbour: its row's range in J / A
+    int32_t je1[WA];
     int32_t sc[3];
 };
 
@@ -673,6 +675,8 @@ __global__ __launch_bounds__(64) void k_extpi_wave(
                 info = v;
             } else if (v == kStrongF) {
                 const int64_t b1 = I[i1], e1 = I[i1 + 1];
+                s.jb1[jj] = int32_t(b1);
+                s.je1[jj] = int32_t(e1);
                 const int64_t d1 = diag_pos(I, J, i1);
                 sgn = (d1 >= 0 && A[d1] < 0) ? -1 : 1;
                 double sum = 0.0;
@@ -700,23 +704,28 @@ __global__ __launch_bounds__(64) void k_extpi_wave(
         s.jdiag[jj] = dterm;
     }
     __syncthreads();
-    // weights: term of entry jj for column p, in row order
-    for (int p = lane; p < c; p += 64) {
-        const int col = s.pcol[p];
-        double acc = 0.0;
-        for (int jj = 0; jj < la; ++jj) {
-            const int info = s.jinfo[jj];
-            if (info == p) {
-                acc += s.aval[jj];
-            } else if (info == -2) {
-                const int i1 = s.acol[jj];
-                const int64_t q = find_off(J, I[i1], I[i1 + 1], i1, col);
-                if (q >= 0 && (s.jsgn[jj] * A[q]) < 0) acc += s.jdist[jj] * A[q];
+    // weights, entry by entry in row order (the host's order of accumulation
+    // for every column): a C neighbour adds a_ij to its column's weight; a
+    // strong F neighbour i1 distributes over the entries of row i1 that are P
+    // columns, a lane per entry (a row's columns are distinct, so no two lanes
+    // add to one weight)
+    for (int p = lane; p < c; p += 64) s.pw[p] = 0.0;
+    __syncthreads();
+    for (int jj = 0; jj < la; ++jj) {
+        const int info = s.jinfo[jj];  // wave-uniform
+        if (info >= 0) {
+            if (lane == 0) s.pw[info] += s.aval[jj];
+        } else if (info == -2) {
+            const double dj = s.jdist[jj];
+            const int sg = s.jsgn[jj];
+            for (int q = s.jb1[jj] + lane; q < s.je1[jj]; q += 64) {
+                const double aq = A[q];
+                const int pos = lds_find<WH>(s.hkey, s.hval, J[q]);  // -1 / kStrongF: no P column
+                if (pos >= 0 && (sg * aq) < 0) s.pw[pos] += dj * aq;
             }
         }
-        s.pw[p] = acc;
+        __syncthreads();  // the next entry's terms may go to the same weights
     }
-    __syncthreads();
     if (lane == 0) {
         double diagonal = 0.0;
         for (int jj = 0; jj < la; ++jj)
