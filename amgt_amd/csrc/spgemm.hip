@@ -41,6 +41,7 @@ constexpr int kBigCap = 256;    // ... and by the long-row kernel before its glo
 constexpr int kCmpBlk = 512;
 constexpr int kCmpVal = 1024;
 constexpr int kCmpValShort = 256;  // ... and rows of at most kNumCap blocks
+constexpr int kCmpMidBlk = 256, kCmpMidVal = 512;  // the long bin's smaller launch
 
 // a C block row of at most BLK blocks and VAL values (row pointers of C)
 template <int BLK, int VAL>
@@ -961,11 +962,13 @@ __global__ __launch_bounds__(256) void k_numeric_lds(int count, const int32_t *_
         numeric_row<VT>(i, g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w], s_st[w]);
 }
 
-// rows that fit compressed accumulators of BLK blocks and VAL values: one wave
-// per row.  <kCmpBlk, kCmpVal> for the long bin (about 16 KB of LDS per wave,
-// two workgroups per CU); <kNumCap, kCmpValShort> for the bin of at most 64
-// blocks (about 5 KB per wave, against 10 KB for dense blocks)
-template <typename VT, int BLK, int VAL>
+// rows that fit compressed accumulators of BLK blocks and VAL values (and not
+// those of a smaller launch, SBLK / SVAL): one wave per row.  For the long bin
+// <kCmpMidBlk, kCmpMidVal> (about 9 KB of LDS per wave, four workgroups per
+// CU), then <kCmpBlk, kCmpVal> (about 16 KB per wave, two workgroups per CU);
+// <kNumCap, kCmpValShort> for the bin of at most 64 blocks (about 5 KB per
+// wave, against 10 KB for dense blocks)
+template <typename VT, int BLK, int VAL, int SBLK = 0, int SVAL = 0>
 __global__ __launch_bounds__(256) void k_numeric_cmp(int count, const int32_t *__restrict__ rows,
                                                      SpgemmArgs g,
                                                      const int32_t *__restrict__ boff,
@@ -986,6 +989,8 @@ __global__ __launch_bounds__(256) void k_numeric_cmp(int count, const int32_t *_
     if (r >= count) return;  // whole wave
     const int i = __builtin_amdgcn_readfirstlane(rows[r]);
     if (!fits_cmp<BLK, VAL>(cptr, cvp, i)) return;  // the dense kernel's row
+    if constexpr (SBLK > 0)
+        if (fits_cmp<SBLK, SVAL>(cptr, cvp, i)) return;  // the smaller launch's row
     if constexpr (std::is_same<VT, double>::value)
         numeric_row_outer<true>(i, g, boff, cptr, ccol, cmap, cvp, cval, s_key[w], s_acc[w],
                                 s_st[w], s_cvo[w], s_cm[w]);
@@ -1430,9 +1435,13 @@ int launch_numeric(const SpgemmArgs &g, const int32_t *boff, const Mbsr &C, int3
                            C.val_ptr, C.val);
     }
     if (big_rows > 0) {
-        hipLaunchKernelGGL((k_numeric_cmp<VT, kCmpBlk, kCmpVal>), dim3((big_rows + 3) / 4), dim3(256), 0, s,
-                           big_rows, list + 2 * int64_t(brow), g, boff, C.blk_ptr, C.blk_col,
-                           C.blk_map, C.val_ptr, C.val);
+        hipLaunchKernelGGL((k_numeric_cmp<VT, kCmpMidBlk, kCmpMidVal>), dim3((big_rows + 3) / 4),
+                           dim3(256), 0, s, big_rows, list + 2 * int64_t(brow), g, boff,
+                           C.blk_ptr, C.blk_col, C.blk_map, C.val_ptr, C.val);
+        hipLaunchKernelGGL((k_numeric_cmp<VT, kCmpBlk, kCmpVal, kCmpMidBlk, kCmpMidVal>),
+                           dim3((big_rows + 3) / 4), dim3(256), 0, s, big_rows,
+                           list + 2 * int64_t(brow), g, boff, C.blk_ptr, C.blk_col, C.blk_map,
+                           C.val_ptr, C.val);
         const int64_t elems = int64_t(C.max_row_blocks) * 16;
         int nwaves = std::min(big_rows, 1024);
         while (nwaves > 4 && elems * nwaves * 8 > (int64_t(1) << 31)) nwaves >>= 1;
