@@ -486,8 +486,11 @@ __global__ __launch_bounds__(kThreads) void k_extpi_row(
 //  * per entry jj of row i (lane per entry): its kind, and for a strong F
 //    neighbour i1 the sum over row i1 in row order, the distribution factor
 //    and its term for the diagonal;
-//  * per P column p (lane per column): w[p] folded over jj in row order, the
-//    entry (i1, p) found by binary search in the sorted row i1;
+//  * the weights, entry by entry jj in row order: a C neighbour's a_ij to its
+//    column (one lane), a strong F neighbour's distribution over the entries
+//    of row i1 that are P columns (a lane per entry of row i1, its column
+//    looked up in the LDS table), so every weight receives its terms in the
+//    host's order;
 //  * the diagonal, the truncation sort and rescale on lane 0, in order.
 // capacities: WA entries of row i (A and S), WT distinct P columns kept in
 // LDS, WH table slots: at most WT + 64 candidate columns (the insertions stop
