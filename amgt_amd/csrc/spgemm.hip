@@ -294,9 +294,16 @@ __device__ __forceinline__ double round_out<double>(double x) {
 // every run of equal columns opens a C block, and every product ORs its
 // bitmap product into its C block's bitmap in LDS (OR does not depend on the
 // order).  No hashing, deterministic.
+//
+// rows / ub_off NULL: the bounded form, every block row 0 .. count-1 with its
+// slot of the temporaries at row * stride, where the host's bound (largest A
+// row times largest B row, at most 64) holds every row's products.  A row
+// that breaks the bound writes no structure and a block count of -1, which
+// the size pass reports as an error.
 __global__ __launch_bounds__(256) void k_symbolic_reg(int count, const int32_t *__restrict__ rows,
                                                       SpgemmArgs g,
                                                       const int64_t *__restrict__ ub_off,
+                                                      int32_t stride, int packed,
                                                       int32_t *__restrict__ tcol,
                                                       uint32_t *__restrict__ tmap,
                                                       int32_t *__restrict__ cnt_out,
@@ -307,7 +314,7 @@ __global__ __launch_bounds__(256) void k_symbolic_reg(int count, const int32_t *
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r = blockIdx.x * 4 + w;
     if (r >= count) return;  // whole wave; the waves share no LDS
-    const int i = rows[r];
+    const int i = rows ? rows[r] : r;
     const int a_beg = g.ap[i], na = g.ap[i + 1] - a_beg;
     int ma = 0, bq0 = 0, blen = 0;
     if (lane < na) {
@@ -319,6 +326,13 @@ __global__ __launch_bounds__(256) void k_symbolic_reg(int count, const int32_t *
     const int linc = wave_incl_sum(blen);
     const int pst = linc - blen;
     const int ptot = __builtin_amdgcn_readlane(linc, 63);
+    if (na > 64 || ptot > (ub_off ? 64 : stride)) {  // outside the row's bin or bound
+        if (lane == 0) {
+            cnt_out[i] = -1;
+            nnz_out[i] = 0;
+        }
+        return;
+    }
     s_sk[w][lane] = 0;
     s_cm[w][lane] = 0u;
     wave_sync();
@@ -334,15 +348,30 @@ __global__ __launch_bounds__(256) void k_symbolic_reg(int count, const int32_t *
         if (mc) key = g.bc[q];
     }
     const int nvalid = __popcll(__ballot(key != INT_MAX));
-    s_key[w][lane] = key;
-    wave_sync();
     int rank = 0;
-    for (int j = 0; j < ptot; j += 4) {
-        const int4 kk = *reinterpret_cast<const int4 *>(&s_key[w][j]);
-        rank += (kk.x < key || (kk.x == key && j < lane)) ? 1 : 0;
-        rank += (kk.y < key || (kk.y == key && j + 1 < lane)) ? 1 : 0;
-        rank += (kk.z < key || (kk.z == key && j + 2 < lane)) ? 1 : 0;
-        rank += (kk.w < key || (kk.w == key && j + 3 < lane)) ? 1 : 0;
+    if (packed) {
+        // block columns below 2^25 - 1: the key and the lane in one integer,
+        // all distinct, so the rank is one comparison per key
+        const int pk = key != INT_MAX ? key * 64 + lane : INT_MAX;
+        s_key[w][lane] = pk;
+        wave_sync();
+        for (int j = 0; j < ptot; j += 4) {
+            const int4 kk = *reinterpret_cast<const int4 *>(&s_key[w][j]);
+            rank += kk.x < pk ? 1 : 0;
+            rank += kk.y < pk ? 1 : 0;
+            rank += kk.z < pk ? 1 : 0;
+            rank += kk.w < pk ? 1 : 0;
+        }
+    } else {
+        s_key[w][lane] = key;
+        wave_sync();
+        for (int j = 0; j < ptot; j += 4) {
+            const int4 kk = *reinterpret_cast<const int4 *>(&s_key[w][j]);
+            rank += (kk.x < key || (kk.x == key && j < lane)) ? 1 : 0;
+            rank += (kk.y < key || (kk.y == key && j + 1 < lane)) ? 1 : 0;
+            rank += (kk.z < key || (kk.z == key && j + 2 < lane)) ? 1 : 0;
+            rank += (kk.w < key || (kk.w == key && j + 3 < lane)) ? 1 : 0;
+        }
     }
     // lanes at or past ptot hold INT_MAX and rank past every valid key
     if (key != INT_MAX) {
@@ -358,7 +387,7 @@ __global__ __launch_bounds__(256) void k_symbolic_reg(int count, const int32_t *
     if (in) atomicOr(&s_cm[w][cslot], s_sm[w][lane]);
     wave_sync();
     const int ncb = __popcll(heads);
-    const int64_t slot = ub_off[i];
+    const int64_t slot = ub_off ? ub_off[i] : int64_t(i) * stride;
     if (head) tcol[slot + cslot] = sk;
     const uint32_t cm = lane < ncb ? s_cm[w][lane] : 0u;
     if (lane < ncb) tmap[slot + lane] = cm;
@@ -491,9 +520,11 @@ __global__ __launch_bounds__(256) void k_upper_bound(int32_t brow, const int32_t
     rowpass::write_part(i < brow, vb, v, 0, kBin1Ub, parts);
 }
 
-// (rows: the block rows to copy, or every row 0 .. count-1 when NULL)
+// (rows: the block rows to copy, or every row 0 .. count-1 when NULL; ub_off
+// NULL: the row's slot is at row * stride)
 __global__ __launch_bounds__(256) void k_compact_c(int32_t count, const int32_t *__restrict__ rows,
                                                    const int64_t *__restrict__ ub_off,
+                                                   int32_t stride,
                                                    const int32_t *__restrict__ cptr,
                                                    const int32_t *__restrict__ tcol,
                                                    const uint32_t *__restrict__ tmap,
@@ -503,7 +534,7 @@ __global__ __launch_bounds__(256) void k_compact_c(int32_t count, const int32_t 
     const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (r >= count) return;
     const int i = rows ? rows[r] : r;
-    const int64_t s = ub_off[i];
+    const int64_t s = ub_off ? ub_off[i] : int64_t(i) * stride;
     const int p0 = cptr[i], n = cptr[i + 1] - p0;
     for (int t = lane; t < n; t += 64) {
         ccol[p0 + t] = tcol[s + t];
@@ -1061,6 +1092,7 @@ __global__ __launch_bounds__(256) void k_numeric_outer(int count, const int32_t 
                                                        const int32_t *__restrict__ boff,
                                                        const int32_t *__restrict__ cptr,
                                                        const int64_t *__restrict__ ub_off,
+                                                       int32_t stride,
                                                        const int32_t *__restrict__ tcol,
                                                        const uint32_t *__restrict__ tmap,
                                                        int32_t *__restrict__ ccol,
@@ -1076,9 +1108,9 @@ __global__ __launch_bounds__(256) void k_numeric_outer(int count, const int32_t 
     double *acc = s_acc[w];
     int32_t *own_of = s_own[w];
     // the row's scalars first, all in flight together (wave-uniform: scalar loads)
-    const int i = __builtin_amdgcn_readfirstlane(rows[r]);
+    const int i = __builtin_amdgcn_readfirstlane(rows ? rows[r] : r);
     const int c0 = cptr[i], cnt = cptr[i + 1] - c0;  // cnt <= min(64, CAPB): the bin's rule
-    const int64_t so = ub_off[i];
+    const int64_t so = ub_off ? ub_off[i] : int64_t(i) * stride;
     const int a_beg = g.ap[i], na = g.ap[i + 1] - a_beg;
     const int avp0 = g.avp[i];
     const int vbase = cvp[i];
@@ -1359,6 +1391,214 @@ struct NumericBin {
     }
 };
 
+// every row in the register bin (the bounded form of the symbolic pass)
+struct RegBin {
+    __device__ int64_t operator()(int) const { return 0; }
+};
+
+// ---- C's sizes in three kernels: every workgroup of 256 block rows sums its
+// rows' block and value counts and bins its rows (k_csize_part), one
+// workgroup scans the partials (k_csize_finish), then every workgroup writes
+// its rows' block and value offsets and numeric list entries (k_csize_write).
+// Together they replace two device scans and a row pass.
+struct CSizePart {
+    int32_t f[5];  // blocks, values, rows per numeric bin; after k_csize_finish: offsets
+    int32_t mx;    // largest block count
+    int32_t bad;   // a row that broke the register bound (block count -1)
+    int32_t pad;
+    unsigned long long sumsq;
+};
+
+struct CSizeTotals {
+    RowStats rs;  // rows per numeric bin, largest block count, sum of squares
+    int32_t nblk, nnz, bad, pad;
+};
+
+template <typename FB>
+__global__ __launch_bounds__(256) void k_csize_part(int32_t n, FB fb,
+                                                    const int32_t *__restrict__ ccnt,
+                                                    const int32_t *__restrict__ cnnz,
+                                                    CSizePart *__restrict__ parts) {
+    __shared__ int s_f[4][7];
+    __shared__ unsigned long long s_sq[4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const bool valid = i < n;
+    const int c = valid ? ccnt[i] : 0, v = valid ? cnnz[i] : 0;
+    const int b = valid ? int(fb(i)) : -1;
+    const int cs = __builtin_amdgcn_readlane(wave_incl_sum(c), 63);
+    const int vs = __builtin_amdgcn_readlane(wave_incl_sum(v), 63);
+    const int mx = __builtin_amdgcn_readlane(wave_incl_max(c), 63);
+    const bool bad = __ballot(c < 0) != 0ull;
+    int nb[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) nb[q] = __popcll(__ballot(b == q));
+    unsigned long long sq = c > 0 ? static_cast<unsigned long long>(c) * unsigned(c) : 0ull;
+    for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off);
+    if (lane == 0) {
+        s_f[w][0] = cs;
+        s_f[w][1] = vs;
+        s_f[w][2] = nb[0];
+        s_f[w][3] = nb[1];
+        s_f[w][4] = nb[2];
+        s_f[w][5] = mx;
+        s_f[w][6] = bad ? 1 : 0;
+        s_sq[w] = sq;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        CSizePart p{};
+#pragma unroll
+        for (int f = 0; f < 5; ++f) p.f[f] = s_f[0][f] + s_f[1][f] + s_f[2][f] + s_f[3][f];
+        p.mx = max(max(s_f[0][5], s_f[1][5]), max(s_f[2][5], s_f[3][5]));
+        p.bad = s_f[0][6] | s_f[1][6] | s_f[2][6] | s_f[3][6];
+        p.sumsq = s_sq[0] + s_sq[1] + s_sq[2] + s_sq[3];
+        parts[blockIdx.x] = p;
+    }
+}
+
+// one workgroup: partials -> per-workgroup offsets (five exclusive scans),
+// totals, and the last entries of C's block and value offsets
+__global__ __launch_bounds__(rowpass::kFinishThreads) void k_csize_finish(
+    int32_t ng, CSizePart *__restrict__ parts, CSizeTotals *__restrict__ tot,
+    int32_t *__restrict__ cptr_end, int32_t *__restrict__ cvp_end) {
+    constexpr int NT = rowpass::kFinishThreads, NW = NT / 64;
+    __shared__ int s_w[NW][5];
+    __shared__ int s_mx[NW], s_bad[NW];
+    __shared__ unsigned long long s_sq[NW];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int run[5] = {0, 0, 0, 0, 0};
+    int mx = 0, bad = 0;
+    unsigned long long sq = 0ull;
+    for (int base = 0; base < ng; base += NT) {
+        const int gi = base + threadIdx.x;
+        CSizePart p{};
+        if (gi < ng) p = parts[gi];
+        int incl[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) incl[q] = wave_incl_sum(p.f[q]);
+        if (lane == 63) {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) s_w[w][q] = incl[q];
+        }
+        __syncthreads();
+        int off[5], all[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            int before = 0, sum = 0;
+            for (int k = 0; k < NW; ++k) {
+                const int c = s_w[k][q];
+                before += k < w ? c : 0;
+                sum += c;
+            }
+            off[q] = run[q] + before + incl[q] - p.f[q];
+            all[q] = sum;
+        }
+        __syncthreads();  // s_w is rewritten by the next pass
+        if (gi < ng) {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) parts[gi].f[q] = off[q];
+            mx = max(mx, p.mx);
+            bad |= p.bad;
+            sq += p.sumsq;
+        }
+#pragma unroll
+        for (int q = 0; q < 5; ++q) run[q] += all[q];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = max(mx, __shfl_xor(mx, o));
+        bad |= __shfl_xor(bad, o);
+        sq += __shfl_xor(sq, o);
+    }
+    if (lane == 0) {
+        s_mx[w] = mx;
+        s_bad[w] = bad;
+        s_sq[w] = sq;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int m = 0, b = 0;
+        unsigned long long t = 0ull;
+        for (int k = 0; k < NW; ++k) {
+            m = max(m, s_mx[k]);
+            b |= s_bad[k];
+            t += s_sq[k];
+        }
+        tot->rs.cnt[0] = run[2];
+        tot->rs.cnt[1] = run[3];
+        tot->rs.cnt[2] = run[4];
+        tot->rs.max_blocks = m;
+        tot->rs.sumsq = t;
+        tot->nblk = run[0];
+        tot->nnz = run[1];
+        tot->bad = b;
+        *cptr_end = run[0];
+        *cvp_end = run[1];
+    }
+}
+
+// C's block and value offsets of every row, and list[b * stride + k] = row for
+// the k-th row (ascending) of numeric bin b
+template <typename FB>
+__global__ __launch_bounds__(256) void k_csize_write(int32_t n, FB fb,
+                                                     const int32_t *__restrict__ ccnt,
+                                                     const int32_t *__restrict__ cnnz,
+                                                     const CSizePart *__restrict__ parts,
+                                                     int32_t *__restrict__ cptr,
+                                                     int32_t *__restrict__ cvp,
+                                                     int32_t *__restrict__ list, int64_t stride) {
+    __shared__ int s_w[4][5];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const bool valid = i < n;
+    const int c = valid ? ccnt[i] : 0, v = valid ? cnnz[i] : 0;
+    const int b = valid ? int(fb(i)) : -1;
+    const int ci = wave_incl_sum(c), vi = wave_incl_sum(v);
+    const uint64_t m0 = __ballot(b == 0), m1 = __ballot(b == 1), m2 = __ballot(b == 2);
+    if (lane == 63) {
+        s_w[w][0] = ci;
+        s_w[w][1] = vi;
+    }
+    if (lane == 0) {
+        s_w[w][2] = __popcll(m0);
+        s_w[w][3] = __popcll(m1);
+        s_w[w][4] = __popcll(m2);
+    }
+    __syncthreads();
+    if (!valid) return;
+    const CSizePart &p = parts[blockIdx.x];
+    int ob = p.f[0], ov = p.f[1];
+    for (int q = 0; q < w; ++q) {
+        ob += s_w[q][0];
+        ov += s_w[q][1];
+    }
+    cptr[i] = ob + ci - c;
+    cvp[i] = ov + vi - v;
+    if (b >= 0) {
+        const uint64_t mb = b == 0 ? m0 : (b == 1 ? m1 : m2);
+        int pos = p.f[2 + b];
+        for (int q = 0; q < w; ++q) pos += s_w[q][2 + b];
+        pos += __popcll(mb & ((1ull << lane) - 1ull));
+        list[int64_t(b) * stride + pos] = i;
+    }
+}
+
+template <typename FB>
+int c_sizes(int32_t brow, FB fb, const int32_t *ccnt, const int32_t *cnnz, CSizePart *parts,
+            CSizeTotals *tot, int32_t *cptr, int32_t *cvp, int32_t *list, hipStream_t s) {
+    const int32_t ng = rowpass::groups(brow);
+    if (ng > 0)
+        hipLaunchKernelGGL((k_csize_part<FB>), dim3(unsigned(ng)), dim3(256), 0, s, brow, fb, ccnt,
+                           cnnz, parts);
+    hipLaunchKernelGGL(k_csize_finish, dim3(1), dim3(rowpass::kFinishThreads), 0, s, ng, parts,
+                       tot, cptr + brow, cvp + brow);
+    if (ng > 0)
+        hipLaunchKernelGGL((k_csize_write<FB>), dim3(unsigned(ng)), dim3(256), 0, s, brow, fb,
+                           ccnt, cnnz, parts, cptr, cvp, list, int64_t(brow));
+    AMGT_HIP(hipGetLastError());
+    return AMGT_OK;
+}
+
 __global__ __launch_bounds__(256) void k_widen(int32_t n, const int32_t *__restrict__ in,
                                                int64_t *__restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1403,23 +1643,23 @@ __global__ __launch_bounds__(256) void k_scale_values(int64_t n, double f, doubl
 template <typename VT>
 int launch_numeric(const SpgemmArgs &g, const int32_t *boff, const Mbsr &C, int32_t brow,
                    const int32_t *list, int32_t reg_rows, int32_t lds_rows, int32_t big_rows,
-                   const int64_t *ub_off, const int32_t *tcol, const uint32_t *tmap,
-                   double **nslab, hipStream_t s) {
+                   const int64_t *ub_off, int32_t stride, const int32_t *tcol,
+                   const uint32_t *tmap, double **nslab, hipStream_t s) {
     if (reg_rows > 0) {
         if constexpr (std::is_same<VT, double>::value) {
             // accumulators for the bin's longest C row (at most 64 blocks)
             const dim3 grid((reg_rows + 3) / 4);
             if (C.max_row_blocks <= 16)
                 hipLaunchKernelGGL((k_numeric_outer<16>), grid, dim3(256), 0, s, reg_rows, list,
-                                   g, boff, C.blk_ptr, ub_off, tcol, tmap, C.blk_col,
+                                   g, boff, C.blk_ptr, ub_off, stride, tcol, tmap, C.blk_col,
                                    C.blk_map, C.val_ptr, C.val);
             else if (C.max_row_blocks <= 32)
                 hipLaunchKernelGGL((k_numeric_outer<32>), grid, dim3(256), 0, s, reg_rows, list,
-                                   g, boff, C.blk_ptr, ub_off, tcol, tmap, C.blk_col,
+                                   g, boff, C.blk_ptr, ub_off, stride, tcol, tmap, C.blk_col,
                                    C.blk_map, C.val_ptr, C.val);
             else
                 hipLaunchKernelGGL((k_numeric_outer<64>), grid, dim3(256), 0, s, reg_rows, list,
-                                   g, boff, C.blk_ptr, ub_off, tcol, tmap, C.blk_col,
+                                   g, boff, C.blk_ptr, ub_off, stride, tcol, tmap, C.blk_col,
                                    C.blk_map, C.val_ptr, C.val);
         } else
             hipLaunchKernelGGL((k_numeric_reg16<VT>), dim3((reg_rows + 3) / 4), dim3(256), 0, s,
@@ -1492,9 +1732,22 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
     SpgemmArgs g{A.blk_ptr, A.blk_col, A.val_ptr, A.blk_map, avals,
                  B.blk_ptr, B.blk_col, B.val_ptr, B.blk_map, bvals, B.nnz};
 
+    // Bounded form: when the largest A row times the largest B row is at most
+    // 64 products, every block row fits the register kernels, and the
+    // symbolic temporaries take a fixed slot per row, so the upper-bound pass,
+    // its scan and its readback are not needed.
+    const bool bounded = brow > 0 && A.max_row_blocks >= 1 && A.max_row_blocks <= 64 &&
+                         B.max_row_blocks >= 1 &&
+                         int64_t(A.max_row_blocks) * B.max_row_blocks <= 64;
+    const int32_t stride = bounded ? A.max_row_blocks * B.max_row_blocks : 0;
+    // the register symbolic kernel ranks (block column, lane) pairs packed in one int
+    const int packed = B.bcol < (1 << 25) - 1 ? 1 : 0;
+
     rowpass::Part *parts = nullptr;
-    RowStats *tots = nullptr;  // [0]: upper-bound pass, [1]: C block counts
-    RowStats ht[2] = {};
+    RowStats *tots = nullptr;  // the upper-bound pass
+    RowStats ht = {};
+    CSizePart *cparts = nullptr;
+    CSizeTotals *ctot = nullptr;
     int64_t *ub = nullptr, *ub_off = nullptr;
     int32_t *list = nullptr, *ccnt = nullptr, *cnnz = nullptr;
     int32_t *tcol = nullptr;
@@ -1506,6 +1759,8 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
     auto cleanup = [&]() {
         tmp_free(parts, s);
         tmp_free(tots, s);
+        tmp_free(cparts, s);
+        tmp_free(ctot, s);
         tmp_free(ub, s);
         tmp_free(ub_off, s);
         tmp_free(list, s);
@@ -1537,31 +1792,35 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
         }                                                                         \
     } while (0)
 
-    SG_TRY(tmp_alloc_t(&parts, size_t(rowpass::groups(brow)) + 1, s));
-    SG_TRY(tmp_alloc_t(&tots, 2, s));
-    SG_TRY(tmp_alloc_t(&ub, size_t(brow) + 1, s));
-    SG_TRY(tmp_alloc_t(&ub_off, size_t(brow) + 1, s));
+    SG_TRY(tmp_alloc_t(&cparts, size_t(rowpass::groups(brow)) + 1, s));
+    SG_TRY(tmp_alloc_t(&ctot, 1, s));
     SG_TRY(tmp_alloc_t(&list, size_t(brow) * 3 + 1, s));
     SG_TRY(tmp_alloc_t(&ccnt, size_t(brow) + 1, s));
     SG_TRY(tmp_alloc_t(&cnnz, size_t(brow) + 1, s));
-    const unsigned gr = unsigned(rowpass::groups(brow));
-    if (brow > 0)
-        hipLaunchKernelGGL(k_upper_bound, dim3(gr), dim3(256), 0, s, brow, A.blk_ptr, A.blk_col,
-                           B.blk_ptr, ub, parts);
-    hipLaunchKernelGGL(rowpass::k_pass_finish, dim3(1), dim3(rowpass::kFinishThreads), 0, s,
-                       int32_t(gr), parts,
-                       &tots[0]);
-    SG_HIP(hipGetLastError());
-    SG_TRY(rowpass::scatter(brow, SymbolicBin{ub, A.blk_ptr}, 0, kBin1Ub, parts, list,
-                            int64_t(brow), 7u, s));
-    SG_TRY(exclusive_scan_i64(ub, ub_off, brow, s));
     int64_t total_ub = 0;
-    {  // sync 1: sizes of the temporaries and bins
+    const int32_t *bin = ht.cnt;
+    if (bounded) {
+        total_ub = int64_t(brow) * stride;
+    } else {
+        SG_TRY(tmp_alloc_t(&parts, size_t(rowpass::groups(brow)) + 1, s));
+        SG_TRY(tmp_alloc_t(&tots, 1, s));
+        SG_TRY(tmp_alloc_t(&ub, size_t(brow) + 1, s));
+        SG_TRY(tmp_alloc_t(&ub_off, size_t(brow) + 1, s));
+        const unsigned gr = unsigned(rowpass::groups(brow));
+        if (brow > 0)
+            hipLaunchKernelGGL(k_upper_bound, dim3(gr), dim3(256), 0, s, brow, A.blk_ptr,
+                               A.blk_col, B.blk_ptr, ub, parts);
+        hipLaunchKernelGGL(rowpass::k_pass_finish, dim3(1), dim3(rowpass::kFinishThreads), 0, s,
+                           int32_t(gr), parts, tots);
+        SG_HIP(hipGetLastError());
+        SG_TRY(rowpass::scatter(brow, SymbolicBin{ub, A.blk_ptr}, 0, kBin1Ub, parts, list,
+                                int64_t(brow), 7u, s));
+        SG_TRY(exclusive_scan_i64(ub, ub_off, brow, s));
+        // sync 1: sizes of the temporaries and bins
         const Readback rb[2] = {{&total_ub, ub_off + brow, sizeof(int64_t)},
-                                {&ht[0], &tots[0], sizeof(RowStats)}};
+                                {&ht, tots, sizeof(RowStats)}};
         SG_TRY(readback(rb, 2, s));
     }
-    const int32_t *bin = ht[0].cnt;
     SG_TRY(tmp_alloc_t(&tcol, size_t(total_ub), s));
     SG_TRY(tmp_alloc_t(&tmap, size_t(total_ub), s));
     if (!C.pattern_only && brow > 0) {
@@ -1582,18 +1841,23 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
         }
         SG_HIP(hipGetLastError());
     }
-    if (bin[0] > 0)
+    if (bounded)
+        hipLaunchKernelGGL(k_symbolic_reg, dim3((brow + 3) / 4), dim3(256), 0, s, brow,
+                           static_cast<const int32_t *>(nullptr), g,
+                           static_cast<const int64_t *>(nullptr), stride, packed, tcol, tmap,
+                           ccnt, cnnz);
+    if (!bounded && bin[0] > 0)
         hipLaunchKernelGGL(k_symbolic_reg, dim3((bin[0] + 3) / 4), dim3(256), 0, s, bin[0],
-                           list, g, ub_off, tcol, tmap, ccnt, cnnz);
-    if (bin[1] > 0) {
+                           list, g, ub_off, 0, packed, tcol, tmap, ccnt, cnnz);
+    if (!bounded && bin[1] > 0) {
         hipLaunchKernelGGL(k_symbolic_wave, dim3((bin[1] + 3) / 4), dim3(256), 0, s, bin[1],
                            list + brow, g, ub_off, tcol, tmap, ccnt, cnnz);
         hipLaunchKernelGGL(k_symbolic_block, dim3(bin[1]), dim3(256), 0, s, bin[1],
                            list + brow, g, ub_off, tcol, tmap, ccnt, cnnz);
     }
-    if (bin[2] > 0) {
+    if (!bounded && bin[2] > 0) {
         int hs = 1;
-        const int64_t want = std::min<int64_t>(2 * int64_t(ht[0].max_blocks), 2 * int64_t(B.bcol) + 2);
+        const int64_t want = std::min<int64_t>(2 * int64_t(ht.max_blocks), 2 * int64_t(B.bcol) + 2);
         while (hs < want) hs <<= 1;
         const int64_t per_wg = 2 * int64_t(hs);
         int nwg = std::min(bin[2], 512);
@@ -1642,26 +1906,31 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
             return AMGT_ERR_UNSUPPORTED;
         }
     }
-    SG_TRY(exclusive_scan_i32(ccnt, C.blk_ptr, brow, s));
-    SG_TRY(exclusive_scan_i32(cnnz, C.val_ptr, brow, s));
-    // C block counts: rows over kNumCap go to the global-memory numeric kernel;
-    // largest count and sum of squares give max_row_blocks and stand
-    const NumericBin nbin{ub, A.blk_ptr, ccnt};
-    SG_TRY(rowpass::count(brow, nbin, rowpass::ArrayI32{ccnt}, 0, 1, parts, &tots[1], s));
-    SG_TRY(rowpass::scatter(brow, nbin, 0, 1, parts, list, int64_t(brow), 7u, s));
-    int32_t nblk = 0, nnz = 0;
+    // C's block and value offsets, the numeric bins (rows over kNumCap blocks go
+    // to the global-memory numeric kernel), the largest block count and the
+    // sum of squares (max_row_blocks and stand)
+    if (bounded)
+        SG_TRY(c_sizes(brow, RegBin{}, ccnt, cnnz, cparts, ctot, C.blk_ptr, C.val_ptr, list, s));
+    else
+        SG_TRY(c_sizes(brow, NumericBin{ub, A.blk_ptr, ccnt}, ccnt, cnnz, cparts, ctot, C.blk_ptr,
+                       C.val_ptr, list, s));
+    CSizeTotals ct{};
     {  // sync 2: sizes of C
-        const Readback rb[3] = {{&nblk, C.blk_ptr + brow, sizeof(int32_t)},
-                                {&nnz, C.val_ptr + brow, sizeof(int32_t)},
-                                {&ht[1], &tots[1], sizeof(RowStats)}};
-        SG_TRY(readback(rb, 3, s));
+        const Readback rb[1] = {{&ct, ctot, sizeof(CSizeTotals)}};
+        SG_TRY(readback(rb, 1, s));
     }
-    const int32_t reg_rows = ht[1].cnt[0], lds_rows = ht[1].cnt[1], big_rows = ht[1].cnt[2];
+    if (ct.bad) {
+        set_error("SpGEMM: a block row exceeds the bound of its operands' row lengths");
+        cleanup();
+        return AMGT_ERR_INPUT;
+    }
+    const int32_t nblk = ct.nblk, nnz = ct.nnz;
+    const int32_t reg_rows = ct.rs.cnt[0], lds_rows = ct.rs.cnt[1], big_rows = ct.rs.cnt[2];
     C.nblk = nblk;
     C.nnz = nnz;
-    C.max_row_blocks = ht[1].max_blocks;
+    C.max_row_blocks = ct.rs.max_blocks;
     C.avg_nnz = nblk ? double(nnz) / double(nblk) : 0.0;
-    C.stand = stand_from(ht[1], nblk, brow);
+    C.stand = stand_from(ct.rs, nblk, brow);
     C.spmv_kernel = default_spmv_kernel(C);
     C.blk_col = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * size_t(nblk ? nblk : 1), s));
     C.blk_map = static_cast<uint16_t *>(dev_alloc(sizeof(uint16_t) * size_t(nblk ? nblk : 1), s));
@@ -1677,30 +1946,31 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
     const bool fp64_numeric = !C.pattern_only && precision != AMGT_FLOAT && precision != AMGT_HALF;
     if (brow > 0 && !fp64_numeric)
         hipLaunchKernelGGL(k_compact_c, dim3((brow + 3) / 4), dim3(256), 0, s, brow,
-                           static_cast<const int32_t *>(nullptr), ub_off, C.blk_ptr, tcol, tmap,
-                           C.blk_col, C.blk_map);
+                           static_cast<const int32_t *>(nullptr), ub_off, stride, C.blk_ptr, tcol,
+                           tmap, C.blk_col, C.blk_map);
     if (fp64_numeric && lds_rows > 0)
         hipLaunchKernelGGL(k_compact_c, dim3((lds_rows + 3) / 4), dim3(256), 0, s, lds_rows,
-                           list + brow, ub_off, C.blk_ptr, tcol, tmap, C.blk_col, C.blk_map);
+                           list + brow, ub_off, stride, C.blk_ptr, tcol, tmap, C.blk_col,
+                           C.blk_map);
     if (fp64_numeric && big_rows > 0)
         hipLaunchKernelGGL(k_compact_c, dim3((big_rows + 3) / 4), dim3(256), 0, s, big_rows,
-                           list + 2 * int64_t(brow), ub_off, C.blk_ptr, tcol, tmap, C.blk_col,
-                           C.blk_map);
+                           list + 2 * int64_t(brow), ub_off, stride, C.blk_ptr, tcol, tmap,
+                           C.blk_col, C.blk_map);
     SG_HIP(hipGetLastError());
     if (!C.pattern_only && brow > 0) {
         switch (precision) {
             case AMGT_FLOAT:
-                st = launch_numeric<float>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, ub_off, tcol, tmap, &nslab, s);
+                st = launch_numeric<float>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, ub_off, stride, tcol, tmap, &nslab, s);
                 break;
             case AMGT_HALF:
-                st = launch_numeric<__half>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, ub_off, tcol, tmap, &nslab, s);
+                st = launch_numeric<__half>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, ub_off, stride, tcol, tmap, &nslab, s);
                 // the FP16 copies hold A and B times half_scale: undo both (powers of two, exact)
                 if (st == AMGT_OK && C.nnz > 0)
                     hipLaunchKernelGGL(k_scale_values, dim3(unsigned((C.nnz + 255) / 256)), dim3(256),
                                        0, s, C.nnz, 1.0 / (A.half_scale * B.half_scale), C.val);
                 break;
             default:
-                st = launch_numeric<double>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, ub_off, tcol, tmap, &nslab, s);
+                st = launch_numeric<double>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, ub_off, stride, tcol, tmap, &nslab, s);
         }
         if (st != AMGT_OK) {
             cleanup();
