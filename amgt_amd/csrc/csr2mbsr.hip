@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void k_sort_wave(
     int32_t nrows, int32_t ncols, const int32_t *__restrict__ I, const int32_t *__restrict__ J,
     const double *__restrict__ A, int32_t brow, double *__restrict__ val,
     int32_t *__restrict__ tmp_col, uint32_t *__restrict__ tmp_map,
-    int32_t *__restrict__ blk_cnt, ConvCounters *cnt, int values_only) {
+    int32_t *__restrict__ blk_cnt, ConvCounters *cnt, int values_only, int packed) {
     __shared__ __align__(16) uint32_t s_key[4][64];
     __shared__ uint32_t s_sorted[4][64];
     const int lane = threadIdx.x & 63;
@@ -105,19 +105,34 @@ __global__ __launch_bounds__(256) void k_sort_wave(
     }
     // Rank sort: every lane counts the keys below its own over an LDS copy of
     // the row's keys (four per read; ties, i.e. duplicate entries, by lane), so
-    // the work is about three VALU operations per key instead of a 21-stage
-    // shuffle network (this kernel is bound by VALU issue).
+    // the work is a few VALU operations per key instead of a 21-stage shuffle
+    // network (this kernel is bound by VALU issue).  With fewer than 2^24 - 4
+    // columns the key and the lane fit one 32-bit integer, all distinct, and
+    // the rank is one comparison per key.
     const int w = threadIdx.x >> 6;
-    s_key[w][lane] = key;
-    wave_sync_conv();
     const bool valid = lane < n;
     int rank = 0;
-    for (int j = 0; j < n; j += 4) {
-        const uint4 kk = *reinterpret_cast<const uint4 *>(&s_key[w][j]);
-        rank += (kk.x < key || (kk.x == key && j < lane)) ? 1 : 0;
-        rank += (kk.y < key || (kk.y == key && j + 1 < lane)) ? 1 : 0;
-        rank += (kk.z < key || (kk.z == key && j + 2 < lane)) ? 1 : 0;
-        rank += (kk.w < key || (kk.w == key && j + 3 < lane)) ? 1 : 0;
+    if (packed) {
+        const uint32_t pk = valid ? (key << 6) | uint32_t(lane) : kPadKey;
+        s_key[w][lane] = pk;
+        wave_sync_conv();
+        for (int j = 0; j < n; j += 4) {
+            const uint4 kk = *reinterpret_cast<const uint4 *>(&s_key[w][j]);
+            rank += kk.x < pk ? 1 : 0;
+            rank += kk.y < pk ? 1 : 0;
+            rank += kk.z < pk ? 1 : 0;
+            rank += kk.w < pk ? 1 : 0;
+        }
+    } else {
+        s_key[w][lane] = key;
+        wave_sync_conv();
+        for (int j = 0; j < n; j += 4) {
+            const uint4 kk = *reinterpret_cast<const uint4 *>(&s_key[w][j]);
+            rank += (kk.x < key || (kk.x == key && j < lane)) ? 1 : 0;
+            rank += (kk.y < key || (kk.y == key && j + 1 < lane)) ? 1 : 0;
+            rank += (kk.z < key || (kk.z == key && j + 2 < lane)) ? 1 : 0;
+            rank += (kk.w < key || (kk.w == key && j + 3 < lane)) ? 1 : 0;
+        }
     }
     if (valid) {
         s_sorted[w][rank] = key;
@@ -316,8 +331,10 @@ int launch_sorts(int32_t nrows, int32_t ncols, const int32_t *I, const int32_t *
     }
     if (brow > 0) {
         int grid = (brow + 3) / 4;
+        // columns <= 2^24 - 5: keys below 2^26 - 16, packed keys below kPadKey
+        const int packed = ncols < (1 << 24) - 4 ? 1 : 0;
         hipLaunchKernelGGL(k_sort_wave, dim3(grid), dim3(256), 0, s, nrows, ncols, I, J, A, brow,
-                           val, tmp_col, tmp_map, blk_cnt, cnt, values_only);
+                           val, tmp_col, tmp_map, blk_cnt, cnt, values_only, packed);
         AMGT_HIP(hipGetLastError());
     }
     if (bin_count[1] > 0) {
