@@ -1405,13 +1405,14 @@ struct CSizePart {
     int32_t f[5];  // blocks, values, rows per numeric bin; after k_csize_finish: offsets
     int32_t mx;    // largest block count
     int32_t bad;   // a row that broke the register bound (block count -1)
-    int32_t pad;
+    int32_t big[3];  // largest values of a bin-1 row, blocks and values of a bin-2 row
     unsigned long long sumsq;
 };
 
 struct CSizeTotals {
     RowStats rs;  // rows per numeric bin, largest block count, sum of squares
-    int32_t nblk, nnz, bad, pad;
+    int32_t nblk, nnz, bad;
+    int32_t big[3];  // as CSizePart: which numeric launches have rows
 };
 
 template <typename FB>
@@ -1419,7 +1420,7 @@ __global__ __launch_bounds__(256) void k_csize_part(int32_t n, FB fb,
                                                     const int32_t *__restrict__ ccnt,
                                                     const int32_t *__restrict__ cnnz,
                                                     CSizePart *__restrict__ parts) {
-    __shared__ int s_f[4][7];
+    __shared__ int s_f[4][10];
     __shared__ unsigned long long s_sq[4];
     const int i = blockIdx.x * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1429,6 +1430,9 @@ __global__ __launch_bounds__(256) void k_csize_part(int32_t n, FB fb,
     const int cs = __builtin_amdgcn_readlane(wave_incl_sum(c), 63);
     const int vs = __builtin_amdgcn_readlane(wave_incl_sum(v), 63);
     const int mx = __builtin_amdgcn_readlane(wave_incl_max(c), 63);
+    const int v1 = __builtin_amdgcn_readlane(wave_incl_max(b == 1 ? v : 0), 63);
+    const int b2 = __builtin_amdgcn_readlane(wave_incl_max(b == 2 ? c : 0), 63);
+    const int v2 = __builtin_amdgcn_readlane(wave_incl_max(b == 2 ? v : 0), 63);
     const bool bad = __ballot(c < 0) != 0ull;
     int nb[3];
 #pragma unroll
@@ -1443,6 +1447,9 @@ __global__ __launch_bounds__(256) void k_csize_part(int32_t n, FB fb,
         s_f[w][4] = nb[2];
         s_f[w][5] = mx;
         s_f[w][6] = bad ? 1 : 0;
+        s_f[w][7] = v1;
+        s_f[w][8] = b2;
+        s_f[w][9] = v2;
         s_sq[w] = sq;
     }
     __syncthreads();
@@ -1452,6 +1459,9 @@ __global__ __launch_bounds__(256) void k_csize_part(int32_t n, FB fb,
         for (int f = 0; f < 5; ++f) p.f[f] = s_f[0][f] + s_f[1][f] + s_f[2][f] + s_f[3][f];
         p.mx = max(max(s_f[0][5], s_f[1][5]), max(s_f[2][5], s_f[3][5]));
         p.bad = s_f[0][6] | s_f[1][6] | s_f[2][6] | s_f[3][6];
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            p.big[q] = max(max(s_f[0][7 + q], s_f[1][7 + q]), max(s_f[2][7 + q], s_f[3][7 + q]));
         p.sumsq = s_sq[0] + s_sq[1] + s_sq[2] + s_sq[3];
         parts[blockIdx.x] = p;
     }
@@ -1464,11 +1474,11 @@ __global__ __launch_bounds__(rowpass::kFinishThreads) void k_csize_finish(
     int32_t *__restrict__ cptr_end, int32_t *__restrict__ cvp_end) {
     constexpr int NT = rowpass::kFinishThreads, NW = NT / 64;
     __shared__ int s_w[NW][5];
-    __shared__ int s_mx[NW], s_bad[NW];
+    __shared__ int s_mx[NW][4], s_bad[NW];
     __shared__ unsigned long long s_sq[NW];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int run[5] = {0, 0, 0, 0, 0};
-    int mx = 0, bad = 0;
+    int mx[4] = {0, 0, 0, 0}, bad = 0;
     unsigned long long sq = 0ull;
     for (int base = 0; base < ng; base += NT) {
         const int gi = base + threadIdx.x;
@@ -1498,7 +1508,9 @@ __global__ __launch_bounds__(rowpass::kFinishThreads) void k_csize_finish(
         if (gi < ng) {
 #pragma unroll
             for (int q = 0; q < 5; ++q) parts[gi].f[q] = off[q];
-            mx = max(mx, p.mx);
+            mx[0] = max(mx[0], p.mx);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) mx[1 + q] = max(mx[1 + q], p.big[q]);
             bad |= p.bad;
             sq += p.sumsq;
         }
@@ -1506,28 +1518,34 @@ __global__ __launch_bounds__(rowpass::kFinishThreads) void k_csize_finish(
         for (int q = 0; q < 5; ++q) run[q] += all[q];
     }
     for (int o = 32; o > 0; o >>= 1) {
-        mx = max(mx, __shfl_xor(mx, o));
+#pragma unroll
+        for (int q = 0; q < 4; ++q) mx[q] = max(mx[q], __shfl_xor(mx[q], o));
         bad |= __shfl_xor(bad, o);
         sq += __shfl_xor(sq, o);
     }
     if (lane == 0) {
-        s_mx[w] = mx;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s_mx[w][q] = mx[q];
         s_bad[w] = bad;
         s_sq[w] = sq;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        int m = 0, b = 0;
+        int m[4] = {0, 0, 0, 0}, b = 0;
         unsigned long long t = 0ull;
         for (int k = 0; k < NW; ++k) {
-            m = max(m, s_mx[k]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) m[q] = max(m[q], s_mx[k][q]);
             b |= s_bad[k];
             t += s_sq[k];
         }
         tot->rs.cnt[0] = run[2];
         tot->rs.cnt[1] = run[3];
         tot->rs.cnt[2] = run[4];
-        tot->rs.max_blocks = m;
+        tot->rs.max_blocks = m[0];
+        tot->big[0] = m[1];
+        tot->big[1] = m[2];
+        tot->big[2] = m[3];
         tot->rs.sumsq = t;
         tot->nblk = run[0];
         tot->nnz = run[1];
@@ -1644,7 +1662,7 @@ template <typename VT>
 int launch_numeric(const SpgemmArgs &g, const int32_t *boff, const Mbsr &C, int32_t brow,
                    const int32_t *list, int32_t reg_rows, int32_t lds_rows, int32_t big_rows,
                    const int64_t *ub_off, int32_t stride, const int32_t *tcol,
-                   const uint32_t *tmap, double **nslab, hipStream_t s) {
+                   const uint32_t *tmap, const int32_t *big, double **nslab, hipStream_t s) {
     if (reg_rows > 0) {
         if constexpr (std::is_same<VT, double>::value) {
             // accumulators for the bin's longest C row (at most 64 blocks)
@@ -1670,18 +1688,23 @@ int launch_numeric(const SpgemmArgs &g, const int32_t *boff, const Mbsr &C, int3
         hipLaunchKernelGGL((k_numeric_cmp<VT, kNumCap, kCmpValShort>), dim3((lds_rows + 3) / 4),
                            dim3(256), 0, s, lds_rows, list + brow, g, boff, C.blk_ptr, C.blk_col,
                            C.blk_map, C.val_ptr, C.val);
-        hipLaunchKernelGGL((k_numeric_lds<VT>), dim3((lds_rows + 3) / 4), dim3(256), 0, s,
-                           lds_rows, list + brow, g, boff, C.blk_ptr, C.blk_col, C.blk_map,
-                           C.val_ptr, C.val);
+        if (big[0] > kCmpValShort)  // a row the compressed accumulators do not hold
+            hipLaunchKernelGGL((k_numeric_lds<VT>), dim3((lds_rows + 3) / 4), dim3(256), 0, s,
+                               lds_rows, list + brow, g, boff, C.blk_ptr, C.blk_col, C.blk_map,
+                               C.val_ptr, C.val);
     }
     if (big_rows > 0) {
         hipLaunchKernelGGL((k_numeric_cmp<VT, kCmpMidBlk, kCmpMidVal>), dim3((big_rows + 3) / 4),
                            dim3(256), 0, s, big_rows, list + 2 * int64_t(brow), g, boff,
                            C.blk_ptr, C.blk_col, C.blk_map, C.val_ptr, C.val);
-        hipLaunchKernelGGL((k_numeric_cmp<VT, kCmpBlk, kCmpVal, kCmpMidBlk, kCmpMidVal>),
-                           dim3((big_rows + 3) / 4), dim3(256), 0, s, big_rows,
-                           list + 2 * int64_t(brow), g, boff, C.blk_ptr, C.blk_col, C.blk_map,
-                           C.val_ptr, C.val);
+        if (big[1] > kCmpMidBlk || big[2] > kCmpMidVal)
+            hipLaunchKernelGGL((k_numeric_cmp<VT, kCmpBlk, kCmpVal, kCmpMidBlk, kCmpMidVal>),
+                               dim3((big_rows + 3) / 4), dim3(256), 0, s, big_rows,
+                               list + 2 * int64_t(brow), g, boff, C.blk_ptr, C.blk_col,
+                               C.blk_map, C.val_ptr, C.val);
+    }
+    // rows over the larger compressed launch's limits: dense LDS or a global slab
+    if (big_rows > 0 && (big[1] > kCmpBlk || big[2] > kCmpVal)) {
         const int64_t elems = int64_t(C.max_row_blocks) * 16;
         int nwaves = std::min(big_rows, 1024);
         while (nwaves > 4 && elems * nwaves * 8 > (int64_t(1) << 31)) nwaves >>= 1;
@@ -1852,8 +1875,9 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
     if (!bounded && bin[1] > 0) {
         hipLaunchKernelGGL(k_symbolic_wave, dim3((bin[1] + 3) / 4), dim3(256), 0, s, bin[1],
                            list + brow, g, ub_off, tcol, tmap, ccnt, cnnz);
-        hipLaunchKernelGGL(k_symbolic_block, dim3(bin[1]), dim3(256), 0, s, bin[1],
-                           list + brow, g, ub_off, tcol, tmap, ccnt, cnnz);
+        if (ht.max_blocks > kWaveUb)  // a row over the wave kernel's table
+            hipLaunchKernelGGL(k_symbolic_block, dim3(bin[1]), dim3(256), 0, s, bin[1],
+                               list + brow, g, ub_off, tcol, tmap, ccnt, cnnz);
     }
     if (!bounded && bin[2] > 0) {
         int hs = 1;
@@ -1960,17 +1984,17 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
     if (!C.pattern_only && brow > 0) {
         switch (precision) {
             case AMGT_FLOAT:
-                st = launch_numeric<float>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, ub_off, stride, tcol, tmap, &nslab, s);
+                st = launch_numeric<float>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, ub_off, stride, tcol, tmap, ct.big, &nslab, s);
                 break;
             case AMGT_HALF:
-                st = launch_numeric<__half>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, ub_off, stride, tcol, tmap, &nslab, s);
+                st = launch_numeric<__half>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, ub_off, stride, tcol, tmap, ct.big, &nslab, s);
                 // the FP16 copies hold A and B times half_scale: undo both (powers of two, exact)
                 if (st == AMGT_OK && C.nnz > 0)
                     hipLaunchKernelGGL(k_scale_values, dim3(unsigned((C.nnz + 255) / 256)), dim3(256),
                                        0, s, C.nnz, 1.0 / (A.half_scale * B.half_scale), C.val);
                 break;
             default:
-                st = launch_numeric<double>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, ub_off, stride, tcol, tmap, &nslab, s);
+                st = launch_numeric<double>(g, boff, C, brow, list, reg_rows, lds_rows, big_rows, ub_off, stride, tcol, tmap, ct.big, &nslab, s);
         }
         if (st != AMGT_OK) {
             cleanup();
