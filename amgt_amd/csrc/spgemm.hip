@@ -68,13 +68,9 @@ __device__ __forceinline__ void grp_sync() {
 template <int NT>
 __device__ __forceinline__ int grp_scan(int v, int *sc, int *tot) {
     const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        int t = __shfl_up(v, d);
-        if (lane >= d) v += t;
-    }
+    v = wave_incl_sum(v);  // every lane of the group calls this
     if constexpr (NT == 64) {
-        *tot = __shfl(v, 63);
+        *tot = __builtin_amdgcn_readlane(v, 63);
         return v;
     } else {
         const int w = (threadIdx.x % NT) >> 6;
@@ -200,8 +196,14 @@ __device__ void symbolic_row(int i, int tid, uint32_t *hkey, uint32_t *hmap, int
             m = hmap[s];
         }
         const int occ = (k != kEmpty);
-        int tot;
-        const int incl = grp_scan<NT>(occ, s_sc, &tot);
+        int tot, incl;
+        if constexpr (NT == 64) {  // a ballot counts the occupied slots
+            const uint64_t bal = __ballot(occ);
+            incl = __popcll(bal & ((2ull << (tid & 63)) - 1ull));
+            tot = __popcll(bal);
+        } else {
+            incl = grp_scan<NT>(occ, s_sc, &tot);
+        }
         grp_sync<NT>();
         if (occ) {
             hkey[base + incl - 1] = k;
@@ -211,6 +213,27 @@ __device__ void symbolic_row(int i, int tid, uint32_t *hkey, uint32_t *hmap, int
         grp_sync<NT>();
     }
     const int cnt = base;
+    if constexpr (NT == 64) {
+        if (cnt <= 64) {
+            // at most one key per lane and all keys distinct: each lane's rank
+            // among the row's keys is its sorted position
+            const uint32_t key = tid < cnt ? hkey[tid] : kEmpty;
+            const uint32_t m = tid < cnt ? hmap[tid] : 0u;
+            int rank = 0;
+            for (int j = 0; j < cnt; ++j) rank += hkey[j] < key ? 1 : 0;
+            if (tid < cnt) {
+                tcol[slot + rank] = int32_t(key);
+                tmap[slot + rank] = m;
+            }
+            const int nz_tot = __builtin_amdgcn_readlane(wave_incl_sum(__popc(m)), 63);
+            if (tid == 0) {
+                cnt_out[i] = cnt;
+                nnz_out[i] = nz_tot;
+            }
+            grp_sync<NT>();
+            return;
+        }
+    }
     int np = 1;
     while (np < cnt) np <<= 1;
     for (int s = cnt + tid; s < np; s += NT) hkey[s] = kEmpty;
