@@ -33,6 +33,7 @@ namespace amgt {
 namespace {
 
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kNoEntryOff = 0xFFFFFFF0u;  // a buffer offset past every resource: reads 0
 constexpr int kBin1Ub = 2048;   // workgroup per row, LDS table of 4096
 constexpr int kNumCap = 64;     // C blocks per row kept in LDS by the numeric kernel
 constexpr int kBigCap = 256;    // ... and by the long-row kernel before its global slab
@@ -282,6 +283,9 @@ struct SpgemmArgs {
     const uint16_t *bm;
     const void *bv;   // B values, same type as av
     int64_t bnnz;     // values of B
+    int64_t annz;     // values of A
+    int fp64_rs;      // FP64 values of A and B both under 2^32 - 16 bytes: whole-array
+                      // buffer resources with 32-bit offsets
 };
 
 // value of type VT at index idx, widened to double
@@ -701,6 +705,11 @@ __device__ void numeric_row(int i, const SpgemmArgs &g, const int32_t *__restric
     }
     for (int x = lane; x < nacc; x += 64) acc[x] = 0.0;
     wave_sync();
+    // whole-array resources over A's and B's values (used when g.fp64_rs)
+    const __amdgpu_buffer_rsrc_t a_rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<void *>(g.av), short(0), g.fp64_rs ? int(uint32_t(g.annz * 8)) : 0, kBufferDword3);
+    const __amdgpu_buffer_rsrc_t b_rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<void *>(g.bv), short(0), g.fp64_rs ? int(uint32_t(g.bnnz * 8)) : 0, kBufferDword3);
     int aoff_run = g.avp[i];
     const int a_beg = g.ap[i], a_end = g.ap[i + 1];
     for (int a0 = a_beg; a0 < a_end; a0 += 64) {
@@ -882,6 +891,11 @@ __device__ void numeric_row_outer(int i, const SpgemmArgs &g, const int32_t *__r
     }
     for (int x = lane; x < nacc; x += 64) acc[x] = 0.0;
     wave_sync();
+    // whole-array resources over A's and B's values (used when g.fp64_rs)
+    const __amdgpu_buffer_rsrc_t a_rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<void *>(g.av), short(0), g.fp64_rs ? int(uint32_t(g.annz * 8)) : 0, kBufferDword3);
+    const __amdgpu_buffer_rsrc_t b_rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<void *>(g.bv), short(0), g.fp64_rs ? int(uint32_t(g.bnnz * 8)) : 0, kBufferDword3);
     int aoff_run = g.avp[i];
     const int a_beg = g.ap[i], a_end = g.ap[i + 1];
     for (int a0 = a_beg; a0 < a_end; a0 += 64) {
@@ -945,10 +959,22 @@ __device__ void numeric_row_outer(int i, const SpgemmArgs &g, const int32_t *__r
                     inf[u] = g0 + u < ng ? st.p_info[idx] : -1;
                     const uint32_t xa = uint32_t(st.p_ma[idx]);
                     const uint32_t xb = uint32_t(inf[u]) & 0xFFFFu;
-                    av[u] = (inf[u] >= 0 && ((xa >> eA) & 1u))
-                                ? ldv<double>(g.av, st.p_aoff[idx] + __popc(xa & belowA)) : 0.0;
-                    bv[u] = (inf[u] >= 0 && ((xb >> eB) & 1u))
-                                ? ldv<double>(g.bv, st.p_boff[idx] + __popc(xb & belowB)) : 0.0;
+                    const bool ina = (inf[u] >= 0) & (((xa >> eA) & 1u) != 0u);
+                    const bool inb = (inf[u] >= 0) & (((xb >> eB) & 1u) != 0u);
+                    const int ia = st.p_aoff[idx] + __popc(xa & belowA);
+                    const int ib = st.p_boff[idx] + __popc(xb & belowB);
+                    if (g.fp64_rs) {
+                        // raw buffer loads: an offset past the resource reads 0.0
+                        const auto a = __builtin_amdgcn_raw_buffer_load_b64(
+                            a_rs, ina ? uint32_t(ia) * 8u : kNoEntryOff, 0, 0);
+                        const auto b = __builtin_amdgcn_raw_buffer_load_b64(
+                            b_rs, inb ? uint32_t(ib) * 8u : kNoEntryOff, 0, 0);
+                        __builtin_memcpy(&av[u], &a, 8);
+                        __builtin_memcpy(&bv[u], &b, 8);
+                    } else {
+                        av[u] = ina ? ldv<double>(g.av, ia) : 0.0;
+                        bv[u] = inb ? ldv<double>(g.bv, ib) : 0.0;
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -1775,8 +1801,11 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
         set_error("SpGEMM: operand values missing for the requested precision");
         return AMGT_ERR_ARG;
     }
+    const int64_t rs_max = int64_t(kNoEntryOff) / 8;  // values a 32-bit byte offset reaches
+    const int fp64_rs = precision != AMGT_FLOAT && precision != AMGT_HALF && A.nnz <= rs_max &&
+                        B.nnz <= rs_max;
     SpgemmArgs g{A.blk_ptr, A.blk_col, A.val_ptr, A.blk_map, avals,
-                 B.blk_ptr, B.blk_col, B.val_ptr, B.blk_map, bvals, B.nnz};
+                 B.blk_ptr, B.blk_col, B.val_ptr, B.blk_map, bvals, B.nnz, A.nnz, fp64_rs};
 
     // Bounded form: when the largest A row times the largest B row is at most
     // 64 products, every block row fits the register kernels, and the
