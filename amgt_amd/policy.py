@@ -5,8 +5,10 @@ FP32 / FP16) and its strength threshold.
 
 It replaces the reference's fixed rule (get_calculationPrecison,
 csr_spgemm_device.c:1847-1862: level 0 FP64, 1 FP32, deeper FP16) and
-BoomerAMG's single strong_threshold.  The finest level always stays FP64 and
-the Galerkin products keep the mixed rule (FP64 on the two finest levels).
+BoomerAMG's single strong_threshold.  The finest level always stays FP64, and
+the Galerkin products are FP64 on every level, as in the sweeps that made the
+labels (the FP32/FP16 products are also slower than the FP64 ones on this
+part: A^2 of 7-pt 128^3 1.68 / 1.85 against 1.06 ms).
 
 Workload: -div(k grad u) = f, 5-point finite volumes, k = exp(GRF)
 (workloads.grf_problem); training problems at 256^2, held-out ones at 1024^2.
@@ -325,8 +327,8 @@ def solve_learned(net: PolicyNet, dI, dJ, dV, n, db, device, tol=1e-8, capture=T
     hook = _FeatureHook(decide)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    amg = G.BoomerAMG(dI, dJ, dV, n, device, level_hook=hook,
-                      spgemm_policy=G.mixed_spgemm_policy, **AMG_ARGS)
+    amg = G.BoomerAMG(dI, dJ, dV, n, device, level_hook=hook, spgemm_policy=G.fp64_policy,
+                      **AMG_ARGS)
     torch.cuda.synchronize()
     tc = time.perf_counter()
     if capture:
