@@ -102,11 +102,19 @@ def csr_flops(AI, AJ, BI):
     return int(2 * blen[AJ].sum())
 
 
-def spgemm_timings(W, MBSRMatrix, M, I, J, A, dev, reps=3):
+def csr_bytes(rows, nnz):
+    """CSR-equivalent bytes of a matrix: 8-byte values, 4-byte columns and row
+    pointers (the SpGEMM's algorithmic bytes count A, B and C this way)."""
+    return 12 * int(nnz) + 4 * (int(rows) + 1)
+
+
+def spgemm_timings(W, MBSRMatrix, M, I, J, A, dev, reps=7):
     """Setup-phase products on the same 128^3 7-pt matrix: A^2 (HYPRE's
     ij_mm benchmark job) and the Galerkin pair A*P, P^T*(AP) with the
     trilinear prolongation to the 64^3 grid (a geometric P, not HYPRE's HMIS +
-    ext+i P).  Wall time per call including the size syncs."""
+    ext+i P).  Wall time per call including the size syncs (median of reps);
+    alg_bytes = CSR-equivalent bytes of both operands and the product, and
+    frac_of_hbm_peak = alg_bytes / time / 8 TB/s."""
     import numpy as np
     import torch
 
@@ -128,8 +136,10 @@ def spgemm_timings(W, MBSRMatrix, M, I, J, A, dev, reps=3):
     a2_ms = timed(lambda: M.spgemm(M))
     C = M.spgemm(M)
     f = csr_flops(I, J, I)
+    ab = 2 * csr_bytes(n, len(J)) + csr_bytes(n, C.info()["nnz"])
     out["a2"] = {"ms": round(a2_ms, 3), "nnz": int(C.info()["nnz"]), "flops": f,
-                 "gflops": round(f / (a2_ms * 1e-3) / 1e9, 2)}
+                 "gflops": round(f / (a2_ms * 1e-3) / 1e9, 2), "alg_bytes": ab,
+                 "frac_of_hbm_peak": round(ab / (a2_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     del C
     from amgt_amd.amg import trilinear_interpolation_device
 
@@ -145,11 +155,15 @@ def spgemm_timings(W, MBSRMatrix, M, I, J, A, dev, reps=3):
     RAP = R.spgemm(AP)
     f_ap = csr_flops(I, J, PI)
     f_rap = csr_flops(RI, RJ, APi.cpu().numpy())
+    ap_nnz, p_nnz = AP.info()["nnz"], P.info()["nnz"]
+    pb = (csr_bytes(n, len(J)) + csr_bytes(n, p_nnz) + csr_bytes(n, ap_nnz) +
+          csr_bytes(nc, p_nnz) + csr_bytes(n, ap_nnz) + csr_bytes(nc, RAP.info()["nnz"]))
     out["rap_level0_trilinear"] = {
         "ap_ms": round(ap_ms, 3), "rap_ms": round(rap_ms, 3),
         "total_ms": round(ap_ms + rap_ms, 3), "coarse_rows": nc,
         "coarse_nnz": int(RAP.info()["nnz"]), "flops": f_ap + f_rap,
-        "gflops": round((f_ap + f_rap) / ((ap_ms + rap_ms) * 1e-3) / 1e9, 2)}
+        "gflops": round((f_ap + f_rap) / ((ap_ms + rap_ms) * 1e-3) / 1e9, 2), "alg_bytes": pb,
+        "frac_of_hbm_peak": round(pb / ((ap_ms + rap_ms) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     return out
 
 
