@@ -197,6 +197,10 @@ __global__ __launch_bounds__(kThreads) void k_gather_pattern(int32_t nnz,
 }
 
 int d2h_sync(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    if (bytes <= kReadbackBytes) {  // small: through the pinned staging area
+        const Readback rb[1] = {{dst, src, bytes}};
+        return readback(rb, 1, s);
+    }
     AMGT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
     AMGT_HIP(hipStreamSynchronize(s));
     return AMGT_OK;
@@ -232,8 +236,11 @@ int first_touch_product(int32_t m, const int32_t *XI, const int32_t *XJ, const d
     AMGT_TRY(exclusive_scan_i64(ub, off, m, s));
     AMGT_TRY(exclusive_scan_i64(ts, toff, m, s));
     int64_t tot[2] = {0, 0};
-    AMGT_TRY(d2h_sync(&tot[0], off + m, sizeof(int64_t), s));
-    AMGT_TRY(d2h_sync(&tot[1], toff + m, sizeof(int64_t), s));
+    {
+        const Readback rb[2] = {{&tot[0], off + m, sizeof(int64_t)},
+                                {&tot[1], toff + m, sizeof(int64_t)}};
+        AMGT_TRY(readback(rb, 2, s));
+    }
     int2 *kv = nullptr;
     int32_t *tj = nullptr;
     double *tv = nullptr;
@@ -339,8 +346,9 @@ int amgt_amg_rap_host_order(int32_t n, int32_t nc, const int32_t *A_i, const int
     Tmp tmp(s);
     int32_t nnzP = 0, nnzA = 0;
     if (n > 0) {
-        AMGT_TRY(d2h_sync(&nnzP, P_i + n, sizeof(int32_t), s));
-        AMGT_TRY(d2h_sync(&nnzA, A_i + n, sizeof(int32_t), s));
+        const Readback rb[2] = {{&nnzP, P_i + n, sizeof(int32_t)},
+                                {&nnzA, A_i + n, sizeof(int32_t)}};
+        AMGT_TRY(readback(rb, 2, s));
     }
     AMGT_CHECK_ARG(nnzP == 0 || (P_j && P_data), "NULL P column or value array");
     AMGT_CHECK_ARG(nnzA == 0 || (A_j && A_data), "NULL A column or value array");

@@ -233,8 +233,10 @@ int amgt_amg_hmis(int32_t n, const int32_t *S_i, const int32_t *S_j, int32_t *cf
     Staging &st = staging();
     std::lock_guard<std::mutex> lock(st.mu);
     int32_t nnz = 0;
-    AMGT_HIP(hipMemcpyAsync(&nnz, S_i + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    AMGT_HIP(hipStreamSynchronize(s));
+    {
+        const Readback rb[1] = {{&nnz, S_i + n, sizeof(int32_t)}};
+        AMGT_TRY(readback(rb, 1, s));
+    }
     AMGT_CHECK_ARG(nnz == 0 || S_j, "NULL column array");
     AMGT_CHECK_ARG(nnz >= 0, "negative nonzero count in S");
     // S^T on the device (rows ascending, as the host sweep reads them)
@@ -251,12 +253,13 @@ int amgt_amg_hmis(int32_t n, const int32_t *S_i, const int32_t *S_j, int32_t *cf
         status = csr_transpose_device(n, n, S_i, S_j, nullptr, nnz, dti, dtj, nullptr, s);
     void *buf = nullptr;
     if (status == AMGT_OK)
-        status = st.get(sizeof(int32_t) * (3 * size_t(n) + 2 + 2 * size_t(nnz)), &buf);
+        status = st.get(sizeof(int32_t) * (3 * size_t(n) + 3 + 2 * size_t(nnz)), &buf);
     int32_t *hi = static_cast<int32_t *>(buf);
     int32_t *hj = hi + n + 1;
     int32_t *hti = hj + nnz;
     int32_t *htj = hti + n + 1;
     int32_t *hcf = htj + nnz;
+    int32_t *hbad_p = hcf + n;  // the device check's flag, in the pinned area too
     if (status == AMGT_OK) {
         hipError_t e = hipMemcpyAsync(hi, S_i, sizeof(int32_t) * (size_t(n) + 1),
                                       hipMemcpyDeviceToHost, s);
@@ -268,8 +271,9 @@ int amgt_amg_hmis(int32_t n, const int32_t *S_i, const int32_t *S_j, int32_t *cf
         if (e == hipSuccess && nnz)
             e = hipMemcpyAsync(htj, dtj, sizeof(int32_t) * size_t(nnz), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(&hbad, dbad, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+            e = hipMemcpyAsync(hbad_p, dbad, sizeof(int32_t), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e == hipSuccess) hbad = *hbad_p;
         if (e != hipSuccess) {
             set_error(std::string("HMIS: copy of S to the host: ") + hipGetErrorString(e));
             status = AMGT_ERR_HIP;

@@ -147,10 +147,8 @@ int amgt_interp_trilinear(int32_t nprob, const int32_t *grids, int32_t *n_fine,
     }
     if (st == AMGT_OK) st = exclusive_scan_i32(cnt, pi, rows, s);
     if (st == AMGT_OK) {
-        if (hipMemcpyAsync(&nnz, pi + rows, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
-                hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess)
-            st = AMGT_ERR_HIP;
+        const Readback rb[1] = {{&nnz, pi + rows, sizeof(int32_t)}};
+        st = readback(rb, 1, s);
     }
     if (st == AMGT_OK) {
         pj = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * (size_t(nnz) + 1), s));

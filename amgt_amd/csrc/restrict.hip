@@ -101,11 +101,10 @@ static int plan_build(const Mbsr &A, const Mbsr &R, hipStream_t s, RestrictPlan 
     }
     if (st == AMGT_OK) st = exclusive_scan_i32(p->nspan, pptr, nc, s);
     int32_t npairs = 0;
-    if (st == AMGT_OK &&
-        (hipMemcpyAsync(&npairs, pptr + nc, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
-             hipSuccess ||
-         hipStreamSynchronize(s) != hipSuccess))
-        st = AMGT_ERR_HIP;
+    if (st == AMGT_OK) {
+        const Readback rb[1] = {{&npairs, pptr + nc, sizeof(int32_t)}};
+        st = readback(rb, 1, s);
+    }
     if (st == AMGT_OK) st = tmp_alloc_t(&ptile, size_t(npairs), s);
     if (st == AMGT_OK) {
         p->tile_c = static_cast<int32_t *>(dev_alloc(sizeof(int32_t) * size_t(npairs > 0 ? npairs : 1), s));

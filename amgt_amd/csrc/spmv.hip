@@ -1580,14 +1580,17 @@ int mbsr_ensure_lowp(Mbsr *A, int *prec, hipStream_t s) {
         hipLaunchKernelGGL(k_absmax_part, dim3(1), dim3(256), 0, s, int64_t(parts), part,
                            part + parts);
         hipError_t e = hipGetLastError();
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(&h_max, part + parts, sizeof(double), hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        int st = AMGT_OK;
+        if (e == hipSuccess) {
+            const Readback rb[1] = {{&h_max, part + parts, sizeof(double)}};
+            st = readback(rb, 1, s);
+        }
         tmp_free(part, s);
         if (e != hipSuccess) {
             set_error(std::string("FP16 range check: ") + hipGetErrorString(e));
             return AMGT_ERR_HIP;
         }
+        AMGT_TRY(st);
         A->half_fits = std::isfinite(h_max) ? 1 : 0;
         // 2^e with max |a| * 2^e in [2^14, 2^15): ilogb gives max |a| = f 2^k, f in [1, 2)
         const int ex = h_max > 0.0 && A->half_fits
