@@ -705,11 +705,6 @@ __device__ void numeric_row(int i, const SpgemmArgs &g, const int32_t *__restric
     }
     for (int x = lane; x < nacc; x += 64) acc[x] = 0.0;
     wave_sync();
-    // whole-array resources over A's and B's values (used when g.fp64_rs)
-    const __amdgpu_buffer_rsrc_t a_rs = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<void *>(g.av), short(0), g.fp64_rs ? int(uint32_t(g.annz * 8)) : 0, kBufferDword3);
-    const __amdgpu_buffer_rsrc_t b_rs = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<void *>(g.bv), short(0), g.fp64_rs ? int(uint32_t(g.bnnz * 8)) : 0, kBufferDword3);
     int aoff_run = g.avp[i];
     const int a_beg = g.ap[i], a_end = g.ap[i + 1];
     for (int a0 = a_beg; a0 < a_end; a0 += 64) {
@@ -892,10 +887,12 @@ __device__ void numeric_row_outer(int i, const SpgemmArgs &g, const int32_t *__r
     for (int x = lane; x < nacc; x += 64) acc[x] = 0.0;
     wave_sync();
     // whole-array resources over A's and B's values (used when g.fp64_rs)
-    const __amdgpu_buffer_rsrc_t a_rs = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<void *>(g.av), short(0), g.fp64_rs ? int(uint32_t(g.annz * 8)) : 0, kBufferDword3);
-    const __amdgpu_buffer_rsrc_t b_rs = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<void *>(g.bv), short(0), g.fp64_rs ? int(uint32_t(g.bnnz * 8)) : 0, kBufferDword3);
+    const __amdgpu_buffer_rsrc_t a_rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(g.av), short(0),
+                                          g.fp64_rs ? int(uint32_t(g.annz * 8)) : 0, kBufferDword3);
+    const __amdgpu_buffer_rsrc_t b_rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(g.bv), short(0),
+                                          g.fp64_rs ? int(uint32_t(g.bnnz * 8)) : 0, kBufferDword3);
     int aoff_run = g.avp[i];
     const int a_beg = g.ap[i], a_end = g.ap[i + 1];
     for (int a0 = a_beg; a0 < a_end; a0 += 64) {
