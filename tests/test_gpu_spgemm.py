@@ -78,6 +78,43 @@ def test_spgemm_random(dev, shape, dens, seed):
     check_product(dev, m, k, n, AI, AJ, AA, BI, BJ, BA, "random")
 
 
+def blocks_per_row_csr(nb_rows, nb_cols, kb, seed, empty_row=None):
+    """CSR matrix of 4 nb_rows x 4 nb_cols whose every block row has exactly kb
+    occupied 4x4 blocks (1-3 entries each, random positions), except block row
+    `empty_row`, which has none."""
+    rng = np.random.default_rng(seed)
+    rows = [dict() for _ in range(4 * nb_rows)]
+    for bi in range(nb_rows):
+        if bi == empty_row:
+            continue
+        for bc in rng.choice(nb_cols, size=kb, replace=False):
+            for _ in range(int(rng.integers(1, 4))):
+                r, c = 4 * bi + int(rng.integers(4)), 4 * int(bc) + int(rng.integers(4))
+                rows[r][c] = float(rng.uniform(-1.0, 1.0))
+    I = np.zeros(4 * nb_rows + 1, dtype=np.int32)
+    J, A = [], []
+    for r, d in enumerate(rows):
+        for c in sorted(d):
+            J.append(c)
+            A.append(d[c])
+        I[r + 1] = len(J)
+    return I, np.array(J, dtype=np.int32), np.array(A, dtype=np.float64)
+
+
+@pytest.mark.parametrize("kb_b", [8, 9])
+def test_spgemm_register_bound(dev, kb_b):
+    # largest A block row (8 blocks) times largest B block row: 64 products is
+    # the bounded register form (fixed temporaries per row, no upper-bound
+    # pass), 72 the general form with bins; both against the oracle, with an
+    # empty block row in A
+    AI, AJ, AA = blocks_per_row_csr(40, 30, 8, 11, empty_row=7)
+    BI, BJ, BA = blocks_per_row_csr(30, 25, kb_b, 12)
+    GC, ref = check_product(dev, 160, 120, 100, AI, AJ, AA, BI, BJ, BA, f"bound kb_b={kb_b}")
+    cnt = np.diff(ref["blk_ptr"])
+    assert cnt[7] == 0 and cnt.max() >= 20  # the empty row, and rows near the bound
+    assert GC.info()["max_row_blocks"] == int(cnt.max())
+
+
 def test_spgemm_stencils(dev):
     for kind, dims in [(7, (10, 9, 8)), (27, (7, 6, 5)), (7, (33, 31, 1)), (9, (20, 17, 1))]:
         I, J, A = W.stencil(kind, *dims)
