@@ -108,11 +108,12 @@ def csr_bytes(rows, nnz):
     return 12 * int(nnz) + 4 * (int(rows) + 1)
 
 
-def spgemm_timings(W, MBSRMatrix, M, I, J, A, dev, reps=7):
+def spgemm_timings(W, MBSRMatrix, M, I, J, A, dev, reps=20, warmup=3):
     """Setup-phase products on the same 128^3 7-pt matrix: A^2 (HYPRE's
     ij_mm benchmark job) and the Galerkin pair A*P, P^T*(AP) with the
     trilinear prolongation to the 64^3 grid (a geometric P, not HYPRE's HMIS +
-    ext+i P).  Wall time per call including the size syncs (median of reps);
+    ext+i P).  Wall time per call including the size syncs (median of reps
+    after warmup calls, the measurement protocol of warm kernels);
     alg_bytes = CSR-equivalent bytes of both operands and the product, and
     frac_of_hbm_peak = alg_bytes / time / 8 TB/s."""
     import numpy as np
@@ -121,7 +122,8 @@ def spgemm_timings(W, MBSRMatrix, M, I, J, A, dev, reps=7):
     n = len(I) - 1
 
     def timed(fn):
-        fn()
+        for _ in range(warmup):
+            fn()
         torch.cuda.synchronize()
         ts = []
         for _ in range(reps):
