@@ -312,7 +312,7 @@ __device__ __forceinline__ double round_out<double>(double x) {
     return x;
 }
 
-// ---- bin 0: rows with at most 64 products and 64 A blocks, one wave per row.
+// ---- bin 0: rows with at most 64 products and 64 A blocks, RPW rows per wave.
 // Lane j holds A block j, lane t product t (its owner A block: every A block
 // with products marks its first one, a max-scan spreads the marks).  The
 // block columns of the products with a non-zero bitmap product are
@@ -320,13 +320,18 @@ __device__ __forceinline__ double round_out<double>(double x) {
 // the 64 keys, ties by lane) and placed in sorted order; the first lane of
 // every run of equal columns opens a C block, and every product ORs its
 // bitmap product into its C block's bitmap in LDS (OR does not depend on the
-// order).  No hashing, deterministic.
+// order).  No hashing, deterministic.  The kernel waits on chains of
+// dependent loads (row pointer, A blocks, B row pointers, B blocks), so a
+// wave takes RPW rows and issues each stage's loads for all of them before
+// it waits.
 //
 // rows / ub_off NULL: the bounded form, every block row 0 .. count-1 with its
 // slot of the temporaries at row * stride, where the host's bound (largest A
 // row times largest B row, at most 64) holds every row's products.  A row
 // that breaks the bound writes no structure and a block count of -1, which
 // the size pass reports as an error.
+constexpr int kSymRegRows = 2;  // rows per wave of k_symbolic_reg
+template <int RPW>
 __global__ __launch_bounds__(256) void k_symbolic_reg(int count, const int32_t *__restrict__ rows,
                                                       SpgemmArgs g,
                                                       const int64_t *__restrict__ ub_off,
@@ -335,93 +340,166 @@ __global__ __launch_bounds__(256) void k_symbolic_reg(int count, const int32_t *
                                                       uint32_t *__restrict__ tmap,
                                                       int32_t *__restrict__ cnt_out,
                                                       int32_t *__restrict__ nnz_out) {
-    __shared__ __align__(16) int32_t s_key[4][64];
-    __shared__ int32_t s_sk[4][64];  // owner marks, then the sorted keys
-    __shared__ uint32_t s_sm[4][64], s_cm[4][64];
+    __shared__ __align__(16) int32_t s_key[4][RPW][64];
+    __shared__ int32_t s_sk[4][RPW][64];  // owner marks, then the sorted keys
+    __shared__ uint32_t s_sm[4][RPW][64], s_cm[4][RPW][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int r = blockIdx.x * 4 + w;
-    if (r >= count) return;  // whole wave; the waves share no LDS
-    const int i = rows ? rows[r] : r;
-    const int a_beg = g.ap[i], na = g.ap[i + 1] - a_beg;
-    int ma = 0, bq0 = 0, blen = 0;
-    if (lane < na) {
-        const int k = g.ac[a_beg + lane];
-        ma = g.am[a_beg + lane];
-        bq0 = g.bp[k];
-        blen = g.bp[k + 1] - bq0;
+    const int r0 = (blockIdx.x * 4 + w) * RPW;
+    if (r0 >= count) return;  // whole wave; the waves share no LDS
+    int i[RPW], na[RPW], a_beg[RPW];
+    bool on[RPW];
+#pragma unroll
+    for (int u = 0; u < RPW; ++u) {
+        on[u] = r0 + u < count;
+        i[u] = on[u] ? (rows ? rows[r0 + u] : r0 + u) : 0;
     }
-    const int linc = wave_incl_sum(blen);
-    const int pst = linc - blen;
-    const int ptot = __builtin_amdgcn_readlane(linc, 63);
-    if (na > 64 || ptot > (ub_off ? 64 : stride)) {  // outside the row's bin or bound
-        if (lane == 0) {
-            cnt_out[i] = -1;
-            nnz_out[i] = 0;
+#pragma unroll
+    for (int u = 0; u < RPW; ++u) {
+        a_beg[u] = g.ap[i[u]];
+        na[u] = on[u] ? g.ap[i[u] + 1] - a_beg[u] : 0;
+    }
+    int kk[RPW], ma[RPW], bq0[RPW], blen[RPW];
+#pragma unroll
+    for (int u = 0; u < RPW; ++u) {
+        kk[u] = 0;
+        ma[u] = 0;
+        if (lane < na[u]) {
+            kk[u] = g.ac[a_beg[u] + lane];
+            ma[u] = g.am[a_beg[u] + lane];
         }
-        return;
     }
-    s_sk[w][lane] = 0;
-    s_cm[w][lane] = 0u;
-    wave_sync();
-    if (blen > 0) s_sk[w][pst] = lane;  // pst < ptot <= 64, distinct
-    wave_sync();
-    const int own = wave_incl_max(s_sk[w][lane]);
-    const int q = __shfl(bq0, own) + (lane - __shfl(pst, own));
-    const uint32_t pma = uint32_t(__shfl(ma, own));
-    int key = INT_MAX;
-    uint32_t mc = 0u;
-    if (lane < ptot) {
-        mc = bitmap_mul(pma, uint32_t(g.bm[q]));
-        if (mc) key = g.bc[q];
+#pragma unroll
+    for (int u = 0; u < RPW; ++u) {
+        bq0[u] = 0;
+        blen[u] = 0;
+        if (lane < na[u]) {
+            bq0[u] = g.bp[kk[u]];
+            blen[u] = g.bp[kk[u] + 1] - bq0[u];
+        }
     }
-    const int nvalid = __popcll(__ballot(key != INT_MAX));
-    int rank = 0;
+    int pst[RPW], ptot[RPW];
+    bool ok[RPW];
+#pragma unroll
+    for (int u = 0; u < RPW; ++u) {
+        const int linc = wave_incl_sum(blen[u]);
+        pst[u] = linc - blen[u];
+        ptot[u] = __builtin_amdgcn_readlane(linc, 63);
+        ok[u] = on[u] && na[u] <= 64 && ptot[u] <= (ub_off ? 64 : stride);
+        if (on[u] && !ok[u] && lane == 0) {  // outside the row's bin or bound
+            cnt_out[i[u]] = -1;
+            nnz_out[i[u]] = 0;
+        }
+        s_sk[w][u][lane] = 0;
+        s_cm[w][u][lane] = 0u;
+    }
+    wave_sync();
+#pragma unroll
+    for (int u = 0; u < RPW; ++u)
+        if (ok[u] && blen[u] > 0) s_sk[w][u][pst[u]] = lane;  // pst < ptot <= 64, distinct
+    wave_sync();
+    int q[RPW];
+    uint32_t pma[RPW];
+#pragma unroll
+    for (int u = 0; u < RPW; ++u) {
+        const int own = wave_incl_max(s_sk[w][u][lane]);
+        q[u] = __shfl(bq0[u], own) + (lane - __shfl(pst[u], own));
+        pma[u] = uint32_t(__shfl(ma[u], own));
+    }
+    int key[RPW];
+    uint32_t mb[RPW];
+#pragma unroll
+    for (int u = 0; u < RPW; ++u) {  // the products' B blocks, all rows' loads together
+        key[u] = INT_MAX;
+        mb[u] = 0u;
+        if (ok[u] && lane < ptot[u]) {
+            mb[u] = uint32_t(g.bm[q[u]]);
+            key[u] = g.bc[q[u]];
+        }
+    }
+    uint32_t mc[RPW];
+    int nvalid[RPW], rank[RPW];
+#pragma unroll
+    for (int u = 0; u < RPW; ++u) {
+        mc[u] = bitmap_mul(pma[u], mb[u]);
+        if (!mc[u]) key[u] = INT_MAX;
+        nvalid[u] = __popcll(__ballot(key[u] != INT_MAX));
+        rank[u] = 0;
+    }
     if (packed) {
         // block columns below 2^25 - 1: the key and the lane in one integer,
         // all distinct, so the rank is one comparison per key
-        const int pk = key != INT_MAX ? key * 64 + lane : INT_MAX;
-        s_key[w][lane] = pk;
+        int pk[RPW];
+#pragma unroll
+        for (int u = 0; u < RPW; ++u) {
+            pk[u] = key[u] != INT_MAX ? key[u] * 64 + lane : INT_MAX;
+            s_key[w][u][lane] = pk[u];
+        }
         wave_sync();
-        for (int j = 0; j < ptot; j += 4) {
-            const int4 kk = *reinterpret_cast<const int4 *>(&s_key[w][j]);
-            rank += kk.x < pk ? 1 : 0;
-            rank += kk.y < pk ? 1 : 0;
-            rank += kk.z < pk ? 1 : 0;
-            rank += kk.w < pk ? 1 : 0;
+#pragma unroll
+        for (int u = 0; u < RPW; ++u) {
+            if (!ok[u]) continue;  // wave-uniform
+            for (int j = 0; j < ptot[u]; j += 4) {
+                const int4 k4 = *reinterpret_cast<const int4 *>(&s_key[w][u][j]);
+                rank[u] += k4.x < pk[u] ? 1 : 0;
+                rank[u] += k4.y < pk[u] ? 1 : 0;
+                rank[u] += k4.z < pk[u] ? 1 : 0;
+                rank[u] += k4.w < pk[u] ? 1 : 0;
+            }
         }
     } else {
-        s_key[w][lane] = key;
+#pragma unroll
+        for (int u = 0; u < RPW; ++u) s_key[w][u][lane] = key[u];
         wave_sync();
-        for (int j = 0; j < ptot; j += 4) {
-            const int4 kk = *reinterpret_cast<const int4 *>(&s_key[w][j]);
-            rank += (kk.x < key || (kk.x == key && j < lane)) ? 1 : 0;
-            rank += (kk.y < key || (kk.y == key && j + 1 < lane)) ? 1 : 0;
-            rank += (kk.z < key || (kk.z == key && j + 2 < lane)) ? 1 : 0;
-            rank += (kk.w < key || (kk.w == key && j + 3 < lane)) ? 1 : 0;
+#pragma unroll
+        for (int u = 0; u < RPW; ++u) {
+            if (!ok[u]) continue;  // wave-uniform
+            const int ky = key[u];
+            for (int j = 0; j < ptot[u]; j += 4) {
+                const int4 k4 = *reinterpret_cast<const int4 *>(&s_key[w][u][j]);
+                rank[u] += (k4.x < ky || (k4.x == ky && j < lane)) ? 1 : 0;
+                rank[u] += (k4.y < ky || (k4.y == ky && j + 1 < lane)) ? 1 : 0;
+                rank[u] += (k4.z < ky || (k4.z == ky && j + 2 < lane)) ? 1 : 0;
+                rank[u] += (k4.w < ky || (k4.w == ky && j + 3 < lane)) ? 1 : 0;
+            }
         }
     }
     // lanes at or past ptot hold INT_MAX and rank past every valid key
-    if (key != INT_MAX) {
-        s_sk[w][rank] = key;
-        s_sm[w][rank] = mc;
+#pragma unroll
+    for (int u = 0; u < RPW; ++u) {
+        if (ok[u] && key[u] != INT_MAX) {
+            s_sk[w][u][rank[u]] = key[u];
+            s_sm[w][u][rank[u]] = mc[u];
+        }
     }
     wave_sync();
-    const bool in = lane < nvalid;
-    const int sk = in ? s_sk[w][lane] : INT_MAX;
-    const bool head = in && (lane == 0 || s_sk[w][lane > 0 ? lane - 1 : 0] != sk);
-    const uint64_t heads = __ballot(head);
-    const int cslot = __popcll(heads & ((2ull << lane) - 1ull)) - 1;  // C block of sorted lane
-    if (in) atomicOr(&s_cm[w][cslot], s_sm[w][lane]);
+    uint64_t heads[RPW];
+    int sk[RPW];
+    bool in[RPW];
+#pragma unroll
+    for (int u = 0; u < RPW; ++u) {
+        in[u] = ok[u] && lane < nvalid[u];
+        sk[u] = in[u] ? s_sk[w][u][lane] : INT_MAX;
+        const bool head = in[u] && (lane == 0 || s_sk[w][u][lane > 0 ? lane - 1 : 0] != sk[u]);
+        heads[u] = __ballot(head);
+        const int cslot = __popcll(heads[u] & ((2ull << lane) - 1ull)) - 1;  // sorted lane's block
+        if (in[u]) atomicOr(&s_cm[w][u][cslot], s_sm[w][u][lane]);
+    }
     wave_sync();
-    const int ncb = __popcll(heads);
-    const int64_t slot = ub_off ? ub_off[i] : int64_t(i) * stride;
-    if (head) tcol[slot + cslot] = sk;
-    const uint32_t cm = lane < ncb ? s_cm[w][lane] : 0u;
-    if (lane < ncb) tmap[slot + lane] = cm;
-    const int nz = __builtin_amdgcn_readlane(wave_incl_sum(__popc(cm)), 63);
-    if (lane == 0) {
-        cnt_out[i] = ncb;
-        nnz_out[i] = nz;
+#pragma unroll
+    for (int u = 0; u < RPW; ++u) {
+        if (!ok[u]) continue;  // wave-uniform
+        const int ncb = __popcll(heads[u]);
+        const int64_t slot = ub_off ? ub_off[i[u]] : int64_t(i[u]) * stride;
+        const bool head = ((heads[u] >> lane) & 1ull) != 0ull;
+        const int cslot = __popcll(heads[u] & ((2ull << lane) - 1ull)) - 1;
+        if (head) tcol[slot + cslot] = sk[u];
+        const uint32_t cm = lane < ncb ? s_cm[w][u][lane] : 0u;
+        if (lane < ncb) tmap[slot + lane] = cm;
+        const int nz = __builtin_amdgcn_readlane(wave_incl_sum(__popc(cm)), 63);
+        if (lane == 0) {
+            cnt_out[i[u]] = ncb;
+            nnz_out[i[u]] = nz;
+        }
     }
 }
 
@@ -1913,13 +1991,15 @@ int spgemm(const Mbsr &A, const Mbsr &B, int precision, hipStream_t s, Mbsr *Cp)
         }
         SG_HIP(hipGetLastError());
     }
+    constexpr int kSymRows = 4 * kSymRegRows;  // rows per workgroup of k_symbolic_reg
     if (bounded)
-        hipLaunchKernelGGL(k_symbolic_reg, dim3((brow + 3) / 4), dim3(256), 0, s, brow,
-                           static_cast<const int32_t *>(nullptr), g,
+        hipLaunchKernelGGL((k_symbolic_reg<kSymRegRows>), dim3((brow + kSymRows - 1) / kSymRows),
+                           dim3(256), 0, s, brow, static_cast<const int32_t *>(nullptr), g,
                            static_cast<const int64_t *>(nullptr), stride, packed, tcol, tmap,
                            ccnt, cnnz);
     if (!bounded && bin[0] > 0)
-        hipLaunchKernelGGL(k_symbolic_reg, dim3((bin[0] + 3) / 4), dim3(256), 0, s, bin[0],
+        hipLaunchKernelGGL((k_symbolic_reg<kSymRegRows>),
+                           dim3((bin[0] + kSymRows - 1) / kSymRows), dim3(256), 0, s, bin[0],
                            list, g, ub_off, 0, packed, tcol, tmap, ccnt, cnnz);
     if (!bounded && bin[1] > 0) {
         hipLaunchKernelGGL(k_symbolic_wave, dim3((bin[1] + 3) / 4), dim3(256), 0, s, bin[1],
