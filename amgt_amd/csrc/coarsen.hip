@@ -682,8 +682,28 @@ __global__ __launch_bounds__(64) void k_extpi_wave(
                 s.je1[jj] = int32_t(e1);
                 const int64_t d1 = diag_pos(I, J, i1);
                 sgn = (d1 >= 0 && A[d1] < 0) ? -1 : 1;
+                // the host's sum in the host's order; the row's entries are
+                // loaded eight at a time so that their loads overlap
                 double sum = 0.0;
-                for (int64_t q = b1; q < e1; ++q) {
+                int64_t q = b1;
+                for (; q + 8 <= e1; q += 8) {
+                    int jq[8];
+                    double aq[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        jq[u] = J[q + u];
+                        aq[u] = A[q + u];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        if (q + u == d1) continue;
+                        const int i2 = jq[u];
+                        if ((i2 == i || lds_find<WH>(s.hkey, s.hval, i2) >= 0) &&
+                            (sgn * aq[u]) < 0)
+                            sum += aq[u];
+                    }
+                }
+                for (; q < e1; ++q) {
                     if (q == d1) continue;
                     const int i2 = J[q];
                     if ((i2 == i || lds_find<WH>(s.hkey, s.hval, i2) >= 0) && (sgn * A[q]) < 0)
