@@ -599,23 +599,41 @@ __global__ __launch_bounds__(64) void k_extpi_wave(
     for (int b0 = 0; b0 < ls; b0 += 64) {
         const int p = b0 + lane;
         int i1 = 0, k = 0;
+        int cf1 = 0, q0 = 0, q1 = 0;
         if (p < ls) {
             i1 = s.scol[p];
-            if (cf[i1] >= 0) {
+            cf1 = cf[i1];
+            if (cf1 >= 0) {
                 k = 1;
-            } else if (cf[i1] != -3) {
-                for (int q = SI[i1]; q < SI[i1 + 1]; ++q) k += cf[SJ[q]] >= 0;
+            } else if (cf1 != -3) {
+                q0 = SI[i1];
+                q1 = SI[i1 + 1];
+                // the strong neighbours of i1 and their markers, eight at a
+                // time so that the loads overlap
+                for (int q = q0; q < q1; q += 8) {
+                    int c8[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) c8[u] = q + u < q1 ? SJ[q + u] : -1;
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) k += (c8[u] >= 0 && cf[c8[u]] >= 0) ? 1 : 0;
+                }
             }
         }
         int tot;
         int o = T + wave_excl(k, &tot);
         if (p < ls && k > 0) {
-            if (cf[i1] >= 0) {
+            if (cf1 >= 0) {
                 cand[o] = i1;
             } else {
-                for (int q = SI[i1]; q < SI[i1 + 1]; ++q) {
-                    const int k1 = SJ[q];
-                    if (cf[k1] >= 0) cand[o++] = k1;
+                for (int q = q0; q < q1; q += 8) {  // in order, as counted
+                    int c8[8], f8[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) c8[u] = q + u < q1 ? SJ[q + u] : -1;
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) f8[u] = c8[u] >= 0 ? cf[c8[u]] : -1;
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        if (f8[u] >= 0) cand[o++] = c8[u];
                 }
             }
         }
