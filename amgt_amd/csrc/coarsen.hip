@@ -517,26 +517,6 @@ struct WaveRow {
     int32_t sc[3];
 };
 
-__device__ __forceinline__ int64_t find_in_row(const int32_t *__restrict__ J, int64_t b, int64_t e,
-                                               int col) {
-    while (b < e) {
-        const int64_t m = (b + e) >> 1;
-        const int v = J[m];
-        if (v == col) return m;
-        if (v < col) b = m + 1;
-        else e = m;
-    }
-    return -1;
-}
-
-// position of column col (!= r) in row r of a row that k_rows_unsorted let
-// through: ascending, or the diagonal first and the rest ascending
-__device__ __forceinline__ int64_t find_off(const int32_t *__restrict__ J, int64_t b, int64_t e,
-                                            int r, int col) {
-    if (b < e && J[b] == r) ++b;
-    return find_in_row(J, b, e, col);
-}
-
 template <int WH>
 __device__ __forceinline__ int lds_find(const int32_t *hkey, const int32_t *hval, int k) {
     for (uint32_t h = h_slot(k, WH - 1);; h = (h + 1) & (WH - 1)) {
@@ -701,9 +681,11 @@ __global__ __launch_bounds__(64) void k_extpi_wave(
                 const int64_t d1 = diag_pos(I, J, i1);
                 sgn = (d1 >= 0 && A[d1] < 0) ? -1 : 1;
                 // the host's sum in the host's order; the row's entries are
-                // loaded eight at a time so that their loads overlap
-                double sum = 0.0;
-                int64_t q = b1;
+                // loaded eight at a time so that their loads overlap, and the
+                // entry of column i is noted on the way (the row's columns are
+                // distinct: the position find_off would search for)
+                double sum = 0.0, aqi = 0.0;
+                int64_t q = b1, qi = -1;
                 for (; q + 8 <= e1; q += 8) {
                     int jq[8];
                     double aq[8];
@@ -716,6 +698,10 @@ __global__ __launch_bounds__(64) void k_extpi_wave(
                     for (int u = 0; u < 8; ++u) {
                         if (q + u == d1) continue;
                         const int i2 = jq[u];
+                        if (i2 == i) {
+                            qi = q + u;
+                            aqi = aq[u];
+                        }
                         if ((i2 == i || lds_find<WH>(s.hkey, s.hval, i2) >= 0) &&
                             (sgn * aq[u]) < 0)
                             sum += aq[u];
@@ -724,14 +710,18 @@ __global__ __launch_bounds__(64) void k_extpi_wave(
                 for (; q < e1; ++q) {
                     if (q == d1) continue;
                     const int i2 = J[q];
-                    if ((i2 == i || lds_find<WH>(s.hkey, s.hval, i2) >= 0) && (sgn * A[q]) < 0)
-                        sum += A[q];
+                    const double aq1 = A[q];
+                    if (i2 == i) {
+                        qi = q;
+                        aqi = aq1;
+                    }
+                    if ((i2 == i || lds_find<WH>(s.hkey, s.hval, i2) >= 0) && (sgn * aq1) < 0)
+                        sum += aq1;
                 }
                 if (sum != 0) {
                     info = -2;
                     dist = s.aval[jj] / sum;
-                    const int64_t qi = find_off(J, b1, e1, i1, i);
-                    if (qi >= 0 && (sgn * A[qi]) < 0) dterm = dist * A[qi];
+                    if (qi >= 0 && (sgn * aqi) < 0) dterm = dist * aqi;
                 } else {
                     info = -3;
                 }
@@ -811,8 +801,8 @@ __global__ __launch_bounds__(64) void k_extpi_wave(
 #pragma clang fp contract(fast)
 
 // rows the wave kernels cannot search: a row must be ascending, or hold its
-// diagonal first and the rest ascending (HYPRE's layout); find_off and
-// diag_pos rely on exactly that
+// diagonal first and the rest ascending (HYPRE's layout); diag_pos relies on
+// exactly that
 __global__ __launch_bounds__(kThreads) void k_rows_unsorted(int32_t n, const int32_t *__restrict__ I,
                                                            const int32_t *__restrict__ J,
                                                            int32_t *__restrict__ flag) {
