@@ -409,11 +409,20 @@ __global__ __launch_bounds__(kThreads) void k_extpi_row(
             }
         } else if (cf[i1] != -3) {
             h_put(kv, mask, i1, kStrongF);
-            for (int q = SI[i1]; q < SI[i1 + 1]; ++q) {
-                const int k1 = SJ[q];
-                if (cf[k1] >= 0 && h_find(kv, mask, k1) < 0) {
-                    h_put(kv, mask, k1, c);
-                    cols[c++] = k1;
+            const int q1 = SI[i1 + 1];
+            for (int q0 = SI[i1]; q0 < q1; q0 += 8) {  // eight strong neighbours at a time
+                int k8[8], f8[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) k8[u] = q0 + u < q1 ? SJ[q0 + u] : -1;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) f8[u] = k8[u] >= 0 ? cf[k8[u]] : -1;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int k1 = k8[u];
+                    if (f8[u] >= 0 && h_find(kv, mask, k1) < 0) {
+                        h_put(kv, mask, k1, c);
+                        cols[c++] = k1;
+                    }
                 }
             }
         }
@@ -433,22 +442,45 @@ __global__ __launch_bounds__(kThreads) void k_extpi_row(
         if (pos == kStrongF) {
             const int64_t d1 = diag_pos(I, J, i1);
             const int sgn = (d1 >= 0 && A[d1] < 0) ? -1 : 1;
+            // both passes over row i1 in the host's order, its entries loaded
+            // eight at a time so that the loads overlap
+            const int64_t b1 = I[i1], e1 = I[i1 + 1];
             double sum = 0.0;
-            for (int64_t q = I[i1]; q < I[i1 + 1]; ++q) {
-                if (q == d1) continue;
-                const int i2 = J[q];
-                if ((i2 == i || h_find(kv, mask, i2) >= 0) && (sgn * A[q]) < 0)
-                    sum += A[q];
+            for (int64_t q0 = b1; q0 < e1; q0 += 8) {
+                int j8[8];
+                double a8[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    j8[u] = q0 + u < e1 ? J[q0 + u] : -1;
+                    a8[u] = q0 + u < e1 ? A[q0 + u] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    if (q0 + u >= e1 || q0 + u == d1) continue;
+                    const int i2 = j8[u];
+                    if ((i2 == i || h_find(kv, mask, i2) >= 0) && (sgn * a8[u]) < 0)
+                        sum += a8[u];
+                }
             }
             if (sum != 0) {
                 const double distribute = A[jj] / sum;
-                for (int64_t q = I[i1]; q < I[i1 + 1]; ++q) {
-                    if (q == d1) continue;
-                    const int i2 = J[q];
-                    if ((sgn * A[q]) < 0) {
-                        const int p2 = h_find(kv, mask, i2);
-                        if (p2 >= 0) w[p2] += distribute * A[q];
-                        if (i2 == i) diagonal += distribute * A[q];
+                for (int64_t q0 = b1; q0 < e1; q0 += 8) {
+                    int j8[8];
+                    double a8[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        j8[u] = q0 + u < e1 ? J[q0 + u] : -1;
+                        a8[u] = q0 + u < e1 ? A[q0 + u] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        if (q0 + u >= e1 || q0 + u == d1) continue;
+                        const int i2 = j8[u];
+                        if ((sgn * a8[u]) < 0) {
+                            const int p2 = h_find(kv, mask, i2);
+                            if (p2 >= 0) w[p2] += distribute * a8[u];
+                            if (i2 == i) diagonal += distribute * a8[u];
+                        }
                     }
                 }
             } else {
