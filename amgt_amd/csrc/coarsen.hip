@@ -715,7 +715,7 @@ __global__ __launch_bounds__(64) void k_extpi_wave(
                 // the host's sum in the host's order; the row's entries are
                 // loaded eight at a time so that their loads overlap, and the
                 // entry of column i is noted on the way (the row's columns are
-                // distinct: the position find_off would search for)
+                // distinct, so there is at most one)
                 double sum = 0.0, aqi = 0.0;
                 int64_t q = b1, qi = -1;
                 for (; q + 8 <= e1; q += 8) {
